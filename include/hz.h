@@ -1,0 +1,163 @@
+/*
+ * hz.h -- C ABI of the MI355X Helmholtz library (libhz.so).
+ *
+ * Matrix-free complex 7-point finite-volume Helmholtz operator with PML,
+ * material (SDF-blended) coefficients, reflecting / open / soft / impedance
+ * faces, the sinc-Neumann source columns, and restarted GMRES on the
+ * device.  Plain pointers and sizes only; device buffers are owned by the
+ * caller unless stated otherwise.
+ *
+ * Conventions
+ *   - Unknowns are numbered Fortran-style: p = i + nx*(j + ny*k).
+ *   - Complex data are interleaved (re, im).  Device vectors passed to the
+ *     ABI are complex128 (16 bytes per value) unless a function says
+ *     otherwise; k right-hand sides are stored column-major with a leading
+ *     dimension ld >= N (column r starts at r*ld).
+ *   - Every function returns an hz_status; on failure hz_last_error()
+ *     returns a message (thread-local).
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *     One operator handle must not be used from two threads at once.
+ *
+ * Reference interfaces replaced (Rain-Kotsuzui/ultrasound_solver):
+ *   hz_create          ConservativeHelmholtz.__init__ + assemble
+ *                      (src/algorithm/physics/conservative_helmholtz.py:18-149);
+ *                      no CSR is formed, coefficients stay per node.
+ *   hz_apply           the SpMV `A_gpu @ vector` and the scaled/preconditioned
+ *                      operators of src/algorithm/solvers/iterative_solver.py:37-56,
+ *                      and the conj-transpose CSR of helmholtz_solver.py:613-620.
+ *   hz_gmres           GpuIterativeSolver.solve (iterative_solver.py:30-104),
+ *                      called from HelmholtzDirectSolver._solve_iterative /
+ *                      _solve_gpu_adjoint (helmholtz_solver.py:589-632).
+ *   hz_set_sources     WarpAssemblyEngine.source_columns
+ *                      (warp_utils.py:95-99; sinc_aperture.py:36-85).
+ *   hz_build_rhs       WarpAssemblyEngine.assemble_rhs (warp_utils.py:118-120)
+ *                      and the basis-build RHS batch (phase_response_basis.py:106-107).
+ *   hz_gather_sources  B^H lambda, the adjoint form of G^H c used by
+ *                      PhaseResponseBasis.phase_vjp (phase_response_basis.py:208-253).
+ *   hz_sample          trilinear rows of PhaseResponseBasis.target_response
+ *                      (phase_response_basis.py:165-174).
+ */
+#ifndef HZ_H
+#define HZ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hz_operator hz_operator;
+
+typedef enum hz_status {
+    HZ_OK = 0,
+    HZ_NOT_CONVERGED = 1,   /* stats are filled in */
+    HZ_BAD_ARGUMENT = 2,
+    HZ_OUT_OF_MEMORY = 3,
+    HZ_DEVICE_ERROR = 4
+} hz_status;
+
+typedef enum hz_face_kind {
+    HZ_FACE_REFLECTING = 0,
+    HZ_FACE_OPEN = 1,
+    HZ_FACE_SOFT = 2,
+    HZ_FACE_IMPEDANCE = 3
+} hz_face_kind;
+
+typedef enum hz_precision {
+    HZ_C128 = 0,            /* complex128 vectors, FP64 coefficients */
+    HZ_C64 = 1              /* complex64 vectors, FP32 coefficients */
+} hz_precision;
+
+typedef enum hz_apply_mode {
+    HZ_APPLY_A = 0,             /* y = A x                                   */
+    HZ_APPLY_RAM = 1,           /* y = R A M x  (row scale R, Jacobi M)      */
+    HZ_APPLY_AH = 2,            /* y = A^H x                                 */
+    HZ_APPLY_RAM_H = 3          /* y = R' A^H M' x (R', M' built from A^H)   */
+} hz_apply_mode;
+
+/* Grid, materials and faces.  All pointers are host memory read during
+ * hz_create only.  Face order: -x, +x, -y, +y, -z, +z.                      */
+typedef struct hz_grid_desc {
+    int32_t nx, ny, nz;
+    double dx;                  /* node spacing h                            */
+    double omega;               /* angular frequency                         */
+    const double *rho;          /* N densities                               */
+    const double *kappa;        /* N compressibilities 1/(rho c^2)           */
+    const double *weight[3];    /* per-axis half-volume weights, length n_a  */
+    const double *stretch[3];   /* per-axis PML stretch, complex, length n_a */
+    int32_t face_kind[6];       /* hz_face_kind per face                     */
+    /* Per-face diagonal term i*omega*area*Y, complex, n_b*n_c values with the
+     * lower remaining axis fastest; NULL for reflecting and soft faces.      */
+    const double *face_term[6];
+} hz_grid_desc;
+
+typedef struct hz_gmres_opts {
+    int32_t restart;            /* Arnoldi steps per cycle (reference: 30)   */
+    int32_t maxiter;            /* inner-step limit (reference: 3000)        */
+    double rtol;                /* scaled residual target (reference: 1e-6)  */
+    double backward_error_tol;  /* componentwise limit (reference: 1e-4)     */
+    int32_t precision;          /* HZ_C128 plain form, HZ_C64 mixed form     */
+    int32_t adjoint;            /* 0: A x = b, 1: A^H x = b                  */
+    int32_t chunk;              /* RHS solved together (mixed form)          */
+    int32_t max_refinements;    /* FP64 refinement steps (mixed form)        */
+    double inner_rtol;          /* inner c64 tolerance floor (mixed form)    */
+    int32_t use_graph;          /* capture each cycle in a hipGraph          */
+} hz_gmres_opts;
+
+typedef struct hz_gmres_stats {
+    int32_t info;               /* 0 converged, else the inner-step count    */
+    int32_t iterations;         /* inner (Arnoldi) steps taken               */
+    int32_t cycles;             /* restart cycles                            */
+    int32_t refinements;        /* FP64 refinement steps (mixed form)        */
+    double scaled_residual;     /* ||R(Ax-b)|| / ||R b||  (FP64)             */
+    double relative_residual;   /* ||Ax-b|| / ||b||       (FP64)             */
+    double backward_error;      /* max_i |Ax-b|_i / (|A||x|+|b|)_i           */
+    int32_t finite;             /* all entries of x finite                   */
+    int32_t converged;          /* reference acceptance test passed          */
+} hz_gmres_stats;
+
+hz_status hz_create(const hz_grid_desc *desc, hz_operator **out);
+hz_status hz_destroy(hz_operator *op);
+
+/* y[:, r] = op(x[:, r]) for r < nrhs; x and y are device buffers of the
+ * precision's complex type (complex128 for HZ_C128, complex64 for HZ_C64). */
+hz_status hz_apply(hz_operator *op, int32_t precision, int32_t mode,
+                   const void *x, void *y, int32_t nrhs, int64_t ld, void *stream);
+
+/* Solve op x = b for nrhs columns (device complex128, leading dimension ld).
+ * x is overwritten (zero initial guess).  stats: host array of nrhs.       */
+hz_status hz_gmres(hz_operator *op, const hz_gmres_opts *opts, const void *b,
+                   void *x, int32_t nrhs, int64_t ld, hz_gmres_stats *stats, void *stream);
+
+/* Source columns B (N x M, real): nnz entries (node, column, value), host. */
+hz_status hz_set_sources(hz_operator *op, int32_t num_columns, int64_t nnz,
+                         const int64_t *node, const int32_t *column, const double *value);
+
+/* b[:, r] = B w[:, r] for r < nrhs; w is device complex128 (M x nrhs,
+ * leading dimension ld_w), b device complex128 (N x nrhs, ld).  With
+ * w = a*exp(i*phi) this is b(phi); with w = diag(a) columns it is the
+ * basis-build batch.                                                      */
+hz_status hz_build_rhs(hz_operator *op, const void *w, int64_t ld_w, void *b,
+                       int32_t nrhs, int64_t ld, void *stream);
+
+/* out[c, r] = sum_p B[p, c] lambda[p, r], i.e. B^T lambda = B^H lambda for
+ * the real source columns; out is device complex128 (M x nrhs, ld_out).   */
+hz_status hz_gather_sources(hz_operator *op, const void *lambda, int64_t ld,
+                            void *out, int64_t ld_out, int32_t nrhs, void *stream);
+
+/* out[q, r] = sum_c w[q, c] u[rows[q, c], r] for 8 trilinear corners;
+ * rows (int64) and weights (double) are device arrays of npoints*8.        */
+hz_status hz_sample(hz_operator *op, const int64_t *rows, const double *weights,
+                    int32_t npoints, const void *u, int64_t ld, void *out,
+                    int64_t ld_out, int32_t nrhs, void *stream);
+
+/* Device bytes held by the operator (coefficients plus cached workspace). */
+int64_t hz_device_bytes(const hz_operator *op);
+
+const char *hz_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HZ_H */

@@ -1,0 +1,45 @@
+"""The C-ABI library loads without a GPU and exports every declared symbol."""
+import ctypes
+import re
+
+from conftest import ROOT
+
+
+def declared_functions():
+    text = (ROOT/"include"/"hz.h").read_text()
+    return sorted(set(re.findall(r"^\s*(?:hz_status|int64_t|const char \*)\s*(hz_\w+)\s*\(", text, re.M)))
+
+
+def test_header_declares_the_abi():
+    names = declared_functions()
+    assert {"hz_create", "hz_destroy", "hz_apply", "hz_gmres", "hz_set_sources", "hz_build_rhs",
+            "hz_gather_sources", "hz_sample", "hz_device_bytes", "hz_last_error"} <= set(names)
+
+
+def test_library_exports_every_declared_symbol():
+    from ultrasound_solver_amd import _capi
+    lib = _capi.lib()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+    assert set(_capi.EXPORTS) == set(declared_functions())
+
+
+def test_struct_layouts_match_header():
+    from ultrasound_solver_amd import _capi
+    # int32 x3, pad, 2 doubles, 2 ptrs, 3+3 ptrs, int32 x6, 6 ptrs
+    assert ctypes.sizeof(_capi.GridDesc) == 16 + 16 + 16 + 48 + 24 + 48
+    assert ctypes.sizeof(_capi.GmresOpts) == 4*2 + 8*2 + 4*4 + 8 + 4 + 4
+    assert ctypes.sizeof(_capi.GmresStats) == 4*4 + 8*3 + 4*2
+
+
+def test_last_error_is_empty_string_initially():
+    from ultrasound_solver_amd import _capi
+    assert isinstance(_capi.lib().hz_last_error(), bytes)
+
+
+def test_bad_arguments_are_rejected_without_touching_the_device():
+    from ultrasound_solver_amd import _capi
+    handle = ctypes.c_void_p()
+    status = _capi.lib().hz_create(None, ctypes.byref(handle))
+    assert status == _capi.HZ_BAD_ARGUMENT
+    assert b"null" in _capi.lib().hz_last_error()

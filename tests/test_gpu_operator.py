@@ -1,0 +1,194 @@
+"""HIP operator and GMRES against the CPU oracle (runs on an MI355X).
+
+Tolerances: complex128 applies match the oracle CSR SpMV to 1e-13
+(relative, max-norm over each vector); complex64 applies to 2e-6; GMRES
+solutions match the complex128 sparse-LU solution to 1e-5 relative L2 (the
+path's field tolerance) at the requested residuals.
+"""
+import json
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import helmholtz as ohz, krylov as okr, losses as olo
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+SCENES = ["mixed_faces", "open_pml_all", "soft_impedance_corner", "reflecting_box"]
+
+
+@pytest.fixture(scope="module")
+def fix(golden_dir):
+    f = np.load(golden_dir/"operator_fixtures.npz")
+    return f, json.loads(str(f["meta"]))
+
+
+def _op(f, meta, name):
+    from ultrasound_solver_amd.operator import HelmholtzOperator
+    s = meta["scenes"][name]
+    op = HelmholtzOperator(f[f"{name}/rho"], f[f"{name}/c"], meta["dx"], meta["omega"], s["boundaries"],
+                           s["pml"], device="cuda:0")
+    a = sp.csr_matrix((f[f"{name}/data"], f[f"{name}/indices"], f[f"{name}/indptr"]), shape=(op.n, op.n))
+    return op, a
+
+
+def _rand(n, k, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal((n, k)) + 1j*rng.standard_normal((n, k))
+
+
+def _rel(got, ref):
+    return np.max(np.abs(got - ref))/np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_apply_c128_matches_reference_matrix(fix, name):
+    f, meta = fix
+    op, a = _op(f, meta, name)
+    x = _rand(op.n, 3, 1)
+    scale = okr.row_scale(a)
+    minv = okr.jacobi_inverse(a, scale)
+    ah = okr.adjoint_matrix(a)
+    scale_h = okr.row_scale(ah)
+    minv_h = okr.jacobi_inverse(ah, scale_h)
+    refs = {"A": a @ x, "AH": ah @ x,
+            "RAM": scale[:, None]*(a @ (minv[:, None]*x)),
+            "RAMH": scale_h[:, None]*(ah @ (minv_h[:, None]*x))}
+    for mode, ref in refs.items():
+        got = op.apply(x, mode=mode).cpu().numpy().T
+        assert _rel(got, ref) < 1e-13, mode
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_apply_c64_matches_reference_matrix(fix, name):
+    f, meta = fix
+    op, a = _op(f, meta, name)
+    x = _rand(op.n, 2, 2)
+    scale = okr.row_scale(a)
+    minv = okr.jacobi_inverse(a, scale)
+    ref = scale[:, None]*(a @ (minv[:, None]*x))
+    got = op.apply(x, mode="RAM", precision="c64").cpu().numpy().T
+    assert _rel(got, ref) < 2e-6
+    got = op.apply(x, mode="A", precision="c64").cpu().numpy().T
+    assert _rel(got, a @ x) < 2e-6
+
+
+def _scene(n, seed=0, soft=False):
+    rng = np.random.default_rng(seed)
+    rho = 1.2*(1 + 0.1*rng.random((n, n, n)))
+    c = 343.0*(1 + 0.03*rng.standard_normal((n, n, n)))
+    bnd = {"-x": "open", "+x": "reflecting", "-y": "open", "+y": "open", "+z": "open"}
+    if soft:
+        bnd["+x"] = "soft"
+    pml = {"cells": 3}
+    return rho, c, bnd, pml
+
+
+def _solver_pair(n, seed=0, soft=False):
+    from ultrasound_solver_amd.operator import HelmholtzOperator
+    rho, c, bnd, pml = _scene(n, seed, soft)
+    dx, omega = 1.07e-3, 2*np.pi*40e3
+    op = HelmholtzOperator(rho, c, dx, omega, bnd, pml, device="cuda:0")
+    a = ohz.OracleOperator(rho, c, dx, omega, bnd, pml).matrix()
+    return op, a
+
+
+def _point_rhs(n_total, k, seed):
+    rng = np.random.default_rng(seed)
+    b = np.zeros((n_total, k), dtype=complex)
+    for r in range(k):
+        b[rng.integers(0, n_total, 3), r] = rng.standard_normal(3) + 1j*rng.standard_normal(3)
+    return b
+
+
+def test_plain_gmres_follows_reference_semantics():
+    op, a = _solver_pair(12)
+    b = _point_rhs(op.n, 1, 3)[:, 0]
+    x, st = op.gmres(b, restart=30, maxiter=3000, rtol=1e-8, precision="c128")
+    ref_solver = okr.ReferenceGmres(restart=30, maxiter=3000, rtol=1e-8)
+    ref_x = ref_solver.solve(a, b)
+    direct = okr.direct_solve(a, b)
+    got = x.cpu().numpy()[0]
+    assert st[0]["converged"] == 1 and st[0]["info"] == 0
+    # Same algorithm in FP64: same number of restart cycles.
+    assert st[0]["iterations"] == ref_solver.last_iterations
+    assert np.linalg.norm(got - ref_x)/np.linalg.norm(ref_x) < 1e-6
+    assert np.linalg.norm(got - direct)/np.linalg.norm(direct) < 1e-5
+    assert st[0]["scaled_residual"] <= 1e-8
+    assert abs(st[0]["scaled_residual"] - okr.scaled_residual(a, got, b)) < 1e-3*1e-8
+    assert abs(st[0]["backward_error"] - okr.backward_error(a, got, b)) <= 1e-6*okr.backward_error(a, got, b) + 1e-18
+
+
+def test_plain_gmres_batched_columns_match_single():
+    op, a = _solver_pair(10, seed=4)
+    b = _point_rhs(op.n, 3, 5)
+    xb, stb = op.gmres(b, rtol=1e-8, chunk=3)
+    direct = okr.direct_solve(a, b)
+    for r in range(3):
+        xs, sts = op.gmres(b[:, r], rtol=1e-8)
+        np.testing.assert_allclose(xb[r].cpu().numpy(), xs[0].cpu().numpy(), rtol=0, atol=1e-12*np.abs(direct[:, r]).max())
+        assert stb[r]["iterations"] == sts[0]["iterations"]
+
+
+@pytest.mark.parametrize("soft", [False, True])
+def test_mixed_gmres_refines_to_fp64_accuracy(soft):
+    op, a = _solver_pair(14, seed=1, soft=soft)
+    b = _point_rhs(op.n, 4, 7)
+    x, st = op.gmres(b, restart=30, maxiter=6000, rtol=1e-8, precision="c64", chunk=4)
+    direct = okr.direct_solve(a, b)
+    for r in range(4):
+        got = x[r].cpu().numpy()
+        assert st[r]["converged"] == 1, st[r]
+        assert st[r]["refinements"] >= 1
+        assert np.linalg.norm(got - direct[:, r])/np.linalg.norm(direct[:, r]) < 1e-5
+        assert okr.scaled_residual(a, got, b[:, r]) <= 1e-8
+
+
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_adjoint_solve_matches_direct(precision):
+    op, a = _solver_pair(12, seed=2, soft=True)
+    c = _point_rhs(op.n, 2, 9)
+    lam, st = op.gmres(c, rtol=1e-8, precision=precision, adjoint=True, chunk=2, maxiter=6000)
+    direct = okr.direct_solve(okr.adjoint_matrix(a), c)
+    for r in range(2):
+        got = lam[r].cpu().numpy()
+        assert st[r]["converged"] == 1, st[r]
+        assert np.linalg.norm(got - direct[:, r])/np.linalg.norm(direct[:, r]) < 1e-5
+
+
+def test_zero_rhs_and_nonconvergence_report():
+    op, a = _solver_pair(10)
+    x, st = op.gmres(np.zeros(op.n), rtol=1e-8)
+    assert st[0]["converged"] == 1 and st[0]["iterations"] == 0
+    assert torch.count_nonzero(x).item() == 0
+    b = _point_rhs(op.n, 1, 1)[:, 0]
+    with pytest.raises(RuntimeError, match="info="):
+        op.gmres(b, restart=2, maxiter=2, rtol=1e-12, raise_on_failure=True)
+
+
+def test_sources_scatter_gather_and_sampling():
+    from ultrasound_solver_amd import physics
+    from oracle import source as osrc
+    n, dx = 24, 1.07e-3
+    op, a = _solver_pair(n)
+    centers = osrc.ideal_grid_centers(2, 6e-3, (n - 1)*dx, (n - 1)*dx)
+    node, col, mass = physics.source_deposition((n, n), dx, centers, 4e-3, 16, 64)
+    scale = physics.flux_scale(dx, 1.21)
+    op.set_sources(node, col, scale*mass, len(centers))
+    masses = osrc.source_mass_matrix((n, n), dx, centers, 4e-3, 16, 64)
+    bref = osrc.flux_rhs_columns(masses, (n, n, n), dx, 1.21)
+    rng = np.random.default_rng(3)
+    w = rng.standard_normal((2, 4)) + 1j*rng.standard_normal((2, 4))
+    got = op.build_rhs(w).cpu().numpy()
+    ref = (bref @ w.T).T
+    assert _rel(got, ref) < 1e-14
+    lam = _rand(op.n, 2, 4)
+    g = op.gather_sources(lam).cpu().numpy()
+    assert _rel(g, (bref.T @ lam).T) < 1e-13
+    pts = np.array([[3.3e-3, 7.1e-3, 5.5e-3], [(n - 1)*dx, 0.0, 2.0e-3]])
+    rows, wts = olo.trilinear_rows(pts, (n, n, n), (0, 0, 0), dx)
+    s = op.sample(rows, wts, lam).cpu().numpy()
+    ref_s = olo.target_response(lam, pts, (n, n, n), (0, 0, 0), dx).T
+    assert _rel(s, ref_s) < 1e-14

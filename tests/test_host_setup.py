@@ -1,0 +1,86 @@
+"""Host-side scene setup of the product against the pinned oracle."""
+import json
+
+import numpy as np
+import pytest
+
+from oracle import helmholtz as ohz, source as osrc, materials as omat
+from ultrasound_solver_amd import physics
+
+
+@pytest.fixture(scope="module")
+def op_fix(golden_dir):
+    f = np.load(golden_dir/"operator_fixtures.npz")
+    return f, json.loads(str(f["meta"]))
+
+
+@pytest.mark.parametrize("name", ["mixed_faces", "open_pml_all", "soft_impedance_corner", "reflecting_box"])
+def test_axis_and_face_tables_match_oracle(op_fix, name):
+    f, meta = op_fix
+    s = meta["scenes"][name]
+    rho, c = f[f"{name}/rho"], f[f"{name}/c"]
+    ref = ohz.OracleOperator(rho, c, meta["dx"], meta["omega"], s["boundaries"], s["pml"])
+    w, st = physics.axis_tables(rho.shape, s["boundaries"], s["pml"])
+    for a in range(3):
+        np.testing.assert_array_equal(w[a], ref.w1d[a])
+        np.testing.assert_array_equal(st[a], ref.s1d[a])
+    kinds, tables = physics.face_tables(rho, c, meta["dx"], meta["omega"], s["boundaries"], w, st)
+    terms = list(ref.face_terms)
+    for idx, table in enumerate(tables):
+        if table is None:
+            continue
+        sel, kind, term = terms.pop(0)
+        assert kind == kinds[idx]
+        np.testing.assert_array_equal(table, np.asarray(term).ravel(order="F"))
+    assert not terms
+    np.testing.assert_array_equal(physics.soft_node_mask(rho.shape, kinds), ref.fixed)
+
+
+def test_validation_matches_reference_messages():
+    with pytest.raises(ValueError, match="PML leaves no physical interior"):
+        physics.axis_tables((6, 6, 6), {"-x": "open", "+x": "open"}, {"cells": 3})
+    with pytest.raises(ValueError, match="PML may only be placed on open faces"):
+        physics.axis_tables((9, 9, 9), {"-x": "soft"}, {"cells": 2, "faces": ["-x"]})
+    with pytest.raises(ValueError, match="Unknown boundary face"):
+        physics.axis_tables((9, 9, 9), {"up": "open"})
+
+
+@pytest.mark.parametrize("name", ["array2x2_default_order", "array3x3_coarse_order"])
+def test_deposition_matches_reference(golden_dir, name):
+    f = np.load(golden_dir/"source_fixtures.npz")
+    case = json.loads(str(f["meta"]))[name]
+    node, col, mass = physics.source_deposition(case["shape_xy"], case["dx"], f[f"{name}/centers"],
+                                                case["diameter"], *case["orders"])
+    # fixture is CSC over (node, column): same ordering (column, then node)
+    np.testing.assert_array_equal(node, f[f"{name}/mass_indices"])
+    np.testing.assert_array_equal(mass, f[f"{name}/mass_data"])
+    counts = np.bincount(col, minlength=len(f[f"{name}/centers"]))
+    np.testing.assert_array_equal(np.concatenate([[0], np.cumsum(counts)]), f[f"{name}/mass_indptr"])
+
+
+def test_centers_and_phases_match_archive(golden_dir):
+    a = np.load(golden_dir/"archive_fixture.npz")
+    cen = physics.ideal_grid_centers(12, 0.01, 0.12, 0.12)
+    np.testing.assert_array_equal(cen, a["source_positions"])
+    gp = physics.geometric_phases(cen, a["target_points"], 343/40e3, 2*np.pi*40e3/343)
+    np.testing.assert_allclose(gp, osrc.geometric_phases(cen, a["target_points"], 343/40e3,
+                                                         2*np.pi*40e3/343), atol=0)
+
+
+def test_sphere_materials_match_oracle():
+    shape, dx = (20, 18, 16), 1.07e-3
+    center, radius = (10*dx, 9*dx, 7*dx), 5*dx
+    rho, c = physics.obstacle_materials(physics.sphere_sdf(shape, dx, center, radius), dx, 1.2, 343.0)
+    r2, c2 = omat.sphere_materials(shape, dx, 1.2, 343.0, center, radius)
+    np.testing.assert_array_equal(rho, r2)
+    np.testing.assert_array_equal(c, c2)
+
+
+def test_box_and_ellipsoid_sdf_closed_forms():
+    dx = 1.0
+    b = physics.box_sdf((11, 11, 11), dx, (5, 5, 5), (2, 3, 4))
+    assert b[5, 5, 5] == -2 and b[8, 5, 5] == 1 and b[5, 5, 10] == 1
+    e = physics.ellipsoid_sdf((21, 21, 21), dx, (10, 10, 10), (5, 5, 5))
+    g = np.ogrid[:21, :21, :21]
+    sph = np.sqrt(sum((g[a] - 10.0)**2 for a in range(3))) - 5
+    np.testing.assert_allclose(e, sph, atol=1e-12)

@@ -1,0 +1,754 @@
+// C ABI of the Helmholtz library: operator handle, stencil applies, the two
+// GMRES forms (plain FP64 with the reference's restart semantics, and
+// complex64 inner solves under FP64 iterative refinement), the source
+// columns and trilinear sampling.  Declarations: include/hz.h.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/hz.h"
+#include "hz_krylov.hpp"
+#include "hz_stencil.hpp"
+
+using namespace hz;
+
+namespace {
+
+thread_local std::string g_error;
+
+struct HipFailure : std::runtime_error {
+    hipError_t code;
+    HipFailure(hipError_t c, const std::string &what) : std::runtime_error(what), code(c) {}
+};
+struct BadArgument : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HZ_HIP(call)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            throw HipFailure(e_, std::string(#call) + ": " + hipGetErrorString(e_) + " (line " +      \
+                                     std::to_string(__LINE__) + ")");                                  \
+    } while (0)
+
+#define HZ_LAUNCHED() HZ_HIP(hipGetLastError())
+
+// Owning device allocation.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void ensure(size_t nbytes) {
+        if (nbytes <= bytes) return;
+        release();
+        HZ_HIP(hipMalloc(&p, std::max<size_t>(nbytes, 256)));
+        bytes = nbytes;
+    }
+    template <typename U>
+    U *as() const { return static_cast<U *>(p); }
+};
+
+template <typename U>
+void upload(DevBuf &dst, const U *src, size_t count) {
+    dst.ensure(count * sizeof(U));
+    HZ_HIP(hipMemcpy(dst.p, src, count * sizeof(U), hipMemcpyHostToDevice));
+}
+
+int64_t blocks_for(int64_t n, int64_t per_block, int64_t cap) {
+    return std::max<int64_t>(1, std::min<int64_t>(cap, (n + per_block - 1) / per_block));
+}
+
+}  // namespace
+
+struct hz_operator {
+    int nx = 0, ny = 0, nz = 0;
+    int64_t n = 0;
+    double dx = 0, omega = 0;
+    int soft = 0;
+    DevBuf rho64, kap64, rho32, kap32;
+    DevBuf a64[3], q64[3], a32[3], q32[3];
+    DevBuf face64[6], face32[6];
+    DevBuf rsf64, mf64, rsa64, ma64, rsf32, mf32, rsa32, ma32;
+    OpView<double> v64{};
+    OpView<float> v32{};
+    TileMap tm{};
+    // Source columns, grouped by node (scatter) and by column (gather).
+    int ncols = 0;
+    int64_t nrows_src = 0;
+    DevBuf src_row_node, src_row_ptr, src_row_col, src_row_val;
+    DevBuf src_col_node, src_col_ptr, src_col_val;
+    // Cached GMRES workspace (grow-only).
+    DevBuf V, X, Bh, dpart, npart, cpart, red, Hm, gc, gs, g, scales, coef, resid, tol, steps, live, member;
+    DevBuf rhs32;
+};
+
+namespace {
+
+template <typename T>
+const OpView<T> &view(const hz_operator *op);
+template <>
+const OpView<double> &view<double>(const hz_operator *op) { return op->v64; }
+template <>
+const OpView<float> &view<float>(const hz_operator *op) { return op->v32; }
+
+int64_t vec_blocks(int64_t n) { return blocks_for(n, 256 * 16, 1024); }
+
+template <typename T, int MODE>
+void launch_apply(const hz_operator *op, const cx<T> *x, int64_t ldx, cx<T> *y, int64_t ldy, int nrhs,
+                  const int *live, const double *scale, hipStream_t s) {
+    hipLaunchKernelGGL((k_apply<T, MODE>), dim3(op->tm.nblocks), dim3(256), 0, s, view<T>(op), op->tm, x, ldx, y,
+                       ldy, nrhs, live, scale);
+    HZ_LAUNCHED();
+}
+
+template <typename T>
+void apply_mode(const hz_operator *op, int mode, const cx<T> *x, int64_t ldx, cx<T> *y, int64_t ldy, int nrhs,
+                const int *live, const double *scale, hipStream_t s) {
+    switch (mode) {
+        case MODE_A: launch_apply<T, MODE_A>(op, x, ldx, y, ldy, nrhs, live, scale, s); break;
+        case MODE_RAM: launch_apply<T, MODE_RAM>(op, x, ldx, y, ldy, nrhs, live, scale, s); break;
+        case MODE_AH: launch_apply<T, MODE_AH>(op, x, ldx, y, ldy, nrhs, live, scale, s); break;
+        case MODE_RAMH: launch_apply<T, MODE_RAMH>(op, x, ldx, y, ldy, nrhs, live, scale, s); break;
+        default: throw BadArgument("unknown apply mode");
+    }
+}
+
+template <typename T>
+void residual_mode(const hz_operator *op, int mode, const cx<T> *x, const cx<T> *b, cx<T> *out, int k,
+                   const int *live, double *partial, hipStream_t s) {
+    const dim3 grid(op->tm.nblocks, k);
+    const int64_t n = op->n;
+    if (mode == MODE_RAM)
+        hipLaunchKernelGGL((k_residual<T, MODE_RAM>), grid, dim3(256), 0, s, view<T>(op), op->tm, x, n, b, n, out,
+                           n, live, partial);
+    else
+        hipLaunchKernelGGL((k_residual<T, MODE_RAMH>), grid, dim3(256), 0, s, view<T>(op), op->tm, x, n, b, n, out,
+                           n, live, partial);
+    HZ_LAUNCHED();
+}
+
+// Per-RHS bookkeeping of one chunk.
+struct ChunkState {
+    int k = 0;
+    std::vector<int> active;      // still being solved
+    std::vector<int> iterations;  // inner steps
+    std::vector<int> cycles;
+    std::vector<double> last_resid;
+};
+
+// Restarted GMRES on k slots of  (R A M) y = Bh  in precision T, y in X.
+//   ref_semantics: full `restart` steps per cycle, convergence tested only at
+//   the cycle top (the reference/CuPy loop); otherwise a slot leaves its cycle
+//   as soon as the Givens estimate meets its tolerance.
+// On return X holds y for every slot that was active on entry.
+template <typename T>
+void gmres_cycles(hz_operator *op, int mode_ram, int k, int restart, int maxiter, bool ref_semantics,
+                  const std::vector<double> &atol, ChunkState &cs, hipStream_t s) {
+    const int64_t n = op->n;
+    const int maxv = restart + 1;
+    const int64_t nblk = vec_blocks(n);
+    const int64_t chunk = (n + nblk - 1) / nblk;
+    cx<T> *V = op->V.as<cx<T>>();
+    cx<T> *X = op->X.as<cx<T>>();
+    cx<T> *Bh = op->Bh.as<cx<T>>();
+    GmresState st;
+    st.H = op->Hm.as<cx<double>>();
+    st.gc = op->gc.as<double>();
+    st.gs = op->gs.as<cx<double>>();
+    st.g = op->g.as<cx<double>>();
+    st.scales = op->scales.as<double>();
+    st.coef = op->coef.as<cx<double>>();
+    st.resid = op->resid.as<double>();
+    st.tol = op->tol.as<double>();
+    st.steps = op->steps.as<int>();
+    st.live = op->live.as<int>();
+    int *member = op->member.as<int>();
+
+    hipLaunchKernelGGL((k_zero<T>), dim3(vec_blocks(n), k), dim3(256), 0, s, X, n, n);
+    HZ_LAUNCHED();
+    std::vector<double> early(k, -1.0), resid(k);
+    if (!ref_semantics)
+        for (int r = 0; r < k; ++r) early[r] = atol[r];
+    HZ_HIP(hipMemcpyAsync(st.tol, early.data(), k * sizeof(double), hipMemcpyHostToDevice, s));
+
+    std::vector<int> live_h(k);
+    while (true) {
+        for (int r = 0; r < k; ++r) live_h[r] = cs.active[r];
+        HZ_HIP(hipMemcpyAsync(st.live, live_h.data(), k * sizeof(int), hipMemcpyHostToDevice, s));
+        // Cycle top: V_0 = Bh - R A M X (true residual of the preconditioned system).
+        residual_mode<T>(op, mode_ram, X, Bh, V, k, st.live, op->npart.as<double>(), s);
+        hipLaunchKernelGGL(k_cycle_start, dim3(k), dim3(256), 0, s, st, op->npart.as<double>(),
+                           (int)op->tm.nblocks, maxv, restart);
+        HZ_LAUNCHED();
+        HZ_HIP(hipMemcpyAsync(resid.data(), st.resid, k * sizeof(double), hipMemcpyDeviceToHost, s));
+        HZ_HIP(hipStreamSynchronize(s));
+        bool any = false;
+        for (int r = 0; r < k; ++r) {
+            if (!cs.active[r]) continue;
+            cs.last_resid[r] = resid[r];
+            if (!(resid[r] > atol[r]) || cs.iterations[r] >= maxiter || !std::isfinite(resid[r])) {
+                cs.active[r] = 0;
+                continue;
+            }
+            any = true;
+        }
+        if (!any) break;
+        for (int r = 0; r < k; ++r) live_h[r] = cs.active[r];
+        HZ_HIP(hipMemcpyAsync(st.live, live_h.data(), k * sizeof(int), hipMemcpyHostToDevice, s));
+        HZ_HIP(hipMemcpyAsync(member, live_h.data(), k * sizeof(int), hipMemcpyHostToDevice, s));
+        for (int j = 0; j < restart; ++j) {
+            const cx<T> *vj = V + (int64_t)j * k * n;
+            cx<T> *w = V + (int64_t)(j + 1) * k * n;
+            apply_mode<T>(op, mode_ram, vj, n, w, n, k, st.live, st.scales + (int64_t)j * k, s);
+            const int groups = (j + 1 + DOT_GROUP - 1) / DOT_GROUP;
+            hipLaunchKernelGGL((k_dots<T>), dim3(nblk, k, groups), dim3(256), 0, s, V, k, n, j + 1, w, st.live,
+                               op->dpart.as<double2>(), maxv, chunk);
+            HZ_LAUNCHED();
+            hipLaunchKernelGGL(k_reduce_dots, dim3(j + 1, k), dim3(256), 0, s, op->dpart.as<double2>(), (int)nblk,
+                               maxv, j, st.scales, st.live, st.H, st.coef, restart);
+            HZ_LAUNCHED();
+            hipLaunchKernelGGL((k_update<T>), dim3(nblk, k), dim3(256), 0, s, V, k, n, j + 1, (const int *)nullptr,
+                               w, n, st.coef, maxv, st.live, op->npart.as<double>(), chunk);
+            HZ_LAUNCHED();
+            hipLaunchKernelGGL(k_arnoldi_finish, dim3(k), dim3(256), 0, s, st, op->npart.as<double>(), (int)nblk,
+                               j, maxv, restart);
+            HZ_LAUNCHED();
+        }
+        hipLaunchKernelGGL(k_trisolve, dim3(k), dim3(64), 0, s, st, maxv, restart, member);
+        HZ_LAUNCHED();
+        hipLaunchKernelGGL((k_update<T>), dim3(nblk, k), dim3(256), 0, s, V, k, n, 0, st.steps, X, n, st.coef, maxv,
+                           member, (double *)nullptr, chunk);
+        HZ_LAUNCHED();
+        std::vector<int> steps_h(k);
+        HZ_HIP(hipMemcpyAsync(steps_h.data(), st.steps, k * sizeof(int), hipMemcpyDeviceToHost, s));
+        HZ_HIP(hipStreamSynchronize(s));
+        for (int r = 0; r < k; ++r) {
+            if (!cs.active[r]) continue;
+            cs.iterations[r] += ref_semantics ? restart : steps_h[r];
+            cs.cycles[r] += 1;
+        }
+    }
+}
+
+void reserve_workspace(hz_operator *op, int k, int restart, size_t elem) {
+    const int64_t n = op->n;
+    const int maxv = restart + 1;
+    const int64_t nblk = vec_blocks(n);
+    op->V.ensure((size_t)maxv * k * n * elem);
+    op->X.ensure((size_t)k * n * elem);
+    op->Bh.ensure((size_t)k * n * elem);
+    op->dpart.ensure((size_t)k * maxv * nblk * sizeof(double2));
+    op->npart.ensure((size_t)k * std::max<int64_t>(nblk, op->tm.nblocks) * 2 * sizeof(double));
+    op->cpart.ensure((size_t)k * op->tm.nblocks * 6 * sizeof(double));
+    op->red.ensure((size_t)k * 8 * sizeof(double));
+    op->Hm.ensure((size_t)k * (restart + 1) * restart * sizeof(cx<double>));
+    op->gc.ensure((size_t)k * restart * sizeof(double));
+    op->gs.ensure((size_t)k * restart * sizeof(cx<double>));
+    op->g.ensure((size_t)k * (restart + 1) * sizeof(cx<double>));
+    op->scales.ensure((size_t)maxv * k * sizeof(double));
+    op->coef.ensure((size_t)k * maxv * sizeof(cx<double>));
+    op->resid.ensure((size_t)k * sizeof(double));
+    op->tol.ensure((size_t)k * sizeof(double));
+    op->steps.ensure((size_t)k * sizeof(int));
+    op->live.ensure((size_t)k * sizeof(int));
+    op->member.ensure((size_t)k * sizeof(int));
+}
+
+struct CheckResult {
+    double se, sb, ue, ub, bwd, bad;
+};
+
+// FP64 acceptance check of x (physical) against b for k columns; optionally
+// writes the scaled refinement residual into op->rhs32.
+std::vector<CheckResult> check(hz_operator *op, int adjoint, const cx<double> *x, int64_t ldx,
+                               const cx<double> *b, int64_t ldb, int k, cx<float> *rhs_out, hipStream_t s) {
+    const dim3 grid(op->tm.nblocks, k);
+    double *part = op->cpart.as<double>();
+    if (adjoint)
+        hipLaunchKernelGGL(k_check<MODE_AH>, grid, dim3(256), 0, s, op->v64, op->tm, x, ldx, b, ldb,
+                           (const int *)nullptr, part, rhs_out, op->n);
+    else
+        hipLaunchKernelGGL(k_check<MODE_A>, grid, dim3(256), 0, s, op->v64, op->tm, x, ldx, b, ldb,
+                           (const int *)nullptr, part, rhs_out, op->n);
+    HZ_LAUNCHED();
+    hipLaunchKernelGGL(k_reduce_rows, dim3(6, k), dim3(256), 0, s, part, (int)op->tm.nblocks, 6, 1 << 4,
+                       op->red.as<double>());
+    HZ_LAUNCHED();
+    std::vector<double> h((size_t)k * 6);
+    HZ_HIP(hipMemcpyAsync(h.data(), op->red.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HZ_HIP(hipStreamSynchronize(s));
+    std::vector<CheckResult> out(k);
+    for (int r = 0; r < k; ++r) out[r] = {h[r * 6], h[r * 6 + 1], h[r * 6 + 2], h[r * 6 + 3], h[r * 6 + 4], h[r * 6 + 5]};
+    return out;
+}
+
+void fill_stats(hz_gmres_stats &st, const CheckResult &c, const hz_gmres_opts &o) {
+    st.scaled_residual = c.sb > 0 ? std::sqrt(c.se / c.sb) : 0.0;
+    st.relative_residual = c.ub > 0 ? std::sqrt(c.ue / c.ub) : 0.0;
+    st.backward_error = c.bwd;
+    st.finite = c.bad == 0 ? 1 : 0;
+    st.converged = (st.info == 0 && st.finite && st.scaled_residual <= o.rtol &&
+                    st.backward_error <= o.backward_error_tol)
+                       ? 1
+                       : 0;
+}
+
+// Plain form: FP64 GMRES with the reference's semantics on one chunk.
+void solve_plain(hz_operator *op, const hz_gmres_opts &o, const cx<double> *b, int64_t ldb, cx<double> *x,
+                 int64_t ldx, int k, hz_gmres_stats *stats, hipStream_t s) {
+    const int64_t n = op->n;
+    reserve_workspace(op, k, o.restart, sizeof(cx<double>));
+    const double *rs = o.adjoint ? op->rsa64.as<double>() : op->rsf64.as<double>();
+    const cx<double> *m = o.adjoint ? op->ma64.as<cx<double>>() : op->mf64.as<cx<double>>();
+    const int64_t nblk = vec_blocks(n);
+    const int64_t chunk = (n + nblk - 1) / nblk;
+    double *part = op->npart.as<double>();
+    hipLaunchKernelGGL((k_scale_rhs<double>), dim3(nblk, k), dim3(256), 0, s, b, ldb, rs, op->Bh.as<cx<double>>(),
+                       n, n, part, chunk);
+    HZ_LAUNCHED();
+    hipLaunchKernelGGL(k_reduce_rows, dim3(2, k), dim3(256), 0, s, part, (int)nblk, 2, 0, op->red.as<double>());
+    HZ_LAUNCHED();
+    std::vector<double> norms((size_t)2 * k);
+    HZ_HIP(hipMemcpyAsync(norms.data(), op->red.p, norms.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HZ_HIP(hipStreamSynchronize(s));
+    ChunkState cs;
+    cs.k = k;
+    cs.active.assign(k, 1);
+    cs.iterations.assign(k, 0);
+    cs.cycles.assign(k, 0);
+    cs.last_resid.assign(k, 0.0);
+    std::vector<double> atol(k);
+    std::vector<int> zero_rhs(k, 0);
+    for (int r = 0; r < k; ++r) {
+        const double bn = std::sqrt(norms[r * 2 + 1]);
+        atol[r] = o.rtol * std::sqrt(norms[r * 2]);
+        if (bn == 0.0) {
+            zero_rhs[r] = 1;
+            cs.active[r] = 0;
+        }
+    }
+    gmres_cycles<double>(op, o.adjoint ? MODE_RAMH : MODE_RAM, k, o.restart, o.maxiter, true, atol, cs, s);
+    // x = M y for every column (zero right-hand sides give y = 0).
+    hipLaunchKernelGGL((k_precond_out<double>), dim3(vec_blocks(n), k), dim3(256), 0, s,
+                       (const cx<double> *)op->X.as<cx<double>>(), n, m, x, ldx, n, (const int *)nullptr, 0);
+    HZ_LAUNCHED();
+    const auto chk = check(op, o.adjoint, x, ldx, b, ldb, k, nullptr, s);
+    for (int r = 0; r < k; ++r) {
+        hz_gmres_stats &st = stats[r];
+        std::memset(&st, 0, sizeof(st));
+        if (zero_rhs[r]) {
+            st.finite = 1;
+            st.converged = 1;
+            continue;
+        }
+        st.iterations = cs.iterations[r];
+        st.cycles = cs.cycles[r];
+        const bool conv = !(cs.last_resid[r] > atol[r]);
+        st.info = (cs.iterations[r] == o.maxiter && !conv) ? cs.iterations[r] : 0;
+        if (!std::isfinite(cs.last_resid[r])) st.info = std::max(1, cs.iterations[r]);
+        fill_stats(st, chk[r], o);
+    }
+}
+
+// Mixed form: complex64 GMRES inner solves under FP64 iterative refinement.
+void solve_mixed(hz_operator *op, const hz_gmres_opts &o, const cx<double> *b, int64_t ldb, cx<double> *x,
+                 int64_t ldx, int k, hz_gmres_stats *stats, hipStream_t s) {
+    const int64_t n = op->n;
+    reserve_workspace(op, k, o.restart, sizeof(cx<float>));
+    op->rhs32.ensure((size_t)k * n * sizeof(cx<float>));
+    const cx<double> *m = o.adjoint ? op->ma64.as<cx<double>>() : op->mf64.as<cx<double>>();
+    hipLaunchKernelGGL((k_zero<double>), dim3(vec_blocks(n), k), dim3(256), 0, s, x, ldx, n);
+    HZ_LAUNCHED();
+    ChunkState cs;
+    cs.k = k;
+    cs.active.assign(k, 1);
+    cs.iterations.assign(k, 0);
+    cs.cycles.assign(k, 0);
+    cs.last_resid.assign(k, 0.0);
+    std::vector<int> done(k, 0), refinements(k, 0), zero_rhs(k, 0), member(k, 0);
+    std::vector<CheckResult> last;
+    for (int ref = 0;; ++ref) {
+        last = check(op, o.adjoint, x, ldx, b, ldb, k, op->rhs32.as<cx<float>>(), s);
+        std::vector<double> atol(k, 0.0);
+        bool any = false;
+        for (int r = 0; r < k; ++r) {
+            if (done[r]) continue;
+            const CheckResult &c = last[r];
+            if (c.ub == 0.0) {
+                zero_rhs[r] = 1;
+                done[r] = 1;
+                continue;
+            }
+            const double scaled = c.sb > 0 ? std::sqrt(c.se / c.sb) : 0.0;
+            const bool ok = c.bad == 0 && scaled <= o.rtol && c.bwd <= o.backward_error_tol;
+            if (ok || ref >= o.max_refinements || cs.iterations[r] >= o.maxiter || c.bad != 0) {
+                done[r] = 1;
+                continue;
+            }
+            atol[r] = std::max(0.5 * o.rtol * std::sqrt(c.sb), o.inner_rtol * std::sqrt(c.se));
+            any = true;
+        }
+        if (!any) break;
+        // The inner right-hand side is the scaled residual written by the check.
+        HZ_HIP(hipMemcpyAsync(op->Bh.p, op->rhs32.p, (size_t)k * n * sizeof(cx<float>), hipMemcpyDeviceToDevice, s));
+        for (int r = 0; r < k; ++r) cs.active[r] = member[r] = done[r] ? 0 : 1;
+        // The iteration budget is cumulative over refinements (cs.iterations carries over).
+        ChunkState inner = cs;
+        gmres_cycles<float>(op, o.adjoint ? MODE_RAMH : MODE_RAM, k, o.restart, o.maxiter, false, atol, inner, s);
+        for (int r = 0; r < k; ++r) {
+            cs.iterations[r] = inner.iterations[r];
+            cs.cycles[r] = inner.cycles[r];
+            if (member[r]) refinements[r] += 1;
+        }
+        HZ_HIP(hipMemcpyAsync(op->member.p, member.data(), k * sizeof(int), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL((k_precond_out<float>), dim3(vec_blocks(n), k), dim3(256), 0, s,
+                           (const cx<float> *)op->X.as<cx<float>>(), n, m, x, ldx, n,
+                           (const int *)op->member.as<int>(), 1);
+        HZ_LAUNCHED();
+    }
+    for (int r = 0; r < k; ++r) {
+        hz_gmres_stats &st = stats[r];
+        std::memset(&st, 0, sizeof(st));
+        if (zero_rhs[r]) {
+            st.finite = 1;
+            st.converged = 1;
+            continue;
+        }
+        st.iterations = cs.iterations[r];
+        st.cycles = cs.cycles[r];
+        st.refinements = refinements[r];
+        fill_stats(st, last[r], o);
+        if (!st.converged) {
+            st.info = std::max(1, cs.iterations[r]);
+            st.converged = 0;
+        }
+    }
+}
+
+template <typename F>
+hz_status guarded(F &&fn) {
+    try {
+        fn();
+        return HZ_OK;
+    } catch (const BadArgument &e) {
+        g_error = e.what();
+        return HZ_BAD_ARGUMENT;
+    } catch (const HipFailure &e) {
+        g_error = e.what();
+        return e.code == hipErrorOutOfMemory ? HZ_OUT_OF_MEMORY : HZ_DEVICE_ERROR;
+    } catch (const std::bad_alloc &) {
+        g_error = "host allocation failed";
+        return HZ_OUT_OF_MEMORY;
+    } catch (const std::exception &e) {
+        g_error = e.what();
+        return HZ_DEVICE_ERROR;
+    }
+}
+
+void check_op(const hz_operator *op) {
+    if (!op) throw BadArgument("null operator handle");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *hz_last_error(void) { return g_error.c_str(); }
+
+hz_status hz_create(const hz_grid_desc *d, hz_operator **out) {
+    if (!out) {
+        g_error = "out is null";
+        return HZ_BAD_ARGUMENT;
+    }
+    *out = nullptr;
+    hz_operator *op = new (std::nothrow) hz_operator();
+    if (!op) {
+        g_error = "host allocation failed";
+        return HZ_OUT_OF_MEMORY;
+    }
+    const hz_status st = guarded([&] {
+        if (!d) throw BadArgument("descriptor is null");
+        if (d->nx < 2 || d->ny < 2 || d->nz < 2) throw BadArgument("every grid edge needs >= 2 nodes");
+        if (!(d->dx > 0) || !(d->omega > 0) || !std::isfinite(d->dx) || !std::isfinite(d->omega))
+            throw BadArgument("dx and omega must be finite and positive");
+        if (!d->rho || !d->kappa) throw BadArgument("rho and kappa are required");
+        op->nx = d->nx;
+        op->ny = d->ny;
+        op->nz = d->nz;
+        op->n = (int64_t)d->nx * d->ny * d->nz;
+        op->dx = d->dx;
+        op->omega = d->omega;
+        const int dims[3] = {d->nx, d->ny, d->nz};
+        for (int f = 0; f < 6; ++f) {
+            if (d->face_kind[f] < HZ_FACE_REFLECTING || d->face_kind[f] > HZ_FACE_IMPEDANCE)
+                throw BadArgument("unknown face kind");
+            if (d->face_kind[f] == HZ_FACE_SOFT) op->soft |= 1 << f;
+            const bool needs = d->face_kind[f] == HZ_FACE_OPEN || d->face_kind[f] == HZ_FACE_IMPEDANCE;
+            if (needs && !d->face_term[f]) throw BadArgument("open/impedance face needs a face-term table");
+        }
+        for (int64_t p = 0; p < op->n; ++p)
+            if (!(d->rho[p] > 0) || !std::isfinite(d->rho[p]) || !(d->kappa[p] > 0) || !std::isfinite(d->kappa[p]))
+                throw BadArgument("rho and kappa must be finite and positive");
+        std::vector<float> f32((size_t)op->n);
+        upload(op->rho64, d->rho, op->n);
+        upload(op->kap64, d->kappa, op->n);
+        for (int64_t p = 0; p < op->n; ++p) f32[p] = (float)d->rho[p];
+        upload(op->rho32, f32.data(), op->n);
+        for (int64_t p = 0; p < op->n; ++p) f32[p] = (float)d->kappa[p];
+        upload(op->kap32, f32.data(), op->n);
+        const double h2 = d->dx * d->dx;
+        for (int ax = 0; ax < 3; ++ax) {
+            const int na = dims[ax];
+            if (!d->weight[ax] || !d->stretch[ax]) throw BadArgument("per-axis weight and stretch are required");
+            std::vector<cx<double>> a(na), q(std::max(na - 1, 1));
+            std::vector<cx<float>> a32(na), q32(std::max(na - 1, 1));
+            for (int i = 0; i < na; ++i) {
+                const cx<double> sv = mk<double>(d->stretch[ax][2 * i], d->stretch[ax][2 * i + 1]);
+                a[i] = d->weight[ax][i] * sv;
+                a32[i] = cast<float>(a[i]);
+            }
+            for (int i = 0; i + 1 < na; ++i) {
+                const cx<double> s0 = mk<double>(d->stretch[ax][2 * i], d->stretch[ax][2 * i + 1]);
+                const cx<double> s1 = mk<double>(d->stretch[ax][2 * i + 2], d->stretch[ax][2 * i + 3]);
+                // 2 / (mid * h^2) with mid = (s_i + s_{i+1}) / 2
+                q[i] = crecip((0.5 * h2) * (s0 + s1));
+                q[i] = 2.0 * q[i];
+                q32[i] = cast<float>(q[i]);
+            }
+            upload(op->a64[ax], a.data(), na);
+            upload(op->q64[ax], q.data(), q.size());
+            upload(op->a32[ax], a32.data(), na);
+            upload(op->q32[ax], q32.data(), q32.size());
+        }
+        for (int f = 0; f < 6; ++f) {
+            if (!(d->face_kind[f] == HZ_FACE_OPEN || d->face_kind[f] == HZ_FACE_IMPEDANCE)) continue;
+            const int ax = f / 2;
+            const int b0 = ax == 0 ? 1 : 0, c0 = ax == 2 ? 1 : 2;
+            const int64_t cnt = (int64_t)dims[b0] * dims[c0];
+            upload(op->face64[f], reinterpret_cast<const cx<double> *>(d->face_term[f]), cnt);
+            std::vector<cx<float>> t32(cnt);
+            for (int64_t e = 0; e < cnt; ++e)
+                t32[e] = mk<float>((float)d->face_term[f][2 * e], (float)d->face_term[f][2 * e + 1]);
+            upload(op->face32[f], t32.data(), cnt);
+        }
+        for (DevBuf *bf : {&op->rsf64, &op->rsa64}) bf->ensure(op->n * sizeof(double));
+        for (DevBuf *bf : {&op->mf64, &op->ma64}) bf->ensure(op->n * sizeof(cx<double>));
+        for (DevBuf *bf : {&op->rsf32, &op->rsa32}) bf->ensure(op->n * sizeof(float));
+        for (DevBuf *bf : {&op->mf32, &op->ma32}) bf->ensure(op->n * sizeof(cx<float>));
+
+        auto fill = [&](auto &v, auto *rho, auto *kap, DevBuf *a, DevBuf *q, DevBuf *face, DevBuf &rsf, DevBuf &mf,
+                        DevBuf &rsa, DevBuf &ma) {
+            using TT = std::remove_pointer_t<decltype(rho)>;
+            v.nx = op->nx;
+            v.ny = op->ny;
+            v.nz = op->nz;
+            v.n = op->n;
+            v.omega2 = (TT)(op->omega * op->omega);
+            v.rho = rho;
+            v.kappa = kap;
+            for (int ax = 0; ax < 3; ++ax) {
+                v.a[ax] = a[ax].template as<cx<TT>>();
+                v.q[ax] = q[ax].template as<cx<TT>>();
+            }
+            for (int f = 0; f < 6; ++f) v.face[f] = face[f].p ? face[f].template as<cx<TT>>() : nullptr;
+            v.soft = op->soft;
+            v.rs_f = rsf.template as<TT>();
+            v.m_f = mf.template as<cx<TT>>();
+            v.rs_a = rsa.template as<TT>();
+            v.m_a = ma.template as<cx<TT>>();
+        };
+        fill(op->v64, op->rho64.as<double>(), op->kap64.as<double>(), op->a64, op->q64, op->face64, op->rsf64,
+             op->mf64, op->rsa64, op->ma64);
+        fill(op->v32, op->rho32.as<float>(), op->kap32.as<float>(), op->a32, op->q32, op->face32, op->rsf32,
+             op->mf32, op->rsa32, op->ma32);
+        op->tm = make_tilemap(op->nx, op->ny, op->nz, 4);
+        const int64_t blocks = (op->n + 255) / 256;
+        hipLaunchKernelGGL(k_setup, dim3(blocks), dim3(256), 0, 0, op->v64, op->rsf64.as<double>(),
+                           op->mf64.as<cx<double>>(), op->rsa64.as<double>(), op->ma64.as<cx<double>>(),
+                           op->rsf32.as<float>(), op->mf32.as<cx<float>>(), op->rsa32.as<float>(),
+                           op->ma32.as<cx<float>>());
+        HZ_LAUNCHED();
+        HZ_HIP(hipDeviceSynchronize());
+    });
+    if (st != HZ_OK) {
+        delete op;
+        return st;
+    }
+    *out = op;
+    return HZ_OK;
+}
+
+hz_status hz_destroy(hz_operator *op) {
+    delete op;
+    return HZ_OK;
+}
+
+hz_status hz_apply(hz_operator *op, int32_t precision, int32_t mode, const void *x, void *y, int32_t nrhs,
+                   int64_t ld, void *stream) {
+    return guarded([&] {
+        check_op(op);
+        if (!x || !y) throw BadArgument("x and y must be device pointers");
+        if (nrhs < 1 || ld < op->n) throw BadArgument("need nrhs >= 1 and ld >= N");
+        if (mode < MODE_A || mode > MODE_RAMH) throw BadArgument("unknown apply mode");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        if (precision == HZ_C128)
+            apply_mode<double>(op, mode, static_cast<const cx<double> *>(x), ld, static_cast<cx<double> *>(y), ld,
+                               nrhs, nullptr, nullptr, s);
+        else if (precision == HZ_C64)
+            apply_mode<float>(op, mode, static_cast<const cx<float> *>(x), ld, static_cast<cx<float> *>(y), ld,
+                              nrhs, nullptr, nullptr, s);
+        else
+            throw BadArgument("unknown precision");
+    });
+}
+
+hz_status hz_gmres(hz_operator *op, const hz_gmres_opts *opts, const void *b, void *x, int32_t nrhs, int64_t ld,
+                   hz_gmres_stats *stats, void *stream) {
+    const hz_status st = guarded([&] {
+        check_op(op);
+        if (!opts || !b || !x || !stats) throw BadArgument("opts, b, x and stats are required");
+        const hz_gmres_opts &o = *opts;
+        if (nrhs < 1 || ld < op->n) throw BadArgument("need nrhs >= 1 and ld >= N");
+        if (o.restart < 1 || o.restart > 63) throw BadArgument("restart must be in [1, 63]");
+        if (o.maxiter < 0) throw BadArgument("maxiter must be >= 0");
+        if (!(o.rtol > 0) || !(o.backward_error_tol > 0)) throw BadArgument("tolerances must be positive");
+        if (o.precision != HZ_C128 && o.precision != HZ_C64) throw BadArgument("unknown precision");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        const int chunk = std::max(1, std::min<int>(nrhs, o.chunk > 0 ? o.chunk : 1));
+        for (int c0 = 0; c0 < nrhs; c0 += chunk) {
+            const int k = std::min(chunk, nrhs - c0);
+            const cx<double> *bc = static_cast<const cx<double> *>(b) + (int64_t)c0 * ld;
+            cx<double> *xc = static_cast<cx<double> *>(x) + (int64_t)c0 * ld;
+            if (o.precision == HZ_C128)
+                solve_plain(op, o, bc, ld, xc, ld, k, stats + c0, s);
+            else
+                solve_mixed(op, o, bc, ld, xc, ld, k, stats + c0, s);
+        }
+    });
+    if (st != HZ_OK) return st;
+    for (int r = 0; r < nrhs; ++r)
+        if (!stats[r].converged) return HZ_NOT_CONVERGED;
+    return HZ_OK;
+}
+
+hz_status hz_set_sources(hz_operator *op, int32_t num_columns, int64_t nnz, const int64_t *node,
+                         const int32_t *column, const double *value) {
+    return guarded([&] {
+        check_op(op);
+        if (num_columns < 1 || nnz < 1 || !node || !column || !value) throw BadArgument("empty source list");
+        for (int64_t e = 0; e < nnz; ++e)
+            if (node[e] < 0 || node[e] >= op->n || column[e] < 0 || column[e] >= num_columns)
+                throw BadArgument("source entry out of range");
+        std::vector<int64_t> order(nnz);
+        for (int64_t e = 0; e < nnz; ++e) order[e] = e;
+        // Grouped by node (scatter).
+        std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+            return node[a] != node[b] ? node[a] < node[b] : column[a] < column[b];
+        });
+        std::vector<int64_t> rnode, rptr{0};
+        std::vector<int> rcol(nnz);
+        std::vector<double> rval(nnz);
+        for (int64_t e = 0; e < nnz; ++e) {
+            const int64_t src = order[e];
+            if (e == 0 || node[src] != node[order[e - 1]]) {
+                if (e) rptr.push_back(e);
+                rnode.push_back(node[src]);
+            }
+            rcol[e] = column[src];
+            rval[e] = value[src];
+        }
+        rptr.push_back(nnz);
+        // Grouped by column (gather).
+        std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+            return column[a] != column[b] ? column[a] < column[b] : node[a] < node[b];
+        });
+        std::vector<int64_t> cnode(nnz), cptr(num_columns + 1, 0);
+        std::vector<double> cval(nnz);
+        for (int64_t e = 0; e < nnz; ++e) {
+            cnode[e] = node[order[e]];
+            cval[e] = value[order[e]];
+            cptr[column[order[e]] + 1] += 1;
+        }
+        for (int c = 0; c < num_columns; ++c) cptr[c + 1] += cptr[c];
+        upload(op->src_row_node, rnode.data(), rnode.size());
+        upload(op->src_row_ptr, rptr.data(), rptr.size());
+        upload(op->src_row_col, rcol.data(), rcol.size());
+        upload(op->src_row_val, rval.data(), rval.size());
+        upload(op->src_col_node, cnode.data(), cnode.size());
+        upload(op->src_col_ptr, cptr.data(), cptr.size());
+        upload(op->src_col_val, cval.data(), cval.size());
+        op->ncols = num_columns;
+        op->nrows_src = (int64_t)rnode.size();
+    });
+}
+
+hz_status hz_build_rhs(hz_operator *op, const void *w, int64_t ld_w, void *b, int32_t nrhs, int64_t ld,
+                       void *stream) {
+    return guarded([&] {
+        check_op(op);
+        if (op->ncols == 0) throw BadArgument("no source columns set");
+        if (!w || !b || nrhs < 1 || ld < op->n || ld_w < op->ncols) throw BadArgument("bad rhs arguments");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        hipLaunchKernelGGL((k_zero<double>), dim3(vec_blocks(op->n), nrhs), dim3(256), 0, s,
+                           static_cast<cx<double> *>(b), ld, op->n);
+        HZ_LAUNCHED();
+        const int64_t blocks = blocks_for(op->nrows_src, 256, 4096);
+        hipLaunchKernelGGL(k_scatter_sources, dim3(blocks, nrhs), dim3(256), 0, s, op->src_row_node.as<int64_t>(),
+                           op->src_row_ptr.as<int64_t>(), op->src_row_col.as<int>(), op->src_row_val.as<double>(),
+                           op->nrows_src, static_cast<const cx<double> *>(w), ld_w, static_cast<cx<double> *>(b), ld);
+        HZ_LAUNCHED();
+    });
+}
+
+hz_status hz_gather_sources(hz_operator *op, const void *lambda, int64_t ld, void *out, int64_t ld_out,
+                            int32_t nrhs, void *stream) {
+    return guarded([&] {
+        check_op(op);
+        if (op->ncols == 0) throw BadArgument("no source columns set");
+        if (!lambda || !out || nrhs < 1 || ld < op->n || ld_out < op->ncols) throw BadArgument("bad gather arguments");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        hipLaunchKernelGGL(k_gather_sources, dim3(op->ncols, nrhs), dim3(256), 0, s, op->src_col_node.as<int64_t>(),
+                           op->src_col_val.as<double>(), op->src_col_ptr.as<int64_t>(),
+                           static_cast<const cx<double> *>(lambda), ld, static_cast<cx<double> *>(out), ld_out);
+        HZ_LAUNCHED();
+    });
+}
+
+hz_status hz_sample(hz_operator *op, const int64_t *rows, const double *weights, int32_t npoints, const void *u,
+                    int64_t ld, void *out, int64_t ld_out, int32_t nrhs, void *stream) {
+    return guarded([&] {
+        check_op(op);
+        if (!rows || !weights || !u || !out || npoints < 1 || nrhs < 1 || ld < op->n || ld_out < npoints)
+            throw BadArgument("bad sample arguments");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        hipLaunchKernelGGL(k_sample, dim3((npoints + 255) / 256, nrhs), dim3(256), 0, s, rows, weights, npoints,
+                           static_cast<const cx<double> *>(u), ld, static_cast<cx<double> *>(out), ld_out);
+        HZ_LAUNCHED();
+    });
+}
+
+int64_t hz_device_bytes(const hz_operator *op) {
+    if (!op) return 0;
+    int64_t total = 0;
+    const DevBuf *bufs[] = {&op->rho64, &op->kap64, &op->rho32, &op->kap32, &op->rsf64, &op->mf64, &op->rsa64,
+                            &op->ma64, &op->rsf32, &op->mf32, &op->rsa32, &op->ma32, &op->V, &op->X, &op->Bh,
+                            &op->dpart, &op->npart, &op->cpart, &op->rhs32};
+    for (const DevBuf *bf : bufs) total += (int64_t)bf->bytes;
+    return total;
+}
+
+}  // extern "C"

@@ -1,0 +1,99 @@
+// Shared device types for the Helmholtz library: interleaved complex values,
+// the per-precision operator view, and wavefront/workgroup reductions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hz {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cx {
+    T re, im;
+};
+
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> mk(T re, T im) { return cx<T>{re, im}; }
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return {a.re - b.re, a.im - b.im}; }
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> operator*(cx<T> a, cx<T> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> operator*(T s, cx<T> a) { return {s * a.re, s * a.im}; }
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> conj(cx<T> a) { return {a.re, -a.im}; }
+template <typename T>
+__host__ __device__ __forceinline__ T norm2(cx<T> a) { return a.re * a.re + a.im * a.im; }
+template <typename T>
+__host__ __device__ __forceinline__ T cabs(cx<T> a) { return sqrt(a.re * a.re + a.im * a.im); }
+// conj(a) * b
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> cdot(cx<T> a, cx<T> b) {
+    return {a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re};
+}
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> cdiv(cx<T> a, cx<T> b) {
+    T d = b.re * b.re + b.im * b.im;
+    return {(a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d};
+}
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> crecip(cx<T> b) {
+    T d = b.re * b.re + b.im * b.im;
+    return {b.re / d, -b.im / d};
+}
+template <typename To, typename From>
+__host__ __device__ __forceinline__ cx<To> cast(cx<From> a) { return {(To)a.re, (To)a.im}; }
+
+// Per-precision operator view passed by value to every kernel.
+template <typename T>
+struct OpView {
+    int nx, ny, nz;
+    int64_t n;
+    T omega2;
+    const T *rho;       // N
+    const T *kappa;     // N
+    const cx<T> *a[3];  // w*s per axis, length n_axis
+    const cx<T> *q[3];  // 4/((s_i+s_{i+1}) h^2), length n_axis-1
+    const cx<T> *face[6];
+    int soft;           // bit f set when face f is pressure-release
+    const T *rs_f;      // row scale of A
+    const cx<T> *m_f;   // Jacobi inverse of R A
+    const T *rs_a;      // row scale of A^H
+    const cx<T> *m_a;   // Jacobi inverse of R' A^H
+};
+
+constexpr int WAVE = 64;
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, WAVE));
+    return v;
+}
+
+// Sum over a 256-thread workgroup in FP64; result valid in thread 0.
+// `scratch` must hold 4 doubles in LDS.
+__device__ __forceinline__ double block_sum_256(double v, double *scratch) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x % WAVE;
+    const int wid = threadIdx.x / WAVE;
+    if (lane == 0) scratch[wid] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) r = (scratch[0] + scratch[1]) + (scratch[2] + scratch[3]);
+    __syncthreads();
+    return r;
+}
+
+}  // namespace hz

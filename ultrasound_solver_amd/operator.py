@@ -1,0 +1,207 @@
+"""Device-resident Helmholtz operator: a thin Python owner of an hz_operator
+handle (``include/hz.h``) with torch tensors as device buffers.
+
+The constructor validates its inputs the way ``ConservativeHelmholtz``
+does (src/algorithm/physics/conservative_helmholtz.py:18-103) and builds the
+per-axis and face tables on the host; the matrix itself is never formed.
+Vectors are torch tensors on a HIP device; k right-hand sides are held as a
+(k, N) contiguous tensor (each column of the reference's (N, k) array is a
+contiguous row here), N in Fortran node order.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _capi, physics
+
+MODES = {"A": _capi.MODE_A, "RAM": _capi.MODE_RAM, "AH": _capi.MODE_AH, "RAMH": _capi.MODE_RAMH}
+PRECISIONS = {"c128": _capi.C128, "c64": _capi.C64}
+
+
+def _dptr(array):
+    return array.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _device_tensor(x, dtype, device):
+    t = torch.as_tensor(x, device=device)
+    if t.dtype != dtype:
+        t = t.to(dtype)
+    return t.contiguous()
+
+
+class HelmholtzOperator:
+    """A = div(rho^-1 grad) + omega^2/(rho c^2) with PML and face rows, on the GPU."""
+
+    def __init__(self, rho, c, dx, omega, boundaries, pml=None, device=None):
+        rho = np.asarray(rho, dtype=np.float64)
+        c = np.asarray(c, dtype=np.float64)
+        if rho.ndim != 3 or c.shape != rho.shape or min(rho.shape) < 2:
+            raise ValueError("rho and c must have the same 3D grid shape, each dimension >= 2")
+        if not (np.isfinite(rho).all() and np.isfinite(c).all() and (rho > 0).all() and (c > 0).all()):
+            raise ValueError("density and sound speed must be finite and positive")
+        if not (np.isfinite(dx) and dx > 0 and np.isfinite(omega) and omega > 0):
+            raise ValueError("dx and omega must be finite and positive")
+        if not torch.cuda.is_available():
+            raise RuntimeError("HelmholtzOperator needs a HIP device; none is visible")
+        self.device = torch.device(device if device is not None else "cuda")
+        self.shape = tuple(int(v) for v in rho.shape)
+        self.n = int(np.prod(self.shape))
+        self.dx, self.omega = float(dx), float(omega)
+        self.boundaries = dict(boundaries)
+        self.weights, self.stretch = physics.axis_tables(self.shape, self.boundaries, pml)
+        self.kinds, self.face_terms = physics.face_tables(rho, c, self.dx, self.omega, self.boundaries,
+                                                          self.weights, self.stretch)
+        self.fixed = physics.soft_node_mask(self.shape, self.kinds)
+        self.pml_mask = physics.pml_node_mask(self.stretch, self.shape)
+        self.rho, self.c = rho, c
+        kappa = 1.0/(rho*c*c)
+        keep = []  # host arrays referenced by the descriptor during hz_create
+
+        def hold(a):
+            a = np.ascontiguousarray(a)
+            keep.append(a)
+            return a
+
+        desc = _capi.GridDesc()
+        desc.nx, desc.ny, desc.nz = self.shape
+        desc.dx, desc.omega = self.dx, self.omega
+        desc.rho = _dptr(hold(rho.ravel(order="F")))
+        desc.kappa = _dptr(hold(kappa.ravel(order="F")))
+        for a in range(3):
+            desc.weight[a] = _dptr(hold(self.weights[a].astype(np.float64)))
+            desc.stretch[a] = _dptr(hold(self.stretch[a].astype(np.complex128).view(np.float64)))
+        for f in range(6):
+            desc.face_kind[f] = _capi.FACE_KIND[self.kinds[f]]
+            table = self.face_terms[f]
+            desc.face_term[f] = (_dptr(hold(table.view(np.float64))) if table is not None
+                                 else ctypes.POINTER(ctypes.c_double)())
+        handle = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _capi.check(_capi.lib().hz_create(ctypes.byref(desc), ctypes.byref(handle)), "hz_create")
+        self._h = handle
+        self.num_sources = 0
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            _capi.lib().hz_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        if not self._h:
+            raise RuntimeError("operator is closed")
+        return self._h
+
+    def device_bytes(self):
+        return int(_capi.lib().hz_device_bytes(self.handle))
+
+    # -- vectors ----------------------------------------------------------
+    def as_columns(self, x, dtype=torch.complex128):
+        """(N,), (N, k) array-like or (k, N) tensor -> (k, N) contiguous device tensor."""
+        if isinstance(x, torch.Tensor) and x.dim() == 2 and x.shape[1] == self.n and x.shape[0] != self.n:
+            return _device_tensor(x, dtype, self.device)
+        t = torch.as_tensor(x)
+        if t.dim() == 1:
+            t = t.reshape(1, -1)
+        elif t.dim() == 2 and t.shape[0] == self.n:
+            t = t.transpose(0, 1)
+        else:
+            raise ValueError(f"expected ({self.n},) or ({self.n}, k), got {tuple(t.shape)}")
+        if t.shape[1] != self.n:
+            raise ValueError("vector length does not match the grid")
+        return _device_tensor(t, dtype, self.device)
+
+    def apply(self, x, mode="A", precision="c128"):
+        """y = op(x) for each column; x is a (k, N) device tensor."""
+        dtype = torch.complex128 if precision == "c128" else torch.complex64
+        x = self.as_columns(x, dtype)
+        y = torch.empty_like(x)
+        _capi.check(_capi.lib().hz_apply(self.handle, PRECISIONS[precision], MODES[mode],
+                                         ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()),
+                                         x.shape[0], self.n, _stream()), "hz_apply")
+        return y
+
+    def gmres(self, b, restart=30, maxiter=3000, rtol=1.0e-6, backward_error_tol=1.0e-4,
+              precision="c128", adjoint=False, chunk=1, max_refinements=10, inner_rtol=1.0e-5,
+              raise_on_failure=False):
+        """Solve A x = b (or A^H x = b) for each column; returns (x, stats)."""
+        b = self.as_columns(b, torch.complex128)
+        k = b.shape[0]
+        x = torch.empty_like(b)
+        opts = _capi.GmresOpts(restart=int(restart), maxiter=int(maxiter), rtol=float(rtol),
+                               backward_error_tol=float(backward_error_tol),
+                               precision=PRECISIONS[precision], adjoint=int(bool(adjoint)),
+                               chunk=int(chunk), max_refinements=int(max_refinements),
+                               inner_rtol=float(inner_rtol), use_graph=0)
+        stats = (_capi.GmresStats*k)()
+        status = _capi.lib().hz_gmres(self.handle, ctypes.byref(opts), ctypes.c_void_p(b.data_ptr()),
+                                      ctypes.c_void_p(x.data_ptr()), k, self.n, stats, _stream())
+        if status not in (_capi.HZ_OK, _capi.HZ_NOT_CONVERGED):
+            _capi.check(status, "hz_gmres")
+        out = [s.as_dict() for s in stats]
+        if raise_on_failure and status == _capi.HZ_NOT_CONVERGED:
+            bad = next(s for s in out if not s["converged"])
+            raise RuntimeError(f"info={bad['info']}, iterations={bad['iterations']}, "
+                               f"scaled residual={bad['scaled_residual']:.3e}, "
+                               f"backward error={bad['backward_error']:.3e}")
+        return x, out
+
+    # -- sources ----------------------------------------------------------
+    def set_sources(self, node, column, value, num_columns):
+        """Source columns B as (node, column, value) triplets (node in 0..N-1)."""
+        node = np.ascontiguousarray(node, dtype=np.int64)
+        column = np.ascontiguousarray(column, dtype=np.int32)
+        value = np.ascontiguousarray(value, dtype=np.float64)
+        _capi.check(_capi.lib().hz_set_sources(
+            self.handle, int(num_columns), len(node), node.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+            column.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+            value.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "hz_set_sources")
+        self.num_sources = int(num_columns)
+
+    def build_rhs(self, w):
+        """b[:, r] = B w[:, r]; w is (k, M) complex (device or host)."""
+        w = _device_tensor(torch.as_tensor(w).reshape(-1, self.num_sources), torch.complex128, self.device)
+        k = w.shape[0]
+        b = torch.empty((k, self.n), dtype=torch.complex128, device=self.device)
+        _capi.check(_capi.lib().hz_build_rhs(self.handle, ctypes.c_void_p(w.data_ptr()), self.num_sources,
+                                             ctypes.c_void_p(b.data_ptr()), k, self.n, _stream()),
+                    "hz_build_rhs")
+        return b
+
+    def gather_sources(self, lam):
+        """(k, M): B^T lambda for each column of lambda."""
+        lam = self.as_columns(lam, torch.complex128)
+        k = lam.shape[0]
+        out = torch.empty((k, self.num_sources), dtype=torch.complex128, device=self.device)
+        _capi.check(_capi.lib().hz_gather_sources(self.handle, ctypes.c_void_p(lam.data_ptr()), self.n,
+                                                  ctypes.c_void_p(out.data_ptr()), self.num_sources, k,
+                                                  _stream()), "hz_gather_sources")
+        return out
+
+    def sample(self, rows, weights, u):
+        """Trilinear samples: rows/weights (P, 8); u (k, N) -> (k, P)."""
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        if rows.ndim != 2 or rows.shape[1] != 8 or rows.min() < 0 or rows.max() >= self.n:
+            raise ValueError("trilinear rows must be (P, 8) node indices inside the grid")
+        rows = _device_tensor(rows, torch.int64, self.device)
+        weights = _device_tensor(np.ascontiguousarray(weights, dtype=np.float64), torch.float64, self.device)
+        u = self.as_columns(u, torch.complex128)
+        k, npts = u.shape[0], rows.shape[0]
+        out = torch.empty((k, npts), dtype=torch.complex128, device=self.device)
+        _capi.check(_capi.lib().hz_sample(self.handle, ctypes.c_void_p(rows.data_ptr()),
+                                          ctypes.c_void_p(weights.data_ptr()), npts,
+                                          ctypes.c_void_p(u.data_ptr()), self.n,
+                                          ctypes.c_void_p(out.data_ptr()), npts, k, _stream()), "hz_sample")
+        return out

@@ -1,0 +1,249 @@
+"""Host-side scene setup for the HIP operator: per-axis tables, face terms,
+materials, transducer centres and the sinc-Neumann source deposition.
+
+Everything here is O(n^2) or O(M * quadrature) preprocessing done once per
+scene; the O(N) operator itself never exists on the host.  Conventions
+follow the reference so that the device operator reproduces its matrix A:
+
+* per-axis half-volume weights and PML stretch s = 1 + i sigma,
+  sigma = strength * max(1 - d/cells, 0)^order
+  (src/algorithm/physics/conservative_helmholtz.py:48-64);
+* open/impedance face terms i omega (w_b w_c s_b s_c / h) Y, Y = 1/(rho c)
+  or 1/Z (conservative_helmholtz.py:89-103);
+* sinc aperture quadrature and bilinear unit-mass deposition
+  (src/algorithm/physics/sinc_aperture.py:19-72), flux scale
+  -2 pi / (rho_ref h^3) (sinc_aperture.py:75-85);
+* material blend of 1/rho and 1/(rho c^2) across a smoothed SDF
+  (src/algorithm/physics/warp_utils.py:11-23, 44-79).
+"""
+import numpy as np
+
+FACES = ("-x", "+x", "-y", "+y", "-z", "+z")
+KINDS = ("reflecting", "open", "soft", "impedance")
+
+
+def boundary_type(value):
+    return value.get("type", "reflecting") if isinstance(value, dict) else value
+
+
+def pml_settings(pml, boundaries):
+    pml = dict(pml or {})
+    cells = int(pml.get("cells", 0))
+    strength = float(pml.get("strength", 2.0))
+    order = int(pml.get("order", 3))
+    if cells < 0 or order < 1 or not np.isfinite(strength) or strength < 0:
+        raise ValueError("Invalid PML cells, strength or order")
+    faces = pml.get("faces")
+    if faces is None:
+        faces = [f for f in FACES if boundary_type(boundaries.get(f, "reflecting")) == "open"]
+    if set(faces) - set(FACES):
+        raise ValueError("Unknown PML face")
+    return cells, strength, order, list(faces)
+
+
+def axis_tables(shape, boundaries, pml=None):
+    """Per-axis weights (real) and stretches (complex), validated like the reference."""
+    if set(boundaries) - set(FACES):
+        raise ValueError("Unknown boundary face")
+    cells, strength, order, faces = pml_settings(pml, boundaries)
+    weights, stretch = [], []
+    for axis, n in enumerate(shape):
+        w = np.ones(n)
+        w[[0, -1]] = 0.5
+        sigma = np.zeros(n)
+        here = [f for f in FACES[2*axis:2*axis + 2] if f in faces]
+        if cells and here:
+            if cells*len(here) >= n - 1:
+                raise ValueError("PML leaves no physical interior on an axis")
+            ramp = strength*np.maximum(1.0 - np.arange(n)/cells, 0.0)**order
+            for f in here:
+                if boundary_type(boundaries.get(f, "reflecting")) != "open":
+                    raise ValueError("PML may only be placed on open faces")
+                sigma = sigma + (ramp if f[0] == "-" else ramp[::-1])
+        weights.append(w)
+        stretch.append(1.0 + 1j*sigma)
+    return weights, stretch
+
+
+def pml_node_mask(stretch, shape):
+    sx, sy, sz = (np.imag(s) for s in stretch)
+    return (sx[:, None, None] + sy[None, :, None] + sz[None, None, :]) > 0
+
+
+def face_tables(rho, c, dx, omega, boundaries, weights, stretch):
+    """(kinds[6], tables[6]); tables are complex (n_b, n_c) arrays flattened with
+    the lower remaining axis fastest, or None for reflecting/soft faces."""
+    kinds, tables = [], []
+    for index, face in enumerate(FACES):
+        axis = index//2
+        spec = boundaries.get(face, "reflecting")
+        kind = boundary_type(spec)
+        if kind not in KINDS:
+            raise ValueError(f"Unsupported boundary {face}: {spec!r}")
+        kinds.append(kind)
+        if kind not in ("open", "impedance"):
+            tables.append(None)
+            continue
+        b, cc = [a for a in range(3) if a != axis]
+        area = (weights[b][:, None]*weights[cc][None, :]*stretch[b][:, None]*stretch[cc][None, :])/dx
+        sel = [slice(None)]*3
+        sel[axis] = 0 if index % 2 == 0 else -1
+        sel = tuple(sel)
+        if kind == "open":
+            admittance = 1.0/(rho[sel]*c[sel])
+        else:
+            if not isinstance(spec, dict):
+                raise ValueError("impedance requires {type: impedance, impedance: [real, imag]}")
+            z = np.asarray(spec.get("impedance"), dtype=float)
+            if z.shape != (2,) or not np.isfinite(z).all() or z[0] < 0 or not np.any(z):
+                raise ValueError("impedance must be finite [real>=0, imag], nonzero, in Pa s/m")
+            admittance = 1.0/complex(z[0], z[1])
+        term = 1j*omega*area*admittance
+        tables.append(np.ascontiguousarray(term.ravel(order="F"), dtype=np.complex128))
+    return kinds, tables
+
+
+def soft_node_mask(shape, kinds):
+    mask = np.zeros(shape, dtype=bool)
+    for index, kind in enumerate(kinds):
+        if kind == "soft":
+            sel = [slice(None)]*3
+            sel[index//2] = 0 if index % 2 == 0 else -1
+            mask[tuple(sel)] = True
+    return mask
+
+
+# ----------------------------------------------------------------- materials
+def blend_obstacle_material(h, rho_obstacle, c_obstacle, rho_medium, c_medium):
+    """1/rho and 1/(rho c^2) linear in h (0 inside the obstacle, 1 in the medium)."""
+    inv_rho = 1.0/rho_obstacle + (1.0/rho_medium - 1.0/rho_obstacle)*h
+    k_obs = 1.0/(rho_obstacle*c_obstacle**2)
+    kappa = k_obs + (1.0/(rho_medium*c_medium**2) - k_obs)*h
+    return 1.0/inv_rho, np.sqrt(inv_rho/kappa)
+
+
+def smoothed_heaviside(sdf, width):
+    if width:
+        t = np.clip(sdf/width, -1.0, 1.0)
+        return 0.5*(1.0 + t + np.sin(np.pi*t)/np.pi)
+    return (sdf > 0).astype(float)
+
+
+def _coords(shape, dx):
+    return np.ogrid[:shape[0], :shape[1], :shape[2]]
+
+
+def sphere_sdf(shape, dx, center, radius):
+    g = _coords(shape, dx)
+    return np.sqrt(sum((g[a]*dx - center[a])**2 for a in range(3))) - radius
+
+
+def box_sdf(shape, dx, center, half_extent):
+    g = _coords(shape, dx)
+    q = [np.abs(g[a]*dx - center[a]) - half_extent[a] for a in range(3)]
+    outside = np.sqrt(sum(np.maximum(qa, 0.0)**2 for qa in q))
+    inside = np.minimum(np.maximum(np.maximum(q[0], q[1]), q[2]), 0.0)
+    return outside + inside
+
+
+def ellipsoid_sdf(shape, dx, center, semi_axes):
+    """First-order ellipsoid distance k0 (k0 - 1)/k1, k0 = |x/r|, k1 = |x/r^2|."""
+    g = _coords(shape, dx)
+    d = [g[a]*dx - center[a] for a in range(3)]
+    k0 = np.sqrt(sum((d[a]/semi_axes[a])**2 for a in range(3)))
+    k1 = np.sqrt(sum((d[a]/semi_axes[a]**2)**2 for a in range(3)))
+    inside_centre = k1 == 0
+    k1 = np.where(inside_centre, 1.0, k1)
+    return np.where(inside_centre, -float(min(semi_axes)), k0*(k0 - 1.0)/k1)
+
+
+def obstacle_materials(sdf, dx, rho0, c0, rho_obstacle=1250.0, c_obstacle=2200.0, smoothing_cells=1.2,
+                       rho_medium=None, c_medium=None):
+    """Blend a (union) SDF into density and sound-speed fields; the medium may
+    itself be a field (rho_medium, c_medium arrays)."""
+    h = smoothed_heaviside(sdf, smoothing_cells*dx)
+    rm = rho0 if rho_medium is None else rho_medium
+    cm = c0 if c_medium is None else c_medium
+    return blend_obstacle_material(h, rho_obstacle, c_obstacle, rm, cm)
+
+
+# ------------------------------------------------------------- transducers
+def ideal_grid_centers(array_n, pitch, lx, ly, origin=(0.0, 0.0, 0.0)):
+    origin = np.asarray(origin, dtype=float)
+    cx, cy = origin[:2] + np.asarray([lx, ly])*0.5
+    half = (array_n - 1)*pitch*0.5
+    steps = np.arange(array_n)*pitch
+    gx, gy = np.meshgrid(cx - half + steps, cy - half + steps, indexing="ij")
+    return np.stack([gx.ravel(), gy.ravel(), np.zeros(gx.size)], axis=1)
+
+
+def geometric_phases(centers, targets, wavelength, k0):
+    if len(targets) == 0:
+        return np.zeros(len(centers))
+    if len(targets) == 1:
+        cyc = np.linalg.norm(centers - np.asarray(targets[0], float), axis=-1)/wavelength
+        return 2.0*np.pi*(np.ceil(cyc) - cyc)
+    field = np.zeros(len(centers), dtype=np.complex128)
+    for t in targets:
+        d = np.linalg.norm(centers - np.asarray(t, float), axis=-1)
+        field += (1.0/d)*np.exp(-1j*k0*d)
+    return np.angle(field)
+
+
+# ------------------------------------------------------------------ sources
+def aperture_quadrature(diameter, radial_order=32, angular_order=128):
+    if not np.isfinite(diameter) or diameter <= 0:
+        raise ValueError("Aperture diameter must be positive")
+    if radial_order < 2 or angular_order < 8:
+        raise ValueError("Aperture quadrature orders are too small")
+    x, w = np.polynomial.legendre.leggauss(radial_order)
+    mu, w = (x + 1)/2, w/2
+    rad = diameter/2*np.sqrt(1 - mu*mu)
+    ang = 2*np.pi*np.arange(angular_order)/angular_order
+    offsets = np.column_stack(((rad[:, None]*np.cos(ang)).ravel(), (rad[:, None]*np.sin(ang)).ravel(),
+                               np.zeros(radial_order*angular_order)))
+    return offsets, np.repeat(w/angular_order, angular_order)
+
+
+def source_deposition(shape_xy, dx, centers, diameter, radial_order=64, angular_order=256):
+    """Unit-mass bilinear deposition of every aperture on the z = 0 plane.
+
+    Returns (node, column, mass) triplets, node = i + nx*j, sorted by column
+    then node; each column sums to 1.  All emitters are deposited at once;
+    per (column, node) the quadrature contributions are accumulated in the
+    reference's order (corners 00, 01, 10, 11, each over the quadrature).
+    """
+    nx, ny = (int(v) for v in shape_xy)
+    centers = np.asarray(centers, float)
+    if (min(nx, ny) < 3 or not np.isfinite(dx) or dx <= 0 or centers.ndim != 2
+            or centers.shape[1] != 3 or not len(centers) or not np.isfinite(centers).all()
+            or np.any(centers[:, 2] != 0)):
+        raise ValueError("Sources require finite centers on z=0 and a valid xy grid")
+    offsets, weights = aperture_quadrature(diameter, radial_order, angular_order)
+    extent = np.array([nx - 1, ny - 1])*dx
+    if np.any(centers[:, :2] - diameter/2 < dx) or np.any(centers[:, :2] + diameter/2 > extent - dx):
+        raise ValueError("Sinc aperture needs one grid cell clearance from side boundaries")
+    m = len(centers)
+    g = (centers[:, None, :2] + offsets[None, :, :2])/dx          # (M, Q, 2)
+    base = np.floor(g).astype(np.int64)
+    t = g - base
+    keys, vals = [], []
+    for a, b in ((0, 0), (0, 1), (1, 0), (1, 1)):
+        fx = t[..., 0] if a else 1 - t[..., 0]
+        fy = t[..., 1] if b else 1 - t[..., 1]
+        node = base[..., 0] + a + nx*(base[..., 1] + b)
+        keys.append(node + np.arange(m)[:, None]*(nx*ny))
+        vals.append(weights[None, :]*fx*fy)
+    keys = np.concatenate(keys, axis=1).ravel()
+    vals = np.concatenate(vals, axis=1).ravel()
+    uniq, inv = np.unique(keys, return_inverse=True)
+    mass = np.bincount(inv, weights=vals)
+    keep = mass != 0
+    uniq, mass = uniq[keep], mass[keep]
+    return uniq % (nx*ny), (uniq//(nx*ny)).astype(np.int32), mass
+
+
+def flux_scale(dx, reference_density):
+    if not np.isfinite(reference_density) or reference_density <= 0:
+        raise ValueError("reference_density must be positive")
+    return -2.0*np.pi/(reference_density*dx**3)
