@@ -1,0 +1,196 @@
+"""Benchmark: RHS solves/s at 128^3 to a 1e-6 scaled residual, mixed batched
+form vs the plain FP64 form, with the stencil's HBM roofline and a CPU
+baseline.
+
+A "step" is one batched solve of `--chunk` columns of the phase response
+basis G (transducers r .. r+chunk-1 at unit amplitude) to the reference's
+acceptance test (scaled residual <= 1e-6, backward error <= 1e-4) on the
+heterogeneous 128^3 scene (c = 343 (1 + 0.05 g), rigid sphere, PML on -x,
++-y, +z, 16x16 array).  Inputs (coefficients, source columns) are resident
+on the device before the timed region.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=2)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--n", type=int, default=128)
+    p.add_argument("--chunk", type=int, default=16)
+    p.add_argument("--restart", type=int, default=30)
+    p.add_argument("--maxiter", type=int, default=60000)
+    p.add_argument("--rtol", type=float, default=1e-6)
+    p.add_argument("--plain-rhs", type=int, default=1, help="columns timed in the plain FP64 form")
+    p.add_argument("--cpu-iters", type=int, default=60, help="scipy GMRES iterations in the CPU sample")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--stencil-reps", type=int, default=20)
+    return p.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by "
+              "right-hand side across GPUs, with no collectives in the solve loop).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from ultrasound_solver_amd import scenes
+    from ultrasound_solver_amd.operator import HelmholtzOperator
+
+    t0 = time.perf_counter()
+    scene = scenes.config_heterogeneous(n=args.n)
+    op = HelmholtzOperator(scene.rho, scene.c, scene.dx, scene.omega, scene.boundaries, scene.pml,
+                           device="cuda:0")
+    node, col, val = scenes.source_triplets(scene)
+    if np.any(op.pml_mask[:, :, 0].ravel(order="F")[node]):
+        raise ValueError("PML overlaps the deposited sinc aperture")
+    op.set_sources(node, col, val, scene.num_sources)
+    m = scene.num_sources
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    log(f"setup {setup_s:.1f}s n={args.n} N={op.n} M={m} ({'; '.join(scene.notes)})")
+
+    def rhs_block(start, k):
+        w = torch.zeros((k, m), dtype=torch.complex128, device="cuda:0")
+        idx = torch.arange(k, device="cuda:0")
+        w[idx, (start + idx) % m] = 1.0
+        return op.build_rhs(w)
+
+    common = dict(restart=args.restart, maxiter=args.maxiter, rtol=args.rtol, backward_error_tol=1e-4)
+    for s in range(args.warmup):
+        b = rhs_block(s*args.chunk, args.chunk)
+        _, st = op.gmres(b, precision="c64", chunk=args.chunk, **common)
+        log(f"warmup step {s}: iterations {[x['iterations'] for x in st]}")
+    torch.cuda.synchronize()
+    solved, iters, refines, worst = 0, [], [], 0.0
+    t_start = time.perf_counter()
+    for s in range(args.steps):
+        b = rhs_block((args.warmup + s)*args.chunk, args.chunk)
+        _, st = op.gmres(b, precision="c64", chunk=args.chunk, **common)
+        solved += sum(x["converged"] for x in st)
+        iters += [x["iterations"] for x in st]
+        refines += [x["refinements"] for x in st]
+        worst = max(worst, max(x["scaled_residual"] for x in st))
+        log(f"step {s}: converged {sum(x['converged'] for x in st)}/{len(st)}")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    mixed_rate = solved/elapsed
+    log(f"mixed: {solved} solves in {elapsed:.2f}s -> {mixed_rate:.3f}/s; median iterations {np.median(iters)}")
+
+    # Plain FP64 form (reference semantics, one right-hand side per launch).
+    plain_iters, plain_solved = [], 0
+    torch.cuda.synchronize()
+    tp = time.perf_counter()
+    for r in range(args.plain_rhs):
+        b = rhs_block(r, 1)
+        _, st = op.gmres(b, precision="c128", chunk=1, **common)
+        plain_solved += st[0]["converged"]
+        plain_iters.append(st[0]["iterations"])
+    torch.cuda.synchronize()
+    plain_elapsed = time.perf_counter() - tp
+    plain_rate = plain_solved/plain_elapsed if plain_elapsed > 0 else 0.0
+    log(f"plain: {plain_solved} solves in {plain_elapsed:.2f}s -> {plain_rate:.3f}/s; iterations {plain_iters}")
+
+    # Stencil roofline: R A M apply over `chunk` complex64 right-hand sides,
+    # timed with events on the stream the kernel is launched on.
+    k = args.chunk
+    x = torch.randn((k, op.n), dtype=torch.complex64, device="cuda:0")
+    for _ in range(3):
+        op.apply(x, mode="RAM", precision="c64")
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.stencil_reps):
+        op.apply(x, mode="RAM", precision="c64")
+    e1.record(stream)
+    torch.cuda.synchronize()
+    stencil_s = e0.elapsed_time(e1)/1e3/args.stencil_reps
+    alg_bytes = (16*k + 8)*op.n
+    achieved = alg_bytes/stencil_s/1e9
+    log(f"stencil k={k}: {stencil_s*1e6:.1f} us/launch, {achieved:.0f} GB/s algorithmic")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(scene, args, plain_iters)
+
+    out = {
+        "metric": "RHS solves/s at 128^3 to 1e-6 rel. residual (mixed vs FP64); stencil HBM GB/s",
+        "value": mixed_rate, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed/args.steps*1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64 (FP64 refinement)", "data": "synthetic",
+        "config": {"workload": f"heterogeneous {args.n}^3 scene, G columns in chunks of {args.chunk}, "
+                               f"Jacobi-GMRES({args.restart}) c64 + FP64 refinement to {args.rtol:g}",
+                   "grid": [args.n]*3, "rhs_per_step": args.chunk, "sources": m},
+        "plain_fp64": {"value": plain_rate, "unit": "solves/s", "rhs": args.plain_rhs,
+                       "iterations": plain_iters},
+        "speedup_mixed_vs_plain": mixed_rate/plain_rate if plain_rate > 0 else None,
+        "iterations_median": float(np.median(iters)) if iters else None,
+        "refinements_median": float(np.median(refines)) if refines else None,
+        "worst_scaled_residual": worst,
+        "setup_s": setup_s,
+        "roofline": {"kernel": "k_apply<float, MODE_RAM>", "bound": "hbm", "achieved": achieved,
+                     "peak": 8000.0, "unit": "GB/s", "frac": achieved/8000.0, "traffic": None,
+                     "alg_bytes_per_launch": alg_bytes, "launch_us": stencil_s*1e6},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(scene, args, plain_iters):
+    """scipy/numpy FP64 Jacobi-GMRES on the oracle CSR, timed on a bounded
+    sample of iterations and scaled by the iteration count the plain form
+    needed for the same column."""
+    from oracle import helmholtz as ohz, krylov as okr
+    try:
+        from threadpoolctl import threadpool_limits
+    except ImportError:
+        threadpool_limits = None
+    t0 = time.perf_counter()
+    a = ohz.OracleOperator(scene.rho, scene.c, scene.dx, scene.omega, scene.boundaries, scene.pml).matrix()
+    assemble_s = time.perf_counter() - t0
+    from ultrasound_solver_amd import scenes
+    node, col, val = scenes.source_triplets(scene)
+    b = np.zeros(a.shape[0], dtype=np.complex128)
+    sel = col == 0
+    b[node[sel]] = val[sel]
+    scale = okr.row_scale(a)
+    minv = okr.jacobi_inverse(a, scale)
+    bh = scale*b
+    ctx = threadpool_limits(1) if threadpool_limits else None
+    try:
+        if ctx:
+            ctx.__enter__()
+        t1 = time.perf_counter()
+        okr.cupy_style_gmres(lambda v: scale*(a @ v), lambda v: minv*v, bh, 1e-30, 0.0, args.restart,
+                             args.cpu_iters)
+        per_iter = (time.perf_counter() - t1)/args.cpu_iters
+    finally:
+        if ctx:
+            ctx.__exit__(None, None, None)
+    needed = plain_iters[0] if plain_iters else None
+    value = 1.0/(per_iter*needed) if needed else None
+    log(f"cpu: assemble {assemble_s:.1f}s, {per_iter*1e3:.1f} ms/iteration, needed {needed}")
+    return {"value": value, "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": f"{args.cpu_iters} Jacobi-GMRES({args.restart}) iterations (scipy CSR SpMV + numpy CGS, "
+                      f"1 thread) on the oracle's {args.n}^3 complex128 matrix, column 0; scaled by the "
+                      f"{needed} iterations the plain FP64 form needed",
+            "seconds_per_iteration": per_iter, "assemble_s": assemble_s}
+
+
+if __name__ == "__main__":
+    main()
