@@ -102,6 +102,7 @@ typedef struct hz_gmres_opts {
     int32_t max_refinements;    /* FP64 refinement steps (mixed form)        */
     double inner_rtol;          /* inner c64 tolerance floor (mixed form)    */
     int32_t use_graph;          /* capture each cycle in a hipGraph          */
+    int32_t preconditioner;     /* 0 Jacobi (reference), 1 multigrid (mixed) */
 } hz_gmres_opts;
 
 typedef struct hz_gmres_stats {
@@ -115,6 +116,20 @@ typedef struct hz_gmres_stats {
     int32_t finite;             /* all entries of x finite                   */
     int32_t converged;          /* reference acceptance test passed          */
 } hz_gmres_stats;
+
+/* One level of the shifted-Laplacian multigrid preconditioner: the same
+ * operator rediscretised on a coarser node set.  Tables are normalised by
+ * the fine spacing dx: weight = dual-cell width / dx, edge = edge length / dx
+ * (length n_a - 1), face terms i*omega*(w_b w_c s_b s_c / dx)*Y.            */
+typedef struct hz_level_desc {
+    int32_t nx, ny, nz;
+    const double *rho;          /* n_l densities                             */
+    const double *kappa;        /* n_l compressibilities                     */
+    const double *weight[3];
+    const double *stretch[3];   /* complex                                   */
+    const double *edge[3];
+    const double *face_term[6]; /* complex, NULL for reflecting/soft faces   */
+} hz_level_desc;
 
 hz_status hz_create(const hz_grid_desc *desc, hz_operator **out);
 hz_status hz_destroy(hz_operator *op);
@@ -150,6 +165,15 @@ hz_status hz_gather_sources(hz_operator *op, const void *lambda, int64_t ld,
 hz_status hz_sample(hz_operator *op, const int64_t *rows, const double *weights,
                     int32_t npoints, const void *u, int64_t ld, void *out,
                     int64_t ld_out, int32_t nrhs, void *stream);
+
+/* Install a multigrid preconditioner (levels[0] is the operator's own grid;
+ * level l+1 keeps the even nodes of level l plus its last node on each axis).
+ * The mass term of every level is shifted by omega^2 -> omega^2 (1 + i*shift);
+ * the smoother is damped Jacobi (weight) with `pre`/`post` sweeps and
+ * `coarse_sweeps` on the coarsest level.  No counterpart in the reference,
+ * which only offers Jacobi (src/algorithm/solvers/iterative_solver.py:46-56). */
+hz_status hz_set_multigrid(hz_operator *op, int32_t nlevels, const hz_level_desc *levels, double shift,
+                           int32_t pre, int32_t post, int32_t coarse_sweeps, double weight);
 
 /* Device bytes held by the operator (coefficients plus cached workspace). */
 int64_t hz_device_bytes(const hz_operator *op);

@@ -192,3 +192,41 @@ def test_sources_scatter_gather_and_sampling():
     s = op.sample(rows, wts, lam).cpu().numpy()
     ref_s = olo.target_response(lam, pts, (n, n, n), (0, 0, 0), dx).T
     assert _rel(s, ref_s) < 1e-14
+
+
+@pytest.mark.parametrize("adjoint,soft", [(False, False), (False, True), (True, True)])
+def test_multigrid_mixed_gmres_matches_direct(adjoint, soft):
+    """Shifted-Laplacian multigrid preconditioning changes the Krylov path, not
+    the answer: solutions match sparse LU to the field tolerance."""
+    from ultrasound_solver_amd.operator import HelmholtzOperator
+    n = 21
+    rho, c, bnd, pml = _scene(n, seed=3, soft=soft)
+    bnd["+y"] = {"type": "impedance", "impedance": [823.2, 205.8]}
+    dx, omega = 1.07e-3, 2*np.pi*40e3
+    op = HelmholtzOperator(rho, c, dx, omega, bnd, pml, device="cuda:0")
+    op.set_multigrid()
+    a = ohz.OracleOperator(rho, c, dx, omega, bnd, pml).matrix()
+    b = _point_rhs(op.n, 3, 11)
+    x, st = op.gmres(b, rtol=1e-8, precision="c64", chunk=3, maxiter=4000, adjoint=adjoint,
+                     preconditioner="multigrid")
+    mat = okr.adjoint_matrix(a) if adjoint else a
+    direct = okr.direct_solve(mat, b)
+    for r in range(3):
+        got = x[r].cpu().numpy()
+        assert st[r]["converged"] == 1, st[r]
+        assert np.linalg.norm(got - direct[:, r])/np.linalg.norm(direct[:, r]) < 1e-5
+
+
+def test_multigrid_needs_fewer_iterations_than_jacobi():
+    from ultrasound_solver_amd.operator import HelmholtzOperator
+    n = 32
+    rho, c, bnd, pml = _scene(n, seed=5)
+    pml = {"cells": 6}
+    op = HelmholtzOperator(rho, c, 1.07e-3, 2*np.pi*40e3, bnd, pml, device="cuda:0")
+    op.set_multigrid()
+    b = _point_rhs(op.n, 2, 13)
+    _, sj = op.gmres(b, rtol=1e-6, precision="c64", chunk=2, maxiter=20000)
+    _, sm = op.gmres(b, rtol=1e-6, precision="c64", chunk=2, maxiter=20000, preconditioner="multigrid")
+    for r in range(2):
+        assert sj[r]["converged"] and sm[r]["converged"]
+        assert sm[r]["iterations"]*3 < sj[r]["iterations"], (sm[r], sj[r])

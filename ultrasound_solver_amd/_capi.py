@@ -15,7 +15,8 @@ C128, C64 = 0, 1
 MODE_A, MODE_RAM, MODE_AH, MODE_RAMH = 0, 1, 2, 3
 
 EXPORTS = ("hz_create", "hz_destroy", "hz_apply", "hz_gmres", "hz_set_sources", "hz_build_rhs",
-           "hz_gather_sources", "hz_sample", "hz_device_bytes", "hz_last_error")
+           "hz_gather_sources", "hz_sample", "hz_device_bytes", "hz_last_error", "hz_set_multigrid")
+PRE_JACOBI, PRE_MULTIGRID = 0, 1
 
 
 class GridDesc(ctypes.Structure):
@@ -30,7 +31,14 @@ class GmresOpts(ctypes.Structure):
     _fields_ = [("restart", c_int32), ("maxiter", c_int32), ("rtol", c_double),
                 ("backward_error_tol", c_double), ("precision", c_int32), ("adjoint", c_int32),
                 ("chunk", c_int32), ("max_refinements", c_int32), ("inner_rtol", c_double),
-                ("use_graph", c_int32)]
+                ("use_graph", c_int32), ("preconditioner", c_int32)]
+
+
+class LevelDesc(ctypes.Structure):
+    _fields_ = [("nx", c_int32), ("ny", c_int32), ("nz", c_int32),
+                ("rho", POINTER(c_double)), ("kappa", POINTER(c_double)),
+                ("weight", POINTER(c_double)*3), ("stretch", POINTER(c_double)*3),
+                ("edge", POINTER(c_double)*3), ("face_term", POINTER(c_double)*6)]
 
 
 class GmresStats(ctypes.Structure):
@@ -71,6 +79,8 @@ def lib():
         handle.hz_gather_sources.argtypes = [c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_int32, c_void_p]
         handle.hz_sample.argtypes = [c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_int64, c_void_p,
                                      c_int64, c_int32, c_void_p]
+        handle.hz_set_multigrid.argtypes = [c_void_p, c_int32, POINTER(LevelDesc), c_double, c_int32, c_int32,
+                                            c_int32, c_double]
         handle.hz_device_bytes.argtypes = [c_void_p]
         handle.hz_device_bytes.restype = c_int64
         handle.hz_last_error.restype = c_char_p

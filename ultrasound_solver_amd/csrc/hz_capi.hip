@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -13,6 +14,7 @@
 
 #include "../../include/hz.h"
 #include "hz_krylov.hpp"
+#include "hz_mg.hpp"
 #include "hz_stencil.hpp"
 
 using namespace hz;
@@ -72,6 +74,24 @@ int64_t blocks_for(int64_t n, int64_t per_block, int64_t cap) {
     return std::max<int64_t>(1, std::min<int64_t>(cap, (n + per_block - 1) / per_block));
 }
 
+// One multigrid level (complex64 only: the preconditioner runs in the mixed form).
+struct MgLevel {
+    int nx = 0, ny = 0, nz = 0;
+    int64_t n = 0;
+    DevBuf rho, kap, a[3], q[3], face[6], dinv;
+    OpView<float> v{};
+    TileMap tm{};
+    DevBuf bb, xx, tt;  // k-slot work vectors: right-hand side, iterate, scratch
+};
+
+struct Multigrid {
+    std::vector<std::unique_ptr<MgLevel>> levels;
+    int pre = 1, post = 1, coarse_sweeps = 30;
+    float weight = 0.7f;
+    double shift = 0.5;
+    int kcap = 0;
+};
+
 }  // namespace
 
 struct hz_operator {
@@ -94,6 +114,7 @@ struct hz_operator {
     // Cached GMRES workspace (grow-only).
     DevBuf V, X, Bh, dpart, npart, cpart, red, Hm, gc, gs, g, scales, coef, resid, tol, steps, live, member;
     DevBuf rhs32;
+    std::unique_ptr<Multigrid> mg;
 };
 
 namespace {
@@ -123,6 +144,8 @@ void apply_mode(const hz_operator *op, int mode, const cx<T> *x, int64_t ldx, cx
         case MODE_RAM: launch_apply<T, MODE_RAM>(op, x, ldx, y, ldy, nrhs, live, scale, s); break;
         case MODE_AH: launch_apply<T, MODE_AH>(op, x, ldx, y, ldy, nrhs, live, scale, s); break;
         case MODE_RAMH: launch_apply<T, MODE_RAMH>(op, x, ldx, y, ldy, nrhs, live, scale, s); break;
+        case MODE_RA: launch_apply<T, MODE_RA>(op, x, ldx, y, ldy, nrhs, live, scale, s); break;
+        case MODE_RAH: launch_apply<T, MODE_RAH>(op, x, ldx, y, ldy, nrhs, live, scale, s); break;
         default: throw BadArgument("unknown apply mode");
     }
 }
@@ -132,13 +155,90 @@ void residual_mode(const hz_operator *op, int mode, const cx<T> *x, const cx<T> 
                    const int *live, double *partial, hipStream_t s) {
     const dim3 grid(op->tm.nblocks, k);
     const int64_t n = op->n;
-    if (mode == MODE_RAM)
-        hipLaunchKernelGGL((k_residual<T, MODE_RAM>), grid, dim3(256), 0, s, view<T>(op), op->tm, x, n, b, n, out,
-                           n, live, partial);
-    else
-        hipLaunchKernelGGL((k_residual<T, MODE_RAMH>), grid, dim3(256), 0, s, view<T>(op), op->tm, x, n, b, n, out,
-                           n, live, partial);
+    switch (mode) {
+        case MODE_RAM:
+            hipLaunchKernelGGL((k_residual<T, MODE_RAM>), grid, dim3(256), 0, s, view<T>(op), op->tm, x, n, b, n,
+                               out, n, live, partial);
+            break;
+        case MODE_RAMH:
+            hipLaunchKernelGGL((k_residual<T, MODE_RAMH>), grid, dim3(256), 0, s, view<T>(op), op->tm, x, n, b, n,
+                               out, n, live, partial);
+            break;
+        case MODE_RA:
+            hipLaunchKernelGGL((k_residual<T, MODE_RA>), grid, dim3(256), 0, s, view<T>(op), op->tm, x, n, b, n,
+                               out, n, live, partial);
+            break;
+        case MODE_RAH:
+            hipLaunchKernelGGL((k_residual<T, MODE_RAH>), grid, dim3(256), 0, s, view<T>(op), op->tm, x, n, b, n,
+                               out, n, live, partial);
+            break;
+        default: throw BadArgument("unknown residual mode");
+    }
     HZ_LAUNCHED();
+}
+
+// ------------------------------------------------------------------ multigrid
+void mg_reserve(Multigrid &mg, int k) {
+    if (k <= mg.kcap) return;
+    for (auto &L : mg.levels) {
+        const size_t bytes = (size_t)k * L->n * sizeof(cx<float>);
+        L->bb.ensure(bytes);
+        L->xx.ensure(bytes);
+        L->tt.ensure(bytes);
+    }
+    mg.kcap = k;
+}
+
+template <int MODE>
+void level_smooth(MgLevel &L, const cx<float> *x, const cx<float> *b, cx<float> *out, float w, int k,
+                  const int *live, hipStream_t s) {
+    hipLaunchKernelGGL((k_level_op<float, MODE, EPI_SMOOTH>), dim3(L.tm.nblocks), dim3(256), 0, s, L.v, L.tm, x, b,
+                       out, (const cx<float> *)L.dinv.as<cx<float>>(), w, k, live);
+    HZ_LAUNCHED();
+}
+
+// x = V-cycle(b) on level l; x and b hold k slots of length n_l.  The result
+// is left in `x` (which must not alias b).
+template <int MODE>
+void vcycle(Multigrid &mg, int l, const cx<float> *b, cx<float> *x, int k, const int *live, hipStream_t s) {
+    MgLevel &L = *mg.levels[l];
+    const bool coarsest = (l + 1 == (int)mg.levels.size());
+    const int64_t n = L.n;
+    cx<float> *cur = x, *alt = L.tt.as<cx<float>>();
+    hipLaunchKernelGGL((k_level_jacobi0<float, MODE>), dim3(blocks_for(n, 256 * 8, 2048), k), dim3(256), 0, s, b,
+                       cur, (const cx<float> *)L.dinv.as<cx<float>>(), mg.weight, n, live);
+    HZ_LAUNCHED();
+    const int first = coarsest ? mg.coarse_sweeps : mg.pre;
+    for (int it = 1; it < first; ++it) {
+        level_smooth<MODE>(L, cur, b, alt, mg.weight, k, live, s);
+        std::swap(cur, alt);
+    }
+    if (!coarsest) {
+        MgLevel &C = *mg.levels[l + 1];
+        hipLaunchKernelGGL((k_level_op<float, MODE, EPI_RESID>), dim3(L.tm.nblocks), dim3(256), 0, s, L.v, L.tm, cur,
+                           b, alt, (const cx<float> *)nullptr, 0.0f, k, live);
+        HZ_LAUNCHED();
+        hipLaunchKernelGGL((k_restrict<float>), dim3((C.n + 255) / 256), dim3(256), 0, s, alt, L.nx, L.ny, L.nz,
+                           C.bb.as<cx<float>>(), C.nx, C.ny, C.nz, k, live);
+        HZ_LAUNCHED();
+        vcycle<MODE>(mg, l + 1, C.bb.as<cx<float>>(), C.xx.as<cx<float>>(), k, live, s);
+        hipLaunchKernelGGL((k_prolong_add<float>), dim3((n + 255) / 256), dim3(256), 0, s,
+                           (const cx<float> *)C.xx.as<cx<float>>(), C.nx, C.ny, C.nz, cur, L.nx, L.ny, L.nz, k, live);
+        HZ_LAUNCHED();
+        for (int it = 0; it < mg.post; ++it) {
+            level_smooth<MODE>(L, cur, b, alt, mg.weight, k, live, s);
+            std::swap(cur, alt);
+        }
+    }
+    if (cur != x) HZ_HIP(hipMemcpyAsync(x, cur, (size_t)k * n * sizeof(cx<float>), hipMemcpyDeviceToDevice, s));
+}
+
+void precondition(hz_operator *op, bool adjoint, const cx<float> *b, cx<float> *x, int k, const int *live,
+                  hipStream_t s) {
+    if (adjoint)
+        vcycle<MODE_AH>(*op->mg, 0, b, x, k, live, s);
+    else
+        vcycle<MODE_A>(*op->mg, 0, b, x, k, live, s);
 }
 
 // Per-RHS bookkeeping of one chunk.
@@ -155,9 +255,16 @@ struct ChunkState {
 //   the cycle top (the reference/CuPy loop); otherwise a slot leaves its cycle
 //   as soon as the Givens estimate meets its tolerance.
 // On return X holds y for every slot that was active on entry.
+// Preconditioner of the inner solve: Jacobi (operator R A M, the reference's
+// choice) or shifted-Laplacian multigrid P (operator R A P).
+enum Precond { PRE_JACOBI = 0, PRE_MULTIGRID = 1 };
+
 template <typename T>
 void gmres_cycles(hz_operator *op, int mode_ram, int k, int restart, int maxiter, bool ref_semantics,
-                  const std::vector<double> &atol, ChunkState &cs, hipStream_t s) {
+                  const std::vector<double> &atol, ChunkState &cs, hipStream_t s, int precond = PRE_JACOBI) {
+    const bool use_mg = precond == PRE_MULTIGRID;
+    const bool adjoint = (mode_ram == MODE_RAMH);
+    const int mode_op = use_mg ? (adjoint ? MODE_RAH : MODE_RA) : mode_ram;
     const int64_t n = op->n;
     const int maxv = restart + 1;
     const int64_t nblk = vec_blocks(n);
@@ -190,7 +297,15 @@ void gmres_cycles(hz_operator *op, int mode_ram, int k, int restart, int maxiter
         for (int r = 0; r < k; ++r) live_h[r] = cs.active[r];
         HZ_HIP(hipMemcpyAsync(st.live, live_h.data(), k * sizeof(int), hipMemcpyHostToDevice, s));
         // Cycle top: V_0 = Bh - R A M X (true residual of the preconditioned system).
-        residual_mode<T>(op, mode_ram, X, Bh, V, k, st.live, op->npart.as<double>(), s);
+        const cx<T> *src = X;
+        if constexpr (std::is_same<T, float>::value) {
+            if (use_mg) {
+                cx<float> *z = op->mg->levels[0]->xx.as<cx<float>>();
+                precondition(op, adjoint, X, z, k, st.live, s);
+                src = z;
+            }
+        }
+        residual_mode<T>(op, mode_op, src, Bh, V, k, st.live, op->npart.as<double>(), s);
         hipLaunchKernelGGL(k_cycle_start, dim3(k), dim3(256), 0, s, st, op->npart.as<double>(),
                            (int)op->tm.nblocks, maxv, restart);
         HZ_LAUNCHED();
@@ -213,7 +328,15 @@ void gmres_cycles(hz_operator *op, int mode_ram, int k, int restart, int maxiter
         for (int j = 0; j < restart; ++j) {
             const cx<T> *vj = V + (int64_t)j * k * n;
             cx<T> *w = V + (int64_t)(j + 1) * k * n;
-            apply_mode<T>(op, mode_ram, vj, n, w, n, k, st.live, st.scales + (int64_t)j * k, s);
+            const cx<T> *in = vj;
+            if constexpr (std::is_same<T, float>::value) {
+                if (use_mg) {
+                    cx<float> *z = op->mg->levels[0]->xx.as<cx<float>>();
+                    precondition(op, adjoint, vj, z, k, st.live, s);
+                    in = z;
+                }
+            }
+            apply_mode<T>(op, mode_op, in, n, w, n, k, st.live, st.scales + (int64_t)j * k, s);
             const int groups = (j + 1 + DOT_GROUP - 1) / DOT_GROUP;
             hipLaunchKernelGGL((k_dots<T>), dim3(nblk, k, groups), dim3(256), 0, s, V, k, n, j + 1, w, st.live,
                                op->dpart.as<double2>(), maxv, chunk);
@@ -408,16 +531,27 @@ void solve_mixed(hz_operator *op, const hz_gmres_opts &o, const cx<double> *b, i
         for (int r = 0; r < k; ++r) cs.active[r] = member[r] = done[r] ? 0 : 1;
         // The iteration budget is cumulative over refinements (cs.iterations carries over).
         ChunkState inner = cs;
-        gmres_cycles<float>(op, o.adjoint ? MODE_RAMH : MODE_RAM, k, o.restart, o.maxiter, false, atol, inner, s);
+        const bool use_mg = o.preconditioner == PRE_MULTIGRID;
+        if (use_mg) mg_reserve(*op->mg, k);
+        gmres_cycles<float>(op, o.adjoint ? MODE_RAMH : MODE_RAM, k, o.restart, o.maxiter, false, atol, inner, s,
+                            o.preconditioner);
         for (int r = 0; r < k; ++r) {
             cs.iterations[r] = inner.iterations[r];
             cs.cycles[r] = inner.cycles[r];
             if (member[r]) refinements[r] += 1;
         }
         HZ_HIP(hipMemcpyAsync(op->member.p, member.data(), k * sizeof(int), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL((k_precond_out<float>), dim3(vec_blocks(n), k), dim3(256), 0, s,
-                           (const cx<float> *)op->X.as<cx<float>>(), n, m, x, ldx, n,
-                           (const int *)op->member.as<int>(), 1);
+        if (use_mg) {
+            // x += P X: one more V-cycle on the accumulated Krylov combination.
+            cx<float> *z = op->mg->levels[0]->xx.as<cx<float>>();
+            precondition(op, o.adjoint != 0, op->X.as<cx<float>>(), z, k, op->member.as<int>(), s);
+            hipLaunchKernelGGL(k_accumulate, dim3(vec_blocks(n), k), dim3(256), 0, s, (const cx<float> *)z, n, x, ldx,
+                               (const int *)op->member.as<int>());
+        } else {
+            hipLaunchKernelGGL((k_precond_out<float>), dim3(vec_blocks(n), k), dim3(256), 0, s,
+                               (const cx<float> *)op->X.as<cx<float>>(), n, m, x, ldx, n,
+                               (const int *)op->member.as<int>(), 1);
+        }
         HZ_LAUNCHED();
     }
     for (int r = 0; r < k; ++r) {
@@ -558,6 +692,7 @@ hz_status hz_create(const hz_grid_desc *d, hz_operator **out) {
             v.nz = op->nz;
             v.n = op->n;
             v.omega2 = (TT)(op->omega * op->omega);
+            v.omega2_im = 0;
             v.rho = rho;
             v.kappa = kap;
             for (int ax = 0; ax < 3; ++ax) {
@@ -627,6 +762,12 @@ hz_status hz_gmres(hz_operator *op, const hz_gmres_opts *opts, const void *b, vo
         if (o.maxiter < 0) throw BadArgument("maxiter must be >= 0");
         if (!(o.rtol > 0) || !(o.backward_error_tol > 0)) throw BadArgument("tolerances must be positive");
         if (o.precision != HZ_C128 && o.precision != HZ_C64) throw BadArgument("unknown precision");
+        if (o.preconditioner == PRE_MULTIGRID) {
+            if (o.precision != HZ_C64) throw BadArgument("the multigrid preconditioner runs in the mixed form only");
+            if (!op->mg) throw BadArgument("no multigrid hierarchy set (hz_set_multigrid)");
+        } else if (o.preconditioner != PRE_JACOBI) {
+            throw BadArgument("unknown preconditioner");
+        }
         hipStream_t s = static_cast<hipStream_t>(stream);
         const int chunk = std::max(1, std::min<int>(nrhs, o.chunk > 0 ? o.chunk : 1));
         for (int c0 = 0; c0 < nrhs; c0 += chunk) {
@@ -643,6 +784,96 @@ hz_status hz_gmres(hz_operator *op, const hz_gmres_opts *opts, const void *b, vo
     for (int r = 0; r < nrhs; ++r)
         if (!stats[r].converged) return HZ_NOT_CONVERGED;
     return HZ_OK;
+}
+
+hz_status hz_set_multigrid(hz_operator *op, int32_t nlevels, const hz_level_desc *levels, double shift, int32_t pre,
+                           int32_t post, int32_t coarse_sweeps, double weight) {
+    return guarded([&] {
+        check_op(op);
+        if (nlevels < 1 || !levels) throw BadArgument("need at least one level");
+        if (pre < 1 || post < 0 || coarse_sweeps < 1 || !(weight > 0) || !std::isfinite(shift))
+            throw BadArgument("bad multigrid smoothing parameters");
+        auto mg = std::make_unique<Multigrid>();
+        mg->pre = pre;
+        mg->post = post;
+        mg->coarse_sweeps = coarse_sweeps;
+        mg->weight = (float)weight;
+        mg->shift = shift;
+        const double h2 = op->dx * op->dx;
+        for (int l = 0; l < nlevels; ++l) {
+            const hz_level_desc &d = levels[l];
+            const int dims[3] = {d.nx, d.ny, d.nz};
+            if (l == 0 && (d.nx != op->nx || d.ny != op->ny || d.nz != op->nz))
+                throw BadArgument("level 0 must be the operator's grid");
+            if (l > 0) {
+                const int prev[3] = {levels[l - 1].nx, levels[l - 1].ny, levels[l - 1].nz};
+                for (int a = 0; a < 3; ++a) {
+                    const int expect = (prev[a] - 1) / 2 + 1 + ((prev[a] - 1) % 2);
+                    if (dims[a] != expect) throw BadArgument("level sizes must follow even-node coarsening");
+                }
+            }
+            if (d.nx < 2 || d.ny < 2 || d.nz < 2 || !d.rho || !d.kappa) throw BadArgument("bad level grid");
+            auto L = std::make_unique<MgLevel>();
+            L->nx = d.nx;
+            L->ny = d.ny;
+            L->nz = d.nz;
+            L->n = (int64_t)d.nx * d.ny * d.nz;
+            std::vector<float> f32((size_t)L->n);
+            for (int64_t p = 0; p < L->n; ++p) f32[p] = (float)d.rho[p];
+            upload(L->rho, f32.data(), L->n);
+            for (int64_t p = 0; p < L->n; ++p) f32[p] = (float)d.kappa[p];
+            upload(L->kap, f32.data(), L->n);
+            for (int a = 0; a < 3; ++a) {
+                const int na = dims[a];
+                if (!d.weight[a] || !d.stretch[a] || !d.edge[a]) throw BadArgument("level tables are required");
+                std::vector<cx<float>> av(na), qv(std::max(na - 1, 1));
+                for (int i = 0; i < na; ++i)
+                    av[i] = cast<float>(d.weight[a][i] * mk<double>(d.stretch[a][2 * i], d.stretch[a][2 * i + 1]));
+                for (int i = 0; i + 1 < na; ++i) {
+                    const cx<double> s0 = mk<double>(d.stretch[a][2 * i], d.stretch[a][2 * i + 1]);
+                    const cx<double> s1 = mk<double>(d.stretch[a][2 * i + 2], d.stretch[a][2 * i + 3]);
+                    qv[i] = cast<float>(4.0 * crecip((d.edge[a][i] * h2) * (s0 + s1)));
+                }
+                upload(L->a[a], av.data(), na);
+                upload(L->q[a], qv.data(), qv.size());
+            }
+            for (int f = 0; f < 6; ++f) {
+                if (!d.face_term[f]) continue;
+                const int ax = f / 2;
+                const int b0 = ax == 0 ? 1 : 0, c0 = ax == 2 ? 1 : 2;
+                const int64_t cnt = (int64_t)dims[b0] * dims[c0];
+                std::vector<cx<float>> t32(cnt);
+                for (int64_t e = 0; e < cnt; ++e)
+                    t32[e] = mk<float>((float)d.face_term[f][2 * e], (float)d.face_term[f][2 * e + 1]);
+                upload(L->face[f], t32.data(), cnt);
+            }
+            OpView<float> &v = L->v;
+            v.nx = L->nx;
+            v.ny = L->ny;
+            v.nz = L->nz;
+            v.n = L->n;
+            v.omega2 = (float)(op->omega * op->omega);
+            v.omega2_im = (float)(shift * op->omega * op->omega);
+            v.rho = L->rho.as<float>();
+            v.kappa = L->kap.as<float>();
+            for (int a = 0; a < 3; ++a) {
+                v.a[a] = L->a[a].as<cx<float>>();
+                v.q[a] = L->q[a].as<cx<float>>();
+            }
+            for (int f = 0; f < 6; ++f) v.face[f] = L->face[f].p ? L->face[f].as<cx<float>>() : nullptr;
+            v.soft = op->soft;
+            v.rs_f = v.rs_a = nullptr;
+            v.m_f = v.m_a = nullptr;
+            L->tm = make_tilemap(L->nx, L->ny, L->nz, 4);
+            L->dinv.ensure(L->n * sizeof(cx<float>));
+            hipLaunchKernelGGL((k_level_dinv<float>), dim3((L->n + 255) / 256), dim3(256), 0, 0, L->v,
+                               L->dinv.as<cx<float>>());
+            HZ_LAUNCHED();
+            mg->levels.push_back(std::move(L));
+        }
+        HZ_HIP(hipDeviceSynchronize());
+        op->mg = std::move(mg);
+    });
 }
 
 hz_status hz_set_sources(hz_operator *op, int32_t num_columns, int64_t nnz, const int64_t *node,
@@ -748,6 +979,9 @@ int64_t hz_device_bytes(const hz_operator *op) {
                             &op->ma64, &op->rsf32, &op->mf32, &op->rsa32, &op->ma32, &op->V, &op->X, &op->Bh,
                             &op->dpart, &op->npart, &op->cpart, &op->rhs32};
     for (const DevBuf *bf : bufs) total += (int64_t)bf->bytes;
+    if (op->mg)
+        for (const auto &L : op->mg->levels)
+            total += (int64_t)(L->rho.bytes + L->kap.bytes + L->dinv.bytes + L->bb.bytes + L->xx.bytes + L->tt.bytes);
     return total;
 }
 

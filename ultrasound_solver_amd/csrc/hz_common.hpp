@@ -54,6 +54,7 @@ struct OpView {
     int nx, ny, nz;
     int64_t n;
     T omega2;
+    T omega2_im;        // imaginary mass shift beta * omega^2 (multigrid levels), else 0
     const T *rho;       // N
     const T *kappa;     // N
     const cx<T> *a[3];  // w*s per axis, length n_axis

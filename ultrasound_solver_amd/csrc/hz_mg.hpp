@@ -1,0 +1,183 @@
+// Complex-shifted-Laplacian multigrid preconditioner kernels.
+//
+// Each level is the same finite-volume operator rediscretised on a coarser
+// node set with the mass term shifted, omega^2 -> omega^2 (1 + i beta)
+// (physical damping for the exp(-i omega t) convention used throughout).
+// Coarse node sets keep every even fine index plus the last node, so any
+// grid size coarsens (the last coarse interval may be one fine cell long);
+// the per-axis dual widths and edge lengths carry the non-uniform spacing.
+//
+// Transfers are trilinear prolongation P and its transpose R = P^T (rows are
+// integrated over control volumes with the fine-grid normalisation on every
+// level, so plain P^T is the consistent restriction).  The smoother is
+// damped Jacobi on the shifted operator.  Every kernel handles k
+// right-hand sides, evaluating the stencil coefficients once per node.
+#pragma once
+
+#include "hz_stencil.hpp"
+
+namespace hz {
+
+// Fine index f is a coarse node iff it is even or the last node.
+__device__ __forceinline__ bool coarse_node(int f, int n) { return (f & 1) == 0 || f == n - 1; }
+// Coarse index of a coarse-node fine index f (m = coarse size).
+__device__ __forceinline__ int coarse_of(int f, int n, int m) { return f == n - 1 ? m - 1 : f >> 1; }
+// Fine index of coarse index c.
+__device__ __forceinline__ int fine_of(int c, int n) { return min(2 * c, n - 1); }
+
+// Inverse diagonal of the shifted level operator (identity rows -> 1); for the
+// adjoint smoother the conjugate is used.
+template <typename T>
+__global__ __launch_bounds__(256) void k_level_dinv(OpView<T> op, cx<T> *__restrict__ dinv) {
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= op.n) return;
+    const int i = (int)(p % op.nx);
+    const int j = (int)((p / op.nx) % op.ny);
+    const int k = (int)(p / ((int64_t)op.nx * op.ny));
+    const Nbr nb = neighbours(i, j, k, op.nx, op.ny, op.nz);
+    cx<T> c[6], d;
+    edge_coefs<T, true>(op, i, j, k, p, nb, c, &d);
+    dinv[p] = node_fixed(op, i, j, k) ? mk<T>(1, 0) : crecip(d);
+}
+
+enum LevelEpi { EPI_SMOOTH = 0, EPI_RESID = 1 };
+
+// EPI_SMOOTH: xo = xi + w * dinv * (b - A xi)      (damped Jacobi sweep)
+// EPI_RESID : xo = b - A xi
+// MODE is MODE_A (forward) or MODE_AH (adjoint; dinv conjugated).
+template <typename T, int MODE, int EPI>
+__global__ __launch_bounds__(256) void k_level_op(OpView<T> op, TileMap tm, const cx<T> *__restrict__ xi,
+                                                  const cx<T> *__restrict__ b, cx<T> *__restrict__ xo,
+                                                  const cx<T> *__restrict__ dinv, T w, int k,
+                                                  const int *__restrict__ live) {
+    const int tile = xcd_tile(blockIdx.x, tm.nblocks);
+    if (tile >= tm.ntiles) return;
+    const int tx = tile % tm.tiles_x;
+    const int ty = (tile / tm.tiles_x) % tm.tiles_y;
+    const int tz = tile / (tm.tiles_x * tm.tiles_y);
+    const int i = tx * 64 + (threadIdx.x & 63);
+    const int j = ty * 4 + (threadIdx.x >> 6);
+    if (i >= op.nx || j >= op.ny) return;
+    const int64_t n = op.n;
+    const int k0 = tz * tm.zb;
+    const int k1 = min(op.nz, k0 + tm.zb);
+    for (int kk = k0; kk < k1; ++kk) {
+        const int64_t p = i + (int64_t)op.nx * (j + (int64_t)op.ny * kk);
+        const Nbr nb = neighbours(i, j, kk, op.nx, op.ny, op.nz);
+        cx<T> cs, c[6];
+        row_coefs<T, MODE>(op, i, j, kk, p, nb, cs, c);
+        cx<T> dv = mk<T>(0, 0);
+        if (EPI == EPI_SMOOTH) dv = (MODE == MODE_AH) ? conj(dinv[p]) : dinv[p];
+        for (int r = 0; r < k; ++r) {
+            if (live && !live[r]) continue;
+            const cx<T> *xr = xi + r * n + p;
+            const cx<T> x0 = xr[0];
+            cx<T> acc = cs * x0;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) acc = acc + c[q] * xr[nb.off[q]];
+            const cx<T> res = b[r * n + p] - acc;
+            xo[r * n + p] = (EPI == EPI_SMOOTH) ? x0 + w * (dv * res) : res;
+        }
+    }
+}
+
+// First sweep from a zero guess: xo = w * dinv * b.
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_level_jacobi0(const cx<T> *__restrict__ b, cx<T> *__restrict__ xo,
+                                                       const cx<T> *__restrict__ dinv, T w, int64_t n,
+                                                       const int *__restrict__ live) {
+    const int r = blockIdx.y;
+    if (live && !live[r]) return;
+    for (int64_t p = blockIdx.x * 256 + threadIdx.x; p < n; p += (int64_t)gridDim.x * 256) {
+        const cx<T> dv = (MODE == MODE_AH) ? conj(dinv[p]) : dinv[p];
+        xo[r * n + p] = w * (dv * b[r * n + p]);
+    }
+}
+
+// Coarse right-hand side bc = P^T rf.  One thread per coarse node.
+template <typename T>
+__global__ __launch_bounds__(256) void k_restrict(const cx<T> *__restrict__ rf, int nxf, int nyf, int nzf,
+                                                  cx<T> *__restrict__ bc, int nxc, int nyc, int nzc, int k,
+                                                  const int *__restrict__ live) {
+    const int64_t nc = (int64_t)nxc * nyc * nzc;
+    const int64_t nf = (int64_t)nxf * nyf * nzf;
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= nc) return;
+    const int ic = (int)(q % nxc);
+    const int jc = (int)((q / nxc) % nyc);
+    const int kc = (int)(q / ((int64_t)nxc * nyc));
+    const int fi = fine_of(ic, nxf), fj = fine_of(jc, nyf), fk = fine_of(kc, nzf);
+    T wx[3], wy[3], wz[3];
+    wx[1] = wy[1] = wz[1] = 1;
+    wx[0] = (fi - 1 >= 0 && !coarse_node(fi - 1, nxf)) ? T(0.5) : T(0);
+    wx[2] = (fi + 1 < nxf && !coarse_node(fi + 1, nxf)) ? T(0.5) : T(0);
+    wy[0] = (fj - 1 >= 0 && !coarse_node(fj - 1, nyf)) ? T(0.5) : T(0);
+    wy[2] = (fj + 1 < nyf && !coarse_node(fj + 1, nyf)) ? T(0.5) : T(0);
+    wz[0] = (fk - 1 >= 0 && !coarse_node(fk - 1, nzf)) ? T(0.5) : T(0);
+    wz[2] = (fk + 1 < nzf && !coarse_node(fk + 1, nzf)) ? T(0.5) : T(0);
+    for (int r = 0; r < k; ++r) {
+        if (live && !live[r]) continue;
+        const cx<T> *src = rf + r * nf;
+        cx<T> acc = mk<T>(0, 0);
+        for (int dz = 0; dz < 3; ++dz) {
+            if (wz[dz] == T(0)) continue;
+            const int64_t zoff = (int64_t)(fk + dz - 1) * nxf * nyf;
+            for (int dy = 0; dy < 3; ++dy) {
+                if (wy[dy] == T(0)) continue;
+                const int64_t yoff = zoff + (int64_t)(fj + dy - 1) * nxf;
+                const T wyz = wz[dz] * wy[dy];
+                for (int dx = 0; dx < 3; ++dx) {
+                    if (wx[dx] == T(0)) continue;
+                    acc = acc + (wyz * wx[dx]) * src[yoff + fi + dx - 1];
+                }
+            }
+        }
+        bc[r * nc + q] = acc;
+    }
+}
+
+// xf += P xc.  One thread per fine node; up to 8 coarse contributions.
+template <typename T>
+__global__ __launch_bounds__(256) void k_prolong_add(const cx<T> *__restrict__ xc, int nxc, int nyc, int nzc,
+                                                     cx<T> *__restrict__ xf, int nxf, int nyf, int nzf, int k,
+                                                     const int *__restrict__ live) {
+    const int64_t nf = (int64_t)nxf * nyf * nzf;
+    const int64_t nc = (int64_t)nxc * nyc * nzc;
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= nf) return;
+    const int i = (int)(p % nxf);
+    const int j = (int)((p / nxf) % nyf);
+    const int kk = (int)(p / ((int64_t)nxf * nyf));
+    int ci[2], cj[2], ck[2];
+    T wi[2], wj[2], wk[2];
+    int ni, nj, nk;
+    if (coarse_node(i, nxf)) { ci[0] = coarse_of(i, nxf, nxc); wi[0] = 1; ni = 1; }
+    else { ci[0] = coarse_of(i - 1, nxf, nxc); ci[1] = coarse_of(i + 1, nxf, nxc); wi[0] = wi[1] = T(0.5); ni = 2; }
+    if (coarse_node(j, nyf)) { cj[0] = coarse_of(j, nyf, nyc); wj[0] = 1; nj = 1; }
+    else { cj[0] = coarse_of(j - 1, nyf, nyc); cj[1] = coarse_of(j + 1, nyf, nyc); wj[0] = wj[1] = T(0.5); nj = 2; }
+    if (coarse_node(kk, nzf)) { ck[0] = coarse_of(kk, nzf, nzc); wk[0] = 1; nk = 1; }
+    else { ck[0] = coarse_of(kk - 1, nzf, nzc); ck[1] = coarse_of(kk + 1, nzf, nzc); wk[0] = wk[1] = T(0.5); nk = 2; }
+    for (int r = 0; r < k; ++r) {
+        if (live && !live[r]) continue;
+        const cx<T> *src = xc + r * nc;
+        cx<T> acc = mk<T>(0, 0);
+        for (int c = 0; c < nk; ++c)
+            for (int b = 0; b < nj; ++b)
+                for (int a = 0; a < ni; ++a)
+                    acc = acc + (wk[c] * wj[b] * wi[a]) *
+                                    src[ci[a] + (int64_t)nxc * (cj[b] + (int64_t)nyc * ck[c])];
+        xf[r * nf + p] = xf[r * nf + p] + acc;
+    }
+}
+
+// x64 += cast<double>(z): apply a complex64 correction to the FP64 iterate.
+__global__ __launch_bounds__(256) void k_accumulate(const cx<float> *__restrict__ z, int64_t n,
+                                                    cx<double> *__restrict__ x, int64_t ldx,
+                                                    const int *__restrict__ member) {
+    const int r = blockIdx.y;
+    if (member && !member[r]) return;
+    for (int64_t p = blockIdx.x * 256 + threadIdx.x; p < n; p += (int64_t)gridDim.x * 256)
+        x[r * ldx + p] = x[r * ldx + p] + cast<double>(z[r * n + p]);
+}
+
+}  // namespace hz

@@ -19,7 +19,7 @@
 
 namespace hz {
 
-enum ApplyMode { MODE_A = 0, MODE_RAM = 1, MODE_AH = 2, MODE_RAMH = 3 };
+enum ApplyMode { MODE_A = 0, MODE_RAM = 1, MODE_AH = 2, MODE_RAMH = 3, MODE_RA = 4, MODE_RAH = 5 };
 
 struct TileMap {
     int tiles_x, tiles_y, tiles_z, zb;
@@ -97,7 +97,10 @@ __device__ __forceinline__ void edge_coefs(const OpView<T> &op, int i, int j, in
         c[5] = zero;
     }
     if (WANT_DIAG) {
-        cx<T> d = (op.omega2 * op.kappa[p]) * (ax * ay * az);
+        const cx<T> mw = ax * ay * az;
+        const T kp = op.kappa[p];
+        cx<T> d = (op.omega2 * kp) * mw;
+        if (op.omega2_im != T(0)) d = d + mk<T>(0, op.omega2_im * kp) * mw;
         // Same subtraction order as the reference: per axis, the +edge then the -edge.
         d = d - c[1];
         d = d - c[0];
@@ -116,14 +119,19 @@ __device__ __forceinline__ void edge_coefs(const OpView<T> &op, int i, int j, in
 }
 
 // Final row coefficients for a mode: y_P = cs * x_P + sum_n c[n] * x_{P+off[n]}.
+//   A, AH      : rows of A / A^H
+//   RA, RAH    : the same rows times the row scale of A / A^H
+//   RAM, RAMH  : R A M with the Jacobi inverse M of R A (likewise for A^H)
 template <typename T, int MODE>
 __device__ __forceinline__ void row_coefs(const OpView<T> &op, int i, int j, int k, int64_t p,
                                           const Nbr &nb, cx<T> &cs, cx<T> c[6]) {
-    constexpr bool DIAG = (MODE == MODE_A || MODE == MODE_AH);
+    constexpr bool ADJ = (MODE == MODE_AH || MODE == MODE_RAMH || MODE == MODE_RAH);
+    constexpr bool PRE = (MODE == MODE_RAM || MODE == MODE_RAMH);
+    constexpr bool SCALED = (MODE == MODE_RA || MODE == MODE_RAH);
     cx<T> d = mk<T>(0, 0);
-    edge_coefs<T, DIAG>(op, i, j, k, p, nb, c, &d);
+    edge_coefs<T, !PRE>(op, i, j, k, p, nb, c, &d);
     const bool fixed_p = node_fixed(op, i, j, k);
-    if (MODE == MODE_A || MODE == MODE_RAM) {
+    if (!ADJ) {
         if (fixed_p) {
 #pragma unroll
             for (int n = 0; n < 6; ++n) c[n] = mk<T>(0, 0);
@@ -141,8 +149,14 @@ __device__ __forceinline__ void row_coefs(const OpView<T> &op, int i, int j, int
         }
         d = conj(d);
     }
-    if (MODE == MODE_A || MODE == MODE_AH) {
+    if (!PRE) {
         cs = fixed_p ? mk<T>(1, 0) : d;
+        if (SCALED) {
+            const T r = (MODE == MODE_RA ? op.rs_f : op.rs_a)[p];
+            cs = r * cs;
+#pragma unroll
+            for (int n = 0; n < 6; ++n) c[n] = r * c[n];
+        }
     } else {
         // R A M: the diagonal is r_P D_P / (r_P D_P) = 1 exactly.
         const T *rs = (MODE == MODE_RAM) ? op.rs_f : op.rs_a;

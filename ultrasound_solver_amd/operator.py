@@ -135,7 +135,7 @@ class HelmholtzOperator:
 
     def gmres(self, b, restart=30, maxiter=3000, rtol=1.0e-6, backward_error_tol=1.0e-4,
               precision="c128", adjoint=False, chunk=1, max_refinements=10, inner_rtol=1.0e-5,
-              raise_on_failure=False):
+              preconditioner="jacobi", raise_on_failure=False):
         """Solve A x = b (or A^H x = b) for each column; returns (x, stats)."""
         b = self.as_columns(b, torch.complex128)
         k = b.shape[0]
@@ -144,7 +144,9 @@ class HelmholtzOperator:
                                backward_error_tol=float(backward_error_tol),
                                precision=PRECISIONS[precision], adjoint=int(bool(adjoint)),
                                chunk=int(chunk), max_refinements=int(max_refinements),
-                               inner_rtol=float(inner_rtol), use_graph=0)
+                               inner_rtol=float(inner_rtol), use_graph=0,
+                               preconditioner={"jacobi": _capi.PRE_JACOBI,
+                                               "multigrid": _capi.PRE_MULTIGRID}[preconditioner])
         stats = (_capi.GmresStats*k)()
         status = _capi.lib().hz_gmres(self.handle, ctypes.byref(opts), ctypes.c_void_p(b.data_ptr()),
                                       ctypes.c_void_p(x.data_ptr()), k, self.n, stats, _stream())
@@ -157,6 +159,12 @@ class HelmholtzOperator:
                                f"scaled residual={bad['scaled_residual']:.3e}, "
                                f"backward error={bad['backward_error']:.3e}")
         return x, out
+
+    def set_multigrid(self, shift=0.5, pre=1, post=1, coarse_sweeps=30, weight=0.7, levels=None):
+        """Install the shifted-Laplacian multigrid preconditioner (mixed form)."""
+        from .multigrid import Hierarchy
+        self.hierarchy = Hierarchy(self, levels=levels).install(self, shift, pre, post, coarse_sweeps, weight)
+        return self.hierarchy
 
     # -- sources ----------------------------------------------------------
     def set_sources(self, node, column, value, num_columns):
