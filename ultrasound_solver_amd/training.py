@@ -1,0 +1,501 @@
+"""Phase response basis, losses and the phase objective on the device.
+
+* ``PhaseResponseBasis`` -- src/algorithm/training/phase_response_basis.py:16-416:
+  G = A^{-1} B diag(a) built in batches (here by the batched device GMRES,
+  which the reference's gpu_iterative backend could not do), stored as the
+  same resumable ``.npy`` memmap (N, M) complex128 + ``.json`` metadata
+  (format 3), then ``field`` (G e^{i phi}), ``phase_vjp``
+  (Re(conj(G^H c) i e^{i phi})) and trilinear ``target_response``.
+* Losses -- src/algorithm/training/phase_adjoint_optimizer.py:78-414:
+  ``AmplitudeTarget``/Gaussian targets, ``AmplitudeFieldLoss``
+  (field_match), ``FocalPressureLoss``, ``FocalContrastLoss`` and the
+  background mask, each with a host (numpy) and a device (torch) evaluation
+  returning (loss, cotangent) with dL = Re(c^H du).
+"""
+import hashlib
+import json
+from dataclasses import asdict
+from pathlib import Path
+
+import numpy as np
+import torch
+
+EPS = 1.0e-12
+
+
+def trilinear_rows(points, shape, origin, dx, box=None):
+    """(rows (P, 8) Fortran node indices, weights (P, 8)) of the trilinear stencil;
+    the lower corner is clipped to n-2 (phase_response_basis.py:165-174)."""
+    points = np.asarray(points, dtype=float)
+    shape = np.asarray(shape)
+    origin = np.asarray(origin, dtype=float)
+    if points.ndim != 2 or points.shape[1] != 3 or not np.isfinite(points).all():
+        raise ValueError("Targets must be finite (N,3) coordinates inside the grid")
+    upper = origin + (shape - 1)*dx if box is None else origin + np.asarray(box, float)
+    if np.any(points < origin) or np.any(points > upper + 1e-12):
+        raise ValueError("Targets must be finite (N,3) coordinates inside the grid")
+    pos = (points - origin)/dx
+    low = np.minimum(np.floor(pos).astype(np.int64), shape - 2)
+    frac = pos - low
+    rows = np.empty((len(points), 8), dtype=np.int64)
+    wts = np.empty((len(points), 8))
+    for c, corner in enumerate(np.ndindex(2, 2, 2)):
+        corner = np.asarray(corner)
+        idx = low + corner
+        wts[:, c] = np.prod(np.where(corner, frac, 1.0 - frac), axis=1)
+        rows[:, c] = idx[:, 0] + shape[0]*(idx[:, 1] + shape[1]*idx[:, 2])
+    return rows, wts
+
+
+class PhaseResponseBasis:
+    FORMAT_VERSION = 3
+
+    def __init__(self, solver, storage_path, rhs_batch_size=8, voxel_chunk_size=262144,
+                 release_factor_after_build=True):
+        self.solver = solver
+        self.storage_path = Path(storage_path)
+        self.metadata_path = Path(f"{self.storage_path}.json")
+        self.rhs_batch_size = int(rhs_batch_size)
+        self.voxel_chunk_size = int(voxel_chunk_size)
+        self.release_factor_after_build = bool(release_factor_after_build)
+        self.dtype = np.dtype(np.complex128)
+        self.basis = None
+        self.basis_gpu = None
+        self.completed_columns = 0
+        self._built_fingerprint = None
+        if self.rhs_batch_size < 1:
+            raise ValueError("rhs_batch_size must be at least 1")
+        if self.voxel_chunk_size < 1:
+            raise ValueError("voxel_chunk_size must be at least 1")
+
+    @property
+    def num_transducers(self):
+        return int(self.solver.transducers.num_transducers)
+
+    @property
+    def shape(self):
+        return self.solver.total_dofs, self.num_transducers
+
+    @property
+    def is_complete(self):
+        return self.completed_columns == self.num_transducers
+
+    def build(self):
+        backend = str(self.solver.cfg.solver.get("backend", "gpu_iterative")).lower()
+        if backend != "gpu_iterative":
+            raise ValueError("this backend builds the phase basis with solver.backend='gpu_iterative'")
+        self._open_or_create_storage()
+        if self.is_complete:
+            return self
+        self.solver.assemble_matrix()
+        counts = np.asarray(abs(self.solver.source_columns).sum(axis=0)).ravel()
+        missing = np.flatnonzero(counts == 0)
+        if missing.size:
+            raise ValueError(f"The grid does not cover transducers {missing.tolist()}")
+        op = self.solver.operator
+        amps = self.solver.transducers.amplitudes
+        it = self.solver.iterative_solver
+        for start in range(self.completed_columns, self.num_transducers, self.rhs_batch_size):
+            stop = min(start + self.rhs_batch_size, self.num_transducers)
+            w = torch.zeros((stop - start, self.num_transducers), dtype=torch.complex128)
+            w[torch.arange(stop - start), torch.arange(start, stop)] = torch.as_tensor(amps[start:stop]).to(torch.complex128)
+            b = op.build_rhs(w)
+            x = it.solve(op, b)
+            self.basis[:, start:stop] = x.cpu().numpy().T
+            self.basis.flush()
+            self.completed_columns = stop
+            self._write_metadata()
+        return self
+
+    def to_gpu(self, device="cuda"):
+        self._require_complete()
+        free, _ = torch.cuda.mem_get_info(torch.device(device))
+        need = int(self.basis.nbytes)
+        if need + 256*1024**2 > free:
+            raise MemoryError("Insufficient GPU memory for phase response basis: "
+                              f"required={need/1024**3:.2f} GiB, free={free/1024**3:.2f} GiB")
+        self.basis_gpu = torch.as_tensor(np.asarray(self.basis), device=device)
+        return self
+
+    def release_gpu(self):
+        self.basis_gpu = None
+
+    def target_response(self, points):
+        self._require_complete()
+        cfg = self.solver.cfg
+        rows, wts = trilinear_rows(points, cfg.domain.grid_size, cfg.domain.origin_m, cfg.domain.dx,
+                                   cfg.domain.box_size)
+        self._check_outside_pml(np.asarray(points, float))
+        out = np.zeros((len(rows), self.num_transducers), dtype=np.complex128)
+        for c in range(8):
+            out += wts[:, c, None]*self.basis[rows[:, c]]
+        return out
+
+    def _check_outside_pml(self, points):
+        from .physics import FACES, boundary_type
+        cfg = self.solver.cfg
+        pml = cfg.solver.get("pml", {}) or {}
+        cells = int(pml.get("cells", 0))
+        if not cells:
+            return
+        faces = pml.get("faces", [f for f in FACES
+                                  if boundary_type(cfg.boundary_conditions.get(f, "reflecting")) == "open"])
+        origin = np.asarray(cfg.domain.origin_m, float)
+        upper = np.asarray(cfg.domain.box_size, float)
+        for face in faces:
+            axis = "xyz".index(face[1])
+            local = points[:, axis] - origin[axis]
+            dist = local if face[0] == "-" else upper[axis] - local
+            if np.any(dist < cells*cfg.domain.dx):
+                raise ValueError("Target lies in PML; enlarge the physical interior")
+
+    def _weights(self, phases, device=None):
+        ph = torch.as_tensor(np.asarray(phases, dtype=np.float64), device=device)
+        return torch.exp(1j*ph)
+
+    def field(self, phases, return_numpy=True):
+        self._require_complete()
+        phases = np.asarray(phases) if not isinstance(phases, torch.Tensor) else phases
+        if tuple(phases.shape) != (self.num_transducers,):
+            raise ValueError(f"phases must have shape ({self.num_transducers},), got {tuple(phases.shape)}")
+        d = self.solver.cfg.domain
+        if self.basis_gpu is not None:
+            ph = torch.as_tensor(phases, dtype=torch.float64, device=self.basis_gpu.device)
+            u = self.basis_gpu @ torch.exp(1j*ph)
+            if return_numpy:
+                return u.cpu().numpy().reshape((d.nx, d.ny, d.nz), order="F")
+            # (nx, ny, nz) view of the Fortran-ordered vector
+            return u.reshape(d.nz, d.ny, d.nx).permute(2, 1, 0)
+        w = np.exp(1j*np.asarray(phases, dtype=np.float64))
+        out = np.empty(self.solver.total_dofs, dtype=np.complex128)
+        for s in range(0, self.solver.total_dofs, self.voxel_chunk_size):
+            out[s:s + self.voxel_chunk_size] = self.basis[s:s + self.voxel_chunk_size] @ w
+        return out.reshape((d.nx, d.ny, d.nz), order="F")
+
+    def phase_vjp(self, field_cotangent, phases, return_numpy=True):
+        """dL/dphi for dL = Re(c^H du) (phase_response_basis.py:208-253)."""
+        self._require_complete()
+        if self.basis_gpu is not None:
+            dev = self.basis_gpu.device
+            c = field_cotangent
+            if isinstance(c, torch.Tensor):
+                c = c.to(dev, torch.complex128)
+                c = c.permute(*reversed(range(c.dim()))).reshape(-1) if c.dim() == 3 else c.reshape(-1)
+            else:
+                c = torch.as_tensor(np.asarray(field_cotangent, np.complex128).reshape(-1, order="F"), device=dev)
+            if c.numel() != self.solver.total_dofs:
+                raise ValueError("field_cotangent has the wrong size")
+            resp = self.basis_gpu.conj().transpose(0, 1) @ c
+            w = torch.exp(1j*torch.as_tensor(np.asarray(phases, np.float64), device=dev))
+            g = torch.real(torch.conj(resp)*(1j*w))
+            return g.cpu().numpy() if return_numpy else g
+        c = np.asarray(field_cotangent, dtype=np.complex128).reshape(-1, order="F")
+        if c.size != self.solver.total_dofs:
+            raise ValueError("field_cotangent has the wrong size")
+        resp = np.zeros(self.num_transducers, dtype=np.complex128)
+        for s in range(0, self.solver.total_dofs, self.voxel_chunk_size):
+            resp += c[s:s + self.voxel_chunk_size].conj() @ self.basis[s:s + self.voxel_chunk_size]
+        resp = resp.conj()
+        return np.real(np.conj(resp)*(1j*np.exp(1j*np.asarray(phases, np.float64))))
+
+    def close(self):
+        self.release_gpu()
+        if isinstance(self.basis, np.memmap):
+            self.basis.flush()
+            mm = self.basis._mmap
+            self.basis = None
+            mm.close()
+
+    # -- storage (format 3, resumable) ---------------------------------------
+    def _open_or_create_storage(self):
+        have_basis, have_meta = self.storage_path.exists(), self.metadata_path.exists()
+        if have_basis != have_meta:
+            raise RuntimeError("Phase basis data and metadata must exist together")
+        if have_basis:
+            meta = json.loads(self.metadata_path.read_text(encoding="utf-8"))
+            self._validate_metadata(meta)
+            self.basis = np.load(self.storage_path, mmap_mode="r+")
+            if self.basis.shape != self.shape:
+                raise RuntimeError("Stored phase basis shape mismatch")
+            if self.basis.dtype != self.dtype:
+                raise RuntimeError("Stored phase basis dtype mismatch")
+            self.completed_columns = int(meta["completed_columns"])
+            self._built_fingerprint = meta["fingerprint"]
+            return
+        self.storage_path.parent.mkdir(parents=True, exist_ok=True)
+        self.basis = np.lib.format.open_memmap(self.storage_path, mode="w+", dtype=self.dtype, shape=self.shape,
+                                               fortran_order=False)
+        self.completed_columns = 0
+        self._write_metadata()
+
+    def _validate_metadata(self, meta):
+        if meta.get("format_version") != self.FORMAT_VERSION:
+            raise RuntimeError("Unsupported phase basis format version")
+        if meta.get("fingerprint") != self._fingerprint():
+            raise RuntimeError("Phase basis does not match the current physical scene")
+        if meta.get("shape") != list(self.shape):
+            raise RuntimeError(f"Phase basis shape mismatch: {meta.get('shape')} != {list(self.shape)}")
+        if meta.get("dtype") != self.dtype.name:
+            raise RuntimeError("Phase basis dtype mismatch")
+        done = int(meta.get("completed_columns", -1))
+        if done < 0 or done > self.num_transducers:
+            raise RuntimeError("Invalid completed column count")
+
+    def _write_metadata(self):
+        fp = self._fingerprint()
+        meta = {"format_version": self.FORMAT_VERSION, "fingerprint": fp, "shape": list(self.shape),
+                "dtype": self.dtype.name, "completed_columns": self.completed_columns}
+        tmp = Path(f"{self.metadata_path}.tmp")
+        tmp.write_text(json.dumps(meta, indent=2, sort_keys=True), encoding="utf-8")
+        tmp.replace(self.metadata_path)
+        self._built_fingerprint = fp
+
+    def _fingerprint(self):
+        cfg = self.solver.cfg
+        payload = {"format_version": self.FORMAT_VERSION, "propagation_model": self.solver.MODEL_VERSION,
+                   "centers_m": self.solver.transducers.centers.tolist(), "physics": asdict(cfg.physics),
+                   "transducer_specs": asdict(cfg.specs), "domain": asdict(cfg.domain),
+                   "boundary_conditions": cfg.boundary_conditions, "obstacles": cfg.obstacles,
+                   "mesh_content_sha256": {}, "amplitudes": self.solver.transducers.amplitudes.tolist(),
+                   "pde_scene": self.solver.scene_fingerprint()}
+        return hashlib.sha256(json.dumps(payload, sort_keys=True, separators=(",", ":")).encode()).hexdigest()
+
+    def _require_complete(self):
+        if self.basis is None or not self.is_complete:
+            raise RuntimeError("Phase response basis is not complete")
+        if self._built_fingerprint is not None and self._fingerprint() != self._built_fingerprint:
+            raise RuntimeError("Physical scene changed; rebuild the phase response basis")
+
+
+# ------------------------------------------------------------------------ losses
+def nearest_grid_index(cfg, point):
+    return tuple(np.rint((np.asarray(point, float) - np.asarray(cfg.domain.origin_m))/cfg.domain.dx).astype(int))
+
+
+def build_gaussian_target_fields(cfg, target_points, peak_pressure, sigma, background_pressure=0.0,
+                                 target_weight=1.0, background_weight=0.05):
+    pts = np.asarray(target_points, dtype=np.float64)
+    if pts.ndim != 2 or pts.shape[1] != 3:
+        raise ValueError("target_points must have shape (num_targets, 3)")
+    if len(pts) == 0:
+        raise ValueError("at least one target point is required")
+    if sigma <= 0:
+        raise ValueError("target sigma must be positive")
+    if peak_pressure < background_pressure:
+        raise ValueError("peak_pressure must be >= background_pressure")
+    d = cfg.domain
+    shape = (d.nx, d.ny, d.nz)
+    o = np.asarray(d.origin_m, float)
+    axes = [np.linspace(o[a], o[a] + d.box_size[a], shape[a]) for a in range(3)]
+    lower, upper = o, o + np.asarray(d.box_size, float)
+    target = np.full(shape, float(background_pressure))
+    weight = np.full(shape, float(background_weight))
+    for p in pts:
+        if np.any(p < lower) or np.any(p > upper):
+            raise ValueError(f"target point is outside the domain: {p.tolist()} not in "
+                             f"[{lower.tolist()}, {upper.tolist()}]")
+        r2 = ((axes[0][:, None, None] - p[0])**2 + (axes[1][None, :, None] - p[1])**2
+              + (axes[2][None, None, :] - p[2])**2)
+        g = np.exp(-0.5*r2/(sigma*sigma))
+        g = g/g[nearest_grid_index(cfg, p)]
+        target = np.maximum(target, background_pressure + (peak_pressure - background_pressure)*g)
+        weight = np.maximum(weight, background_weight + (target_weight - background_weight)*g)
+    return target, weight
+
+
+def build_background_mask(cfg, target_points, target_radius, source_exclusion_layers):
+    if target_radius <= 0:
+        raise ValueError("training.target_radius must be positive")
+    d = cfg.domain
+    shape = (d.nx, d.ny, d.nz)
+    axes = [np.linspace(0.0, d.box_size[a], shape[a]) for a in range(3)]
+    mask = np.ones(shape, dtype=bool)
+    layers = max(int(source_exclusion_layers), 0)
+    if layers:
+        mask[:, :, :layers] = False
+    for p in np.asarray(target_points, float).reshape(-1, 3):
+        r2 = ((axes[0][:, None, None] - p[0])**2 + (axes[1][None, :, None] - p[1])**2
+              + (axes[2][None, None, :] - p[2])**2)
+        mask[r2 <= target_radius*target_radius] = False
+    return mask
+
+
+class AmplitudeTarget:
+    def __init__(self, target, weight):
+        if target.shape != weight.shape:
+            raise ValueError("target and weight must have the same shape")
+        self.target = np.asarray(target, dtype=np.float64)
+        self.weight = np.asarray(weight, dtype=np.float64)
+
+    @classmethod
+    def from_config(cls, cfg):
+        tr = cfg.training
+        if tr.target_field_file:
+            path = Path(tr.target_field_file)
+            if path.suffix.lower() == ".npy":
+                target = np.load(path, allow_pickle=False)
+                weight = np.ones_like(target, dtype=np.float64)
+            elif path.suffix.lower() == ".npz":
+                with np.load(path, allow_pickle=False) as data:
+                    target = np.asarray(data["target"], np.float64)
+                    weight = np.asarray(data["weight"], np.float64) if "weight" in data else np.ones_like(target)
+            else:
+                raise ValueError("target_field_file must be .npy or .npz")
+            if target.shape != tuple(cfg.domain.grid_size):
+                raise ValueError(f"target field shape must be {tuple(cfg.domain.grid_size)}, got {target.shape}")
+            return cls(target, weight)
+        if not cfg.targets:
+            raise ValueError("phase_optimization requires targets or training.target_field_file")
+        return cls(*build_gaussian_target_fields(cfg, np.asarray(cfg.targets, float), tr.target_peak_pressure,
+                                                 tr.target_sigma, tr.background_pressure, tr.target_weight,
+                                                 tr.background_weight))
+
+
+def _as_device(a, like):
+    return torch.as_tensor(a, device=like.device)
+
+
+def _focal_term(field, indices, weight, with_cotangent, focal_field):
+    """Device/host focal part: -w mean |u_t|; exact external samples carry their
+    own VJP, so the grid cotangent is then zero (phase_adjoint_optimizer.py:210-223)."""
+    dev = isinstance(field, torch.Tensor)
+    idx = tuple(np.asarray(indices, dtype=np.intp).T)
+    if focal_field is None:
+        values = field[idx]
+    else:
+        values = focal_field
+    amp = values.abs() if dev else np.abs(values)
+    loss = -weight*(amp.mean() if dev else np.mean(amp))
+    cot = None
+    if with_cotangent:
+        cot = torch.zeros_like(field, dtype=torch.complex128) if dev else np.zeros(field.shape, np.complex128)
+        if focal_field is None:
+            contrib = -weight/len(indices)*values/(amp + EPS)
+            if dev:
+                where = tuple(torch.as_tensor(a, device=field.device) for a in idx)
+                cot.index_put_(where, contrib.to(torch.complex128), accumulate=True)
+            else:
+                np.add.at(cot, idx, contrib)
+    return loss, cot
+
+
+class AmplitudeFieldLoss:
+    def __init__(self, amplitude_target, eps=EPS):
+        self.target = amplitude_target.target
+        self.weight = amplitude_target.weight
+        self.eps = float(eps)
+        self._dev = {}
+
+    def _d(self, name, like):
+        if name not in self._dev:
+            self._dev[name] = _as_device(getattr(self, name), like)
+        return self._dev[name]
+
+    def evaluate(self, field, with_cotangent=True):
+        amp = np.abs(field)
+        res = amp - self.target
+        wres = self.weight*res
+        loss = 0.5*float(np.vdot(wres, wres).real)
+        if not with_cotangent:
+            return loss, None
+        return loss, (self.weight*self.weight*res*field/(amp + self.eps)).astype(np.complex128, copy=False)
+
+    def evaluate_device(self, field, with_cotangent=True, focal_field=None):
+        target, weight = self._d("target", field), self._d("weight", field)
+        amp = field.abs()
+        res = amp - target
+        wres = weight*res
+        loss = 0.5*torch.sum(wres*wres)
+        if not with_cotangent:
+            return loss, None
+        return loss, weight*weight*res*field/(amp + self.eps)
+
+
+class FocalPressureLoss:
+    def __init__(self, cfg, amplitude_target, eps=EPS):
+        self.cfg = cfg
+        self.target = amplitude_target.target
+        self.weight = amplitude_target.weight
+        self.eps = float(eps)
+        self.target_indices = [nearest_grid_index(cfg, p) for p in cfg.targets]
+        self.target_weight = float(cfg.training.target_weight)
+        self.background_weight = float(cfg.training.background_weight)
+
+    def evaluate(self, field, with_cotangent=True, focal_field=None):
+        amp = np.abs(field)
+        loss, cot = _focal_term(field, self.target_indices, self.target_weight, with_cotangent, focal_field)
+        if self.background_weight > 0:
+            loss = loss + 0.5*self.background_weight*float(np.mean(amp*amp))
+            if with_cotangent:
+                cot = cot + self.background_weight/field.size*field
+        return float(loss), cot
+
+    def evaluate_device(self, field, with_cotangent=True, focal_field=None):
+        amp = field.abs()
+        loss, cot = _focal_term(field, self.target_indices, self.target_weight, with_cotangent, focal_field)
+        if self.background_weight > 0:
+            loss = loss + 0.5*self.background_weight*torch.mean(amp*amp)
+            if with_cotangent:
+                cot = cot + self.background_weight/field.numel()*field
+        return loss, cot
+
+
+class FocalContrastLoss:
+    def __init__(self, cfg, amplitude_target, eps=EPS):
+        tr = cfg.training
+        self.cfg = cfg
+        self.target = amplitude_target.target
+        self.weight = amplitude_target.weight
+        self.eps = float(eps)
+        self.target_indices = [nearest_grid_index(cfg, p) for p in cfg.targets]
+        self.target_weight = float(tr.target_weight)
+        self.sidelobe_weight = float(tr.background_weight)
+        self.temperature = float(tr.sidelobe_temperature)
+        if self.temperature <= 0:
+            raise ValueError("training.sidelobe_temperature must be positive")
+        self.background_mask = build_background_mask(cfg, np.asarray(cfg.targets, float), float(tr.target_radius),
+                                                     int(tr.source_exclusion_layers))
+        if not np.any(self.background_mask):
+            raise ValueError("focal_contrast background mask is empty")
+        self._mask_dev = None
+
+    def evaluate(self, field, with_cotangent=True, focal_field=None):
+        amp = np.abs(field)
+        loss, cot = _focal_term(field, self.target_indices, self.target_weight, with_cotangent, focal_field)
+        bg = amp[self.background_mask]
+        top = float(np.max(bg))
+        ex = np.exp((bg - top)/self.temperature)
+        prob = ex/np.sum(ex)
+        loss = float(loss) + self.sidelobe_weight*float(top + self.temperature*(np.log(np.sum(ex)) - np.log(bg.size)))
+        if not with_cotangent:
+            return loss, None
+        cot[self.background_mask] += self.sidelobe_weight*prob*field[self.background_mask]/(bg + self.eps)
+        return loss, cot
+
+    def evaluate_device(self, field, with_cotangent=True, focal_field=None):
+        if self._mask_dev is None:
+            self._mask_dev = torch.as_tensor(self.background_mask, device=field.device)
+        mask = self._mask_dev
+        amp = field.abs()
+        loss, cot = _focal_term(field, self.target_indices, self.target_weight, with_cotangent, focal_field)
+        bg = amp[mask]
+        top = bg.max()
+        ex = torch.exp((bg - top)/self.temperature)
+        prob = ex/ex.sum()
+        loss = loss + self.sidelobe_weight*(top + self.temperature*(torch.log(ex.sum()) - np.log(bg.numel())))
+        if not with_cotangent:
+            return loss, None
+        cot[mask] += self.sidelobe_weight*prob*field[mask]/(bg + self.eps)
+        return loss, cot
+
+
+def make_loss(cfg, target=None):
+    target = target or AmplitudeTarget.from_config(cfg)
+    kind = cfg.training.loss_type
+    if kind == "field_match":
+        return AmplitudeFieldLoss(target)
+    if kind == "focal_pressure":
+        return FocalPressureLoss(cfg, target)
+    if kind == "focal_contrast":
+        return FocalContrastLoss(cfg, target)
+    raise ValueError("Unknown training.loss_type")
