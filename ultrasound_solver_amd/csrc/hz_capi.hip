@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -22,6 +23,15 @@ using namespace hz;
 namespace {
 
 thread_local std::string g_error;
+
+// HZ_VERBOSE=1 prints the residual at every restart-cycle top to stderr.
+bool verbose() {
+    static const bool on = [] {
+        const char *v = std::getenv("HZ_VERBOSE");
+        return v && v[0] && v[0] != '0';
+    }();
+    return on;
+}
 
 struct HipFailure : std::runtime_error {
     hipError_t code;
@@ -314,6 +324,9 @@ void gmres_cycles(hz_operator *op, int mode_ram, int k, int restart, int maxiter
         bool any = false;
         for (int r = 0; r < k; ++r) {
             if (!cs.active[r]) continue;
+            if (verbose())
+                std::fprintf(stderr, "[hz] %s cycle %d rhs %d: residual %.6e (target %.6e)\n",
+                             sizeof(T) == 8 ? "c128" : "c64", cs.cycles[r], r, resid[r], atol[r]);
             cs.last_resid[r] = resid[r];
             if (!(resid[r] > atol[r]) || cs.iterations[r] >= maxiter || !std::isfinite(resid[r])) {
                 cs.active[r] = 0;
@@ -420,6 +433,9 @@ std::vector<CheckResult> check(hz_operator *op, int adjoint, const cx<double> *x
 }
 
 void fill_stats(hz_gmres_stats &st, const CheckResult &c, const hz_gmres_opts &o) {
+    if (verbose())
+        std::fprintf(stderr, "[hz] check: |Re|^2 %.6e |Rb|^2 %.6e |e|^2 %.6e |b|^2 %.6e bwd %.6e nonfinite %g\n", c.se,
+                     c.sb, c.ue, c.ub, c.bwd, c.bad);
     st.scaled_residual = c.sb > 0 ? std::sqrt(c.se / c.sb) : 0.0;
     st.relative_residual = c.ub > 0 ? std::sqrt(c.ue / c.ub) : 0.0;
     st.backward_error = c.bwd;

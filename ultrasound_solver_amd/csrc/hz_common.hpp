@@ -83,6 +83,19 @@ __device__ __forceinline__ T wave_max(T v) {
     return v;
 }
 
+// Combine the 4 wavefront totals of a 256-thread workgroup in FP64 when every
+// lane already holds its wavefront's total; result valid in thread 0.
+__device__ __forceinline__ double block_combine_256(double wave_total, double *scratch) {
+    const int lane = threadIdx.x % WAVE;
+    const int wid = threadIdx.x / WAVE;
+    if (lane == 0) scratch[wid] = wave_total;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) r = (scratch[0] + scratch[1]) + (scratch[2] + scratch[3]);
+    __syncthreads();
+    return r;
+}
+
 // Sum over a 256-thread workgroup in FP64; result valid in thread 0.
 // `scratch` must hold 4 doubles in LDS.
 __device__ __forceinline__ double block_sum_256(double v, double *scratch) {

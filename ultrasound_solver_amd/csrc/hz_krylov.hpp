@@ -50,8 +50,9 @@ __global__ __launch_bounds__(256) void k_dots(const cx<T> *__restrict__ V, int k
 #pragma unroll
     for (int q = 0; q < DOT_GROUP; ++q) {
         if (q < cnt) {
-            const double sre = block_sum_256((double)wave_sum(are[q]), red);
-            const double sim = block_sum_256((double)wave_sum(aim[q]), red);
+            // FP32 (or FP64) wavefront sums, combined across wavefronts in FP64.
+            const double sre = block_combine_256((double)wave_sum(are[q]), red);
+            const double sim = block_combine_256((double)wave_sum(aim[q]), red);
             if (threadIdx.x == 0)
                 partial[((int64_t)r * maxv + g0 + q) * gridDim.x + blockIdx.x] = make_double2(sre, sim);
         }
@@ -121,7 +122,7 @@ __global__ __launch_bounds__(256) void k_update(const cx<T> *__restrict__ V, int
         nrm += norm2(acc);
     }
     if (norm_partial) {
-        const double s = block_sum_256((double)wave_sum(nrm), red);
+        const double s = block_combine_256((double)wave_sum(nrm), red);
         if (threadIdx.x == 0) norm_partial[(int64_t)r * gridDim.x + blockIdx.x] = s;
     }
 }
