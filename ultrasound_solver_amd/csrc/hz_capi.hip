@@ -229,11 +229,12 @@ void vcycle(Multigrid &mg, int l, const cx<float> *b, cx<float> *x, int k, const
                            b, alt, (const cx<float> *)nullptr, 0.0f, k, live);
         HZ_LAUNCHED();
         hipLaunchKernelGGL((k_restrict<float>), dim3((C.n + 255) / 256), dim3(256), 0, s, alt, L.nx, L.ny, L.nz,
-                           C.bb.as<cx<float>>(), C.nx, C.ny, C.nz, k, live);
+                           C.bb.as<cx<float>>(), C.nx, C.ny, C.nz, k, live, L.v.soft);
         HZ_LAUNCHED();
         vcycle<MODE>(mg, l + 1, C.bb.as<cx<float>>(), C.xx.as<cx<float>>(), k, live, s);
         hipLaunchKernelGGL((k_prolong_add<float>), dim3((n + 255) / 256), dim3(256), 0, s,
-                           (const cx<float> *)C.xx.as<cx<float>>(), C.nx, C.ny, C.nz, cur, L.nx, L.ny, L.nz, k, live);
+                           (const cx<float> *)C.xx.as<cx<float>>(), C.nx, C.ny, C.nz, cur, L.nx, L.ny, L.nz, k, live,
+                           L.v.soft);
         HZ_LAUNCHED();
         for (int it = 0; it < mg.post; ++it) {
             level_smooth<MODE>(L, cur, b, alt, mg.weight, k, live, s);

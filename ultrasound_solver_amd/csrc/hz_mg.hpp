@@ -18,6 +18,12 @@
 
 namespace hz {
 
+// Node on a pressure-release face (identity row) for face bitmask `soft`.
+__device__ __forceinline__ bool on_soft_face(int i, int j, int k, int nx, int ny, int nz, int soft) {
+    return ((soft & 1) && i == 0) || ((soft & 2) && i == nx - 1) || ((soft & 4) && j == 0) ||
+           ((soft & 8) && j == ny - 1) || ((soft & 16) && k == 0) || ((soft & 32) && k == nz - 1);
+}
+
 // Fine index f is a coarse node iff it is even or the last node.
 __device__ __forceinline__ bool coarse_node(int f, int n) { return (f & 1) == 0 || f == n - 1; }
 // Coarse index of a coarse-node fine index f (m = coarse size).
@@ -94,11 +100,14 @@ __global__ __launch_bounds__(256) void k_level_jacobi0(const cx<T> *__restrict__
     }
 }
 
-// Coarse right-hand side bc = P^T rf.  One thread per coarse node.
+// Coarse right-hand side bc = P^T rf.  One thread per coarse node.  Identity
+// rows (pressure-release faces) take no part in the coarse correction: their
+// fine residuals are not restricted and coarse identity rows get zero, so a
+// residual-scaled value never lands in a solution-scaled identity row.
 template <typename T>
 __global__ __launch_bounds__(256) void k_restrict(const cx<T> *__restrict__ rf, int nxf, int nyf, int nzf,
                                                   cx<T> *__restrict__ bc, int nxc, int nyc, int nzc, int k,
-                                                  const int *__restrict__ live) {
+                                                  const int *__restrict__ live, int soft) {
     const int64_t nc = (int64_t)nxc * nyc * nzc;
     const int64_t nf = (int64_t)nxf * nyf * nzf;
     const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -107,6 +116,11 @@ __global__ __launch_bounds__(256) void k_restrict(const cx<T> *__restrict__ rf, 
     const int jc = (int)((q / nxc) % nyc);
     const int kc = (int)(q / ((int64_t)nxc * nyc));
     const int fi = fine_of(ic, nxf), fj = fine_of(jc, nyf), fk = fine_of(kc, nzf);
+    if (soft && on_soft_face(ic, jc, kc, nxc, nyc, nzc, soft)) {
+        for (int r = 0; r < k; ++r)
+            if (!live || live[r]) bc[r * nc + q] = mk<T>(0, 0);
+        return;
+    }
     T wx[3], wy[3], wz[3];
     wx[1] = wy[1] = wz[1] = 1;
     wx[0] = (fi - 1 >= 0 && !coarse_node(fi - 1, nxf)) ? T(0.5) : T(0);
@@ -128,6 +142,7 @@ __global__ __launch_bounds__(256) void k_restrict(const cx<T> *__restrict__ rf, 
                 const T wyz = wz[dz] * wy[dy];
                 for (int dx = 0; dx < 3; ++dx) {
                     if (wx[dx] == T(0)) continue;
+                    if (soft && on_soft_face(fi + dx - 1, fj + dy - 1, fk + dz - 1, nxf, nyf, nzf, soft)) continue;
                     acc = acc + (wyz * wx[dx]) * src[yoff + fi + dx - 1];
                 }
             }
@@ -140,7 +155,7 @@ __global__ __launch_bounds__(256) void k_restrict(const cx<T> *__restrict__ rf, 
 template <typename T>
 __global__ __launch_bounds__(256) void k_prolong_add(const cx<T> *__restrict__ xc, int nxc, int nyc, int nzc,
                                                      cx<T> *__restrict__ xf, int nxf, int nyf, int nzf, int k,
-                                                     const int *__restrict__ live) {
+                                                     const int *__restrict__ live, int soft) {
     const int64_t nf = (int64_t)nxf * nyf * nzf;
     const int64_t nc = (int64_t)nxc * nyc * nzc;
     const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -148,6 +163,7 @@ __global__ __launch_bounds__(256) void k_prolong_add(const cx<T> *__restrict__ x
     const int i = (int)(p % nxf);
     const int j = (int)((p / nxf) % nyf);
     const int kk = (int)(p / ((int64_t)nxf * nyf));
+    if (soft && on_soft_face(i, j, kk, nxf, nyf, nzf, soft)) return;  // identity rows get no correction
     int ci[2], cj[2], ck[2];
     T wi[2], wj[2], wk[2];
     int ni, nj, nk;
