@@ -201,32 +201,34 @@ void mg_reserve(Multigrid &mg, int k) {
 
 template <int MODE>
 void level_smooth(MgLevel &L, const cx<float> *x, const cx<float> *b, cx<float> *out, float w, int k,
-                  const int *live, hipStream_t s) {
+                  const int *live, const float *bdiv, hipStream_t s) {
     hipLaunchKernelGGL((k_level_op<float, MODE, EPI_SMOOTH>), dim3(L.tm.nblocks), dim3(256), 0, s, L.v, L.tm, x, b,
-                       out, (const cx<float> *)L.dinv.as<cx<float>>(), w, k, live);
+                       out, (const cx<float> *)L.dinv.as<cx<float>>(), w, k, live, bdiv);
     HZ_LAUNCHED();
 }
 
-// x = V-cycle(b) on level l; x and b hold k slots of length n_l.  The result
-// is left in `x` (which must not alias b).
+// x = V-cycle(b / bdiv) on level l; x and b hold k slots of length n_l.  The
+// result is left in `x` (which must not alias b).  bdiv (finest level only)
+// is the Krylov system's row scale.
 template <int MODE>
-void vcycle(Multigrid &mg, int l, const cx<float> *b, cx<float> *x, int k, const int *live, hipStream_t s) {
+void vcycle(Multigrid &mg, int l, const cx<float> *b, cx<float> *x, int k, const int *live, hipStream_t s,
+            const float *bdiv = nullptr) {
     MgLevel &L = *mg.levels[l];
     const bool coarsest = (l + 1 == (int)mg.levels.size());
     const int64_t n = L.n;
     cx<float> *cur = x, *alt = L.tt.as<cx<float>>();
     hipLaunchKernelGGL((k_level_jacobi0<float, MODE>), dim3(blocks_for(n, 256 * 8, 2048), k), dim3(256), 0, s, b,
-                       cur, (const cx<float> *)L.dinv.as<cx<float>>(), mg.weight, n, live);
+                       cur, (const cx<float> *)L.dinv.as<cx<float>>(), mg.weight, n, live, bdiv);
     HZ_LAUNCHED();
     const int first = coarsest ? mg.coarse_sweeps : mg.pre;
     for (int it = 1; it < first; ++it) {
-        level_smooth<MODE>(L, cur, b, alt, mg.weight, k, live, s);
+        level_smooth<MODE>(L, cur, b, alt, mg.weight, k, live, bdiv, s);
         std::swap(cur, alt);
     }
     if (!coarsest) {
         MgLevel &C = *mg.levels[l + 1];
         hipLaunchKernelGGL((k_level_op<float, MODE, EPI_RESID>), dim3(L.tm.nblocks), dim3(256), 0, s, L.v, L.tm, cur,
-                           b, alt, (const cx<float> *)nullptr, 0.0f, k, live);
+                           b, alt, (const cx<float> *)nullptr, 0.0f, k, live, bdiv);
         HZ_LAUNCHED();
         hipLaunchKernelGGL((k_restrict<float>), dim3((C.n + 255) / 256), dim3(256), 0, s, alt, L.nx, L.ny, L.nz,
                            C.bb.as<cx<float>>(), C.nx, C.ny, C.nz, k, live, L.v.soft);
@@ -237,19 +239,20 @@ void vcycle(Multigrid &mg, int l, const cx<float> *b, cx<float> *x, int k, const
                            L.v.soft);
         HZ_LAUNCHED();
         for (int it = 0; it < mg.post; ++it) {
-            level_smooth<MODE>(L, cur, b, alt, mg.weight, k, live, s);
+            level_smooth<MODE>(L, cur, b, alt, mg.weight, k, live, bdiv, s);
             std::swap(cur, alt);
         }
     }
     if (cur != x) HZ_HIP(hipMemcpyAsync(x, cur, (size_t)k * n * sizeof(cx<float>), hipMemcpyDeviceToDevice, s));
 }
 
+// z = P R^-1 v: one V-cycle on the unscaled right-hand side.
 void precondition(hz_operator *op, bool adjoint, const cx<float> *b, cx<float> *x, int k, const int *live,
                   hipStream_t s) {
     if (adjoint)
-        vcycle<MODE_AH>(*op->mg, 0, b, x, k, live, s);
+        vcycle<MODE_AH>(*op->mg, 0, b, x, k, live, s, op->v32.rs_a);
     else
-        vcycle<MODE_A>(*op->mg, 0, b, x, k, live, s);
+        vcycle<MODE_A>(*op->mg, 0, b, x, k, live, s, op->v32.rs_f);
 }
 
 // Per-RHS bookkeeping of one chunk.

@@ -50,12 +50,16 @@ enum LevelEpi { EPI_SMOOTH = 0, EPI_RESID = 1 };
 
 // EPI_SMOOTH: xo = xi + w * dinv * (b - A xi)      (damped Jacobi sweep)
 // EPI_RESID : xo = b - A xi
-// MODE is MODE_A (forward) or MODE_AH (adjoint; dinv conjugated).
+// MODE is MODE_A (forward) or MODE_AH (adjoint; dinv conjugated).  When
+// `bdiv` is set the right-hand side is read as b / bdiv: on the finest level
+// this undoes the row scale of the Krylov system, so that R A (P R^-1) is a
+// similarity transform of A P and its spectrum does not inherit the spread
+// of R (identity rows have scale 1, the others ~1/|A_PP|).
 template <typename T, int MODE, int EPI>
 __global__ __launch_bounds__(256) void k_level_op(OpView<T> op, TileMap tm, const cx<T> *__restrict__ xi,
                                                   const cx<T> *__restrict__ b, cx<T> *__restrict__ xo,
                                                   const cx<T> *__restrict__ dinv, T w, int k,
-                                                  const int *__restrict__ live) {
+                                                  const int *__restrict__ live, const T *__restrict__ bdiv) {
     const int tile = xcd_tile(blockIdx.x, tm.nblocks);
     if (tile >= tm.ntiles) return;
     const int tx = tile % tm.tiles_x;
@@ -74,6 +78,7 @@ __global__ __launch_bounds__(256) void k_level_op(OpView<T> op, TileMap tm, cons
         row_coefs<T, MODE>(op, i, j, kk, p, nb, cs, c);
         cx<T> dv = mk<T>(0, 0);
         if (EPI == EPI_SMOOTH) dv = (MODE == MODE_AH) ? conj(dinv[p]) : dinv[p];
+        const T bs = bdiv ? T(1) / bdiv[p] : T(1);
         for (int r = 0; r < k; ++r) {
             if (live && !live[r]) continue;
             const cx<T> *xr = xi + r * n + p;
@@ -81,7 +86,7 @@ __global__ __launch_bounds__(256) void k_level_op(OpView<T> op, TileMap tm, cons
             cx<T> acc = cs * x0;
 #pragma unroll
             for (int q = 0; q < 6; ++q) acc = acc + c[q] * xr[nb.off[q]];
-            const cx<T> res = b[r * n + p] - acc;
+            const cx<T> res = bs * b[r * n + p] - acc;
             xo[r * n + p] = (EPI == EPI_SMOOTH) ? x0 + w * (dv * res) : res;
         }
     }
@@ -91,12 +96,13 @@ __global__ __launch_bounds__(256) void k_level_op(OpView<T> op, TileMap tm, cons
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_level_jacobi0(const cx<T> *__restrict__ b, cx<T> *__restrict__ xo,
                                                        const cx<T> *__restrict__ dinv, T w, int64_t n,
-                                                       const int *__restrict__ live) {
+                                                       const int *__restrict__ live, const T *__restrict__ bdiv) {
     const int r = blockIdx.y;
     if (live && !live[r]) return;
     for (int64_t p = blockIdx.x * 256 + threadIdx.x; p < n; p += (int64_t)gridDim.x * 256) {
         const cx<T> dv = (MODE == MODE_AH) ? conj(dinv[p]) : dinv[p];
-        xo[r * n + p] = w * (dv * b[r * n + p]);
+        const T ws = bdiv ? w / bdiv[p] : w;
+        xo[r * n + p] = ws * (dv * b[r * n + p]);
     }
 }
 
