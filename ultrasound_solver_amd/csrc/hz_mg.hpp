@@ -179,6 +179,17 @@ __global__ __launch_bounds__(256) void k_prolong_add(const cx<T> *__restrict__ x
     else { cj[0] = coarse_of(j - 1, nyf, nyc); cj[1] = coarse_of(j + 1, nyf, nyc); wj[0] = wj[1] = T(0.5); nj = 2; }
     if (coarse_node(kk, nzf)) { ck[0] = coarse_of(kk, nzf, nzc); wk[0] = 1; nk = 1; }
     else { ck[0] = coarse_of(kk - 1, nzf, nzc); ck[1] = coarse_of(kk + 1, nzf, nzc); wk[0] = wk[1] = T(0.5); nk = 2; }
+    // Coarse identity-row nodes carry no correction (homogeneous Dirichlet for
+    // the correction).  For A^H their values are -A_IF^H x_I, scaled like a
+    // residual, and must not leak into interior fine nodes.
+    if (soft) {
+        for (int a = 0; a < ni; ++a)
+            if (on_soft_face(ci[a], 0, 1, nxc, 3, 3, soft & 3)) wi[a] = 0;
+        for (int b = 0; b < nj; ++b)
+            if (on_soft_face(1, cj[b], 1, 3, nyc, 3, soft & 12)) wj[b] = 0;
+        for (int c = 0; c < nk; ++c)
+            if (on_soft_face(1, 1, ck[c], 3, 3, nzc, soft & 48)) wk[c] = 0;
+    }
     for (int r = 0; r < k; ++r) {
         if (live && !live[r]) continue;
         const cx<T> *src = xc + r * nc;
