@@ -60,6 +60,15 @@ __global__ __launch_bounds__(256) void k_level_op(OpView<T> op, TileMap tm, cons
                                                   const cx<T> *__restrict__ b, cx<T> *__restrict__ xo,
                                                   const cx<T> *__restrict__ dinv, T w, int k,
                                                   const int *__restrict__ live, const T *__restrict__ bdiv) {
+    __shared__ int s_act[APPLY_MAX_RHS];
+    __shared__ int s_na;
+    if (threadIdx.x == 0) {
+        int na = 0;
+        for (int r = 0; r < k; ++r)
+            if (!live || live[r]) s_act[na++] = r;
+        s_na = na;
+    }
+    __syncthreads();
     const int tile = xcd_tile(blockIdx.x, tm.nblocks);
     if (tile >= tm.ntiles) return;
     const int tx = tile % tm.tiles_x;
@@ -69,6 +78,7 @@ __global__ __launch_bounds__(256) void k_level_op(OpView<T> op, TileMap tm, cons
     const int j = ty * 4 + (threadIdx.x >> 6);
     if (i >= op.nx || j >= op.ny) return;
     const int64_t n = op.n;
+    const int na = s_na;
     const int k0 = tz * tm.zb;
     const int k1 = min(op.nz, k0 + tm.zb);
     for (int kk = k0; kk < k1; ++kk) {
@@ -79,15 +89,39 @@ __global__ __launch_bounds__(256) void k_level_op(OpView<T> op, TileMap tm, cons
         cx<T> dv = mk<T>(0, 0);
         if (EPI == EPI_SMOOTH) dv = (MODE == MODE_AH) ? conj(dinv[p]) : dinv[p];
         const T bs = bdiv ? T(1) / bdiv[p] : T(1);
-        for (int r = 0; r < k; ++r) {
-            if (live && !live[r]) continue;
-            const cx<T> *xr = xi + r * n + p;
-            const cx<T> x0 = xr[0];
-            cx<T> acc = cs * x0;
+        int t = 0;
+        for (; t + 2 <= na; t += 2) {
+            const int r0 = s_act[t], r1 = s_act[t + 1];
+            const cx<T> *x0 = xi + r0 * n + p;
+            const cx<T> *x1 = xi + r1 * n + p;
+            cx<T> u0[7], u1[7];
+            u0[6] = x0[0];
+            u1[6] = x1[0];
 #pragma unroll
-            for (int q = 0; q < 6; ++q) acc = acc + c[q] * xr[nb.off[q]];
-            const cx<T> res = bs * b[r * n + p] - acc;
-            xo[r * n + p] = (EPI == EPI_SMOOTH) ? x0 + w * (dv * res) : res;
+            for (int q = 0; q < 6; ++q) {
+                u0[q] = x0[nb.off[q]];
+                u1[q] = x1[nb.off[q]];
+            }
+            const cx<T> b0 = b[r0 * n + p], b1 = b[r1 * n + p];
+            cx<T> a0 = cs * u0[6], a1 = cs * u1[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                a0 = a0 + c[q] * u0[q];
+                a1 = a1 + c[q] * u1[q];
+            }
+            const cx<T> e0 = bs * b0 - a0, e1 = bs * b1 - a1;
+            xo[r0 * n + p] = (EPI == EPI_SMOOTH) ? u0[6] + w * (dv * e0) : e0;
+            xo[r1 * n + p] = (EPI == EPI_SMOOTH) ? u1[6] + w * (dv * e1) : e1;
+        }
+        if (t < na) {
+            const int r0 = s_act[t];
+            const cx<T> *x0 = xi + r0 * n + p;
+            const cx<T> u06 = x0[0];
+            cx<T> a0 = cs * u06;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) a0 = a0 + c[q] * x0[nb.off[q]];
+            const cx<T> e0 = bs * b[r0 * n + p] - a0;
+            xo[r0 * n + p] = (EPI == EPI_SMOOTH) ? u06 + w * (dv * e0) : e0;
         }
     }
 }
@@ -103,6 +137,60 @@ __global__ __launch_bounds__(256) void k_level_jacobi0(const cx<T> *__restrict__
         const cx<T> dv = (MODE == MODE_AH) ? conj(dinv[p]) : dinv[p];
         const T ws = bdiv ? w / bdiv[p] : w;
         xo[r * n + p] = ws * (dv * b[r * n + p]);
+    }
+}
+
+constexpr int COARSE_MAX_NODES = 1024;
+
+// All damped-Jacobi sweeps of the coarsest level in one launch: one
+// workgroup per right-hand side keeps the iterate in LDS (ping-pong) and
+// each thread holds the coefficients of its <= 4 nodes in registers.
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_coarse_jacobi(OpView<T> op, const cx<T> *__restrict__ b,
+                                                       cx<T> *__restrict__ x, const cx<T> *__restrict__ dinv, T w,
+                                                       int sweeps, const int *__restrict__ live) {
+    constexpr int PER = COARSE_MAX_NODES / 256;
+    __shared__ cx<T> xs[2][COARSE_MAX_NODES];
+    const int r = blockIdx.x;
+    if (live && !live[r]) return;
+    const int n = (int)op.n;
+    cx<T> cs[PER], c[PER][6], dv[PER], bv[PER];
+    int off[PER][6];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int p = threadIdx.x + 256 * q;
+        if (p < n) {
+            const int i = p % op.nx, j = (p / op.nx) % op.ny, k = p / (op.nx * op.ny);
+            const Nbr nb = neighbours(i, j, k, op.nx, op.ny, op.nz);
+            row_coefs<T, MODE>(op, i, j, k, p, nb, cs[q], c[q]);
+#pragma unroll
+            for (int e = 0; e < 6; ++e) off[q][e] = (int)nb.off[e];
+            dv[q] = (MODE == MODE_AH) ? conj(dinv[p]) : dinv[p];
+            bv[q] = b[(int64_t)r * n + p];
+            xs[0][p] = w * (dv[q] * bv[q]);
+        }
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int s = 1; s < sweeps; ++s) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int p = threadIdx.x + 256 * q;
+            if (p < n) {
+                const cx<T> x0 = xs[cur][p];
+                cx<T> acc = cs[q] * x0;
+#pragma unroll
+                for (int e = 0; e < 6; ++e) acc = acc + c[q][e] * xs[cur][p + off[q][e]];
+                xs[cur ^ 1][p] = x0 + w * (dv[q] * (bv[q] - acc));
+            }
+        }
+        cur ^= 1;
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int p = threadIdx.x + 256 * q;
+        if (p < n) x[(int64_t)r * n + p] = xs[cur][p];
     }
 }
 

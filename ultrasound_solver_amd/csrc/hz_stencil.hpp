@@ -76,28 +76,30 @@ __device__ __forceinline__ Nbr neighbours(int i, int j, int k, int nx, int ny, i
 }
 
 // Edge coefficients coef_PQ (zero where the neighbour is absent) and, when
-// asked, the unmodified diagonal D_P (mass - sum coef + face terms).
+// asked, the unmodified diagonal D_P (mass - sum coef + face terms).  All
+// neighbour loads are issued unconditionally (an absent neighbour aliases P
+// and is weighted by zero) so that they overlap instead of serialising
+// behind per-neighbour branches.
 template <typename T, bool WANT_DIAG>
 __device__ __forceinline__ void edge_coefs(const OpView<T> &op, int i, int j, int k, int64_t p,
                                            const Nbr &nb, cx<T> c[6], cx<T> *diag) {
-    const cx<T> ax = op.a[0][i], ay = op.a[1][j], az = op.a[2][k];
+    T rq[6];
+#pragma unroll
+    for (int n = 0; n < 6; ++n) rq[n] = op.rho[p + nb.off[n]];
     const T rp = op.rho[p];
-    const cx<T> ayz = ay * az, axz = ax * az;
-    const cx<T> zero = mk<T>(0, 0);
-    c[0] = nb.has[0] ? (T(1) / (rp + op.rho[p + nb.off[0]])) * (ayz * op.q[0][i - 1]) : zero;
-    c[1] = nb.has[1] ? (T(1) / (rp + op.rho[p + nb.off[1]])) * (ayz * op.q[0][i]) : zero;
-    c[2] = nb.has[2] ? (T(1) / (rp + op.rho[p + nb.off[2]])) * (axz * op.q[1][j - 1]) : zero;
-    c[3] = nb.has[3] ? (T(1) / (rp + op.rho[p + nb.off[3]])) * (axz * op.q[1][j]) : zero;
-    if (nb.has[4] || nb.has[5]) {
-        const cx<T> axy = ax * ay;
-        c[4] = nb.has[4] ? (T(1) / (rp + op.rho[p + nb.off[4]])) * (axy * op.q[2][k - 1]) : zero;
-        c[5] = nb.has[5] ? (T(1) / (rp + op.rho[p + nb.off[5]])) * (axy * op.q[2][k]) : zero;
-    } else {
-        c[4] = zero;
-        c[5] = zero;
-    }
+    const cx<T> ax = op.a[0][i], ay = op.a[1][j], az = op.a[2][k];
+    const cx<T> q0 = op.q[0][i > 0 ? i - 1 : 0], q1 = op.q[0][i < op.nx - 1 ? i : op.nx - 2];
+    const cx<T> q2 = op.q[1][j > 0 ? j - 1 : 0], q3 = op.q[1][j < op.ny - 1 ? j : op.ny - 2];
+    const cx<T> q4 = op.q[2][k > 0 ? k - 1 : 0], q5 = op.q[2][k < op.nz - 1 ? k : op.nz - 2];
+    const cx<T> ayz = ay * az, axz = ax * az, axy = ax * ay;
+    c[0] = ((nb.has[0] ? T(1) : T(0)) / (rp + rq[0])) * (ayz * q0);
+    c[1] = ((nb.has[1] ? T(1) : T(0)) / (rp + rq[1])) * (ayz * q1);
+    c[2] = ((nb.has[2] ? T(1) : T(0)) / (rp + rq[2])) * (axz * q2);
+    c[3] = ((nb.has[3] ? T(1) : T(0)) / (rp + rq[3])) * (axz * q3);
+    c[4] = ((nb.has[4] ? T(1) : T(0)) / (rp + rq[4])) * (axy * q4);
+    c[5] = ((nb.has[5] ? T(1) : T(0)) / (rp + rq[5])) * (axy * q5);
     if (WANT_DIAG) {
-        const cx<T> mw = ax * ay * az;
+        const cx<T> mw = axy * az;
         const T kp = op.kappa[p];
         cx<T> d = (op.omega2 * kp) * mw;
         if (op.omega2_im != T(0)) d = d + mk<T>(0, op.omega2_im * kp) * mw;
@@ -161,21 +163,39 @@ __device__ __forceinline__ void row_coefs(const OpView<T> &op, int i, int j, int
         // R A M: the diagonal is r_P D_P / (r_P D_P) = 1 exactly.
         const T *rs = (MODE == MODE_RAM) ? op.rs_f : op.rs_a;
         const cx<T> *m = (MODE == MODE_RAM) ? op.m_f : op.m_a;
+        cx<T> mq[6];
+#pragma unroll
+        for (int n = 0; n < 6; ++n) mq[n] = m[p + nb.off[n]];
         const T r = rs[p];
         cs = mk<T>(1, 0);
 #pragma unroll
-        for (int n = 0; n < 6; ++n) {
-            if (nb.has[n]) c[n] = r * (c[n] * m[p + nb.off[n]]);
-        }
+        for (int n = 0; n < 6; ++n) c[n] = r * (c[n] * mq[n]);
     }
 }
 
+constexpr int APPLY_MAX_RHS = 256;
+
 // y[:, r] = s_r * op(x[:, r]) for active r.  One coefficient evaluation per
-// node serves every right-hand side.
+// node serves every right-hand side; the active list and scales are staged in
+// LDS and the right-hand sides are processed in pairs to keep 14 loads in flight.
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_apply(OpView<T> op, TileMap tm, const cx<T> *__restrict__ x,
                                                int64_t ldx, cx<T> *__restrict__ y, int64_t ldy, int nrhs,
                                                const int *__restrict__ live, const double *__restrict__ scale) {
+    __shared__ int s_act[APPLY_MAX_RHS];
+    __shared__ T s_sc[APPLY_MAX_RHS];
+    __shared__ int s_na;
+    if (threadIdx.x == 0) {
+        int na = 0;
+        for (int r = 0; r < nrhs; ++r) {
+            if (live && !live[r]) continue;
+            s_act[na] = r;
+            s_sc[na] = scale ? (T)scale[r] : T(1);
+            ++na;
+        }
+        s_na = na;
+    }
+    __syncthreads();
     const int tile = xcd_tile(blockIdx.x, tm.nblocks);
     if (tile >= tm.ntiles) return;
     const int tx = tile % tm.tiles_x;
@@ -184,6 +204,7 @@ __global__ __launch_bounds__(256) void k_apply(OpView<T> op, TileMap tm, const c
     const int i = tx * 64 + (threadIdx.x & 63);
     const int j = ty * 4 + (threadIdx.x >> 6);
     if (i >= op.nx || j >= op.ny) return;
+    const int na = s_na;
     const int k0 = tz * tm.zb;
     const int k1 = min(op.nz, k0 + tm.zb);
     for (int k = k0; k < k1; ++k) {
@@ -191,14 +212,35 @@ __global__ __launch_bounds__(256) void k_apply(OpView<T> op, TileMap tm, const c
         const Nbr nb = neighbours(i, j, k, op.nx, op.ny, op.nz);
         cx<T> cs, c[6];
         row_coefs<T, MODE>(op, i, j, k, p, nb, cs, c);
-        for (int r = 0; r < nrhs; ++r) {
-            if (live && !live[r]) continue;
-            const cx<T> *xr = x + r * ldx + p;
-            cx<T> acc = cs * xr[0];
+        int t = 0;
+        for (; t + 2 <= na; t += 2) {
+            const int r0 = s_act[t], r1 = s_act[t + 1];
+            const cx<T> *x0 = x + r0 * ldx + p;
+            const cx<T> *x1 = x + r1 * ldx + p;
+            cx<T> v0[7], v1[7];
+            v0[6] = x0[0];
+            v1[6] = x1[0];
 #pragma unroll
-            for (int n = 0; n < 6; ++n) acc = acc + c[n] * xr[nb.off[n]];
-            if (scale) acc = (T)scale[r] * acc;
-            y[r * ldy + p] = acc;
+            for (int n = 0; n < 6; ++n) {
+                v0[n] = x0[nb.off[n]];
+                v1[n] = x1[nb.off[n]];
+            }
+            cx<T> a0 = cs * v0[6], a1 = cs * v1[6];
+#pragma unroll
+            for (int n = 0; n < 6; ++n) {
+                a0 = a0 + c[n] * v0[n];
+                a1 = a1 + c[n] * v1[n];
+            }
+            y[r0 * ldy + p] = s_sc[t] * a0;
+            y[r1 * ldy + p] = s_sc[t + 1] * a1;
+        }
+        if (t < na) {
+            const int r0 = s_act[t];
+            const cx<T> *x0 = x + r0 * ldx + p;
+            cx<T> a0 = cs * x0[0];
+#pragma unroll
+            for (int n = 0; n < 6; ++n) a0 = a0 + c[n] * x0[nb.off[n]];
+            y[r0 * ldy + p] = s_sc[t] * a0;
         }
     }
 }
