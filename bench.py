@@ -37,6 +37,12 @@ def parse():
     p.add_argument("--cpu-iters", type=int, default=60, help="scipy GMRES iterations in the CPU sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--stencil-reps", type=int, default=20)
+    p.add_argument("--stencil-only", action="store_true",
+                   help="only the stencil launches and a 512 MiB copy (for hardware counter runs)")
+    p.add_argument("--mg-shift", type=float, default=0.5)
+    p.add_argument("--mg-pre", type=int, default=1)
+    p.add_argument("--mg-post", type=int, default=1)
+    p.add_argument("--mg-weight", type=float, default=0.7)
     p.add_argument("--precond", choices=["multigrid", "jacobi"], default="multigrid",
                    help="inner preconditioner of the mixed form (the plain form always uses Jacobi)")
     return p.parse_args()
@@ -64,8 +70,11 @@ def main():
     if np.any(op.pml_mask[:, :, 0].ravel(order="F")[node]):
         raise ValueError("PML overlaps the deposited sinc aperture")
     op.set_sources(node, col, val, scene.num_sources)
+    if args.stencil_only:
+        stencil_only(op, args)
+        return
     if args.precond == "multigrid":
-        op.set_multigrid()
+        op.set_multigrid(shift=args.mg_shift, pre=args.mg_pre, post=args.mg_post, weight=args.mg_weight)
     m = scene.num_sources
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
@@ -156,6 +165,23 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def stencil_only(op, args):
+    """Stencil launches (k complex64 RHS, R A M mode) and a copy of known size,
+    so that hardware byte counters can be scaled: the copy moves exactly
+    512 MiB in and 512 MiB out."""
+    import torch
+    k = args.chunk
+    x = torch.randn((k, op.n), dtype=torch.complex64, device="cuda:0")
+    for _ in range(args.stencil_reps):
+        op.apply(x, mode="RAM", precision="c64")
+    src = torch.ones(128*1024*1024, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"stencil_launches": args.stencil_reps, "rhs": k, "n": op.n,
+                      "alg_bytes_per_launch": (16*k + 8)*op.n, "copy_bytes_each_way": src.numel()*4}))
 
 
 def cpu_baseline(scene, args, plain_iters):
