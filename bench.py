@@ -39,7 +39,7 @@ def parse():
     p.add_argument("--stencil-reps", type=int, default=20)
     p.add_argument("--stencil-only", action="store_true",
                    help="only the stencil launches and a 512 MiB copy (for hardware counter runs)")
-    p.add_argument("--mg-shift", type=float, default=0.5)
+    p.add_argument("--mg-shift", type=float, default=0.7)
     p.add_argument("--mg-pre", type=int, default=1)
     p.add_argument("--mg-post", type=int, default=1)
     p.add_argument("--mg-weight", type=float, default=0.7)
@@ -169,14 +169,17 @@ def main():
 
 def stencil_only(op, args):
     """Stencil launches (k complex64 RHS, R A M mode) and a copy of known size,
-    so that hardware byte counters can be scaled: the copy moves exactly
-    512 MiB in and 512 MiB out."""
+    so that hardware byte counters can be scaled.  The copy moves 2 GiB each
+    way from a source evicted from the 256 MB last-level cache by a 1 GiB
+    fill in between."""
     import torch
     k = args.chunk
     x = torch.randn((k, op.n), dtype=torch.complex64, device="cuda:0")
     for _ in range(args.stencil_reps):
         op.apply(x, mode="RAM", precision="c64")
-    src = torch.ones(128*1024*1024, dtype=torch.float32, device="cuda:0")
+    src = torch.ones(512*1024*1024, dtype=torch.float32, device="cuda:0")
+    flush = torch.empty(256*1024*1024, dtype=torch.float32, device="cuda:0")
+    flush.fill_(2.0)
     dst = torch.empty_like(src)
     dst.copy_(src)
     torch.cuda.synchronize()

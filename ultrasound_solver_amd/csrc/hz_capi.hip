@@ -16,6 +16,7 @@
 #include "../../include/hz.h"
 #include "hz_krylov.hpp"
 #include "hz_mg.hpp"
+#include "hz_slab.hpp"
 #include "hz_stencil.hpp"
 
 using namespace hz;
@@ -143,9 +144,19 @@ void launch_apply(const hz_operator *op, const cx<T> *x, int64_t ldx, cx<T> *y, 
                   const int *live, const double *scale, hipStream_t s) {
     for (int r0 = 0; r0 < nrhs; r0 += APPLY_MAX_RHS) {
         const int kk = std::min(APPLY_MAX_RHS, nrhs - r0);
-        hipLaunchKernelGGL((k_apply<T, MODE>), dim3(op->tm.nblocks), dim3(256), 0, s, view<T>(op), op->tm,
-                           x + (int64_t)r0 * ldx, ldx, y + (int64_t)r0 * ldy, ldy, kk, live ? live + r0 : nullptr,
-                           scale ? scale + r0 : nullptr);
+        const cx<T> *xs = x + (int64_t)r0 * ldx;
+        cx<T> *ys = y + (int64_t)r0 * ldy;
+        const int *lv = live ? live + r0 : nullptr;
+        const double *sc = scale ? scale + r0 : nullptr;
+        if constexpr (std::is_same<T, float>::value) {
+            // complex64: the 2.5-D slab kernel (register z-window, DPP x-neighbours)
+            hipLaunchKernelGGL((k_slab<MODE, SLAB_APPLY>), dim3(op->tm.nblocks), dim3(256), 0, s, op->v32, op->tm, xs,
+                               ldx, ys, ldy, kk, lv, sc, (const cx<float> *)nullptr, (const cx<float> *)nullptr, 0.0f,
+                               (const float *)nullptr);
+        } else {
+            hipLaunchKernelGGL((k_apply<T, MODE>), dim3(op->tm.nblocks), dim3(256), 0, s, view<T>(op), op->tm, xs, ldx,
+                               ys, ldy, kk, lv, sc);
+        }
         HZ_LAUNCHED();
     }
 }
@@ -206,8 +217,8 @@ void mg_reserve(Multigrid &mg, int k) {
 template <int MODE>
 void level_smooth(MgLevel &L, const cx<float> *x, const cx<float> *b, cx<float> *out, float w, int k,
                   const int *live, const float *bdiv, hipStream_t s) {
-    hipLaunchKernelGGL((k_level_op<float, MODE, EPI_SMOOTH>), dim3(L.tm.nblocks), dim3(256), 0, s, L.v, L.tm, x, b,
-                       out, (const cx<float> *)L.dinv.as<cx<float>>(), w, k, live, bdiv);
+    hipLaunchKernelGGL((k_slab<MODE, SLAB_SMOOTH>), dim3(L.tm.nblocks), dim3(256), 0, s, L.v, L.tm, x, L.n, out, L.n,
+                       k, live, (const double *)nullptr, b, (const cx<float> *)L.dinv.as<cx<float>>(), w, bdiv);
     HZ_LAUNCHED();
 }
 
@@ -241,8 +252,8 @@ void vcycle(Multigrid &mg, int l, const cx<float> *b, cx<float> *x, int k, const
     }
     if (!coarsest) {
         MgLevel &C = *mg.levels[l + 1];
-        hipLaunchKernelGGL((k_level_op<float, MODE, EPI_RESID>), dim3(L.tm.nblocks), dim3(256), 0, s, L.v, L.tm, cur,
-                           b, alt, (const cx<float> *)nullptr, 0.0f, k, live, bdiv);
+        hipLaunchKernelGGL((k_slab<MODE, SLAB_RESID>), dim3(L.tm.nblocks), dim3(256), 0, s, L.v, L.tm, cur, L.n, alt,
+                           L.n, k, live, (const double *)nullptr, b, (const cx<float> *)nullptr, 0.0f, bdiv);
         HZ_LAUNCHED();
         hipLaunchKernelGGL((k_restrict<float>), dim3((C.n + 255) / 256), dim3(256), 0, s, alt, L.nx, L.ny, L.nz,
                            C.bb.as<cx<float>>(), C.nx, C.ny, C.nz, k, live, L.v.soft);

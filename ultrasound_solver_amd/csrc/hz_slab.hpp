@@ -1,0 +1,122 @@
+// 2.5-D complex64 stencil: the hot apply of the mixed form and the
+// multigrid smoother/residual.
+//
+// A 256-thread workgroup owns a 64 x 4 (x, y) tile and a slab of SLAB_Z
+// z-planes.  The row coefficients of the thread's SLAB_Z nodes are computed
+// once and kept in registers; then, for each right-hand side, the thread
+// marches z with the (z-1, z, z+1) values in registers (one new load per
+// node), takes its x-neighbours from the adjacent lanes with DPP wave
+// shifts (only lanes 0 and 63 load across the tile edge), and loads the two
+// y-neighbours (cache hits inside the tile).  That is about 3 vector loads
+// per node and right-hand side instead of 7.
+#pragma once
+
+#include "hz_stencil.hpp"
+
+namespace hz {
+
+constexpr int SLAB_Z = 4;
+
+enum SlabEpi { SLAB_APPLY = 0, SLAB_SMOOTH = 1, SLAB_RESID = 2 };
+
+// Value of lane l-1 (wave_shr:1) / lane l+1 (wave_shl:1); lanes at the wave
+// edge receive 0 and are fixed up by an explicit load.
+__device__ __forceinline__ float lane_prev(float v) { return __builtin_amdgcn_update_dpp(0.0f, v, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ float lane_next(float v) { return __builtin_amdgcn_update_dpp(0.0f, v, 0x130, 0xf, 0xf, false); }
+
+// APPLY : y = s_r * op(x)
+// SMOOTH: y = x + w * dinv * (b / bdiv - op(x))     (damped Jacobi)
+// RESID : y = b / bdiv - op(x)
+template <int MODE, int EPI>
+__global__ __launch_bounds__(256) void k_slab(OpView<float> op, TileMap tm, const cx<float> *__restrict__ xin,
+                                              int64_t ldx, cx<float> *__restrict__ yout, int64_t ldy, int nrhs,
+                                              const int *__restrict__ live, const double *__restrict__ scale,
+                                              const cx<float> *__restrict__ b, const cx<float> *__restrict__ dinv,
+                                              float w, const float *__restrict__ bdiv) {
+    __shared__ int s_act[APPLY_MAX_RHS];
+    __shared__ float s_sc[APPLY_MAX_RHS];
+    __shared__ int s_na;
+    if (threadIdx.x == 0) {
+        int na = 0;
+        for (int r = 0; r < nrhs; ++r) {
+            if (live && !live[r]) continue;
+            s_act[na] = r;
+            s_sc[na] = scale ? (float)scale[r] : 1.0f;
+            ++na;
+        }
+        s_na = na;
+    }
+    __syncthreads();
+    const int tile = xcd_tile(blockIdx.x, tm.nblocks);
+    if (tile >= tm.ntiles) return;  // whole workgroup
+    const int tx = tile % tm.tiles_x;
+    const int ty = (tile / tm.tiles_x) % tm.tiles_y;
+    const int tz = tile / (tm.tiles_x * tm.tiles_y);
+    const int lane = threadIdx.x & 63;
+    const int i_raw = tx * 64 + lane;
+    const int j_raw = ty * 4 + (threadIdx.x >> 6);
+    // Lanes past the grid edge keep running (they feed the DPP shifts) on a
+    // clamped node and never store.
+    const bool valid = i_raw < op.nx && j_raw < op.ny;
+    const int i = min(i_raw, op.nx - 1);
+    const int j = min(j_raw, op.ny - 1);
+    const int k0 = tz * tm.zb;
+    const int nzl = min(op.nz - k0, SLAB_Z);
+    const int64_t sxy = (int64_t)op.nx * op.ny;
+    const int64_t p0 = i + (int64_t)op.nx * (j + (int64_t)op.ny * k0);
+
+    cx<float> cs[SLAB_Z], c[SLAB_Z][6], dv[SLAB_Z];
+    float bs[SLAB_Z];
+#pragma unroll
+    for (int q = 0; q < SLAB_Z; ++q) {
+        const int kq = min(k0 + q, op.nz - 1);  // planes past the slab end repeat the last one
+        const int64_t p = i + (int64_t)op.nx * (j + (int64_t)op.ny * kq);
+        const Nbr nb = neighbours(i, j, kq, op.nx, op.ny, op.nz);
+        row_coefs<float, MODE>(op, i, j, kq, p, nb, cs[q], c[q]);
+        dv[q] = mk<float>(0, 0);
+        bs[q] = 1.0f;
+        if (EPI == SLAB_SMOOTH) dv[q] = (MODE == MODE_AH) ? conj(dinv[p]) : dinv[p];
+        if (EPI != SLAB_APPLY && bdiv) bs[q] = 1.0f / bdiv[p];
+    }
+    const bool has_xm = i > 0, has_xp = i < op.nx - 1;
+    const int64_t oym = j > 0 ? -(int64_t)op.nx : 0;
+    const int64_t oyp = j < op.ny - 1 ? (int64_t)op.nx : 0;
+    const int na = s_na;
+    for (int t = 0; t < na; ++t) {
+        const int r = s_act[t];
+        const cx<float> *xr = xin + r * ldx;
+        cx<float> xm = k0 > 0 ? xr[p0 - sxy] : mk<float>(0, 0);
+        cx<float> xc = xr[p0];
+#pragma unroll
+        for (int q = 0; q < SLAB_Z; ++q) {
+            if (q >= nzl) break;
+            const int64_t p = p0 + q * sxy;
+            const cx<float> xp = (k0 + q + 1 < op.nz) ? xr[p + sxy] : mk<float>(0, 0);
+            cx<float> xl = mk<float>(lane_prev(xc.re), lane_prev(xc.im));
+            cx<float> xn = mk<float>(lane_next(xc.re), lane_next(xc.im));
+            if (lane == 0 && has_xm) xl = xr[p - 1];
+            if (lane == 63 && has_xp) xn = xr[p + 1];
+            const cx<float> ym = xr[p + oym];
+            const cx<float> yp = xr[p + oyp];
+            cx<float> acc = cs[q] * xc;
+            acc = acc + c[q][0] * xl;
+            acc = acc + c[q][1] * xn;
+            acc = acc + c[q][2] * ym;
+            acc = acc + c[q][3] * yp;
+            acc = acc + c[q][4] * xm;
+            acc = acc + c[q][5] * xp;
+            if (valid) {
+                if (EPI == SLAB_APPLY) {
+                    yout[r * ldy + p] = s_sc[t] * acc;
+                } else {
+                    const cx<float> e = bs[q] * b[r * ldy + p] - acc;
+                    yout[r * ldy + p] = (EPI == SLAB_SMOOTH) ? xc + w * (dv[q] * e) : e;
+                }
+            }
+            xm = xc;
+            xc = xp;
+        }
+    }
+}
+
+}  // namespace hz
