@@ -1,0 +1,148 @@
+"""The benchmark workloads at full size, checked through size-independent
+properties against the CPU oracle.
+
+* FP64 residuals of the returned fields are recomputed on the CPU with the
+  oracle's complex128 CSR matrix and source columns (reference acceptance:
+  scaled residual <= tolerance).
+* Linearity of the basis: G e^{i phi} equals a direct solve with b(phi)
+  (relative L2 1e-7 for the FP64 basis).
+* Adjoint phase gradients: central differences (delta = 1e-4 rad) and the
+  plain FP64 adjoint gradient agree with the mixed-form gradient to 1e-4
+  (scaled error of the reference's gradient check).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import helmholtz as ohz, krylov as okr, source as osrc
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def build(scene, multigrid=False):
+    from ultrasound_solver_amd import scenes
+    from ultrasound_solver_amd.operator import HelmholtzOperator
+    op = HelmholtzOperator(scene.rho, scene.c, scene.dx, scene.omega, scene.boundaries, scene.pml, device="cuda:0")
+    node, col, val = scenes.source_triplets(scene)
+    op.set_sources(node, col, val, scene.num_sources)
+    if multigrid:
+        op.set_multigrid()
+    return op
+
+
+def oracle(scene):
+    a = ohz.OracleOperator(scene.rho, scene.c, scene.dx, scene.omega, scene.boundaries, scene.pml).matrix()
+    m = osrc.source_mass_matrix(scene.shape[:2], scene.dx, scene.centers, 2*scene.radius)
+    bsrc = osrc.flux_rhs_columns(m, scene.shape, scene.dx, scene.rho_ref)
+    return a, bsrc
+
+
+def unit_columns(op, cols):
+    w = torch.zeros((len(cols), op.num_sources), dtype=torch.complex128, device="cuda:0")
+    for r, c in enumerate(cols):
+        w[r, c] = 1.0
+    return op.build_rhs(w)
+
+
+def test_small_homogeneous_plain_fp64_basis():
+    """64^3 air, 4x4 array: 16 FP64 GMRES(50) solves to 1e-8 build G; G e^{i phi}
+    matches a separate solve of b(phi) with phi ~ U[0, 2 pi) (seed 0)."""
+    from ultrasound_solver_amd import scenes
+    sc = scenes.config_small(64)
+    op = build(sc)
+    a, bsrc = oracle(sc)
+    b = unit_columns(op, range(16))
+    g, st = op.gmres(b, restart=50, maxiter=20000, rtol=1e-8, precision="c128", chunk=16)
+    assert all(s["converged"] for s in st), st
+    gh = g.cpu().numpy().T
+    for r in (0, 7, 15):
+        rr = bsrc[:, r].toarray().ravel() - a @ gh[:, r]
+        assert okr.scaled_residual(a, gh[:, r], bsrc[:, r].toarray().ravel()) <= 1e-8
+        assert np.linalg.norm(rr) <= 1e-6*np.linalg.norm(bsrc[:, r].toarray())
+    phi = np.random.default_rng(0).uniform(0, 2*np.pi, 16)
+    u_basis = gh @ np.exp(1j*phi)
+    bphi = op.build_rhs(torch.as_tensor(np.exp(1j*phi)[None, :]))
+    u_direct, st2 = op.gmres(bphi, restart=50, maxiter=20000, rtol=1e-10, precision="c128")
+    u_direct = u_direct.cpu().numpy()[0]
+    assert np.linalg.norm(u_basis - u_direct)/np.linalg.norm(u_direct) < 1e-7
+
+
+def test_heterogeneous_128_mixed_multigrid_residuals():
+    from ultrasound_solver_amd import scenes
+    sc = scenes.config_heterogeneous(128)
+    op = build(sc, multigrid=True)
+    a, bsrc = oracle(sc)
+    cols = [0, 85, 170, 255]
+    b = unit_columns(op, cols)
+    x, st = op.gmres(b, restart=30, maxiter=60000, rtol=1e-6, precision="c64", chunk=4, preconditioner="multigrid")
+    assert all(s["converged"] for s in st), st
+    xh = x.cpu().numpy().T
+    for r, c in enumerate(cols):
+        bc = bsrc[:, c].toarray().ravel()
+        assert okr.scaled_residual(a, xh[:, r], bc) <= 1e-6
+        assert okr.backward_error(a, xh[:, r], bc) <= 1e-4
+        assert st[r]["scaled_residual"] == pytest.approx(okr.scaled_residual(a, xh[:, r], bc), rel=1e-6, abs=1e-12)
+
+
+def test_adjoint_gradient_160_fd_and_fp64_reference():
+    """160^3 air, 16x16 array, reference focal_contrast loss on a 5-cell ball
+    around a seeded target at z ~ 0.7 Lz; 4 seeded phase vectors batched."""
+    from ultrasound_solver_amd import autograd as hza, scenes
+    n = 160
+    sc = scenes.config_small(n, array_n=16)
+    op = build(sc, multigrid=True)
+    rng = np.random.default_rng(3)
+    tgt = np.array([rng.integers(n//2 - 10, n//2 + 10), rng.integers(n//2 - 10, n//2 + 10), int(0.7*(n - 1))])
+    ii, jj, kk = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    ball = (ii - tgt[0])**2 + (jj - tgt[1])**2 + (kk - tgt[2])**2 <= 25
+    mask = (~ball) & (kk >= n//2)
+    mask_t = torch.as_tensor(mask, device="cuda:0")
+    amps = np.ones(sc.num_sources)
+    eng = hza.PhaseEngine(op, amps, precision="c64", preconditioner="multigrid", rtol=1e-8, maxiter=60000, chunk=8)
+    eng64 = hza.PhaseEngine(op, amps, precision="c128", preconditioner="jacobi", rtol=1e-8, maxiter=400000, chunk=1)
+    idx = [tuple(tgt)]
+
+    def loss_of(u):
+        return hza.focal_contrast_loss(hza.grid_view(u, sc.shape), idx, 1.0, 1.3, 5.0, mask_t)
+
+    phs = rng.uniform(0, 2*np.pi, (4, sc.num_sources))
+    ph = torch.tensor(phs, device="cuda:0", requires_grad=True)
+    u = hza.phase_field(ph, eng)
+    total = sum(loss_of(u[r]) for r in range(4))
+    total.backward()
+    grad = ph.grad.cpu().numpy()
+    # FD on two random emitters of the first phase vector, batched as one solve.
+    pick = rng.choice(sc.num_sources, 2, replace=False)
+    batch = []
+    for e in pick:
+        for sgn in (1, -1):
+            p2 = phs[0].copy()
+            p2[e] += sgn*1e-4
+            batch.append(p2)
+    with torch.no_grad():
+        uf = hza.phase_field(torch.tensor(np.array(batch), device="cuda:0"), eng)
+        vals = [float(loss_of(uf[q])) for q in range(4)]
+    for q, e in enumerate(pick):
+        fd = (vals[2*q] - vals[2*q + 1])/2e-4
+        assert abs(grad[0, e] - fd)/max(abs(grad[0, e]), abs(fd), 1.0) < 1e-4
+    # FP64 plain-form (reference semantics) gradient of the first vector.
+    p64 = torch.tensor(phs[:1], device="cuda:0", requires_grad=True)
+    loss_of(hza.phase_field(p64, eng64)[0]).backward()
+    g64 = p64.grad.cpu().numpy()[0]
+    assert np.max(np.abs(grad[0] - g64))/np.max(np.abs(g64)) < 1e-4
+
+
+@pytest.mark.parametrize("index", [0, 1])
+def test_uneven_scenes_with_impedance_walls(index):
+    from ultrasound_solver_amd import scenes
+    sc = scenes.config_uneven(index)
+    op = build(sc, multigrid=True)
+    a, bsrc = oracle(sc)
+    cols = [0, 27, 63]
+    b = unit_columns(op, cols)
+    x, st = op.gmres(b, rtol=1e-6, precision="c64", chunk=3, maxiter=60000, preconditioner="multigrid")
+    assert all(s["converged"] for s in st), (sc.name, st)
+    xh = x.cpu().numpy().T
+    for r, c in enumerate(cols):
+        assert okr.scaled_residual(a, xh[:, r], bsrc[:, c].toarray().ravel()) <= 1e-6

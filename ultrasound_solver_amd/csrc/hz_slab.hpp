@@ -85,27 +85,43 @@ __global__ __launch_bounds__(256) void k_slab(OpView<float> op, TileMap tm, cons
     for (int t = 0; t < na; ++t) {
         const int r = s_act[t];
         const cx<float> *xr = xin + r * ldx;
-        cx<float> xm = k0 > 0 ? xr[p0 - sxy] : mk<float>(0, 0);
-        cx<float> xc = xr[p0];
+        // Issue every load of this right-hand side's column first: planes
+        // k0-1 .. k0+SLAB_Z (absent planes read a valid node and are zeroed),
+        // the y-neighbours, and the x-edge values of lanes 0 and 63.
+        cx<float> v[SLAB_Z + 2], ym[SLAB_Z], yp[SLAB_Z], xe[SLAB_Z];
+#pragma unroll
+        for (int q = 0; q < SLAB_Z + 2; ++q) {
+            const int kq = k0 + q - 1;
+            const int kc = min(max(kq, 0), op.nz - 1);
+            v[q] = xr[p0 + (int64_t)(kc - k0) * sxy];
+            if (kq < 0 || kq >= op.nz) v[q] = mk<float>(0, 0);
+        }
 #pragma unroll
         for (int q = 0; q < SLAB_Z; ++q) {
-            if (q >= nzl) break;
-            const int64_t p = p0 + q * sxy;
-            const cx<float> xp = (k0 + q + 1 < op.nz) ? xr[p + sxy] : mk<float>(0, 0);
+            const int kc = min(k0 + q, op.nz - 1);
+            const int64_t p = p0 + (int64_t)(kc - k0) * sxy;
+            ym[q] = xr[p + oym];
+            yp[q] = xr[p + oyp];
+            xe[q] = mk<float>(0, 0);
+            if (lane == 0 && has_xm) xe[q] = xr[p - 1];
+            if (lane == 63 && has_xp) xe[q] = xr[p + 1];
+        }
+#pragma unroll
+        for (int q = 0; q < SLAB_Z; ++q) {
+            const cx<float> xc = v[q + 1];
             cx<float> xl = mk<float>(lane_prev(xc.re), lane_prev(xc.im));
             cx<float> xn = mk<float>(lane_next(xc.re), lane_next(xc.im));
-            if (lane == 0 && has_xm) xl = xr[p - 1];
-            if (lane == 63 && has_xp) xn = xr[p + 1];
-            const cx<float> ym = xr[p + oym];
-            const cx<float> yp = xr[p + oyp];
+            if (lane == 0) xl = xe[q];
+            if (lane == 63) xn = xe[q];
             cx<float> acc = cs[q] * xc;
             acc = acc + c[q][0] * xl;
             acc = acc + c[q][1] * xn;
-            acc = acc + c[q][2] * ym;
-            acc = acc + c[q][3] * yp;
-            acc = acc + c[q][4] * xm;
-            acc = acc + c[q][5] * xp;
-            if (valid) {
+            acc = acc + c[q][2] * ym[q];
+            acc = acc + c[q][3] * yp[q];
+            acc = acc + c[q][4] * v[q];
+            acc = acc + c[q][5] * v[q + 2];
+            if (valid && q < nzl) {
+                const int64_t p = p0 + (int64_t)q * sxy;
                 if (EPI == SLAB_APPLY) {
                     yout[r * ldy + p] = s_sc[t] * acc;
                 } else {
@@ -113,8 +129,6 @@ __global__ __launch_bounds__(256) void k_slab(OpView<float> op, TileMap tm, cons
                     yout[r * ldy + p] = (EPI == SLAB_SMOOTH) ? xc + w * (dv[q] * e) : e;
                 }
             }
-            xm = xc;
-            xc = xp;
         }
     }
 }

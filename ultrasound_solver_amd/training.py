@@ -499,3 +499,108 @@ def make_loss(cfg, target=None):
     if kind == "focal_contrast":
         return FocalContrastLoss(cfg, target)
     raise ValueError("Unknown training.loss_type")
+
+
+# --------------------------------------------------------------- the objective
+def wrap_phases(phases):
+    phases = np.asarray(phases, dtype=np.float64)
+    if not np.isfinite(phases).all():
+        raise ValueError("phases must be finite")
+    return np.mod(phases, 2*np.pi)
+
+
+class PhaseObjective:
+    """Loss and phase gradient of one scene, evaluated on the device.
+
+    Mirrors ``PhaseProblem.evaluate`` (src/algorithm/baselines/common.py:275-343):
+    phases wrapped mod 2 pi, field u(phi), loss with the exact focal samples
+    (trilinear target response) when targets exist, gradient = phase VJP of the
+    grid cotangent plus the focal gradient (common.py:189-195), finiteness
+    checks and an evaluation history.  The field comes from the basis G when
+    one is given (``basis.field``/``phase_vjp``), otherwise from one forward
+    solve and the gradient from one adjoint solve (A^H lambda = c,
+    G^H c = a * B^T lambda) without forming G.
+    """
+
+    def __init__(self, solver, basis=None, engine_kwargs=None):
+        self.solver = solver
+        self.cfg = solver.cfg
+        self.basis = basis
+        self.loss_model = make_loss(self.cfg)
+        self.size = solver.transducers.num_transducers
+        self.shape = tuple(self.cfg.domain.grid_size)
+        self.history = []
+        self.field_evaluations = 0
+        self.vjp_evaluations = 0
+        if basis is None:
+            from .autograd import PhaseEngine
+            solver.assemble_matrix()
+            kw = dict(precision="c64", preconditioner="multigrid" if solver.operator.__dict__.get("hierarchy")
+                      else "jacobi", rtol=1e-8, maxiter=100000, chunk=8)
+            kw.update(engine_kwargs or {})
+            self.engine = PhaseEngine(solver.operator, solver.transducers.amplitudes, **kw)
+        if self.cfg.targets and self.cfg.training.loss_type != "field_match":
+            self.response = (basis.target_response(np.asarray(self.cfg.targets)) if basis is not None
+                             else solver.target_response(np.asarray(self.cfg.targets)))
+        else:
+            self.response = None
+
+    def _field(self, phases):
+        if self.basis is not None:
+            return self.basis.field(phases)
+        w = torch.as_tensor(phases, device=self.solver.operator.device)
+        u = self.engine.forward(w.reshape(1, -1))[0]
+        return u.cpu().numpy().reshape(self.shape, order="F")
+
+    def _vjp(self, cot, phases):
+        if self.basis is not None:
+            return self.basis.phase_vjp(cot, phases)
+        c = torch.as_tensor(np.asarray(cot, np.complex128).reshape(1, -1, order="F"), device=self.solver.operator.device)
+        resp = self.engine.response_adjoint(c)[0].cpu().numpy()
+        return np.real(np.conj(resp)*(1j*np.exp(1j*phases)))
+
+    def evaluate(self, phases, gradient=False, stage="optimization"):
+        phases = wrap_phases(phases)
+        if phases.shape != (self.size,):
+            raise ValueError(f"Expected {self.size} phases")
+        self.field_evaluations += 1
+        u = self._field(phases)
+        focal = self.response @ np.exp(1j*phases) if self.response is not None else None
+        kw = {"focal_field": focal} if focal is not None else {}
+        if isinstance(self.loss_model, AmplitudeFieldLoss):
+            loss, cot = self.loss_model.evaluate(u, with_cotangent=gradient)
+        else:
+            loss, cot = self.loss_model.evaluate(u, with_cotangent=gradient, **kw)
+        if not np.isfinite(loss) or not np.isfinite(u).all():
+            raise FloatingPointError("Non-finite field or loss")
+        grad = None
+        if gradient:
+            self.vjp_evaluations += 1
+            grad = self._vjp(cot, phases)
+            if focal is not None:
+                lm = self.loss_model
+                fc = -lm.target_weight/len(focal)*focal/(np.abs(focal) + lm.eps)
+                adj = self.response.conj().T @ fc
+                grad = grad + np.real(adj.conj()*1j*np.exp(1j*phases))
+            if not np.isfinite(grad).all():
+                raise FloatingPointError("Non-finite phase gradient")
+        self.history.append({"evaluation": self.field_evaluations, "vjp_evaluations": self.vjp_evaluations,
+                             "loss": float(loss), "stage": stage,
+                             "gradient_norm": float(np.linalg.norm(grad)) if grad is not None else None})
+        return float(loss), grad
+
+    def gradient_check(self, phases, step=1.0e-3, count=8):
+        """Central differences on the first min(count, M) emitters with the
+        scaled error |g - fd| / max(|g|, |fd|, 1) (baselines/adjoint/__init__.py:44-58)."""
+        _, grad = self.evaluate(phases, gradient=True)
+        checks = []
+        for index in range(min(count, self.size)):
+            e = np.zeros(self.size)
+            e[index] = step
+            plus, _ = self.evaluate(phases + e, stage="gradient_check")
+            minus, _ = self.evaluate(phases - e, stage="gradient_check")
+            fd = (plus - minus)/(2*step)
+            err = abs(grad[index] - fd)/max(abs(grad[index]), abs(fd), 1.0)
+            checks.append({"index": index, "analytic": float(grad[index]), "finite_difference": float(fd),
+                           "scaled_error": float(err)})
+        return checks
