@@ -39,7 +39,7 @@ def parse():
     p.add_argument("--stencil-reps", type=int, default=20)
     p.add_argument("--stencil-only", action="store_true",
                    help="only the stencil launches and a 512 MiB copy (for hardware counter runs)")
-    p.add_argument("--mg-shift", type=float, default=0.7)
+    p.add_argument("--mg-shift", type=float, default=1.2)
     p.add_argument("--mg-pre", type=int, default=1)
     p.add_argument("--mg-post", type=int, default=1)
     p.add_argument("--mg-weight", type=float, default=0.7)
@@ -143,6 +143,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(scene, args, plain_iters)
+    traffic = measured_traffic(args)
 
     out = {
         "metric": "RHS solves/s at 128^3 to 1e-6 rel. residual (mixed vs FP64); stencil HBM GB/s",
@@ -159,12 +160,28 @@ def main():
         "refinements_median": float(np.median(refines)) if refines else None,
         "worst_scaled_residual": worst,
         "setup_s": setup_s,
-        "roofline": {"kernel": "k_apply<float, MODE_RAM>", "bound": "hbm", "achieved": achieved,
-                     "peak": 8000.0, "unit": "GB/s", "frac": achieved/8000.0, "traffic": None,
-                     "alg_bytes_per_launch": alg_bytes, "launch_us": stencil_s*1e6},
+        "roofline": {"kernel": "hz::k_slab<1, 0> (complex64 R A M apply)", "bound": "hbm",
+                     "achieved": achieved, "peak": 8000.0, "unit": "GB/s", "frac": achieved/8000.0,
+                     "traffic": traffic, "alg_bytes_per_launch": alg_bytes, "launch_us": stencil_s*1e6},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "stencil_128_k16_traffic.md")
+
+
+def measured_traffic(args):
+    """HBM bytes per stencil launch from the FETCH_SIZE/WRITE_SIZE counter runs
+    of the same kernel and workload (128^3, 16 columns), scaled by the copy
+    calibration recorded beside them; None for other workloads."""
+    if args.n != 128 or args.chunk != 16 or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    with open(TRAFFIC_PROFILE) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    return None
 
 
 def stencil_only(op, args):

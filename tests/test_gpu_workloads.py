@@ -8,7 +8,9 @@ properties against the CPU oracle.
   (relative L2 1e-7 for the FP64 basis).
 * Adjoint phase gradients: central differences (delta = 1e-4 rad) and the
   plain FP64 adjoint gradient agree with the mixed-form gradient to 1e-4
-  (scaled error of the reference's gradient check).
+  (scaled error of the reference's gradient check). The solves run to a
+  scaled residual of 1e-10: a central difference over 2e-4 rad divides the
+  loss error by 2e-4, so a 1e-8 solve leaves ~1e-3 of noise in it.
 """
 import numpy as np
 import pytest
@@ -87,7 +89,8 @@ def test_heterogeneous_128_mixed_multigrid_residuals():
 
 def test_adjoint_gradient_160_fd_and_fp64_reference():
     """160^3 air, 16x16 array, reference focal_contrast loss on a 5-cell ball
-    around a seeded target at z ~ 0.7 Lz; 4 seeded phase vectors batched."""
+    around a seeded target at z ~ 0.7 Lz; 4 seeded phase vectors batched;
+    central differences on 8 emitters drawn with seed 4."""
     from ultrasound_solver_amd import autograd as hza, scenes
     n = 160
     sc = scenes.config_small(n, array_n=16)
@@ -99,8 +102,8 @@ def test_adjoint_gradient_160_fd_and_fp64_reference():
     mask = (~ball) & (kk >= n//2)
     mask_t = torch.as_tensor(mask, device="cuda:0")
     amps = np.ones(sc.num_sources)
-    eng = hza.PhaseEngine(op, amps, precision="c64", preconditioner="multigrid", rtol=1e-8, maxiter=60000, chunk=8)
-    eng64 = hza.PhaseEngine(op, amps, precision="c128", preconditioner="jacobi", rtol=1e-8, maxiter=400000, chunk=1)
+    eng = hza.PhaseEngine(op, amps, precision="c64", preconditioner="multigrid", rtol=1e-10, maxiter=60000, chunk=8)
+    eng64 = hza.PhaseEngine(op, amps, precision="c128", preconditioner="jacobi", rtol=1e-10, maxiter=400000, chunk=1)
     idx = [tuple(tgt)]
 
     def loss_of(u):
@@ -112,8 +115,8 @@ def test_adjoint_gradient_160_fd_and_fp64_reference():
     total = sum(loss_of(u[r]) for r in range(4))
     total.backward()
     grad = ph.grad.cpu().numpy()
-    # FD on two random emitters of the first phase vector, batched as one solve.
-    pick = rng.choice(sc.num_sources, 2, replace=False)
+    # FD on 8 random emitters (seed 4) of the first phase vector, batched.
+    pick = np.random.default_rng(4).choice(sc.num_sources, 8, replace=False)
     batch = []
     for e in pick:
         for sgn in (1, -1):
@@ -122,9 +125,10 @@ def test_adjoint_gradient_160_fd_and_fp64_reference():
             batch.append(p2)
     with torch.no_grad():
         uf = hza.phase_field(torch.tensor(np.array(batch), device="cuda:0"), eng)
-        vals = [float(loss_of(uf[q])) for q in range(4)]
+        vals = [float(loss_of(uf[q])) for q in range(len(batch))]
     for q, e in enumerate(pick):
         fd = (vals[2*q] - vals[2*q + 1])/2e-4
+        print(f"emitter {e}: adjoint {grad[0, e]:.10g} central difference {fd:.10g}", flush=True)
         assert abs(grad[0, e] - fd)/max(abs(grad[0, e]), abs(fd), 1.0) < 1e-4
     # FP64 plain-form (reference semantics) gradient of the first vector.
     p64 = torch.tensor(phs[:1], device="cuda:0", requires_grad=True)

@@ -160,7 +160,7 @@ class HelmholtzOperator:
                                f"backward error={bad['backward_error']:.3e}")
         return x, out
 
-    def set_multigrid(self, shift=0.7, pre=1, post=1, coarse_sweeps=30, weight=0.7, levels=None):
+    def set_multigrid(self, shift=1.2, pre=1, post=1, coarse_sweeps=30, weight=0.7, levels=None):
         """Install the shifted-Laplacian multigrid preconditioner (mixed form)."""
         from .multigrid import Hierarchy
         self.hierarchy = Hierarchy(self, levels=levels).install(self, shift, pre, post, coarse_sweeps, weight)
