@@ -27,7 +27,7 @@ def parse():
     p.add_argument("--steps", type=int, default=2)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--n", type=int, default=128)
-    p.add_argument("--chunk", type=int, default=16)
+    p.add_argument("--chunk", type=int, default=0, help="RHS per step; 0 = autotuned")
     p.add_argument("--restart", type=int, default=30)
     p.add_argument("--maxiter", type=int, default=60000)
     p.add_argument("--plain-maxiter", type=int, default=300000,
@@ -43,6 +43,7 @@ def parse():
     p.add_argument("--mg-pre", type=int, default=1)
     p.add_argument("--mg-post", type=int, default=1)
     p.add_argument("--mg-weight", type=float, default=0.7)
+    p.add_argument("--graph", type=int, default=0, help="replay each GMRES cycle from a captured hipGraph")
     p.add_argument("--precond", choices=["multigrid", "jacobi"], default="multigrid",
                    help="inner preconditioner of the mixed form (the plain form always uses Jacobi)")
     return p.parse_args()
@@ -71,14 +72,17 @@ def main():
         raise ValueError("PML overlaps the deposited sinc aperture")
     op.set_sources(node, col, val, scene.num_sources)
     if args.stencil_only:
-        stencil_only(op, args)
+        stencil_only(op, args, scene.num_sources)
         return
     if args.precond == "multigrid":
         op.set_multigrid(shift=args.mg_shift, pre=args.mg_pre, post=args.mg_post, weight=args.mg_weight)
     m = scene.num_sources
+    tuning = op.autotune(m, max_chunk=args.chunk or 64, restart=args.restart, preconditioner=args.precond)
+    if not args.chunk:
+        args.chunk = tuning["chunk"]
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
-    log(f"setup {setup_s:.1f}s n={args.n} N={op.n} M={m} ({'; '.join(scene.notes)})")
+    log(f"setup {setup_s:.1f}s n={args.n} N={op.n} M={m} ({'; '.join(scene.notes)}); autotune {tuning}")
 
     def rhs_block(start, k):
         w = torch.zeros((k, m), dtype=torch.complex128, device="cuda:0")
@@ -87,7 +91,7 @@ def main():
         return op.build_rhs(w)
 
     common = dict(restart=args.restart, maxiter=args.maxiter, rtol=args.rtol, backward_error_tol=1e-4)
-    mixed = dict(common, precision="c64", chunk=args.chunk, preconditioner=args.precond)
+    mixed = dict(common, precision="c64", chunk=args.chunk, preconditioner=args.precond, use_graph=bool(args.graph))
     for s in range(args.warmup):
         b = rhs_block(s*args.chunk, args.chunk)
         _, st = op.gmres(b, **mixed)
@@ -114,7 +118,8 @@ def main():
     tp = time.perf_counter()
     for r in range(args.plain_rhs):
         b = rhs_block(r, 1)
-        _, st = op.gmres(b, precision="c128", chunk=1, **dict(common, maxiter=args.plain_maxiter))
+        _, st = op.gmres(b, precision="c128", chunk=1, use_graph=bool(args.graph),
+                         **dict(common, maxiter=args.plain_maxiter))
         plain_solved += st[0]["converged"]
         plain_iters.append(st[0]["iterations"])
     torch.cuda.synchronize()
@@ -124,7 +129,7 @@ def main():
 
     # Stencil roofline: R A M apply over `chunk` complex64 right-hand sides,
     # timed with events on the stream the kernel is launched on.
-    k = args.chunk
+    k = STENCIL_RHS
     x = torch.randn((k, op.n), dtype=torch.complex64, device="cuda:0")
     for _ in range(3):
         op.apply(x, mode="RAM", precision="c64")
@@ -143,7 +148,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(scene, args, plain_iters)
-    traffic = measured_traffic(args)
+    zb, ty = SLAB_SHAPES[tuning["shapes"][0]]
+    traffic = measured_traffic(args, zb, ty)
 
     out = {
         "metric": "RHS solves/s at 128^3 to 1e-6 rel. residual (mixed vs FP64); stencil HBM GB/s",
@@ -152,7 +158,8 @@ def main():
         "vs_baseline": None, "dtype": "c64 (FP64 refinement)", "data": "synthetic",
         "config": {"workload": f"heterogeneous {args.n}^3 scene, G columns in chunks of {args.chunk}, "
                                f"{args.precond}-preconditioned GMRES({args.restart}) c64 + FP64 refinement to {args.rtol:g}",
-                   "grid": [args.n]*3, "rhs_per_step": args.chunk, "sources": m},
+                   "grid": [args.n]*3, "rhs_per_step": args.chunk, "sources": m,
+                   "autotune": tuning},
         "plain_fp64": {"value": plain_rate, "unit": "solves/s", "rhs": args.plain_rhs,
                        "iterations": plain_iters},
         "speedup_mixed_vs_plain": mixed_rate/plain_rate if plain_rate > 0 else None,
@@ -160,37 +167,46 @@ def main():
         "refinements_median": float(np.median(refines)) if refines else None,
         "worst_scaled_residual": worst,
         "setup_s": setup_s,
-        "roofline": {"kernel": "hz::k_slab<1, 0> (complex64 R A M apply)", "bound": "hbm",
-                     "achieved": achieved, "peak": 8000.0, "unit": "GB/s", "frac": achieved/8000.0,
-                     "traffic": traffic, "alg_bytes_per_launch": alg_bytes, "launch_us": stencil_s*1e6},
+        "roofline": {"kernel": f"hz::k_slab<1, 0, {zb}, {ty}> (complex64 R A M apply, {k} columns)",
+                     "bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                     "frac": achieved/8000.0, "traffic": traffic, "alg_bytes_per_launch": alg_bytes,
+                     "launch_us": stencil_s*1e6},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "stencil_128_k16_traffic.md")
+STENCIL_RHS = 16  # columns per stencil launch in the roofline measurement
+# (z-planes per thread, tile rows) of the slab kernel's tile shapes, by index.
+SLAB_SHAPES = [(4, 4), (2, 4), (4, 8), (2, 8), (4, 2)]
 
 
-def measured_traffic(args):
+def measured_traffic(args, zb, ty):
     """HBM bytes per stencil launch from the FETCH_SIZE/WRITE_SIZE counter runs
-    of the same kernel and workload (128^3, 16 columns), scaled by the copy
-    calibration recorded beside them; None for other workloads."""
-    if args.n != 128 or args.chunk != 16 or not os.path.exists(TRAFFIC_PROFILE):
+    of the same kernel, tile shape and workload (128^3, 16 columns), scaled by
+    the copy calibration recorded beside them; None for other workloads."""
+    if args.n != 128 or not os.path.exists(TRAFFIC_PROFILE):
         return None
+    key = f"traffic_bytes_per_launch[zb={zb},ty={ty}]:"
     with open(TRAFFIC_PROFILE) as f:
         for line in f:
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key):
                 return int(line.split(":")[1])
     return None
 
 
-def stencil_only(op, args):
+def stencil_only(op, args, m):
     """Stencil launches (k complex64 RHS, R A M mode) and a copy of known size,
     so that hardware byte counters can be scaled.  The copy moves 2 GiB each
     way from a source evicted from the 256 MB last-level cache by a 1 GiB
     fill in between."""
     import torch
-    k = args.chunk
+    k = STENCIL_RHS
+    if args.precond == "multigrid":
+        op.set_multigrid(shift=args.mg_shift, pre=args.mg_pre, post=args.mg_post, weight=args.mg_weight)
+    print(json.dumps({"autotune": op.autotune(m, max_chunk=16, restart=args.restart,
+                                              preconditioner=args.precond)}), flush=True)
     x = torch.randn((k, op.n), dtype=torch.complex64, device="cuda:0")
     for _ in range(args.stencil_reps):
         op.apply(x, mode="RAM", precision="c64")

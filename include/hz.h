@@ -175,6 +175,29 @@ hz_status hz_sample(hz_operator *op, const int64_t *rows, const double *weights,
 hz_status hz_set_multigrid(hz_operator *op, int32_t nlevels, const hz_level_desc *levels, double shift,
                            int32_t pre, int32_t post, int32_t coarse_sweeps, double weight);
 
+/* Autotuner.  No counterpart in the reference (its CuPy GMRES has no
+ * tunables).  Times every slab tile shape of the complex64 stencil on this
+ * operator's grid for the fine apply and, with `preconditioner` =
+ * multigrid, for each level's smoother, residual and first-sweep residual,
+ * and keeps the fastest for each kernel.  Then picks the right-hand-side
+ * chunk width of the mixed form: the candidate in {4, 8, 16, 32, 64} (at
+ * most max_chunk and nrhs, workspace within memory_budget bytes, or half
+ * of the free device memory when memory_budget <= 0) with the least time
+ * per column of one Arnoldi step at j = restart / 2, preconditioner
+ * included; a wider chunk must be 2 % faster per column to win.  Writes the
+ * width to *chunk_out and, when shapes_out is not null, the shape indices
+ * (apply, then smoother / residual / first-sweep residual of each level) to
+ * shapes_out[0 .. shapes_len).  The workspace used for timing is released. */
+hz_status hz_autotune(hz_operator *op, int32_t nrhs, int32_t max_chunk, int32_t restart,
+                      int32_t preconditioner, int64_t memory_budget, int32_t *chunk_out,
+                      int32_t *shapes_out, int32_t shapes_len, void *stream);
+
+/* Pin the slab tile shapes (the layout hz_autotune reports: the apply, then
+ * smoother / residual / first-sweep residual per multigrid level; len must
+ * be 1 + 3 * levels).  Every shape computes the same arithmetic per node;
+ * only the speed differs.  No counterpart in the reference. */
+hz_status hz_set_tile_shapes(hz_operator *op, const int32_t *shapes, int32_t len);
+
 /* Device bytes held by the operator (coefficients plus cached workspace). */
 int64_t hz_device_bytes(const hz_operator *op);
 

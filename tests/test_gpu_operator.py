@@ -230,3 +230,58 @@ def test_multigrid_needs_fewer_iterations_than_jacobi():
     for r in range(2):
         assert sj[r]["converged"] and sm[r]["converged"]
         assert sm[r]["iterations"]*3 < sj[r]["iterations"], (sm[r], sj[r])
+
+
+def test_tile_shapes_are_bitwise_identical_and_autotune_picks_one():
+    """Every slab tile shape evaluates the same arithmetic per node: applies
+    and multigrid-preconditioned solves agree bit for bit across shapes.  The
+    autotuner returns valid shapes and a chunk width within its bounds."""
+    from ultrasound_solver_amd.operator import HelmholtzOperator
+    n = 37  # odd sizes leave partial tiles in x, y and z
+    rho, c, bnd, pml = _scene(n, seed=7, soft=True)
+    bnd["+y"] = {"type": "impedance", "impedance": [823.2, 205.8]}
+    op = HelmholtzOperator(rho, c, 1.07e-3, 2*np.pi*40e3, bnd, pml, device="cuda:0")
+    op.set_multigrid()
+    nshape = 1 + 3*len(op.hierarchy.levels)
+    x = torch.as_tensor(_rand(op.n, 5, 21).T.copy(), device="cuda:0").to(torch.complex64)
+    b = _point_rhs(op.n, 3, 17)
+    ref_apply, ref_x, ref_it = None, None, None
+    for shape in range(5):
+        op.set_tile_shapes([shape]*nshape)
+        ys = [op.apply(x, mode=m, precision="c64") for m in ("A", "RAM", "AH", "RAMH")]
+        sol, st = op.gmres(b, rtol=1e-8, precision="c64", chunk=3, maxiter=4000, preconditioner="multigrid")
+        its = [s["iterations"] for s in st]
+        if ref_apply is None:
+            ref_apply, ref_x, ref_it = ys, sol, its
+            continue
+        for y, y0 in zip(ys, ref_apply):
+            assert torch.equal(y, y0), f"shape {shape}: apply differs"
+        assert its == ref_it, (shape, its, ref_it)
+        assert torch.equal(sol, ref_x), f"shape {shape}: solution differs"
+    tune = op.autotune(64, max_chunk=32, restart=30, preconditioner="multigrid")
+    assert tune["chunk"] in (4, 8, 16, 32)
+    assert len(tune["shapes"]) == nshape and all(0 <= v < 5 for v in tune["shapes"])
+    sol, st = op.gmres(b, rtol=1e-8, precision="c64", chunk=3, maxiter=4000, preconditioner="multigrid")
+    assert torch.equal(sol, ref_x)
+
+
+@pytest.mark.parametrize("precision,pre", [("c128", "jacobi"), ("c64", "multigrid")])
+def test_graph_replay_matches_eager(precision, pre):
+    """A captured cycle body replayed once per restart cycle (eager cycle-top
+    work and host synchronisation between replays) gives bit-identical
+    iterates and statistics to the eager launches; a second solve reuses the
+    cached graph."""
+    from ultrasound_solver_amd.operator import HelmholtzOperator
+    n = 24
+    rho, c, bnd, pml = _scene(n, seed=9, soft=True)
+    op = HelmholtzOperator(rho, c, 1.07e-3, 2*np.pi*40e3, bnd, pml, device="cuda:0")
+    if pre == "multigrid":
+        op.set_multigrid()
+    b = _point_rhs(op.n, 3, 19)
+    kw = dict(restart=10, rtol=1e-8, precision=precision, chunk=3, maxiter=20000, preconditioner=pre)
+    x0, s0 = op.gmres(b, **kw)
+    assert all(s["cycles"] > 2 for s in s0), s0  # several replays
+    for _ in range(2):
+        x1, s1 = op.gmres(b, use_graph=True, **kw)
+        assert torch.equal(x1, x0)
+        assert s1 == s0

@@ -8,10 +8,13 @@ properties against the CPU oracle.
   (relative L2 1e-7 for the FP64 basis).
 * Adjoint phase gradients: central differences (delta = 1e-4 rad) and the
   plain FP64 adjoint gradient agree with the mixed-form gradient to 1e-4
-  (scaled error of the reference's gradient check). The solves run to a
-  scaled residual of 1e-10: a central difference over 2e-4 rad divides the
-  loss error by 2e-4, so a 1e-8 solve leaves ~1e-3 of noise in it.
+  (scaled error of the reference's gradient check). The mixed solves run
+  to a scaled residual of 1e-10: a central difference over 2e-4 rad divides
+  the loss error by 2e-4, so a 1e-8 solve leaves ~1e-3 of noise in it.  The
+  FP64 reference gradient needs no difference quotient and runs to 1e-8.
 """
+import time
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -103,7 +106,7 @@ def test_adjoint_gradient_160_fd_and_fp64_reference():
     mask_t = torch.as_tensor(mask, device="cuda:0")
     amps = np.ones(sc.num_sources)
     eng = hza.PhaseEngine(op, amps, precision="c64", preconditioner="multigrid", rtol=1e-10, maxiter=60000, chunk=8)
-    eng64 = hza.PhaseEngine(op, amps, precision="c128", preconditioner="jacobi", rtol=1e-10, maxiter=400000, chunk=1)
+    eng64 = hza.PhaseEngine(op, amps, precision="c128", preconditioner="jacobi", rtol=1e-8, maxiter=400000, chunk=1)
     idx = [tuple(tgt)]
 
     def loss_of(u):
@@ -111,10 +114,12 @@ def test_adjoint_gradient_160_fd_and_fp64_reference():
 
     phs = rng.uniform(0, 2*np.pi, (4, sc.num_sources))
     ph = torch.tensor(phs, device="cuda:0", requires_grad=True)
+    t0 = time.perf_counter()
     u = hza.phase_field(ph, eng)
     total = sum(loss_of(u[r]) for r in range(4))
     total.backward()
     grad = ph.grad.cpu().numpy()
+    print(f"160^3: 4 forward + adjoint solves (c64 multigrid, 1e-10) in {time.perf_counter() - t0:.1f}s", flush=True)
     # FD on 8 random emitters (seed 4) of the first phase vector, batched.
     pick = np.random.default_rng(4).choice(sc.num_sources, 8, replace=False)
     batch = []
@@ -126,6 +131,7 @@ def test_adjoint_gradient_160_fd_and_fp64_reference():
     with torch.no_grad():
         uf = hza.phase_field(torch.tensor(np.array(batch), device="cuda:0"), eng)
         vals = [float(loss_of(uf[q])) for q in range(len(batch))]
+    print(f"160^3: 16 finite-difference solves done at {time.perf_counter() - t0:.1f}s", flush=True)
     for q, e in enumerate(pick):
         fd = (vals[2*q] - vals[2*q + 1])/2e-4
         print(f"emitter {e}: adjoint {grad[0, e]:.10g} central difference {fd:.10g}", flush=True)
@@ -134,6 +140,7 @@ def test_adjoint_gradient_160_fd_and_fp64_reference():
     p64 = torch.tensor(phs[:1], device="cuda:0", requires_grad=True)
     loss_of(hza.phase_field(p64, eng64)[0]).backward()
     g64 = p64.grad.cpu().numpy()[0]
+    print(f"160^3: FP64 Jacobi forward + adjoint done at {time.perf_counter() - t0:.1f}s", flush=True)
     assert np.max(np.abs(grad[0] - g64))/np.max(np.abs(g64)) < 1e-4
 
 

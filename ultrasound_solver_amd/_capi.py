@@ -15,7 +15,8 @@ C128, C64 = 0, 1
 MODE_A, MODE_RAM, MODE_AH, MODE_RAMH = 0, 1, 2, 3
 
 EXPORTS = ("hz_create", "hz_destroy", "hz_apply", "hz_gmres", "hz_set_sources", "hz_build_rhs",
-           "hz_gather_sources", "hz_sample", "hz_device_bytes", "hz_last_error", "hz_set_multigrid")
+           "hz_gather_sources", "hz_sample", "hz_device_bytes", "hz_last_error", "hz_set_multigrid",
+           "hz_autotune", "hz_set_tile_shapes")
 PRE_JACOBI, PRE_MULTIGRID = 0, 1
 
 
@@ -81,6 +82,9 @@ def lib():
                                      c_int64, c_int32, c_void_p]
         handle.hz_set_multigrid.argtypes = [c_void_p, c_int32, POINTER(LevelDesc), c_double, c_int32, c_int32,
                                             c_int32, c_double]
+        handle.hz_autotune.argtypes = [c_void_p, c_int32, c_int32, c_int32, c_int32, c_int64, POINTER(c_int32),
+                                       POINTER(c_int32), c_int32, c_void_p]
+        handle.hz_set_tile_shapes.argtypes = [c_void_p, POINTER(c_int32), c_int32]
         handle.hz_device_bytes.argtypes = [c_void_p]
         handle.hz_device_bytes.restype = c_int64
         handle.hz_last_error.restype = c_char_p

@@ -92,6 +92,9 @@ struct MgLevel {
     DevBuf rho, kap, a[3], q[3], face[6], dinv;
     OpView<float> v{};
     TileMap tm{};
+    // Slab tile shapes of this level's smoother, residual and first-sweep
+    // residual (the last holds more live registers: two planes per thread).
+    int shape[3] = {0, 0, 1};
     DevBuf bb, xx, tt;  // k-slot work vectors: right-hand side, iterate, scratch
 };
 
@@ -117,6 +120,7 @@ struct hz_operator {
     OpView<double> v64{};
     OpView<float> v32{};
     TileMap tm{};
+    int slab_shape = 0;  // slab tile shape of the complex64 apply
     // Source columns, grouped by node (scatter) and by column (gather).
     int ncols = 0;
     int64_t nrows_src = 0;
@@ -126,6 +130,20 @@ struct hz_operator {
     DevBuf V, X, Bh, dpart, npart, cpart, red, Hm, gc, gs, g, scales, coef, resid, tol, steps, live, member;
     DevBuf rhs32;
     std::unique_ptr<Multigrid> mg;
+    // Captured GMRES cycle bodies, keyed by everything their launches bake in.
+    struct Graph {
+        std::vector<uintptr_t> sig;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<Graph> graphs;
+    hipStream_t capture = nullptr;  // non-default stream used only for capture
+    hz_operator() = default;
+    hz_operator(const hz_operator &) = delete;
+    hz_operator &operator=(const hz_operator &) = delete;
+    ~hz_operator() {
+        for (Graph &g : graphs) (void)hipGraphExecDestroy(g.exec);
+        if (capture) (void)hipStreamDestroy(capture);
+    }
 };
 
 namespace {
@@ -139,6 +157,38 @@ const OpView<float> &view<float>(const hz_operator *op) { return op->v32; }
 
 int64_t vec_blocks(int64_t n) { return blocks_for(n, 256 * 16, 1024); }
 
+// Grid of the per-node multigrid transfers: 64 x 4 (x, y) tiles, one z-plane
+// per block row, one right-hand side per block layer.
+dim3 node_grid(int nx, int ny, int nz, int k) {
+    return dim3((unsigned)(((nx + 63) / 64) * ((ny + 3) / 4)), (unsigned)nz, (unsigned)k);
+}
+
+// One slab-stencil launch with tile shape `shape` (index into SLAB_SHAPE).
+template <int MODE, int EPI>
+void launch_slab(int shape, const OpView<float> &v, const cx<float> *x, int64_t ldx, cx<float> *y, int64_t ldy,
+                 int k, const int *live, const double *scale, const cx<float> *b, const cx<float> *dinv, float w,
+                 const float *bdiv, hipStream_t s) {
+    const TileMap tm = slab_map(v.nx, v.ny, v.nz, shape);
+    const dim3 grid(tm.nblocks), block(64 * SLAB_SHAPE[shape].ty);
+#define HZ_SLAB_CASE(ID, ZB, TY)                                                                                   \
+    case ID:                                                                                                       \
+        static_assert(SLAB_SHAPE[ID].zb == ZB && SLAB_SHAPE[ID].ty == TY, "slab shape table");                    \
+        hipLaunchKernelGGL((k_slab<MODE, EPI, ZB, TY>), grid, block, 0, s, v, tm, x, ldx, y, ldy, k, live, scale, b, \
+                           dinv, w, bdiv);                                                                         \
+        break;
+    switch (shape) {
+        HZ_SLAB_CASE(0, 4, 4)
+        HZ_SLAB_CASE(1, 2, 4)
+        HZ_SLAB_CASE(2, 4, 8)
+        HZ_SLAB_CASE(3, 2, 8)
+        HZ_SLAB_CASE(4, 4, 2)
+        default:
+            throw BadArgument("unknown slab tile shape");
+    }
+#undef HZ_SLAB_CASE
+    HZ_LAUNCHED();
+}
+
 template <typename T, int MODE>
 void launch_apply(const hz_operator *op, const cx<T> *x, int64_t ldx, cx<T> *y, int64_t ldy, int nrhs,
                   const int *live, const double *scale, hipStream_t s) {
@@ -150,9 +200,8 @@ void launch_apply(const hz_operator *op, const cx<T> *x, int64_t ldx, cx<T> *y, 
         const double *sc = scale ? scale + r0 : nullptr;
         if constexpr (std::is_same<T, float>::value) {
             // complex64: the 2.5-D slab kernel (register z-window, DPP x-neighbours)
-            hipLaunchKernelGGL((k_slab<MODE, SLAB_APPLY>), dim3(op->tm.nblocks), dim3(256), 0, s, op->v32, op->tm, xs,
-                               ldx, ys, ldy, kk, lv, sc, (const cx<float> *)nullptr, (const cx<float> *)nullptr, 0.0f,
-                               (const float *)nullptr);
+            launch_slab<MODE, SLAB_APPLY>(op->slab_shape, op->v32, xs, ldx, ys, ldy, kk, lv, sc, nullptr, nullptr,
+                                          0.0f, nullptr, s);
         } else {
             hipLaunchKernelGGL((k_apply<T, MODE>), dim3(op->tm.nblocks), dim3(256), 0, s, view<T>(op), op->tm, xs, ldx,
                                ys, ldy, kk, lv, sc);
@@ -217,9 +266,8 @@ void mg_reserve(Multigrid &mg, int k) {
 template <int MODE>
 void level_smooth(MgLevel &L, const cx<float> *x, const cx<float> *b, cx<float> *out, float w, int k,
                   const int *live, const float *bdiv, hipStream_t s) {
-    hipLaunchKernelGGL((k_slab<MODE, SLAB_SMOOTH>), dim3(L.tm.nblocks), dim3(256), 0, s, L.v, L.tm, x, L.n, out, L.n,
-                       k, live, (const double *)nullptr, b, (const cx<float> *)L.dinv.as<cx<float>>(), w, bdiv);
-    HZ_LAUNCHED();
+    launch_slab<MODE, SLAB_SMOOTH>(L.shape[0], L.v, x, L.n, out, L.n, k, live, nullptr, b, L.dinv.as<cx<float>>(), w,
+                                   bdiv, s);
 }
 
 // x = V-cycle(b / bdiv) on level l; x and b hold k slots of length n_l.  The
@@ -237,13 +285,40 @@ void vcycle(Multigrid &mg, int l, const cx<float> *b, cx<float> *x, int k, const
         HZ_LAUNCHED();
         return;
     }
+    const cx<float> *dinv = L.dinv.as<cx<float>>();
+    if (!coarsest && mg.pre == 1) {
+        // One pre-sweep: its iterate f b (f = w dinv / bdiv) is never stored.
+        // The residual b/bdiv - A (f b) comes from b in one stencil pass and
+        // the prolongation rebuilds f b + P e from b.
+        MgLevel &C = *mg.levels[l + 1];
+        cx<float> *r = L.tt.as<cx<float>>();
+        launch_slab<MODE, SLAB_RESID0>(L.shape[2], L.v, b, L.n, r, L.n, k, live, nullptr, nullptr, dinv, mg.weight, bdiv,
+                                       s);
+        hipLaunchKernelGGL((k_restrict<float>), node_grid(C.nx, C.ny, C.nz, k), dim3(256), 0, s, r, L.nx, L.ny, L.nz,
+                           C.bb.as<cx<float>>(), C.nx, C.ny, C.nz, live, L.v.soft);
+        HZ_LAUNCHED();
+        vcycle<MODE>(mg, l + 1, C.bb.as<cx<float>>(), C.xx.as<cx<float>>(), k, live, s);
+        // Post-sweeps ping-pong between x and the level scratch; start where
+        // the last one lands in x.
+        cx<float> *cur = (mg.post % 2 == 0) ? x : L.tt.as<cx<float>>();
+        cx<float> *alt = (cur == x) ? L.tt.as<cx<float>>() : x;
+        hipLaunchKernelGGL((k_prolong<float>), node_grid(L.nx, L.ny, L.nz, k), dim3(256), 0, s,
+                           (const cx<float> *)C.xx.as<cx<float>>(), C.nx, C.ny, C.nz, cur, L.nx, L.ny, L.nz, live,
+                           L.v.soft, b, dinv, mg.weight, bdiv, (int)(MODE == MODE_AH));
+        HZ_LAUNCHED();
+        for (int it = 0; it < mg.post; ++it) {
+            level_smooth<MODE>(L, cur, b, alt, mg.weight, k, live, bdiv, s);
+            std::swap(cur, alt);
+        }
+        return;
+    }
     // Jacobi sweeps ping-pong between x and the level scratch; start in the
     // buffer that makes the last sweep land in x (no trailing copy).
     const int swaps = coarsest ? mg.coarse_sweeps - 1 : (mg.pre - 1) + mg.post;
     cx<float> *cur = (swaps % 2 == 0) ? x : L.tt.as<cx<float>>();
     cx<float> *alt = (cur == x) ? L.tt.as<cx<float>>() : x;
     hipLaunchKernelGGL((k_level_jacobi0<float, MODE>), dim3(blocks_for(n, 256 * 8, 2048), k), dim3(256), 0, s, b,
-                       cur, (const cx<float> *)L.dinv.as<cx<float>>(), mg.weight, n, live, bdiv);
+                       cur, dinv, mg.weight, n, live, bdiv);
     HZ_LAUNCHED();
     const int first = coarsest ? mg.coarse_sweeps : mg.pre;
     for (int it = 1; it < first; ++it) {
@@ -252,16 +327,14 @@ void vcycle(Multigrid &mg, int l, const cx<float> *b, cx<float> *x, int k, const
     }
     if (!coarsest) {
         MgLevel &C = *mg.levels[l + 1];
-        hipLaunchKernelGGL((k_slab<MODE, SLAB_RESID>), dim3(L.tm.nblocks), dim3(256), 0, s, L.v, L.tm, cur, L.n, alt,
-                           L.n, k, live, (const double *)nullptr, b, (const cx<float> *)nullptr, 0.0f, bdiv);
-        HZ_LAUNCHED();
-        hipLaunchKernelGGL((k_restrict<float>), dim3((C.n + 255) / 256), dim3(256), 0, s, alt, L.nx, L.ny, L.nz,
-                           C.bb.as<cx<float>>(), C.nx, C.ny, C.nz, k, live, L.v.soft);
+        launch_slab<MODE, SLAB_RESID>(L.shape[1], L.v, cur, L.n, alt, L.n, k, live, nullptr, b, nullptr, 0.0f, bdiv, s);
+        hipLaunchKernelGGL((k_restrict<float>), node_grid(C.nx, C.ny, C.nz, k), dim3(256), 0, s, alt, L.nx, L.ny, L.nz,
+                           C.bb.as<cx<float>>(), C.nx, C.ny, C.nz, live, L.v.soft);
         HZ_LAUNCHED();
         vcycle<MODE>(mg, l + 1, C.bb.as<cx<float>>(), C.xx.as<cx<float>>(), k, live, s);
-        hipLaunchKernelGGL((k_prolong_add<float>), dim3((n + 255) / 256), dim3(256), 0, s,
-                           (const cx<float> *)C.xx.as<cx<float>>(), C.nx, C.ny, C.nz, cur, L.nx, L.ny, L.nz, k, live,
-                           L.v.soft);
+        hipLaunchKernelGGL((k_prolong<float>), node_grid(L.nx, L.ny, L.nz, k), dim3(256), 0, s,
+                           (const cx<float> *)C.xx.as<cx<float>>(), C.nx, C.ny, C.nz, cur, L.nx, L.ny, L.nz, live,
+                           L.v.soft, (const cx<float> *)nullptr, dinv, mg.weight, bdiv, 0);
         HZ_LAUNCHED();
         for (int it = 0; it < mg.post; ++it) {
             level_smooth<MODE>(L, cur, b, alt, mg.weight, k, live, bdiv, s);
@@ -278,6 +351,53 @@ void precondition(hz_operator *op, bool adjoint, const cx<float> *b, cx<float> *
         vcycle<MODE_AH>(*op->mg, 0, b, x, k, live, s, op->v32.rs_a);
     else
         vcycle<MODE_A>(*op->mg, 0, b, x, k, live, s, op->v32.rs_f);
+}
+
+// Everything a captured cycle bakes into its launches: precision, mode, chunk
+// width, restart, preconditioner, tile shapes and every buffer address.
+std::vector<uintptr_t> graph_signature(const hz_operator *op, size_t elem, int mode, int k, int restart,
+                                       bool use_mg) {
+    std::vector<uintptr_t> sig = {elem, (uintptr_t)mode, (uintptr_t)k, (uintptr_t)restart, (uintptr_t)use_mg,
+                                  (uintptr_t)op->slab_shape};
+    for (const DevBuf *b : {&op->V, &op->X, &op->dpart, &op->npart, &op->Hm, &op->gc, &op->gs, &op->g, &op->scales,
+                            &op->coef, &op->resid, &op->tol, &op->steps, &op->live, &op->member})
+        sig.push_back((uintptr_t)b->p);
+    if (use_mg)
+        for (const auto &L : op->mg->levels) {
+            for (const DevBuf *b : {&L->bb, &L->xx, &L->tt}) sig.push_back((uintptr_t)b->p);
+            for (int e = 0; e < 3; ++e) sig.push_back((uintptr_t)L->shape[e]);
+        }
+    return sig;
+}
+
+// Executable graph of one cycle body, captured on first use for `sig`.  A
+// capture records launches only (no memset nodes); the graph is replayed on
+// the caller's stream.
+template <typename F>
+hipGraphExec_t cycle_graph(hz_operator *op, std::vector<uintptr_t> sig, F &&body) {
+    for (auto &g : op->graphs)
+        if (g.sig == sig) return g.exec;
+    if (!op->capture) HZ_HIP(hipStreamCreateWithFlags(&op->capture, hipStreamNonBlocking));
+    HZ_HIP(hipStreamBeginCapture(op->capture, hipStreamCaptureModeThreadLocal));
+    hipGraph_t graph = nullptr;
+    try {
+        body(op->capture);
+    } catch (...) {
+        (void)hipStreamEndCapture(op->capture, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    HZ_HIP(hipStreamEndCapture(op->capture, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HZ_HIP(err);
+    if (op->graphs.size() >= 8) {  // bounded cache: drop the oldest
+        (void)hipGraphExecDestroy(op->graphs.front().exec);
+        op->graphs.erase(op->graphs.begin());
+    }
+    op->graphs.push_back({std::move(sig), exec});
+    return exec;
 }
 
 // Per-RHS bookkeeping of one chunk.
@@ -300,7 +420,8 @@ enum Precond { PRE_JACOBI = 0, PRE_MULTIGRID = 1 };
 
 template <typename T>
 void gmres_cycles(hz_operator *op, int mode_ram, int k, int restart, int maxiter, bool ref_semantics,
-                  const std::vector<double> &atol, ChunkState &cs, hipStream_t s, int precond = PRE_JACOBI) {
+                  const std::vector<double> &atol, ChunkState &cs, hipStream_t s, int precond = PRE_JACOBI,
+                  bool use_graph = false) {
     const bool use_mg = precond == PRE_MULTIGRID;
     const bool adjoint = (mode_ram == MODE_RAMH);
     const int mode_op = use_mg ? (adjoint ? MODE_RAH : MODE_RA) : mode_ram;
@@ -331,7 +452,8 @@ void gmres_cycles(hz_operator *op, int mode_ram, int k, int restart, int maxiter
         for (int r = 0; r < k; ++r) early[r] = atol[r];
     HZ_HIP(hipMemcpyAsync(st.tol, early.data(), k * sizeof(double), hipMemcpyHostToDevice, s));
 
-    std::vector<int> live_h(k);
+    std::vector<int> live_h(k), early_exit(k, 0);
+    std::vector<double> prev_top(k, 0.0);
     while (true) {
         for (int r = 0; r < k; ++r) live_h[r] = cs.active[r];
         HZ_HIP(hipMemcpyAsync(st.live, live_h.data(), k * sizeof(int), hipMemcpyHostToDevice, s));
@@ -361,43 +483,63 @@ void gmres_cycles(hz_operator *op, int mode_ram, int k, int restart, int maxiter
                 cs.active[r] = 0;
                 continue;
             }
+            // Working-precision floor: the last cycle stopped early on its
+            // Givens estimate, yet the true residual did not even halve.  More
+            // cycles cannot help; the caller's FP64 refinement takes over.
+            if (early_exit[r] && resid[r] > 0.5 * prev_top[r]) {
+                cs.active[r] = 0;
+                continue;
+            }
+            prev_top[r] = resid[r];
             any = true;
         }
         if (!any) break;
         for (int r = 0; r < k; ++r) live_h[r] = cs.active[r];
         HZ_HIP(hipMemcpyAsync(st.live, live_h.data(), k * sizeof(int), hipMemcpyHostToDevice, s));
         HZ_HIP(hipMemcpyAsync(member, live_h.data(), k * sizeof(int), hipMemcpyHostToDevice, s));
-        for (int j = 0; j < restart; ++j) {
-            const cx<T> *vj = V + (int64_t)j * k * n;
-            cx<T> *w = V + (int64_t)(j + 1) * k * n;
-            const cx<T> *in = vj;
-            if constexpr (std::is_same<T, float>::value) {
-                if (use_mg) {
-                    cx<float> *z = op->mg->levels[0]->xx.as<cx<float>>();
-                    precondition(op, adjoint, vj, z, k, st.live, s);
-                    in = z;
+        // The cycle body: `restart` Arnoldi steps, the triangular solve and
+        // the iterate update.  Slots that leave early are skipped inside the
+        // kernels (device live flags), so the launch sequence is the same
+        // every cycle and can be replayed from a captured graph.
+        auto body = [&](hipStream_t cs_stream) {
+            for (int j = 0; j < restart; ++j) {
+                const cx<T> *vj = V + (int64_t)j * k * n;
+                cx<T> *w = V + (int64_t)(j + 1) * k * n;
+                const cx<T> *in = vj;
+                if constexpr (std::is_same<T, float>::value) {
+                    if (use_mg) {
+                        cx<float> *z = op->mg->levels[0]->xx.as<cx<float>>();
+                        precondition(op, adjoint, vj, z, k, st.live, cs_stream);
+                        in = z;
+                    }
                 }
+                apply_mode<T>(op, mode_op, in, n, w, n, k, st.live, st.scales + (int64_t)j * k, cs_stream);
+                const int groups = (j + 1 + DOT_GROUP - 1) / DOT_GROUP;
+                hipLaunchKernelGGL((k_dots<T>), dim3(nblk, k, groups), dim3(256), 0, cs_stream, V, k, n, j + 1, w,
+                                   st.live, op->dpart.as<double2>(), maxv, chunk);
+                HZ_LAUNCHED();
+                hipLaunchKernelGGL(k_reduce_dots, dim3(j + 1, k), dim3(256), 0, cs_stream, op->dpart.as<double2>(),
+                                   (int)nblk, maxv, j, st.scales, st.live, st.H, st.coef, restart);
+                HZ_LAUNCHED();
+                hipLaunchKernelGGL((k_update<T>), dim3(nblk, k), dim3(256), 0, cs_stream, V, k, n, j + 1,
+                                   (const int *)nullptr, w, n, st.coef, maxv, st.live, op->npart.as<double>(), chunk);
+                HZ_LAUNCHED();
+                hipLaunchKernelGGL(k_arnoldi_finish, dim3(k), dim3(256), 0, cs_stream, st, op->npart.as<double>(),
+                                   (int)nblk, j, maxv, restart);
+                HZ_LAUNCHED();
             }
-            apply_mode<T>(op, mode_op, in, n, w, n, k, st.live, st.scales + (int64_t)j * k, s);
-            const int groups = (j + 1 + DOT_GROUP - 1) / DOT_GROUP;
-            hipLaunchKernelGGL((k_dots<T>), dim3(nblk, k, groups), dim3(256), 0, s, V, k, n, j + 1, w, st.live,
-                               op->dpart.as<double2>(), maxv, chunk);
+            hipLaunchKernelGGL(k_trisolve, dim3(k), dim3(64), 0, cs_stream, st, maxv, restart, member);
             HZ_LAUNCHED();
-            hipLaunchKernelGGL(k_reduce_dots, dim3(j + 1, k), dim3(256), 0, s, op->dpart.as<double2>(), (int)nblk,
-                               maxv, j, st.scales, st.live, st.H, st.coef, restart);
+            hipLaunchKernelGGL((k_update<T>), dim3(nblk, k), dim3(256), 0, cs_stream, V, k, n, 0, st.steps, X, n,
+                               st.coef, maxv, member, (double *)nullptr, chunk);
             HZ_LAUNCHED();
-            hipLaunchKernelGGL((k_update<T>), dim3(nblk, k), dim3(256), 0, s, V, k, n, j + 1, (const int *)nullptr,
-                               w, n, st.coef, maxv, st.live, op->npart.as<double>(), chunk);
-            HZ_LAUNCHED();
-            hipLaunchKernelGGL(k_arnoldi_finish, dim3(k), dim3(256), 0, s, st, op->npart.as<double>(), (int)nblk,
-                               j, maxv, restart);
-            HZ_LAUNCHED();
+        };
+        if (use_graph) {
+            hipGraphExec_t exec = cycle_graph(op, graph_signature(op, sizeof(T), mode_op, k, restart, use_mg), body);
+            HZ_HIP(hipGraphLaunch(exec, s));
+        } else {
+            body(s);
         }
-        hipLaunchKernelGGL(k_trisolve, dim3(k), dim3(64), 0, s, st, maxv, restart, member);
-        HZ_LAUNCHED();
-        hipLaunchKernelGGL((k_update<T>), dim3(nblk, k), dim3(256), 0, s, V, k, n, 0, st.steps, X, n, st.coef, maxv,
-                           member, (double *)nullptr, chunk);
-        HZ_LAUNCHED();
         std::vector<int> steps_h(k);
         HZ_HIP(hipMemcpyAsync(steps_h.data(), st.steps, k * sizeof(int), hipMemcpyDeviceToHost, s));
         HZ_HIP(hipStreamSynchronize(s));
@@ -405,6 +547,7 @@ void gmres_cycles(hz_operator *op, int mode_ram, int k, int restart, int maxiter
             if (!cs.active[r]) continue;
             cs.iterations[r] += ref_semantics ? restart : steps_h[r];
             cs.cycles[r] += 1;
+            early_exit[r] = !ref_semantics && steps_h[r] < restart;
         }
     }
 }
@@ -509,7 +652,8 @@ void solve_plain(hz_operator *op, const hz_gmres_opts &o, const cx<double> *b, i
             cs.active[r] = 0;
         }
     }
-    gmres_cycles<double>(op, o.adjoint ? MODE_RAMH : MODE_RAM, k, o.restart, o.maxiter, true, atol, cs, s);
+    gmres_cycles<double>(op, o.adjoint ? MODE_RAMH : MODE_RAM, k, o.restart, o.maxiter, true, atol, cs, s,
+                         PRE_JACOBI, o.use_graph != 0);
     // x = M y for every column (zero right-hand sides give y = 0).
     hipLaunchKernelGGL((k_precond_out<double>), dim3(vec_blocks(n), k), dim3(256), 0, s,
                        (const cx<double> *)op->X.as<cx<double>>(), n, m, x, ldx, n, (const int *)nullptr, 0);
@@ -582,7 +726,7 @@ void solve_mixed(hz_operator *op, const hz_gmres_opts &o, const cx<double> *b, i
         const bool use_mg = o.preconditioner == PRE_MULTIGRID;
         if (use_mg) mg_reserve(*op->mg, k);
         gmres_cycles<float>(op, o.adjoint ? MODE_RAMH : MODE_RAM, k, o.restart, o.maxiter, false, atol, inner, s,
-                            o.preconditioner);
+                            o.preconditioner, o.use_graph != 0);
         for (int r = 0; r < k; ++r) {
             cs.iterations[r] = inner.iterations[r];
             cs.cycles[r] = inner.cycles[r];
@@ -619,6 +763,92 @@ void solve_mixed(hz_operator *op, const hz_gmres_opts &o, const cx<double> *b, i
             st.converged = 0;
         }
     }
+}
+
+// ---------------------------------------------------------------- autotuner
+
+// Mean duration (ms) of `reps` back-to-back launches after one warm-up.
+template <typename F>
+float time_launches(F &&launch, int reps, hipStream_t s) {
+    hipEvent_t e0, e1;
+    HZ_HIP(hipEventCreate(&e0));
+    HZ_HIP(hipEventCreate(&e1));
+    launch();
+    HZ_HIP(hipEventRecord(e0, s));
+    for (int i = 0; i < reps; ++i) launch();
+    HZ_HIP(hipEventRecord(e1, s));
+    HZ_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    HZ_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms / reps;
+}
+
+// Fastest slab shape for one kernel: `launch(shape)` runs it once.
+template <typename F>
+int pick_shape(F &&launch, hipStream_t s) {
+    int best = 0;
+    float best_ms = 0;
+    for (int sh = 0; sh < SLAB_SHAPES; ++sh) {
+        const float ms = time_launches([&] { launch(sh); }, 5, s);
+        if (verbose()) std::fprintf(stderr, "[hz] autotune shape %d (zb %d, ty %d): %.1f us\n", sh, SLAB_SHAPE[sh].zb,
+                                    SLAB_SHAPE[sh].ty, ms * 1e3);
+        if (sh == 0 || ms < best_ms) {
+            best = sh;
+            best_ms = ms;
+        }
+    }
+    return best;
+}
+
+// Device bytes a mixed-form chunk of k columns needs besides the caller's
+// FP64 x and b: the complex64 basis, iterate, right-hand sides and the
+// multigrid level vectors.
+size_t chunk_bytes(const hz_operator *op, int k, int restart) {
+    size_t nodes = (size_t)op->n * (restart + 1 + 3);
+    if (op->mg)
+        for (const auto &L : op->mg->levels) nodes += 3 * (size_t)L->n;
+    return nodes * k * sizeof(cx<float>);
+}
+
+void release_workspace(hz_operator *op) {
+    for (DevBuf *b : {&op->V, &op->X, &op->Bh, &op->rhs32}) b->release();
+    if (op->mg) {
+        for (auto &L : op->mg->levels) {
+            L->bb.release();
+            L->xx.release();
+            L->tt.release();
+        }
+        op->mg->kcap = 0;
+    }
+}
+
+// One Arnoldi step of the mixed form on k columns at basis index j (timing
+// only: preconditioner, stencil, Gram-Schmidt dots and update).
+void arnoldi_step_for_timing(hz_operator *op, int k, int j, int restart, bool use_mg, hipStream_t s) {
+    const int64_t n = op->n;
+    const int maxv = restart + 1;
+    const int64_t nblk = vec_blocks(n);
+    const int64_t chunk = (n + nblk - 1) / nblk;
+    cx<float> *V = op->V.as<cx<float>>();
+    const cx<float> *in = V + (int64_t)j * k * n;
+    cx<float> *w = V + (int64_t)(j + 1) * k * n;
+    const int *live = op->live.as<int>();
+    if (use_mg) {
+        cx<float> *z = op->mg->levels[0]->xx.as<cx<float>>();
+        precondition(op, false, in, z, k, live, s);
+        in = z;
+    }
+    apply_mode<float>(op, use_mg ? MODE_RA : MODE_RAM, in, n, w, n, k, live, op->scales.as<double>(), s);
+    const int groups = (j + 1 + DOT_GROUP - 1) / DOT_GROUP;
+    hipLaunchKernelGGL((k_dots<float>), dim3(nblk, k, groups), dim3(256), 0, s, (const cx<float> *)V, k, n, j + 1,
+                       (const cx<float> *)w, live, op->dpart.as<double2>(), maxv, chunk);
+    HZ_LAUNCHED();
+    hipLaunchKernelGGL((k_update<float>), dim3(nblk, k), dim3(256), 0, s, (const cx<float> *)V, k, n, j + 1,
+                       (const int *)nullptr, w, n, (const cx<double> *)op->coef.as<cx<double>>(), maxv, live,
+                       op->npart.as<double>(), chunk);
+    HZ_LAUNCHED();
 }
 
 template <typename F>
@@ -1018,6 +1248,138 @@ hz_status hz_sample(hz_operator *op, const int64_t *rows, const double *weights,
         hipLaunchKernelGGL(k_sample, dim3((npoints + 255) / 256, nrhs), dim3(256), 0, s, rows, weights, npoints,
                            static_cast<const cx<double> *>(u), ld, static_cast<cx<double> *>(out), ld_out);
         HZ_LAUNCHED();
+    });
+}
+
+hz_status hz_autotune(hz_operator *op, int32_t nrhs, int32_t max_chunk, int32_t restart, int32_t preconditioner,
+                      int64_t memory_budget, int32_t *chunk_out, int32_t *shapes_out, int32_t shapes_len,
+                      void *stream) {
+    return guarded([&] {
+        check_op(op);
+        if (nrhs < 1 || max_chunk < 1) throw BadArgument("need nrhs >= 1 and max_chunk >= 1");
+        if (restart < 1 || restart > 63) throw BadArgument("restart must be in [1, 63]");
+        const bool use_mg = preconditioner == PRE_MULTIGRID;
+        if (use_mg && !op->mg) throw BadArgument("multigrid preconditioner requested but no hierarchy installed");
+        if (!use_mg && preconditioner != PRE_JACOBI) throw BadArgument("unknown preconditioner");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        const int64_t n = op->n;
+        size_t free_b = 0, total_b = 0;
+        HZ_HIP(hipMemGetInfo(&free_b, &total_b));
+        const size_t budget = memory_budget > 0 ? (size_t)memory_budget : free_b / 2;
+        // Candidate chunk widths that fit the budget.
+        std::vector<int> cand;
+        for (int c : {4, 8, 16, 32, 64})
+            if (c <= max_chunk && c <= std::max(nrhs, 4) && chunk_bytes(op, c, restart) <= budget) cand.push_back(c);
+        if (cand.empty()) cand.push_back(std::max(1, std::min({max_chunk, nrhs, 4})));
+        const int kt = std::min(16, cand.back());  // columns for the shape timings
+        const int kmax = cand.back();
+        reserve_workspace(op, kmax, restart, sizeof(cx<float>));
+        if (use_mg) mg_reserve(*op->mg, kmax);
+        // Finite, non-zero data: the basis and iterate start as ones; unit scales.
+        {
+            std::vector<int> ones_i(kmax, 1);
+            std::vector<double> ones_d((size_t)(restart + 1) * kmax, 1.0);
+            std::vector<cx<double>> coef((size_t)kmax * (restart + 1), mk<double>(1e-3, 0));
+            HZ_HIP(hipMemcpyAsync(op->live.p, ones_i.data(), kmax * sizeof(int), hipMemcpyHostToDevice, s));
+            HZ_HIP(hipMemcpyAsync(op->scales.p, ones_d.data(), ones_d.size() * sizeof(double), hipMemcpyHostToDevice,
+                                  s));
+            HZ_HIP(hipMemcpyAsync(op->coef.p, coef.data(), coef.size() * sizeof(cx<double>), hipMemcpyHostToDevice,
+                                  s));
+            hipLaunchKernelGGL((k_fill<float>), dim3(vec_blocks(n), (unsigned)((restart + 1) * kmax)), dim3(256), 0, s,
+                               op->V.as<cx<float>>(), n, n, mk<float>(1, 0));
+            HZ_LAUNCHED();
+            if (use_mg)
+                for (auto &L : op->mg->levels)
+                    for (DevBuf *b : {&L->bb, &L->xx, &L->tt}) {
+                        hipLaunchKernelGGL((k_fill<float>), dim3(vec_blocks(L->n), kmax), dim3(256), 0, s,
+                                           b->as<cx<float>>(), L->n, L->n, mk<float>(1, 0));
+                        HZ_LAUNCHED();
+                    }
+        }
+        const int *live = op->live.as<int>();
+        cx<float> *x = op->V.as<cx<float>>();
+        cx<float> *y = x + (int64_t)kt * n;
+        // Fine-grid apply.
+        const int mode = use_mg ? MODE_RA : MODE_RAM;
+        op->slab_shape = pick_shape(
+            [&](int sh) {
+                if (mode == MODE_RA)
+                    launch_slab<MODE_RA, SLAB_APPLY>(sh, op->v32, x, n, y, n, kt, live, op->scales.as<double>(),
+                                                     nullptr, nullptr, 0.0f, nullptr, s);
+                else
+                    launch_slab<MODE_RAM, SLAB_APPLY>(sh, op->v32, x, n, y, n, kt, live, op->scales.as<double>(),
+                                                      nullptr, nullptr, 0.0f, nullptr, s);
+            },
+            s);
+        // Multigrid level kernels (forward operator; the adjoint uses the same shapes).
+        if (use_mg) {
+            Multigrid &mg = *op->mg;
+            for (size_t l = 0; l + 1 < mg.levels.size() || (l < mg.levels.size() && mg.levels[l]->n > COARSE_MAX_NODES);
+                 ++l) {
+                MgLevel &L = *mg.levels[l];
+                const cx<float> *lb = L.bb.as<cx<float>>();
+                cx<float> *lx = L.xx.as<cx<float>>(), *lt = L.tt.as<cx<float>>();
+                const cx<float> *dv = L.dinv.as<cx<float>>();
+                const float *bd = l == 0 ? op->v32.rs_f : nullptr;
+                L.shape[0] = pick_shape(
+                    [&](int sh) {
+                        launch_slab<MODE_A, SLAB_SMOOTH>(sh, L.v, lx, L.n, lt, L.n, kt, live, nullptr, lb, dv,
+                                                         mg.weight, bd, s);
+                    },
+                    s);
+                L.shape[1] = pick_shape(
+                    [&](int sh) {
+                        launch_slab<MODE_A, SLAB_RESID>(sh, L.v, lx, L.n, lt, L.n, kt, live, nullptr, lb, nullptr, 0.0f,
+                                                        bd, s);
+                    },
+                    s);
+                L.shape[2] = pick_shape(
+                    [&](int sh) {
+                        launch_slab<MODE_A, SLAB_RESID0>(sh, L.v, lb, L.n, lt, L.n, kt, live, nullptr, nullptr, dv,
+                                                         mg.weight, bd, s);
+                    },
+                    s);
+            }
+        }
+        // Chunk width: least time per column of one Arnoldi step at j = restart / 2.
+        int best = cand.front();
+        double best_per = 0;
+        for (size_t ci = 0; ci < cand.size(); ++ci) {
+            const int c = cand[ci];
+            const float ms = time_launches([&] { arnoldi_step_for_timing(op, c, restart / 2, restart, use_mg, s); },
+                                           3, s);
+            const double per = ms / c;
+            if (verbose()) std::fprintf(stderr, "[hz] autotune chunk %d: %.3f ms/step, %.1f us per column\n", c, ms,
+                                        per * 1e3);
+            // A wider chunk must win by more than 2 % (it costs memory).
+            if (ci == 0 || per < 0.98 * best_per) {
+                best = c;
+                best_per = per;
+            }
+        }
+        HZ_HIP(hipStreamSynchronize(s));
+        release_workspace(op);
+        if (chunk_out) *chunk_out = best;
+        if (shapes_out && shapes_len > 0) {
+            shapes_out[0] = op->slab_shape;
+            int at = 1;
+            if (op->mg)
+                for (const auto &L : op->mg->levels)
+                    for (int e = 0; e < 3 && at < shapes_len; ++e) shapes_out[at++] = L->shape[e];
+        }
+    });
+}
+
+hz_status hz_set_tile_shapes(hz_operator *op, const int32_t *shapes, int32_t len) {
+    return guarded([&] {
+        check_op(op);
+        const int levels = op->mg ? (int)op->mg->levels.size() : 0;
+        if (!shapes || len != 1 + 3 * levels) throw BadArgument("need 1 + 3 * (multigrid levels) shape indices");
+        for (int i = 0; i < len; ++i)
+            if (shapes[i] < 0 || shapes[i] >= SLAB_SHAPES) throw BadArgument("shape index out of range");
+        op->slab_shape = shapes[0];
+        for (int l = 0; l < levels; ++l)
+            for (int e = 0; e < 3; ++e) op->mg->levels[l]->shape[e] = shapes[1 + 3 * l + e];
     });
 }
 

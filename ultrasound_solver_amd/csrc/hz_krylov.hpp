@@ -281,6 +281,12 @@ __global__ __launch_bounds__(256) void k_zero(cx<T> *__restrict__ x, int64_t ldx
         x[r * ldx + p] = mk<T>(0, 0);
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill(cx<T> *__restrict__ x, int64_t ldx, int64_t n, cx<T> value) {
+    const int r = blockIdx.y;
+    for (int64_t p = blockIdx.x * 256 + threadIdx.x; p < n; p += (int64_t)gridDim.x * 256) x[r * ldx + p] = value;
+}
+
 // Sum (or max, where bit q of max_mask is set) of nblk partials for each of
 // `width` values per RHS: out[r*width + q] = reduce_b partial[(r*nblk + b)*width + q].
 // Grid (width, k).

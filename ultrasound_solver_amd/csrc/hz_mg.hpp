@@ -194,27 +194,30 @@ __global__ __launch_bounds__(256) void k_coarse_jacobi(OpView<T> op, const cx<T>
     }
 }
 
-// Coarse right-hand side bc = P^T rf.  One thread per coarse node.  Identity
-// rows (pressure-release faces) take no part in the coarse correction: their
-// fine residuals are not restricted and coarse identity rows get zero, so a
-// residual-scaled value never lands in a solution-scaled identity row.
+// Coarse right-hand side bc = P^T rf.  One thread per (coarse node, RHS):
+// 64 x 4 threads per (x, y) tile, grid (tiles, nzc, k).  Identity rows (pressure-release faces) take no
+// part in the coarse correction: their fine residuals are not restricted and
+// coarse identity rows get zero, so a residual-scaled value never lands in a
+// solution-scaled identity row.
 template <typename T>
 __global__ __launch_bounds__(256) void k_restrict(const cx<T> *__restrict__ rf, int nxf, int nyf, int nzf,
-                                                  cx<T> *__restrict__ bc, int nxc, int nyc, int nzc, int k,
+                                                  cx<T> *__restrict__ bc, int nxc, int nyc, int nzc,
                                                   const int *__restrict__ live, int soft) {
+    const int r = blockIdx.z;
+    if (live && !live[r]) return;
+    const int txs = (nxc + 63) >> 6;
+    const int ic = (int)(blockIdx.x % txs) * 64 + (threadIdx.x & 63);
+    const int jc = (int)(blockIdx.x / txs) * 4 + (threadIdx.x >> 6);
+    const int kc = blockIdx.y;
+    if (ic >= nxc || jc >= nyc) return;
     const int64_t nc = (int64_t)nxc * nyc * nzc;
     const int64_t nf = (int64_t)nxf * nyf * nzf;
-    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (q >= nc) return;
-    const int ic = (int)(q % nxc);
-    const int jc = (int)((q / nxc) % nyc);
-    const int kc = (int)(q / ((int64_t)nxc * nyc));
-    const int fi = fine_of(ic, nxf), fj = fine_of(jc, nyf), fk = fine_of(kc, nzf);
+    const int64_t q = ic + (int64_t)nxc * (jc + (int64_t)nyc * kc);
     if (soft && on_soft_face(ic, jc, kc, nxc, nyc, nzc, soft)) {
-        for (int r = 0; r < k; ++r)
-            if (!live || live[r]) bc[r * nc + q] = mk<T>(0, 0);
+        bc[r * nc + q] = mk<T>(0, 0);
         return;
     }
+    const int fi = fine_of(ic, nxf), fj = fine_of(jc, nyf), fk = fine_of(kc, nzf);
     T wx[3], wy[3], wz[3];
     wx[1] = wy[1] = wz[1] = 1;
     wx[0] = (fi - 1 >= 0 && !coarse_node(fi - 1, nxf)) ? T(0.5) : T(0);
@@ -223,41 +226,59 @@ __global__ __launch_bounds__(256) void k_restrict(const cx<T> *__restrict__ rf, 
     wy[2] = (fj + 1 < nyf && !coarse_node(fj + 1, nyf)) ? T(0.5) : T(0);
     wz[0] = (fk - 1 >= 0 && !coarse_node(fk - 1, nzf)) ? T(0.5) : T(0);
     wz[2] = (fk + 1 < nzf && !coarse_node(fk + 1, nzf)) ? T(0.5) : T(0);
-    for (int r = 0; r < k; ++r) {
-        if (live && !live[r]) continue;
-        const cx<T> *src = rf + r * nf;
-        cx<T> acc = mk<T>(0, 0);
-        for (int dz = 0; dz < 3; ++dz) {
-            if (wz[dz] == T(0)) continue;
-            const int64_t zoff = (int64_t)(fk + dz - 1) * nxf * nyf;
-            for (int dy = 0; dy < 3; ++dy) {
-                if (wy[dy] == T(0)) continue;
-                const int64_t yoff = zoff + (int64_t)(fj + dy - 1) * nxf;
-                const T wyz = wz[dz] * wy[dy];
-                for (int dx = 0; dx < 3; ++dx) {
-                    if (wx[dx] == T(0)) continue;
-                    if (soft && on_soft_face(fi + dx - 1, fj + dy - 1, fk + dz - 1, nxf, nyf, nzf, soft)) continue;
-                    acc = acc + (wyz * wx[dx]) * src[yoff + fi + dx - 1];
-                }
+    const cx<T> *src = rf + r * nf;
+    cx<T> acc = mk<T>(0, 0);
+    for (int dz = 0; dz < 3; ++dz) {
+        if (wz[dz] == T(0)) continue;
+        const int64_t zoff = (int64_t)(fk + dz - 1) * nxf * nyf;
+        for (int dy = 0; dy < 3; ++dy) {
+            if (wy[dy] == T(0)) continue;
+            const int64_t yoff = zoff + (int64_t)(fj + dy - 1) * nxf;
+            const T wyz = wz[dz] * wy[dy];
+            for (int dx = 0; dx < 3; ++dx) {
+                if (wx[dx] == T(0)) continue;
+                if (soft && on_soft_face(fi + dx - 1, fj + dy - 1, fk + dz - 1, nxf, nyf, nzf, soft)) continue;
+                acc = acc + (wyz * wx[dx]) * src[yoff + fi + dx - 1];
             }
         }
-        bc[r * nc + q] = acc;
     }
+    bc[r * nc + q] = acc;
 }
 
-// xf += P xc.  One thread per fine node; up to 8 coarse contributions.
+// Fine iterate after the coarse correction, one thread per (fine node, RHS):
+// 64 x 4 threads per (x, y) tile, grid (tiles, nzf, k).
+//   b == nullptr: xf += P xc
+//   b != nullptr: xf = f b + P xc with f = w dinv / bdiv (dinv conjugated when
+//                 conj_d): the first Jacobi sweep from a zero guess is
+//                 recomputed from b instead of being stored and read back.
+// Fine identity rows get no correction.
 template <typename T>
-__global__ __launch_bounds__(256) void k_prolong_add(const cx<T> *__restrict__ xc, int nxc, int nyc, int nzc,
-                                                     cx<T> *__restrict__ xf, int nxf, int nyf, int nzf, int k,
-                                                     const int *__restrict__ live, int soft) {
+__global__ __launch_bounds__(256) void k_prolong(const cx<T> *__restrict__ xc, int nxc, int nyc, int nzc,
+                                                 cx<T> *__restrict__ xf, int nxf, int nyf, int nzf,
+                                                 const int *__restrict__ live, int soft,
+                                                 const cx<T> *__restrict__ b, const cx<T> *__restrict__ dinv, T w,
+                                                 const T *__restrict__ bdiv, int conj_d) {
+    const int r = blockIdx.z;
+    if (live && !live[r]) return;
+    const int txs = (nxf + 63) >> 6;
+    const int i = (int)(blockIdx.x % txs) * 64 + (threadIdx.x & 63);
+    const int j = (int)(blockIdx.x / txs) * 4 + (threadIdx.x >> 6);
+    const int kk = blockIdx.y;
+    if (i >= nxf || j >= nyf) return;
     const int64_t nf = (int64_t)nxf * nyf * nzf;
     const int64_t nc = (int64_t)nxc * nyc * nzc;
-    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (p >= nf) return;
-    const int i = (int)(p % nxf);
-    const int j = (int)((p / nxf) % nyf);
-    const int kk = (int)(p / ((int64_t)nxf * nyf));
-    if (soft && on_soft_face(i, j, kk, nxf, nyf, nzf, soft)) return;  // identity rows get no correction
+    const int64_t p = i + (int64_t)nxf * (j + (int64_t)nyf * kk);
+    cx<T> base;
+    if (b) {
+        const cx<T> d = conj_d ? conj(dinv[p]) : dinv[p];
+        base = ((bdiv ? w / bdiv[p] : w) * d) * b[r * nf + p];
+    } else {
+        base = xf[r * nf + p];
+    }
+    if (soft && on_soft_face(i, j, kk, nxf, nyf, nzf, soft)) {  // identity rows get no correction
+        if (b) xf[r * nf + p] = base;
+        return;
+    }
     int ci[2], cj[2], ck[2];
     T wi[2], wj[2], wk[2];
     int ni, nj, nk;
@@ -273,22 +294,18 @@ __global__ __launch_bounds__(256) void k_prolong_add(const cx<T> *__restrict__ x
     if (soft) {
         for (int a = 0; a < ni; ++a)
             if (on_soft_face(ci[a], 0, 1, nxc, 3, 3, soft & 3)) wi[a] = 0;
-        for (int b = 0; b < nj; ++b)
-            if (on_soft_face(1, cj[b], 1, 3, nyc, 3, soft & 12)) wj[b] = 0;
+        for (int e = 0; e < nj; ++e)
+            if (on_soft_face(1, cj[e], 1, 3, nyc, 3, soft & 12)) wj[e] = 0;
         for (int c = 0; c < nk; ++c)
             if (on_soft_face(1, 1, ck[c], 3, 3, nzc, soft & 48)) wk[c] = 0;
     }
-    for (int r = 0; r < k; ++r) {
-        if (live && !live[r]) continue;
-        const cx<T> *src = xc + r * nc;
-        cx<T> acc = mk<T>(0, 0);
-        for (int c = 0; c < nk; ++c)
-            for (int b = 0; b < nj; ++b)
-                for (int a = 0; a < ni; ++a)
-                    acc = acc + (wk[c] * wj[b] * wi[a]) *
-                                    src[ci[a] + (int64_t)nxc * (cj[b] + (int64_t)nyc * ck[c])];
-        xf[r * nf + p] = xf[r * nf + p] + acc;
-    }
+    const cx<T> *src = xc + r * nc;
+    cx<T> acc = base;
+    for (int c = 0; c < nk; ++c)
+        for (int e = 0; e < nj; ++e)
+            for (int a = 0; a < ni; ++a)
+                acc = acc + (wk[c] * wj[e] * wi[a]) * src[ci[a] + (int64_t)nxc * (cj[e] + (int64_t)nyc * ck[c])];
+    xf[r * nf + p] = acc;
 }
 
 // x64 += cast<double>(z): apply a complex64 correction to the FP64 iterate.

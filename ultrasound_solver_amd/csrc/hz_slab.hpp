@@ -1,34 +1,66 @@
 // 2.5-D complex64 stencil: the hot apply of the mixed form and the
 // multigrid smoother/residual.
 //
-// A 256-thread workgroup owns a 64 x 4 (x, y) tile and a slab of SLAB_Z
-// z-planes.  The row coefficients of the thread's SLAB_Z nodes are computed
-// once and kept in registers; then, for each right-hand side, the thread
-// marches z with the (z-1, z, z+1) values in registers (one new load per
-// node), takes its x-neighbours from the adjacent lanes with DPP wave
-// shifts (only lanes 0 and 63 load across the tile edge), and loads the two
-// y-neighbours (cache hits inside the tile).  That is about 3 vector loads
-// per node and right-hand side instead of 7.
+// A workgroup of 64 x TY threads owns a 64 x TY (x, y) tile and a slab of ZB
+// z-planes; the autotuner picks (ZB, TY) per grid from a few shapes.  The
+// row coefficients of the thread's ZB nodes are computed once and kept in
+// registers; then, for each right-hand side, the thread issues all loads of
+// its column (planes k0-1 .. k0+ZB, the two y-neighbours of each plane and,
+// in lanes 0 and 63 only, the x-neighbour across the tile edge), takes the
+// other x-neighbours from the adjacent lanes with DPP wave shifts, and
+// marches z with the (z-1, z, z+1) values in registers.  That is about
+// (ZB + 2 + 2 ZB) / ZB vector loads per node and right-hand side instead of 7.
 #pragma once
 
 #include "hz_stencil.hpp"
 
 namespace hz {
 
-constexpr int SLAB_Z = 4;
+// Tile shapes the autotuner chooses from: (z-planes per thread, tile rows).
+struct SlabShape {
+    int zb, ty;
+};
+constexpr int SLAB_SHAPES = 5;
+constexpr SlabShape SLAB_SHAPE[SLAB_SHAPES] = {{4, 4}, {2, 4}, {4, 8}, {2, 8}, {4, 2}};
 
-enum SlabEpi { SLAB_APPLY = 0, SLAB_SMOOTH = 1, SLAB_RESID = 2 };
+__host__ inline TileMap slab_map(int nx, int ny, int nz, int shape) {
+    const SlabShape sh = SLAB_SHAPE[shape];
+    TileMap t;
+    t.tiles_x = (nx + 63) / 64;
+    t.tiles_y = (ny + sh.ty - 1) / sh.ty;
+    t.zb = sh.zb;
+    t.tiles_z = (nz + sh.zb - 1) / sh.zb;
+    t.ntiles = t.tiles_x * t.tiles_y * t.tiles_z;
+    t.nblocks = (t.ntiles + 7) / 8 * 8;
+    return t;
+}
+
+enum SlabEpi { SLAB_APPLY = 0, SLAB_SMOOTH = 1, SLAB_RESID = 2, SLAB_RESID0 = 3 };
 
 // Value of lane l-1 (wave_shr:1) / lane l+1 (wave_shl:1); lanes at the wave
 // edge receive 0 and are fixed up by an explicit load.
 __device__ __forceinline__ float lane_prev(float v) { return __builtin_amdgcn_update_dpp(0.0f, v, 0x138, 0xf, 0xf, false); }
 __device__ __forceinline__ float lane_next(float v) { return __builtin_amdgcn_update_dpp(0.0f, v, 0x130, 0xf, 0xf, false); }
 
+// Multiplier of the first damped-Jacobi sweep from a zero guess,
+// x0 = f * b with f = w * dinv / bdiv (dinv conjugated for the adjoint).
+template <int MODE>
+__device__ __forceinline__ cx<float> first_sweep_factor(const cx<float> *__restrict__ dinv,
+                                                        const float *__restrict__ bdiv, float w, int64_t p) {
+    const cx<float> d = (MODE == MODE_AH) ? conj(dinv[p]) : dinv[p];
+    return (bdiv ? w / bdiv[p] : w) * d;
+}
+
 // APPLY : y = s_r * op(x)
 // SMOOTH: y = x + w * dinv * (b / bdiv - op(x))     (damped Jacobi)
 // RESID : y = b / bdiv - op(x)
-template <int MODE, int EPI>
-__global__ __launch_bounds__(256) void k_slab(OpView<float> op, TileMap tm, const cx<float> *__restrict__ xin,
+// RESID0: y = b / bdiv - op(f b), f = w dinv / bdiv, with xin = b: the
+//         residual after the first Jacobi sweep from a zero guess, without
+//         writing that sweep.  f of each neighbour is folded into the row
+//         coefficients once per node, so the right-hand-side loop is the
+//         RESID loop reading b in place of x.
+template <int MODE, int EPI, int ZB, int TY>
+__global__ __launch_bounds__(64 * TY) void k_slab(OpView<float> op, TileMap tm, const cx<float> *__restrict__ xin,
                                               int64_t ldx, cx<float> *__restrict__ yout, int64_t ldy, int nrhs,
                                               const int *__restrict__ live, const double *__restrict__ scale,
                                               const cx<float> *__restrict__ b, const cx<float> *__restrict__ dinv,
@@ -54,21 +86,21 @@ __global__ __launch_bounds__(256) void k_slab(OpView<float> op, TileMap tm, cons
     const int tz = tile / (tm.tiles_x * tm.tiles_y);
     const int lane = threadIdx.x & 63;
     const int i_raw = tx * 64 + lane;
-    const int j_raw = ty * 4 + (threadIdx.x >> 6);
+    const int j_raw = ty * TY + (threadIdx.x >> 6);
     // Lanes past the grid edge keep running (they feed the DPP shifts) on a
     // clamped node and never store.
     const bool valid = i_raw < op.nx && j_raw < op.ny;
     const int i = min(i_raw, op.nx - 1);
     const int j = min(j_raw, op.ny - 1);
     const int k0 = tz * tm.zb;
-    const int nzl = min(op.nz - k0, SLAB_Z);
+    const int nzl = min(op.nz - k0, ZB);
     const int64_t sxy = (int64_t)op.nx * op.ny;
     const int64_t p0 = i + (int64_t)op.nx * (j + (int64_t)op.ny * k0);
 
-    cx<float> cs[SLAB_Z], c[SLAB_Z][6], dv[SLAB_Z];
-    float bs[SLAB_Z];
+    cx<float> cs[ZB], c[ZB][6], dv[ZB];
+    float bs[ZB];
 #pragma unroll
-    for (int q = 0; q < SLAB_Z; ++q) {
+    for (int q = 0; q < ZB; ++q) {
         const int kq = min(k0 + q, op.nz - 1);  // planes past the slab end repeat the last one
         const int64_t p = i + (int64_t)op.nx * (j + (int64_t)op.ny * kq);
         const Nbr nb = neighbours(i, j, kq, op.nx, op.ny, op.nz);
@@ -77,6 +109,11 @@ __global__ __launch_bounds__(256) void k_slab(OpView<float> op, TileMap tm, cons
         bs[q] = 1.0f;
         if (EPI == SLAB_SMOOTH) dv[q] = (MODE == MODE_AH) ? conj(dinv[p]) : dinv[p];
         if (EPI != SLAB_APPLY && bdiv) bs[q] = 1.0f / bdiv[p];
+        if (EPI == SLAB_RESID0) {
+            cs[q] = cs[q] * first_sweep_factor<MODE>(dinv, bdiv, w, p);
+#pragma unroll
+            for (int e = 0; e < 6; ++e) c[q][e] = c[q][e] * first_sweep_factor<MODE>(dinv, bdiv, w, p + nb.off[e]);
+        }
     }
     const bool has_xm = i > 0, has_xp = i < op.nx - 1;
     const int64_t oym = j > 0 ? -(int64_t)op.nx : 0;
@@ -86,18 +123,18 @@ __global__ __launch_bounds__(256) void k_slab(OpView<float> op, TileMap tm, cons
         const int r = s_act[t];
         const cx<float> *xr = xin + r * ldx;
         // Issue every load of this right-hand side's column first: planes
-        // k0-1 .. k0+SLAB_Z (absent planes read a valid node and are zeroed),
+        // k0-1 .. k0+ZB (absent planes read a valid node and are zeroed),
         // the y-neighbours, and the x-edge values of lanes 0 and 63.
-        cx<float> v[SLAB_Z + 2], ym[SLAB_Z], yp[SLAB_Z], xe[SLAB_Z];
+        cx<float> v[ZB + 2], ym[ZB], yp[ZB], xe[ZB];
 #pragma unroll
-        for (int q = 0; q < SLAB_Z + 2; ++q) {
+        for (int q = 0; q < ZB + 2; ++q) {
             const int kq = k0 + q - 1;
             const int kc = min(max(kq, 0), op.nz - 1);
             v[q] = xr[p0 + (int64_t)(kc - k0) * sxy];
             if (kq < 0 || kq >= op.nz) v[q] = mk<float>(0, 0);
         }
 #pragma unroll
-        for (int q = 0; q < SLAB_Z; ++q) {
+        for (int q = 0; q < ZB; ++q) {
             const int kc = min(k0 + q, op.nz - 1);
             const int64_t p = p0 + (int64_t)(kc - k0) * sxy;
             ym[q] = xr[p + oym];
@@ -107,7 +144,7 @@ __global__ __launch_bounds__(256) void k_slab(OpView<float> op, TileMap tm, cons
             if (lane == 63 && has_xp) xe[q] = xr[p + 1];
         }
 #pragma unroll
-        for (int q = 0; q < SLAB_Z; ++q) {
+        for (int q = 0; q < ZB; ++q) {
             const cx<float> xc = v[q + 1];
             cx<float> xl = mk<float>(lane_prev(xc.re), lane_prev(xc.im));
             cx<float> xn = mk<float>(lane_next(xc.re), lane_next(xc.im));
@@ -124,6 +161,8 @@ __global__ __launch_bounds__(256) void k_slab(OpView<float> op, TileMap tm, cons
                 const int64_t p = p0 + (int64_t)q * sxy;
                 if (EPI == SLAB_APPLY) {
                     yout[r * ldy + p] = s_sc[t] * acc;
+                } else if (EPI == SLAB_RESID0) {
+                    yout[r * ldy + p] = bs[q] * xc - acc;
                 } else {
                     const cx<float> e = bs[q] * b[r * ldy + p] - acc;
                     yout[r * ldy + p] = (EPI == SLAB_SMOOTH) ? xc + w * (dv[q] * e) : e;

@@ -59,6 +59,8 @@ class HelmholtzOperator:
         self.fixed = physics.soft_node_mask(self.shape, self.kinds)
         self.pml_mask = physics.pml_node_mask(self.stretch, self.shape)
         self.rho, self.c = rho, c
+        self.hierarchy = None
+        self.tuning = None
         kappa = 1.0/(rho*c*c)
         keep = []  # host arrays referenced by the descriptor during hz_create
 
@@ -135,7 +137,7 @@ class HelmholtzOperator:
 
     def gmres(self, b, restart=30, maxiter=3000, rtol=1.0e-6, backward_error_tol=1.0e-4,
               precision="c128", adjoint=False, chunk=1, max_refinements=10, inner_rtol=1.0e-5,
-              preconditioner="jacobi", raise_on_failure=False):
+              preconditioner="jacobi", raise_on_failure=False, use_graph=False):
         """Solve A x = b (or A^H x = b) for each column; returns (x, stats)."""
         b = self.as_columns(b, torch.complex128)
         k = b.shape[0]
@@ -144,7 +146,7 @@ class HelmholtzOperator:
                                backward_error_tol=float(backward_error_tol),
                                precision=PRECISIONS[precision], adjoint=int(bool(adjoint)),
                                chunk=int(chunk), max_refinements=int(max_refinements),
-                               inner_rtol=float(inner_rtol), use_graph=0,
+                               inner_rtol=float(inner_rtol), use_graph=int(bool(use_graph)),
                                preconditioner={"jacobi": _capi.PRE_JACOBI,
                                                "multigrid": _capi.PRE_MULTIGRID}[preconditioner])
         stats = (_capi.GmresStats*k)()
@@ -159,6 +161,26 @@ class HelmholtzOperator:
                                f"scaled residual={bad['scaled_residual']:.3e}, "
                                f"backward error={bad['backward_error']:.3e}")
         return x, out
+
+    def autotune(self, nrhs, max_chunk=64, restart=30, preconditioner="multigrid", memory_budget=0):
+        """Pick the stencil tile shapes and the mixed-form RHS chunk width by
+        timing them on this grid (and multigrid hierarchy).  Returns
+        {"chunk": c, "shapes": [...]}; the shapes stay in effect for later
+        applies and solves."""
+        pre = {"jacobi": _capi.PRE_JACOBI, "multigrid": _capi.PRE_MULTIGRID}[preconditioner]
+        nshape = 1 + 3*len(self.hierarchy.levels) if self.hierarchy is not None else 1
+        chunk = ctypes.c_int32(0)
+        shapes = (ctypes.c_int32*nshape)()
+        _capi.check(_capi.lib().hz_autotune(self.handle, int(nrhs), int(max_chunk), int(restart), pre,
+                                            int(memory_budget), ctypes.byref(chunk), shapes, nshape, _stream()),
+                    "hz_autotune")
+        self.tuning = {"chunk": chunk.value, "shapes": list(shapes)}
+        return self.tuning
+
+    def set_tile_shapes(self, shapes):
+        """Pin the slab tile shapes (the layout `autotune` reports)."""
+        arr = (ctypes.c_int32*len(shapes))(*[int(v) for v in shapes])
+        _capi.check(_capi.lib().hz_set_tile_shapes(self.handle, arr, len(shapes)), "hz_set_tile_shapes")
 
     def set_multigrid(self, shift=1.2, pre=1, post=1, coarse_sweeps=30, weight=0.7, levels=None):
         """Install the shifted-Laplacian multigrid preconditioner (mixed form)."""
