@@ -156,6 +156,45 @@ template <>
 const OpView<float> &view<float>(const hz_operator *op) { return op->v32; }
 
 int64_t vec_blocks(int64_t n) { return blocks_for(n, 256 * 16, 1024); }
+// Elements per workgroup of the vector kernels, even so that paired loads
+// never straddle two workgroups.
+int64_t vec_chunk(int64_t n, int64_t nblk) { return ((n + nblk - 1) / nblk + 1) & ~int64_t(1); }
+// Paired 16-byte loads for complex64 vectors of even length.
+template <typename T>
+bool paired(int64_t n) {
+    return std::is_same<T, float>::value && n % 2 == 0;
+}
+
+template <typename T>
+void launch_dots(dim3 grid, hipStream_t s, const cx<T> *V, int k, int64_t n, int jp1, const cx<T> *w,
+                 const int *live, double2 *part, int maxv, int64_t chunk) {
+    if constexpr (std::is_same<T, float>::value) {
+        if (paired<T>(n)) {
+            hipLaunchKernelGGL((k_dots<T, 2>), grid, dim3(256), 0, s, V, k, n, jp1, w, live, part, maxv, chunk);
+            HZ_LAUNCHED();
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_dots<T, 1>), grid, dim3(256), 0, s, V, k, n, jp1, w, live, part, maxv, chunk);
+    HZ_LAUNCHED();
+}
+
+template <typename T>
+void launch_update(dim3 grid, hipStream_t s, const cx<T> *V, int k, int64_t n, int cnt_all, const int *cnt_rhs,
+                   cx<T> *target, int64_t ldt, const cx<double> *coef, int maxv, const int *live, double *norm_partial,
+                   int64_t chunk) {
+    if constexpr (std::is_same<T, float>::value) {
+        if (paired<T>(n) && ldt % 2 == 0) {
+            hipLaunchKernelGGL((k_update<T, 2>), grid, dim3(256), 0, s, V, k, n, cnt_all, cnt_rhs, target, ldt, coef,
+                               maxv, live, norm_partial, chunk);
+            HZ_LAUNCHED();
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_update<T, 1>), grid, dim3(256), 0, s, V, k, n, cnt_all, cnt_rhs, target, ldt, coef, maxv,
+                       live, norm_partial, chunk);
+    HZ_LAUNCHED();
+}
 
 // Grid of the per-node multigrid transfers: 64 x 4 (x, y) tiles, one z-plane
 // per block row, one right-hand side per block layer.
@@ -428,7 +467,7 @@ void gmres_cycles(hz_operator *op, int mode_ram, int k, int restart, int maxiter
     const int64_t n = op->n;
     const int maxv = restart + 1;
     const int64_t nblk = vec_blocks(n);
-    const int64_t chunk = (n + nblk - 1) / nblk;
+    const int64_t chunk = vec_chunk(n, nblk);
     cx<T> *V = op->V.as<cx<T>>();
     cx<T> *X = op->X.as<cx<T>>();
     cx<T> *Bh = op->Bh.as<cx<T>>();
@@ -515,24 +554,21 @@ void gmres_cycles(hz_operator *op, int mode_ram, int k, int restart, int maxiter
                 }
                 apply_mode<T>(op, mode_op, in, n, w, n, k, st.live, st.scales + (int64_t)j * k, cs_stream);
                 const int groups = (j + 1 + DOT_GROUP - 1) / DOT_GROUP;
-                hipLaunchKernelGGL((k_dots<T>), dim3(nblk, k, groups), dim3(256), 0, cs_stream, V, k, n, j + 1, w,
-                                   st.live, op->dpart.as<double2>(), maxv, chunk);
-                HZ_LAUNCHED();
+                launch_dots<T>(dim3(nblk, k, groups), cs_stream, V, k, n, j + 1, w, st.live, op->dpart.as<double2>(),
+                               maxv, chunk);
                 hipLaunchKernelGGL(k_reduce_dots, dim3(j + 1, k), dim3(256), 0, cs_stream, op->dpart.as<double2>(),
                                    (int)nblk, maxv, j, st.scales, st.live, st.H, st.coef, restart);
                 HZ_LAUNCHED();
-                hipLaunchKernelGGL((k_update<T>), dim3(nblk, k), dim3(256), 0, cs_stream, V, k, n, j + 1,
-                                   (const int *)nullptr, w, n, st.coef, maxv, st.live, op->npart.as<double>(), chunk);
-                HZ_LAUNCHED();
+                launch_update<T>(dim3(nblk, k), cs_stream, V, k, n, j + 1, nullptr, w, n, st.coef, maxv, st.live,
+                                 op->npart.as<double>(), chunk);
                 hipLaunchKernelGGL(k_arnoldi_finish, dim3(k), dim3(256), 0, cs_stream, st, op->npart.as<double>(),
                                    (int)nblk, j, maxv, restart);
                 HZ_LAUNCHED();
             }
             hipLaunchKernelGGL(k_trisolve, dim3(k), dim3(64), 0, cs_stream, st, maxv, restart, member);
             HZ_LAUNCHED();
-            hipLaunchKernelGGL((k_update<T>), dim3(nblk, k), dim3(256), 0, cs_stream, V, k, n, 0, st.steps, X, n,
-                               st.coef, maxv, member, (double *)nullptr, chunk);
-            HZ_LAUNCHED();
+            launch_update<T>(dim3(nblk, k), cs_stream, V, k, n, 0, st.steps, X, n, st.coef, maxv, member, nullptr,
+                             chunk);
         };
         if (use_graph) {
             hipGraphExec_t exec = cycle_graph(op, graph_signature(op, sizeof(T), mode_op, k, restart, use_mg), body);
@@ -626,7 +662,7 @@ void solve_plain(hz_operator *op, const hz_gmres_opts &o, const cx<double> *b, i
     const double *rs = o.adjoint ? op->rsa64.as<double>() : op->rsf64.as<double>();
     const cx<double> *m = o.adjoint ? op->ma64.as<cx<double>>() : op->mf64.as<cx<double>>();
     const int64_t nblk = vec_blocks(n);
-    const int64_t chunk = (n + nblk - 1) / nblk;
+    const int64_t chunk = vec_chunk(n, nblk);
     double *part = op->npart.as<double>();
     hipLaunchKernelGGL((k_scale_rhs<double>), dim3(nblk, k), dim3(256), 0, s, b, ldb, rs, op->Bh.as<cx<double>>(),
                        n, n, part, chunk);
@@ -785,16 +821,19 @@ float time_launches(F &&launch, int reps, hipStream_t s) {
     return ms / reps;
 }
 
-// Fastest slab shape for one kernel: `launch(shape)` runs it once.
+// Fastest slab shape for one kernel: `launch(shape)` runs it once.  A shape
+// replaces the current choice only when it is more than 3 % faster, so that
+// timing noise does not flip between equivalent shapes (earlier shapes in the
+// table win ties; `preferred` is tried first).
 template <typename F>
-int pick_shape(F &&launch, hipStream_t s) {
-    int best = 0;
-    float best_ms = 0;
+int pick_shape(F &&launch, hipStream_t s, int preferred = 0) {
+    int best = preferred;
+    float best_ms = time_launches([&] { launch(preferred); }, 5, s);
     for (int sh = 0; sh < SLAB_SHAPES; ++sh) {
-        const float ms = time_launches([&] { launch(sh); }, 5, s);
+        const float ms = sh == preferred ? best_ms : time_launches([&] { launch(sh); }, 5, s);
         if (verbose()) std::fprintf(stderr, "[hz] autotune shape %d (zb %d, ty %d): %.1f us\n", sh, SLAB_SHAPE[sh].zb,
                                     SLAB_SHAPE[sh].ty, ms * 1e3);
-        if (sh == 0 || ms < best_ms) {
+        if (ms < 0.97f * best_ms) {
             best = sh;
             best_ms = ms;
         }
@@ -830,7 +869,7 @@ void arnoldi_step_for_timing(hz_operator *op, int k, int j, int restart, bool us
     const int64_t n = op->n;
     const int maxv = restart + 1;
     const int64_t nblk = vec_blocks(n);
-    const int64_t chunk = (n + nblk - 1) / nblk;
+    const int64_t chunk = vec_chunk(n, nblk);
     cx<float> *V = op->V.as<cx<float>>();
     const cx<float> *in = V + (int64_t)j * k * n;
     cx<float> *w = V + (int64_t)(j + 1) * k * n;
@@ -842,13 +881,9 @@ void arnoldi_step_for_timing(hz_operator *op, int k, int j, int restart, bool us
     }
     apply_mode<float>(op, use_mg ? MODE_RA : MODE_RAM, in, n, w, n, k, live, op->scales.as<double>(), s);
     const int groups = (j + 1 + DOT_GROUP - 1) / DOT_GROUP;
-    hipLaunchKernelGGL((k_dots<float>), dim3(nblk, k, groups), dim3(256), 0, s, (const cx<float> *)V, k, n, j + 1,
-                       (const cx<float> *)w, live, op->dpart.as<double2>(), maxv, chunk);
-    HZ_LAUNCHED();
-    hipLaunchKernelGGL((k_update<float>), dim3(nblk, k), dim3(256), 0, s, (const cx<float> *)V, k, n, j + 1,
-                       (const int *)nullptr, w, n, (const cx<double> *)op->coef.as<cx<double>>(), maxv, live,
-                       op->npart.as<double>(), chunk);
-    HZ_LAUNCHED();
+    launch_dots<float>(dim3(nblk, k, groups), s, V, k, n, j + 1, w, live, op->dpart.as<double2>(), maxv, chunk);
+    launch_update<float>(dim3(nblk, k), s, V, k, n, j + 1, nullptr, w, n, op->coef.as<cx<double>>(), maxv, live,
+                         op->npart.as<double>(), chunk);
 }
 
 template <typename F>
@@ -1338,7 +1373,7 @@ hz_status hz_autotune(hz_operator *op, int32_t nrhs, int32_t max_chunk, int32_t 
                         launch_slab<MODE_A, SLAB_RESID0>(sh, L.v, lb, L.n, lt, L.n, kt, live, nullptr, nullptr, dv,
                                                          mg.weight, bd, s);
                     },
-                    s);
+                    s, 1);
             }
         }
         // Chunk width: least time per column of one Arnoldi step at j = restart / 2.

@@ -20,7 +20,9 @@ constexpr int DOT_GROUP = 16;  // basis vectors per dot-product pass
 
 // partial[((r*maxv + i)*nblk + blk)] = sum over the workgroup's chunk of
 // conj(V_i[r]) * w[r], for i in the group [g*16, min(g*16+16, jp1)).
-template <typename T>
+// VW = 2 (complex64 only; n and chunk even): each lane loads two complex
+// values per 16-byte access.
+template <typename T, int VW>
 __global__ __launch_bounds__(256) void k_dots(const cx<T> *__restrict__ V, int k, int64_t n, int jp1,
                                               const cx<T> *__restrict__ w, const int *__restrict__ live,
                                               double2 *__restrict__ partial, int maxv, int64_t chunk) {
@@ -36,14 +38,30 @@ __global__ __launch_bounds__(256) void k_dots(const cx<T> *__restrict__ V, int k
     T are[DOT_GROUP], aim[DOT_GROUP];
 #pragma unroll
     for (int q = 0; q < DOT_GROUP; ++q) are[q] = aim[q] = 0;
-    for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
-        const cx<T> wv = wr[p];
+    if constexpr (VW == 2) {
+        static_assert(sizeof(T) == 4, "paired loads are for complex64");
+        const float4 *w4 = reinterpret_cast<const float4 *>(wr);
+        for (int64_t p = (p0 >> 1) + threadIdx.x; p < (p1 >> 1); p += 256) {
+            const float4 wv = w4[p];
 #pragma unroll
-        for (int q = 0; q < DOT_GROUP; ++q) {
-            if (q < cnt) {
-                const cx<T> v = V[((int64_t)(g0 + q) * k + r) * n + p];
-                are[q] += v.re * wv.re + v.im * wv.im;
-                aim[q] += v.re * wv.im - v.im * wv.re;
+            for (int q = 0; q < DOT_GROUP; ++q) {
+                if (q < cnt) {
+                    const float4 v = reinterpret_cast<const float4 *>(V + ((int64_t)(g0 + q) * k + r) * n)[p];
+                    are[q] += v.x * wv.x + v.y * wv.y + v.z * wv.z + v.w * wv.w;
+                    aim[q] += v.x * wv.y - v.y * wv.x + v.z * wv.w - v.w * wv.z;
+                }
+            }
+        }
+    } else {
+        for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
+            const cx<T> wv = wr[p];
+#pragma unroll
+            for (int q = 0; q < DOT_GROUP; ++q) {
+                if (q < cnt) {
+                    const cx<T> v = V[((int64_t)(g0 + q) * k + r) * n + p];
+                    are[q] += v.re * wv.re + v.im * wv.im;
+                    aim[q] += v.re * wv.im - v.im * wv.re;
+                }
             }
         }
     }
@@ -87,7 +105,8 @@ __global__ __launch_bounds__(256) void k_reduce_dots(const double2 *__restrict__
 
 // target[r] -= sum_{i<cnt} coef[r][i] * V_i[r]; optional FP64 partial of
 // |target|^2 per workgroup (norm_partial[r*nblk + blk]).  Grid (nblk, k).
-template <typename T>
+// VW = 2 as in k_dots (target leading dimension even as well).
+template <typename T, int VW>
 __global__ __launch_bounds__(256) void k_update(const cx<T> *__restrict__ V, int k, int64_t n, int cnt_all,
                                                 const int *__restrict__ cnt_rhs, cx<T> *__restrict__ target,
                                                 int64_t ldt, const cx<double> *__restrict__ coef, int maxv,
@@ -104,22 +123,55 @@ __global__ __launch_bounds__(256) void k_update(const cx<T> *__restrict__ V, int
     const int64_t p1 = min(n, p0 + chunk);
     cx<T> *tr = target + (int64_t)r * ldt;
     T nrm = 0;
-    for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
-        cx<T> acc = tr[p];
-        int q = 0;
-        for (; q + 4 <= cnt; q += 4) {
-            const cx<T> v0 = V[((int64_t)(q + 0) * k + r) * n + p];
-            const cx<T> v1 = V[((int64_t)(q + 1) * k + r) * n + p];
-            const cx<T> v2 = V[((int64_t)(q + 2) * k + r) * n + p];
-            const cx<T> v3 = V[((int64_t)(q + 3) * k + r) * n + p];
-            acc = acc - cf[q] * v0;
-            acc = acc - cf[q + 1] * v1;
-            acc = acc - cf[q + 2] * v2;
-            acc = acc - cf[q + 3] * v3;
+    if constexpr (VW == 2) {
+        static_assert(sizeof(T) == 4, "paired loads are for complex64");
+        float4 *t4 = reinterpret_cast<float4 *>(tr);
+        for (int64_t p = (p0 >> 1) + threadIdx.x; p < (p1 >> 1); p += 256) {
+            float4 acc = t4[p];
+            int q = 0;
+            for (; q + 4 <= cnt; q += 4) {
+                float4 v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    v[u] = reinterpret_cast<const float4 *>(V + ((int64_t)(q + u) * k + r) * n)[p];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const cx<float> c = cf[q + u];
+                    acc.x -= c.re * v[u].x - c.im * v[u].y;
+                    acc.y -= c.re * v[u].y + c.im * v[u].x;
+                    acc.z -= c.re * v[u].z - c.im * v[u].w;
+                    acc.w -= c.re * v[u].w + c.im * v[u].z;
+                }
+            }
+            for (; q < cnt; ++q) {
+                const float4 v = reinterpret_cast<const float4 *>(V + ((int64_t)q * k + r) * n)[p];
+                const cx<float> c = cf[q];
+                acc.x -= c.re * v.x - c.im * v.y;
+                acc.y -= c.re * v.y + c.im * v.x;
+                acc.z -= c.re * v.z - c.im * v.w;
+                acc.w -= c.re * v.w + c.im * v.z;
+            }
+            t4[p] = acc;
+            nrm += acc.x * acc.x + acc.y * acc.y + acc.z * acc.z + acc.w * acc.w;
         }
-        for (; q < cnt; ++q) acc = acc - cf[q] * V[((int64_t)q * k + r) * n + p];
-        tr[p] = acc;
-        nrm += norm2(acc);
+    } else {
+        for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
+            cx<T> acc = tr[p];
+            int q = 0;
+            for (; q + 4 <= cnt; q += 4) {
+                const cx<T> v0 = V[((int64_t)(q + 0) * k + r) * n + p];
+                const cx<T> v1 = V[((int64_t)(q + 1) * k + r) * n + p];
+                const cx<T> v2 = V[((int64_t)(q + 2) * k + r) * n + p];
+                const cx<T> v3 = V[((int64_t)(q + 3) * k + r) * n + p];
+                acc = acc - cf[q] * v0;
+                acc = acc - cf[q + 1] * v1;
+                acc = acc - cf[q + 2] * v2;
+                acc = acc - cf[q + 3] * v3;
+            }
+            for (; q < cnt; ++q) acc = acc - cf[q] * V[((int64_t)q * k + r) * n + p];
+            tr[p] = acc;
+            nrm += norm2(acc);
+        }
     }
     if (norm_partial) {
         const double s = block_combine_256((double)wave_sum(nrm), red);
