@@ -196,10 +196,26 @@ void launch_update(dim3 grid, hipStream_t s, const cx<T> *V, int k, int64_t n, i
     HZ_LAUNCHED();
 }
 
-// Grid of the per-node multigrid transfers: 64 x 4 (x, y) tiles, one z-plane
-// per block row, one right-hand side per block layer.
-dim3 node_grid(int nx, int ny, int nz, int k) {
-    return dim3((unsigned)(((nx + 63) / 64) * ((ny + 3) / 4)), (unsigned)nz, (unsigned)k);
+// Workgroups of the per-node multigrid transfers: one per 64 x 4 (x, y) tile
+// and z-plane, padded to a multiple of 8 for the XCD remap.
+int plane_blocks(int nx, int ny, int nz) {
+    const int64_t t = (int64_t)((nx + 63) / 64) * ((ny + 3) / 4) * nz;
+    return (int)((t + 7) / 8 * 8);
+}
+
+void launch_restrict(const MgLevel &L, const MgLevel &C, const cx<float> *r, int k, const int *live, hipStream_t s) {
+    const int nb = plane_blocks(C.nx, C.ny, C.nz);
+    hipLaunchKernelGGL((k_restrict<float>), dim3(nb), dim3(256), 0, s, r, L.nx, L.ny, L.nz, C.bb.as<cx<float>>(), C.nx,
+                       C.ny, C.nz, k, live, L.v.soft, nb);
+    HZ_LAUNCHED();
+}
+
+void launch_prolong(const MgLevel &L, const MgLevel &C, cx<float> *xf, int k, const int *live, const cx<float> *b,
+                    const cx<float> *dinv, float w, const float *bdiv, int conj_d, hipStream_t s) {
+    const int nb = plane_blocks(L.nx, L.ny, L.nz);
+    hipLaunchKernelGGL((k_prolong<float>), dim3(nb), dim3(256), 0, s, (const cx<float> *)C.xx.as<cx<float>>(), C.nx,
+                       C.ny, C.nz, xf, L.nx, L.ny, L.nz, k, live, L.v.soft, b, dinv, w, bdiv, conj_d, nb);
+    HZ_LAUNCHED();
 }
 
 // One slab-stencil launch with tile shape `shape` (index into SLAB_SHAPE).
@@ -209,18 +225,23 @@ void launch_slab(int shape, const OpView<float> &v, const cx<float> *x, int64_t 
                  const float *bdiv, hipStream_t s) {
     const TileMap tm = slab_map(v.nx, v.ny, v.nz, shape);
     const dim3 grid(tm.nblocks), block(64 * SLAB_SHAPE[shape].ty);
-#define HZ_SLAB_CASE(ID, ZB, TY)                                                                                   \
+#define HZ_SLAB_CASE(ID, ZB, TY, RP, NT)                                                                           \
     case ID:                                                                                                       \
-        static_assert(SLAB_SHAPE[ID].zb == ZB && SLAB_SHAPE[ID].ty == TY, "slab shape table");                    \
-        hipLaunchKernelGGL((k_slab<MODE, EPI, ZB, TY>), grid, block, 0, s, v, tm, x, ldx, y, ldy, k, live, scale, b, \
-                           dinv, w, bdiv);                                                                         \
+        static_assert(SLAB_SHAPE[ID].zb == ZB && SLAB_SHAPE[ID].ty == TY && SLAB_SHAPE[ID].rp == RP &&             \
+                          SLAB_SHAPE[ID].nt == NT,                                                                 \
+                      "slab shape table");                                                                         \
+        hipLaunchKernelGGL((k_slab<MODE, EPI, ZB, TY, RP, NT>), grid, block, 0, s, v, tm, x, ldx, y, ldy, k, live,   \
+                           scale, b, dinv, w, bdiv);                                                               \
         break;
     switch (shape) {
-        HZ_SLAB_CASE(0, 4, 4)
-        HZ_SLAB_CASE(1, 2, 4)
-        HZ_SLAB_CASE(2, 4, 8)
-        HZ_SLAB_CASE(3, 2, 8)
-        HZ_SLAB_CASE(4, 4, 2)
+        HZ_SLAB_CASE(0, 4, 4, 1, 0)
+        HZ_SLAB_CASE(1, 2, 4, 1, 0)
+        HZ_SLAB_CASE(2, 4, 8, 1, 0)
+        HZ_SLAB_CASE(3, 2, 8, 1, 0)
+        HZ_SLAB_CASE(4, 4, 2, 1, 0)
+        HZ_SLAB_CASE(5, 2, 4, 2, 0)
+        HZ_SLAB_CASE(6, 4, 4, 1, 1)
+        HZ_SLAB_CASE(7, 2, 4, 2, 1)
         default:
             throw BadArgument("unknown slab tile shape");
     }
@@ -333,18 +354,13 @@ void vcycle(Multigrid &mg, int l, const cx<float> *b, cx<float> *x, int k, const
         cx<float> *r = L.tt.as<cx<float>>();
         launch_slab<MODE, SLAB_RESID0>(L.shape[2], L.v, b, L.n, r, L.n, k, live, nullptr, nullptr, dinv, mg.weight, bdiv,
                                        s);
-        hipLaunchKernelGGL((k_restrict<float>), node_grid(C.nx, C.ny, C.nz, k), dim3(256), 0, s, r, L.nx, L.ny, L.nz,
-                           C.bb.as<cx<float>>(), C.nx, C.ny, C.nz, live, L.v.soft);
-        HZ_LAUNCHED();
+        launch_restrict(L, C, r, k, live, s);
         vcycle<MODE>(mg, l + 1, C.bb.as<cx<float>>(), C.xx.as<cx<float>>(), k, live, s);
         // Post-sweeps ping-pong between x and the level scratch; start where
         // the last one lands in x.
         cx<float> *cur = (mg.post % 2 == 0) ? x : L.tt.as<cx<float>>();
         cx<float> *alt = (cur == x) ? L.tt.as<cx<float>>() : x;
-        hipLaunchKernelGGL((k_prolong<float>), node_grid(L.nx, L.ny, L.nz, k), dim3(256), 0, s,
-                           (const cx<float> *)C.xx.as<cx<float>>(), C.nx, C.ny, C.nz, cur, L.nx, L.ny, L.nz, live,
-                           L.v.soft, b, dinv, mg.weight, bdiv, (int)(MODE == MODE_AH));
-        HZ_LAUNCHED();
+        launch_prolong(L, C, cur, k, live, b, dinv, mg.weight, bdiv, (int)(MODE == MODE_AH), s);
         for (int it = 0; it < mg.post; ++it) {
             level_smooth<MODE>(L, cur, b, alt, mg.weight, k, live, bdiv, s);
             std::swap(cur, alt);
@@ -367,14 +383,9 @@ void vcycle(Multigrid &mg, int l, const cx<float> *b, cx<float> *x, int k, const
     if (!coarsest) {
         MgLevel &C = *mg.levels[l + 1];
         launch_slab<MODE, SLAB_RESID>(L.shape[1], L.v, cur, L.n, alt, L.n, k, live, nullptr, b, nullptr, 0.0f, bdiv, s);
-        hipLaunchKernelGGL((k_restrict<float>), node_grid(C.nx, C.ny, C.nz, k), dim3(256), 0, s, alt, L.nx, L.ny, L.nz,
-                           C.bb.as<cx<float>>(), C.nx, C.ny, C.nz, live, L.v.soft);
-        HZ_LAUNCHED();
+        launch_restrict(L, C, alt, k, live, s);
         vcycle<MODE>(mg, l + 1, C.bb.as<cx<float>>(), C.xx.as<cx<float>>(), k, live, s);
-        hipLaunchKernelGGL((k_prolong<float>), node_grid(L.nx, L.ny, L.nz, k), dim3(256), 0, s,
-                           (const cx<float> *)C.xx.as<cx<float>>(), C.nx, C.ny, C.nz, cur, L.nx, L.ny, L.nz, live,
-                           L.v.soft, (const cx<float> *)nullptr, dinv, mg.weight, bdiv, 0);
-        HZ_LAUNCHED();
+        launch_prolong(L, C, cur, k, live, nullptr, dinv, mg.weight, bdiv, 0, s);
         for (int it = 0; it < mg.post; ++it) {
             level_smooth<MODE>(L, cur, b, alt, mg.weight, k, live, bdiv, s);
             std::swap(cur, alt);
@@ -831,8 +842,9 @@ int pick_shape(F &&launch, hipStream_t s, int preferred = 0) {
     float best_ms = time_launches([&] { launch(preferred); }, 5, s);
     for (int sh = 0; sh < SLAB_SHAPES; ++sh) {
         const float ms = sh == preferred ? best_ms : time_launches([&] { launch(sh); }, 5, s);
-        if (verbose()) std::fprintf(stderr, "[hz] autotune shape %d (zb %d, ty %d): %.1f us\n", sh, SLAB_SHAPE[sh].zb,
-                                    SLAB_SHAPE[sh].ty, ms * 1e3);
+        if (verbose())
+            std::fprintf(stderr, "[hz] autotune shape %d (zb %d, ty %d, rp %d, nt %d): %.1f us\n", sh,
+                         SLAB_SHAPE[sh].zb, SLAB_SHAPE[sh].ty, SLAB_SHAPE[sh].rp, SLAB_SHAPE[sh].nt, ms * 1e3);
         if (ms < 0.97f * best_ms) {
             best = sh;
             best_ms = ms;

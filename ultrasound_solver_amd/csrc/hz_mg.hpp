@@ -194,31 +194,64 @@ __global__ __launch_bounds__(256) void k_coarse_jacobi(OpView<T> op, const cx<T>
     }
 }
 
-// Coarse right-hand side bc = P^T rf.  One thread per (coarse node, RHS):
-// 64 x 4 threads per (x, y) tile, grid (tiles, nzc, k).  Identity rows (pressure-release faces) take no
-// part in the coarse correction: their fine residuals are not restricted and
-// coarse identity rows get zero, so a residual-scaled value never lands in a
-// solution-scaled identity row.
+// Live right-hand sides of a launch, listed once per workgroup in LDS.
+__device__ __forceinline__ int live_list(const int *__restrict__ live, int k, int *s_act, int *s_na) {
+    if (threadIdx.x == 0) {
+        int na = 0;
+        for (int r = 0; r < k; ++r)
+            if (!live || live[r]) s_act[na++] = r;
+        *s_na = na;
+    }
+    __syncthreads();
+    return *s_na;
+}
+
+// Workgroup -> (64 x 4 tile, z-plane) of an nx x ny x nz grid, consecutive
+// planes on one XCD so that the transfer's shared rows stay in its L2.
+struct PlaneTile {
+    int i, j, k;
+    bool in_grid;
+};
+__device__ __forceinline__ PlaneTile plane_tile(int nx, int ny, int nz, int nblocks) {
+    const int tile = xcd_tile(blockIdx.x, nblocks);
+    const int tx = (nx + 63) >> 6, ty = (ny + 3) >> 2;
+    PlaneTile t;
+    t.k = tile / (tx * ty);
+    const int txy = tile - t.k * tx * ty;
+    t.i = (txy % tx) * 64 + (threadIdx.x & 63);
+    t.j = (txy / tx) * 4 + (threadIdx.x >> 6);
+    t.in_grid = t.k < nz && t.i < nx && t.j < ny;
+    return t;
+}
+
+// Coarse right-hand side bc = P^T rf, one thread per coarse node looping over
+// the live right-hand sides; grid: plane tiles of the coarse grid, padded to
+// a multiple of 8 (nblocks).  The 27 fine taps and their separable weights
+// are set up once per node (absent taps get weight 0 and a valid address),
+// so every right-hand side issues the same 27 loads.  Identity rows
+// (pressure-release faces) take no part in the coarse correction: their fine
+// residuals are not restricted and coarse identity rows get zero, so a
+// residual-scaled value never lands in a solution-scaled identity row.
 template <typename T>
 __global__ __launch_bounds__(256) void k_restrict(const cx<T> *__restrict__ rf, int nxf, int nyf, int nzf,
-                                                  cx<T> *__restrict__ bc, int nxc, int nyc, int nzc,
-                                                  const int *__restrict__ live, int soft) {
-    const int r = blockIdx.z;
-    if (live && !live[r]) return;
-    const int txs = (nxc + 63) >> 6;
-    const int ic = (int)(blockIdx.x % txs) * 64 + (threadIdx.x & 63);
-    const int jc = (int)(blockIdx.x / txs) * 4 + (threadIdx.x >> 6);
-    const int kc = blockIdx.y;
-    if (ic >= nxc || jc >= nyc) return;
+                                                  cx<T> *__restrict__ bc, int nxc, int nyc, int nzc, int k,
+                                                  const int *__restrict__ live, int soft, int nblocks) {
+    __shared__ int s_act[APPLY_MAX_RHS];
+    __shared__ int s_na;
+    const int na = live_list(live, k, s_act, &s_na);
+    const PlaneTile t = plane_tile(nxc, nyc, nzc, nblocks);
+    if (!t.in_grid) return;
+    const int ic = t.i, jc = t.j, kc = t.k;
     const int64_t nc = (int64_t)nxc * nyc * nzc;
     const int64_t nf = (int64_t)nxf * nyf * nzf;
     const int64_t q = ic + (int64_t)nxc * (jc + (int64_t)nyc * kc);
     if (soft && on_soft_face(ic, jc, kc, nxc, nyc, nzc, soft)) {
-        bc[r * nc + q] = mk<T>(0, 0);
+        for (int u = 0; u < na; ++u) bc[s_act[u] * nc + q] = mk<T>(0, 0);
         return;
     }
     const int fi = fine_of(ic, nxf), fj = fine_of(jc, nyf), fk = fine_of(kc, nzf);
     T wx[3], wy[3], wz[3];
+    int ox[3], oy[3], oz[3];
     wx[1] = wy[1] = wz[1] = 1;
     wx[0] = (fi - 1 >= 0 && !coarse_node(fi - 1, nxf)) ? T(0.5) : T(0);
     wx[2] = (fi + 1 < nxf && !coarse_node(fi + 1, nxf)) ? T(0.5) : T(0);
@@ -226,86 +259,112 @@ __global__ __launch_bounds__(256) void k_restrict(const cx<T> *__restrict__ rf, 
     wy[2] = (fj + 1 < nyf && !coarse_node(fj + 1, nyf)) ? T(0.5) : T(0);
     wz[0] = (fk - 1 >= 0 && !coarse_node(fk - 1, nzf)) ? T(0.5) : T(0);
     wz[2] = (fk + 1 < nzf && !coarse_node(fk + 1, nzf)) ? T(0.5) : T(0);
-    const cx<T> *src = rf + r * nf;
-    cx<T> acc = mk<T>(0, 0);
-    for (int dz = 0; dz < 3; ++dz) {
-        if (wz[dz] == T(0)) continue;
-        const int64_t zoff = (int64_t)(fk + dz - 1) * nxf * nyf;
-        for (int dy = 0; dy < 3; ++dy) {
-            if (wy[dy] == T(0)) continue;
-            const int64_t yoff = zoff + (int64_t)(fj + dy - 1) * nxf;
-            const T wyz = wz[dz] * wy[dy];
-            for (int dx = 0; dx < 3; ++dx) {
-                if (wx[dx] == T(0)) continue;
-                if (soft && on_soft_face(fi + dx - 1, fj + dy - 1, fk + dz - 1, nxf, nyf, nzf, soft)) continue;
-                acc = acc + (wyz * wx[dx]) * src[yoff + fi + dx - 1];
-            }
-        }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        ox[d] = min(max(fi + d - 1, 0), nxf - 1);
+        oy[d] = min(max(fj + d - 1, 0), nyf - 1);
+        oz[d] = min(max(fk + d - 1, 0), nzf - 1);
     }
-    bc[r * nc + q] = acc;
+    int64_t off[27];
+    T wt[27];
+#pragma unroll
+    for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                const int e = (dz * 3 + dy) * 3 + dx;
+                off[e] = ox[dx] + (int64_t)nxf * (oy[dy] + (int64_t)nyf * oz[dz]);
+                T wv = wz[dz] * wy[dy] * wx[dx];
+                if (soft && on_soft_face(ox[dx], oy[dy], oz[dz], nxf, nyf, nzf, soft)) wv = T(0);
+                wt[e] = wv;
+            }
+    for (int u = 0; u < na; ++u) {
+        const int r = s_act[u];
+        const cx<T> *src = rf + r * nf;
+        cx<T> val[27];
+#pragma unroll
+        for (int e = 0; e < 27; ++e) val[e] = src[off[e]];
+        cx<T> acc = mk<T>(0, 0);
+#pragma unroll
+        for (int e = 0; e < 27; ++e) acc = acc + wt[e] * val[e];
+        bc[r * nc + q] = acc;
+    }
 }
 
-// Fine iterate after the coarse correction, one thread per (fine node, RHS):
-// 64 x 4 threads per (x, y) tile, grid (tiles, nzf, k).
+// Fine iterate after the coarse correction, one thread per fine node looping
+// over the live right-hand sides; grid: plane tiles of the fine grid, padded
+// to a multiple of 8 (nblocks).
 //   b == nullptr: xf += P xc
 //   b != nullptr: xf = f b + P xc with f = w dinv / bdiv (dinv conjugated when
 //                 conj_d): the first Jacobi sweep from a zero guess is
 //                 recomputed from b instead of being stored and read back.
-// Fine identity rows get no correction.
+// The 8 coarse taps are set up once per node (absent taps weight 0, valid
+// address).  Fine identity rows get no correction.
 template <typename T>
 __global__ __launch_bounds__(256) void k_prolong(const cx<T> *__restrict__ xc, int nxc, int nyc, int nzc,
-                                                 cx<T> *__restrict__ xf, int nxf, int nyf, int nzf,
+                                                 cx<T> *__restrict__ xf, int nxf, int nyf, int nzf, int k,
                                                  const int *__restrict__ live, int soft,
                                                  const cx<T> *__restrict__ b, const cx<T> *__restrict__ dinv, T w,
-                                                 const T *__restrict__ bdiv, int conj_d) {
-    const int r = blockIdx.z;
-    if (live && !live[r]) return;
-    const int txs = (nxf + 63) >> 6;
-    const int i = (int)(blockIdx.x % txs) * 64 + (threadIdx.x & 63);
-    const int j = (int)(blockIdx.x / txs) * 4 + (threadIdx.x >> 6);
-    const int kk = blockIdx.y;
-    if (i >= nxf || j >= nyf) return;
+                                                 const T *__restrict__ bdiv, int conj_d, int nblocks) {
+    __shared__ int s_act[APPLY_MAX_RHS];
+    __shared__ int s_na;
+    const int na = live_list(live, k, s_act, &s_na);
+    const PlaneTile t = plane_tile(nxf, nyf, nzf, nblocks);
+    if (!t.in_grid) return;
+    const int i = t.i, j = t.j, kk = t.k;
     const int64_t nf = (int64_t)nxf * nyf * nzf;
     const int64_t nc = (int64_t)nxc * nyc * nzc;
     const int64_t p = i + (int64_t)nxf * (j + (int64_t)nyf * kk);
-    cx<T> base;
+    cx<T> f = mk<T>(0, 0);
     if (b) {
         const cx<T> d = conj_d ? conj(dinv[p]) : dinv[p];
-        base = ((bdiv ? w / bdiv[p] : w) * d) * b[r * nf + p];
-    } else {
-        base = xf[r * nf + p];
+        f = (bdiv ? w / bdiv[p] : w) * d;
     }
-    if (soft && on_soft_face(i, j, kk, nxf, nyf, nzf, soft)) {  // identity rows get no correction
-        if (b) xf[r * nf + p] = base;
-        return;
-    }
+    const bool fixed = soft && on_soft_face(i, j, kk, nxf, nyf, nzf, soft);
+    // Per axis: two coarse indices and weights (the second has weight 0 on
+    // coarse nodes).  Coarse identity-row nodes carry no correction
+    // (homogeneous Dirichlet for the correction): for A^H their values are
+    // -A_IF^H x_I, scaled like a residual, and must not leak into interior
+    // fine nodes.
     int ci[2], cj[2], ck[2];
     T wi[2], wj[2], wk[2];
-    int ni, nj, nk;
-    if (coarse_node(i, nxf)) { ci[0] = coarse_of(i, nxf, nxc); wi[0] = 1; ni = 1; }
-    else { ci[0] = coarse_of(i - 1, nxf, nxc); ci[1] = coarse_of(i + 1, nxf, nxc); wi[0] = wi[1] = T(0.5); ni = 2; }
-    if (coarse_node(j, nyf)) { cj[0] = coarse_of(j, nyf, nyc); wj[0] = 1; nj = 1; }
-    else { cj[0] = coarse_of(j - 1, nyf, nyc); cj[1] = coarse_of(j + 1, nyf, nyc); wj[0] = wj[1] = T(0.5); nj = 2; }
-    if (coarse_node(kk, nzf)) { ck[0] = coarse_of(kk, nzf, nzc); wk[0] = 1; nk = 1; }
-    else { ck[0] = coarse_of(kk - 1, nzf, nzc); ck[1] = coarse_of(kk + 1, nzf, nzc); wk[0] = wk[1] = T(0.5); nk = 2; }
-    // Coarse identity-row nodes carry no correction (homogeneous Dirichlet for
-    // the correction).  For A^H their values are -A_IF^H x_I, scaled like a
-    // residual, and must not leak into interior fine nodes.
+    if (coarse_node(i, nxf)) { ci[0] = ci[1] = coarse_of(i, nxf, nxc); wi[0] = 1; wi[1] = 0; }
+    else { ci[0] = coarse_of(i - 1, nxf, nxc); ci[1] = coarse_of(i + 1, nxf, nxc); wi[0] = wi[1] = T(0.5); }
+    if (coarse_node(j, nyf)) { cj[0] = cj[1] = coarse_of(j, nyf, nyc); wj[0] = 1; wj[1] = 0; }
+    else { cj[0] = coarse_of(j - 1, nyf, nyc); cj[1] = coarse_of(j + 1, nyf, nyc); wj[0] = wj[1] = T(0.5); }
+    if (coarse_node(kk, nzf)) { ck[0] = ck[1] = coarse_of(kk, nzf, nzc); wk[0] = 1; wk[1] = 0; }
+    else { ck[0] = coarse_of(kk - 1, nzf, nzc); ck[1] = coarse_of(kk + 1, nzf, nzc); wk[0] = wk[1] = T(0.5); }
     if (soft) {
-        for (int a = 0; a < ni; ++a)
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
             if (on_soft_face(ci[a], 0, 1, nxc, 3, 3, soft & 3)) wi[a] = 0;
-        for (int e = 0; e < nj; ++e)
-            if (on_soft_face(1, cj[e], 1, 3, nyc, 3, soft & 12)) wj[e] = 0;
-        for (int c = 0; c < nk; ++c)
-            if (on_soft_face(1, 1, ck[c], 3, 3, nzc, soft & 48)) wk[c] = 0;
+            if (on_soft_face(1, cj[a], 1, 3, nyc, 3, soft & 12)) wj[a] = 0;
+            if (on_soft_face(1, 1, ck[a], 3, 3, nzc, soft & 48)) wk[a] = 0;
+        }
     }
-    const cx<T> *src = xc + r * nc;
-    cx<T> acc = base;
-    for (int c = 0; c < nk; ++c)
-        for (int e = 0; e < nj; ++e)
-            for (int a = 0; a < ni; ++a)
-                acc = acc + (wk[c] * wj[e] * wi[a]) * src[ci[a] + (int64_t)nxc * (cj[e] + (int64_t)nyc * ck[c])];
-    xf[r * nf + p] = acc;
+    int64_t off[8];
+    T wt[8];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int e = 0; e < 2; ++e)
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                off[(c * 2 + e) * 2 + a] = ci[a] + (int64_t)nxc * (cj[e] + (int64_t)nyc * ck[c]);
+                wt[(c * 2 + e) * 2 + a] = fixed ? T(0) : wk[c] * wj[e] * wi[a];
+            }
+    for (int u = 0; u < na; ++u) {
+        const int r = s_act[u];
+        const cx<T> *src = xc + r * nc;
+        cx<T> val[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) val[e] = src[off[e]];
+        cx<T> acc = b ? f * b[r * nf + p] : xf[r * nf + p];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) acc = acc + wt[e] * val[e];
+        xf[r * nf + p] = acc;
+    }
 }
 
 // x64 += cast<double>(z): apply a complex64 correction to the FP64 iterate.

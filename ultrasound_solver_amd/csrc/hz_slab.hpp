@@ -16,12 +16,15 @@
 
 namespace hz {
 
-// Tile shapes the autotuner chooses from: (z-planes per thread, tile rows).
+// Tile shapes the autotuner chooses from: z-planes per thread, tile rows,
+// right-hand sides per pass (loads of all of them issued together) and
+// non-temporal output stores.
 struct SlabShape {
-    int zb, ty;
+    int zb, ty, rp, nt;
 };
-constexpr int SLAB_SHAPES = 5;
-constexpr SlabShape SLAB_SHAPE[SLAB_SHAPES] = {{4, 4}, {2, 4}, {4, 8}, {2, 8}, {4, 2}};
+constexpr int SLAB_SHAPES = 8;
+constexpr SlabShape SLAB_SHAPE[SLAB_SHAPES] = {{4, 4, 1, 0}, {2, 4, 1, 0}, {4, 8, 1, 0}, {2, 8, 1, 0},
+                                               {4, 2, 1, 0}, {2, 4, 2, 0}, {4, 4, 1, 1}, {2, 4, 2, 1}};
 
 __host__ inline TileMap slab_map(int nx, int ny, int nz, int shape) {
     const SlabShape sh = SLAB_SHAPE[shape];
@@ -59,7 +62,22 @@ __device__ __forceinline__ cx<float> first_sweep_factor(const cx<float> *__restr
 //         writing that sweep.  f of each neighbour is folded into the row
 //         coefficients once per node, so the right-hand-side loop is the
 //         RESID loop reading b in place of x.
-template <int MODE, int EPI, int ZB, int TY>
+// Output store; NT = 1 marks it non-temporal (streamed past the caches, so
+// that they keep the input halo rows the neighbouring tiles re-read).
+typedef float hz_f2 __attribute__((ext_vector_type(2)));
+template <int NT>
+__device__ __forceinline__ void store_out(cx<float> *dst, cx<float> v) {
+    if (NT) {
+        hz_f2 t;
+        t.x = v.re;
+        t.y = v.im;
+        __builtin_nontemporal_store(t, reinterpret_cast<hz_f2 *>(dst));
+    } else {
+        *dst = v;
+    }
+}
+
+template <int MODE, int EPI, int ZB, int TY, int RP, int NT>
 __global__ __launch_bounds__(64 * TY) void k_slab(OpView<float> op, TileMap tm, const cx<float> *__restrict__ xin,
                                               int64_t ldx, cx<float> *__restrict__ yout, int64_t ldy, int nrhs,
                                               const int *__restrict__ live, const double *__restrict__ scale,
@@ -119,53 +137,68 @@ __global__ __launch_bounds__(64 * TY) void k_slab(OpView<float> op, TileMap tm, 
     const int64_t oym = j > 0 ? -(int64_t)op.nx : 0;
     const int64_t oyp = j < op.ny - 1 ? (int64_t)op.nx : 0;
     const int na = s_na;
-    for (int t = 0; t < na; ++t) {
-        const int r = s_act[t];
-        const cx<float> *xr = xin + r * ldx;
-        // Issue every load of this right-hand side's column first: planes
-        // k0-1 .. k0+ZB (absent planes read a valid node and are zeroed),
-        // the y-neighbours, and the x-edge values of lanes 0 and 63.
-        cx<float> v[ZB + 2], ym[ZB], yp[ZB], xe[ZB];
+    for (int t = 0; t < na; t += RP) {
+        // RP right-hand sides per pass.  Every load of their columns is
+        // issued before any arithmetic: planes k0-1 .. k0+ZB (absent planes
+        // read a valid node and are zeroed), the y-neighbours, the x-edge
+        // values of lanes 0 and 63, and b for the SMOOTH/RESID epilogues.  A
+        // pass past the last column repeats it and does not store.
+        int rr[RP];
+        cx<float> v[RP][ZB + 2], ym[RP][ZB], yp[RP][ZB], xe[RP][ZB], bb[RP][ZB];
 #pragma unroll
-        for (int q = 0; q < ZB + 2; ++q) {
-            const int kq = k0 + q - 1;
-            const int kc = min(max(kq, 0), op.nz - 1);
-            v[q] = xr[p0 + (int64_t)(kc - k0) * sxy];
-            if (kq < 0 || kq >= op.nz) v[q] = mk<float>(0, 0);
+        for (int u = 0; u < RP; ++u) {
+            rr[u] = s_act[min(t + u, na - 1)];
+            const cx<float> *xr = xin + rr[u] * ldx;
+#pragma unroll
+            for (int q = 0; q < ZB + 2; ++q) {
+                const int kq = k0 + q - 1;
+                const int kc = min(max(kq, 0), op.nz - 1);
+                v[u][q] = xr[p0 + (int64_t)(kc - k0) * sxy];
+                if (kq < 0 || kq >= op.nz) v[u][q] = mk<float>(0, 0);
+            }
+#pragma unroll
+            for (int q = 0; q < ZB; ++q) {
+                const int kc = min(k0 + q, op.nz - 1);
+                const int64_t p = p0 + (int64_t)(kc - k0) * sxy;
+                ym[u][q] = xr[p + oym];
+                yp[u][q] = xr[p + oyp];
+                xe[u][q] = mk<float>(0, 0);
+                if (lane == 0 && has_xm) xe[u][q] = xr[p - 1];
+                if (lane == 63 && has_xp) xe[u][q] = xr[p + 1];
+                bb[u][q] = mk<float>(0, 0);
+                if (EPI == SLAB_SMOOTH || EPI == SLAB_RESID) bb[u][q] = b[rr[u] * ldy + p];
+            }
         }
 #pragma unroll
-        for (int q = 0; q < ZB; ++q) {
-            const int kc = min(k0 + q, op.nz - 1);
-            const int64_t p = p0 + (int64_t)(kc - k0) * sxy;
-            ym[q] = xr[p + oym];
-            yp[q] = xr[p + oyp];
-            xe[q] = mk<float>(0, 0);
-            if (lane == 0 && has_xm) xe[q] = xr[p - 1];
-            if (lane == 63 && has_xp) xe[q] = xr[p + 1];
-        }
+        for (int u = 0; u < RP; ++u) {
+            if (u > 0 && t + u >= na) break;
+            const int r = rr[u];
 #pragma unroll
-        for (int q = 0; q < ZB; ++q) {
-            const cx<float> xc = v[q + 1];
-            cx<float> xl = mk<float>(lane_prev(xc.re), lane_prev(xc.im));
-            cx<float> xn = mk<float>(lane_next(xc.re), lane_next(xc.im));
-            if (lane == 0) xl = xe[q];
-            if (lane == 63) xn = xe[q];
-            cx<float> acc = cs[q] * xc;
-            acc = acc + c[q][0] * xl;
-            acc = acc + c[q][1] * xn;
-            acc = acc + c[q][2] * ym[q];
-            acc = acc + c[q][3] * yp[q];
-            acc = acc + c[q][4] * v[q];
-            acc = acc + c[q][5] * v[q + 2];
-            if (valid && q < nzl) {
-                const int64_t p = p0 + (int64_t)q * sxy;
-                if (EPI == SLAB_APPLY) {
-                    yout[r * ldy + p] = s_sc[t] * acc;
-                } else if (EPI == SLAB_RESID0) {
-                    yout[r * ldy + p] = bs[q] * xc - acc;
-                } else {
-                    const cx<float> e = bs[q] * b[r * ldy + p] - acc;
-                    yout[r * ldy + p] = (EPI == SLAB_SMOOTH) ? xc + w * (dv[q] * e) : e;
+            for (int q = 0; q < ZB; ++q) {
+                const cx<float> xc = v[u][q + 1];
+                cx<float> xl = mk<float>(lane_prev(xc.re), lane_prev(xc.im));
+                cx<float> xn = mk<float>(lane_next(xc.re), lane_next(xc.im));
+                if (lane == 0) xl = xe[u][q];
+                if (lane == 63) xn = xe[u][q];
+                cx<float> acc = cs[q] * xc;
+                acc = acc + c[q][0] * xl;
+                acc = acc + c[q][1] * xn;
+                acc = acc + c[q][2] * ym[u][q];
+                acc = acc + c[q][3] * yp[u][q];
+                acc = acc + c[q][4] * v[u][q];
+                acc = acc + c[q][5] * v[u][q + 2];
+                if (valid && q < nzl) {
+                    const int64_t p = p0 + (int64_t)q * sxy;
+                    cx<float> out;
+                    if (EPI == SLAB_APPLY) {
+                        out = s_sc[t + u] * acc;
+                    } else if (EPI == SLAB_RESID0) {
+                        out = bs[q] * xc - acc;
+                    } else {
+                        const cx<float> e = bs[q] * bb[u][q] - acc;
+                        out = (EPI == SLAB_SMOOTH) ? xc + w * (dv[q] * e) : e;
+                    }
+                    store_out<NT>(yout + r * ldy + p, out);
                 }
             }
         }

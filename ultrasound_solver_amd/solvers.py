@@ -18,7 +18,8 @@ The backend string keeps the reference's ``solver.backend`` key; only
 ``gpu_iterative`` exists here (the cuDSS direct backends are NVIDIA-only).
 Extra optional ``solver:`` keys: ``gmres_precision`` ("fp64" plain form,
 "mixed" complex64 + FP64 refinement), ``gmres_preconditioner`` ("jacobi" or
-"multigrid", mixed form only), ``gmres_chunk`` (right-hand sides per batch),
+"multigrid", mixed form only), ``gmres_chunk`` (right-hand sides per batch, or
+"auto" for the autotuner's choice),
 ``gmres_max_refinements``, ``gmres_inner_rtol``.
 """
 import hashlib
@@ -79,7 +80,9 @@ class GpuIterativeSolver:
         if preconditioner == "multigrid" and precision != "mixed":
             raise ValueError("the multigrid preconditioner runs in the mixed form (gmres_precision: mixed)")
         self.precision = precision
-        self.chunk = int(chunk)
+        # "auto" (or 0): the first mixed-form solve on an operator runs the
+        # autotuner (tile shapes and chunk width) and keeps its choice.
+        self.chunk = 0 if chunk in ("auto", 0, None) else int(chunk)
         self.preconditioner = preconditioner
         self.max_refinements = int(max_refinements)
         self.inner_rtol = float(inner_rtol)
@@ -94,11 +97,18 @@ class GpuIterativeSolver:
         (N,) / (N, k) host data or a (k, N) device tensor.  Returns a device
         tensor of the same column count ((N,) stays one row)."""
         b = operator.as_columns(rhs)
+        chunk = 1
+        if self.precision == "mixed":
+            chunk = self.chunk
+            if not chunk:
+                if operator.tuning is None:
+                    operator.autotune(max(b.shape[0], 64), restart=self.restart, preconditioner=self.preconditioner)
+                chunk = operator.tuning["chunk"]
         x, stats = operator.gmres(
             b, restart=self.restart, maxiter=self.maxiter, rtol=self.rtol,
             backward_error_tol=self.backward_error_tol,
             precision="c128" if self.precision == "fp64" else "c64", adjoint=adjoint,
-            chunk=self.chunk if self.precision == "mixed" else 1, max_refinements=self.max_refinements,
+            chunk=chunk, max_refinements=self.max_refinements,
             inner_rtol=self.inner_rtol, preconditioner=self.preconditioner)
         self.last_stats = stats
         self.last_iterations = max(s["iterations"] for s in stats)
@@ -149,7 +159,7 @@ class HelmholtzSolver:
             restart=int(s.get("gmres_restart", 30)), maxiter=int(s.get("gmres_maxiter", 3000)),
             rtol=float(s.get("gmres_rtol", 1.0e-6)),
             backward_error_tol=float(s.get("gmres_backward_error_tol", 1.0e-4)),
-            precision=str(s.get("gmres_precision", "fp64")), chunk=int(s.get("gmres_chunk", 8)),
+            precision=str(s.get("gmres_precision", "fp64")), chunk=s.get("gmres_chunk", 8),
             preconditioner=str(s.get("gmres_preconditioner", "jacobi")),
             max_refinements=int(s.get("gmres_max_refinements", 10)),
             inner_rtol=float(s.get("gmres_inner_rtol", 1.0e-5)))
