@@ -148,8 +148,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(scene, args, plain_iters)
-    zb, ty = SLAB_SHAPES[tuning["shapes"][0]]
-    traffic = measured_traffic(args, zb, ty)
+    zb, ty, rp, nt = HelmholtzOperator.SLAB_SHAPES[tuning["shapes"][0]]
+    traffic = measured_traffic(args, zb, ty, rp, nt)
 
     out = {
         "metric": "RHS solves/s at 128^3 to 1e-6 rel. residual (mixed vs FP64); stencil HBM GB/s",
@@ -167,7 +167,7 @@ def main():
         "refinements_median": float(np.median(refines)) if refines else None,
         "worst_scaled_residual": worst,
         "setup_s": setup_s,
-        "roofline": {"kernel": f"hz::k_slab<1, 0, {zb}, {ty}> (complex64 R A M apply, {k} columns)",
+        "roofline": {"kernel": f"hz::k_slab<1, 0, {zb}, {ty}, {rp}, {nt}> (complex64 R A M apply, {k} columns)",
                      "bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                      "frac": achieved/8000.0, "traffic": traffic, "alg_bytes_per_launch": alg_bytes,
                      "launch_us": stencil_s*1e6},
@@ -178,17 +178,15 @@ def main():
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "stencil_128_k16_traffic.md")
 STENCIL_RHS = 16  # columns per stencil launch in the roofline measurement
-# (z-planes per thread, tile rows) of the slab kernel's tile shapes, by index.
-SLAB_SHAPES = [(4, 4), (2, 4), (4, 8), (2, 8), (4, 2)]
 
 
-def measured_traffic(args, zb, ty):
+def measured_traffic(args, zb, ty, rp, nt):
     """HBM bytes per stencil launch from the FETCH_SIZE/WRITE_SIZE counter runs
     of the same kernel, tile shape and workload (128^3, 16 columns), scaled by
     the copy calibration recorded beside them; None for other workloads."""
     if args.n != 128 or not os.path.exists(TRAFFIC_PROFILE):
         return None
-    key = f"traffic_bytes_per_launch[zb={zb},ty={ty}]:"
+    key = f"traffic_bytes_per_launch[zb={zb},ty={ty},rp={rp},nt={nt}]:"
     with open(TRAFFIC_PROFILE) as f:
         for line in f:
             if line.startswith(key):

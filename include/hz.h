@@ -145,6 +145,19 @@ hz_status hz_gmres(hz_operator *op, const hz_gmres_opts *opts, const void *b,
                    void *x, int32_t nrhs, int64_t ld, hz_gmres_stats *stats, void *stream);
 
 /* Source columns B (N x M, real): nnz entries (node, column, value), host. */
+/* Several independent scenes in one call: scene s solves nrhs[s] columns
+ * b[s] (leading dimension ld[s] >= N_s) into x[s] with the same options and
+ * writes stats[s][0 .. nrhs[s]).  Scenes run concurrently on up to `workers`
+ * internal streams (<= 0: 4; at most 16), longest first by N x nrhs, each
+ * as hz_gmres would run it (graph replay included).  Work on the internal
+ * streams starts after the work already queued on `stream`, and `stream` is
+ * ordered after all of it on return.  Each operator may appear once.  No
+ * single-call counterpart in the reference: its solver handles one scene per
+ * process (src/algorithm/solvers/helmholtz_solver.py:589-604). */
+hz_status hz_gmres_batch(int32_t nscenes, hz_operator *const *ops, const hz_gmres_opts *opts,
+                         const void *const *b, void *const *x, const int32_t *nrhs, const int64_t *ld,
+                         hz_gmres_stats *const *stats, int32_t workers, void *stream);
+
 hz_status hz_set_sources(hz_operator *op, int32_t num_columns, int64_t nnz,
                          const int64_t *node, const int32_t *column, const double *value);
 

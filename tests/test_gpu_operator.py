@@ -246,7 +246,8 @@ def test_tile_shapes_are_bitwise_identical_and_autotune_picks_one():
     x = torch.as_tensor(_rand(op.n, 5, 21).T.copy(), device="cuda:0").to(torch.complex64)
     b = _point_rhs(op.n, 3, 17)
     ref_apply, ref_x, ref_it = None, None, None
-    for shape in range(5):
+    nvar = len(HelmholtzOperator.SLAB_SHAPES)
+    for shape in range(nvar):
         op.set_tile_shapes([shape]*nshape)
         ys = [op.apply(x, mode=m, precision="c64") for m in ("A", "RAM", "AH", "RAMH")]
         sol, st = op.gmres(b, rtol=1e-8, precision="c64", chunk=3, maxiter=4000, preconditioner="multigrid")
@@ -260,7 +261,7 @@ def test_tile_shapes_are_bitwise_identical_and_autotune_picks_one():
         assert torch.equal(sol, ref_x), f"shape {shape}: solution differs"
     tune = op.autotune(64, max_chunk=32, restart=30, preconditioner="multigrid")
     assert tune["chunk"] in (4, 8, 16, 32)
-    assert len(tune["shapes"]) == nshape and all(0 <= v < 5 for v in tune["shapes"])
+    assert len(tune["shapes"]) == nshape and all(0 <= v < nvar for v in tune["shapes"])
     sol, st = op.gmres(b, rtol=1e-8, precision="c64", chunk=3, maxiter=4000, preconditioner="multigrid")
     assert torch.equal(sol, ref_x)
 
@@ -285,3 +286,29 @@ def test_graph_replay_matches_eager(precision, pre):
         x1, s1 = op.gmres(b, use_graph=True, **kw)
         assert torch.equal(x1, x0)
         assert s1 == s0
+
+
+def test_scene_batch_matches_sequential_solves():
+    """hz_gmres_batch runs uneven scenes concurrently on worker streams; each
+    scene's result and statistics equal its own hz_gmres call bit for bit,
+    with and without graph replay.  A failing scene is named in the error."""
+    from ultrasound_solver_amd.operator import HelmholtzOperator, gmres_batch
+    ops, bs = [], []
+    for n, seed in ((17, 1), (23, 2), (29, 3)):
+        rho, c, bnd, pml = _scene(n, seed=seed, soft=(seed == 2))
+        bnd["+y"] = {"type": "impedance", "impedance": [823.2, 205.8]}
+        op = HelmholtzOperator(rho, c, 1.07e-3, 2*np.pi*40e3, bnd, pml, device="cuda:0")
+        op.set_multigrid()
+        ops.append(op)
+        bs.append(_point_rhs(op.n, 2 + seed, 30 + seed))
+    kw = dict(rtol=1e-8, precision="c64", chunk=3, maxiter=4000, preconditioner="multigrid")
+    seq = [op.gmres(b, **kw) for op, b in zip(ops, bs)]
+    for use_graph in (False, True):
+        out = gmres_batch(ops, bs, workers=3, use_graph=use_graph, **kw)
+        for (x, st), (x0, st0) in zip(out, seq):
+            assert torch.equal(x, x0)
+            assert st == st0
+    rho, c, bnd, pml = _scene(15, seed=4)
+    plain = HelmholtzOperator(rho, c, 1.07e-3, 2*np.pi*40e3, bnd, pml, device="cuda:0")  # no hierarchy
+    with pytest.raises(RuntimeError, match="scene 1"):
+        gmres_batch([ops[0], plain], [bs[0], _point_rhs(plain.n, 1, 5)], **kw)

@@ -12,6 +12,8 @@
 #include <string>
 #include <type_traits>
 #include <vector>
+#include <thread>
+#include <atomic>
 
 #include "../../include/hz.h"
 #include "hz_krylov.hpp"
@@ -812,6 +814,36 @@ void solve_mixed(hz_operator *op, const hz_gmres_opts &o, const cx<double> *b, i
     }
 }
 
+// Validated hz_gmres on one stream (throws on bad arguments or HIP errors).
+void gmres_run(hz_operator *op, const hz_gmres_opts *opts, const void *b, void *x, int nrhs, int64_t ld,
+               hz_gmres_stats *stats, hipStream_t s) {
+    if (!op) throw BadArgument("null operator handle");
+    if (!opts || !b || !x || !stats) throw BadArgument("opts, b, x and stats are required");
+    const hz_gmres_opts &o = *opts;
+    if (nrhs < 1 || ld < op->n) throw BadArgument("need nrhs >= 1 and ld >= N");
+    if (o.restart < 1 || o.restart > 63) throw BadArgument("restart must be in [1, 63]");
+    if (o.maxiter < 0) throw BadArgument("maxiter must be >= 0");
+    if (!(o.rtol > 0) || !(o.backward_error_tol > 0)) throw BadArgument("tolerances must be positive");
+    if (o.precision != HZ_C128 && o.precision != HZ_C64) throw BadArgument("unknown precision");
+    if (o.preconditioner == PRE_MULTIGRID) {
+        if (o.precision != HZ_C64) throw BadArgument("the multigrid preconditioner runs in the mixed form only");
+        if (!op->mg) throw BadArgument("no multigrid hierarchy set (hz_set_multigrid)");
+    } else if (o.preconditioner != PRE_JACOBI) {
+        throw BadArgument("unknown preconditioner");
+    }
+    // Right-hand sides per batch; the stencil kernels stage at most APPLY_MAX_RHS in LDS.
+    const int chunk = std::max(1, std::min<int>(std::min<int>(nrhs, APPLY_MAX_RHS), o.chunk > 0 ? o.chunk : 1));
+    for (int c0 = 0; c0 < nrhs; c0 += chunk) {
+        const int k = std::min(chunk, nrhs - c0);
+        const cx<double> *bc = static_cast<const cx<double> *>(b) + (int64_t)c0 * ld;
+        cx<double> *xc = static_cast<cx<double> *>(x) + (int64_t)c0 * ld;
+        if (o.precision == HZ_C128)
+            solve_plain(op, o, bc, ld, xc, ld, k, stats + c0, s);
+        else
+            solve_mixed(op, o, bc, ld, xc, ld, k, stats + c0, s);
+    }
+}
+
 // ---------------------------------------------------------------- autotuner
 
 // Mean duration (ms) of `reps` back-to-back launches after one warm-up.
@@ -1078,37 +1110,87 @@ hz_status hz_apply(hz_operator *op, int32_t precision, int32_t mode, const void 
 
 hz_status hz_gmres(hz_operator *op, const hz_gmres_opts *opts, const void *b, void *x, int32_t nrhs, int64_t ld,
                    hz_gmres_stats *stats, void *stream) {
-    const hz_status st = guarded([&] {
-        check_op(op);
-        if (!opts || !b || !x || !stats) throw BadArgument("opts, b, x and stats are required");
-        const hz_gmres_opts &o = *opts;
-        if (nrhs < 1 || ld < op->n) throw BadArgument("need nrhs >= 1 and ld >= N");
-        if (o.restart < 1 || o.restart > 63) throw BadArgument("restart must be in [1, 63]");
-        if (o.maxiter < 0) throw BadArgument("maxiter must be >= 0");
-        if (!(o.rtol > 0) || !(o.backward_error_tol > 0)) throw BadArgument("tolerances must be positive");
-        if (o.precision != HZ_C128 && o.precision != HZ_C64) throw BadArgument("unknown precision");
-        if (o.preconditioner == PRE_MULTIGRID) {
-            if (o.precision != HZ_C64) throw BadArgument("the multigrid preconditioner runs in the mixed form only");
-            if (!op->mg) throw BadArgument("no multigrid hierarchy set (hz_set_multigrid)");
-        } else if (o.preconditioner != PRE_JACOBI) {
-            throw BadArgument("unknown preconditioner");
-        }
-        hipStream_t s = static_cast<hipStream_t>(stream);
-        // Right-hand sides per batch; the stencil kernels stage at most APPLY_MAX_RHS in LDS.
-        const int chunk = std::max(1, std::min<int>(std::min<int>(nrhs, APPLY_MAX_RHS), o.chunk > 0 ? o.chunk : 1));
-        for (int c0 = 0; c0 < nrhs; c0 += chunk) {
-            const int k = std::min(chunk, nrhs - c0);
-            const cx<double> *bc = static_cast<const cx<double> *>(b) + (int64_t)c0 * ld;
-            cx<double> *xc = static_cast<cx<double> *>(x) + (int64_t)c0 * ld;
-            if (o.precision == HZ_C128)
-                solve_plain(op, o, bc, ld, xc, ld, k, stats + c0, s);
-            else
-                solve_mixed(op, o, bc, ld, xc, ld, k, stats + c0, s);
-        }
-    });
+    const hz_status st = guarded([&] { gmres_run(op, opts, b, x, nrhs, ld, stats, static_cast<hipStream_t>(stream)); });
     if (st != HZ_OK) return st;
     for (int r = 0; r < nrhs; ++r)
         if (!stats[r].converged) return HZ_NOT_CONVERGED;
+    return HZ_OK;
+}
+
+hz_status hz_gmres_batch(int32_t nscenes, hz_operator *const *ops, const hz_gmres_opts *opts, const void *const *b,
+                         void *const *x, const int32_t *nrhs, const int64_t *ld, hz_gmres_stats *const *stats,
+                         int32_t workers, void *stream) {
+    hipStream_t caller = static_cast<hipStream_t>(stream);
+    std::vector<int> order;
+    std::vector<hipStream_t> streams;
+    hz_status st = guarded([&] {
+        if (nscenes < 1 || !ops || !opts || !b || !x || !nrhs || !ld || !stats)
+            throw BadArgument("need nscenes >= 1 and every per-scene array");
+        for (int i = 0; i < nscenes; ++i) {
+            check_op(ops[i]);
+            for (int j = 0; j < i; ++j)
+                if (ops[j] == ops[i]) throw BadArgument("an operator appears twice in one batch");
+            order.push_back(i);
+        }
+        // Longest first, by nodes x right-hand sides.
+        std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+            return (double)ops[a]->n * nrhs[a] > (double)ops[c]->n * nrhs[c];
+        });
+        const int nw = std::max(1, std::min({workers > 0 ? (int)workers : 4, (int)nscenes, 16}));
+        hipEvent_t start;
+        HZ_HIP(hipEventCreateWithFlags(&start, hipEventDisableTiming));
+        HZ_HIP(hipEventRecord(start, caller));
+        for (int w = 0; w < nw; ++w) {
+            hipStream_t ws;
+            HZ_HIP(hipStreamCreateWithFlags(&ws, hipStreamNonBlocking));
+            streams.push_back(ws);
+            HZ_HIP(hipStreamWaitEvent(ws, start, 0));
+        }
+        (void)hipEventDestroy(start);
+    });
+    if (st != HZ_OK) {
+        for (hipStream_t ws : streams) (void)hipStreamDestroy(ws);
+        return st;
+    }
+    int device = 0;
+    (void)hipGetDevice(&device);
+    std::vector<hz_status> status(nscenes, HZ_OK);
+    std::vector<std::string> error(nscenes);
+    std::atomic<int> next{0};
+    auto worker = [&](hipStream_t ws) {
+        (void)hipSetDevice(device);
+        for (int idx = next++; idx < nscenes; idx = next++) {
+            const int sc = order[idx];
+            status[sc] = guarded([&] { gmres_run(ops[sc], opts, b[sc], x[sc], nrhs[sc], ld[sc], stats[sc], ws); });
+            if (status[sc] != HZ_OK) error[sc] = g_error;
+        }
+    };
+    std::vector<std::thread> pool;
+    for (size_t w = 1; w < streams.size(); ++w) pool.emplace_back(worker, streams[w]);
+    worker(streams[0]);
+    for (auto &t : pool) t.join();
+    st = guarded([&] {
+        for (hipStream_t ws : streams) {
+            hipEvent_t done;
+            HZ_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+            HZ_HIP(hipEventRecord(done, ws));
+            HZ_HIP(hipStreamWaitEvent(caller, done, 0));
+            (void)hipEventDestroy(done);
+        }
+    });
+    for (hipStream_t ws : streams) {
+        (void)hipStreamSynchronize(ws);
+        (void)hipStreamDestroy(ws);
+    }
+    if (st != HZ_OK) return st;
+    for (int sc = 0; sc < nscenes; ++sc)
+        if (status[sc] != HZ_OK) {
+            g_error = "scene " + std::to_string(sc) + ": " + error[sc];
+            return status[sc];
+        }
+    for (int sc = 0; sc < nscenes; ++sc)
+        for (int r = 0; r < nrhs[sc]; ++r)
+            if (!stats[sc][r].converged) return HZ_NOT_CONVERGED;
     return HZ_OK;
 }
 

@@ -34,6 +34,44 @@ def _device_tensor(x, dtype, device):
     return t.contiguous()
 
 
+def gmres_options(restart=30, maxiter=3000, rtol=1.0e-6, backward_error_tol=1.0e-4, precision="c128",
+                  adjoint=False, chunk=1, max_refinements=10, inner_rtol=1.0e-5, preconditioner="jacobi",
+                  use_graph=False):
+    return _capi.GmresOpts(restart=int(restart), maxiter=int(maxiter), rtol=float(rtol),
+                           backward_error_tol=float(backward_error_tol),
+                           precision=PRECISIONS[precision], adjoint=int(bool(adjoint)),
+                           chunk=int(chunk), max_refinements=int(max_refinements),
+                           inner_rtol=float(inner_rtol), use_graph=int(bool(use_graph)),
+                           preconditioner={"jacobi": _capi.PRE_JACOBI,
+                                           "multigrid": _capi.PRE_MULTIGRID}[preconditioner])
+
+
+def gmres_batch(operators, rhs, workers=4, **options):
+    """Solve several scenes concurrently (one hz_gmres_batch call).
+
+    operators: HelmholtzOperator per scene (each at most once); rhs: matching
+    right-hand sides ((N,), (N, k) or (k, N) device tensors).  Options as in
+    HelmholtzOperator.gmres.  Returns [(x, stats)] in scene order; raises
+    RuntimeError naming the scene on an argument or device error."""
+    ns = len(operators)
+    if ns != len(rhs) or ns == 0:
+        raise ValueError("need one right-hand side per operator")
+    bs = [op.as_columns(b, torch.complex128) for op, b in zip(operators, rhs)]
+    xs = [torch.empty_like(b) for b in bs]
+    stats = [(_capi.GmresStats*b.shape[0])() for b in bs]
+    opts = gmres_options(**options)
+    vp = ctypes.c_void_p
+    status = _capi.lib().hz_gmres_batch(
+        ns, (vp*ns)(*[op.handle for op in operators]), ctypes.byref(opts),
+        (vp*ns)(*[b.data_ptr() for b in bs]), (vp*ns)(*[x.data_ptr() for x in xs]),
+        (ctypes.c_int32*ns)(*[b.shape[0] for b in bs]), (ctypes.c_int64*ns)(*[op.n for op in operators]),
+        (ctypes.POINTER(_capi.GmresStats)*ns)(*[ctypes.cast(st, ctypes.POINTER(_capi.GmresStats)) for st in stats]),
+        int(workers), _stream())
+    if status not in (_capi.HZ_OK, _capi.HZ_NOT_CONVERGED):
+        _capi.check(status, "hz_gmres_batch")
+    return [(x, [s.as_dict() for s in st]) for x, st in zip(xs, stats)]
+
+
 class HelmholtzOperator:
     """A = div(rho^-1 grad) + omega^2/(rho c^2) with PML and face rows, on the GPU."""
 
@@ -142,13 +180,8 @@ class HelmholtzOperator:
         b = self.as_columns(b, torch.complex128)
         k = b.shape[0]
         x = torch.empty_like(b)
-        opts = _capi.GmresOpts(restart=int(restart), maxiter=int(maxiter), rtol=float(rtol),
-                               backward_error_tol=float(backward_error_tol),
-                               precision=PRECISIONS[precision], adjoint=int(bool(adjoint)),
-                               chunk=int(chunk), max_refinements=int(max_refinements),
-                               inner_rtol=float(inner_rtol), use_graph=int(bool(use_graph)),
-                               preconditioner={"jacobi": _capi.PRE_JACOBI,
-                                               "multigrid": _capi.PRE_MULTIGRID}[preconditioner])
+        opts = gmres_options(restart, maxiter, rtol, backward_error_tol, precision, adjoint, chunk,
+                             max_refinements, inner_rtol, preconditioner, use_graph)
         stats = (_capi.GmresStats*k)()
         status = _capi.lib().hz_gmres(self.handle, ctypes.byref(opts), ctypes.c_void_p(b.data_ptr()),
                                       ctypes.c_void_p(x.data_ptr()), k, self.n, stats, _stream())
@@ -176,6 +209,11 @@ class HelmholtzOperator:
                     "hz_autotune")
         self.tuning = {"chunk": chunk.value, "shapes": list(shapes)}
         return self.tuning
+
+    #: Slab tile shapes by index (z-planes per thread, tile rows, right-hand
+    #: sides per pass, non-temporal stores), as in the library's table.
+    SLAB_SHAPES = ((4, 4, 1, 0), (2, 4, 1, 0), (4, 8, 1, 0), (2, 8, 1, 0),
+                   (4, 2, 1, 0), (2, 4, 2, 0), (4, 4, 1, 1), (2, 4, 2, 1))
 
     def set_tile_shapes(self, shapes):
         """Pin the slab tile shapes (the layout `autotune` reports)."""
