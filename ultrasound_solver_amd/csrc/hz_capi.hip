@@ -198,15 +198,15 @@ void launch_update(dim3 grid, hipStream_t s, const cx<T> *V, int k, int64_t n, i
     HZ_LAUNCHED();
 }
 
-// Workgroups of the per-node multigrid transfers: one per 64 x 4 (x, y) tile
-// and z-plane, padded to a multiple of 8 for the XCD remap.
-int plane_blocks(int nx, int ny, int nz) {
-    const int64_t t = (int64_t)((nx + 63) / 64) * ((ny + 3) / 4) * nz;
+// Workgroups of the per-node multigrid transfers: one per 64 x 4 (x, y) tile,
+// z-plane and slot, padded to a multiple of 8 for the XCD remap.
+int plane_blocks(int nx, int ny, int nz, int k) {
+    const int64_t t = (int64_t)((nx + 63) / 64) * ((ny + 3) / 4) * nz * k;
     return (int)((t + 7) / 8 * 8);
 }
 
 void launch_restrict(const MgLevel &L, const MgLevel &C, const cx<float> *r, int k, const int *live, hipStream_t s) {
-    const int nb = plane_blocks(C.nx, C.ny, C.nz);
+    const int nb = plane_blocks(C.nx, C.ny, C.nz, k);
     hipLaunchKernelGGL((k_restrict<float>), dim3(nb), dim3(256), 0, s, r, L.nx, L.ny, L.nz, C.bb.as<cx<float>>(), C.nx,
                        C.ny, C.nz, k, live, L.v.soft, nb);
     HZ_LAUNCHED();
@@ -214,7 +214,7 @@ void launch_restrict(const MgLevel &L, const MgLevel &C, const cx<float> *r, int
 
 void launch_prolong(const MgLevel &L, const MgLevel &C, cx<float> *xf, int k, const int *live, const cx<float> *b,
                     const cx<float> *dinv, float w, const float *bdiv, int conj_d, hipStream_t s) {
-    const int nb = plane_blocks(L.nx, L.ny, L.nz);
+    const int nb = plane_blocks(L.nx, L.ny, L.nz, k);
     hipLaunchKernelGGL((k_prolong<float>), dim3(nb), dim3(256), 0, s, (const cx<float> *)C.xx.as<cx<float>>(), C.nx,
                        C.ny, C.nz, xf, L.nx, L.ny, L.nz, k, live, L.v.soft, b, dinv, w, bdiv, conj_d, nb);
     HZ_LAUNCHED();

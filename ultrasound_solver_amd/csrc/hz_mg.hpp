@@ -194,59 +194,49 @@ __global__ __launch_bounds__(256) void k_coarse_jacobi(OpView<T> op, const cx<T>
     }
 }
 
-// Live right-hand sides of a launch, listed once per workgroup in LDS.
-__device__ __forceinline__ int live_list(const int *__restrict__ live, int k, int *s_act, int *s_na) {
-    if (threadIdx.x == 0) {
-        int na = 0;
-        for (int r = 0; r < k; ++r)
-            if (!live || live[r]) s_act[na++] = r;
-        *s_na = na;
-    }
-    __syncthreads();
-    return *s_na;
-}
-
-// Workgroup -> (64 x 4 tile, z-plane) of an nx x ny x nz grid, consecutive
-// planes on one XCD so that the transfer's shared rows stay in its L2.
+// Workgroup -> (64 x 4 tile, z-plane, right-hand side) of an nx x ny x nz
+// grid with k slots: the slot is the slowest index and consecutive planes of
+// one slot land on one XCD, so the rows that neighbouring tiles share stay in
+// that XCD's L2.
 struct PlaneTile {
-    int i, j, k;
+    int i, j, k, r;
     bool in_grid;
 };
-__device__ __forceinline__ PlaneTile plane_tile(int nx, int ny, int nz, int nblocks) {
+__device__ __forceinline__ PlaneTile plane_tile(int nx, int ny, int nz, int nslots, int nblocks) {
     const int tile = xcd_tile(blockIdx.x, nblocks);
     const int tx = (nx + 63) >> 6, ty = (ny + 3) >> 2;
+    const int per_slot = tx * ty * nz;
     PlaneTile t;
-    t.k = tile / (tx * ty);
-    const int txy = tile - t.k * tx * ty;
+    t.r = tile / per_slot;
+    const int rest = tile - t.r * per_slot;
+    t.k = rest / (tx * ty);
+    const int txy = rest - t.k * tx * ty;
     t.i = (txy % tx) * 64 + (threadIdx.x & 63);
     t.j = (txy / tx) * 4 + (threadIdx.x >> 6);
-    t.in_grid = t.k < nz && t.i < nx && t.j < ny;
+    t.in_grid = t.r < nslots && t.i < nx && t.j < ny;
     return t;
 }
 
-// Coarse right-hand side bc = P^T rf, one thread per coarse node looping over
-// the live right-hand sides; grid: plane tiles of the coarse grid, padded to
-// a multiple of 8 (nblocks).  The 27 fine taps and their separable weights
-// are set up once per node (absent taps get weight 0 and a valid address),
-// so every right-hand side issues the same 27 loads.  Identity rows
-// (pressure-release faces) take no part in the coarse correction: their fine
-// residuals are not restricted and coarse identity rows get zero, so a
-// residual-scaled value never lands in a solution-scaled identity row.
+// Coarse right-hand side bc = P^T rf, one thread per (coarse node, slot);
+// grid: plane tiles of the coarse grid times k slots, padded to a multiple
+// of 8 (nblocks).  The 27 fine taps get separable weights; absent taps get
+// weight 0 and a valid address, so every thread issues the same 27 loads.
+// Identity rows (pressure-release faces) take no part in the coarse
+// correction: their fine residuals are not restricted and coarse identity
+// rows get zero, so a residual-scaled value never lands in a solution-scaled
+// identity row.
 template <typename T>
 __global__ __launch_bounds__(256) void k_restrict(const cx<T> *__restrict__ rf, int nxf, int nyf, int nzf,
                                                   cx<T> *__restrict__ bc, int nxc, int nyc, int nzc, int k,
                                                   const int *__restrict__ live, int soft, int nblocks) {
-    __shared__ int s_act[APPLY_MAX_RHS];
-    __shared__ int s_na;
-    const int na = live_list(live, k, s_act, &s_na);
-    const PlaneTile t = plane_tile(nxc, nyc, nzc, nblocks);
-    if (!t.in_grid) return;
-    const int ic = t.i, jc = t.j, kc = t.k;
+    const PlaneTile t = plane_tile(nxc, nyc, nzc, k, nblocks);
+    if (!t.in_grid || (live && !live[t.r])) return;
+    const int ic = t.i, jc = t.j, kc = t.k, r = t.r;
     const int64_t nc = (int64_t)nxc * nyc * nzc;
     const int64_t nf = (int64_t)nxf * nyf * nzf;
     const int64_t q = ic + (int64_t)nxc * (jc + (int64_t)nyc * kc);
     if (soft && on_soft_face(ic, jc, kc, nxc, nyc, nzc, soft)) {
-        for (int u = 0; u < na; ++u) bc[s_act[u] * nc + q] = mk<T>(0, 0);
+        bc[r * nc + q] = mk<T>(0, 0);
         return;
     }
     const int fi = fine_of(ic, nxf), fj = fine_of(jc, nyf), fk = fine_of(kc, nzf);
@@ -265,7 +255,8 @@ __global__ __launch_bounds__(256) void k_restrict(const cx<T> *__restrict__ rf, 
         oy[d] = min(max(fj + d - 1, 0), nyf - 1);
         oz[d] = min(max(fk + d - 1, 0), nzf - 1);
     }
-    int64_t off[27];
+    const cx<T> *src = rf + r * nf;
+    cx<T> val[27];
     T wt[27];
 #pragma unroll
     for (int dz = 0; dz < 3; ++dz)
@@ -274,53 +265,38 @@ __global__ __launch_bounds__(256) void k_restrict(const cx<T> *__restrict__ rf, 
 #pragma unroll
             for (int dx = 0; dx < 3; ++dx) {
                 const int e = (dz * 3 + dy) * 3 + dx;
-                off[e] = ox[dx] + (int64_t)nxf * (oy[dy] + (int64_t)nyf * oz[dz]);
+                val[e] = src[ox[dx] + (int64_t)nxf * (oy[dy] + (int64_t)nyf * oz[dz])];
                 T wv = wz[dz] * wy[dy] * wx[dx];
                 if (soft && on_soft_face(ox[dx], oy[dy], oz[dz], nxf, nyf, nzf, soft)) wv = T(0);
                 wt[e] = wv;
             }
-    for (int u = 0; u < na; ++u) {
-        const int r = s_act[u];
-        const cx<T> *src = rf + r * nf;
-        cx<T> val[27];
+    cx<T> acc = mk<T>(0, 0);
 #pragma unroll
-        for (int e = 0; e < 27; ++e) val[e] = src[off[e]];
-        cx<T> acc = mk<T>(0, 0);
-#pragma unroll
-        for (int e = 0; e < 27; ++e) acc = acc + wt[e] * val[e];
-        bc[r * nc + q] = acc;
-    }
+    for (int e = 0; e < 27; ++e) acc = acc + wt[e] * val[e];
+    bc[r * nc + q] = acc;
 }
 
-// Fine iterate after the coarse correction, one thread per fine node looping
-// over the live right-hand sides; grid: plane tiles of the fine grid, padded
-// to a multiple of 8 (nblocks).
+// Fine iterate after the coarse correction, one thread per (fine node,
+// slot); grid: plane tiles of the fine grid times k slots, padded to a
+// multiple of 8 (nblocks).
 //   b == nullptr: xf += P xc
 //   b != nullptr: xf = f b + P xc with f = w dinv / bdiv (dinv conjugated when
 //                 conj_d): the first Jacobi sweep from a zero guess is
 //                 recomputed from b instead of being stored and read back.
-// The 8 coarse taps are set up once per node (absent taps weight 0, valid
-// address).  Fine identity rows get no correction.
+// The 8 coarse taps always load (absent taps weight 0, valid address).  Fine
+// identity rows get no correction.
 template <typename T>
 __global__ __launch_bounds__(256) void k_prolong(const cx<T> *__restrict__ xc, int nxc, int nyc, int nzc,
                                                  cx<T> *__restrict__ xf, int nxf, int nyf, int nzf, int k,
                                                  const int *__restrict__ live, int soft,
                                                  const cx<T> *__restrict__ b, const cx<T> *__restrict__ dinv, T w,
                                                  const T *__restrict__ bdiv, int conj_d, int nblocks) {
-    __shared__ int s_act[APPLY_MAX_RHS];
-    __shared__ int s_na;
-    const int na = live_list(live, k, s_act, &s_na);
-    const PlaneTile t = plane_tile(nxf, nyf, nzf, nblocks);
-    if (!t.in_grid) return;
-    const int i = t.i, j = t.j, kk = t.k;
+    const PlaneTile t = plane_tile(nxf, nyf, nzf, k, nblocks);
+    if (!t.in_grid || (live && !live[t.r])) return;
+    const int i = t.i, j = t.j, kk = t.k, r = t.r;
     const int64_t nf = (int64_t)nxf * nyf * nzf;
     const int64_t nc = (int64_t)nxc * nyc * nzc;
     const int64_t p = i + (int64_t)nxf * (j + (int64_t)nyf * kk);
-    cx<T> f = mk<T>(0, 0);
-    if (b) {
-        const cx<T> d = conj_d ? conj(dinv[p]) : dinv[p];
-        f = (bdiv ? w / bdiv[p] : w) * d;
-    }
     const bool fixed = soft && on_soft_face(i, j, kk, nxf, nyf, nzf, soft);
     // Per axis: two coarse indices and weights (the second has weight 0 on
     // coarse nodes).  Coarse identity-row nodes carry no correction
@@ -343,7 +319,8 @@ __global__ __launch_bounds__(256) void k_prolong(const cx<T> *__restrict__ xc, i
             if (on_soft_face(1, 1, ck[a], 3, 3, nzc, soft & 48)) wk[a] = 0;
         }
     }
-    int64_t off[8];
+    const cx<T> *src = xc + r * nc;
+    cx<T> val[8];
     T wt[8];
 #pragma unroll
     for (int c = 0; c < 2; ++c)
@@ -351,20 +328,20 @@ __global__ __launch_bounds__(256) void k_prolong(const cx<T> *__restrict__ xc, i
         for (int e = 0; e < 2; ++e)
 #pragma unroll
             for (int a = 0; a < 2; ++a) {
-                off[(c * 2 + e) * 2 + a] = ci[a] + (int64_t)nxc * (cj[e] + (int64_t)nyc * ck[c]);
-                wt[(c * 2 + e) * 2 + a] = fixed ? T(0) : wk[c] * wj[e] * wi[a];
+                const int u = (c * 2 + e) * 2 + a;
+                val[u] = src[ci[a] + (int64_t)nxc * (cj[e] + (int64_t)nyc * ck[c])];
+                wt[u] = fixed ? T(0) : wk[c] * wj[e] * wi[a];
             }
-    for (int u = 0; u < na; ++u) {
-        const int r = s_act[u];
-        const cx<T> *src = xc + r * nc;
-        cx<T> val[8];
-#pragma unroll
-        for (int e = 0; e < 8; ++e) val[e] = src[off[e]];
-        cx<T> acc = b ? f * b[r * nf + p] : xf[r * nf + p];
-#pragma unroll
-        for (int e = 0; e < 8; ++e) acc = acc + wt[e] * val[e];
-        xf[r * nf + p] = acc;
+    cx<T> acc;
+    if (b) {
+        const cx<T> d = conj_d ? conj(dinv[p]) : dinv[p];
+        acc = ((bdiv ? w / bdiv[p] : w) * d) * b[r * nf + p];
+    } else {
+        acc = xf[r * nf + p];
     }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc = acc + wt[u] * val[u];
+    xf[r * nf + p] = acc;
 }
 
 // x64 += cast<double>(z): apply a complex64 correction to the FP64 iterate.
