@@ -44,6 +44,12 @@ def parse():
     p.add_argument("--mg-post", type=int, default=1)
     p.add_argument("--mg-weight", type=float, default=0.7)
     p.add_argument("--graph", type=int, default=0, help="replay each GMRES cycle from a captured hipGraph")
+    p.add_argument("--workload", choices=["g128", "uneven", "large"], default="g128",
+                   help="g128: G columns of the heterogeneous 128^3 scene (the headline); uneven: the "
+                        "32-scene batch; large: the 256^3 two-layer scene")
+    p.add_argument("--scenes", type=int, default=32, help="uneven: scenes in the batch")
+    p.add_argument("--scene-rhs", type=int, default=16, help="uneven: right-hand sides per scene")
+    p.add_argument("--workers", type=int, default=4, help="uneven: concurrent scene streams")
     p.add_argument("--precond", choices=["multigrid", "jacobi"], default="multigrid",
                    help="inner preconditioner of the mixed form (the plain form always uses Jacobi)")
     return p.parse_args()
@@ -63,6 +69,10 @@ def main():
     from ultrasound_solver_amd import scenes
     from ultrasound_solver_amd.operator import HelmholtzOperator
 
+    if args.workload == "uneven":
+        return bench_uneven(args)
+    if args.workload == "large":
+        return bench_large(args)
     t0 = time.perf_counter()
     scene = scenes.config_heterogeneous(n=args.n)
     op = HelmholtzOperator(scene.rho, scene.c, scene.dx, scene.omega, scene.boundaries, scene.pml,
@@ -216,6 +226,127 @@ def stencil_only(op, args, m):
     torch.cuda.synchronize()
     print(json.dumps({"stencil_launches": args.stencil_reps, "rhs": k, "n": op.n,
                       "alg_bytes_per_launch": (16*k + 8)*op.n, "copy_bytes_each_way": src.numel()*4}))
+
+
+def _unit_rhs(op, m, start, k):
+    import torch
+    w = torch.zeros((k, m), dtype=torch.complex128, device="cuda:0")
+    idx = torch.arange(k, device="cuda:0")
+    w[idx, (start + idx) % m] = 1.0
+    return op.build_rhs(w)
+
+
+def _scene_operator(scene, multigrid=True, **mg):
+    from ultrasound_solver_amd import scenes
+    from ultrasound_solver_amd.operator import HelmholtzOperator
+    op = HelmholtzOperator(scene.rho, scene.c, scene.dx, scene.omega, scene.boundaries, scene.pml, device="cuda:0")
+    node, col, val = scenes.source_triplets(scene)
+    op.set_sources(node, col, val, scene.num_sources)
+    if multigrid:
+        op.set_multigrid(**mg)
+    return op
+
+
+def bench_uneven(args):
+    """The uneven batch: `--scenes` scenes (edge 64/96/128, 1-3 obstacles,
+    impedance walls, 8x8 arrays), `--scene-rhs` columns of G each, solved
+    scene after scene and then as one hz_gmres_batch call on `--workers`
+    streams.  `value` is the batch's solves/s."""
+    import torch
+    from ultrasound_solver_amd import scenes
+    from ultrasound_solver_amd.operator import gmres_batch
+    t0 = time.perf_counter()
+    sc, ops = [], []
+    for i in range(args.scenes):
+        sc.append(scenes.config_uneven(i))
+        ops.append(_scene_operator(sc[-1], shift=args.mg_shift))
+        if i % 8 == 7:
+            log(f"uneven: {i + 1} scenes set up")
+    bs = [_unit_rhs(op, s.num_sources, 0, args.scene_rhs) for op, s in zip(ops, sc)]
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    edges = [s.shape[0] for s in sc]
+    log(f"uneven setup {setup_s:.1f}s: edges {edges}")
+    kw = dict(restart=args.restart, maxiter=args.maxiter, rtol=args.rtol, backward_error_tol=1e-4,
+              precision="c64", chunk=args.chunk or 16, preconditioner="multigrid", use_graph=bool(args.graph))
+    # Warm-up: one small batch (kernel loading, graph capture paths).
+    gmres_batch(ops[:2], [b[:1] for b in bs[:2]], workers=2, **kw)
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    seq_ok = 0
+    for op, b in zip(ops, bs):
+        _, st = op.gmres(b, **kw)
+        seq_ok += sum(x["converged"] for x in st)
+        op.release_workspace()
+    torch.cuda.synchronize()
+    seq_s = time.perf_counter() - ts
+    log(f"uneven sequential: {seq_ok} solves in {seq_s:.2f}s")
+    tb = time.perf_counter()
+    out = gmres_batch(ops, bs, workers=args.workers, **kw)
+    torch.cuda.synchronize()
+    batch_s = time.perf_counter() - tb
+    ok = sum(x["converged"] for _, st in out for x in st)
+    iters = [x["iterations"] for _, st in out for x in st]
+    log(f"uneven batch ({args.workers} streams): {ok} solves in {batch_s:.2f}s")
+    print(json.dumps({
+        "metric": "uneven 32-scene batch: RHS solves/s to 1e-6 rel. residual (mixed form)",
+        "value": ok/batch_s, "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "c64 (FP64 refinement)", "data": "synthetic",
+        "config": {"workload": f"{args.scenes} uneven scenes (edges {sorted(set(edges))}), {args.scene_rhs} G "
+                               f"columns each, multigrid GMRES({args.restart}), {args.workers} streams",
+                   "edges": edges, "rhs_per_scene": args.scene_rhs},
+        "sequential": {"value": seq_ok/seq_s, "unit": "solves/s"},
+        "iterations_median": float(np.median(iters)), "setup_s": setup_s}), flush=True)
+
+
+def bench_large(args):
+    """The 256^3 two-layer scene: one chunk of 16 G columns (multigrid GMRES,
+    FP64 refinement), the stencil's algorithmic GB/s at 16 columns, and the
+    peak device memory."""
+    import torch
+    from ultrasound_solver_amd import scenes
+    t0 = time.perf_counter()
+    scene = scenes.config_two_layer(256)
+    op = _scene_operator(scene, shift=args.mg_shift)
+    m = scene.num_sources
+    tuning = op.autotune(m, max_chunk=16, restart=args.restart, preconditioner="multigrid")
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    log(f"large setup {setup_s:.1f}s N={op.n}; autotune {tuning}")
+    k = 16
+    b = _unit_rhs(op, m, 0, k)
+    torch.cuda.reset_peak_memory_stats()
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    _, st = op.gmres(b, restart=args.restart, maxiter=args.maxiter, rtol=args.rtol, backward_error_tol=1e-4,
+                     precision="c64", chunk=k, preconditioner="multigrid", use_graph=bool(args.graph))
+    torch.cuda.synchronize()
+    solve_s = time.perf_counter() - ts
+    ok = sum(x["converged"] for x in st)
+    peak = torch.cuda.max_memory_allocated() + op.device_bytes()
+    log(f"large: {ok}/{k} solves in {solve_s:.2f}s; median iterations {np.median([x['iterations'] for x in st])}")
+    x = torch.randn((k, op.n), dtype=torch.complex64, device="cuda:0")
+    for _ in range(3):
+        op.apply(x, mode="RAM", precision="c64")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.stencil_reps):
+        op.apply(x, mode="RAM", precision="c64")
+    e1.record()
+    torch.cuda.synchronize()
+    st_s = e0.elapsed_time(e1)/1e3/args.stencil_reps
+    alg = (16*k + 8)*op.n
+    print(json.dumps({
+        "metric": "256^3 two-layer scene: RHS solves/s to 1e-6 rel. residual (mixed form); stencil HBM GB/s",
+        "value": ok/solve_s, "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "c64 (FP64 refinement)", "data": "synthetic",
+        "config": {"workload": "two-layer 256^3 scene with rigid ellipsoid, 16 G columns, multigrid "
+                               f"GMRES({args.restart}) c64 + FP64 refinement", "grid": [256]*3, "rhs": k,
+                   "autotune": tuning},
+        "iterations_median": float(np.median([x["iterations"] for x in st])),
+        "stencil": {"achieved_GBps": alg/st_s/1e9, "peak_GBps": 8000.0, "frac": alg/st_s/1e9/8000.0,
+                    "launch_us": st_s*1e6, "alg_bytes_per_launch": alg},
+        "peak_device_bytes": int(peak), "setup_s": setup_s}), flush=True)
 
 
 def cpu_baseline(scene, args, plain_iters):

@@ -149,7 +149,9 @@ hz_status hz_gmres(hz_operator *op, const hz_gmres_opts *opts, const void *b,
  * b[s] (leading dimension ld[s] >= N_s) into x[s] with the same options and
  * writes stats[s][0 .. nrhs[s]).  Scenes run concurrently on up to `workers`
  * internal streams (<= 0: 4; at most 16), longest first by N x nrhs, each
- * as hz_gmres would run it (graph replay included).  Work on the internal
+ * as hz_gmres would run it (graph replay included), and releases its
+ * cached workspace when done, so device memory is bounded by the scenes in
+ * flight rather than the batch.  Work on the internal
  * streams starts after the work already queued on `stream`, and `stream` is
  * ordered after all of it on return.  Each operator may appear once.  No
  * single-call counterpart in the reference: its solver handles one scene per
@@ -204,6 +206,12 @@ hz_status hz_set_multigrid(hz_operator *op, int32_t nlevels, const hz_level_desc
 hz_status hz_autotune(hz_operator *op, int32_t nrhs, int32_t max_chunk, int32_t restart,
                       int32_t preconditioner, int64_t memory_budget, int32_t *chunk_out,
                       int32_t *shapes_out, int32_t shapes_len, void *stream);
+
+/* Free the cached solver workspace (Krylov basis, multigrid level vectors,
+ * captured graphs); the next solve allocates what it needs.  Waits for the
+ * device first.  No counterpart in the reference (CuPy's pool frees on its
+ * own schedule). */
+hz_status hz_release_workspace(hz_operator *op);
 
 /* Pin the slab tile shapes (the layout hz_autotune reports: the apply, then
  * smoother / residual / first-sweep residual per multigrid level; len must

@@ -895,6 +895,11 @@ size_t chunk_bytes(const hz_operator *op, int k, int restart) {
     return nodes * k * sizeof(cx<float>);
 }
 
+void drop_graphs(hz_operator *op) {
+    for (auto &g : op->graphs) (void)hipGraphExecDestroy(g.exec);
+    op->graphs.clear();
+}
+
 void release_workspace(hz_operator *op) {
     for (DevBuf *b : {&op->V, &op->X, &op->Bh, &op->rhs32}) b->release();
     if (op->mg) {
@@ -1161,7 +1166,14 @@ hz_status hz_gmres_batch(int32_t nscenes, hz_operator *const *ops, const hz_gmre
         (void)hipSetDevice(device);
         for (int idx = next++; idx < nscenes; idx = next++) {
             const int sc = order[idx];
-            status[sc] = guarded([&] { gmres_run(ops[sc], opts, b[sc], x[sc], nrhs[sc], ld[sc], stats[sc], ws); });
+            status[sc] = guarded([&] {
+                gmres_run(ops[sc], opts, b[sc], x[sc], nrhs[sc], ld[sc], stats[sc], ws);
+                // Bound the batch's memory to the scenes in flight: the solve
+                // has synchronised its stream, so its workspace can go.
+                HZ_HIP(hipStreamSynchronize(ws));
+                drop_graphs(ops[sc]);
+                release_workspace(ops[sc]);
+            });
             if (status[sc] != HZ_OK) error[sc] = g_error;
         }
     };
@@ -1496,6 +1508,15 @@ hz_status hz_autotune(hz_operator *op, int32_t nrhs, int32_t max_chunk, int32_t 
                 for (const auto &L : op->mg->levels)
                     for (int e = 0; e < 3 && at < shapes_len; ++e) shapes_out[at++] = L->shape[e];
         }
+    });
+}
+
+hz_status hz_release_workspace(hz_operator *op) {
+    return guarded([&] {
+        check_op(op);
+        HZ_HIP(hipDeviceSynchronize());
+        drop_graphs(op);
+        release_workspace(op);
     });
 }
 

@@ -215,6 +215,10 @@ class HelmholtzOperator:
     SLAB_SHAPES = ((4, 4, 1, 0), (2, 4, 1, 0), (4, 8, 1, 0), (2, 8, 1, 0),
                    (4, 2, 1, 0), (2, 4, 2, 0), (4, 4, 1, 1), (2, 4, 2, 1))
 
+    def release_workspace(self):
+        """Free the cached solver workspace (the next solve re-allocates)."""
+        _capi.check(_capi.lib().hz_release_workspace(self.handle), "hz_release_workspace")
+
     def set_tile_shapes(self, shapes):
         """Pin the slab tile shapes (the layout `autotune` reports)."""
         arr = (ctypes.c_int32*len(shapes))(*[int(v) for v in shapes])
