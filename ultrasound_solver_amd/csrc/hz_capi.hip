@@ -214,7 +214,7 @@ void launch_restrict(const MgLevel &L, const MgLevel &C, const cx<float> *r, int
 
 void launch_prolong(const MgLevel &L, const MgLevel &C, cx<float> *xf, int k, const int *live, const cx<float> *b,
                     const cx<float> *dinv, float w, const float *bdiv, int conj_d, hipStream_t s) {
-    const int nb = plane_blocks(L.nx, L.ny, L.nz, k);
+    const int nb = plane_blocks(C.nx, C.ny, C.nz, k);  // one thread per coarse cell
     hipLaunchKernelGGL((k_prolong<float>), dim3(nb), dim3(256), 0, s, (const cx<float> *)C.xx.as<cx<float>>(), C.nx,
                        C.ny, C.nz, xf, L.nx, L.ny, L.nz, k, live, L.v.soft, b, dinv, w, bdiv, conj_d, nb);
     HZ_LAUNCHED();

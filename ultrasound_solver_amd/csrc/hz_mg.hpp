@@ -276,72 +276,101 @@ __global__ __launch_bounds__(256) void k_restrict(const cx<T> *__restrict__ rf, 
     bc[r * nc + q] = acc;
 }
 
-// Fine iterate after the coarse correction, one thread per (fine node,
-// slot); grid: plane tiles of the fine grid times k slots, padded to a
-// multiple of 8 (nblocks).
+// Fine nodes of coarse cell c along one axis (fine size n, coarse size m):
+// cell c < m-1 spans fine [fine_of(c), fine_of(c+1)) (one or two nodes), the
+// last cell is the last fine node.  The first node of a cell is the coarse
+// node itself (weight 1 from corner c); a second node sits halfway (0.5 from
+// corners c and c+1).
+struct CellAxis {
+    int f0, count, c1;  // first fine node, fine nodes in the cell, second corner (clamped)
+};
+__device__ __forceinline__ CellAxis cell_axis(int c, int n, int m) {
+    CellAxis a;
+    a.f0 = fine_of(c, n);
+    a.count = c + 1 < m ? fine_of(c + 1, n) - a.f0 : 1;
+    a.c1 = min(c + 1, m - 1);
+    return a;
+}
+
+// Fine iterate after the coarse correction, one thread per (coarse cell,
+// slot): the cell's 8 coarse corners are loaded once and its (up to 2 x 2 x
+// 2) fine nodes written; grid: plane tiles of the coarse grid times k slots,
+// padded to a multiple of 8 (nblocks).
 //   b == nullptr: xf += P xc
 //   b != nullptr: xf = f b + P xc with f = w dinv / bdiv (dinv conjugated when
 //                 conj_d): the first Jacobi sweep from a zero guess is
 //                 recomputed from b instead of being stored and read back.
-// The 8 coarse taps always load (absent taps weight 0, valid address).  Fine
-// identity rows get no correction.
+// Coarse identity-row corners carry no correction (homogeneous Dirichlet for
+// the correction: for A^H their values are -A_IF^H x_I, scaled like a
+// residual, and must not leak into interior fine nodes), and fine identity
+// rows get none.
 template <typename T>
 __global__ __launch_bounds__(256) void k_prolong(const cx<T> *__restrict__ xc, int nxc, int nyc, int nzc,
                                                  cx<T> *__restrict__ xf, int nxf, int nyf, int nzf, int k,
                                                  const int *__restrict__ live, int soft,
                                                  const cx<T> *__restrict__ b, const cx<T> *__restrict__ dinv, T w,
                                                  const T *__restrict__ bdiv, int conj_d, int nblocks) {
-    const PlaneTile t = plane_tile(nxf, nyf, nzf, k, nblocks);
+    const PlaneTile t = plane_tile(nxc, nyc, nzc, k, nblocks);
     if (!t.in_grid || (live && !live[t.r])) return;
-    const int i = t.i, j = t.j, kk = t.k, r = t.r;
+    const int r = t.r;
     const int64_t nf = (int64_t)nxf * nyf * nzf;
     const int64_t nc = (int64_t)nxc * nyc * nzc;
-    const int64_t p = i + (int64_t)nxf * (j + (int64_t)nyf * kk);
-    const bool fixed = soft && on_soft_face(i, j, kk, nxf, nyf, nzf, soft);
-    // Per axis: two coarse indices and weights (the second has weight 0 on
-    // coarse nodes).  Coarse identity-row nodes carry no correction
-    // (homogeneous Dirichlet for the correction): for A^H their values are
-    // -A_IF^H x_I, scaled like a residual, and must not leak into interior
-    // fine nodes.
-    int ci[2], cj[2], ck[2];
-    T wi[2], wj[2], wk[2];
-    if (coarse_node(i, nxf)) { ci[0] = ci[1] = coarse_of(i, nxf, nxc); wi[0] = 1; wi[1] = 0; }
-    else { ci[0] = coarse_of(i - 1, nxf, nxc); ci[1] = coarse_of(i + 1, nxf, nxc); wi[0] = wi[1] = T(0.5); }
-    if (coarse_node(j, nyf)) { cj[0] = cj[1] = coarse_of(j, nyf, nyc); wj[0] = 1; wj[1] = 0; }
-    else { cj[0] = coarse_of(j - 1, nyf, nyc); cj[1] = coarse_of(j + 1, nyf, nyc); wj[0] = wj[1] = T(0.5); }
-    if (coarse_node(kk, nzf)) { ck[0] = ck[1] = coarse_of(kk, nzf, nzc); wk[0] = 1; wk[1] = 0; }
-    else { ck[0] = coarse_of(kk - 1, nzf, nzc); ck[1] = coarse_of(kk + 1, nzf, nzc); wk[0] = wk[1] = T(0.5); }
-    if (soft) {
-#pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            if (on_soft_face(ci[a], 0, 1, nxc, 3, 3, soft & 3)) wi[a] = 0;
-            if (on_soft_face(1, cj[a], 1, 3, nyc, 3, soft & 12)) wj[a] = 0;
-            if (on_soft_face(1, 1, ck[a], 3, 3, nzc, soft & 48)) wk[a] = 0;
-        }
-    }
+    const CellAxis ax = cell_axis(t.i, nxf, nxc), ay = cell_axis(t.j, nyf, nyc), az = cell_axis(t.k, nzf, nzc);
+    // Corner weights per axis: corner 0 is the cell's own coarse node.
+    T cw[2][3];  // [axis][corner 0/1 usable]
+    cw[0][0] = (soft && on_soft_face(t.i, 0, 1, nxc, 3, 3, soft & 3)) ? T(0) : T(1);
+    cw[0][1] = (soft && on_soft_face(ax.c1, 0, 1, nxc, 3, 3, soft & 3)) ? T(0) : T(1);
+    cw[1][0] = (soft && on_soft_face(1, t.j, 1, 3, nyc, 3, soft & 12)) ? T(0) : T(1);
+    cw[1][1] = (soft && on_soft_face(1, ay.c1, 1, 3, nyc, 3, soft & 12)) ? T(0) : T(1);
+    cw[2][0] = (soft && on_soft_face(1, 1, t.k, 3, 3, nzc, soft & 48)) ? T(0) : T(1);
+    cw[2][1] = (soft && on_soft_face(1, 1, az.c1, 3, 3, nzc, soft & 48)) ? T(0) : T(1);
     const cx<T> *src = xc + r * nc;
-    cx<T> val[8];
-    T wt[8];
+    const int cx0[2] = {t.i, ax.c1}, cy0[2] = {t.j, ay.c1}, cz0[2] = {t.k, az.c1};
+    cx<T> e[2][2][2];
 #pragma unroll
     for (int c = 0; c < 2; ++c)
 #pragma unroll
-        for (int e = 0; e < 2; ++e)
+        for (int bq = 0; bq < 2; ++bq)
 #pragma unroll
-            for (int a = 0; a < 2; ++a) {
-                const int u = (c * 2 + e) * 2 + a;
-                val[u] = src[ci[a] + (int64_t)nxc * (cj[e] + (int64_t)nyc * ck[c])];
-                wt[u] = fixed ? T(0) : wk[c] * wj[e] * wi[a];
+            for (int a = 0; a < 2; ++a)
+                e[c][bq][a] = src[cx0[a] + (int64_t)nxc * (cy0[bq] + (int64_t)nyc * cz0[c])];
+    // Weights of fine offset d (0: the coarse node, 1: the midpoint) from corner q.
+    auto wgt = [](int d, int q) -> T { return d == 0 ? (q == 0 ? T(1) : T(0)) : T(0.5); };
+#pragma unroll
+    for (int dz = 0; dz < 2; ++dz) {
+        if (dz >= az.count) break;
+        const int fk = az.f0 + dz;
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy) {
+            if (dy >= ay.count) break;
+            const int fj = ay.f0 + dy;
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx) {
+                if (dx >= ax.count) break;
+                const int fi = ax.f0 + dx;
+                const int64_t p = fi + (int64_t)nxf * (fj + (int64_t)nyf * fk);
+                cx<T> acc;
+                if (b) {
+                    const cx<T> d = conj_d ? conj(dinv[p]) : dinv[p];
+                    acc = ((bdiv ? w / bdiv[p] : w) * d) * b[r * nf + p];
+                } else {
+                    acc = xf[r * nf + p];
+                }
+                if (!(soft && on_soft_face(fi, fj, fk, nxf, nyf, nzf, soft))) {
+#pragma unroll
+                    for (int c = 0; c < 2; ++c)
+#pragma unroll
+                        for (int bq = 0; bq < 2; ++bq)
+#pragma unroll
+                            for (int a = 0; a < 2; ++a) {
+                                const T wt = wgt(dz, c) * cw[2][c] * wgt(dy, bq) * cw[1][bq] * wgt(dx, a) * cw[0][a];
+                                acc = acc + wt * e[c][bq][a];
+                            }
+                }
+                xf[r * nf + p] = acc;
             }
-    cx<T> acc;
-    if (b) {
-        const cx<T> d = conj_d ? conj(dinv[p]) : dinv[p];
-        acc = ((bdiv ? w / bdiv[p] : w) * d) * b[r * nf + p];
-    } else {
-        acc = xf[r * nf + p];
+        }
     }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) acc = acc + wt[u] * val[u];
-    xf[r * nf + p] = acc;
 }
 
 // x64 += cast<double>(z): apply a complex64 correction to the FP64 iterate.
