@@ -154,12 +154,26 @@ def main():
     alg_bytes = (16*k + 8)*op.n
     achieved = alg_bytes/stencil_s/1e9
     log(f"stencil k={k}: {stencil_s*1e6:.1f} us/launch, {achieved:.0f} GB/s algorithmic")
+    # Reference stream: a device-to-device copy of the same k columns (read +
+    # write 8 B per node and column), the rate a plain copy reaches here.
+    dst = torch.empty_like(x)
+    for _ in range(3):
+        dst.copy_(x)
+    e0.record(stream)
+    for _ in range(args.stencil_reps):
+        dst.copy_(x)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    copy_s = e0.elapsed_time(e1)/1e3/args.stencil_reps
+    copy_gbps = 16*k*op.n/copy_s/1e9
+    log(f"copy of {k} columns: {copy_s*1e6:.1f} us, {copy_gbps:.0f} GB/s")
+    del dst
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(scene, args, plain_iters)
-    zb, ty, rp, nt = HelmholtzOperator.SLAB_SHAPES[tuning["shapes"][0]]
-    traffic = measured_traffic(args, zb, ty, rp, nt)
+    zb, ty, rp, nt, flat = HelmholtzOperator.SLAB_SHAPES[tuning["shapes"][0]]
+    traffic = measured_traffic(args, zb, ty, rp, nt, flat)
 
     out = {
         "metric": "RHS solves/s at 128^3 to 1e-6 rel. residual (mixed vs FP64); stencil HBM GB/s",
@@ -177,10 +191,13 @@ def main():
         "refinements_median": float(np.median(refines)) if refines else None,
         "worst_scaled_residual": worst,
         "setup_s": setup_s,
-        "roofline": {"kernel": f"hz::k_slab<1, 0, {zb}, {ty}, {rp}, {nt}> (complex64 R A M apply, {k} columns)",
+        "roofline": {"kernel": f"hz::k_slab<1, 0, {zb}, {ty}, {rp}, {nt}, {flat}> (complex64 R A M apply, "
+                               f"{k} columns)",
                      "bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                      "frac": achieved/8000.0, "traffic": traffic, "alg_bytes_per_launch": alg_bytes,
-                     "launch_us": stencil_s*1e6},
+                     "launch_us": stencil_s*1e6,
+                     "copy_reference": {"what": f"device copy of the same {k} complex64 columns",
+                                        "achieved": copy_gbps, "frac": copy_gbps/8000.0}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -190,13 +207,13 @@ TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "stencil_128_k16_traffic.md")
 STENCIL_RHS = 16  # columns per stencil launch in the roofline measurement
 
 
-def measured_traffic(args, zb, ty, rp, nt):
+def measured_traffic(args, zb, ty, rp, nt, flat):
     """HBM bytes per stencil launch from the FETCH_SIZE/WRITE_SIZE counter runs
     of the same kernel, tile shape and workload (128^3, 16 columns), scaled by
     the copy calibration recorded beside them; None for other workloads."""
     if args.n != 128 or not os.path.exists(TRAFFIC_PROFILE):
         return None
-    key = f"traffic_bytes_per_launch[zb={zb},ty={ty},rp={rp},nt={nt}]:"
+    key = f"traffic_bytes_per_launch[zb={zb},ty={ty},rp={rp},nt={nt},flat={flat}]:"
     with open(TRAFFIC_PROFILE) as f:
         for line in f:
             if line.startswith(key):

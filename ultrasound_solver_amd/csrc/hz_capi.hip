@@ -198,17 +198,21 @@ void launch_update(dim3 grid, hipStream_t s, const cx<T> *V, int k, int64_t n, i
     HZ_LAUNCHED();
 }
 
-// Workgroups of the per-node multigrid transfers: one per 64 x 4 (x, y) tile,
-// z-plane and slot, padded to a multiple of 8 for the XCD remap.
+// Workgroups of the per-node multigrid transfers: one per 256 nodes of an
+// xy-plane, z-plane and slot, padded to a multiple of 8 for the XCD remap.
 int plane_blocks(int nx, int ny, int nz, int k) {
-    const int64_t t = (int64_t)((nx + 63) / 64) * ((ny + 3) / 4) * nz * k;
+    const int64_t t = (int64_t)((nx * ny + 255) / 256) * nz * k;
     return (int)((t + 7) / 8 * 8);
 }
 
 void launch_restrict(const MgLevel &L, const MgLevel &C, const cx<float> *r, int k, const int *live, hipStream_t s) {
     const int nb = plane_blocks(C.nx, C.ny, C.nz, k);
-    hipLaunchKernelGGL((k_restrict<float>), dim3(nb), dim3(256), 0, s, r, L.nx, L.ny, L.nz, C.bb.as<cx<float>>(), C.nx,
-                       C.ny, C.nz, k, live, L.v.soft, nb);
+    if (L.nx % 2 == 0)
+        hipLaunchKernelGGL((k_restrict<float, 1>), dim3(nb), dim3(256), 0, s, r, L.nx, L.ny, L.nz,
+                           C.bb.as<cx<float>>(), C.nx, C.ny, C.nz, k, live, L.v.soft, nb);
+    else
+        hipLaunchKernelGGL((k_restrict<float, 0>), dim3(nb), dim3(256), 0, s, r, L.nx, L.ny, L.nz,
+                           C.bb.as<cx<float>>(), C.nx, C.ny, C.nz, k, live, L.v.soft, nb);
     HZ_LAUNCHED();
 }
 
@@ -227,23 +231,26 @@ void launch_slab(int shape, const OpView<float> &v, const cx<float> *x, int64_t 
                  const float *bdiv, hipStream_t s) {
     const TileMap tm = slab_map(v.nx, v.ny, v.nz, shape);
     const dim3 grid(tm.nblocks), block(64 * SLAB_SHAPE[shape].ty);
-#define HZ_SLAB_CASE(ID, ZB, TY, RP, NT)                                                                           \
+#define HZ_SLAB_CASE(ID, ZB, TY, RP, NT, FLAT)                                                                     \
     case ID:                                                                                                       \
         static_assert(SLAB_SHAPE[ID].zb == ZB && SLAB_SHAPE[ID].ty == TY && SLAB_SHAPE[ID].rp == RP &&             \
-                          SLAB_SHAPE[ID].nt == NT,                                                                 \
+                          SLAB_SHAPE[ID].nt == NT && SLAB_SHAPE[ID].flat == FLAT,                                  \
                       "slab shape table");                                                                         \
-        hipLaunchKernelGGL((k_slab<MODE, EPI, ZB, TY, RP, NT>), grid, block, 0, s, v, tm, x, ldx, y, ldy, k, live,   \
-                           scale, b, dinv, w, bdiv);                                                               \
+        hipLaunchKernelGGL((k_slab<MODE, EPI, ZB, TY, RP, NT, FLAT>), grid, block, 0, s, v, tm, x, ldx, y, ldy, k,   \
+                           live, scale, b, dinv, w, bdiv);                                                         \
         break;
     switch (shape) {
-        HZ_SLAB_CASE(0, 4, 4, 1, 0)
-        HZ_SLAB_CASE(1, 2, 4, 1, 0)
-        HZ_SLAB_CASE(2, 4, 8, 1, 0)
-        HZ_SLAB_CASE(3, 2, 8, 1, 0)
-        HZ_SLAB_CASE(4, 4, 2, 1, 0)
-        HZ_SLAB_CASE(5, 2, 4, 2, 0)
-        HZ_SLAB_CASE(6, 4, 4, 1, 1)
-        HZ_SLAB_CASE(7, 2, 4, 2, 1)
+        HZ_SLAB_CASE(0, 4, 4, 1, 0, 0)
+        HZ_SLAB_CASE(1, 2, 4, 1, 0, 0)
+        HZ_SLAB_CASE(2, 4, 8, 1, 0, 0)
+        HZ_SLAB_CASE(3, 2, 8, 1, 0, 0)
+        HZ_SLAB_CASE(4, 4, 2, 1, 0, 0)
+        HZ_SLAB_CASE(5, 2, 4, 2, 0, 0)
+        HZ_SLAB_CASE(6, 4, 4, 1, 1, 0)
+        HZ_SLAB_CASE(7, 2, 4, 2, 1, 0)
+        HZ_SLAB_CASE(8, 4, 4, 1, 1, 1)
+        HZ_SLAB_CASE(9, 2, 4, 1, 1, 1)
+        HZ_SLAB_CASE(10, 2, 4, 2, 1, 1)
         default:
             throw BadArgument("unknown slab tile shape");
     }
@@ -875,8 +882,9 @@ int pick_shape(F &&launch, hipStream_t s, int preferred = 0) {
     for (int sh = 0; sh < SLAB_SHAPES; ++sh) {
         const float ms = sh == preferred ? best_ms : time_launches([&] { launch(sh); }, 5, s);
         if (verbose())
-            std::fprintf(stderr, "[hz] autotune shape %d (zb %d, ty %d, rp %d, nt %d): %.1f us\n", sh,
-                         SLAB_SHAPE[sh].zb, SLAB_SHAPE[sh].ty, SLAB_SHAPE[sh].rp, SLAB_SHAPE[sh].nt, ms * 1e3);
+            std::fprintf(stderr, "[hz] autotune shape %d (zb %d, ty %d, rp %d, nt %d, flat %d): %.1f us\n", sh,
+                         SLAB_SHAPE[sh].zb, SLAB_SHAPE[sh].ty, SLAB_SHAPE[sh].rp, SLAB_SHAPE[sh].nt,
+                         SLAB_SHAPE[sh].flat, ms * 1e3);
         if (ms < 0.97f * best_ms) {
             best = sh;
             best_ms = ms;

@@ -110,4 +110,9 @@ __device__ __forceinline__ double block_sum_256(double v, double *scratch) {
     return r;
 }
 
+// Value of lane l-1 (wave_shr:1) / lane l+1 (wave_shl:1); lanes at the wave
+// edge receive 0 and are fixed up by an explicit load.
+__device__ __forceinline__ float lane_prev(float v) { return __builtin_amdgcn_update_dpp(0.0f, v, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ float lane_next(float v) { return __builtin_amdgcn_update_dpp(0.0f, v, 0x130, 0xf, 0xf, false); }
+
 }  // namespace hz

@@ -194,51 +194,58 @@ __global__ __launch_bounds__(256) void k_coarse_jacobi(OpView<T> op, const cx<T>
     }
 }
 
-// Workgroup -> (64 x 4 tile, z-plane, right-hand side) of an nx x ny x nz
-// grid with k slots: the slot is the slowest index and consecutive planes of
-// one slot land on one XCD, so the rows that neighbouring tiles share stay in
-// that XCD's L2.
+// Workgroup -> (256 consecutive nodes of one xy-plane, z-plane, slot) of an
+// nx x ny x nz grid with k slots.  Flattening the plane leaves no idle lanes
+// on awkward widths (the coarse levels are 65, 33, 17 ... wide).  The slot is
+// the slowest index and consecutive planes of one slot land on one XCD, so
+// the rows that neighbouring workgroups share stay in that XCD's L2.  Lanes
+// past the plane's end are clamped to its last node and flagged.
 struct PlaneTile {
     int i, j, k, r;
     bool in_grid;
 };
 __device__ __forceinline__ PlaneTile plane_tile(int nx, int ny, int nz, int nslots, int nblocks) {
     const int tile = xcd_tile(blockIdx.x, nblocks);
-    const int tx = (nx + 63) >> 6, ty = (ny + 3) >> 2;
-    const int per_slot = tx * ty * nz;
+    const int nxy = nx * ny;
+    const int per_plane = (nxy + 255) >> 8;
+    const int per_slot = per_plane * nz;
     PlaneTile t;
     t.r = tile / per_slot;
     const int rest = tile - t.r * per_slot;
-    t.k = rest / (tx * ty);
-    const int txy = rest - t.k * tx * ty;
-    t.i = (txy % tx) * 64 + (threadIdx.x & 63);
-    t.j = (txy / tx) * 4 + (threadIdx.x >> 6);
-    t.in_grid = t.r < nslots && t.i < nx && t.j < ny;
+    t.k = rest / per_plane;
+    const int pxy_raw = (rest - t.k * per_plane) * 256 + (int)threadIdx.x;
+    const int pxy = min(pxy_raw, nxy - 1);
+    t.j = pxy / nx;
+    t.i = pxy - t.j * nx;
+    t.in_grid = t.r < nslots && pxy_raw < nxy;
     return t;
 }
 
 // Coarse right-hand side bc = P^T rf, one thread per (coarse node, slot);
-// grid: plane tiles of the coarse grid times k slots, padded to a multiple
-// of 8 (nblocks).  The 27 fine taps get separable weights; absent taps get
-// weight 0 and a valid address, so every thread issues the same 27 loads.
-// Identity rows (pressure-release faces) take no part in the coarse
-// correction: their fine residuals are not restricted and coarse identity
-// rows get zero, so a residual-scaled value never lands in a solution-scaled
-// identity row.
-template <typename T>
+// grid: flattened plane tiles of the coarse grid times k slots, padded to a
+// multiple of 8 (nblocks).  Along x the taps are fine 2i-1, 2i, 2i+1 with weights 1/2, 1,
+// 1/2 (0 where absent or a coarse node); along y and z likewise.  PAIR (even
+// fine x-size): each lane loads its (2i, 2i+1) pair with one 16-byte access
+// and takes 2i-1 from the previous lane (DPP), lane 0 loading it itself:
+// 9 loads per slot instead of 27.  Lanes past the plane's end run on clamped
+// indices (they feed the shifts) and do not store.  Identity rows
+// (pressure-release faces) take no part in the coarse correction: their
+// fine residuals are not restricted and coarse identity rows get zero, so a
+// residual-scaled value never lands in a solution-scaled identity row.
+template <typename T, int PAIR>
 __global__ __launch_bounds__(256) void k_restrict(const cx<T> *__restrict__ rf, int nxf, int nyf, int nzf,
                                                   cx<T> *__restrict__ bc, int nxc, int nyc, int nzc, int k,
                                                   const int *__restrict__ live, int soft, int nblocks) {
     const PlaneTile t = plane_tile(nxc, nyc, nzc, k, nblocks);
-    if (!t.in_grid || (live && !live[t.r])) return;
+    // Slot and plane are uniform over the workgroup.
+    if (t.r >= k || (live && !live[t.r])) return;
+    const bool valid = t.in_grid;
     const int ic = t.i, jc = t.j, kc = t.k, r = t.r;
+    const int lane = threadIdx.x & 63;
     const int64_t nc = (int64_t)nxc * nyc * nzc;
     const int64_t nf = (int64_t)nxf * nyf * nzf;
     const int64_t q = ic + (int64_t)nxc * (jc + (int64_t)nyc * kc);
-    if (soft && on_soft_face(ic, jc, kc, nxc, nyc, nzc, soft)) {
-        bc[r * nc + q] = mk<T>(0, 0);
-        return;
-    }
+    const bool fixed = soft && on_soft_face(ic, jc, kc, nxc, nyc, nzc, soft);
     const int fi = fine_of(ic, nxf), fj = fine_of(jc, nyf), fk = fine_of(kc, nzf);
     T wx[3], wy[3], wz[3];
     int ox[3], oy[3], oz[3];
@@ -256,24 +263,38 @@ __global__ __launch_bounds__(256) void k_restrict(const cx<T> *__restrict__ rf, 
         oz[d] = min(max(fk + d - 1, 0), nzf - 1);
     }
     const cx<T> *src = rf + r * nf;
-    cx<T> val[27];
-    T wt[27];
+    cx<T> acc = mk<T>(0, 0);
 #pragma unroll
     for (int dz = 0; dz < 3; ++dz)
 #pragma unroll
-        for (int dy = 0; dy < 3; ++dy)
+        for (int dy = 0; dy < 3; ++dy) {
+            const int64_t row = (int64_t)nxf * (oy[dy] + (int64_t)nyf * oz[dz]);
+            cx<T> v[3];
+            if constexpr (PAIR) {
+                static_assert(sizeof(T) == 4, "paired loads are for complex64");
+                // Pair (2i, 2i+1); the last coarse node of an even grid is fine
+                // node nxf-1, the second element of the pair (nxf-2, nxf-1).
+                const int pidx = min(2 * ic, nxf - 2);
+                const float4 pr = *reinterpret_cast<const float4 *>(src + row + pidx);
+                const bool first = 2 * ic <= nxf - 2;
+                v[1] = first ? mk<float>(pr.x, pr.y) : mk<float>(pr.z, pr.w);
+                v[2] = first ? mk<float>(pr.z, pr.w) : mk<float>(0, 0);
+                // The previous lane holds x-1 of the same row except in lane 0
+                // and at a row start (where the left tap has weight 0).
+                v[0] = mk<float>(lane_prev(pr.z), lane_prev(pr.w));
+                if (lane == 0 || ic == 0) v[0] = src[row + ox[0]];
+            } else {
+#pragma unroll
+                for (int dx = 0; dx < 3; ++dx) v[dx] = src[row + ox[dx]];
+            }
 #pragma unroll
             for (int dx = 0; dx < 3; ++dx) {
-                const int e = (dz * 3 + dy) * 3 + dx;
-                val[e] = src[ox[dx] + (int64_t)nxf * (oy[dy] + (int64_t)nyf * oz[dz])];
                 T wv = wz[dz] * wy[dy] * wx[dx];
                 if (soft && on_soft_face(ox[dx], oy[dy], oz[dz], nxf, nyf, nzf, soft)) wv = T(0);
-                wt[e] = wv;
+                acc = acc + wv * v[dx];
             }
-    cx<T> acc = mk<T>(0, 0);
-#pragma unroll
-    for (int e = 0; e < 27; ++e) acc = acc + wt[e] * val[e];
-    bc[r * nc + q] = acc;
+        }
+    if (valid) bc[r * nc + q] = fixed ? mk<T>(0, 0) : acc;
 }
 
 // Fine nodes of coarse cell c along one axis (fine size n, coarse size m):
@@ -294,8 +315,8 @@ __device__ __forceinline__ CellAxis cell_axis(int c, int n, int m) {
 
 // Fine iterate after the coarse correction, one thread per (coarse cell,
 // slot): the cell's 8 coarse corners are loaded once and its (up to 2 x 2 x
-// 2) fine nodes written; grid: plane tiles of the coarse grid times k slots,
-// padded to a multiple of 8 (nblocks).
+// 2) fine nodes written; grid: flattened plane tiles of the coarse grid times
+// k slots, padded to a multiple of 8 (nblocks).
 //   b == nullptr: xf += P xc
 //   b != nullptr: xf = f b + P xc with f = w dinv / bdiv (dinv conjugated when
 //                 conj_d): the first Jacobi sweep from a zero guess is
@@ -317,7 +338,7 @@ __global__ __launch_bounds__(256) void k_prolong(const cx<T> *__restrict__ xc, i
     const int64_t nc = (int64_t)nxc * nyc * nzc;
     const CellAxis ax = cell_axis(t.i, nxf, nxc), ay = cell_axis(t.j, nyf, nyc), az = cell_axis(t.k, nzf, nzc);
     // Corner weights per axis: corner 0 is the cell's own coarse node.
-    T cw[2][3];  // [axis][corner 0/1 usable]
+    T cw[3][2];  // [axis][corner]: 0 for a coarse identity-row corner
     cw[0][0] = (soft && on_soft_face(t.i, 0, 1, nxc, 3, 3, soft & 3)) ? T(0) : T(1);
     cw[0][1] = (soft && on_soft_face(ax.c1, 0, 1, nxc, 3, 3, soft & 3)) ? T(0) : T(1);
     cw[1][0] = (soft && on_soft_face(1, t.j, 1, 3, nyc, 3, soft & 12)) ? T(0) : T(1);
@@ -338,15 +359,13 @@ __global__ __launch_bounds__(256) void k_prolong(const cx<T> *__restrict__ xc, i
     auto wgt = [](int d, int q) -> T { return d == 0 ? (q == 0 ? T(1) : T(0)) : T(0.5); };
 #pragma unroll
     for (int dz = 0; dz < 2; ++dz) {
-        if (dz >= az.count) break;
         const int fk = az.f0 + dz;
 #pragma unroll
         for (int dy = 0; dy < 2; ++dy) {
-            if (dy >= ay.count) break;
             const int fj = ay.f0 + dy;
 #pragma unroll
             for (int dx = 0; dx < 2; ++dx) {
-                if (dx >= ax.count) break;
+                if (dz >= az.count || dy >= ay.count || dx >= ax.count) continue;
                 const int fi = ax.f0 + dx;
                 const int64_t p = fi + (int64_t)nxf * (fj + (int64_t)nyf * fk);
                 cx<T> acc;

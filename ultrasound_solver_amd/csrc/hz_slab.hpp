@@ -17,20 +17,24 @@
 namespace hz {
 
 // Tile shapes the autotuner chooses from: z-planes per thread, tile rows,
-// right-hand sides per pass (loads of all of them issued together) and
-// non-temporal output stores.
+// right-hand sides per pass (loads of all of them issued together),
+// non-temporal output stores, and flat tiles (64 * ty consecutive nodes of
+// an xy-plane instead of a 64 x ty rectangle: no idle lanes on widths just
+// above a multiple of 64, such as the 65-wide first coarse level).
 struct SlabShape {
-    int zb, ty, rp, nt;
+    int zb, ty, rp, nt, flat;
 };
-constexpr int SLAB_SHAPES = 8;
-constexpr SlabShape SLAB_SHAPE[SLAB_SHAPES] = {{4, 4, 1, 0}, {2, 4, 1, 0}, {4, 8, 1, 0}, {2, 8, 1, 0},
-                                               {4, 2, 1, 0}, {2, 4, 2, 0}, {4, 4, 1, 1}, {2, 4, 2, 1}};
+constexpr int SLAB_SHAPES = 11;
+constexpr SlabShape SLAB_SHAPE[SLAB_SHAPES] = {
+    {4, 4, 1, 0, 0}, {2, 4, 1, 0, 0}, {4, 8, 1, 0, 0}, {2, 8, 1, 0, 0}, {4, 2, 1, 0, 0}, {2, 4, 2, 0, 0},
+    {4, 4, 1, 1, 0}, {2, 4, 2, 1, 0}, {4, 4, 1, 1, 1}, {2, 4, 1, 1, 1}, {2, 4, 2, 1, 1}};
 
 __host__ inline TileMap slab_map(int nx, int ny, int nz, int shape) {
     const SlabShape sh = SLAB_SHAPE[shape];
     TileMap t;
-    t.tiles_x = (nx + 63) / 64;
-    t.tiles_y = (ny + sh.ty - 1) / sh.ty;
+    // Flat tiles: tiles_x = 1 and tiles_y counts 64*ty-node pieces of a plane.
+    t.tiles_x = sh.flat ? 1 : (nx + 63) / 64;
+    t.tiles_y = sh.flat ? (nx * ny + 64 * sh.ty - 1) / (64 * sh.ty) : (ny + sh.ty - 1) / sh.ty;
     t.zb = sh.zb;
     t.tiles_z = (nz + sh.zb - 1) / sh.zb;
     t.ntiles = t.tiles_x * t.tiles_y * t.tiles_z;
@@ -40,10 +44,6 @@ __host__ inline TileMap slab_map(int nx, int ny, int nz, int shape) {
 
 enum SlabEpi { SLAB_APPLY = 0, SLAB_SMOOTH = 1, SLAB_RESID = 2, SLAB_RESID0 = 3 };
 
-// Value of lane l-1 (wave_shr:1) / lane l+1 (wave_shl:1); lanes at the wave
-// edge receive 0 and are fixed up by an explicit load.
-__device__ __forceinline__ float lane_prev(float v) { return __builtin_amdgcn_update_dpp(0.0f, v, 0x138, 0xf, 0xf, false); }
-__device__ __forceinline__ float lane_next(float v) { return __builtin_amdgcn_update_dpp(0.0f, v, 0x130, 0xf, 0xf, false); }
 
 // Multiplier of the first damped-Jacobi sweep from a zero guess,
 // x0 = f * b with f = w * dinv / bdiv (dinv conjugated for the adjoint).
@@ -77,7 +77,7 @@ __device__ __forceinline__ void store_out(cx<float> *dst, cx<float> v) {
     }
 }
 
-template <int MODE, int EPI, int ZB, int TY, int RP, int NT>
+template <int MODE, int EPI, int ZB, int TY, int RP, int NT, int FLAT>
 __global__ __launch_bounds__(64 * TY) void k_slab(OpView<float> op, TileMap tm, const cx<float> *__restrict__ xin,
                                               int64_t ldx, cx<float> *__restrict__ yout, int64_t ldy, int nrhs,
                                               const int *__restrict__ live, const double *__restrict__ scale,
@@ -103,13 +103,26 @@ __global__ __launch_bounds__(64 * TY) void k_slab(OpView<float> op, TileMap tm, 
     const int ty = (tile / tm.tiles_x) % tm.tiles_y;
     const int tz = tile / (tm.tiles_x * tm.tiles_y);
     const int lane = threadIdx.x & 63;
-    const int i_raw = tx * 64 + lane;
-    const int j_raw = ty * TY + (threadIdx.x >> 6);
     // Lanes past the grid edge keep running (they feed the DPP shifts) on a
-    // clamped node and never store.
-    const bool valid = i_raw < op.nx && j_raw < op.ny;
-    const int i = min(i_raw, op.nx - 1);
-    const int j = min(j_raw, op.ny - 1);
+    // clamped node and never store.  In a flat tile the previous/next lane is
+    // x-1/x+1 of the same row except at a row end, where the coefficient to
+    // the absent neighbour is zero and any finite value may stand in.
+    bool valid;
+    int i, j;
+    if (FLAT) {
+        const int nxy = op.nx * op.ny;
+        const int raw = ty * (64 * TY) + (int)threadIdx.x;
+        const int pxy = min(raw, nxy - 1);
+        valid = raw < nxy;
+        j = pxy / op.nx;
+        i = pxy - j * op.nx;
+    } else {
+        const int i_raw = tx * 64 + lane;
+        const int j_raw = ty * TY + (threadIdx.x >> 6);
+        valid = i_raw < op.nx && j_raw < op.ny;
+        i = min(i_raw, op.nx - 1);
+        j = min(j_raw, op.ny - 1);
+    }
     const int k0 = tz * tm.zb;
     const int nzl = min(op.nz - k0, ZB);
     const int64_t sxy = (int64_t)op.nx * op.ny;

@@ -211,9 +211,10 @@ class HelmholtzOperator:
         return self.tuning
 
     #: Slab tile shapes by index (z-planes per thread, tile rows, right-hand
-    #: sides per pass, non-temporal stores), as in the library's table.
-    SLAB_SHAPES = ((4, 4, 1, 0), (2, 4, 1, 0), (4, 8, 1, 0), (2, 8, 1, 0),
-                   (4, 2, 1, 0), (2, 4, 2, 0), (4, 4, 1, 1), (2, 4, 2, 1))
+    #: sides per pass, non-temporal stores, flat tiles), as in the library's table.
+    SLAB_SHAPES = ((4, 4, 1, 0, 0), (2, 4, 1, 0, 0), (4, 8, 1, 0, 0), (2, 8, 1, 0, 0), (4, 2, 1, 0, 0),
+                   (2, 4, 2, 0, 0), (4, 4, 1, 1, 0), (2, 4, 2, 1, 0), (4, 4, 1, 1, 1), (2, 4, 1, 1, 1),
+                   (2, 4, 2, 1, 1))
 
     def release_workspace(self):
         """Free the cached solver workspace (the next solve re-allocates)."""
