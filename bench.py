@@ -41,7 +41,7 @@ def parse():
                    help="only the stencil launches and a 512 MiB copy (for hardware counter runs)")
     p.add_argument("--mg-shift", type=float, default=1.2)
     p.add_argument("--mg-pre", type=int, default=1)
-    p.add_argument("--mg-post", type=int, default=1)
+    p.add_argument("--mg-post", type=int, default=3)
     p.add_argument("--mg-weight", type=float, default=0.7)
     p.add_argument("--graph", type=int, default=0, help="replay each GMRES cycle from a captured hipGraph")
     p.add_argument("--workload", choices=["g128", "uneven", "large"], default="g128",

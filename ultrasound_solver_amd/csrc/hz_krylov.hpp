@@ -129,6 +129,20 @@ __global__ __launch_bounds__(256) void k_update(const cx<T> *__restrict__ V, int
         for (int64_t p = (p0 >> 1) + threadIdx.x; p < (p1 >> 1); p += 256) {
             float4 acc = t4[p];
             int q = 0;
+            for (; q + 8 <= cnt; q += 8) {
+                float4 v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    v[u] = reinterpret_cast<const float4 *>(V + ((int64_t)(q + u) * k + r) * n)[p];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const cx<float> c = cf[q + u];
+                    acc.x -= c.re * v[u].x - c.im * v[u].y;
+                    acc.y -= c.re * v[u].y + c.im * v[u].x;
+                    acc.z -= c.re * v[u].z - c.im * v[u].w;
+                    acc.w -= c.re * v[u].w + c.im * v[u].z;
+                }
+            }
             for (; q + 4 <= cnt; q += 4) {
                 float4 v[4];
 #pragma unroll

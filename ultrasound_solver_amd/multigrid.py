@@ -116,7 +116,7 @@ class Hierarchy:
             out.append(np.ascontiguousarray((1j*op.omega*area*adm).ravel(order="F"), dtype=np.complex128))
         return out
 
-    def install(self, op, shift=1.2, pre=1, post=1, coarse_sweeps=30, weight=0.7):
+    def install(self, op, shift=1.2, pre=1, post=3, coarse_sweeps=30, weight=0.7):
         keep = []
 
         def ptr(a):
