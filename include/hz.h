@@ -207,6 +207,15 @@ hz_status hz_autotune(hz_operator *op, int32_t nrhs, int32_t max_chunk, int32_t 
                       int32_t preconditioner, int64_t memory_budget, int32_t *chunk_out,
                       int32_t *shapes_out, int32_t shapes_len, void *stream);
 
+/* Material gradient, replacing ConservativeHelmholtz.material_vjp
+ * (src/algorithm/physics/conservative_helmholtz.py:151-175): for
+ * dL = -Re(lambda^H dA u) writes dL/drho and dL/dc per node (FP64, N each)
+ * with lambda zeroed on identity rows.  u and lambda hold nrhs complex128
+ * columns (leading dimension ld >= N); the gradients of the nrhs pairs are
+ * summed (nrhs = 1 is the reference call). */
+hz_status hz_material_vjp(hz_operator *op, const void *u, const void *lambda, int32_t nrhs, int64_t ld,
+                          double *g_rho, double *g_c, void *stream);
+
 /* Free the cached solver workspace (Krylov basis, multigrid level vectors,
  * captured graphs); the next solve allocates what it needs.  Waits for the
  * device first.  No counterpart in the reference (CuPy's pool frees on its

@@ -312,3 +312,27 @@ def test_scene_batch_matches_sequential_solves():
     plain = HelmholtzOperator(rho, c, 1.07e-3, 2*np.pi*40e3, bnd, pml, device="cuda:0")  # no hierarchy
     with pytest.raises(RuntimeError, match="scene 1"):
         gmres_batch([ops[0], plain], [bs[0], _point_rhs(plain.n, 1, 5)], **kw)
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_material_vjp_matches_reference_fixture(fix, name):
+    """hz_material_vjp against the gradients the reference's own
+    ConservativeHelmholtz.material_vjp wrote for the fixture scenes (every
+    face kind, PML, soft rows), to FP64 rounding; two column pairs sum."""
+    f, meta = fix
+    op, _ = _op(f, meta, name)
+    u = f[f"{name}/vjp_u"].ravel(order="F")
+    lam = f[f"{name}/vjp_lambda"].ravel(order="F")
+    g_rho, g_c = op.material_vjp(u, lam)
+    ref_rho = f[f"{name}/grad_rho"].reshape(op.shape, order="F")
+    ref_c = f[f"{name}/grad_c"].reshape(op.shape, order="F")
+    assert _rel(g_rho.cpu().numpy(), ref_rho) < 1e-12
+    assert _rel(g_c.cpu().numpy(), ref_c) < 1e-12
+    # Two pairs give the sum of their gradients.
+    u2, lam2 = np.stack([u, 2*u]), np.stack([lam, 1j*lam])
+    g2_rho, g2_c = op.material_vjp(u2, lam2)
+    h_rho, h_c = op.material_vjp(2*u, 1j*lam)
+    np.testing.assert_allclose(g2_rho.cpu().numpy(), (g_rho + h_rho).cpu().numpy(), rtol=0,
+                               atol=1e-12*np.abs(ref_rho).max())
+    np.testing.assert_allclose(g2_c.cpu().numpy(), (g_c + h_c).cpu().numpy(), rtol=0,
+                               atol=1e-12*np.abs(ref_c).max())

@@ -16,6 +16,7 @@
 #include <atomic>
 
 #include "../../include/hz.h"
+#include "hz_grad.hpp"
 #include "hz_krylov.hpp"
 #include "hz_mg.hpp"
 #include "hz_slab.hpp"
@@ -115,6 +116,7 @@ struct hz_operator {
     int64_t n = 0;
     double dx = 0, omega = 0;
     int soft = 0;
+    int open_mask = 0;  // bit f: face f is open (radiating)
     DevBuf rho64, kap64, rho32, kap32;
     DevBuf a64[3], q64[3], a32[3], q32[3];
     DevBuf face64[6], face32[6];
@@ -231,10 +233,18 @@ void launch_slab(int shape, const OpView<float> &v, const cx<float> *x, int64_t 
                  const float *bdiv, hipStream_t s) {
     const TileMap tm = slab_map(v.nx, v.ny, v.nz, shape);
     const dim3 grid(tm.nblocks), block(64 * SLAB_SHAPE[shape].ty);
+#define HZ_MARCH_CASE(ID, ZC, TY, RP, NT)                                                                          \
+    case ID:                                                                                                       \
+        static_assert(SLAB_SHAPE[ID].zb == ZC && SLAB_SHAPE[ID].ty == TY && SLAB_SHAPE[ID].rp == RP &&             \
+                          SLAB_SHAPE[ID].nt == NT && SLAB_SHAPE[ID].march == 1,                                    \
+                      "slab shape table");                                                                         \
+        hipLaunchKernelGGL((k_march<MODE, EPI, ZC, TY, RP, NT>), grid, block, 0, s, v, tm, x, ldx, y, ldy, k, live,  \
+                           scale, b, dinv, w, bdiv);                                                               \
+        break;
 #define HZ_SLAB_CASE(ID, ZB, TY, RP, NT, FLAT)                                                                     \
     case ID:                                                                                                       \
         static_assert(SLAB_SHAPE[ID].zb == ZB && SLAB_SHAPE[ID].ty == TY && SLAB_SHAPE[ID].rp == RP &&             \
-                          SLAB_SHAPE[ID].nt == NT && SLAB_SHAPE[ID].flat == FLAT,                                  \
+                          SLAB_SHAPE[ID].nt == NT && SLAB_SHAPE[ID].flat == FLAT && SLAB_SHAPE[ID].march == 0,     \
                       "slab shape table");                                                                         \
         hipLaunchKernelGGL((k_slab<MODE, EPI, ZB, TY, RP, NT, FLAT>), grid, block, 0, s, v, tm, x, ldx, y, ldy, k,   \
                            live, scale, b, dinv, w, bdiv);                                                         \
@@ -251,10 +261,15 @@ void launch_slab(int shape, const OpView<float> &v, const cx<float> *x, int64_t 
         HZ_SLAB_CASE(8, 4, 4, 1, 1, 1)
         HZ_SLAB_CASE(9, 2, 4, 1, 1, 1)
         HZ_SLAB_CASE(10, 2, 4, 2, 1, 1)
+        HZ_MARCH_CASE(11, 8, 4, 8, 1)
+        HZ_MARCH_CASE(12, 16, 4, 8, 1)
+        HZ_MARCH_CASE(13, 8, 2, 8, 1)
+        HZ_MARCH_CASE(14, 8, 4, 4, 1)
         default:
             throw BadArgument("unknown slab tile shape");
     }
 #undef HZ_SLAB_CASE
+#undef HZ_MARCH_CASE
     HZ_LAUNCHED();
 }
 
@@ -882,9 +897,9 @@ int pick_shape(F &&launch, hipStream_t s, int preferred = 0) {
     for (int sh = 0; sh < SLAB_SHAPES; ++sh) {
         const float ms = sh == preferred ? best_ms : time_launches([&] { launch(sh); }, 5, s);
         if (verbose())
-            std::fprintf(stderr, "[hz] autotune shape %d (zb %d, ty %d, rp %d, nt %d, flat %d): %.1f us\n", sh,
-                         SLAB_SHAPE[sh].zb, SLAB_SHAPE[sh].ty, SLAB_SHAPE[sh].rp, SLAB_SHAPE[sh].nt,
-                         SLAB_SHAPE[sh].flat, ms * 1e3);
+            std::fprintf(stderr, "[hz] autotune shape %d (zb %d, ty %d, rp %d, nt %d, flat %d, march %d): %.1f us\n",
+                         sh, SLAB_SHAPE[sh].zb, SLAB_SHAPE[sh].ty, SLAB_SHAPE[sh].rp, SLAB_SHAPE[sh].nt,
+                         SLAB_SHAPE[sh].flat, SLAB_SHAPE[sh].march, ms * 1e3);
         if (ms < 0.97f * best_ms) {
             best = sh;
             best_ms = ms;
@@ -1001,6 +1016,7 @@ hz_status hz_create(const hz_grid_desc *d, hz_operator **out) {
             if (d->face_kind[f] < HZ_FACE_REFLECTING || d->face_kind[f] > HZ_FACE_IMPEDANCE)
                 throw BadArgument("unknown face kind");
             if (d->face_kind[f] == HZ_FACE_SOFT) op->soft |= 1 << f;
+            if (d->face_kind[f] == HZ_FACE_OPEN) op->open_mask |= 1 << f;
             const bool needs = d->face_kind[f] == HZ_FACE_OPEN || d->face_kind[f] == HZ_FACE_IMPEDANCE;
             if (needs && !d->face_term[f]) throw BadArgument("open/impedance face needs a face-term table");
         }
@@ -1516,6 +1532,20 @@ hz_status hz_autotune(hz_operator *op, int32_t nrhs, int32_t max_chunk, int32_t 
                 for (const auto &L : op->mg->levels)
                     for (int e = 0; e < 3 && at < shapes_len; ++e) shapes_out[at++] = L->shape[e];
         }
+    });
+}
+
+hz_status hz_material_vjp(hz_operator *op, const void *u, const void *lambda, int32_t nrhs, int64_t ld, double *g_rho,
+                          double *g_c, void *stream) {
+    return guarded([&] {
+        check_op(op);
+        if (!u || !lambda || !g_rho || !g_c) throw BadArgument("u, lambda, g_rho and g_c must be device pointers");
+        if (nrhs < 1 || ld < op->n) throw BadArgument("need nrhs >= 1 and ld >= N");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        hipLaunchKernelGGL(k_material_vjp, dim3(blocks_for(op->n, 256, 65536)), dim3(256), 0, s, op->v64,
+                           op->open_mask, static_cast<const cx<double> *>(u), static_cast<const cx<double> *>(lambda),
+                           ld, (int)nrhs, g_rho, g_c);
+        HZ_LAUNCHED();
     });
 }
 

@@ -369,9 +369,12 @@ class HelmholtzSolver:
         return self._reshape_solution(x[:, 0] if squeeze else x)
 
     def material_vjp(self, field, field_cotangent):
-        """dL/drho, dL/dc with one adjoint solve (conservative_helmholtz.py:151-175)."""
+        """dL/drho, dL/dc (grid arrays) with one adjoint solve and the device
+        material-gradient kernel (conservative_helmholtz.py:151-175)."""
         lam = self.solve_adjoint(field_cotangent)
-        return material_vjp(self.operator, field, lam)
+        u = np.asarray(field).reshape(self.shape, order="F").ravel(order="F")
+        g_rho, g_c = self.operator.material_vjp(u, np.asarray(lam).reshape(self.shape, order="F").ravel(order="F"))
+        return g_rho.cpu().numpy(), g_c.cpu().numpy()
 
     def target_response(self, points):
         """(P, M) response of the discrete PDE at physical points: the trilinear
@@ -402,43 +405,3 @@ class HelmholtzSolver:
 
 
 HelmholtzDirectSolver = HelmholtzSolver
-
-
-def material_vjp(op, field, adjoint):
-    """-Re(lambda^H dA u) for density and sound speed, lambda zeroed on
-    identity rows (conservative_helmholtz.py:151-175), from the operator's
-    host tables."""
-    shape = op.shape
-    u = np.asarray(field).reshape(shape, order="F")
-    lam = np.array(np.asarray(adjoint).reshape(shape, order="F"))
-    lam[op.fixed] = 0
-    rho, c = op.rho, op.c
-    w = [op.weights[a].reshape([-1 if b == a else 1 for b in range(3)]) for a in range(3)]
-    s = [op.stretch[a].reshape([-1 if b == a else 1 for b in range(3)]) for a in range(3)]
-    mass = w[0]*w[1]*w[2]*s[0]*s[1]*s[2]*op.omega**2/(rho*c**2)
-    prod = np.conj(lam)*u*mass
-    g_rho = np.real(prod/rho)
-    g_c = np.real(2*prod/c)
-    for axis in range(3):
-        lo = [slice(None)]*3
-        hi = [slice(None)]*3
-        lo[axis], hi[axis] = slice(None, -1), slice(1, None)
-        lo, hi = tuple(lo), tuple(hi)
-        b, cc = [a for a in range(3) if a != axis]
-        mid = 0.5*(s[axis][lo] + s[axis][hi])
-        factor = w[b]*w[cc]*s[b]*s[cc]/mid/op.dx**2
-        d_coef = -2*factor/(rho[lo] + rho[hi])**2
-        contrib = -np.real((np.conj(lam[lo]) - np.conj(lam[hi]))*(u[hi] - u[lo])*d_coef)
-        g_rho[lo] += contrib
-        g_rho[hi] += contrib
-    for index, kind in enumerate(op.kinds):
-        if kind != "open":
-            continue
-        sel = [slice(None)]*3
-        sel[index//2] = 0 if index % 2 == 0 else -1
-        sel = tuple(sel)
-        term = op.face_terms[index].reshape(np.asarray(u[sel]).shape, order="F")
-        contrib = np.real(np.conj(lam[sel])*u[sel]*term)
-        g_rho[sel] += contrib/rho[sel]
-        g_c[sel] += contrib/c[sel]
-    return g_rho, g_c
