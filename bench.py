@@ -172,8 +172,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(scene, args, plain_iters)
-    zb, ty, rp, nt, flat, march = HelmholtzOperator.SLAB_SHAPES[tuning["shapes"][0]]
-    traffic = measured_traffic(args, zb, ty, rp, nt, flat, march)
+    zb, ty, rp, nt, flat = HelmholtzOperator.SLAB_SHAPES[tuning["shapes"][0]]
+    traffic = measured_traffic(args, zb, ty, rp, nt, flat)
 
     out = {
         "metric": "RHS solves/s at 128^3 to 1e-6 rel. residual (mixed vs FP64); stencil HBM GB/s",
@@ -191,9 +191,8 @@ def main():
         "refinements_median": float(np.median(refines)) if refines else None,
         "worst_scaled_residual": worst,
         "setup_s": setup_s,
-        "roofline": {"kernel": (f"hz::k_march<1, 0, {zb}, {ty}, {rp}, {nt}>" if march else
-                                f"hz::k_slab<1, 0, {zb}, {ty}, {rp}, {nt}, {flat}>") +
-                               f" (complex64 R A M apply, {k} columns)",
+        "roofline": {"kernel": f"hz::k_slab<1, 0, {zb}, {ty}, {rp}, {nt}, {flat}> (complex64 R A M apply, "
+                               f"{k} columns)",
                      "bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                      "frac": achieved/8000.0, "traffic": traffic, "alg_bytes_per_launch": alg_bytes,
                      "launch_us": stencil_s*1e6,
@@ -208,13 +207,13 @@ TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "stencil_128_k16_traffic.md")
 STENCIL_RHS = 16  # columns per stencil launch in the roofline measurement
 
 
-def measured_traffic(args, zb, ty, rp, nt, flat, march):
+def measured_traffic(args, zb, ty, rp, nt, flat):
     """HBM bytes per stencil launch from the FETCH_SIZE/WRITE_SIZE counter runs
     of the same kernel, tile shape and workload (128^3, 16 columns), scaled by
     the copy calibration recorded beside them; None for other workloads."""
     if args.n != 128 or not os.path.exists(TRAFFIC_PROFILE):
         return None
-    key = f"traffic_bytes_per_launch[zb={zb},ty={ty},rp={rp},nt={nt},flat={flat},march={march}]:"
+    key = f"traffic_bytes_per_launch[zb={zb},ty={ty},rp={rp},nt={nt},flat={flat}]:"
     with open(TRAFFIC_PROFILE) as f:
         for line in f:
             if line.startswith(key):

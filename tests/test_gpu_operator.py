@@ -329,7 +329,7 @@ def test_material_vjp_matches_reference_fixture(fix, name):
     assert _rel(g_rho.cpu().numpy(), ref_rho) < 1e-12
     assert _rel(g_c.cpu().numpy(), ref_c) < 1e-12
     # Two pairs give the sum of their gradients.
-    u2, lam2 = np.stack([u, 2*u]), np.stack([lam, 1j*lam])
+    u2, lam2 = np.stack([u, 2*u], axis=1), np.stack([lam, 1j*lam], axis=1)
     g2_rho, g2_c = op.material_vjp(u2, lam2)
     h_rho, h_c = op.material_vjp(2*u, 1j*lam)
     np.testing.assert_allclose(g2_rho.cpu().numpy(), (g_rho + h_rho).cpu().numpy(), rtol=0,

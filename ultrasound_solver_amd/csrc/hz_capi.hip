@@ -233,18 +233,10 @@ void launch_slab(int shape, const OpView<float> &v, const cx<float> *x, int64_t 
                  const float *bdiv, hipStream_t s) {
     const TileMap tm = slab_map(v.nx, v.ny, v.nz, shape);
     const dim3 grid(tm.nblocks), block(64 * SLAB_SHAPE[shape].ty);
-#define HZ_MARCH_CASE(ID, ZC, TY, RP, NT)                                                                          \
-    case ID:                                                                                                       \
-        static_assert(SLAB_SHAPE[ID].zb == ZC && SLAB_SHAPE[ID].ty == TY && SLAB_SHAPE[ID].rp == RP &&             \
-                          SLAB_SHAPE[ID].nt == NT && SLAB_SHAPE[ID].march == 1,                                    \
-                      "slab shape table");                                                                         \
-        hipLaunchKernelGGL((k_march<MODE, EPI, ZC, TY, RP, NT>), grid, block, 0, s, v, tm, x, ldx, y, ldy, k, live,  \
-                           scale, b, dinv, w, bdiv);                                                               \
-        break;
 #define HZ_SLAB_CASE(ID, ZB, TY, RP, NT, FLAT)                                                                     \
     case ID:                                                                                                       \
         static_assert(SLAB_SHAPE[ID].zb == ZB && SLAB_SHAPE[ID].ty == TY && SLAB_SHAPE[ID].rp == RP &&             \
-                          SLAB_SHAPE[ID].nt == NT && SLAB_SHAPE[ID].flat == FLAT && SLAB_SHAPE[ID].march == 0,     \
+                          SLAB_SHAPE[ID].nt == NT && SLAB_SHAPE[ID].flat == FLAT,                                  \
                       "slab shape table");                                                                         \
         hipLaunchKernelGGL((k_slab<MODE, EPI, ZB, TY, RP, NT, FLAT>), grid, block, 0, s, v, tm, x, ldx, y, ldy, k,   \
                            live, scale, b, dinv, w, bdiv);                                                         \
@@ -261,15 +253,10 @@ void launch_slab(int shape, const OpView<float> &v, const cx<float> *x, int64_t 
         HZ_SLAB_CASE(8, 4, 4, 1, 1, 1)
         HZ_SLAB_CASE(9, 2, 4, 1, 1, 1)
         HZ_SLAB_CASE(10, 2, 4, 2, 1, 1)
-        HZ_MARCH_CASE(11, 8, 4, 8, 1)
-        HZ_MARCH_CASE(12, 16, 4, 8, 1)
-        HZ_MARCH_CASE(13, 8, 2, 8, 1)
-        HZ_MARCH_CASE(14, 8, 4, 4, 1)
         default:
             throw BadArgument("unknown slab tile shape");
     }
 #undef HZ_SLAB_CASE
-#undef HZ_MARCH_CASE
     HZ_LAUNCHED();
 }
 
@@ -897,9 +884,9 @@ int pick_shape(F &&launch, hipStream_t s, int preferred = 0) {
     for (int sh = 0; sh < SLAB_SHAPES; ++sh) {
         const float ms = sh == preferred ? best_ms : time_launches([&] { launch(sh); }, 5, s);
         if (verbose())
-            std::fprintf(stderr, "[hz] autotune shape %d (zb %d, ty %d, rp %d, nt %d, flat %d, march %d): %.1f us\n",
-                         sh, SLAB_SHAPE[sh].zb, SLAB_SHAPE[sh].ty, SLAB_SHAPE[sh].rp, SLAB_SHAPE[sh].nt,
-                         SLAB_SHAPE[sh].flat, SLAB_SHAPE[sh].march, ms * 1e3);
+            std::fprintf(stderr, "[hz] autotune shape %d (zb %d, ty %d, rp %d, nt %d, flat %d): %.1f us\n", sh,
+                         SLAB_SHAPE[sh].zb, SLAB_SHAPE[sh].ty, SLAB_SHAPE[sh].rp, SLAB_SHAPE[sh].nt,
+                         SLAB_SHAPE[sh].flat, ms * 1e3);
         if (ms < 0.97f * best_ms) {
             best = sh;
             best_ms = ms;

@@ -21,16 +21,13 @@ namespace hz {
 // non-temporal output stores, and flat tiles (64 * ty consecutive nodes of
 // an xy-plane instead of a 64 x ty rectangle: no idle lanes on widths just
 // above a multiple of 64, such as the 65-wide first coarse level).
-// March shapes (k_march): zb is the z-range a workgroup marches through and
-// rp the right-hand sides whose (z-1, z, z+1) windows it keeps in registers.
 struct SlabShape {
-    int zb, ty, rp, nt, flat, march;
+    int zb, ty, rp, nt, flat;
 };
-constexpr int SLAB_SHAPES = 15;
+constexpr int SLAB_SHAPES = 11;
 constexpr SlabShape SLAB_SHAPE[SLAB_SHAPES] = {
-    {4, 4, 1, 0, 0, 0}, {2, 4, 1, 0, 0, 0}, {4, 8, 1, 0, 0, 0}, {2, 8, 1, 0, 0, 0}, {4, 2, 1, 0, 0, 0},
-    {2, 4, 2, 0, 0, 0}, {4, 4, 1, 1, 0, 0}, {2, 4, 2, 1, 0, 0}, {4, 4, 1, 1, 1, 0}, {2, 4, 1, 1, 1, 0},
-    {2, 4, 2, 1, 1, 0}, {8, 4, 8, 1, 0, 1}, {16, 4, 8, 1, 0, 1}, {8, 2, 8, 1, 0, 1}, {8, 4, 4, 1, 0, 1}};
+    {4, 4, 1, 0, 0}, {2, 4, 1, 0, 0}, {4, 8, 1, 0, 0}, {2, 8, 1, 0, 0}, {4, 2, 1, 0, 0}, {2, 4, 2, 0, 0},
+    {4, 4, 1, 1, 0}, {2, 4, 2, 1, 0}, {4, 4, 1, 1, 1}, {2, 4, 1, 1, 1}, {2, 4, 2, 1, 1}};
 
 __host__ inline TileMap slab_map(int nx, int ny, int nz, int shape) {
     const SlabShape sh = SLAB_SHAPE[shape];
@@ -216,129 +213,6 @@ __global__ __launch_bounds__(64 * TY) void k_slab(OpView<float> op, TileMap tm, 
                     }
                     store_out<NT>(yout + r * ldy + p, out);
                 }
-            }
-        }
-    }
-}
-
-// Z-marching variant of k_slab.  A workgroup of 64 x TY threads owns a
-// 64 x TY tile and the z-range [k0, k0 + ZC).  For RP right-hand sides at a
-// time each thread keeps the (z-1, z) values of its column in registers and
-// loads only plane z+1, the two y-neighbours and (lanes 0 and 63) the
-// x-neighbour across the tile edge per plane: the z-halo is read twice only
-// at the ends of the range.  Row coefficients are computed per plane, once
-// per RP right-hand sides.  Epilogues as in k_slab.
-template <int MODE, int EPI, int ZC, int TY, int RP, int NT>
-__global__ __launch_bounds__(64 * TY) void k_march(OpView<float> op, TileMap tm, const cx<float> *__restrict__ xin,
-                                               int64_t ldx, cx<float> *__restrict__ yout, int64_t ldy, int nrhs,
-                                               const int *__restrict__ live, const double *__restrict__ scale,
-                                               const cx<float> *__restrict__ b, const cx<float> *__restrict__ dinv,
-                                               float w, const float *__restrict__ bdiv) {
-    __shared__ int s_act[APPLY_MAX_RHS];
-    __shared__ float s_sc[APPLY_MAX_RHS];
-    __shared__ int s_na;
-    if (threadIdx.x == 0) {
-        int na = 0;
-        for (int r = 0; r < nrhs; ++r) {
-            if (live && !live[r]) continue;
-            s_act[na] = r;
-            s_sc[na] = scale ? (float)scale[r] : 1.0f;
-            ++na;
-        }
-        s_na = na;
-    }
-    __syncthreads();
-    const int tile = xcd_tile(blockIdx.x, tm.nblocks);
-    if (tile >= tm.ntiles) return;  // whole workgroup
-    const int tx = tile % tm.tiles_x;
-    const int ty = (tile / tm.tiles_x) % tm.tiles_y;
-    const int tz = tile / (tm.tiles_x * tm.tiles_y);
-    const int lane = threadIdx.x & 63;
-    const int i_raw = tx * 64 + lane;
-    const int j_raw = ty * TY + (threadIdx.x >> 6);
-    // Lanes past the grid edge keep running (they feed the DPP shifts) on a
-    // clamped node and never store.
-    const bool valid = i_raw < op.nx && j_raw < op.ny;
-    const int i = min(i_raw, op.nx - 1);
-    const int j = min(j_raw, op.ny - 1);
-    const int k0 = tz * ZC;
-    const int k1 = min(op.nz, k0 + ZC);
-    const int64_t sxy = (int64_t)op.nx * op.ny;
-    const int64_t pij = i + (int64_t)op.nx * j;
-    const bool has_xm = i > 0, has_xp = i < op.nx - 1;
-    const int64_t oym = j > 0 ? -(int64_t)op.nx : 0;
-    const int64_t oyp = j < op.ny - 1 ? (int64_t)op.nx : 0;
-    const int na = s_na;
-    for (int t = 0; t < na; t += RP) {
-        int rr[RP];
-        cx<float> prv[RP], cur[RP];
-#pragma unroll
-        for (int u = 0; u < RP; ++u) {
-            // The column is the same for the whole wave: keep it scalar.
-            rr[u] = __builtin_amdgcn_readfirstlane(s_act[min(t + u, na - 1)]);
-            const cx<float> *xr = xin + rr[u] * ldx;
-            prv[u] = k0 > 0 ? xr[pij + (int64_t)(k0 - 1) * sxy] : mk<float>(0, 0);
-            cur[u] = xr[pij + (int64_t)k0 * sxy];
-        }
-        for (int kk = k0; kk < k1; ++kk) {
-            const int64_t p = pij + (int64_t)kk * sxy;
-            const bool has_zp = kk + 1 < op.nz;
-            // All loads of this plane first: z+1, the y-neighbours, the x-edge
-            // values of lanes 0 and 63, b for the SMOOTH/RESID epilogues.
-            cx<float> nxt[RP], ym[RP], yp[RP], xe[RP], bb[RP];
-#pragma unroll
-            for (int u = 0; u < RP; ++u) {
-                const cx<float> *xr = xin + rr[u] * ldx;
-                nxt[u] = xr[p + (has_zp ? sxy : 0)];
-                if (!has_zp) nxt[u] = mk<float>(0, 0);
-                ym[u] = xr[p + oym];
-                yp[u] = xr[p + oyp];
-                xe[u] = mk<float>(0, 0);
-                if (lane == 0 && has_xm) xe[u] = xr[p - 1];
-                if (lane == 63 && has_xp) xe[u] = xr[p + 1];
-                bb[u] = mk<float>(0, 0);
-                if (EPI == SLAB_SMOOTH || EPI == SLAB_RESID) bb[u] = b[rr[u] * ldy + p];
-            }
-            const Nbr nb = neighbours(i, j, kk, op.nx, op.ny, op.nz);
-            cx<float> cs, c[6];
-            row_coefs<float, MODE>(op, i, j, kk, p, nb, cs, c);
-            cx<float> dv = mk<float>(0, 0);
-            float bs = 1.0f;
-            if (EPI == SLAB_SMOOTH) dv = (MODE == MODE_AH) ? conj(dinv[p]) : dinv[p];
-            if (EPI != SLAB_APPLY && bdiv) bs = 1.0f / bdiv[p];
-            if (EPI == SLAB_RESID0) {
-                cs = cs * first_sweep_factor<MODE>(dinv, bdiv, w, p);
-#pragma unroll
-                for (int e = 0; e < 6; ++e) c[e] = c[e] * first_sweep_factor<MODE>(dinv, bdiv, w, p + nb.off[e]);
-            }
-#pragma unroll
-            for (int u = 0; u < RP; ++u) {
-                const cx<float> xc = cur[u];
-                cx<float> xl = mk<float>(lane_prev(xc.re), lane_prev(xc.im));
-                cx<float> xn = mk<float>(lane_next(xc.re), lane_next(xc.im));
-                if (lane == 0) xl = xe[u];
-                if (lane == 63) xn = xe[u];
-                cx<float> acc = cs * xc;
-                acc = acc + c[0] * xl;
-                acc = acc + c[1] * xn;
-                acc = acc + c[2] * ym[u];
-                acc = acc + c[3] * yp[u];
-                acc = acc + c[4] * prv[u];
-                acc = acc + c[5] * nxt[u];
-                if (valid && t + u < na) {
-                    cx<float> out;
-                    if (EPI == SLAB_APPLY) {
-                        out = s_sc[t + u] * acc;
-                    } else if (EPI == SLAB_RESID0) {
-                        out = bs * xc - acc;
-                    } else {
-                        const cx<float> e = bs * bb[u] - acc;
-                        out = (EPI == SLAB_SMOOTH) ? xc + w * (dv * e) : e;
-                    }
-                    store_out<NT>(yout + rr[u] * ldy + p, out);
-                }
-                prv[u] = xc;
-                cur[u] = nxt[u];
             }
         }
     }
