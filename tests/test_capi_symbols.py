@@ -44,3 +44,12 @@ def test_bad_arguments_are_rejected_without_touching_the_device():
     status = _capi.lib().hz_create(None, ctypes.byref(handle))
     assert status == _capi.HZ_BAD_ARGUMENT
     assert b"null" in _capi.lib().hz_last_error()
+
+
+def test_operator_exposes_every_entry_point():
+    """Each C entry point has its Python method (no compute: attribute check)."""
+    from ultrasound_solver_amd import operator
+    for name in ("apply", "gmres", "autotune", "set_tile_shapes", "material_vjp", "release_workspace",
+                 "set_multigrid", "set_sources", "build_rhs", "gather_sources", "sample", "device_bytes"):
+        assert callable(getattr(operator.HelmholtzOperator, name)), name
+    assert callable(operator.gmres_batch)

@@ -216,6 +216,27 @@ class HelmholtzOperator:
                    (2, 4, 2, 0, 0), (4, 4, 1, 1, 0), (2, 4, 2, 1, 0), (4, 4, 1, 1, 1), (2, 4, 1, 1, 1),
                    (2, 4, 2, 1, 1))
 
+    def material_vjp(self, u, lam):
+        """dL/drho, dL/dc (grid-shaped, FP64, on the device) for
+        dL = -Re(lambda^H dA u), summed over the columns of u and lambda
+        ((N,), (N, k) or (k, N)); lambda is zeroed on identity rows."""
+        u = self.as_columns(u, torch.complex128)
+        lam = self.as_columns(lam, torch.complex128)
+        if u.shape != lam.shape:
+            raise ValueError("u and lambda must have the same columns")
+        g_rho = torch.empty(self.n, dtype=torch.float64, device=self.device)
+        g_c = torch.empty_like(g_rho)
+        _capi.check(_capi.lib().hz_material_vjp(self.handle, ctypes.c_void_p(u.data_ptr()),
+                                                ctypes.c_void_p(lam.data_ptr()), u.shape[0], self.n,
+                                                ctypes.c_void_p(g_rho.data_ptr()), ctypes.c_void_p(g_c.data_ptr()),
+                                                _stream()), "hz_material_vjp")
+        # Fortran node order p = i + nx (j + ny k) -> an (nx, ny, nz) view.
+        return tuple(g.view(self.shape[::-1]).permute(2, 1, 0) for g in (g_rho, g_c))
+
+    def release_workspace(self):
+        """Free the cached solver workspace (the next solve re-allocates)."""
+        _capi.check(_capi.lib().hz_release_workspace(self.handle), "hz_release_workspace")
+
     def set_tile_shapes(self, shapes):
         """Pin the slab tile shapes (the layout `autotune` reports)."""
         arr = (ctypes.c_int32*len(shapes))(*[int(v) for v in shapes])
