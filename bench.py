@@ -37,6 +37,8 @@ def parse():
     p.add_argument("--cpu-iters", type=int, default=60, help="scipy GMRES iterations in the CPU sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--stencil-reps", type=int, default=20)
+    p.add_argument("--stencil-shape", type=int, default=-1,
+                   help="stencil-only: pin the fine apply's tile shape (index) after autotuning")
     p.add_argument("--stencil-only", action="store_true",
                    help="only the stencil launches and a 512 MiB copy (for hardware counter runs)")
     p.add_argument("--mg-shift", type=float, default=1.2)
@@ -230,8 +232,10 @@ def stencil_only(op, args, m):
     k = STENCIL_RHS
     if args.precond == "multigrid":
         op.set_multigrid(shift=args.mg_shift, pre=args.mg_pre, post=args.mg_post, weight=args.mg_weight)
-    print(json.dumps({"autotune": op.autotune(m, max_chunk=16, restart=args.restart,
-                                              preconditioner=args.precond)}), flush=True)
+    tuning = op.autotune(m, max_chunk=16, restart=args.restart, preconditioner=args.precond)
+    if args.stencil_shape >= 0:
+        op.set_tile_shapes([args.stencil_shape] + tuning["shapes"][1:])
+    print(json.dumps({"autotune": tuning, "stencil_shape": args.stencil_shape}), flush=True)
     x = torch.randn((k, op.n), dtype=torch.complex64, device="cuda:0")
     for _ in range(args.stencil_reps):
         op.apply(x, mode="RAM", precision="c64")

@@ -103,6 +103,9 @@ typedef struct hz_gmres_opts {
     double inner_rtol;          /* inner c64 tolerance floor (mixed form)    */
     int32_t use_graph;          /* capture each cycle in a hipGraph          */
     int32_t preconditioner;     /* 0 Jacobi (reference), 1 multigrid (mixed) */
+    int32_t arnoldi_block;      /* 1: one Arnoldi step at a time; 2: two
+                                   basis vectors per Gram-Schmidt pass
+                                   (mixed form only)                         */
 } hz_gmres_opts;
 
 typedef struct hz_gmres_stats {

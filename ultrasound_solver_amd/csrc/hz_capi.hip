@@ -132,6 +132,7 @@ struct hz_operator {
     DevBuf src_col_node, src_col_ptr, src_col_val;
     // Cached GMRES workspace (grow-only).
     DevBuf V, X, Bh, dpart, npart, cpart, red, Hm, gc, gs, g, scales, coef, resid, tol, steps, live, member;
+    DevBuf Hu, coef2, alpha, gram;  // two-step Arnoldi blocks
     DevBuf rhs32;
     std::unique_ptr<Multigrid> mg;
     // Captured GMRES cycle bodies, keyed by everything their launches bake in.
@@ -482,7 +483,7 @@ enum Precond { PRE_JACOBI = 0, PRE_MULTIGRID = 1 };
 template <typename T>
 void gmres_cycles(hz_operator *op, int mode_ram, int k, int restart, int maxiter, bool ref_semantics,
                   const std::vector<double> &atol, ChunkState &cs, hipStream_t s, int precond = PRE_JACOBI,
-                  bool use_graph = false) {
+                  bool use_graph = false, int block = 1) {
     const bool use_mg = precond == PRE_MULTIGRID;
     const bool adjoint = (mode_ram == MODE_RAMH);
     const int mode_op = use_mg ? (adjoint ? MODE_RAH : MODE_RA) : mode_ram;
@@ -504,6 +505,9 @@ void gmres_cycles(hz_operator *op, int mode_ram, int k, int restart, int maxiter
     st.tol = op->tol.as<double>();
     st.steps = op->steps.as<int>();
     st.live = op->live.as<int>();
+    st.Hu = op->Hu.as<cx<double>>();
+    st.coef2 = op->coef2.as<cx<double>>();
+    st.alpha = op->alpha.as<cx<double>>();
     int *member = op->member.as<int>();
 
     hipLaunchKernelGGL((k_zero<T>), dim3(vec_blocks(n), k), dim3(256), 0, s, X, n, n);
@@ -562,8 +566,71 @@ void gmres_cycles(hz_operator *op, int mode_ram, int k, int restart, int maxiter
         // the iterate update.  Slots that leave early are skipped inside the
         // kernels (device live flags), so the launch sequence is the same
         // every cycle and can be replayed from a captured graph.
+        // Two-step blocks (complex64 only): w1 = A M q_j, w2 = A M w1, one dot
+        // pass and one update pass over V_0..V_j for both, an intra-block axpy.
+        const bool two = block == 2 && std::is_same<T, float>::value;
+        auto block_step = [&](int j, hipStream_t cs_stream) {
+            if constexpr (std::is_same<T, float>::value) {
+                cx<float> *w1 = V + (int64_t)(j + 1) * k * n;
+                cx<float> *w2 = V + (int64_t)(j + 2) * k * n;
+                const cx<float> *in = V + (int64_t)j * k * n;
+                cx<float> *z = use_mg ? op->mg->levels[0]->xx.as<cx<float>>() : nullptr;
+                if (use_mg) {
+                    precondition(op, adjoint, in, z, k, st.live, cs_stream);
+                    in = z;
+                }
+                apply_mode<float>(op, mode_op, in, n, w1, n, k, st.live, st.scales + (int64_t)j * k, cs_stream);
+                in = w1;
+                if (use_mg) {
+                    precondition(op, adjoint, w1, z, k, st.live, cs_stream);
+                    in = z;
+                }
+                apply_mode<float>(op, mode_op, in, n, w2, n, k, st.live, nullptr, cs_stream);
+                const int groups = (j + 1 + DOT_GROUP / 2 - 1) / (DOT_GROUP / 2);
+                const bool pr = paired<float>(n);
+                if (pr)
+                    hipLaunchKernelGGL((k_dots2<float, 2>), dim3(nblk, k, groups), dim3(256), 0, cs_stream,
+                                       (const cx<float> *)V, k, n, j + 1, (const cx<float> *)w1,
+                                       (const cx<float> *)w2, st.live, op->dpart.as<double2>(), maxv, chunk);
+                else
+                    hipLaunchKernelGGL((k_dots2<float, 1>), dim3(nblk, k, groups), dim3(256), 0, cs_stream,
+                                       (const cx<float> *)V, k, n, j + 1, (const cx<float> *)w1,
+                                       (const cx<float> *)w2, st.live, op->dpart.as<double2>(), maxv, chunk);
+                HZ_LAUNCHED();
+                hipLaunchKernelGGL(k_reduce_dots2, dim3(j + 1, k), dim3(256), 0, cs_stream, op->dpart.as<double2>(),
+                                   (int)nblk, maxv, j, st, restart);
+                HZ_LAUNCHED();
+                if (pr)
+                    hipLaunchKernelGGL((k_update2<float, 2>), dim3(nblk, k), dim3(256), 0, cs_stream,
+                                       (const cx<float> *)V, k, n, j + 1, w1, w2, (const cx<double> *)st.coef,
+                                       (const cx<double> *)st.coef2, maxv, st.live, op->gram.as<double>(), chunk);
+                else
+                    hipLaunchKernelGGL((k_update2<float, 1>), dim3(nblk, k), dim3(256), 0, cs_stream,
+                                       (const cx<float> *)V, k, n, j + 1, w1, w2, (const cx<double> *)st.coef,
+                                       (const cx<double> *)st.coef2, maxv, st.live, op->gram.as<double>(), chunk);
+                HZ_LAUNCHED();
+                hipLaunchKernelGGL(k_block_mid, dim3(k), dim3(256), 0, cs_stream, st, op->gram.as<double>(),
+                                   (int)nblk, j, restart);
+                HZ_LAUNCHED();
+                if (pr)
+                    hipLaunchKernelGGL((k_axpy_norm<float, 2>), dim3(nblk, k), dim3(256), 0, cs_stream,
+                                       (const cx<float> *)w1, w2, n, (const cx<double> *)st.alpha, st.live,
+                                       op->npart.as<double>(), chunk);
+                else
+                    hipLaunchKernelGGL((k_axpy_norm<float, 1>), dim3(nblk, k), dim3(256), 0, cs_stream,
+                                       (const cx<float> *)w1, w2, n, (const cx<double> *)st.alpha, st.live,
+                                       op->npart.as<double>(), chunk);
+                HZ_LAUNCHED();
+                hipLaunchKernelGGL(k_block_finish, dim3(k), dim3(256), 0, cs_stream, st, op->npart.as<double>(),
+                                   (int)nblk, j, restart);
+                HZ_LAUNCHED();
+            }
+        };
         auto body = [&](hipStream_t cs_stream) {
-            for (int j = 0; j < restart; ++j) {
+            int j0 = 0;
+            if (two)
+                for (; j0 + 1 < restart; j0 += 2) block_step(j0, cs_stream);
+            for (int j = j0; j < restart; ++j) {
                 const cx<T> *vj = V + (int64_t)j * k * n;
                 cx<T> *w = V + (int64_t)(j + 1) * k * n;
                 const cx<T> *in = vj;
@@ -593,7 +660,8 @@ void gmres_cycles(hz_operator *op, int mode_ram, int k, int restart, int maxiter
                              chunk);
         };
         if (use_graph) {
-            hipGraphExec_t exec = cycle_graph(op, graph_signature(op, sizeof(T), mode_op, k, restart, use_mg), body);
+            hipGraphExec_t exec =
+                cycle_graph(op, graph_signature(op, sizeof(T), mode_op + 16 * (two ? 1 : 0), k, restart, use_mg), body);
             HZ_HIP(hipGraphLaunch(exec, s));
         } else {
             body(s);
@@ -617,7 +685,7 @@ void reserve_workspace(hz_operator *op, int k, int restart, size_t elem) {
     op->V.ensure((size_t)maxv * k * n * elem);
     op->X.ensure((size_t)k * n * elem);
     op->Bh.ensure((size_t)k * n * elem);
-    op->dpart.ensure((size_t)k * maxv * nblk * sizeof(double2));
+    op->dpart.ensure((size_t)2 * k * maxv * nblk * sizeof(double2));  // two targets in a block step
     op->npart.ensure((size_t)k * std::max<int64_t>(nblk, op->tm.nblocks) * 2 * sizeof(double));
     op->cpart.ensure((size_t)k * op->tm.nblocks * 6 * sizeof(double));
     op->red.ensure((size_t)k * 8 * sizeof(double));
@@ -627,6 +695,10 @@ void reserve_workspace(hz_operator *op, int k, int restart, size_t elem) {
     op->g.ensure((size_t)k * (restart + 1) * sizeof(cx<double>));
     op->scales.ensure((size_t)maxv * k * sizeof(double));
     op->coef.ensure((size_t)k * maxv * sizeof(cx<double>));
+    op->coef2.ensure((size_t)k * maxv * sizeof(cx<double>));
+    op->Hu.ensure((size_t)k * (restart + 1) * restart * sizeof(cx<double>));
+    op->alpha.ensure((size_t)k * sizeof(cx<double>));
+    op->gram.ensure((size_t)k * nblk * 4 * sizeof(double));
     op->resid.ensure((size_t)k * sizeof(double));
     op->tol.ensure((size_t)k * sizeof(double));
     op->steps.ensure((size_t)k * sizeof(int));
@@ -784,7 +856,7 @@ void solve_mixed(hz_operator *op, const hz_gmres_opts &o, const cx<double> *b, i
         const bool use_mg = o.preconditioner == PRE_MULTIGRID;
         if (use_mg) mg_reserve(*op->mg, k);
         gmres_cycles<float>(op, o.adjoint ? MODE_RAMH : MODE_RAM, k, o.restart, o.maxiter, false, atol, inner, s,
-                            o.preconditioner, o.use_graph != 0);
+                            o.preconditioner, o.use_graph != 0, o.arnoldi_block);
         for (int r = 0; r < k; ++r) {
             cs.iterations[r] = inner.iterations[r];
             cs.cycles[r] = inner.cycles[r];
@@ -834,6 +906,8 @@ void gmres_run(hz_operator *op, const hz_gmres_opts *opts, const void *b, void *
     if (o.maxiter < 0) throw BadArgument("maxiter must be >= 0");
     if (!(o.rtol > 0) || !(o.backward_error_tol > 0)) throw BadArgument("tolerances must be positive");
     if (o.precision != HZ_C128 && o.precision != HZ_C64) throw BadArgument("unknown precision");
+    if (o.arnoldi_block != 1 && o.arnoldi_block != 2) throw BadArgument("arnoldi_block must be 1 or 2");
+    if (o.arnoldi_block == 2 && o.precision != HZ_C64) throw BadArgument("two-step blocks run in the mixed form");
     if (o.preconditioner == PRE_MULTIGRID) {
         if (o.precision != HZ_C64) throw BadArgument("the multigrid preconditioner runs in the mixed form only");
         if (!op->mg) throw BadArgument("no multigrid hierarchy set (hz_set_multigrid)");

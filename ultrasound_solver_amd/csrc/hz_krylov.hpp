@@ -195,7 +195,10 @@ __global__ __launch_bounds__(256) void k_update(const cx<T> *__restrict__ V, int
 
 // Per-RHS GMRES state, FP64, on the device.
 struct GmresState {
-    cx<double> *H;     // [k][restart+1][restart]
+    cx<double> *H;     // [k][restart+1][restart], Givens-rotated in place
+    cx<double> *Hu;    // [k][restart+1][restart], unrotated (two-step blocks only)
+    cx<double> *coef2; // second-target update coefficients [k][maxv] (two-step blocks)
+    cx<double> *alpha; // per-RHS intra-block coefficient (two-step blocks)
     double *gc;        // Givens cosines [k][restart]
     cx<double> *gs;    // Givens sines   [k][restart]
     cx<double> *g;     // rotated RHS    [k][restart+1]
@@ -226,22 +229,14 @@ __global__ __launch_bounds__(256) void k_cycle_start(GmresState st, const double
     }
 }
 
-// Finish Arnoldi step j: beta = ||w||, H(j+1, j) = beta, Givens update,
-// residual estimate |g_{j+1}|, early exit when it meets st.tol[r].
-__global__ __launch_bounds__(256) void k_arnoldi_finish(GmresState st, const double *__restrict__ norm_partial,
-                                                        int nblk, int j, int maxv, int restart) {
-    __shared__ double red[4];
-    const int r = blockIdx.x;
-    if (!st.live[r]) return;
-    double s = 0;
-    for (int b = threadIdx.x; b < nblk; b += 256) s += norm_partial[(int64_t)r * nblk + b];
-    s = block_sum_256(s, red);
-    if (threadIdx.x != 0) return;
-    const double beta = sqrt(s);
+// Givens update of column j of H (rotated storage) whose subdiagonal entry
+// is beta (real, >= 0): apply the previous rotations, make the new one,
+// rotate g, record the residual estimate |g_{j+1}| and the step count, and
+// retire the RHS when the estimate meets st.tol[r] or beta vanished.
+// Returns whether the RHS is still live.
+__device__ bool givens_column(GmresState &st, int r, int j, double beta, int restart) {
     cx<double> *col = st.H + (int64_t)r * (restart + 1) * restart;  // H(i, j) at col[i*restart + j]
     col[(j + 1) * restart + j] = mk<double>(beta, 0);
-    st.scales[(int64_t)(j + 1) * gridDim.x + r] = beta > 0 ? 1.0 / beta : 0.0;
-    // Apply the previous rotations to column j.
     for (int q = 0; q < j; ++q) {
         const double c = st.gc[r * restart + q];
         const cx<double> sn = st.gs[r * restart + q];
@@ -250,7 +245,6 @@ __global__ __launch_bounds__(256) void k_arnoldi_finish(GmresState st, const dou
         col[q * restart + j] = c * a + sn * b;
         col[(q + 1) * restart + j] = c * b - conj(sn) * a;
     }
-    // New rotation zeroing H(j+1, j) = beta (real, >= 0).
     const cx<double> a = col[j * restart + j];
     const double aa = cabs(a);
     double c;
@@ -277,7 +271,334 @@ __global__ __launch_bounds__(256) void k_arnoldi_finish(GmresState st, const dou
     const double est = cabs(g[j + 1]);
     st.resid[r] = est;
     st.steps[r] = j + 1;
-    if ((st.tol[r] >= 0 && est <= st.tol[r]) || beta == 0.0) st.live[r] = 0;
+    if ((st.tol[r] >= 0 && est <= st.tol[r]) || beta == 0.0) {
+        st.live[r] = 0;
+        return false;
+    }
+    return true;
+}
+
+// Finish Arnoldi step j: beta = ||w||, H(j+1, j) = beta, Givens update,
+// residual estimate |g_{j+1}|, early exit when it meets st.tol[r].
+__global__ __launch_bounds__(256) void k_arnoldi_finish(GmresState st, const double *__restrict__ norm_partial,
+                                                        int nblk, int j, int maxv, int restart) {
+    __shared__ double red[4];
+    const int r = blockIdx.x;
+    if (!st.live[r]) return;
+    double s = 0;
+    for (int b = threadIdx.x; b < nblk; b += 256) s += norm_partial[(int64_t)r * nblk + b];
+    s = block_sum_256(s, red);
+    if (threadIdx.x != 0) return;
+    const double beta = sqrt(s);
+    st.scales[(int64_t)(j + 1) * gridDim.x + r] = beta > 0 ? 1.0 / beta : 0.0;
+    givens_column(st, r, j, beta, restart);
+}
+
+// ------------------------------------------------------- two-step Arnoldi blocks
+// A block at basis index j generates w1 = A M q_j (slot j+1) and
+// w2 = A M w1 (slot j+2) before any orthogonalisation, so that one dot pass
+// and one update pass over V_0..V_j serve both vectors:
+//   W1 = w1 - sum_i h1_i q_i,          h1_i = q_i^H w1
+//   W2 = w2 - sum_i h2_i q_i,          h2_i = q_i^H w2
+//   q_{j+1} = W1 / r11,  r11 = |W1|
+//   q_{j+2} = (W2 - r12 q_{j+1}) / r22, r12 = q_{j+1}^H W2   (one axpy pass)
+// Column j of H is (h1, r11); column j+1 follows from linearity:
+//   A M q_{j+1} = (w2 - sum_{i<=j} h1_i A M q_i) / r11,
+// with A M q_i = sum_l Hu(l, i) q_l the unrotated columns already built.
+
+// Dot partials of both targets: partial[((t*k + r)*maxv + i)*nblk + blk].
+template <typename T, int VW>
+__global__ __launch_bounds__(256) void k_dots2(const cx<T> *__restrict__ V, int k, int64_t n, int jp1,
+                                               const cx<T> *__restrict__ w1, const cx<T> *__restrict__ w2,
+                                               const int *__restrict__ live, double2 *__restrict__ partial, int maxv,
+                                               int64_t chunk) {
+    constexpr int G = DOT_GROUP / 2;
+    __shared__ double red[4];
+    const int r = blockIdx.y;
+    if (!live[r]) return;
+    const int g0 = blockIdx.z * G;
+    const int cnt = min(G, jp1 - g0);
+    if (cnt <= 0) return;
+    const int64_t p0 = blockIdx.x * chunk;
+    const int64_t p1 = min(n, p0 + chunk);
+    const cx<T> *ar = w1 + (int64_t)r * n;
+    const cx<T> *br = w2 + (int64_t)r * n;
+    T are[G], aim[G], bre[G], bim[G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) are[q] = aim[q] = bre[q] = bim[q] = 0;
+    if constexpr (VW == 2) {
+        static_assert(sizeof(T) == 4, "paired loads are for complex64");
+        const float4 *a4 = reinterpret_cast<const float4 *>(ar);
+        const float4 *b4 = reinterpret_cast<const float4 *>(br);
+        for (int64_t p = (p0 >> 1) + threadIdx.x; p < (p1 >> 1); p += 256) {
+            const float4 av = a4[p], bv = b4[p];
+#pragma unroll
+            for (int q = 0; q < G; ++q) {
+                if (q < cnt) {
+                    const float4 v = reinterpret_cast<const float4 *>(V + ((int64_t)(g0 + q) * k + r) * n)[p];
+                    are[q] += v.x * av.x + v.y * av.y + v.z * av.z + v.w * av.w;
+                    aim[q] += v.x * av.y - v.y * av.x + v.z * av.w - v.w * av.z;
+                    bre[q] += v.x * bv.x + v.y * bv.y + v.z * bv.z + v.w * bv.w;
+                    bim[q] += v.x * bv.y - v.y * bv.x + v.z * bv.w - v.w * bv.z;
+                }
+            }
+        }
+    } else {
+        for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
+            const cx<T> av = ar[p], bv = br[p];
+#pragma unroll
+            for (int q = 0; q < G; ++q) {
+                if (q < cnt) {
+                    const cx<T> v = V[((int64_t)(g0 + q) * k + r) * n + p];
+                    are[q] += v.re * av.re + v.im * av.im;
+                    aim[q] += v.re * av.im - v.im * av.re;
+                    bre[q] += v.re * bv.re + v.im * bv.im;
+                    bim[q] += v.re * bv.im - v.im * bv.re;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+        if (q < cnt) {
+            const double s0 = block_combine_256((double)wave_sum(are[q]), red);
+            const double s1 = block_combine_256((double)wave_sum(aim[q]), red);
+            const double s2 = block_combine_256((double)wave_sum(bre[q]), red);
+            const double s3 = block_combine_256((double)wave_sum(bim[q]), red);
+            if (threadIdx.x == 0) {
+                partial[(((int64_t)0 * k + r) * maxv + g0 + q) * gridDim.x + blockIdx.x] = make_double2(s0, s1);
+                partial[(((int64_t)1 * k + r) * maxv + g0 + q) * gridDim.x + blockIdx.x] = make_double2(s2, s3);
+            }
+        }
+    }
+}
+
+// Combine both targets' partials (grid (jp1, k)): Hu(i, j) = h1_i = s_i d1_i,
+// Hu(i, j+1) = h2_i (held there until the block finishes), and the update
+// coefficients coef = s_i^2 d1_i, coef2 = s_i^2 d2_i.
+__global__ __launch_bounds__(256) void k_reduce_dots2(const double2 *__restrict__ partial, int nblk, int maxv,
+                                                      int j, GmresState st, int restart) {
+    __shared__ double red[4];
+    const int i = blockIdx.x;
+    const int r = blockIdx.y;
+    const int k = gridDim.y;
+    if (!st.live[r]) return;
+    double v[4] = {0, 0, 0, 0};
+    for (int t = 0; t < 2; ++t) {
+        const double2 *src = partial + (((int64_t)t * k + r) * maxv + i) * nblk;
+        for (int b = threadIdx.x; b < nblk; b += 256) {
+            v[2 * t] += src[b].x;
+            v[2 * t + 1] += src[b].y;
+        }
+    }
+    for (int q = 0; q < 4; ++q) v[q] = block_sum_256(v[q], red);
+    if (threadIdx.x == 0) {
+        const double s = st.scales[(int64_t)i * k + r];
+        const cx<double> d1 = mk<double>(v[0], v[1]), d2 = mk<double>(v[2], v[3]);
+        cx<double> *hu = st.Hu + (int64_t)r * (restart + 1) * restart;
+        hu[i * restart + j] = s * d1;
+        hu[i * restart + j + 1] = s * d2;
+        st.coef[r * maxv + i] = (s * s) * d1;
+        st.coef2[r * maxv + i] = (s * s) * d2;
+    }
+}
+
+// W1 = w1 - sum coef V_i, W2 = w2 - sum coef2 V_i in place, with FP64 partials
+// of |W1|^2, Re/Im W1^H W2 and |W2|^2 (gram[(r*nblk + blk)*4 + 0..3]).
+template <typename T, int VW>
+__global__ __launch_bounds__(256) void k_update2(const cx<T> *__restrict__ V, int k, int64_t n, int cnt,
+                                                 cx<T> *__restrict__ w1, cx<T> *__restrict__ w2,
+                                                 const cx<double> *__restrict__ coef,
+                                                 const cx<double> *__restrict__ coef2, int maxv,
+                                                 const int *__restrict__ live, double *__restrict__ gram,
+                                                 int64_t chunk) {
+    __shared__ cx<T> cf[64], cf2[64];
+    __shared__ double red[4];
+    const int r = blockIdx.y;
+    if (!live[r]) return;
+    for (int q = threadIdx.x; q < cnt; q += 256) {
+        cf[q] = cast<T>(coef[r * maxv + q]);
+        cf2[q] = cast<T>(coef2[r * maxv + q]);
+    }
+    __syncthreads();
+    const int64_t p0 = blockIdx.x * chunk;
+    const int64_t p1 = min(n, p0 + chunk);
+    cx<T> *ar = w1 + (int64_t)r * n;
+    cx<T> *br = w2 + (int64_t)r * n;
+    T g11 = 0, g22 = 0, g12r = 0, g12i = 0;
+    if constexpr (VW == 2) {
+        static_assert(sizeof(T) == 4, "paired loads are for complex64");
+        float4 *a4 = reinterpret_cast<float4 *>(ar);
+        float4 *b4 = reinterpret_cast<float4 *>(br);
+        for (int64_t p = (p0 >> 1) + threadIdx.x; p < (p1 >> 1); p += 256) {
+            float4 a = a4[p], b = b4[p];
+            int q = 0;
+            for (; q + 4 <= cnt; q += 4) {
+                float4 v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    v[u] = reinterpret_cast<const float4 *>(V + ((int64_t)(q + u) * k + r) * n)[p];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const cx<float> c = cf[q + u], d = cf2[q + u];
+                    a.x -= c.re * v[u].x - c.im * v[u].y;
+                    a.y -= c.re * v[u].y + c.im * v[u].x;
+                    a.z -= c.re * v[u].z - c.im * v[u].w;
+                    a.w -= c.re * v[u].w + c.im * v[u].z;
+                    b.x -= d.re * v[u].x - d.im * v[u].y;
+                    b.y -= d.re * v[u].y + d.im * v[u].x;
+                    b.z -= d.re * v[u].z - d.im * v[u].w;
+                    b.w -= d.re * v[u].w + d.im * v[u].z;
+                }
+            }
+            for (; q < cnt; ++q) {
+                const float4 v = reinterpret_cast<const float4 *>(V + ((int64_t)q * k + r) * n)[p];
+                const cx<float> c = cf[q], d = cf2[q];
+                a.x -= c.re * v.x - c.im * v.y;
+                a.y -= c.re * v.y + c.im * v.x;
+                a.z -= c.re * v.z - c.im * v.w;
+                a.w -= c.re * v.w + c.im * v.z;
+                b.x -= d.re * v.x - d.im * v.y;
+                b.y -= d.re * v.y + d.im * v.x;
+                b.z -= d.re * v.z - d.im * v.w;
+                b.w -= d.re * v.w + d.im * v.z;
+            }
+            a4[p] = a;
+            b4[p] = b;
+            g11 += a.x * a.x + a.y * a.y + a.z * a.z + a.w * a.w;
+            g22 += b.x * b.x + b.y * b.y + b.z * b.z + b.w * b.w;
+            g12r += a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w;
+            g12i += a.x * b.y - a.y * b.x + a.z * b.w - a.w * b.z;
+        }
+    } else {
+        for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
+            cx<T> a = ar[p], b = br[p];
+            for (int q = 0; q < cnt; ++q) {
+                const cx<T> v = V[((int64_t)q * k + r) * n + p];
+                a = a - cf[q] * v;
+                b = b - cf2[q] * v;
+            }
+            ar[p] = a;
+            br[p] = b;
+            g11 += norm2(a);
+            g22 += norm2(b);
+            g12r += a.re * b.re + a.im * b.im;
+            g12i += a.re * b.im - a.im * b.re;
+        }
+    }
+    const double s0 = block_combine_256((double)wave_sum(g11), red);
+    const double s1 = block_combine_256((double)wave_sum(g12r), red);
+    const double s2 = block_combine_256((double)wave_sum(g12i), red);
+    const double s3 = block_combine_256((double)wave_sum(g22), red);
+    if (threadIdx.x == 0) {
+        double *o = gram + ((int64_t)r * gridDim.x + blockIdx.x) * 4;
+        o[0] = s0;
+        o[1] = s1;
+        o[2] = s2;
+        o[3] = s3;
+    }
+}
+
+// After the update pass (grid (k)): r11 = |W1|, r12 = q_{j+1}^H W2, the
+// intra-block coefficient alpha = r12 / r11 for W2 -= alpha W1, and the
+// scale of slot j+1.  r11 and r12 are parked in Hu(j+1, j), Hu(j+1, j+1).
+__global__ __launch_bounds__(256) void k_block_mid(GmresState st, const double *__restrict__ gram, int nblk, int j,
+                                                   int restart) {
+    __shared__ double red[4];
+    const int r = blockIdx.x;
+    if (!st.live[r]) return;
+    double v[3] = {0, 0, 0};
+    for (int b = threadIdx.x; b < nblk; b += 256) {
+        const double *o = gram + ((int64_t)r * nblk + b) * 4;
+        v[0] += o[0];
+        v[1] += o[1];
+        v[2] += o[2];
+    }
+    for (int q = 0; q < 3; ++q) v[q] = block_sum_256(v[q], red);
+    if (threadIdx.x != 0) return;
+    const double r11 = sqrt(v[0]);
+    const cx<double> r12 = r11 > 0 ? (1.0 / r11) * mk<double>(v[1], v[2]) : mk<double>(0, 0);
+    cx<double> *hu = st.Hu + (int64_t)r * (restart + 1) * restart;
+    hu[(j + 1) * restart + j] = mk<double>(r11, 0);
+    hu[(j + 1) * restart + j + 1] = r12;
+    st.alpha[r] = r11 > 0 ? (1.0 / r11) * r12 : mk<double>(0, 0);
+    st.scales[(int64_t)(j + 1) * gridDim.x + r] = r11 > 0 ? 1.0 / r11 : 0.0;
+}
+
+// W2 -= alpha W1 in place with FP64 partials of |W2|^2.  Grid (nblk, k).
+template <typename T, int VW>
+__global__ __launch_bounds__(256) void k_axpy_norm(const cx<T> *__restrict__ w1, cx<T> *__restrict__ w2, int64_t n,
+                                                   const cx<double> *__restrict__ alpha,
+                                                   const int *__restrict__ live, double *__restrict__ norm_partial,
+                                                   int64_t chunk) {
+    __shared__ double red[4];
+    const int r = blockIdx.y;
+    if (!live[r]) return;
+    const cx<T> al = cast<T>(alpha[r]);
+    const int64_t p0 = blockIdx.x * chunk;
+    const int64_t p1 = min(n, p0 + chunk);
+    const cx<T> *ar = w1 + (int64_t)r * n;
+    cx<T> *br = w2 + (int64_t)r * n;
+    T nrm = 0;
+    if constexpr (VW == 2) {
+        const float4 *a4 = reinterpret_cast<const float4 *>(ar);
+        float4 *b4 = reinterpret_cast<float4 *>(br);
+        for (int64_t p = (p0 >> 1) + threadIdx.x; p < (p1 >> 1); p += 256) {
+            const float4 a = a4[p];
+            float4 b = b4[p];
+            b.x -= al.re * a.x - al.im * a.y;
+            b.y -= al.re * a.y + al.im * a.x;
+            b.z -= al.re * a.z - al.im * a.w;
+            b.w -= al.re * a.w + al.im * a.z;
+            b4[p] = b;
+            nrm += b.x * b.x + b.y * b.y + b.z * b.z + b.w * b.w;
+        }
+    } else {
+        for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
+            const cx<T> b = br[p] - al * ar[p];
+            br[p] = b;
+            nrm += norm2(b);
+        }
+    }
+    const double s = block_combine_256((double)wave_sum(nrm), red);
+    if (threadIdx.x == 0) norm_partial[(int64_t)r * gridDim.x + blockIdx.x] = s;
+}
+
+// Finish the block (grid (k)): r22 = |W2 - alpha W1|, scale of slot j+2,
+// column j+1 of Hu by linearity, then the Givens updates of columns j and
+// j+1 (the second only if the RHS is still live after the first).
+__global__ __launch_bounds__(256) void k_block_finish(GmresState st, const double *__restrict__ norm_partial,
+                                                      int nblk, int j, int restart) {
+    __shared__ double red[4];
+    const int r = blockIdx.x;
+    if (!st.live[r]) return;
+    double s = 0;
+    for (int b = threadIdx.x; b < nblk; b += 256) s += norm_partial[(int64_t)r * nblk + b];
+    s = block_sum_256(s, red);
+    if (threadIdx.x != 0) return;
+    const double r22 = sqrt(s);
+    st.scales[(int64_t)(j + 2) * gridDim.x + r] = r22 > 0 ? 1.0 / r22 : 0.0;
+    cx<double> *hu = st.Hu + (int64_t)r * (restart + 1) * restart;
+    cx<double> *h = st.H + (int64_t)r * (restart + 1) * restart;
+    const double r11 = hu[(j + 1) * restart + j].re;
+    const cx<double> r12 = hu[(j + 1) * restart + j + 1];
+    // Coefficients of w2 in q_0 .. q_{j+2}: (h2_0 .. h2_j, r12, r22); subtract
+    // h1_i * Hu(:, i) for i <= j (column j itself is (h1, r11)).
+    cx<double> col[66];
+    for (int l = 0; l <= j; ++l) col[l] = hu[l * restart + j + 1];
+    col[j + 1] = r12;
+    col[j + 2] = mk<double>(r22, 0);
+    for (int i = 0; i <= j; ++i) {
+        const cx<double> h1 = hu[i * restart + j];
+        for (int l = 0; l <= i + 1; ++l) col[l] = col[l] - h1 * hu[l * restart + i];
+    }
+    const double inv = r11 > 0 ? 1.0 / r11 : 0.0;
+    for (int l = 0; l <= j + 2; ++l) hu[l * restart + j + 1] = inv * col[l];
+    // Rotated storage: copy the unrotated columns, then the Givens updates.
+    for (int l = 0; l <= j; ++l) h[l * restart + j] = hu[l * restart + j];
+    if (!givens_column(st, r, j, r11, restart)) return;
+    for (int l = 0; l <= j + 1; ++l) h[l * restart + j + 1] = hu[l * restart + j + 1];
+    givens_column(st, r, j + 1, hu[(j + 2) * restart + j + 1].re, restart);
 }
 
 // Back substitution on the rotated Hessenberg matrix; coef[r][i] = -y_i s_i so

@@ -36,14 +36,15 @@ def _device_tensor(x, dtype, device):
 
 def gmres_options(restart=30, maxiter=3000, rtol=1.0e-6, backward_error_tol=1.0e-4, precision="c128",
                   adjoint=False, chunk=1, max_refinements=10, inner_rtol=1.0e-5, preconditioner="jacobi",
-                  use_graph=False):
+                  use_graph=False, arnoldi_block=1):
     return _capi.GmresOpts(restart=int(restart), maxiter=int(maxiter), rtol=float(rtol),
                            backward_error_tol=float(backward_error_tol),
                            precision=PRECISIONS[precision], adjoint=int(bool(adjoint)),
                            chunk=int(chunk), max_refinements=int(max_refinements),
                            inner_rtol=float(inner_rtol), use_graph=int(bool(use_graph)),
                            preconditioner={"jacobi": _capi.PRE_JACOBI,
-                                           "multigrid": _capi.PRE_MULTIGRID}[preconditioner])
+                                           "multigrid": _capi.PRE_MULTIGRID}[preconditioner],
+                           arnoldi_block=int(arnoldi_block))
 
 
 def gmres_batch(operators, rhs, workers=4, **options):
@@ -175,13 +176,13 @@ class HelmholtzOperator:
 
     def gmres(self, b, restart=30, maxiter=3000, rtol=1.0e-6, backward_error_tol=1.0e-4,
               precision="c128", adjoint=False, chunk=1, max_refinements=10, inner_rtol=1.0e-5,
-              preconditioner="jacobi", raise_on_failure=False, use_graph=False):
+              preconditioner="jacobi", raise_on_failure=False, use_graph=False, arnoldi_block=1):
         """Solve A x = b (or A^H x = b) for each column; returns (x, stats)."""
         b = self.as_columns(b, torch.complex128)
         k = b.shape[0]
         x = torch.empty_like(b)
         opts = gmres_options(restart, maxiter, rtol, backward_error_tol, precision, adjoint, chunk,
-                             max_refinements, inner_rtol, preconditioner, use_graph)
+                             max_refinements, inner_rtol, preconditioner, use_graph, arnoldi_block)
         stats = (_capi.GmresStats*k)()
         status = _capi.lib().hz_gmres(self.handle, ctypes.byref(opts), ctypes.c_void_p(b.data_ptr()),
                                       ctypes.c_void_p(x.data_ptr()), k, self.n, stats, _stream())
