@@ -46,6 +46,8 @@ def parse():
     p.add_argument("--mg-post", type=int, default=3)
     p.add_argument("--mg-weight", type=float, default=0.7)
     p.add_argument("--graph", type=int, default=0, help="replay each GMRES cycle from a captured hipGraph")
+    p.add_argument("--arnoldi-block", type=int, default=2, choices=[1, 2],
+                   help="mixed form: basis vectors per Gram-Schmidt pass")
     p.add_argument("--workload", choices=["g128", "uneven", "large"], default="g128",
                    help="g128: G columns of the heterogeneous 128^3 scene (the headline); uneven: the "
                         "32-scene batch; large: the 256^3 two-layer scene")
@@ -103,7 +105,8 @@ def main():
         return op.build_rhs(w)
 
     common = dict(restart=args.restart, maxiter=args.maxiter, rtol=args.rtol, backward_error_tol=1e-4)
-    mixed = dict(common, precision="c64", chunk=args.chunk, preconditioner=args.precond, use_graph=bool(args.graph))
+    mixed = dict(common, precision="c64", chunk=args.chunk, preconditioner=args.precond, use_graph=bool(args.graph),
+                 arnoldi_block=args.arnoldi_block)
     for s in range(args.warmup):
         b = rhs_block(s*args.chunk, args.chunk)
         _, st = op.gmres(b, **mixed)
