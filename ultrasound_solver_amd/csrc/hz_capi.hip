@@ -254,6 +254,9 @@ void launch_slab(int shape, const OpView<float> &v, const cx<float> *x, int64_t 
         HZ_SLAB_CASE(8, 4, 4, 1, 1, 1)
         HZ_SLAB_CASE(9, 2, 4, 1, 1, 1)
         HZ_SLAB_CASE(10, 2, 4, 2, 1, 1)
+        HZ_SLAB_CASE(11, 4, 4, 1, 1, 2)
+        HZ_SLAB_CASE(12, 2, 4, 2, 1, 2)
+        HZ_SLAB_CASE(13, 4, 8, 1, 1, 2)
         default:
             throw BadArgument("unknown slab tile shape");
     }

@@ -21,20 +21,25 @@ namespace hz {
 // non-temporal output stores, and flat tiles (64 * ty consecutive nodes of
 // an xy-plane instead of a 64 x ty rectangle: no idle lanes on widths just
 // above a multiple of 64, such as the 65-wide first coarse level).
+// flat = 2: z-fastest tile order (rectangular tiles): the workgroups resident
+// together on one XCD hold whole z-columns of slabs, so each slab's z-halo
+// planes are the neighbouring slabs' own planes, loaded at about the same time.
 struct SlabShape {
     int zb, ty, rp, nt, flat;
 };
-constexpr int SLAB_SHAPES = 11;
+constexpr int SLAB_SHAPES = 14;
 constexpr SlabShape SLAB_SHAPE[SLAB_SHAPES] = {
-    {4, 4, 1, 0, 0}, {2, 4, 1, 0, 0}, {4, 8, 1, 0, 0}, {2, 8, 1, 0, 0}, {4, 2, 1, 0, 0}, {2, 4, 2, 0, 0},
-    {4, 4, 1, 1, 0}, {2, 4, 2, 1, 0}, {4, 4, 1, 1, 1}, {2, 4, 1, 1, 1}, {2, 4, 2, 1, 1}};
+    {4, 4, 1, 0, 0}, {2, 4, 1, 0, 0}, {4, 8, 1, 0, 0}, {2, 8, 1, 0, 0}, {4, 2, 1, 0, 0},
+    {2, 4, 2, 0, 0}, {4, 4, 1, 1, 0}, {2, 4, 2, 1, 0}, {4, 4, 1, 1, 1}, {2, 4, 1, 1, 1},
+    {2, 4, 2, 1, 1}, {4, 4, 1, 1, 2}, {2, 4, 2, 1, 2}, {4, 8, 1, 1, 2}};
 
 __host__ inline TileMap slab_map(int nx, int ny, int nz, int shape) {
     const SlabShape sh = SLAB_SHAPE[shape];
     TileMap t;
     // Flat tiles: tiles_x = 1 and tiles_y counts 64*ty-node pieces of a plane.
-    t.tiles_x = sh.flat ? 1 : (nx + 63) / 64;
-    t.tiles_y = sh.flat ? (nx * ny + 64 * sh.ty - 1) / (64 * sh.ty) : (ny + sh.ty - 1) / sh.ty;
+    const bool flat = sh.flat == 1;
+    t.tiles_x = flat ? 1 : (nx + 63) / 64;
+    t.tiles_y = flat ? (nx * ny + 64 * sh.ty - 1) / (64 * sh.ty) : (ny + sh.ty - 1) / sh.ty;
     t.zb = sh.zb;
     t.tiles_z = (nz + sh.zb - 1) / sh.zb;
     t.ntiles = t.tiles_x * t.tiles_y * t.tiles_z;
@@ -99,9 +104,16 @@ __global__ __launch_bounds__(64 * TY) void k_slab(OpView<float> op, TileMap tm, 
     __syncthreads();
     const int tile = xcd_tile(blockIdx.x, tm.nblocks);
     if (tile >= tm.ntiles) return;  // whole workgroup
-    const int tx = tile % tm.tiles_x;
-    const int ty = (tile / tm.tiles_x) % tm.tiles_y;
-    const int tz = tile / (tm.tiles_x * tm.tiles_y);
+    int tx, ty, tz;
+    if (FLAT == 2) {  // z-fastest order
+        tz = tile % tm.tiles_z;
+        tx = (tile / tm.tiles_z) % tm.tiles_x;
+        ty = tile / (tm.tiles_z * tm.tiles_x);
+    } else {
+        tx = tile % tm.tiles_x;
+        ty = (tile / tm.tiles_x) % tm.tiles_y;
+        tz = tile / (tm.tiles_x * tm.tiles_y);
+    }
     const int lane = threadIdx.x & 63;
     // Lanes past the grid edge keep running (they feed the DPP shifts) on a
     // clamped node and never store.  In a flat tile the previous/next lane is
@@ -109,7 +121,7 @@ __global__ __launch_bounds__(64 * TY) void k_slab(OpView<float> op, TileMap tm, 
     // the absent neighbour is zero and any finite value may stand in.
     bool valid;
     int i, j;
-    if (FLAT) {
+    if (FLAT == 1) {
         const int nxy = op.nx * op.ny;
         const int raw = ty * (64 * TY) + (int)threadIdx.x;
         const int pxy = min(raw, nxy - 1);
