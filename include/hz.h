@@ -210,6 +210,27 @@ hz_status hz_autotune(hz_operator *op, int32_t nrhs, int32_t max_chunk, int32_t 
                       int32_t preconditioner, int64_t memory_budget, int32_t *chunk_out,
                       int32_t *shapes_out, int32_t shapes_len, void *stream);
 
+/* Online response-basis path (phase_response_basis.py:177-253).  G is the
+ * phase response basis on the device: nrows rows (nodes, Fortran order) of
+ * m complex128 emitter entries, row stride ldg >= m, m <= 2048.  All
+ * vectors are device pointers.  Partial sums live in a per-device workspace
+ * kept by the library: calls that overlap on one device race on it.
+ *   hz_basis_field:   u = G w                         (field, :177-206)
+ *   hz_basis_adjoint: g = G^H c                       (phase_vjp, :208-253)
+ *   hz_basis_field_match: one pass over G for the reference's field_match
+ *     loss (phase_adjoint_optimizer.py:163-200): u = G w, loss
+ *     0.5 sum (weight (|u| - target))^2 into *loss, cotangent
+ *     c = weight^2 (|u| - target) u / (|u| + eps) and g = G^H c (m <= 512);
+ *     u is written when not null.  The phase gradient is
+ *     Re(conj(g) * i w) per emitter. */
+hz_status hz_basis_field(const void *G, int64_t ldg, int64_t nrows, int32_t m, const void *w, void *u,
+                         void *stream);
+hz_status hz_basis_adjoint(const void *G, int64_t ldg, int64_t nrows, int32_t m, const void *c, void *g,
+                           void *stream);
+hz_status hz_basis_field_match(const void *G, int64_t ldg, int64_t nrows, int32_t m, const void *w,
+                               const double *target, const double *weight, double eps, double *loss,
+                               void *g, void *u, void *stream);
+
 /* Material gradient, replacing ConservativeHelmholtz.material_vjp
  * (src/algorithm/physics/conservative_helmholtz.py:151-175): for
  * dL = -Re(lambda^H dA u) writes dL/drho and dL/dc per node (FP64, N each)

@@ -118,27 +118,6 @@ def test_autograd_losses_give_reference_cotangents(kind):
     np.testing.assert_allclose(u.grad.numpy(), ref[1], rtol=1e-8, atol=1e-10*np.abs(ref[1]).max())
 
 
-def test_basis_autograd_vjp_matches_oracle_and_fd():
-    shape = (6, 5, 4)
-    n, m = int(np.prod(shape)), 7
-    rng = np.random.default_rng(3)
-    g = rng.standard_normal((n, m)) + 1j*rng.standard_normal((n, m))
-    target = np.abs(rng.standard_normal(shape))
-    weight = rng.uniform(0.1, 1, shape)
-    ph = torch.tensor(rng.uniform(0, 2*np.pi, m), requires_grad=True)
-    u = hza.BasisPhaseField.apply(ph.reshape(1, -1), torch.as_tensor(g))[0]
-    val = hza.field_match_loss(hza.grid_view(u, shape), torch.as_tensor(target), torch.as_tensor(weight))
-    val.backward()
-    u_np = olo.basis_field(g, ph.detach().numpy(), shape)
-    _, cot = olo.field_match(u_np, target, weight)
-    ref = olo.phase_vjp(g, cot, ph.detach().numpy())
-    np.testing.assert_allclose(ph.grad.numpy(), ref, rtol=1e-9, atol=1e-12)
-    fd = olo.central_difference(
-        lambda p: olo.field_match(olo.basis_field(g, p, shape), target, weight, False)[0],
-        ph.detach().numpy(), 1e-6, count=m)
-    assert max(olo.scaled_gradient_error(a, b) for a, b in zip(ph.grad.numpy(), fd)) < 1e-6
-
-
 class _FakeSolver:
     """Stands in for the device solver to exercise storage and resume logic."""
     MODEL_VERSION = "fake"

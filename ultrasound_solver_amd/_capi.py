@@ -17,7 +17,7 @@ MODE_A, MODE_RAM, MODE_AH, MODE_RAMH = 0, 1, 2, 3
 EXPORTS = ("hz_create", "hz_destroy", "hz_apply", "hz_gmres", "hz_set_sources", "hz_build_rhs",
            "hz_gather_sources", "hz_sample", "hz_device_bytes", "hz_last_error", "hz_set_multigrid",
            "hz_autotune", "hz_set_tile_shapes", "hz_gmres_batch", "hz_release_workspace",
-           "hz_material_vjp")
+           "hz_material_vjp", "hz_basis_field", "hz_basis_adjoint", "hz_basis_field_match")
 PRE_JACOBI, PRE_MULTIGRID = 0, 1
 
 
@@ -89,6 +89,10 @@ def lib():
                                           POINTER(c_void_p), POINTER(c_int32), POINTER(c_int64),
                                           POINTER(POINTER(GmresStats)), c_int32, c_void_p]
         handle.hz_release_workspace.argtypes = [c_void_p]
+        handle.hz_basis_field.argtypes = [c_void_p, c_int64, c_int64, c_int32, c_void_p, c_void_p, c_void_p]
+        handle.hz_basis_adjoint.argtypes = [c_void_p, c_int64, c_int64, c_int32, c_void_p, c_void_p, c_void_p]
+        handle.hz_basis_field_match.argtypes = [c_void_p, c_int64, c_int64, c_int32, c_void_p, c_void_p, c_void_p,
+                                                c_double, c_void_p, c_void_p, c_void_p, c_void_p]
         handle.hz_material_vjp.argtypes = [c_void_p, c_void_p, c_void_p, c_int32, c_int64, c_void_p, c_void_p,
                                            c_void_p]
         handle.hz_set_tile_shapes.argtypes = [c_void_p, POINTER(c_int32), c_int32]

@@ -71,17 +71,24 @@ class PhaseField(torch.autograd.Function):
 
 
 class BasisPhaseField(torch.autograd.Function):
+    """u = G e^{i phi} for (B, M) phases on a device basis G (N, M), through
+    the library's one-pass basis kernels; backward is G^H c per row."""
+
     @staticmethod
     def forward(ctx, phases, basis):
+        from . import basis as dev_basis
         ctx.basis = basis
         ctx.save_for_backward(phases)
-        return (basis @ torch.exp(1j*phases.to(torch.float64)).T).T
+        w = torch.exp(1j*phases.to(torch.float64))
+        return torch.stack([dev_basis.field(basis, w[b]) for b in range(w.shape[0])])
 
     @staticmethod
     def backward(ctx, grad_u):
+        from . import basis as dev_basis
         (phases,) = ctx.saved_tensors
-        resp = (ctx.basis.conj().T @ grad_u.T).T
-        grad = torch.real(torch.conj(resp)*(1j*torch.exp(1j*phases.to(torch.float64))))
+        w = torch.exp(1j*phases.to(torch.float64))
+        resp = torch.stack([dev_basis.adjoint(ctx.basis, grad_u[b].contiguous()) for b in range(w.shape[0])])
+        grad = dev_basis.phase_gradient(resp, w)
         return grad.to(phases.dtype), None
 
 

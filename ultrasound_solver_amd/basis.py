@@ -1,0 +1,76 @@
+"""Device response-basis operations on a phase response basis G held as an
+(N, M) complex128 torch tensor (rows = nodes in Fortran order), through the
+library's one-pass kernels (phase_response_basis.py:177-253,
+phase_adjoint_optimizer.py:163-200)."""
+import ctypes
+
+import torch
+
+from . import _capi
+from .operator import _stream
+
+EPS = 1.0e-12  # amplitude regulariser of the reference's cotangents
+
+
+def _check(G):
+    if not isinstance(G, torch.Tensor) or G.dtype != torch.complex128 or G.dim() != 2 or not G.is_cuda:
+        raise ValueError("G must be a (N, M) complex128 device tensor")
+    if G.stride(1) != 1:
+        raise ValueError("G rows must be contiguous")
+    return G.shape[0], G.shape[1], G.stride(0)
+
+
+def _vec(x, n, like):
+    x = torch.as_tensor(x, device=like.device).to(torch.complex128).contiguous().reshape(-1)
+    if x.numel() != n:
+        raise ValueError(f"expected {n} entries, got {x.numel()}")
+    return x
+
+
+def field(G, w):
+    """u = G w."""
+    n, m, ld = _check(G)
+    w = _vec(w, m, G)
+    u = torch.empty(n, dtype=torch.complex128, device=G.device)
+    _capi.check(_capi.lib().hz_basis_field(ctypes.c_void_p(G.data_ptr()), ld, n, m, ctypes.c_void_p(w.data_ptr()),
+                                           ctypes.c_void_p(u.data_ptr()), _stream()), "hz_basis_field")
+    return u
+
+
+def adjoint(G, c):
+    """g = G^H c."""
+    n, m, ld = _check(G)
+    c = _vec(c, n, G)
+    g = torch.empty(m, dtype=torch.complex128, device=G.device)
+    _capi.check(_capi.lib().hz_basis_adjoint(ctypes.c_void_p(G.data_ptr()), ld, n, m, ctypes.c_void_p(c.data_ptr()),
+                                             ctypes.c_void_p(g.data_ptr()), _stream()), "hz_basis_adjoint")
+    return g
+
+
+def phase_gradient(g, w):
+    """dL/dphi = Re(conj(G^H c) * i w) from g = G^H c and w = a e^{i phi}."""
+    return torch.real(torch.conj(g)*(1j*w))
+
+
+def field_match(G, phases, target, weight, eps=EPS, return_field=False):
+    """One pass over G: (loss, dL/dphi[, u]) of the reference's field_match
+    loss 0.5 |W (|u| - a)|^2 with u = G e^{i phi} (target a, weight W given
+    per node in Fortran order)."""
+    n, m, ld = _check(G)
+    ph = torch.as_tensor(phases, device=G.device, dtype=torch.float64).reshape(-1)
+    if ph.numel() != m:
+        raise ValueError(f"expected {m} phases")
+    w = torch.exp(1j*ph).contiguous()
+    tgt = torch.as_tensor(target, device=G.device, dtype=torch.float64).contiguous().reshape(-1)
+    wt = torch.as_tensor(weight, device=G.device, dtype=torch.float64).contiguous().reshape(-1)
+    if tgt.numel() != n or wt.numel() != n:
+        raise ValueError("target and weight need one entry per basis row")
+    loss = torch.empty(1, dtype=torch.float64, device=G.device)
+    g = torch.empty(m, dtype=torch.complex128, device=G.device)
+    u = torch.empty(n, dtype=torch.complex128, device=G.device) if return_field else None
+    _capi.check(_capi.lib().hz_basis_field_match(
+        ctypes.c_void_p(G.data_ptr()), ld, n, m, ctypes.c_void_p(w.data_ptr()), ctypes.c_void_p(tgt.data_ptr()),
+        ctypes.c_void_p(wt.data_ptr()), float(eps), ctypes.c_void_p(loss.data_ptr()), ctypes.c_void_p(g.data_ptr()),
+        ctypes.c_void_p(u.data_ptr() if u is not None else 0), _stream()), "hz_basis_field_match")
+    out = (loss[0], phase_gradient(g, w))
+    return out + (u,) if return_field else out

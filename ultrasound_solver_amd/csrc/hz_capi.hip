@@ -13,9 +13,12 @@
 #include <type_traits>
 #include <vector>
 #include <thread>
+#include <mutex>
+#include <map>
 #include <atomic>
 
 #include "../../include/hz.h"
+#include "hz_basis.hpp"
 #include "hz_grad.hpp"
 #include "hz_krylov.hpp"
 #include "hz_mg.hpp"
@@ -1046,6 +1049,26 @@ void check_op(const hz_operator *op) {
     if (!op) throw BadArgument("null operator handle");
 }
 
+// Partial-sum workspace of the response-basis kernels, one per device,
+// grow-only for the life of the process.
+DevBuf &basis_workspace(size_t bytes) {
+    static std::mutex mu;
+    static std::map<int, std::unique_ptr<DevBuf>> bufs;
+    int dev = 0;
+    HZ_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    auto &b = bufs[dev];
+    if (!b) b = std::make_unique<DevBuf>();
+    b->ensure(bytes);
+    return *b;
+}
+
+void check_basis_args(const void *G, int64_t ldg, int64_t nrows, int32_t m) {
+    if (!G) throw BadArgument("G must be a device pointer");
+    if (nrows < 1 || m < 1 || ldg < m) throw BadArgument("need nrows >= 1, m >= 1 and ldg >= m");
+    if (m > BASIS_MAX_M) throw BadArgument("at most 2048 emitters per basis row");
+}
+
 }  // namespace
 
 extern "C" {
@@ -1596,6 +1619,62 @@ hz_status hz_autotune(hz_operator *op, int32_t nrhs, int32_t max_chunk, int32_t 
                 for (const auto &L : op->mg->levels)
                     for (int e = 0; e < 3 && at < shapes_len; ++e) shapes_out[at++] = L->shape[e];
         }
+    });
+}
+
+hz_status hz_basis_field(const void *G, int64_t ldg, int64_t nrows, int32_t m, const void *w, void *u,
+                         void *stream) {
+    return guarded([&] {
+        check_basis_args(G, ldg, nrows, m);
+        if (!w || !u) throw BadArgument("w and u must be device pointers");
+        const int nb = (int)std::min<int64_t>(4096, (nrows + 3) / 4);
+        hipLaunchKernelGGL(k_basis_field, dim3(nb), dim3(256), 0, static_cast<hipStream_t>(stream),
+                           static_cast<const cx<double> *>(G), ldg, nrows, (int)m, static_cast<const cx<double> *>(w),
+                           static_cast<cx<double> *>(u));
+        HZ_LAUNCHED();
+    });
+}
+
+hz_status hz_basis_adjoint(const void *G, int64_t ldg, int64_t nrows, int32_t m, const void *c, void *g,
+                           void *stream) {
+    return guarded([&] {
+        check_basis_args(G, ldg, nrows, m);
+        if (!c || !g) throw BadArgument("c and g must be device pointers");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        const int nb = (int)std::min<int64_t>(1024, nrows);
+        const int64_t rows = (nrows + nb - 1) / nb;
+        DevBuf &ws = basis_workspace((size_t)nb * m * sizeof(cx<double>));
+        hipLaunchKernelGGL(k_basis_adjoint, dim3(nb), dim3(256), 0, s, static_cast<const cx<double> *>(G), ldg, nrows,
+                           (int)m, static_cast<const cx<double> *>(c), rows, ws.as<cx<double>>());
+        HZ_LAUNCHED();
+        hipLaunchKernelGGL(k_basis_combine, dim3((m + 255) / 256), dim3(256), 0, s,
+                           (const cx<double> *)ws.as<cx<double>>(), nb, (int)m, static_cast<cx<double> *>(g),
+                           (const double *)nullptr, (double *)nullptr);
+        HZ_LAUNCHED();
+    });
+}
+
+hz_status hz_basis_field_match(const void *G, int64_t ldg, int64_t nrows, int32_t m, const void *w,
+                               const double *target, const double *weight, double eps, double *loss, void *g, void *u,
+                               void *stream) {
+    return guarded([&] {
+        check_basis_args(G, ldg, nrows, m);
+        if (m > BASIS_FUSED_MAX_M) throw BadArgument("the fused field_match kernel takes at most 512 emitters");
+        if (!w || !target || !weight || !loss || !g) throw BadArgument("w, target, weight, loss and g are required");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        const int nb = (int)std::min<int64_t>(1024, (nrows + 3) / 4);
+        const int64_t nwaves = (int64_t)nb * 4;
+        const int64_t rows = (nrows + nwaves - 1) / nwaves;
+        DevBuf &ws = basis_workspace((size_t)nwaves * m * sizeof(cx<double>) + (size_t)nwaves * sizeof(double));
+        cx<double> *part = ws.as<cx<double>>();
+        double *lpart = reinterpret_cast<double *>(part + nwaves * m);
+        hipLaunchKernelGGL(k_basis_field_match, dim3(nb), dim3(256), 0, s, static_cast<const cx<double> *>(G), ldg,
+                           nrows, (int)m, static_cast<const cx<double> *>(w), target, weight, eps, rows, part, lpart,
+                           static_cast<cx<double> *>(u));
+        HZ_LAUNCHED();
+        hipLaunchKernelGGL(k_basis_combine, dim3((m + 255) / 256), dim3(256), 0, s, (const cx<double> *)part,
+                           (int)nwaves, (int)m, static_cast<cx<double> *>(g), (const double *)lpart, loss);
+        HZ_LAUNCHED();
     });
 }
 

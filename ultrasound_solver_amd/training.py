@@ -160,8 +160,9 @@ class PhaseResponseBasis:
             raise ValueError(f"phases must have shape ({self.num_transducers},), got {tuple(phases.shape)}")
         d = self.solver.cfg.domain
         if self.basis_gpu is not None:
+            from . import basis as dev_basis
             ph = torch.as_tensor(phases, dtype=torch.float64, device=self.basis_gpu.device)
-            u = self.basis_gpu @ torch.exp(1j*ph)
+            u = dev_basis.field(self.basis_gpu, torch.exp(1j*ph))
             if return_numpy:
                 return u.cpu().numpy().reshape((d.nx, d.ny, d.nz), order="F")
             # (nx, ny, nz) view of the Fortran-ordered vector
@@ -185,9 +186,10 @@ class PhaseResponseBasis:
                 c = torch.as_tensor(np.asarray(field_cotangent, np.complex128).reshape(-1, order="F"), device=dev)
             if c.numel() != self.solver.total_dofs:
                 raise ValueError("field_cotangent has the wrong size")
-            resp = self.basis_gpu.conj().transpose(0, 1) @ c
+            from . import basis as dev_basis
+            resp = dev_basis.adjoint(self.basis_gpu, c)
             w = torch.exp(1j*torch.as_tensor(np.asarray(phases, np.float64), device=dev))
-            g = torch.real(torch.conj(resp)*(1j*w))
+            g = dev_basis.phase_gradient(resp, w)
             return g.cpu().numpy() if return_numpy else g
         c = np.asarray(field_cotangent, dtype=np.complex128).reshape(-1, order="F")
         if c.size != self.solver.total_dofs:
