@@ -45,6 +45,7 @@ def parse():
     p.add_argument("--mg-pre", type=int, default=1)
     p.add_argument("--mg-post", type=int, default=3)
     p.add_argument("--mg-weight", type=float, default=0.7)
+    p.add_argument("--mg-cycle", type=int, default=1, choices=[1, 2], help="1: V-cycle, 2: W below the finest level")
     p.add_argument("--graph", type=int, default=0, help="replay each GMRES cycle from a captured hipGraph")
     p.add_argument("--arnoldi-block", type=int, default=2, choices=[1, 2],
                    help="mixed form: basis vectors per Gram-Schmidt pass")
@@ -89,7 +90,8 @@ def main():
         stencil_only(op, args, scene.num_sources)
         return
     if args.precond == "multigrid":
-        op.set_multigrid(shift=args.mg_shift, pre=args.mg_pre, post=args.mg_post, weight=args.mg_weight)
+        op.set_multigrid(shift=args.mg_shift, pre=args.mg_pre, post=args.mg_post, weight=args.mg_weight,
+                         cycle=args.mg_cycle)
     m = scene.num_sources
     tuning = op.autotune(m, max_chunk=args.chunk or 64, restart=args.restart, preconditioner=args.precond)
     if not args.chunk:

@@ -107,6 +107,7 @@ struct MgLevel {
 struct Multigrid {
     std::vector<std::unique_ptr<MgLevel>> levels;
     int pre = 1, post = 1, coarse_sweeps = 30;
+    int cycle = 1;  // coarse-grid visits per level below the finest (1: V, 2: W)
     float weight = 0.7f;
     double shift = 0.5;
     int kcap = 0;
@@ -382,6 +383,21 @@ void vcycle(Multigrid &mg, int l, const cx<float> *b, cx<float> *x, int k, const
         for (int it = 0; it < mg.post; ++it) {
             level_smooth<MODE>(L, cur, b, alt, mg.weight, k, live, bdiv, s);
             std::swap(cur, alt);
+        }
+        // W-cycle below the finest level: further coarse-grid corrections of x.
+        for (int visit = 1; l >= 1 && visit < mg.cycle; ++visit) {
+            cx<float> *t = L.tt.as<cx<float>>();
+            launch_slab<MODE, SLAB_RESID>(L.shape[1], L.v, x, L.n, t, L.n, k, live, nullptr, b, nullptr, 0.0f, bdiv,
+                                          s);
+            launch_restrict(L, C, t, k, live, s);
+            vcycle<MODE>(mg, l + 1, C.bb.as<cx<float>>(), C.xx.as<cx<float>>(), k, live, s);
+            launch_prolong(L, C, x, k, live, nullptr, dinv, mg.weight, bdiv, 0, s);
+            cx<float> *c2 = x, *a2 = t;
+            for (int it = 0; it < mg.post; ++it) {
+                level_smooth<MODE>(L, c2, b, a2, mg.weight, k, live, bdiv, s);
+                std::swap(c2, a2);
+            }
+            if (c2 != x) HZ_HIP(hipMemcpyAsync(x, c2, (size_t)k * n * sizeof(cx<float>), hipMemcpyDeviceToDevice, s));
         }
         return;
     }
@@ -1318,13 +1334,15 @@ hz_status hz_gmres_batch(int32_t nscenes, hz_operator *const *ops, const hz_gmre
 }
 
 hz_status hz_set_multigrid(hz_operator *op, int32_t nlevels, const hz_level_desc *levels, double shift, int32_t pre,
-                           int32_t post, int32_t coarse_sweeps, double weight) {
+                           int32_t post, int32_t coarse_sweeps, double weight, int32_t cycle) {
     return guarded([&] {
         check_op(op);
         if (nlevels < 1 || !levels) throw BadArgument("need at least one level");
         if (pre < 1 || post < 0 || coarse_sweeps < 1 || !(weight > 0) || !std::isfinite(shift))
             throw BadArgument("bad multigrid smoothing parameters");
+        if (cycle < 1 || cycle > 2) throw BadArgument("cycle must be 1 (V) or 2 (W)");
         auto mg = std::make_unique<Multigrid>();
+        mg->cycle = cycle;
         mg->pre = pre;
         mg->post = post;
         mg->coarse_sweeps = coarse_sweeps;
