@@ -46,86 +46,6 @@ __global__ __launch_bounds__(256) void k_level_dinv(OpView<T> op, cx<T> *__restr
     dinv[p] = node_fixed(op, i, j, k) ? mk<T>(1, 0) : crecip(d);
 }
 
-enum LevelEpi { EPI_SMOOTH = 0, EPI_RESID = 1 };
-
-// EPI_SMOOTH: xo = xi + w * dinv * (b - A xi)      (damped Jacobi sweep)
-// EPI_RESID : xo = b - A xi
-// MODE is MODE_A (forward) or MODE_AH (adjoint; dinv conjugated).  When
-// `bdiv` is set the right-hand side is read as b / bdiv: on the finest level
-// this undoes the row scale of the Krylov system, so that R A (P R^-1) is a
-// similarity transform of A P and its spectrum does not inherit the spread
-// of R (identity rows have scale 1, the others ~1/|A_PP|).
-template <typename T, int MODE, int EPI>
-__global__ __launch_bounds__(256) void k_level_op(OpView<T> op, TileMap tm, const cx<T> *__restrict__ xi,
-                                                  const cx<T> *__restrict__ b, cx<T> *__restrict__ xo,
-                                                  const cx<T> *__restrict__ dinv, T w, int k,
-                                                  const int *__restrict__ live, const T *__restrict__ bdiv) {
-    __shared__ int s_act[APPLY_MAX_RHS];
-    __shared__ int s_na;
-    if (threadIdx.x == 0) {
-        int na = 0;
-        for (int r = 0; r < k; ++r)
-            if (!live || live[r]) s_act[na++] = r;
-        s_na = na;
-    }
-    __syncthreads();
-    const int tile = xcd_tile(blockIdx.x, tm.nblocks);
-    if (tile >= tm.ntiles) return;
-    const int tx = tile % tm.tiles_x;
-    const int ty = (tile / tm.tiles_x) % tm.tiles_y;
-    const int tz = tile / (tm.tiles_x * tm.tiles_y);
-    const int i = tx * 64 + (threadIdx.x & 63);
-    const int j = ty * 4 + (threadIdx.x >> 6);
-    if (i >= op.nx || j >= op.ny) return;
-    const int64_t n = op.n;
-    const int na = s_na;
-    const int k0 = tz * tm.zb;
-    const int k1 = min(op.nz, k0 + tm.zb);
-    for (int kk = k0; kk < k1; ++kk) {
-        const int64_t p = i + (int64_t)op.nx * (j + (int64_t)op.ny * kk);
-        const Nbr nb = neighbours(i, j, kk, op.nx, op.ny, op.nz);
-        cx<T> cs, c[6];
-        row_coefs<T, MODE>(op, i, j, kk, p, nb, cs, c);
-        cx<T> dv = mk<T>(0, 0);
-        if (EPI == EPI_SMOOTH) dv = (MODE == MODE_AH) ? conj(dinv[p]) : dinv[p];
-        const T bs = bdiv ? T(1) / bdiv[p] : T(1);
-        int t = 0;
-        for (; t + 2 <= na; t += 2) {
-            const int r0 = s_act[t], r1 = s_act[t + 1];
-            const cx<T> *x0 = xi + r0 * n + p;
-            const cx<T> *x1 = xi + r1 * n + p;
-            cx<T> u0[7], u1[7];
-            u0[6] = x0[0];
-            u1[6] = x1[0];
-#pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                u0[q] = x0[nb.off[q]];
-                u1[q] = x1[nb.off[q]];
-            }
-            const cx<T> b0 = b[r0 * n + p], b1 = b[r1 * n + p];
-            cx<T> a0 = cs * u0[6], a1 = cs * u1[6];
-#pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                a0 = a0 + c[q] * u0[q];
-                a1 = a1 + c[q] * u1[q];
-            }
-            const cx<T> e0 = bs * b0 - a0, e1 = bs * b1 - a1;
-            xo[r0 * n + p] = (EPI == EPI_SMOOTH) ? u0[6] + w * (dv * e0) : e0;
-            xo[r1 * n + p] = (EPI == EPI_SMOOTH) ? u1[6] + w * (dv * e1) : e1;
-        }
-        if (t < na) {
-            const int r0 = s_act[t];
-            const cx<T> *x0 = xi + r0 * n + p;
-            const cx<T> u06 = x0[0];
-            cx<T> a0 = cs * u06;
-#pragma unroll
-            for (int q = 0; q < 6; ++q) a0 = a0 + c[q] * x0[nb.off[q]];
-            const cx<T> e0 = bs * b[r0 * n + p] - a0;
-            xo[r0 * n + p] = (EPI == EPI_SMOOTH) ? u06 + w * (dv * e0) : e0;
-        }
-    }
-}
-
 // First sweep from a zero guess: xo = w * dinv * b.
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_level_jacobi0(const cx<T> *__restrict__ b, cx<T> *__restrict__ xo,
