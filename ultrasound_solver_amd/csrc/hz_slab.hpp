@@ -5,11 +5,12 @@
 // z-planes; the autotuner picks (ZB, TY) per grid from a few shapes.  The
 // row coefficients of the thread's ZB nodes are computed once and kept in
 // registers; then, for each right-hand side, the thread issues all loads of
-// its column (planes k0-1 .. k0+ZB, the two y-neighbours of each plane and,
-// in lanes 0 and 63 only, the x-neighbour across the tile edge), takes the
-// other x-neighbours from the adjacent lanes with DPP wave shifts, and
+// its column (planes k0-1 .. k0+ZB, in the first and last tile rows the halo
+// row beyond the tile and, in lanes 0 and 63 only, the x-neighbour across the
+// tile edge), takes the other x-neighbours from the adjacent lanes with DPP
+// wave shifts and the other y-neighbours from the block's planes in LDS, and
 // marches z with the (z-1, z, z+1) values in registers.  That is about
-// (ZB + 2 + 2 ZB) / ZB vector loads per node and right-hand side instead of 7.
+// (ZB + 2) / ZB + 2 / TY vector loads per node and right-hand side instead of 7.
 #pragma once
 
 #include "hz_stencil.hpp"
@@ -91,19 +92,15 @@ __global__ __launch_bounds__(64 * TY) void k_slab(OpView<float> op, TileMap tm, 
     __shared__ int s_act[APPLY_MAX_RHS];
     __shared__ float s_sc[APPLY_MAX_RHS];
     __shared__ int s_na;
-    if (threadIdx.x == 0) {
-        int na = 0;
-        for (int r = 0; r < nrhs; ++r) {
-            if (live && !live[r]) continue;
-            s_act[na] = r;
-            s_sc[na] = scale ? (float)scale[r] : 1.0f;
-            ++na;
-        }
-        s_na = na;
-    }
-    __syncthreads();
+    // Rectangular tiles take the y-neighbours of interior rows from the
+    // block's own planes in LDS (double-buffered over passes, one barrier per
+    // pass); only the first and last tile rows load a halo row.  Separate
+    // global loads of the y-neighbour rows cost a third of the throughput.
+    constexpr bool YLDS = FLAT != 1;
+    __shared__ cx<float> s_pl[YLDS ? 2 : 1][YLDS ? RP : 1][YLDS ? TY : 1][YLDS ? ZB : 1][64];
     const int tile = xcd_tile(blockIdx.x, tm.nblocks);
     if (tile >= tm.ntiles) return;  // whole workgroup
+    stage_live_columns<float>(nrhs, live, scale, s_act, s_sc, &s_na);
     int tx, ty, tz;
     if (FLAT == 2) {  // z-fastest order
         tz = tile % tm.tiles_z;
@@ -115,6 +112,7 @@ __global__ __launch_bounds__(64 * TY) void k_slab(OpView<float> op, TileMap tm, 
         tz = tile / (tm.tiles_x * tm.tiles_y);
     }
     const int lane = threadIdx.x & 63;
+    const int row = threadIdx.x >> 6;
     // Lanes past the grid edge keep running (they feed the DPP shifts) on a
     // clamped node and never store.  In a flat tile the previous/next lane is
     // x-1/x+1 of the same row except at a row end, where the coefficient to
@@ -158,14 +156,16 @@ __global__ __launch_bounds__(64 * TY) void k_slab(OpView<float> op, TileMap tm, 
             for (int e = 0; e < 6; ++e) c[q][e] = c[q][e] * first_sweep_factor<MODE>(dinv, bdiv, w, p + nb.off[e]);
         }
     }
+    __syncthreads();  // s_act: staged while the coefficients were computed
     const bool has_xm = i > 0, has_xp = i < op.nx - 1;
     const int64_t oym = j > 0 ? -(int64_t)op.nx : 0;
     const int64_t oyp = j < op.ny - 1 ? (int64_t)op.nx : 0;
     const int na = s_na;
+    int buf = 0;
     for (int t = 0; t < na; t += RP) {
         // RP right-hand sides per pass.  Every load of their columns is
         // issued before any arithmetic: planes k0-1 .. k0+ZB (absent planes
-        // read a valid node and are zeroed), the y-neighbours, the x-edge
+        // read a valid node and are zeroed), the halo rows, the x-edge
         // values of lanes 0 and 63, and b for the SMOOTH/RESID epilogues.  A
         // pass past the last column repeats it and does not store.
         int rr[RP];
@@ -185,14 +185,31 @@ __global__ __launch_bounds__(64 * TY) void k_slab(OpView<float> op, TileMap tm, 
             for (int q = 0; q < ZB; ++q) {
                 const int kc = min(k0 + q, op.nz - 1);
                 const int64_t p = p0 + (int64_t)(kc - k0) * sxy;
-                ym[u][q] = xr[p + oym];
-                yp[u][q] = xr[p + oyp];
+                ym[u][q] = mk<float>(0, 0);
+                yp[u][q] = mk<float>(0, 0);
+                if (!YLDS || row == 0) ym[u][q] = xr[p + oym];
+                if (!YLDS || row == TY - 1) yp[u][q] = xr[p + oyp];
                 xe[u][q] = mk<float>(0, 0);
                 if (lane == 0 && has_xm) xe[u][q] = xr[p - 1];
                 if (lane == 63 && has_xp) xe[u][q] = xr[p + 1];
                 bb[u][q] = mk<float>(0, 0);
                 if (EPI == SLAB_SMOOTH || EPI == SLAB_RESID) bb[u][q] = b[rr[u] * ldy + p];
             }
+        }
+        if (YLDS) {
+#pragma unroll
+            for (int u = 0; u < RP; ++u)
+#pragma unroll
+                for (int q = 0; q < ZB; ++q) s_pl[buf][u][row][q][lane] = v[u][q + 1];
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < RP; ++u)
+#pragma unroll
+                for (int q = 0; q < ZB; ++q) {
+                    if (row > 0) ym[u][q] = s_pl[buf][u][row - 1][q][lane];
+                    if (row < TY - 1) yp[u][q] = s_pl[buf][u][row + 1][q][lane];
+                }
+            buf ^= 1;
         }
 #pragma unroll
         for (int u = 0; u < RP; ++u) {

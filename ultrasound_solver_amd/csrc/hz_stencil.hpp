@@ -175,6 +175,32 @@ __device__ __forceinline__ void row_coefs(const OpView<T> &op, int i, int j, int
 
 constexpr int APPLY_MAX_RHS = 256;
 
+// Compacts the live right-hand sides (all of them when live is null) into
+// s_act and their scales into s_sc, with one wave testing 64 columns per
+// round: one load latency per round instead of one per column.  Called by
+// the whole workgroup; the caller's barrier publishes the result.
+template <typename S>
+__device__ __forceinline__ void stage_live_columns(int nrhs, const int *__restrict__ live,
+                                                   const double *__restrict__ scale, int *s_act, S *s_sc,
+                                                   int *s_na) {
+    if (threadIdx.x >= 64) return;
+    const int lane = threadIdx.x;
+    const uint64_t below = (uint64_t(1) << lane) - 1;
+    int base = 0;
+    for (int r0 = 0; r0 < nrhs; r0 += 64) {
+        const int r = r0 + lane;
+        const bool on = r < nrhs && (!live || live[r] != 0);
+        const uint64_t mask = __ballot(on);
+        if (on) {
+            const int pos = base + __popcll(mask & below);
+            s_act[pos] = r;
+            s_sc[pos] = scale ? (S)scale[r] : S(1);
+        }
+        base += __popcll(mask);
+    }
+    if (lane == 0) *s_na = base;
+}
+
 // y[:, r] = s_r * op(x[:, r]) for active r.  One coefficient evaluation per
 // node serves every right-hand side; the active list and scales are staged in
 // LDS and the right-hand sides are processed in pairs to keep 14 loads in flight.
@@ -185,16 +211,7 @@ __global__ __launch_bounds__(256) void k_apply(OpView<T> op, TileMap tm, const c
     __shared__ int s_act[APPLY_MAX_RHS];
     __shared__ T s_sc[APPLY_MAX_RHS];
     __shared__ int s_na;
-    if (threadIdx.x == 0) {
-        int na = 0;
-        for (int r = 0; r < nrhs; ++r) {
-            if (live && !live[r]) continue;
-            s_act[na] = r;
-            s_sc[na] = scale ? (T)scale[r] : T(1);
-            ++na;
-        }
-        s_na = na;
-    }
+    stage_live_columns<T>(nrhs, live, scale, s_act, s_sc, &s_na);
     __syncthreads();
     const int tile = xcd_tile(blockIdx.x, tm.nblocks);
     if (tile >= tm.ntiles) return;
