@@ -36,6 +36,11 @@
  *                      PhaseResponseBasis.phase_vjp (phase_response_basis.py:208-253).
  *   hz_sample          trilinear rows of PhaseResponseBasis.target_response
  *                      (phase_response_basis.py:165-174).
+ *   hz_z_transfer, hz_surface_*
+ *                      HomogeneousSurfaceSolver (src/algorithm/solvers/
+ *                      homogeneous_surface.py:41-259), the exact target
+ *                      response of HelmholtzDirectSolver.target_response
+ *                      (helmholtz_solver.py:179-208).
  */
 #ifndef HZ_H
 #define HZ_H
@@ -255,6 +260,77 @@ hz_status hz_set_tile_shapes(hz_operator *op, const int32_t *shapes, int32_t len
 
 /* Device bytes held by the operator (coefficients plus cached workspace). */
 int64_t hz_device_bytes(const hz_operator *op);
+
+/* ---- Homogeneous surface condensation -------------------------------------
+ * The exact strategy of HomogeneousSurfaceSolver
+ * (src/algorithm/solvers/homogeneous_surface.py) for a homogeneous medium
+ * over a reflecting -z baffle: modes of the x and y axis operators decouple
+ * into tridiagonal z systems, solved by transfer ratios H[j] = u[j+1]/u[0]
+ * and the surface Schur complement.  The eigendecompositions of the two
+ * axis operators are computed by the caller (host LAPACK, as the reference
+ * does, :20-38) and passed in.  Planes are nx x ny complex128, x fastest. */
+typedef struct hz_surface hz_surface;
+
+typedef struct hz_surface_desc {
+    int32_t nx, ny, nz;
+    double dx;
+    const double *vx, *ix;      /* complex nx x nx, column-major (host)      */
+    const double *vy, *iy;      /* complex ny x ny: eigenvectors, inverse    */
+    const double *shift;        /* complex nx x ny, x fastest:
+                                   lambda_x + lambda_y + (omega / c)^2       */
+    const double *z_diagonal;   /* complex nz: diagonal of the z operator    */
+    const double *z_upper;      /* complex nz-1: z[j, j+1]                   */
+    const double *z_lower;      /* complex nz-1: z[j+1, j]                   */
+} hz_surface_desc;
+
+/* Transfer ratios and Schur complement of `modes` independent z systems
+ * (device complex128): h is (nz-1) x modes, mode fastest; schur is modes.
+ * Replaces the CUDA kernel z_transfer (homogeneous_surface.py:41-82) and
+ * the Schur line (:164). */
+hz_status hz_z_transfer(const void *shift, const void *diagonal, const void *upper, const void *lower,
+                        int32_t nz, int64_t modes, void *h, void *schur, void *stream);
+
+/* HomogeneousSurfaceSolver.__init__ after the eigendecompositions
+ * (homogeneous_surface.py:140-164): uploads the tables and runs
+ * hz_z_transfer.  nx, ny >= 2, nz >= 2. */
+hz_status hz_surface_create(const hz_surface_desc *desc, hz_surface **out);
+hz_status hz_surface_destroy(hz_surface *s);
+
+/* Copy H ((nz-1) x nx*ny) and the Schur complement (nx*ny) into device buffers. */
+hz_status hz_surface_tables(const hz_surface *s, void *h, void *schur, void *stream);
+
+/* Source masses (nx*ny x M, real) as nnz (plane node, column, value)
+ * triplets on the host: the masses argument of flux_target_response and
+ * solve_flux (homogeneous_surface.py:175-187). */
+hz_status hz_surface_set_sources(hz_surface *s, int32_t num_columns, int64_t nnz, const int64_t *node,
+                                 const int32_t *column, const double *value);
+
+/* y[p] = ix x[p] iy^T (inverse = 0) or vx x[p] vy^T (inverse = 1) for
+ * `count` device planes: transform / inverse_transform (:166-173). */
+hz_status hz_surface_transform(hz_surface *s, int32_t inverse, const void *x, void *y, int32_t count,
+                               void *stream);
+
+/* solve_flux (:207-229) for the emitter weights w = a*exp(i*phi) (device
+ * complex128, M): surface load -4 pi/dx^3 masses w into rhs, surface modes
+ * (also kept for hz_surface_planes / hz_surface_sample) into modes and the
+ * surface plane into `surface`; each output may be NULL. */
+hz_status hz_surface_solve_flux(hz_surface *s, const void *w, void *rhs, void *modes, void *surface, void *stream);
+
+/* Planes z[0..count) of the last solved field into out (count planes;
+ * z is a host array): plane (:231-236). */
+hz_status hz_surface_planes(hz_surface *s, const int32_t *z, int32_t count, void *out, void *stream);
+
+/* Trilinear samples of the last solved field at npoints points given by
+ * their low grid corner (host int32, 3 per point) and fractions (host
+ * double, 3 per point); out is device complex128 (npoints): sample (:250-259). */
+hz_status hz_surface_sample(hz_surface *s, const int32_t *low, const double *frac, int32_t npoints, void *out,
+                            void *stream);
+
+/* out[p, m] (device complex128, npoints x M, row-major): the response of
+ * point p to emitter m with ordinary transposes, so that p = out w
+ * (flux_target_response, :189-205). */
+hz_status hz_surface_target_response(hz_surface *s, const int32_t *low, const double *frac, int32_t npoints,
+                                     void *out, void *stream);
 
 const char *hz_last_error(void);
 

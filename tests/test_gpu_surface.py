@@ -1,0 +1,136 @@
+"""Device surface condensation (hz_z_transfer, hz_surface_*) against the
+CPU oracle (oracle/surface.py, pinned to the 3-D direct solve on the CPU).
+
+FP64 throughout; products are summed in a different order than numpy's, so
+results agree to rounding: relative 1e-11 of each quantity's magnitude.
+"""
+import numpy as np
+import pytest
+
+from oracle import source as osrc, surface as osu
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+SHAPE = (22, 20, 15)
+DX = 1.0e-3
+FREQ = 40e3
+C0, RHO0 = 343.0, 1.2
+BOUNDARIES = {"-x": "open", "+x": "open", "-y": {"type": "impedance", "impedance": [823.2, 205.8]},
+              "+y": "reflecting", "-z": "reflecting", "+z": "open"}
+PML = {"cells": 3, "strength": 2.0, "order": 3}
+CENTERS = np.array([[8.5e-3, 9.2e-3, 0.0], [13.1e-3, 10.4e-3, 0.0]])
+DIAMETER = 4.2e-3
+POINTS = np.array([[10.3e-3, 9.9e-3, 6.2e-3], [4.0e-3, 15.5e-3, 0.0], [12.0e-3, 7.0e-3, 11.75e-3],
+                   [21.0e-3, 19.0e-3, 14.0e-3]])
+
+
+def close(a, b, rel=1e-11):
+    a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape, (a.shape, b.shape)
+    err = np.abs(a - b).max()
+    assert err <= rel*np.abs(b).max(), (err, np.abs(b).max())
+
+
+@pytest.fixture(scope="module")
+def pair():
+    from ultrasound_solver_amd.surface import HomogeneousSurfaceSolver
+    dev = HomogeneousSurfaceSolver(SHAPE, DX, FREQ, C0, RHO0, BOUNDARIES, PML, device="cuda:0")
+    ref = osu.SurfaceSolver(SHAPE, DX, FREQ, C0, RHO0, BOUNDARIES, PML)
+    return dev, ref
+
+
+def masses():
+    return osrc.source_mass_matrix(SHAPE[:2], DX, CENTERS, DIAMETER, 8, 16)
+
+
+def test_z_transfer_entry_point_matches_the_recurrence():
+    import ctypes
+    from ultrasound_solver_amd import _capi
+    rng = np.random.default_rng(5)
+    nz, modes = 19, 333
+    z = np.diag(rng.standard_normal(nz) + 1j*rng.standard_normal(nz) - 4) \
+        + np.diag(1 + 0.1j*rng.standard_normal(nz - 1), 1) + np.diag(1 + 0.1j*rng.standard_normal(nz - 1), -1)
+    shift = (rng.standard_normal(modes) + 1j*rng.standard_normal(modes))*2
+    want = osu.z_transfer(shift, z)
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.complex128, device="cuda:0")
+    s, d, u, lo = dev(shift), dev(np.diag(z)), dev(np.diag(z, 1)), dev(np.diag(z, -1))
+    h = torch.empty((nz - 1, modes), dtype=torch.complex128, device="cuda:0")
+    schur = torch.empty(modes, dtype=torch.complex128, device="cuda:0")
+    ptr = lambda t: ctypes.c_void_p(t.data_ptr())
+    _capi.check(_capi.lib().hz_z_transfer(ptr(s), ptr(d), ptr(u), ptr(lo), nz, modes, ptr(h), ptr(schur), None),
+                "hz_z_transfer")
+    close(h.cpu().numpy(), want, 1e-12)
+    close(schur.cpu().numpy(), shift + z[0, 0] + z[0, 1]*want[0], 1e-12)
+
+
+def test_tables_and_transforms(pair):
+    dev, ref = pair
+    close(dev.h.cpu().numpy(), ref.h, 1e-12)
+    close(dev.schur.cpu().numpy(), ref.schur, 1e-12)
+    rng = np.random.default_rng(1)
+    planes = rng.standard_normal((3, *SHAPE[:2])) + 1j*rng.standard_normal((3, *SHAPE[:2]))
+    close(dev.transform(planes[0]).cpu().numpy(), ref.transform(planes[0]))
+    close(dev.inverse_transform(planes).cpu().numpy(), np.stack([ref.inverse_transform(p) for p in planes]))
+
+
+def test_solve_flux_planes_and_samples(pair):
+    dev, ref = pair
+    m = masses()
+    phases, amps = np.array([0.3, -1.9]), np.array([1.0, 0.7])
+    diag = dev.solve_flux(m, phases, amps)
+    ref.solve_flux(m, phases, amps)
+    assert diag["iterations"] == 0 and len(diag["eigenvector_condition"]) == 2
+    close(dev.surface_rhs.cpu().numpy(), ref.surface_rhs, 1e-13)
+    close(dev.surface_modes.cpu().numpy(), ref.surface_modes)
+    for z in range(SHAPE[2]):
+        close(dev.plane(z), ref.plane(z))
+    close(dev.sample(POINTS), ref.sample(POINTS))
+
+
+def test_target_response(pair):
+    dev, ref = pair
+    m = masses()
+    close(dev.flux_target_response(m, POINTS), ref.flux_target_response(m, POINTS))
+
+
+def test_validation_matches_the_reference_messages(pair):
+    from ultrasound_solver_amd.surface import HomogeneousSurfaceSolver
+    dev, _ = pair
+    with pytest.raises(ValueError, match="inside the local grid"):
+        dev.flux_target_response(masses(), np.array([[0.0, 0.0, -1e-3]]))
+    bad = dict(BOUNDARIES, **{"-z": "open"})
+    with pytest.raises(ValueError, match="reflecting/baffled bottom"):
+        HomogeneousSurfaceSolver(SHAPE, DX, FREQ, C0, RHO0, bad, PML, device="cuda:0").solve_flux(masses(), [0, 0])
+    with pytest.raises(ValueError, match="overlaps PML or side"):
+        edge = osrc.source_mass_matrix(SHAPE[:2], DX, np.array([[4.2e-3, 9.0e-3, 0.0]]), DIAMETER, 8, 16)
+        dev.solve_flux(edge, [0.0])
+    with pytest.raises(ValueError, match="positive scalar"):
+        HomogeneousSurfaceSolver(SHAPE, DX, FREQ, -1.0, RHO0, BOUNDARIES, PML, device="cuda:0")
+    with pytest.raises(ValueError, match="open/reflecting/impedance"):
+        HomogeneousSurfaceSolver(SHAPE, DX, FREQ, C0, RHO0, dict(BOUNDARIES, **{"+y": "soft"}), PML, device="cuda:0")
+
+
+def test_full_size_128_array_of_256():
+    """128^3 air, PML 8, 16 x 16 array: every response column and sampled
+    field against the oracle at 6 points, including the surface and top."""
+    from ultrasound_solver_amd import scenes
+    from ultrasound_solver_amd.surface import HomogeneousSurfaceSolver
+    sc = scenes.config_small(128, array_n=16)
+    freq = sc.omega/(2*np.pi)
+    c0, rho0 = float(sc.c.flat[0]), float(sc.rho.flat[0])
+    dev = HomogeneousSurfaceSolver(sc.shape, sc.dx, freq, c0, rho0, sc.boundaries, sc.pml, device="cuda:0")
+    ref = osu.SurfaceSolver(sc.shape, sc.dx, freq, c0, rho0, sc.boundaries, sc.pml)
+    m = osrc.source_mass_matrix(sc.shape[:2], sc.dx, sc.centers, 2*sc.radius, 16, 32)
+    ext = (np.asarray(sc.shape) - 1)*sc.dx
+    pts = np.array([[0.5, 0.5, 0.6], [0.31, 0.62, 0.25], [0.5, 0.5, 0.0], [0.7, 0.4, 1.0],
+                    [0.45, 0.55, 0.83], [0.52, 0.47, 0.1]])*ext
+    resp = dev.flux_target_response(m, pts)
+    close(resp, ref.flux_target_response(m, pts), 1e-10)
+    phases = np.random.default_rng(8).uniform(0, 2*np.pi, sc.num_sources)
+    dev.solve_flux(m, phases)
+    ref.solve_flux(m, phases)
+    got = dev.sample(pts)
+    close(got, ref.sample(pts), 1e-10)
+    # Linearity: the response rows reproduce the sampled field.
+    close(resp @ np.exp(1j*phases), got, 1e-10)

@@ -84,3 +84,20 @@ def test_box_and_ellipsoid_sdf_closed_forms():
     g = np.ogrid[:21, :21, :21]
     sph = np.sqrt(sum((g[a] - 10.0)**2 for a in range(3))) - 5
     np.testing.assert_allclose(e, sph, atol=1e-12)
+
+
+def test_surface_axis_matrices_match_the_oracle():
+    """Host setup of the surface solver: the 1-D axis operators (PML, open and
+    impedance faces) equal the oracle's restatement of the reference's."""
+    from oracle import surface as osu
+    from ultrasound_solver_amd import surface
+    faces = {"-x": "open", "+x": "open", "-y": {"type": "impedance", "impedance": [823.2, 205.8]},
+             "+y": "reflecting", "-z": "reflecting", "+z": "open"}
+    pml = {"cells": 4, "strength": 1.7, "order": 2}
+    for axis, n in enumerate((23, 19, 17)):
+        got, s_got = surface.axis_matrix(n, 1.1e-3, 2*np.pi*40e3, 1.2, 343.0, faces, pml, axis)
+        want, s_want = osu.axis_matrix(n, 1.1e-3, 2*np.pi*40e3, 1.2, 343.0, faces, pml, axis)
+        np.testing.assert_array_equal(s_got, s_want)
+        np.testing.assert_allclose(got, want, rtol=1e-15, atol=0)
+    with pytest.raises(ValueError, match="open/reflecting/impedance"):
+        surface.axis_matrix(10, 1e-3, 1.0, 1.2, 343.0, {"-x": "soft"}, {}, 0)

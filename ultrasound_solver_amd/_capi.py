@@ -17,7 +17,10 @@ MODE_A, MODE_RAM, MODE_AH, MODE_RAMH = 0, 1, 2, 3
 EXPORTS = ("hz_create", "hz_destroy", "hz_apply", "hz_gmres", "hz_set_sources", "hz_build_rhs",
            "hz_gather_sources", "hz_sample", "hz_device_bytes", "hz_last_error", "hz_set_multigrid",
            "hz_autotune", "hz_set_tile_shapes", "hz_gmres_batch", "hz_release_workspace",
-           "hz_material_vjp", "hz_basis_field", "hz_basis_adjoint", "hz_basis_field_match")
+           "hz_material_vjp", "hz_basis_field", "hz_basis_adjoint", "hz_basis_field_match",
+           "hz_z_transfer", "hz_surface_create", "hz_surface_destroy", "hz_surface_tables",
+           "hz_surface_set_sources", "hz_surface_transform", "hz_surface_solve_flux", "hz_surface_planes",
+           "hz_surface_sample", "hz_surface_target_response")
 PRE_JACOBI, PRE_MULTIGRID = 0, 1
 
 
@@ -41,6 +44,14 @@ class LevelDesc(ctypes.Structure):
                 ("rho", POINTER(c_double)), ("kappa", POINTER(c_double)),
                 ("weight", POINTER(c_double)*3), ("stretch", POINTER(c_double)*3),
                 ("edge", POINTER(c_double)*3), ("face_term", POINTER(c_double)*6)]
+
+
+class SurfaceDesc(ctypes.Structure):
+    _fields_ = [("nx", c_int32), ("ny", c_int32), ("nz", c_int32), ("dx", c_double),
+                ("vx", POINTER(c_double)), ("ix", POINTER(c_double)),
+                ("vy", POINTER(c_double)), ("iy", POINTER(c_double)),
+                ("shift", POINTER(c_double)), ("z_diagonal", POINTER(c_double)),
+                ("z_upper", POINTER(c_double)), ("z_lower", POINTER(c_double))]
 
 
 class GmresStats(ctypes.Structure):
@@ -96,6 +107,20 @@ def lib():
         handle.hz_material_vjp.argtypes = [c_void_p, c_void_p, c_void_p, c_int32, c_int64, c_void_p, c_void_p,
                                            c_void_p]
         handle.hz_set_tile_shapes.argtypes = [c_void_p, POINTER(c_int32), c_int32]
+        handle.hz_z_transfer.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int64, c_void_p,
+                                         c_void_p, c_void_p]
+        handle.hz_surface_create.argtypes = [POINTER(SurfaceDesc), POINTER(c_void_p)]
+        handle.hz_surface_destroy.argtypes = [c_void_p]
+        handle.hz_surface_tables.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p]
+        handle.hz_surface_set_sources.argtypes = [c_void_p, c_int32, c_int64, POINTER(c_int64), POINTER(c_int32),
+                                                  POINTER(c_double)]
+        handle.hz_surface_transform.argtypes = [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_void_p]
+        handle.hz_surface_solve_flux.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
+        handle.hz_surface_planes.argtypes = [c_void_p, POINTER(c_int32), c_int32, c_void_p, c_void_p]
+        handle.hz_surface_sample.argtypes = [c_void_p, POINTER(c_int32), POINTER(c_double), c_int32, c_void_p,
+                                             c_void_p]
+        handle.hz_surface_target_response.argtypes = [c_void_p, POINTER(c_int32), POINTER(c_double), c_int32,
+                                                       c_void_p, c_void_p]
         handle.hz_device_bytes.argtypes = [c_void_p]
         handle.hz_device_bytes.restype = c_int64
         handle.hz_last_error.restype = c_char_p

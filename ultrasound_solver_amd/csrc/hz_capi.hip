@@ -24,6 +24,7 @@
 #include "hz_mg.hpp"
 #include "hz_slab.hpp"
 #include "hz_stencil.hpp"
+#include "hz_surface.hpp"
 
 using namespace hz;
 
@@ -1743,6 +1744,329 @@ int64_t hz_device_bytes(const hz_operator *op) {
         for (const auto &L : op->mg->levels)
             total += (int64_t)(L->rho.bytes + L->kap.bytes + L->dinv.bytes + L->bb.bytes + L->xx.bytes + L->tt.bytes);
     return total;
+}
+
+}  // extern "C"
+
+// ---- Homogeneous surface condensation ------------------------------------
+
+struct hz_surface {
+    int nx = 0, ny = 0, nz = 0;
+    int64_t nxy = 0;
+    double dx = 0;
+    DevBuf vx, ix, vy, iy, shift, zd, zu, zl, h, schur;
+    DevBuf modes;                         // surface modes of the last solve_flux
+    bool solved = false;
+    int32_t ncols = 0;
+    DevBuf node_ptr, node_col, node_val;  // masses grouped by plane node (CSR)
+    DevBuf col_ptr, col_node, col_val;    // masses grouped by emitter (CSR of the transpose)
+    DevBuf t1, t2, sel;                   // stage-1 products, planes / observation kernels, plane selectors
+};
+
+namespace {
+
+void check_surface(const hz_surface *s) {
+    if (!s) throw BadArgument("null surface handle");
+}
+
+// C[p] = A[p] B[p] for `count` products, in grid-z slices of at most 65535.
+void zgemm(ZgemmArgs g, int count, hipStream_t st) {
+    for (int p0 = 0; p0 < count; p0 += 65535) {
+        const int c = std::min(65535, count - p0);
+        ZgemmArgs q = g;
+        q.a = g.a + p0 * g.a_sp;
+        q.b = g.b + p0 * g.b_sp;
+        q.c = g.c + p0 * g.c_sp;
+        if (g.sel) q.sel = g.sel + p0;
+        else if (g.w) q.w = g.w + p0 * g.w_sp;
+        hipLaunchKernelGGL(k_zgemm, dim3((g.m + ZG_TILE - 1) / ZG_TILE, (g.n + ZG_TILE - 1) / ZG_TILE, c), dim3(256),
+                           0, st, q);
+        HZ_LAUNCHED();
+    }
+}
+
+// Y[p] = op(L) (W[p] o X[p]) op(R)^T with L, R the x / y matrices of the
+// forward (ix, iy) or inverse (vx, vy) transform; op = transpose when asked.
+// W[p] = h[sel[p]] (no weight for sel[p] < 0) when sel is given; Y is
+// divided by D elementwise when D is given.  Stage 1 goes through S->t1.
+void modal_product(hz_surface *S, bool inverse, bool trans_l, bool trans_r, const cx<double> *x, int64_t x_sp,
+                   const int *sel, const cx<double> *d, cx<double> *y, int count, hipStream_t st) {
+    const int64_t nx = S->nx, ny = S->ny, nxy = S->nxy;
+    S->t1.ensure((size_t)count * nxy * sizeof(cx<double>));
+    const cx<double> *L = (inverse ? S->vx : S->ix).as<cx<double>>();
+    const cx<double> *R = (inverse ? S->vy : S->iy).as<cx<double>>();
+    ZgemmArgs g{};
+    // Stage 1: T = (W o X) op(R)^T, nx x ny with inner ny.
+    g.m = S->nx, g.n = S->ny, g.k = S->ny;
+    g.a = x, g.a_si = 1, g.a_sk = nx, g.a_sp = x_sp;
+    g.w = sel ? S->h.as<cx<double>>() : nullptr, g.w_sp = nxy, g.sel = sel;
+    g.b = R, g.b_sp = 0;
+    if (trans_r) g.b_sk = 1, g.b_sj = ny;
+    else g.b_sk = ny, g.b_sj = 1;
+    g.c = S->t1.as<cx<double>>(), g.ldc = nx, g.c_sp = nxy, g.d = nullptr;
+    zgemm(g, count, st);
+    // Stage 2: Y = op(L) T, nx x ny with inner nx.
+    ZgemmArgs f{};
+    f.m = S->nx, f.n = S->ny, f.k = S->nx;
+    f.a = L, f.a_sp = 0;
+    if (trans_l) f.a_si = nx, f.a_sk = 1;
+    else f.a_si = 1, f.a_sk = nx;
+    f.b = S->t1.as<cx<double>>(), f.b_sk = 1, f.b_sj = nx, f.b_sp = nxy;
+    f.c = y, f.ldc = nx, f.c_sp = nxy, f.d = d;
+    zgemm(f, count, st);
+}
+
+// Host corner indices checked against the grid; uploads low and frac next
+// to each other in S->sel (int32 then double, 8-byte aligned).
+void upload_points(hz_surface *S, const int32_t *low, const double *frac, int32_t n, int extra_ints,
+                   int **d_low, double **d_frac, int **d_extra) {
+    if (n < 1 || !low || !frac) throw BadArgument("need at least one point with low corners and fractions");
+    for (int32_t p = 0; p < n; ++p) {
+        const int lim[3] = {S->nx - 2, S->ny - 2, S->nz - 2};
+        for (int a = 0; a < 3; ++a) {
+            if (low[3 * p + a] < 0 || low[3 * p + a] > lim[a]) throw BadArgument("point corner outside the grid");
+            if (!(frac[3 * p + a] >= 0.0 && frac[3 * p + a] <= 1.0)) throw BadArgument("point fraction outside [0, 1]");
+        }
+    }
+    const size_t ints = (size_t)3 * n + (size_t)extra_ints;
+    const size_t ioff = (ints * sizeof(int) + 7) / 8 * 8;
+    S->sel.ensure(ioff + (size_t)3 * n * sizeof(double));
+    char *base = S->sel.as<char>();
+    HZ_HIP(hipMemcpy(base, low, (size_t)3 * n * sizeof(int), hipMemcpyHostToDevice));
+    HZ_HIP(hipMemcpy(base + ioff, frac, (size_t)3 * n * sizeof(double), hipMemcpyHostToDevice));
+    *d_low = reinterpret_cast<int *>(base);
+    *d_frac = reinterpret_cast<double *>(base + ioff);
+    if (d_extra) *d_extra = reinterpret_cast<int *>(base) + 3 * n;
+}
+
+}  // namespace
+
+extern "C" {
+
+hz_status hz_z_transfer(const void *shift, const void *diagonal, const void *upper, const void *lower, int32_t nz,
+                        int64_t modes, void *h, void *schur, void *stream) {
+    return guarded([&] {
+        if (!shift || !diagonal || !upper || !lower || !h || !schur) throw BadArgument("null device pointer");
+        if (nz < 2 || modes < 1) throw BadArgument("need nz >= 2 and at least one mode");
+        hipLaunchKernelGGL(k_z_transfer, dim3((unsigned)((modes + 255) / 256)), dim3(256), 0,
+                           static_cast<hipStream_t>(stream), static_cast<const cx<double> *>(shift),
+                           static_cast<const cx<double> *>(diagonal), static_cast<const cx<double> *>(upper),
+                           static_cast<const cx<double> *>(lower), (int)nz, modes, static_cast<cx<double> *>(h),
+                           static_cast<cx<double> *>(schur));
+        HZ_LAUNCHED();
+    });
+}
+
+hz_status hz_surface_create(const hz_surface_desc *d, hz_surface **out) {
+    return guarded([&] {
+        if (!d || !out) throw BadArgument("null descriptor or output");
+        *out = nullptr;
+        if (d->nx < 2 || d->ny < 2 || d->nz < 2) throw BadArgument("need nx, ny, nz >= 2");
+        if (!(d->dx > 0) || !std::isfinite(d->dx)) throw BadArgument("dx must be finite and positive");
+        if (!d->vx || !d->ix || !d->vy || !d->iy || !d->shift || !d->z_diagonal || !d->z_upper || !d->z_lower)
+            throw BadArgument("missing surface table");
+        auto S = std::make_unique<hz_surface>();
+        S->nx = d->nx, S->ny = d->ny, S->nz = d->nz, S->dx = d->dx;
+        S->nxy = (int64_t)d->nx * d->ny;
+        upload(S->vx, d->vx, (size_t)2 * d->nx * d->nx);
+        upload(S->ix, d->ix, (size_t)2 * d->nx * d->nx);
+        upload(S->vy, d->vy, (size_t)2 * d->ny * d->ny);
+        upload(S->iy, d->iy, (size_t)2 * d->ny * d->ny);
+        upload(S->shift, d->shift, (size_t)2 * S->nxy);
+        upload(S->zd, d->z_diagonal, (size_t)2 * d->nz);
+        upload(S->zu, d->z_upper, (size_t)2 * (d->nz - 1));
+        upload(S->zl, d->z_lower, (size_t)2 * (d->nz - 1));
+        S->h.ensure((size_t)(d->nz - 1) * S->nxy * sizeof(cx<double>));
+        S->schur.ensure((size_t)S->nxy * sizeof(cx<double>));
+        hipLaunchKernelGGL(k_z_transfer, dim3((unsigned)((S->nxy + 255) / 256)), dim3(256), 0, nullptr,
+                           S->shift.as<cx<double>>(), S->zd.as<cx<double>>(), S->zu.as<cx<double>>(),
+                           S->zl.as<cx<double>>(), S->nz, S->nxy, S->h.as<cx<double>>(), S->schur.as<cx<double>>());
+        HZ_LAUNCHED();
+        HZ_HIP(hipDeviceSynchronize());
+        *out = S.release();
+    });
+}
+
+hz_status hz_surface_destroy(hz_surface *s) {
+    return guarded([&] {
+        if (s) {
+            HZ_HIP(hipDeviceSynchronize());
+            delete s;
+        }
+    });
+}
+
+hz_status hz_surface_tables(const hz_surface *s, void *h, void *schur, void *stream) {
+    return guarded([&] {
+        check_surface(s);
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        if (h)
+            HZ_HIP(hipMemcpyAsync(h, s->h.p, (size_t)(s->nz - 1) * s->nxy * sizeof(cx<double>),
+                                  hipMemcpyDeviceToDevice, st));
+        if (schur) HZ_HIP(hipMemcpyAsync(schur, s->schur.p, (size_t)s->nxy * sizeof(cx<double>), hipMemcpyDeviceToDevice, st));
+    });
+}
+
+hz_status hz_surface_set_sources(hz_surface *s, int32_t num_columns, int64_t nnz, const int64_t *node,
+                                 const int32_t *column, const double *value) {
+    return guarded([&] {
+        check_surface(s);
+        if (num_columns < 1 || nnz < 1 || !node || !column || !value) throw BadArgument("empty source list");
+        for (int64_t e = 0; e < nnz; ++e)
+            if (node[e] < 0 || node[e] >= s->nxy || column[e] < 0 || column[e] >= num_columns || !std::isfinite(value[e]))
+                throw BadArgument("source entry out of range or not finite");
+        // CSR by plane node and by emitter, each sorted by the other index.
+        auto build = [&](const int64_t *row_of, int64_t nrows, const auto &col_of, DevBuf &ptr, DevBuf &col,
+                         DevBuf &val) {
+            std::vector<int64_t> order(nnz), rp(nrows + 1, 0);
+            for (int64_t e = 0; e < nnz; ++e) order[e] = e;
+            std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+                return row_of[a] != row_of[b] ? row_of[a] < row_of[b] : col_of(a) < col_of(b);
+            });
+            std::vector<int32_t> cc(nnz);
+            std::vector<double> vv(nnz);
+            for (int64_t t = 0; t < nnz; ++t) {
+                const int64_t e = order[t];
+                rp[row_of[e] + 1] += 1;
+                cc[t] = (int32_t)col_of(e);
+                vv[t] = value[e];
+            }
+            for (int64_t r = 0; r < nrows; ++r) rp[r + 1] += rp[r];
+            upload(ptr, rp.data(), rp.size());
+            upload(col, cc.data(), cc.size());
+            upload(val, vv.data(), vv.size());
+        };
+        std::vector<int64_t> col64(column, column + nnz);
+        build(node, s->nxy, [&](int64_t e) { return (int64_t)column[e]; }, s->node_ptr, s->node_col, s->node_val);
+        build(col64.data(), num_columns, [&](int64_t e) { return node[e]; }, s->col_ptr, s->col_node, s->col_val);
+        s->ncols = num_columns;
+    });
+}
+
+hz_status hz_surface_transform(hz_surface *s, int32_t inverse, const void *x, void *y, int32_t count,
+                               void *stream) {
+    return guarded([&] {
+        check_surface(s);
+        if (!x || !y || count < 1) throw BadArgument("need device planes and count >= 1");
+        if (x == y) throw BadArgument("x and y must not alias");
+        modal_product(s, inverse != 0, false, false, static_cast<const cx<double> *>(x), s->nxy, nullptr, nullptr,
+                      static_cast<cx<double> *>(y), count, static_cast<hipStream_t>(stream));
+    });
+}
+
+hz_status hz_surface_solve_flux(hz_surface *s, const void *w, void *rhs, void *modes, void *surface, void *stream) {
+    return guarded([&] {
+        check_surface(s);
+        if (s->ncols == 0) throw BadArgument("no source masses set");
+        if (!w) throw BadArgument("w must be a device pointer");
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        s->t2.ensure((size_t)s->nxy * sizeof(cx<double>));
+        s->modes.ensure((size_t)s->nxy * sizeof(cx<double>));
+        cx<double> *load = rhs ? static_cast<cx<double> *>(rhs) : s->t2.as<cx<double>>();
+        const double scale = -4.0 * M_PI / (s->dx * s->dx * s->dx);
+        const int64_t threads = s->nxy * WAVE;
+        hipLaunchKernelGGL(k_csr_rows, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, (int)s->nxy, 1,
+                           s->node_ptr.as<int64_t>(), s->node_col.as<int>(), s->node_val.as<double>(),
+                           static_cast<const cx<double> *>(w), (int64_t)0, scale, load, s->nxy);
+        HZ_LAUNCHED();
+        modal_product(s, false, false, false, load, s->nxy, nullptr, s->schur.as<cx<double>>(), s->modes.as<cx<double>>(),
+                      1, st);
+        s->solved = true;
+        if (modes)
+            HZ_HIP(hipMemcpyAsync(modes, s->modes.p, (size_t)s->nxy * sizeof(cx<double>), hipMemcpyDeviceToDevice, st));
+        if (surface)
+            modal_product(s, true, false, false, s->modes.as<cx<double>>(), 0, nullptr, nullptr,
+                          static_cast<cx<double> *>(surface), 1, st);
+    });
+}
+
+hz_status hz_surface_planes(hz_surface *s, const int32_t *z, int32_t count, void *out, void *stream) {
+    return guarded([&] {
+        check_surface(s);
+        if (!s->solved) throw BadArgument("no solved field: call hz_surface_solve_flux first");
+        if (!z || !out || count < 1) throw BadArgument("need plane indices and an output");
+        std::vector<int> sel(count);
+        for (int p = 0; p < count; ++p) {
+            if (z[p] < 0 || z[p] >= s->nz) throw BadArgument("z index outside grid");
+            sel[p] = z[p] - 1;
+        }
+        upload(s->sel, sel.data(), sel.size());
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        modal_product(s, true, false, false, s->modes.as<cx<double>>(), 0, s->sel.as<int>(), nullptr,
+                      static_cast<cx<double> *>(out), count, st);
+        HZ_HIP(hipStreamSynchronize(st));  // the selectors are reused by the next call
+    });
+}
+
+hz_status hz_surface_sample(hz_surface *s, const int32_t *low, const double *frac, int32_t npoints, void *out,
+                            void *stream) {
+    return guarded([&] {
+        check_surface(s);
+        if (!s->solved) throw BadArgument("no solved field: call hz_surface_solve_flux first");
+        if (!out) throw BadArgument("out must be a device pointer");
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        // The distinct planes the points touch, ascending; two slots per point.
+        if (npoints < 1 || !low) throw BadArgument("need at least one point with low corners and fractions");
+        std::vector<int> planes;
+        for (int p = 0; p < npoints; ++p)
+            for (int dz = 0; dz < 2; ++dz) planes.push_back(low[3 * p + 2] + dz);
+        std::sort(planes.begin(), planes.end());
+        planes.erase(std::unique(planes.begin(), planes.end()), planes.end());
+        std::vector<int> extra(planes.size() + 2 * (size_t)npoints);
+        for (size_t q = 0; q < planes.size(); ++q) extra[q] = planes[q] - 1;
+        for (int p = 0; p < npoints; ++p)
+            for (int dz = 0; dz < 2; ++dz)
+                extra[planes.size() + 2 * p + dz] =
+                    (int)(std::lower_bound(planes.begin(), planes.end(), low[3 * p + 2] + dz) - planes.begin());
+        int *d_low, *d_extra;
+        double *d_frac;
+        upload_points(s, low, frac, npoints, (int)extra.size(), &d_low, &d_frac, &d_extra);
+        HZ_HIP(hipMemcpy(d_extra, extra.data(), extra.size() * sizeof(int), hipMemcpyHostToDevice));
+        const int np = (int)planes.size();
+        s->t2.ensure((size_t)np * s->nxy * sizeof(cx<double>));
+        modal_product(s, true, false, false, s->modes.as<cx<double>>(), 0, d_extra, nullptr, s->t2.as<cx<double>>(), np,
+                      st);
+        hipLaunchKernelGGL(k_sample_planes, dim3((npoints + 255) / 256), dim3(256), 0, st, s->nx, s->ny, (int)npoints,
+                           (const cx<double> *)s->t2.as<cx<double>>(), (const int *)(d_extra + np), (const int *)d_low,
+                           (const double *)d_frac, static_cast<cx<double> *>(out));
+        HZ_LAUNCHED();
+        HZ_HIP(hipStreamSynchronize(st));  // the index arrays are reused by the next call
+    });
+}
+
+hz_status hz_surface_target_response(hz_surface *s, const int32_t *low, const double *frac, int32_t npoints,
+                                     void *out, void *stream) {
+    return guarded([&] {
+        check_surface(s);
+        if (s->ncols == 0) throw BadArgument("no source masses set");
+        if (!out) throw BadArgument("out must be a device pointer");
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        int *d_low;
+        double *d_frac;
+        upload_points(s, low, frac, npoints, 0, &d_low, &d_frac, nullptr);
+        const double scale = -4.0 * M_PI / (s->dx * s->dx * s->dx);
+        constexpr int CHUNK = 128;  // observation kernels per pass
+        for (int p0 = 0; p0 < npoints; p0 += CHUNK) {
+            const int c = std::min(CHUNK, npoints - p0);
+            s->t2.ensure((size_t)2 * c * s->nxy * sizeof(cx<double>));
+            cx<double> *obs = s->t2.as<cx<double>>();
+            cx<double> *ker = obs + (size_t)c * s->nxy;
+            hipLaunchKernelGGL(k_observation, dim3((unsigned)((s->nxy + 255) / 256), c), dim3(256), 0, st, s->nx, s->ny,
+                               s->nz, s->vx.as<cx<double>>(), s->vy.as<cx<double>>(), s->h.as<cx<double>>(),
+                               s->schur.as<cx<double>>(), (const int *)(d_low + 3 * p0),
+                               (const double *)(d_frac + 3 * p0), obs);
+            HZ_LAUNCHED();
+            // kernel_p = ix^T O_p iy
+            modal_product(s, false, true, true, obs, s->nxy, nullptr, nullptr, ker, c, st);
+            const int64_t threads = (int64_t)s->ncols * c * WAVE;
+            hipLaunchKernelGGL(k_csr_rows, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, (int)s->ncols, c,
+                               s->col_ptr.as<int64_t>(), s->col_node.as<int>(), s->col_val.as<double>(),
+                               (const cx<double> *)ker, s->nxy, scale,
+                               static_cast<cx<double> *>(out) + (int64_t)p0 * s->ncols, (int64_t)s->ncols);
+            HZ_LAUNCHED();
+        }
+        HZ_HIP(hipStreamSynchronize(st));  // the point arrays are reused by the next call
+    });
 }
 
 }  // extern "C"
