@@ -49,7 +49,7 @@ def parse():
     p.add_argument("--graph", type=int, default=0, help="replay each GMRES cycle from a captured hipGraph")
     p.add_argument("--arnoldi-block", type=int, default=2, choices=[1, 2],
                    help="mixed form: basis vectors per Gram-Schmidt pass")
-    p.add_argument("--workload", choices=["g128", "uneven", "large"], default="g128",
+    p.add_argument("--workload", choices=["g128", "uneven", "large", "surface"], default="g128",
                    help="g128: G columns of the heterogeneous 128^3 scene (the headline); uneven: the "
                         "32-scene batch; large: the 256^3 two-layer scene")
     p.add_argument("--scenes", type=int, default=32, help="uneven: scenes in the batch")
@@ -78,6 +78,8 @@ def main():
         return bench_uneven(args)
     if args.workload == "large":
         return bench_large(args)
+    if args.workload == "surface":
+        return bench_surface(args)
     t0 = time.perf_counter()
     scene = scenes.config_heterogeneous(n=args.n)
     op = HelmholtzOperator(scene.rho, scene.c, scene.dx, scene.omega, scene.boundaries, scene.pml,
@@ -373,6 +375,85 @@ def bench_large(args):
         "stencil": {"achieved_GBps": alg/st_s/1e9, "peak_GBps": 8000.0, "frac": alg/st_s/1e9/8000.0,
                     "launch_us": st_s*1e6, "alg_bytes_per_launch": alg},
         "peak_device_bytes": int(peak), "setup_s": setup_s}), flush=True)
+
+
+def bench_surface(args, points=64):
+    """Homogeneous 128^3 air scene with a 16 x 16 array: the exact target
+    response of `points` target points to all 256 emitters through the
+    surface condensation (HelmholtzSolver.target_response's path).  Timed:
+    the responses (observation kernels, two-sided modal products, emitter
+    sums); reported beside it: the setup (host eigendecompositions plus the
+    device transfer-ratio recurrence) and the modal-product kernel against
+    the FP64 peak.  CPU baseline: the oracle's numpy path on 4 points."""
+    import scipy.sparse as sps
+    import torch
+    from ultrasound_solver_amd import physics, scenes
+    from ultrasound_solver_amd.surface import HomogeneousSurfaceSolver
+    sc = scenes.config_small(args.n, array_n=16)
+    freq = sc.omega/(2*np.pi)
+    c0, rho0 = float(sc.c.flat[0]), float(sc.rho.flat[0])
+    node, col, mass = physics.source_deposition(sc.shape[:2], sc.dx, sc.centers, 2*sc.radius)
+    masses = sps.csr_matrix((mass, (node, col)), shape=(sc.shape[0]*sc.shape[1], sc.num_sources))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev = HomogeneousSurfaceSolver(sc.shape, sc.dx, freq, c0, rho0, sc.boundaries, sc.pml, device="cuda:0")
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    ext = (np.asarray(sc.shape) - 1)*sc.dx
+    rng = np.random.default_rng(11)
+    pts = (0.3 + 0.4*rng.random((points, 3)))*ext
+    for _ in range(args.warmup):
+        dev.flux_target_response(masses, pts)
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    for _ in range(args.steps):
+        resp = dev.flux_target_response(masses, pts)
+    torch.cuda.synchronize()
+    step_s = (time.perf_counter() - ts)/args.steps
+    # The modal product alone: 2 launches of 64 (nx x ny) products, 8 flops per complex MAC.
+    nx, ny, _ = sc.shape
+    planes = torch.randn((points, nx, ny), dtype=torch.complex128, device="cuda:0")
+    stream = torch.cuda.current_stream()
+    for _ in range(3):
+        dev.transform(planes)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    e0.record(stream)
+    for _ in range(reps):
+        dev.transform(planes)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    launch_s = e0.elapsed_time(e1)/1e3/reps/2
+    flops = 8.0*nx*ny*ny*points  # stage 1 (inner ny); stage 2 has inner nx = ny here
+    cpu = None
+    if not args.no_cpu:
+        from oracle import surface as osu
+        ref = osu.SurfaceSolver(sc.shape, sc.dx, freq, c0, rho0, sc.boundaries, sc.pml)
+        tc = time.perf_counter()
+        want = ref.flux_target_response(masses, pts[:4])
+        cpu_s = time.perf_counter() - tc
+        err = float(np.abs(resp[:4] - want).max()/np.abs(want).max())
+        cpu = {"value": 4/cpu_s, "unit": "target points/s", "cores": _threads(), "kind": "port",
+               "sample": "oracle numpy flux_target_response on 4 of the points, 256 emitters, 128^3",
+               "max_rel_diff_vs_device": err}
+    print(json.dumps({
+        "metric": "homogeneous 128^3 exact target response: target points/s (256 emitters each)",
+        "value": points/step_s, "unit": "target points/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": step_s*1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c128", "data": "synthetic",
+        "config": {"workload": f"surface condensation, {args.n}^3 air, PML 8, 16x16 array, {points} points"},
+        "setup_s": setup_s,
+        "roofline": {"kernel": "hz::k_zgemm", "bound": "fp64", "achieved": flops/launch_s/1e12, "peak": 78.6,
+                     "unit": "TFLOP/s", "frac": flops/launch_s/1e12/78.6, "launch_us": launch_s*1e6},
+        "cpu_baseline": cpu}), flush=True)
+
+
+def _threads():
+    try:
+        from threadpoolctl import threadpool_info
+        return max((i.get("num_threads", 1) for i in threadpool_info()), default=1)
+    except ImportError:
+        return 1
 
 
 def cpu_baseline(scene, args, plain_iters):

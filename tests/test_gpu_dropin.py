@@ -14,7 +14,8 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 
-def small_cfg(tmp_path=None, precision="fp64", pre="jacobi", loss="focal_contrast", n=25):
+def small_cfg(tmp_path=None, precision="fp64", pre="jacobi", loss="focal_contrast", n=25, obstacle=True,
+              top="soft"):
     from ultrasound_solver_amd.config import SimulationConfig
     h = 1.07e-3
     raw = {
@@ -24,10 +25,10 @@ def small_cfg(tmp_path=None, precision="fp64", pre="jacobi", loss="focal_contras
                              "source_radial_order": 16, "source_angular_order": 32},
         "domain": {"box_size": [(n - 1)*h]*3, "grid_size": [n]*3},
         "boundary_conditions": {"-x": "open", "+x": "reflecting", "-y": "open", "+y": "open",
-                                "-z": "reflecting", "+z": "soft"},
+                                "-z": "reflecting", "+z": top},
         "targets": [{"point": [(n - 1)*h*0.5, (n - 1)*h*0.5, (n - 1)*h*0.6]}],
         "obstacles": [{"type": "sphere", "center": [(n - 1)*h*0.55, (n - 1)*h*0.5, (n - 1)*h*0.75],
-                       "radius": 3*h}],
+                       "radius": 3*h}] if obstacle else [],
         "training": {"loss_type": loss, "phase_basis_file": str((tmp_path or ".")) + "/G.npy",
                      "phase_basis_batch_size": 3, "background_weight": 0.8, "target_radius": 0.004,
                      "source_exclusion_layers": 3, "sidelobe_temperature": 2.0},
@@ -95,6 +96,31 @@ def test_basis_build_field_vjp_and_target_response(tmp_path):
     resp_recip = s.target_response(pts)
     assert np.max(np.abs(resp_recip - resp_basis))/np.max(np.abs(resp_basis)) < 1e-5
     basis.close()
+    s.close()
+
+
+def test_homogeneous_target_response_is_exact_and_cached(tmp_path):
+    """Homogeneous scene: target_response comes from the surface condensation
+    and equals the trilinear rows of the direct-solve basis to 1e-9; a second
+    call reads the response cache file; field_at_direct agrees with field_at."""
+    from ultrasound_solver_amd.solvers import HelmholtzDirectSolver
+    cfg = small_cfg(tmp_path, obstacle=False, top="open")
+    cfg.solver["target_response_cache"] = str(tmp_path/"responses.npz")
+    s = HelmholtzDirectSolver(cfg)
+    a, b = oracle_system(s)
+    g_ref = okr.direct_solve(a, b.toarray())
+    n = cfg.domain.grid_size[0]
+    h = cfg.domain.dx
+    pts = np.array([[0.5, 0.5, 0.6], [0.41, 0.55, 0.3], [0.6, 0.45, 0.12]])*(n - 1)*h
+    resp = s.target_response(pts)
+    assert s._target_surface_solver is not None and s.operator is None  # no 3-D operator was built
+    ref = olo.target_response(g_ref, pts, (n, n, n), (0, 0, 0), h)
+    assert np.max(np.abs(resp - ref))/np.max(np.abs(ref)) < 1e-9
+    s2 = HelmholtzDirectSolver(cfg)
+    s2._ensure_target_surface_solver = None  # a cache hit must not need the solver
+    np.testing.assert_array_equal(s2.target_response(pts[::-1]), resp[::-1])
+    ph = np.random.default_rng(3).uniform(0, 2*np.pi, 4)
+    np.testing.assert_allclose(s.field_at_direct(pts, ph), s.field_at(pts, ph), rtol=1e-10)
     s.close()
 
 

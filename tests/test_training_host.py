@@ -180,3 +180,47 @@ def test_basis_rejects_changed_scene(tmp_path):
     s2.scene_fingerprint = lambda: "other"
     with pytest.raises(RuntimeError, match="does not match"):
         training.PhaseResponseBasis(s2, path).build()
+
+
+def _homogeneous_cfg(tmp_path):
+    h = 1.07e-3
+    n = 25
+    return SimulationConfig.from_dict({
+        "mode": "phase_optimization",
+        "physics": {"sound_speed": 343.0, "medium_density": 1.2},
+        "transducer_specs": {"frequency": 40000.0, "diameter": 0.0034, "pitch": 0.0042, "array_n": 2,
+                             "source_radial_order": 16, "source_angular_order": 32},
+        "domain": {"box_size": [(n - 1)*h]*3, "grid_size": [n]*3},
+        "boundary_conditions": {"-x": "open", "+x": "reflecting", "-y": "open", "+y": "open",
+                                "-z": "reflecting", "+z": "open"},
+        "targets": [{"point": [(n - 1)*h*0.5]*3}],
+        "solver": {"backend": "gpu_iterative", "pml": {"cells": 4},
+                   "target_response_cache": str(tmp_path/"responses.npz")},
+    })
+
+
+def test_target_points_and_response_cache(tmp_path):
+    """Target validation (inside the domain, outside the PML) and the response
+    cache file: round trip, and a fingerprint that changes with the scene."""
+    from ultrasound_solver_amd.solvers import HelmholtzSolver
+    cfg = _homogeneous_cfg(tmp_path)
+    s = HelmholtzSolver(cfg)
+    h = cfg.domain.dx
+    with pytest.raises(ValueError, match="inside the PDE domain"):
+        s._local_target_points(np.array([[0.0, 0.0, -h]]))
+    with pytest.raises(ValueError, match="lies in PML"):
+        s._local_target_points(np.array([[2*h, 12*h, 5*h]]))
+    pts = np.array([[12*h, 12*h, 8*h], [10*h, 13*h, 3*h]])
+    assert np.allclose(s._local_target_points(pts), pts)
+    path, cached = s._read_target_response_cache()
+    assert cached is None and path == tmp_path/"responses.npz"
+    resp = np.arange(8).reshape(2, 4)*(1 + 2j)
+    s._write_target_response_cache(path, pts, resp)
+    _, (p2, r2) = s._read_target_response_cache()
+    np.testing.assert_array_equal(p2, pts)
+    np.testing.assert_array_equal(r2, resp)
+    s._ensure_target_surface_solver = None  # every point is cached
+    np.testing.assert_array_equal(s.target_response(pts[::-1]), resp[::-1])
+    cfg2 = _homogeneous_cfg(tmp_path)
+    cfg2.boundary_conditions["+x"] = "open"
+    assert HelmholtzSolver(cfg2)._read_target_response_cache()[1] is None

@@ -11,7 +11,9 @@
 * ``HelmholtzSolver`` (alias ``HelmholtzDirectSolver``) --
   src/algorithm/solvers/helmholtz_solver.py:29-740: ``assemble_matrix``,
   ``solve``, ``solve_rhs``, ``solve_adjoint``, ``material_vjp``,
-  ``set_material_fields``, ``scene_fingerprint``, ``target_response``,
+  ``set_material_fields``, ``scene_fingerprint``, ``target_response``
+  (exact homogeneous surface condensation with the reference's response
+  cache file), ``field_at``, ``field_at_direct``,
   ``build_phase_response_basis``, ``close``; Fortran-order reshapes.
 
 The backend string keeps the reference's ``solver.backend`` key; only
@@ -154,6 +156,8 @@ class HelmholtzSolver:
         self._source_definition = asdict(config.specs)
         self.source_columns = None
         self.warp_engine = SimpleNamespace(source_columns=None, operator=None)
+        self._target_surface_solver = None
+        self._target_source_masses = None
         s = config.solver
         self.iterative_solver = GpuIterativeSolver(
             restart=int(s.get("gmres_restart", 30)), maxiter=int(s.get("gmres_maxiter", 3000)),
@@ -306,6 +310,8 @@ class HelmholtzSolver:
         self.assemble_matrix()
 
     def close(self):
+        self._target_surface_solver = None
+        self._target_source_masses = None
         self.invalidate_factorization()
 
     # -- solves -----------------------------------------------------------
@@ -376,26 +382,157 @@ class HelmholtzSolver:
         g_rho, g_c = self.operator.material_vjp(u, np.asarray(lam).reshape(self.shape, order="F").ravel(order="F"))
         return g_rho.cpu().numpy(), g_c.cpu().numpy()
 
-    def target_response(self, points):
-        """(P, M) response of the discrete PDE at physical points: the trilinear
-        rows of G, obtained by reciprocity with one batched adjoint solve
-        (conj(lambda) = A^{-H} conj(t), response = a * B^T lambda)."""
+    # -- target responses ---------------------------------------------------
+    def _homogeneous(self):
+        return not (self.cfg.obstacles or self.cfg.physics.material_field_file or self._material_fields is not None)
+
+    def _ensure_target_surface_solver(self):
+        """Exact homogeneous surface condensation (helmholtz_solver.py:78-103)."""
+        if not self._homogeneous():
+            raise ValueError("Surface condensation requires a homogeneous medium")
+        if physics.boundary_type(self.cfg.boundary_conditions.get("-z", "reflecting")) != "reflecting":
+            raise ValueError("Surface condensation requires a reflecting baffled -z face")
+        if self._target_surface_solver is not None:
+            return
+        from .surface import HomogeneousSurfaceSolver
+        d, sp_ = self.cfg.domain, self.cfg.specs
+        node, col, mass = physics.source_deposition(self.shape[:2], d.dx, self.transducers.centers - self.origin,
+                                                    sp_.diameter, sp_.source_radial_order, sp_.source_angular_order)
+        self._target_source_masses = sp.csr_matrix((mass, (node, col)),
+                                                   shape=(self.shape[0]*self.shape[1], self.transducers.num_transducers))
+        self._target_surface_solver = HomogeneousSurfaceSolver(
+            self.shape, d.dx, self.cfg.frequency, self.cfg.physics.sound_speed, self.cfg.physics.medium_density,
+            self.cfg.boundary_conditions, pml=self.cfg.solver.get("pml", {}), device=self.device)
+
+    def _local_target_points(self, points):
+        """Physical points -> local coordinates, rejected outside the domain or
+        inside the PML (helmholtz_solver.py:105-123)."""
+        points = np.asarray(points, dtype=float)
+        local = points - self.origin
+        extent = np.asarray(self.cfg.domain.box_size, dtype=float)
+        if (points.ndim != 2 or points.shape[1] != 3 or not np.isfinite(points).all()
+                or np.any(local < -1e-12) or np.any(local > extent + 1e-12)):
+            raise ValueError("Targets must be finite physical coordinates inside the PDE domain")
+        pml = self.cfg.solver.get("pml", {})
+        cells = int(pml.get("cells", 0))
+        if cells:
+            faces = pml.get("faces", [f for f in physics.FACES
+                                      if physics.boundary_type(self.cfg.boundary_conditions[f]) == "open"])
+            thickness = cells*self.cfg.domain.dx
+            for face in faces:
+                axis = "xyz".index(face[1])
+                distance = local[:, axis] if face[0] == "-" else extent[axis] - local[:, axis]
+                if np.any(distance < thickness - 1e-12):
+                    raise ValueError("Target lies in PML; enlarge the physical interior")
+        return local
+
+    def _target_response_cache_fingerprint(self):
+        """Scene fingerprint plus the implementation that computes responses
+        (helmholtz_solver.py:125-144): this module, the surface solver, the
+        host physics and the device library."""
+        from pathlib import Path
+        here = Path(__file__).resolve().parent
+        digest = hashlib.sha256()
+        for path in (here/"solvers.py", here/"surface.py", here/"physics.py", here/"csrc"/"hz_surface.hpp"):
+            digest.update(path.read_bytes())
+        payload = {"model": self.MODEL_VERSION, "scene": self.scene_fingerprint(),
+                   "origin_m": self.origin.tolist(), "implementation_sha256": digest.hexdigest()}
+        return hashlib.sha256(json.dumps(payload, sort_keys=True, separators=(",", ":")).encode()).hexdigest()
+
+    def _read_target_response_cache(self):
+        """(path, (points, response) or None) (helmholtz_solver.py:146-166)."""
+        from pathlib import Path
+        filename = str(self.cfg.solver.get("target_response_cache", "")).strip()
+        path = Path(filename) if filename else None
+        if path is None or not path.is_file():
+            return path, None
+        try:
+            with np.load(path, allow_pickle=False) as data:
+                if str(data["fingerprint"]) != self._target_response_cache_fingerprint():
+                    return path, None
+                points = np.asarray(data["points_m"], dtype=float)
+                response = np.asarray(data["response"], dtype=complex)
+        except (KeyError, OSError, ValueError):
+            return path, None
+        if (points.ndim != 2 or points.shape[1] != 3
+                or response.shape != (len(points), self.transducers.num_transducers)
+                or not np.isfinite(points).all() or not np.isfinite(response).all()):
+            return path, None
+        return path, (points, response)
+
+    def _write_target_response_cache(self, path, points, response):
+        """Atomic rewrite of the cache file (helmholtz_solver.py:168-177)."""
+        if path is None:
+            return
+        path.parent.mkdir(parents=True, exist_ok=True)
+        temporary = path.with_suffix(".tmp.npz")
+        np.savez_compressed(temporary, fingerprint=self._target_response_cache_fingerprint(),
+                            points_m=points, response=response)
+        temporary.replace(path)
+
+    def _adjoint_target_response(self, local):
+        """Rows of G at local points by reciprocity, one batched adjoint solve
+        (conj(lambda) = A^{-H} conj(t), response = a * B^T lambda).  Serves
+        heterogeneous scenes, where the reference raises."""
         from .training import trilinear_rows
         self.assemble_matrix()
-        points = np.asarray(points, dtype=float)
-        rows, weights = trilinear_rows(points, self.shape, self.origin, self.cfg.domain.dx,
+        rows, weights = trilinear_rows(local + self.origin, self.shape, self.origin, self.cfg.domain.dx,
                                        np.asarray(self.cfg.domain.box_size, float))
-        t = np.zeros((self.total_dofs, len(points)), dtype=np.complex128)
-        for q in range(len(points)):
+        t = np.zeros((self.total_dofs, len(local)), dtype=np.complex128)
+        for q in range(len(local)):
             np.add.at(t[:, q], rows[q], weights[q])
         lam = np.conj(self.iterative_solver.solve(self.operator, t, adjoint=True).cpu().numpy().T)
         return (self.source_columns.T @ lam).T*self.transducers.amplitudes[None, :]
+
+    def target_response(self, points):
+        """(P, M) responses of the corrected discrete PDE at physical points
+        (helmholtz_solver.py:179-208): cached rows are reused, the rest come
+        from the exact homogeneous surface condensation on the device.  A
+        heterogeneous scene (obstacle, material arrays or file), where the
+        reference raises, is served by the adjoint solve instead."""
+        points = np.asarray(points, dtype=float)
+        local = self._local_target_points(points)
+        path, cached = self._read_target_response_cache()
+        channels = self.transducers.num_transducers
+        result = np.empty((len(points), channels), complex)
+        if cached is None:
+            cached_points, cached_response = np.empty((0, 3), float), np.empty((0, channels), complex)
+        else:
+            cached_points, cached_response = cached
+        missing = []
+        for index, point in enumerate(points):
+            matches = np.flatnonzero(np.all(np.isclose(cached_points, point, rtol=0, atol=1e-12), axis=1))
+            if matches.size:
+                result[index] = cached_response[matches[0]]
+            else:
+                missing.append(index)
+        if missing:
+            if self._homogeneous():
+                self._ensure_target_surface_solver()
+                computed = self._target_surface_solver.flux_target_response(self._target_source_masses,
+                                                                            local[missing])
+            else:
+                computed = self._adjoint_target_response(local[missing])
+            result[missing] = computed
+            self._write_target_response_cache(path, np.vstack((cached_points, points[missing])),
+                                              np.vstack((cached_response, computed)))
+        return result
 
     def field_at(self, points, phases=None):
         values = self.transducers.phases if phases is None else np.asarray(phases, dtype=float)
         if values.shape != (self.transducers.num_transducers,) or not np.isfinite(values).all():
             raise ValueError("phases must match the transducer array and be finite")
         return self.target_response(points) @ np.exp(1j*values)
+
+    def field_at_direct(self, points, phases=None):
+        """Surface solve for the phases, then trilinear samples
+        (helmholtz_solver.py:216-222)."""
+        values = self.transducers.phases if phases is None else np.asarray(phases, dtype=float)
+        if values.shape != (self.transducers.num_transducers,) or not np.isfinite(values).all():
+            raise ValueError("phases must match the transducer array and be finite")
+        self._ensure_target_surface_solver()
+        self._target_surface_solver.solve_flux(self._target_source_masses, values)
+        return self._target_surface_solver.sample(self._local_target_points(points))
 
     def build_phase_response_basis(self):
         from .training import PhaseResponseBasis
