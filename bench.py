@@ -28,7 +28,9 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--n", type=int, default=128)
     p.add_argument("--chunk", type=int, default=0, help="RHS per step; 0 = autotuned")
-    p.add_argument("--restart", type=int, default=30)
+    p.add_argument("--restart", type=int, default=30, help="plain FP64 form (the reference's GMRES restart)")
+    p.add_argument("--mixed-restart", type=int, default=10,
+                   help="restart length of the mixed form's inner complex64 GMRES")
     p.add_argument("--maxiter", type=int, default=60000)
     p.add_argument("--plain-maxiter", type=int, default=300000,
                    help="inner-step cap of the plain form (Jacobi needs far more steps than multigrid)")
@@ -95,7 +97,7 @@ def main():
         op.set_multigrid(shift=args.mg_shift, pre=args.mg_pre, post=args.mg_post, weight=args.mg_weight,
                          cycle=args.mg_cycle)
     m = scene.num_sources
-    tuning = op.autotune(m, max_chunk=args.chunk or 64, restart=args.restart, preconditioner=args.precond)
+    tuning = op.autotune(m, max_chunk=args.chunk or 64, restart=args.mixed_restart, preconditioner=args.precond)
     if not args.chunk:
         args.chunk = tuning["chunk"]
     torch.cuda.synchronize()
@@ -109,8 +111,8 @@ def main():
         return op.build_rhs(w)
 
     common = dict(restart=args.restart, maxiter=args.maxiter, rtol=args.rtol, backward_error_tol=1e-4)
-    mixed = dict(common, precision="c64", chunk=args.chunk, preconditioner=args.precond, use_graph=bool(args.graph),
-                 arnoldi_block=args.arnoldi_block)
+    mixed = dict(common, restart=args.mixed_restart, precision="c64", chunk=args.chunk, preconditioner=args.precond,
+                 use_graph=bool(args.graph), arnoldi_block=args.arnoldi_block)
     for s in range(args.warmup):
         b = rhs_block(s*args.chunk, args.chunk)
         _, st = op.gmres(b, **mixed)
@@ -190,7 +192,7 @@ def main():
         "ms_per_step": elapsed/args.steps*1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "c64 (FP64 refinement)", "data": "synthetic",
         "config": {"workload": f"heterogeneous {args.n}^3 scene, G columns in chunks of {args.chunk}, "
-                               f"{args.precond}-preconditioned GMRES({args.restart}) c64 + FP64 refinement to {args.rtol:g}",
+                               f"{args.precond}-preconditioned GMRES({args.mixed_restart}) c64 + FP64 refinement to {args.rtol:g}",
                    "grid": [args.n]*3, "rhs_per_step": args.chunk, "sources": m,
                    "autotune": tuning},
         "plain_fp64": {"value": plain_rate, "unit": "solves/s", "rhs": args.plain_rhs,
@@ -239,7 +241,7 @@ def stencil_only(op, args, m):
     k = STENCIL_RHS
     if args.precond == "multigrid":
         op.set_multigrid(shift=args.mg_shift, pre=args.mg_pre, post=args.mg_post, weight=args.mg_weight)
-    tuning = op.autotune(m, max_chunk=16, restart=args.restart, preconditioner=args.precond)
+    tuning = op.autotune(m, max_chunk=16, restart=args.mixed_restart, preconditioner=args.precond)
     if args.stencil_shape >= 0:
         op.set_tile_shapes([args.stencil_shape] + tuning["shapes"][1:])
     print(json.dumps({"autotune": tuning, "stencil_shape": args.stencil_shape}), flush=True)
@@ -295,7 +297,7 @@ def bench_uneven(args):
     setup_s = time.perf_counter() - t0
     edges = [s.shape[0] for s in sc]
     log(f"uneven setup {setup_s:.1f}s: edges {edges}")
-    kw = dict(restart=args.restart, maxiter=args.maxiter, rtol=args.rtol, backward_error_tol=1e-4,
+    kw = dict(restart=args.mixed_restart, maxiter=args.maxiter, rtol=args.rtol, backward_error_tol=1e-4,
               precision="c64", chunk=args.chunk or 16, preconditioner="multigrid", use_graph=bool(args.graph))
     # Warm-up: one small batch (kernel loading, graph capture paths).
     gmres_batch(ops[:2], [b[:1] for b in bs[:2]], workers=2, **kw)
@@ -321,7 +323,7 @@ def bench_uneven(args):
         "value": ok/batch_s, "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
         "dtype": "c64 (FP64 refinement)", "data": "synthetic",
         "config": {"workload": f"{args.scenes} uneven scenes (edges {sorted(set(edges))}), {args.scene_rhs} G "
-                               f"columns each, multigrid GMRES({args.restart}), {args.workers} streams",
+                               f"columns each, multigrid GMRES({args.mixed_restart}), {args.workers} streams",
                    "edges": edges, "rhs_per_scene": args.scene_rhs},
         "sequential": {"value": seq_ok/seq_s, "unit": "solves/s"},
         "iterations_median": float(np.median(iters)), "setup_s": setup_s}), flush=True)
@@ -337,7 +339,7 @@ def bench_large(args):
     scene = scenes.config_two_layer(256)
     op = _scene_operator(scene, shift=args.mg_shift)
     m = scene.num_sources
-    tuning = op.autotune(m, max_chunk=16, restart=args.restart, preconditioner="multigrid")
+    tuning = op.autotune(m, max_chunk=16, restart=args.mixed_restart, preconditioner="multigrid")
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
     log(f"large setup {setup_s:.1f}s N={op.n}; autotune {tuning}")
@@ -346,7 +348,7 @@ def bench_large(args):
     torch.cuda.reset_peak_memory_stats()
     torch.cuda.synchronize()
     ts = time.perf_counter()
-    _, st = op.gmres(b, restart=args.restart, maxiter=args.maxiter, rtol=args.rtol, backward_error_tol=1e-4,
+    _, st = op.gmres(b, restart=args.mixed_restart, maxiter=args.maxiter, rtol=args.rtol, backward_error_tol=1e-4,
                      precision="c64", chunk=k, preconditioner="multigrid", use_graph=bool(args.graph))
     torch.cuda.synchronize()
     solve_s = time.perf_counter() - ts
@@ -369,7 +371,7 @@ def bench_large(args):
         "value": ok/solve_s, "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
         "dtype": "c64 (FP64 refinement)", "data": "synthetic",
         "config": {"workload": "two-layer 256^3 scene with rigid ellipsoid, 16 G columns, multigrid "
-                               f"GMRES({args.restart}) c64 + FP64 refinement", "grid": [256]*3, "rhs": k,
+                               f"GMRES({args.mixed_restart}) c64 + FP64 refinement", "grid": [256]*3, "rhs": k,
                    "autotune": tuning},
         "iterations_median": float(np.median([x["iterations"] for x in st])),
         "stencil": {"achieved_GBps": alg/st_s/1e9, "peak_GBps": 8000.0, "frac": alg/st_s/1e9/8000.0,
@@ -410,17 +412,27 @@ def bench_surface(args, points=64):
         resp = dev.flux_target_response(masses, pts)
     torch.cuda.synchronize()
     step_s = (time.perf_counter() - ts)/args.steps
-    # The modal product alone: 2 launches of 64 (nx x ny) products, 8 flops per complex MAC.
+    # The modal product alone, through the C ABI on device planes already in
+    # its layout: 2 launches of `points` (nx x ny) products, 8 flops per complex MAC.
+    import ctypes
+    from ultrasound_solver_amd import _capi
     nx, ny, _ = sc.shape
-    planes = torch.randn((points, nx, ny), dtype=torch.complex128, device="cuda:0")
+    src = torch.randn((points, ny, nx), dtype=torch.complex128, device="cuda:0")
+    dst = torch.empty_like(src)
     stream = torch.cuda.current_stream()
+    sp_ = ctypes.c_void_p(stream.cuda_stream)
+
+    def transform():
+        _capi.check(_capi.lib().hz_surface_transform(dev.handle, 0, ctypes.c_void_p(src.data_ptr()),
+                                                     ctypes.c_void_p(dst.data_ptr()), points, sp_),
+                    "hz_surface_transform")
     for _ in range(3):
-        dev.transform(planes)
+        transform()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     reps = 20
     e0.record(stream)
     for _ in range(reps):
-        dev.transform(planes)
+        transform()
     e1.record(stream)
     torch.cuda.synchronize()
     launch_s = e0.elapsed_time(e1)/1e3/reps/2

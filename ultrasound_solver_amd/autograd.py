@@ -22,8 +22,12 @@ import torch
 class PhaseEngine:
     """Solver settings shared by the forward and adjoint solves of a scene."""
 
-    def __init__(self, op, amplitudes, precision="c64", preconditioner="jacobi", rtol=1e-6, restart=30,
+    def __init__(self, op, amplitudes, precision="c64", preconditioner="jacobi", rtol=1e-6, restart=None,
                  maxiter=20000, chunk=32, backward_error_tol=1e-4):
+        """restart: GMRES restart length; by default 10 for the mixed form
+        (its inner complex64 GMRES) and 30, the reference's, for FP64."""
+        if restart is None:
+            restart = 10 if precision == "c64" else 30
         self.op = op
         self.amplitudes = torch.as_tensor(np.asarray(amplitudes, dtype=np.float64), device=op.device)
         self.kw = dict(precision=precision, preconditioner=preconditioner, rtol=rtol, restart=restart,

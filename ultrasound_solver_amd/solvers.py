@@ -22,7 +22,9 @@ Extra optional ``solver:`` keys: ``gmres_precision`` ("fp64" plain form,
 "mixed" complex64 + FP64 refinement), ``gmres_preconditioner`` ("jacobi" or
 "multigrid", mixed form only), ``gmres_chunk`` (right-hand sides per batch, or
 "auto" for the autotuner's choice),
-``gmres_max_refinements``, ``gmres_inner_rtol``.
+``gmres_max_refinements``, ``gmres_inner_rtol``, ``gmres_mixed_restart`` (restart
+length of the mixed form's inner GMRES, 10 by default; ``gmres_restart`` is the
+plain form's, 30 as in the reference).
 """
 import hashlib
 import json
@@ -70,8 +72,14 @@ class GpuIterativeSolver:
 
     def __init__(self, restart=30, maxiter=3000, rtol=1.0e-6, backward_error_tol=1.0e-4,
                  precision="fp64", chunk=8, preconditioner="jacobi", max_refinements=10, inner_rtol=1.0e-5,
-                 verbose=True):
+                 verbose=True, mixed_restart=10):
         self.restart = int(restart)
+        # The mixed form's inner complex64 GMRES restarts every `mixed_restart`
+        # steps: with the multigrid preconditioner the step count barely
+        # depends on it, while the Gram-Schmidt work per step grows with it.
+        self.mixed_restart = int(mixed_restart)
+        if self.restart < 1 or self.mixed_restart < 1:
+            raise ValueError("GMRES restart lengths must be >= 1")
         self.maxiter = int(maxiter)
         self.rtol = float(rtol)
         self.backward_error_tol = float(backward_error_tol)
@@ -100,14 +108,16 @@ class GpuIterativeSolver:
         tensor of the same column count ((N,) stays one row)."""
         b = operator.as_columns(rhs)
         chunk = 1
+        restart = self.restart
         if self.precision == "mixed":
+            restart = self.mixed_restart
             chunk = self.chunk
             if not chunk:
                 if operator.tuning is None:
-                    operator.autotune(max(b.shape[0], 64), restart=self.restart, preconditioner=self.preconditioner)
+                    operator.autotune(max(b.shape[0], 64), restart=restart, preconditioner=self.preconditioner)
                 chunk = operator.tuning["chunk"]
         x, stats = operator.gmres(
-            b, restart=self.restart, maxiter=self.maxiter, rtol=self.rtol,
+            b, restart=restart, maxiter=self.maxiter, rtol=self.rtol,
             backward_error_tol=self.backward_error_tol,
             precision="c128" if self.precision == "fp64" else "c64", adjoint=adjoint,
             chunk=chunk, max_refinements=self.max_refinements,
@@ -166,7 +176,8 @@ class HelmholtzSolver:
             precision=str(s.get("gmres_precision", "fp64")), chunk=s.get("gmres_chunk", 8),
             preconditioner=str(s.get("gmres_preconditioner", "jacobi")),
             max_refinements=int(s.get("gmres_max_refinements", 10)),
-            inner_rtol=float(s.get("gmres_inner_rtol", 1.0e-5)))
+            inner_rtol=float(s.get("gmres_inner_rtol", 1.0e-5)),
+            mixed_restart=int(s.get("gmres_mixed_restart", 10)))
 
     # -- scene ------------------------------------------------------------
     @property
