@@ -36,6 +36,8 @@
  *                      PhaseResponseBasis.phase_vjp (phase_response_basis.py:208-253).
  *   hz_sample          trilinear rows of PhaseResponseBasis.target_response
  *                      (phase_response_basis.py:165-174).
+ *   hz_deposit_sources the aperture deposition of source_mass_matrix
+ *                      (src/algorithm/physics/sinc_aperture.py:36-72).
  *   hz_z_transfer, hz_surface_*
  *                      HomogeneousSurfaceSolver (src/algorithm/solvers/
  *                      homogeneous_surface.py:41-259), the exact target
@@ -260,6 +262,20 @@ hz_status hz_set_tile_shapes(hz_operator *op, const int32_t *shapes, int32_t len
 
 /* Device bytes held by the operator (coefficients plus cached workspace). */
 int64_t hz_device_bytes(const hz_operator *op);
+
+/* Bilinear deposition of sinc apertures on the z = 0 plane
+ * (sinc_aperture.py:36-72, the masses of WarpAssemblyEngine.source_columns):
+ * emitter e has centre (centers[2e], centers[2e+1]) and the q quadrature
+ * points centre + (offsets[2p], offsets[2p+1]) with weights[p].  mass
+ * (m x capacity, row-major) receives, for every node of the emitter's box
+ * boxes[4e..4e+3] = (i0, j0, width, height) in the order t = di + width*dj,
+ * the mass of node (i0 + di, j0 + dj): one sequential FP64 sum in the
+ * reference's order (corners (0,0), (0,1), (1,0), (1,1), quadrature index
+ * ascending), bit-identical to it; 0 where no point reaches the node.  All
+ * arrays are host memory; the call is synchronous. */
+hz_status hz_deposit_sources(int32_t m, const double *centers, int32_t q, const double *offsets,
+                             const double *weights, double dx, const int32_t *boxes, int32_t capacity,
+                             double *mass);
 
 /* ---- Homogeneous surface condensation -------------------------------------
  * The exact strategy of HomogeneousSurfaceSolver

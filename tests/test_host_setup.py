@@ -45,19 +45,6 @@ def test_validation_matches_reference_messages():
         physics.axis_tables((9, 9, 9), {"up": "open"})
 
 
-@pytest.mark.parametrize("name", ["array2x2_default_order", "array3x3_coarse_order"])
-def test_deposition_matches_reference(golden_dir, name):
-    f = np.load(golden_dir/"source_fixtures.npz")
-    case = json.loads(str(f["meta"]))[name]
-    node, col, mass = physics.source_deposition(case["shape_xy"], case["dx"], f[f"{name}/centers"],
-                                                case["diameter"], *case["orders"])
-    # fixture is CSC over (node, column): same ordering (column, then node)
-    np.testing.assert_array_equal(node, f[f"{name}/mass_indices"])
-    np.testing.assert_array_equal(mass, f[f"{name}/mass_data"])
-    counts = np.bincount(col, minlength=len(f[f"{name}/centers"]))
-    np.testing.assert_array_equal(np.concatenate([[0], np.cumsum(counts)]), f[f"{name}/mass_indptr"])
-
-
 def test_centers_and_phases_match_archive(golden_dir):
     a = np.load(golden_dir/"archive_fixture.npz")
     cen = physics.ideal_grid_centers(12, 0.01, 0.12, 0.12)

@@ -20,7 +20,7 @@ EXPORTS = ("hz_create", "hz_destroy", "hz_apply", "hz_gmres", "hz_set_sources", 
            "hz_material_vjp", "hz_basis_field", "hz_basis_adjoint", "hz_basis_field_match",
            "hz_z_transfer", "hz_surface_create", "hz_surface_destroy", "hz_surface_tables",
            "hz_surface_set_sources", "hz_surface_transform", "hz_surface_solve_flux", "hz_surface_planes",
-           "hz_surface_sample", "hz_surface_target_response")
+           "hz_surface_sample", "hz_surface_target_response", "hz_deposit_sources")
 PRE_JACOBI, PRE_MULTIGRID = 0, 1
 
 
@@ -107,6 +107,9 @@ def lib():
         handle.hz_material_vjp.argtypes = [c_void_p, c_void_p, c_void_p, c_int32, c_int64, c_void_p, c_void_p,
                                            c_void_p]
         handle.hz_set_tile_shapes.argtypes = [c_void_p, POINTER(c_int32), c_int32]
+        handle.hz_deposit_sources.argtypes = [c_int32, POINTER(c_double), c_int32, POINTER(c_double),
+                                              POINTER(c_double), c_double, POINTER(c_int32), c_int32,
+                                              POINTER(c_double)]
         handle.hz_z_transfer.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int64, c_void_p,
                                          c_void_p, c_void_p]
         handle.hz_surface_create.argtypes = [POINTER(SurfaceDesc), POINTER(c_void_p)]

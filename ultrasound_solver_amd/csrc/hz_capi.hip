@@ -23,6 +23,7 @@
 #include "hz_krylov.hpp"
 #include "hz_mg.hpp"
 #include "hz_slab.hpp"
+#include "hz_source.hpp"
 #include "hz_stencil.hpp"
 #include "hz_surface.hpp"
 
@@ -2066,6 +2067,36 @@ hz_status hz_surface_target_response(hz_surface *s, const int32_t *low, const do
             HZ_LAUNCHED();
         }
         HZ_HIP(hipStreamSynchronize(st));  // the point arrays are reused by the next call
+    });
+}
+
+}  // extern "C"
+
+// ---- Source deposition ------------------------------------------------------
+
+extern "C" {
+
+hz_status hz_deposit_sources(int32_t m, const double *centers, int32_t q, const double *offsets,
+                             const double *weights, double dx, const int32_t *boxes, int32_t capacity, double *mass) {
+    return guarded([&] {
+        if (m < 1 || q < 1 || capacity < 1) throw BadArgument("need m, q, capacity >= 1");
+        if (!centers || !offsets || !weights || !boxes || !mass) throw BadArgument("null host array");
+        if (!(dx > 0) || !std::isfinite(dx)) throw BadArgument("dx must be finite and positive");
+        for (int e = 0; e < m; ++e) {
+            const int32_t *b = boxes + 4 * e;
+            if (b[2] < 1 || b[3] < 1 || (int64_t)b[2] * b[3] > capacity || b[0] < 0 || b[1] < 0)
+                throw BadArgument("emitter box empty, negative or larger than capacity");
+        }
+        DevBuf dc, doff, dw, db, dm;
+        upload(dc, centers, (size_t)2 * m);
+        upload(doff, offsets, (size_t)2 * q);
+        upload(dw, weights, (size_t)q);
+        upload(db, boxes, (size_t)4 * m);
+        dm.ensure((size_t)m * capacity * sizeof(double));
+        hipLaunchKernelGGL(k_deposit, dim3(m), dim3(DEPOSIT_CHUNK), 0, nullptr, dx, (int)q, dc.as<double>(),
+                           doff.as<double>(), dw.as<double>(), db.as<int>(), (int)capacity, dm.as<double>());
+        HZ_LAUNCHED();
+        HZ_HIP(hipMemcpy(mass, dm.p, (size_t)m * capacity * sizeof(double), hipMemcpyDeviceToHost));
     });
 }
 

@@ -10,8 +10,8 @@ follow the reference so that the device operator reproduces its matrix A:
   (src/algorithm/physics/conservative_helmholtz.py:48-64);
 * open/impedance face terms i omega (w_b w_c s_b s_c / h) Y, Y = 1/(rho c)
   or 1/Z (conservative_helmholtz.py:89-103);
-* sinc aperture quadrature and bilinear unit-mass deposition
-  (src/algorithm/physics/sinc_aperture.py:19-72), flux scale
+* sinc aperture quadrature and bilinear unit-mass deposition, the masses
+  computed on the device (src/algorithm/physics/sinc_aperture.py:19-72), flux scale
   -2 pi / (rho_ref h^3) (sinc_aperture.py:75-85);
 * material blend of 1/rho and 1/(rho c^2) across a smoothed SDF
   (src/algorithm/physics/warp_utils.py:11-23, 44-79).
@@ -209,10 +209,13 @@ def source_deposition(shape_xy, dx, centers, diameter, radial_order=64, angular_
     """Unit-mass bilinear deposition of every aperture on the z = 0 plane.
 
     Returns (node, column, mass) triplets, node = i + nx*j, sorted by column
-    then node; each column sums to 1.  All emitters are deposited at once;
-    per (column, node) the quadrature contributions are accumulated in the
-    reference's order (corners 00, 01, 10, 11, each over the quadrature).
+    then node; each column sums to 1 (sinc_aperture.py:36-72).  Validation,
+    quadrature and each emitter's node box are host work; the masses come
+    from the device (``hz_deposit_sources``), one sequential sum per node in
+    the reference's order, so they are bit-identical to the reference's.
     """
+    import ctypes
+    from . import _capi
     nx, ny = (int(v) for v in shape_xy)
     centers = np.asarray(centers, float)
     if (min(nx, ny) < 3 or not np.isfinite(dx) or dx <= 0 or centers.ndim != 2
@@ -224,23 +227,28 @@ def source_deposition(shape_xy, dx, centers, diameter, radial_order=64, angular_
     if np.any(centers[:, :2] - diameter/2 < dx) or np.any(centers[:, :2] + diameter/2 > extent - dx):
         raise ValueError("Sinc aperture needs one grid cell clearance from side boundaries")
     m = len(centers)
-    g = (centers[:, None, :2] + offsets[None, :, :2])/dx          # (M, Q, 2)
-    base = np.floor(g).astype(np.int64)
-    t = g - base
-    keys, vals = [], []
-    for a, b in ((0, 0), (0, 1), (1, 0), (1, 1)):
-        fx = t[..., 0] if a else 1 - t[..., 0]
-        fy = t[..., 1] if b else 1 - t[..., 1]
-        node = base[..., 0] + a + nx*(base[..., 1] + b)
-        keys.append(node + np.arange(m)[:, None]*(nx*ny))
-        vals.append(weights[None, :]*fx*fy)
-    keys = np.concatenate(keys, axis=1).ravel()
-    vals = np.concatenate(vals, axis=1).ravel()
-    uniq, inv = np.unique(keys, return_inverse=True)
-    mass = np.bincount(inv, weights=vals)
-    keep = mass != 0
-    uniq, mass = uniq[keep], mass[keep]
-    return uniq % (nx*ny), (uniq//(nx*ny)).astype(np.int32), mass
+    # Node box of each aperture, one cell of margin on each side: every
+    # corner node of every quadrature point lies inside.
+    r = diameter/2
+    lo = np.maximum(np.floor((centers[:, :2] - r)/dx).astype(np.int64) - 1, 0)
+    hi = np.minimum(np.floor((centers[:, :2] + r)/dx).astype(np.int64) + 2, np.array([nx - 1, ny - 1]))
+    size = hi - lo + 1
+    boxes = np.ascontiguousarray(np.column_stack((lo, size)), dtype=np.int32)
+    cap = int((size[:, 0]*size[:, 1]).max())
+    mass = np.empty((m, cap))
+    xy = np.ascontiguousarray(centers[:, :2])
+    off = np.ascontiguousarray(offsets[:, :2])
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    dptr = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    _capi.check(_capi.lib().hz_deposit_sources(m, dptr(xy), len(w), dptr(off), dptr(w), float(dx),
+                                               boxes.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), cap,
+                                               dptr(mass)), "hz_deposit_sources")
+    t = np.arange(cap)
+    width = size[:, 0]
+    keep = (t[None, :] < (size[:, 0]*size[:, 1])[:, None]) & (mass != 0)
+    col, tt = np.nonzero(keep)
+    node = (lo[col, 0] + tt % width[col]) + nx*(lo[col, 1] + tt//width[col])
+    return node.astype(np.int64), col.astype(np.int32), mass[col, tt]
 
 
 def flux_scale(dx, reference_density):
