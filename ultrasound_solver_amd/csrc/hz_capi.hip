@@ -227,9 +227,15 @@ void launch_restrict(const MgLevel &L, const MgLevel &C, const cx<float> *r, int
 
 void launch_prolong(const MgLevel &L, const MgLevel &C, cx<float> *xf, int k, const int *live, const cx<float> *b,
                     const cx<float> *dinv, float w, const float *bdiv, int conj_d, hipStream_t s) {
-    const int nb = plane_blocks(C.nx, C.ny, C.nz, k);  // one thread per coarse cell
+    // One thread per coarse cell and group of rs slots: the slots of a group
+    // share the cell's weights and first-sweep factors.  Groups are as large
+    // as the grid allows while it still has >= 2048 workgroups.
+    const int per_slot = plane_blocks(C.nx, C.ny, C.nz, 1);
+    const int groups = std::max(1, std::min(k, (2048 + per_slot - 1) / per_slot));
+    const int rs = (k + groups - 1) / groups;
+    const int nb = plane_blocks(C.nx, C.ny, C.nz, (k + rs - 1) / rs);
     hipLaunchKernelGGL((k_prolong<float>), dim3(nb), dim3(256), 0, s, (const cx<float> *)C.xx.as<cx<float>>(), C.nx,
-                       C.ny, C.nz, xf, L.nx, L.ny, L.nz, k, live, L.v.soft, b, dinv, w, bdiv, conj_d, nb);
+                       C.ny, C.nz, xf, L.nx, L.ny, L.nz, k, rs, live, L.v.soft, b, dinv, w, bdiv, conj_d, nb);
     HZ_LAUNCHED();
 }
 

@@ -234,9 +234,11 @@ __device__ __forceinline__ CellAxis cell_axis(int c, int n, int m) {
 }
 
 // Fine iterate after the coarse correction, one thread per (coarse cell,
-// slot): the cell's 8 coarse corners are loaded once and its (up to 2 x 2 x
-// 2) fine nodes written; grid: flattened plane tiles of the coarse grid times
-// k slots, padded to a multiple of 8 (nblocks).
+// group of rs slots): the cell's corner weights and its (up to 2 x 2 x 2)
+// fine nodes' first-sweep factors are computed once, then for each live slot
+// of the group the 8 coarse corners are loaded and the fine nodes written.
+// Grid: flattened plane tiles of the coarse grid times ceil(k / rs) slot
+// groups, padded to a multiple of 8 (nblocks).
 //   b == nullptr: xf += P xc
 //   b != nullptr: xf = f b + P xc with f = w dinv / bdiv (dinv conjugated when
 //                 conj_d): the first Jacobi sweep from a zero guess is
@@ -247,13 +249,13 @@ __device__ __forceinline__ CellAxis cell_axis(int c, int n, int m) {
 // rows get none.
 template <typename T>
 __global__ __launch_bounds__(256) void k_prolong(const cx<T> *__restrict__ xc, int nxc, int nyc, int nzc,
-                                                 cx<T> *__restrict__ xf, int nxf, int nyf, int nzf, int k,
+                                                 cx<T> *__restrict__ xf, int nxf, int nyf, int nzf, int k, int rs,
                                                  const int *__restrict__ live, int soft,
                                                  const cx<T> *__restrict__ b, const cx<T> *__restrict__ dinv, T w,
                                                  const T *__restrict__ bdiv, int conj_d, int nblocks) {
-    const PlaneTile t = plane_tile(nxc, nyc, nzc, k, nblocks);
-    if (!t.in_grid || (live && !live[t.r])) return;
-    const int r = t.r;
+    const PlaneTile t = plane_tile(nxc, nyc, nzc, (k + rs - 1) / rs, nblocks);
+    if (!t.in_grid) return;
+    const int r0 = t.r * rs, r1 = min(k, r0 + rs);
     const int64_t nf = (int64_t)nxf * nyf * nzf;
     const int64_t nc = (int64_t)nxc * nyc * nzc;
     const CellAxis ax = cell_axis(t.i, nxf, nxc), ay = cell_axis(t.j, nyf, nyc), az = cell_axis(t.k, nzf, nzc);
@@ -265,49 +267,61 @@ __global__ __launch_bounds__(256) void k_prolong(const cx<T> *__restrict__ xc, i
     cw[1][1] = (soft && on_soft_face(1, ay.c1, 1, 3, nyc, 3, soft & 12)) ? T(0) : T(1);
     cw[2][0] = (soft && on_soft_face(1, 1, t.k, 3, 3, nzc, soft & 48)) ? T(0) : T(1);
     cw[2][1] = (soft && on_soft_face(1, 1, az.c1, 3, 3, nzc, soft & 48)) ? T(0) : T(1);
-    const cx<T> *src = xc + r * nc;
     const int cx0[2] = {t.i, ax.c1}, cy0[2] = {t.j, ay.c1}, cz0[2] = {t.k, az.c1};
-    cx<T> e[2][2][2];
+    int64_t corner[2][2][2];
 #pragma unroll
     for (int c = 0; c < 2; ++c)
 #pragma unroll
         for (int bq = 0; bq < 2; ++bq)
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
-                e[c][bq][a] = src[cx0[a] + (int64_t)nxc * (cy0[bq] + (int64_t)nyc * cz0[c])];
+            for (int a = 0; a < 2; ++a) corner[c][bq][a] = cx0[a] + (int64_t)nxc * (cy0[bq] + (int64_t)nyc * cz0[c]);
     // Weights of fine offset d (0: the coarse node, 1: the midpoint) from corner q.
     auto wgt = [](int d, int q) -> T { return d == 0 ? (q == 0 ? T(1) : T(0)) : T(0.5); };
+    // Per fine node of the cell (index dz*4 + dy*2 + dx): offset, presence,
+    // identity-row flag and first-sweep factor.
+    int64_t pf[8];
+    bool here[8], fixed[8];
+    cx<T> f[8];
 #pragma unroll
-    for (int dz = 0; dz < 2; ++dz) {
-        const int fk = az.f0 + dz;
+    for (int q = 0; q < 8; ++q) {
+        const int dz = q >> 2, dy = (q >> 1) & 1, dx = q & 1;
+        here[q] = dz < az.count && dy < ay.count && dx < ax.count;
+        const int fi = ax.f0 + (here[q] ? dx : 0), fj = ay.f0 + (here[q] ? dy : 0), fk = az.f0 + (here[q] ? dz : 0);
+        pf[q] = fi + (int64_t)nxf * (fj + (int64_t)nyf * fk);
+        fixed[q] = soft && on_soft_face(fi, fj, fk, nxf, nyf, nzf, soft);
+        f[q] = mk<T>(0, 0);
+        if (b && here[q]) {
+            const cx<T> d = conj_d ? conj(dinv[pf[q]]) : dinv[pf[q]];
+            f[q] = (bdiv ? w / bdiv[pf[q]] : w) * d;
+        }
+    }
+    for (int r = r0; r < r1; ++r) {
+        if (live && !live[r]) continue;
+        const cx<T> *src = xc + r * nc;
+        cx<T> e[2][2][2];
 #pragma unroll
-        for (int dy = 0; dy < 2; ++dy) {
-            const int fj = ay.f0 + dy;
+        for (int c = 0; c < 2; ++c)
 #pragma unroll
-            for (int dx = 0; dx < 2; ++dx) {
-                if (dz >= az.count || dy >= ay.count || dx >= ax.count) continue;
-                const int fi = ax.f0 + dx;
-                const int64_t p = fi + (int64_t)nxf * (fj + (int64_t)nyf * fk);
-                cx<T> acc;
-                if (b) {
-                    const cx<T> d = conj_d ? conj(dinv[p]) : dinv[p];
-                    acc = ((bdiv ? w / bdiv[p] : w) * d) * b[r * nf + p];
-                } else {
-                    acc = xf[r * nf + p];
-                }
-                if (!(soft && on_soft_face(fi, fj, fk, nxf, nyf, nzf, soft))) {
+            for (int bq = 0; bq < 2; ++bq)
 #pragma unroll
-                    for (int c = 0; c < 2; ++c)
+                for (int a = 0; a < 2; ++a) e[c][bq][a] = src[corner[c][bq][a]];
 #pragma unroll
-                        for (int bq = 0; bq < 2; ++bq)
+        for (int q = 0; q < 8; ++q) {
+            if (!here[q]) continue;
+            const int dz = q >> 2, dy = (q >> 1) & 1, dx = q & 1;
+            cx<T> acc = b ? f[q] * b[r * nf + pf[q]] : xf[r * nf + pf[q]];
+            if (!fixed[q]) {
 #pragma unroll
-                            for (int a = 0; a < 2; ++a) {
-                                const T wt = wgt(dz, c) * cw[2][c] * wgt(dy, bq) * cw[1][bq] * wgt(dx, a) * cw[0][a];
-                                acc = acc + wt * e[c][bq][a];
-                            }
-                }
-                xf[r * nf + p] = acc;
+                for (int c = 0; c < 2; ++c)
+#pragma unroll
+                    for (int bq = 0; bq < 2; ++bq)
+#pragma unroll
+                        for (int a = 0; a < 2; ++a) {
+                            const T wt = wgt(dz, c) * cw[2][c] * wgt(dy, bq) * cw[1][bq] * wgt(dx, a) * cw[0][a];
+                            acc = acc + wt * e[c][bq][a];
+                        }
             }
+            xf[r * nf + pf[q]] = acc;
         }
     }
 }
