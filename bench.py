@@ -45,7 +45,8 @@ def parse():
                    help="only the stencil launches and a 512 MiB copy (for hardware counter runs)")
     p.add_argument("--mg-shift", type=float, default=1.2)
     p.add_argument("--mg-pre", type=int, default=1)
-    p.add_argument("--mg-post", type=int, default=3)
+    p.add_argument("--mg-post", type=int, default=1, help="post-sweeps on the finest multigrid level")
+    p.add_argument("--mg-post-coarse", type=int, default=4, help="post-sweeps on the coarser levels")
     p.add_argument("--mg-weight", type=float, default=0.7)
     p.add_argument("--mg-cycle", type=int, default=1, choices=[1, 2], help="1: V-cycle, 2: W below the finest level")
     p.add_argument("--graph", type=int, default=0, help="replay each GMRES cycle from a captured hipGraph")
@@ -94,7 +95,8 @@ def main():
         stencil_only(op, args, scene.num_sources)
         return
     if args.precond == "multigrid":
-        op.set_multigrid(shift=args.mg_shift, pre=args.mg_pre, post=args.mg_post, weight=args.mg_weight,
+        op.set_multigrid(shift=args.mg_shift, pre=args.mg_pre, post=args.mg_post, post_coarse=args.mg_post_coarse,
+                         weight=args.mg_weight,
                          cycle=args.mg_cycle)
     m = scene.num_sources
     tuning = op.autotune(m, max_chunk=args.chunk or 64, restart=args.mixed_restart, preconditioner=args.precond)
@@ -240,7 +242,8 @@ def stencil_only(op, args, m):
     import torch
     k = STENCIL_RHS
     if args.precond == "multigrid":
-        op.set_multigrid(shift=args.mg_shift, pre=args.mg_pre, post=args.mg_post, weight=args.mg_weight)
+        op.set_multigrid(shift=args.mg_shift, pre=args.mg_pre, post=args.mg_post, post_coarse=args.mg_post_coarse,
+                         weight=args.mg_weight)
     tuning = op.autotune(m, max_chunk=16, restart=args.mixed_restart, preconditioner=args.precond)
     if args.stencil_shape >= 0:
         op.set_tile_shapes([args.stencil_shape] + tuning["shapes"][1:])

@@ -194,12 +194,14 @@ hz_status hz_sample(hz_operator *op, const int64_t *rows, const double *weights,
 /* Install a multigrid preconditioner (levels[0] is the operator's own grid;
  * level l+1 keeps the even nodes of level l plus its last node on each axis).
  * The mass term of every level is shifted by omega^2 -> omega^2 (1 + i*shift);
- * the smoother is damped Jacobi (weight) with `pre`/`post` sweeps and
+ * the smoother is damped Jacobi (weight) with `pre` sweeps, `post` sweeps on
+ * the finest level and `post_coarse` on the levels below it, and
  * `coarse_sweeps` on the coarsest level; `cycle` coarse-grid corrections per
  * level below the finest (1: V-cycle, 2: W-cycle).  No counterpart in the reference,
  * which only offers Jacobi (src/algorithm/solvers/iterative_solver.py:46-56). */
 hz_status hz_set_multigrid(hz_operator *op, int32_t nlevels, const hz_level_desc *levels, double shift,
-                           int32_t pre, int32_t post, int32_t coarse_sweeps, double weight, int32_t cycle);
+                           int32_t pre, int32_t post, int32_t post_coarse, int32_t coarse_sweeps, double weight,
+                           int32_t cycle);
 
 /* Autotuner.  No counterpart in the reference (its CuPy GMRES has no
  * tunables).  Times every slab tile shape of the complex64 stencil on this

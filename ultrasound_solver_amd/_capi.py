@@ -93,7 +93,7 @@ def lib():
         handle.hz_sample.argtypes = [c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_int64, c_void_p,
                                      c_int64, c_int32, c_void_p]
         handle.hz_set_multigrid.argtypes = [c_void_p, c_int32, POINTER(LevelDesc), c_double, c_int32, c_int32,
-                                            c_int32, c_double, c_int32]
+                                            c_int32, c_int32, c_double, c_int32]
         handle.hz_autotune.argtypes = [c_void_p, c_int32, c_int32, c_int32, c_int32, c_int64, POINTER(c_int32),
                                        POINTER(c_int32), c_int32, c_void_p]
         handle.hz_gmres_batch.argtypes = [c_int32, POINTER(c_void_p), POINTER(GmresOpts), POINTER(c_void_p),

@@ -109,6 +109,7 @@ struct MgLevel {
 struct Multigrid {
     std::vector<std::unique_ptr<MgLevel>> levels;
     int pre = 1, post = 1, coarse_sweeps = 30;
+    int post_coarse = 1;  // post-sweeps on the levels between the finest and the coarsest
     int cycle = 1;  // coarse-grid visits per level below the finest (1: V, 2: W)
     float weight = 0.7f;
     double shift = 0.5;
@@ -385,10 +386,11 @@ void vcycle(Multigrid &mg, int l, const cx<float> *b, cx<float> *x, int k, const
         vcycle<MODE>(mg, l + 1, C.bb.as<cx<float>>(), C.xx.as<cx<float>>(), k, live, s);
         // Post-sweeps ping-pong between x and the level scratch; start where
         // the last one lands in x.
-        cx<float> *cur = (mg.post % 2 == 0) ? x : L.tt.as<cx<float>>();
+        const int post = l == 0 ? mg.post : mg.post_coarse;
+        cx<float> *cur = (post % 2 == 0) ? x : L.tt.as<cx<float>>();
         cx<float> *alt = (cur == x) ? L.tt.as<cx<float>>() : x;
         launch_prolong(L, C, cur, k, live, b, dinv, mg.weight, bdiv, (int)(MODE == MODE_AH), s);
-        for (int it = 0; it < mg.post; ++it) {
+        for (int it = 0; it < post; ++it) {
             level_smooth<MODE>(L, cur, b, alt, mg.weight, k, live, bdiv, s);
             std::swap(cur, alt);
         }
@@ -401,7 +403,7 @@ void vcycle(Multigrid &mg, int l, const cx<float> *b, cx<float> *x, int k, const
             vcycle<MODE>(mg, l + 1, C.bb.as<cx<float>>(), C.xx.as<cx<float>>(), k, live, s);
             launch_prolong(L, C, x, k, live, nullptr, dinv, mg.weight, bdiv, 0, s);
             cx<float> *c2 = x, *a2 = t;
-            for (int it = 0; it < mg.post; ++it) {
+            for (int it = 0; it < post; ++it) {
                 level_smooth<MODE>(L, c2, b, a2, mg.weight, k, live, bdiv, s);
                 std::swap(c2, a2);
             }
@@ -411,7 +413,8 @@ void vcycle(Multigrid &mg, int l, const cx<float> *b, cx<float> *x, int k, const
     }
     // Jacobi sweeps ping-pong between x and the level scratch; start in the
     // buffer that makes the last sweep land in x (no trailing copy).
-    const int swaps = coarsest ? mg.coarse_sweeps - 1 : (mg.pre - 1) + mg.post;
+    const int post = l == 0 ? mg.post : mg.post_coarse;
+    const int swaps = coarsest ? mg.coarse_sweeps - 1 : (mg.pre - 1) + post;
     cx<float> *cur = (swaps % 2 == 0) ? x : L.tt.as<cx<float>>();
     cx<float> *alt = (cur == x) ? L.tt.as<cx<float>>() : x;
     hipLaunchKernelGGL((k_level_jacobi0<float, MODE>), dim3(blocks_for(n, 256 * 8, 2048), k), dim3(256), 0, s, b,
@@ -428,7 +431,7 @@ void vcycle(Multigrid &mg, int l, const cx<float> *b, cx<float> *x, int k, const
         launch_restrict(L, C, alt, k, live, s);
         vcycle<MODE>(mg, l + 1, C.bb.as<cx<float>>(), C.xx.as<cx<float>>(), k, live, s);
         launch_prolong(L, C, cur, k, live, nullptr, dinv, mg.weight, bdiv, 0, s);
-        for (int it = 0; it < mg.post; ++it) {
+        for (int it = 0; it < post; ++it) {
             level_smooth<MODE>(L, cur, b, alt, mg.weight, k, live, bdiv, s);
             std::swap(cur, alt);
         }
@@ -1342,17 +1345,18 @@ hz_status hz_gmres_batch(int32_t nscenes, hz_operator *const *ops, const hz_gmre
 }
 
 hz_status hz_set_multigrid(hz_operator *op, int32_t nlevels, const hz_level_desc *levels, double shift, int32_t pre,
-                           int32_t post, int32_t coarse_sweeps, double weight, int32_t cycle) {
+                           int32_t post, int32_t post_coarse, int32_t coarse_sweeps, double weight, int32_t cycle) {
     return guarded([&] {
         check_op(op);
         if (nlevels < 1 || !levels) throw BadArgument("need at least one level");
-        if (pre < 1 || post < 0 || coarse_sweeps < 1 || !(weight > 0) || !std::isfinite(shift))
+        if (pre < 1 || post < 0 || post_coarse < 0 || coarse_sweeps < 1 || !(weight > 0) || !std::isfinite(shift))
             throw BadArgument("bad multigrid smoothing parameters");
         if (cycle < 1 || cycle > 2) throw BadArgument("cycle must be 1 (V) or 2 (W)");
         auto mg = std::make_unique<Multigrid>();
         mg->cycle = cycle;
         mg->pre = pre;
         mg->post = post;
+        mg->post_coarse = post_coarse;
         mg->coarse_sweeps = coarse_sweeps;
         mg->weight = (float)weight;
         mg->shift = shift;

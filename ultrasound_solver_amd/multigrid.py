@@ -116,7 +116,7 @@ class Hierarchy:
             out.append(np.ascontiguousarray((1j*op.omega*area*adm).ravel(order="F"), dtype=np.complex128))
         return out
 
-    def install(self, op, shift=1.2, pre=1, post=3, coarse_sweeps=30, weight=0.7, cycle=1):
+    def install(self, op, shift=1.2, pre=1, post=1, coarse_sweeps=30, weight=0.7, cycle=1, post_coarse=4):
         keep = []
 
         def ptr(a):
@@ -137,6 +137,7 @@ class Hierarchy:
                 t = lev["faces"][f]
                 d.face_term[f] = ptr(t.view(np.float64)) if t is not None else ctypes.POINTER(ctypes.c_double)()
         _capi.check(_capi.lib().hz_set_multigrid(op.handle, len(self.levels), descs, float(shift), int(pre),
-                                                 int(post), int(coarse_sweeps), float(weight), int(cycle)),
+                                                 int(post), int(post_coarse), int(coarse_sweeps), float(weight),
+                                                 int(cycle)),
                     "hz_set_multigrid")
         return self

@@ -251,11 +251,15 @@ class HelmholtzOperator:
         arr = (ctypes.c_int32*len(shapes))(*[int(v) for v in shapes])
         _capi.check(_capi.lib().hz_set_tile_shapes(self.handle, arr, len(shapes)), "hz_set_tile_shapes")
 
-    def set_multigrid(self, shift=1.2, pre=1, post=3, coarse_sweeps=30, weight=0.7, levels=None, cycle=1):
-        """Install the shifted-Laplacian multigrid preconditioner (mixed form);
-        cycle 2 makes it a W-cycle below the finest level."""
+    def set_multigrid(self, shift=1.2, pre=1, post=1, coarse_sweeps=30, weight=0.7, levels=None, cycle=1,
+                      post_coarse=4):
+        """Install the shifted-Laplacian multigrid preconditioner (mixed form):
+        `post` damped-Jacobi sweeps after the correction on the finest level,
+        `post_coarse` on the levels below it (coarse sweeps cost an eighth per
+        level); cycle 2 makes it a W-cycle below the finest level."""
         from .multigrid import Hierarchy
-        self.hierarchy = Hierarchy(self, levels=levels).install(self, shift, pre, post, coarse_sweeps, weight, cycle)
+        self.hierarchy = Hierarchy(self, levels=levels).install(self, shift, pre, post, coarse_sweeps, weight, cycle,
+                                                                post_coarse)
         return self.hierarchy
 
     # -- sources ----------------------------------------------------------
