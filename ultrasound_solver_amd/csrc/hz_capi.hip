@@ -235,8 +235,12 @@ void launch_prolong(const MgLevel &L, const MgLevel &C, cx<float> *xf, int k, co
     const int groups = std::max(1, std::min(k, (2048 + per_slot - 1) / per_slot));
     const int rs = (k + groups - 1) / groups;
     const int nb = plane_blocks(C.nx, C.ny, C.nz, (k + rs - 1) / rs);
-    hipLaunchKernelGGL((k_prolong<float>), dim3(nb), dim3(256), 0, s, (const cx<float> *)C.xx.as<cx<float>>(), C.nx,
-                       C.ny, C.nz, xf, L.nx, L.ny, L.nz, k, rs, live, L.v.soft, b, dinv, w, bdiv, conj_d, nb);
+    if (L.nx % 2 == 0)
+        hipLaunchKernelGGL((k_prolong<float, 1>), dim3(nb), dim3(256), 0, s, (const cx<float> *)C.xx.as<cx<float>>(),
+                           C.nx, C.ny, C.nz, xf, L.nx, L.ny, L.nz, k, rs, live, L.v.soft, b, dinv, w, bdiv, conj_d, nb);
+    else
+        hipLaunchKernelGGL((k_prolong<float, 0>), dim3(nb), dim3(256), 0, s, (const cx<float> *)C.xx.as<cx<float>>(),
+                           C.nx, C.ny, C.nz, xf, L.nx, L.ny, L.nz, k, rs, live, L.v.soft, b, dinv, w, bdiv, conj_d, nb);
     HZ_LAUNCHED();
 }
 

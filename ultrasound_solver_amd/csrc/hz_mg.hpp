@@ -246,8 +246,11 @@ __device__ __forceinline__ CellAxis cell_axis(int c, int n, int m) {
 // Coarse identity-row corners carry no correction (homogeneous Dirichlet for
 // the correction: for A^H their values are -A_IF^H x_I, scaled like a
 // residual, and must not leak into interior fine nodes), and fine identity
-// rows get none.
-template <typename T>
+// rows get none.  PAIR (even fine x-size, complex64): the two fine nodes of a
+// cell row, 2i and 2i+1, are read (b, or the iterate) and written with one
+// 16-byte access each.
+typedef float hz_f4 __attribute__((ext_vector_type(4)));
+template <typename T, int PAIR>
 __global__ __launch_bounds__(256) void k_prolong(const cx<T> *__restrict__ xc, int nxc, int nyc, int nzc,
                                                  cx<T> *__restrict__ xf, int nxf, int nyf, int nzf, int k, int rs,
                                                  const int *__restrict__ live, int soft,
@@ -305,11 +308,8 @@ __global__ __launch_bounds__(256) void k_prolong(const cx<T> *__restrict__ xc, i
             for (int bq = 0; bq < 2; ++bq)
 #pragma unroll
                 for (int a = 0; a < 2; ++a) e[c][bq][a] = src[corner[c][bq][a]];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            if (!here[q]) continue;
+        auto correct = [&](int q, cx<T> acc) {
             const int dz = q >> 2, dy = (q >> 1) & 1, dx = q & 1;
-            cx<T> acc = b ? f[q] * b[r * nf + pf[q]] : xf[r * nf + pf[q]];
             if (!fixed[q]) {
 #pragma unroll
                 for (int c = 0; c < 2; ++c)
@@ -321,7 +321,34 @@ __global__ __launch_bounds__(256) void k_prolong(const cx<T> *__restrict__ xc, i
                             acc = acc + wt * e[c][bq][a];
                         }
             }
-            xf[r * nf + pf[q]] = acc;
+            return acc;
+        };
+#pragma unroll
+        for (int row = 0; row < 4; ++row) {
+            const int q0 = 2 * row, q1 = q0 + 1;
+            if (PAIR && here[q1]) {  // here[q1] implies here[q0]; pf[q0] is even
+                const hz_f4 in = *reinterpret_cast<const hz_f4 *>((b ? b : xf) + r * nf + pf[q0]);
+                cx<T> a0 = mk<T>(in.x, in.y), a1 = mk<T>(in.z, in.w);
+                if (b) {
+                    a0 = f[q0] * a0;
+                    a1 = f[q1] * a1;
+                }
+                a0 = correct(q0, a0);
+                a1 = correct(q1, a1);
+                hz_f4 out;
+                out.x = a0.re;
+                out.y = a0.im;
+                out.z = a1.re;
+                out.w = a1.im;
+                *reinterpret_cast<hz_f4 *>(xf + r * nf + pf[q0]) = out;
+                continue;
+            }
+#pragma unroll
+            for (int q = q0; q <= q1; ++q) {
+                if (!here[q]) continue;
+                const cx<T> acc = b ? f[q] * b[r * nf + pf[q]] : xf[r * nf + pf[q]];
+                xf[r * nf + pf[q]] = correct(q, acc);
+            }
         }
     }
 }
