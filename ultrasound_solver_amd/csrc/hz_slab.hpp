@@ -10,7 +10,8 @@
 // tile edge), takes the other x-neighbours from the adjacent lanes with DPP
 // wave shifts and the other y-neighbours from the block's planes in LDS, and
 // marches z with the (z-1, z, z+1) values in registers.  That is about
-// (ZB + 2) / ZB + 2 / TY vector loads per node and right-hand side instead of 7.
+// (ZB + 2) / ZB + 2 / TY vector loads per node and right-hand side instead of 7
+// (flat tiles of nx < 64 TY nodes: 2 nx / (64 TY) halo loads per node).
 #pragma once
 
 #include "hz_stencil.hpp"
@@ -92,12 +93,15 @@ __global__ __launch_bounds__(64 * TY) void k_slab(OpView<float> op, TileMap tm, 
     __shared__ int s_act[APPLY_MAX_RHS];
     __shared__ float s_sc[APPLY_MAX_RHS];
     __shared__ int s_na;
-    // Rectangular tiles take the y-neighbours of interior rows from the
-    // block's own planes in LDS (double-buffered over passes, one barrier per
-    // pass); only the first and last tile rows load a halo row.  Separate
-    // global loads of the y-neighbour rows cost a third of the throughput.
-    constexpr bool YLDS = FLAT != 1;
-    __shared__ cx<float> s_pl[YLDS ? 2 : 1][YLDS ? RP : 1][YLDS ? TY : 1][YLDS ? ZB : 1][64];
+    // The y-neighbours come from the block's own planes in LDS (double-
+    // buffered over passes, one barrier per pass) wherever they lie in the
+    // tile: one row up or down (64 threads) in a rectangular tile, nx nodes
+    // back or ahead in a flat one.  Only the others are loaded: the halo
+    // rows of rectangular tiles, the first and last nx nodes of flat ones.
+    // Separate global loads of the y-neighbour rows cost a third of the
+    // throughput.
+    constexpr int NT_ = 64 * TY;
+    __shared__ cx<float> s_pl[2][RP][ZB][NT_];
     const int tile = xcd_tile(blockIdx.x, tm.nblocks);
     if (tile >= tm.ntiles) return;  // whole workgroup
     stage_live_columns<float>(nrhs, live, scale, s_act, s_sc, &s_na);
@@ -112,7 +116,6 @@ __global__ __launch_bounds__(64 * TY) void k_slab(OpView<float> op, TileMap tm, 
         tz = tile / (tm.tiles_x * tm.tiles_y);
     }
     const int lane = threadIdx.x & 63;
-    const int row = threadIdx.x >> 6;
     // Lanes past the grid edge keep running (they feed the DPP shifts) on a
     // clamped node and never store.  In a flat tile the previous/next lane is
     // x-1/x+1 of the same row except at a row end, where the coefficient to
@@ -160,6 +163,9 @@ __global__ __launch_bounds__(64 * TY) void k_slab(OpView<float> op, TileMap tm, 
     const bool has_xm = i > 0, has_xp = i < op.nx - 1;
     const int64_t oym = j > 0 ? -(int64_t)op.nx : 0;
     const int64_t oyp = j < op.ny - 1 ? (int64_t)op.nx : 0;
+    const int tid = (int)threadIdx.x;
+    const int ystep = FLAT == 1 ? op.nx : 64;  // LDS index distance of a y-step
+    const bool lds_ym = tid >= ystep, lds_yp = tid + ystep < NT_;
     const int na = s_na;
     int buf = 0;
     for (int t = 0; t < na; t += RP) {
@@ -187,8 +193,8 @@ __global__ __launch_bounds__(64 * TY) void k_slab(OpView<float> op, TileMap tm, 
                 const int64_t p = p0 + (int64_t)(kc - k0) * sxy;
                 ym[u][q] = mk<float>(0, 0);
                 yp[u][q] = mk<float>(0, 0);
-                if (!YLDS || row == 0) ym[u][q] = xr[p + oym];
-                if (!YLDS || row == TY - 1) yp[u][q] = xr[p + oyp];
+                if (!lds_ym) ym[u][q] = xr[p + oym];
+                if (!lds_yp) yp[u][q] = xr[p + oyp];
                 xe[u][q] = mk<float>(0, 0);
                 if (lane == 0 && has_xm) xe[u][q] = xr[p - 1];
                 if (lane == 63 && has_xp) xe[u][q] = xr[p + 1];
@@ -196,18 +202,18 @@ __global__ __launch_bounds__(64 * TY) void k_slab(OpView<float> op, TileMap tm, 
                 if (EPI == SLAB_SMOOTH || EPI == SLAB_RESID) bb[u][q] = b[rr[u] * ldy + p];
             }
         }
-        if (YLDS) {
+        {
 #pragma unroll
             for (int u = 0; u < RP; ++u)
 #pragma unroll
-                for (int q = 0; q < ZB; ++q) s_pl[buf][u][row][q][lane] = v[u][q + 1];
+                for (int q = 0; q < ZB; ++q) s_pl[buf][u][q][tid] = v[u][q + 1];
             __syncthreads();
 #pragma unroll
             for (int u = 0; u < RP; ++u)
 #pragma unroll
                 for (int q = 0; q < ZB; ++q) {
-                    if (row > 0) ym[u][q] = s_pl[buf][u][row - 1][q][lane];
-                    if (row < TY - 1) yp[u][q] = s_pl[buf][u][row + 1][q][lane];
+                    if (lds_ym) ym[u][q] = s_pl[buf][u][q][tid - ystep];
+                    if (lds_yp) yp[u][q] = s_pl[buf][u][q][tid + ystep];
                 }
             buf ^= 1;
         }
