@@ -36,6 +36,8 @@
  *                      PhaseResponseBasis.phase_vjp (phase_response_basis.py:208-253).
  *   hz_sample          trilinear rows of PhaseResponseBasis.target_response
  *                      (phase_response_basis.py:165-174).
+ *   hz_rasterize_materials  the sphere SDF, smoothing and material blend of
+ *                      WarpAssemblyEngine.assemble_system_gpu (warp_utils.py:44-79).
  *   hz_deposit_sources the aperture deposition of source_mass_matrix
  *                      (src/algorithm/physics/sinc_aperture.py:36-72).
  *   hz_z_transfer, hz_surface_*
@@ -264,6 +266,27 @@ hz_status hz_set_tile_shapes(hz_operator *op, const int32_t *shapes, int32_t len
 
 /* Device bytes held by the operator (coefficients plus cached workspace). */
 int64_t hz_device_bytes(const hz_operator *op);
+
+/* Scene rasterisation: materials of a union of obstacle primitives in a
+ * medium (warp_utils.py:11-23 blend, :52-77 sphere SDF and smoothing).  Per
+ * node (i dx, j dx, k dx): sdf = min over the primitives, h = smoothed
+ * Heaviside of sdf over `width` (m; 0 for a sharp step), and the blend of
+ * 1/rho and 1/(rho c^2) linear in h between the obstacle material and the
+ * medium (constants, or per-node device fields when rho_field / c_field
+ * are given).  With no primitive the medium is copied.  Outputs rho, c and
+ * sdf are device FP64 arrays of nx*ny*nz (Fortran order); each may be NULL.
+ * The reference rasterises one sphere (or a precomputed mesh SDF); the
+ * union, boxes and ellipsoids serve the benchmark scenes. */
+typedef struct hz_primitive {
+    int32_t kind;               /* 0 sphere, 1 box, 2 first-order ellipsoid  */
+    double center[3];           /* m                                         */
+    double size[3];             /* radius in size[0] / half extents / semi-axes (m) */
+} hz_primitive;
+
+hz_status hz_rasterize_materials(int32_t nx, int32_t ny, int32_t nz, double dx, int32_t nprim,
+                                 const hz_primitive *prims, double width, double rho_obstacle, double c_obstacle,
+                                 double rho_medium, double c_medium, const double *rho_field, const double *c_field,
+                                 double *rho, double *c, double *sdf, void *stream);
 
 /* Bilinear deposition of sinc apertures on the z = 0 plane
  * (sinc_aperture.py:36-72, the masses of WarpAssemblyEngine.source_columns):

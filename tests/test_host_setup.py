@@ -4,7 +4,7 @@ import json
 import numpy as np
 import pytest
 
-from oracle import helmholtz as ohz, source as osrc, materials as omat
+from oracle import helmholtz as ohz, source as osrc
 from ultrasound_solver_amd import physics
 
 
@@ -52,25 +52,6 @@ def test_centers_and_phases_match_archive(golden_dir):
     gp = physics.geometric_phases(cen, a["target_points"], 343/40e3, 2*np.pi*40e3/343)
     np.testing.assert_allclose(gp, osrc.geometric_phases(cen, a["target_points"], 343/40e3,
                                                          2*np.pi*40e3/343), atol=0)
-
-
-def test_sphere_materials_match_oracle():
-    shape, dx = (20, 18, 16), 1.07e-3
-    center, radius = (10*dx, 9*dx, 7*dx), 5*dx
-    rho, c = physics.obstacle_materials(physics.sphere_sdf(shape, dx, center, radius), dx, 1.2, 343.0)
-    r2, c2 = omat.sphere_materials(shape, dx, 1.2, 343.0, center, radius)
-    np.testing.assert_array_equal(rho, r2)
-    np.testing.assert_array_equal(c, c2)
-
-
-def test_box_and_ellipsoid_sdf_closed_forms():
-    dx = 1.0
-    b = physics.box_sdf((11, 11, 11), dx, (5, 5, 5), (2, 3, 4))
-    assert b[5, 5, 5] == -2 and b[8, 5, 5] == 1 and b[5, 5, 10] == 1
-    e = physics.ellipsoid_sdf((21, 21, 21), dx, (10, 10, 10), (5, 5, 5))
-    g = np.ogrid[:21, :21, :21]
-    sph = np.sqrt(sum((g[a] - 10.0)**2 for a in range(3))) - 5
-    np.testing.assert_allclose(e, sph, atol=1e-12)
 
 
 def test_surface_axis_matrices_match_the_oracle():

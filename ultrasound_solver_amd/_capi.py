@@ -20,7 +20,8 @@ EXPORTS = ("hz_create", "hz_destroy", "hz_apply", "hz_gmres", "hz_set_sources", 
            "hz_material_vjp", "hz_basis_field", "hz_basis_adjoint", "hz_basis_field_match",
            "hz_z_transfer", "hz_surface_create", "hz_surface_destroy", "hz_surface_tables",
            "hz_surface_set_sources", "hz_surface_transform", "hz_surface_solve_flux", "hz_surface_planes",
-           "hz_surface_sample", "hz_surface_target_response", "hz_deposit_sources")
+           "hz_surface_sample", "hz_surface_target_response", "hz_deposit_sources",
+           "hz_rasterize_materials")
 PRE_JACOBI, PRE_MULTIGRID = 0, 1
 
 
@@ -44,6 +45,10 @@ class LevelDesc(ctypes.Structure):
                 ("rho", POINTER(c_double)), ("kappa", POINTER(c_double)),
                 ("weight", POINTER(c_double)*3), ("stretch", POINTER(c_double)*3),
                 ("edge", POINTER(c_double)*3), ("face_term", POINTER(c_double)*6)]
+
+
+class Primitive(ctypes.Structure):
+    _fields_ = [("kind", c_int32), ("center", c_double*3), ("size", c_double*3)]
 
 
 class SurfaceDesc(ctypes.Structure):
@@ -107,6 +112,9 @@ def lib():
         handle.hz_material_vjp.argtypes = [c_void_p, c_void_p, c_void_p, c_int32, c_int64, c_void_p, c_void_p,
                                            c_void_p]
         handle.hz_set_tile_shapes.argtypes = [c_void_p, POINTER(c_int32), c_int32]
+        handle.hz_rasterize_materials.argtypes = [c_int32, c_int32, c_int32, c_double, c_int32, POINTER(Primitive),
+                                                  c_double, c_double, c_double, c_double, c_double, c_void_p,
+                                                  c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
         handle.hz_deposit_sources.argtypes = [c_int32, POINTER(c_double), c_int32, POINTER(c_double),
                                               POINTER(c_double), c_double, POINTER(c_int32), c_int32,
                                               POINTER(c_double)]

@@ -23,6 +23,7 @@
 #include "hz_krylov.hpp"
 #include "hz_mg.hpp"
 #include "hz_slab.hpp"
+#include "hz_materials.hpp"
 #include "hz_source.hpp"
 #include "hz_stencil.hpp"
 #include "hz_surface.hpp"
@@ -2111,6 +2112,48 @@ hz_status hz_deposit_sources(int32_t m, const double *centers, int32_t q, const 
                            doff.as<double>(), dw.as<double>(), db.as<int>(), (int)capacity, dm.as<double>());
         HZ_LAUNCHED();
         HZ_HIP(hipMemcpy(mass, dm.p, (size_t)m * capacity * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+}  // extern "C"
+
+// ---- Scene rasterisation ------------------------------------------------------
+
+extern "C" {
+
+hz_status hz_rasterize_materials(int32_t nx, int32_t ny, int32_t nz, double dx, int32_t nprim,
+                                 const hz_primitive *prims, double width, double rho_obstacle, double c_obstacle,
+                                 double rho_medium, double c_medium, const double *rho_field, const double *c_field,
+                                 double *rho, double *c, double *sdf, void *stream) {
+    return guarded([&] {
+        if (nx < 1 || ny < 1 || nz < 1) throw BadArgument("empty grid");
+        if (!(dx > 0) || !std::isfinite(dx) || !(width >= 0) || !std::isfinite(width))
+            throw BadArgument("dx must be positive and width non-negative");
+        if (nprim < 0 || nprim > MAX_PRIMITIVES) throw BadArgument("between 0 and 16 primitives");
+        if (nprim > 0 && !prims) throw BadArgument("null primitive list");
+        if (!rho && !c && !sdf) throw BadArgument("no output requested");
+        if (!(rho_obstacle > 0) || !(c_obstacle > 0) || (!rho_field && !(rho_medium > 0)) || (!c_field && !(c_medium > 0)))
+            throw BadArgument("densities and sound speeds must be positive");
+        MaterialArgs a{};
+        a.nx = nx, a.ny = ny, a.nz = nz, a.dx = dx, a.width = width, a.nprim = nprim;
+        for (int q = 0; q < nprim; ++q) {
+            const hz_primitive &p = prims[q];
+            if (p.kind < PRIM_SPHERE || p.kind > PRIM_ELLIPSOID) throw BadArgument("unknown primitive kind");
+            const int need = p.kind == PRIM_SPHERE ? 1 : 3;
+            for (int e = 0; e < 3; ++e) {
+                if (!std::isfinite(p.center[e])) throw BadArgument("primitive centre not finite");
+                if (e < need && !(p.size[e] > 0 && std::isfinite(p.size[e]))) throw BadArgument("primitive size must be positive");
+                a.prim[q].c[e] = p.center[e];
+                a.prim[q].s[e] = p.size[e];
+            }
+            a.prim[q].kind = p.kind;
+        }
+        a.rho_obs = rho_obstacle, a.c_obs = c_obstacle, a.rho_med = rho_medium, a.c_med = c_medium;
+        a.rho_field = rho_field, a.c_field = c_field, a.rho = rho, a.c = c, a.sdf = sdf;
+        const int64_t n = (int64_t)nx * ny * nz;
+        hipLaunchKernelGGL(k_materials, dim3((unsigned)blocks_for(n, 256, 65536)), dim3(256), 0,
+                           static_cast<hipStream_t>(stream), a);
+        HZ_LAUNCHED();
     });
 }
 

@@ -13,8 +13,8 @@ follow the reference so that the device operator reproduces its matrix A:
 * sinc aperture quadrature and bilinear unit-mass deposition, the masses
   computed on the device (src/algorithm/physics/sinc_aperture.py:19-72), flux scale
   -2 pi / (rho_ref h^3) (sinc_aperture.py:75-85);
-* material blend of 1/rho and 1/(rho c^2) across a smoothed SDF
-  (src/algorithm/physics/warp_utils.py:11-23, 44-79).
+* material blend of 1/rho and 1/(rho c^2) across a smoothed SDF, rasterised
+  on the device (src/algorithm/physics/warp_utils.py:11-23, 44-79).
 """
 import numpy as np
 
@@ -114,57 +114,64 @@ def soft_node_mask(shape, kinds):
 
 
 # ----------------------------------------------------------------- materials
-def blend_obstacle_material(h, rho_obstacle, c_obstacle, rho_medium, c_medium):
-    """1/rho and 1/(rho c^2) linear in h (0 inside the obstacle, 1 in the medium)."""
-    inv_rho = 1.0/rho_obstacle + (1.0/rho_medium - 1.0/rho_obstacle)*h
-    k_obs = 1.0/(rho_obstacle*c_obstacle**2)
-    kappa = k_obs + (1.0/(rho_medium*c_medium**2) - k_obs)*h
-    return 1.0/inv_rho, np.sqrt(inv_rho/kappa)
+PRIMITIVES = {"sphere": 0, "box": 1, "ellipsoid": 2}
 
 
-def smoothed_heaviside(sdf, width):
-    if width:
-        t = np.clip(sdf/width, -1.0, 1.0)
-        return 0.5*(1.0 + t + np.sin(np.pi*t)/np.pi)
-    return (sdf > 0).astype(float)
+def rasterize_materials(shape, dx, primitives, rho0, c0, rho_obstacle=1250.0, c_obstacle=2200.0,
+                        smoothing_cells=1.2, rho_medium=None, c_medium=None, want_sdf=False):
+    """Density and sound speed of a union of obstacles in a medium, on the
+    device (``hz_rasterize_materials``; warp_utils.py:11-23, 52-77).
 
+    primitives: ``(kind, centre, size)`` with kind "sphere" (size = radius),
+    "box" (half extents) or "ellipsoid" (semi-axes, first-order distance);
+    metres, node (i, j, k) at (i dx, j dx, k dx).  The SDF is the union's
+    (minimum); the blend variable is the smoothed Heaviside over
+    ``smoothing_cells * dx`` (a sharp step for 0), and 1/rho and 1/(rho c^2)
+    are linear in it between the obstacle material and the medium
+    (``rho0``/``c0``, or the per-node ``rho_medium``/``c_medium`` arrays).
+    Returns host (nx, ny, nz) arrays (rho, c), and the SDF when asked.
+    """
+    import ctypes
+    import torch
+    from . import _capi
+    shape = tuple(int(v) for v in shape)
+    if len(primitives) > 16:
+        raise ValueError("at most 16 obstacle primitives")
+    width = float(smoothing_cells)*dx
+    if not np.isfinite(width) or width < 0:
+        raise ValueError("smoothing_width_cells must be finite and >= 0")
+    prims = (_capi.Primitive*max(1, len(primitives)))()
+    for q, (kind, centre, size) in enumerate(primitives):
+        if kind not in PRIMITIVES:
+            raise ValueError(f"unknown obstacle primitive {kind!r}")
+        prims[q].kind = PRIMITIVES[kind]
+        size = np.broadcast_to(np.asarray(size, float), (3,)) if kind == "sphere" else np.asarray(size, float)
+        for e in range(3):
+            prims[q].center[e] = float(centre[e])
+            prims[q].size[e] = float(size[e])
+    dev = torch.device("cuda", torch.cuda.current_device())
+    fortran = shape[::-1]  # a C-contiguous (nz, ny, nx) tensor holds a Fortran-order grid
 
-def _coords(shape, dx):
-    return np.ogrid[:shape[0], :shape[1], :shape[2]]
-
-
-def sphere_sdf(shape, dx, center, radius):
-    g = _coords(shape, dx)
-    return np.sqrt(sum((g[a]*dx - center[a])**2 for a in range(3))) - radius
-
-
-def box_sdf(shape, dx, center, half_extent):
-    g = _coords(shape, dx)
-    q = [np.abs(g[a]*dx - center[a]) - half_extent[a] for a in range(3)]
-    outside = np.sqrt(sum(np.maximum(qa, 0.0)**2 for qa in q))
-    inside = np.minimum(np.maximum(np.maximum(q[0], q[1]), q[2]), 0.0)
-    return outside + inside
-
-
-def ellipsoid_sdf(shape, dx, center, semi_axes):
-    """First-order ellipsoid distance k0 (k0 - 1)/k1, k0 = |x/r|, k1 = |x/r^2|."""
-    g = _coords(shape, dx)
-    d = [g[a]*dx - center[a] for a in range(3)]
-    k0 = np.sqrt(sum((d[a]/semi_axes[a])**2 for a in range(3)))
-    k1 = np.sqrt(sum((d[a]/semi_axes[a]**2)**2 for a in range(3)))
-    inside_centre = k1 == 0
-    k1 = np.where(inside_centre, 1.0, k1)
-    return np.where(inside_centre, -float(min(semi_axes)), k0*(k0 - 1.0)/k1)
-
-
-def obstacle_materials(sdf, dx, rho0, c0, rho_obstacle=1250.0, c_obstacle=2200.0, smoothing_cells=1.2,
-                       rho_medium=None, c_medium=None):
-    """Blend a (union) SDF into density and sound-speed fields; the medium may
-    itself be a field (rho_medium, c_medium arrays)."""
-    h = smoothed_heaviside(sdf, smoothing_cells*dx)
-    rm = rho0 if rho_medium is None else rho_medium
-    cm = c0 if c_medium is None else c_medium
-    return blend_obstacle_material(h, rho_obstacle, c_obstacle, rm, cm)
+    def field(a):
+        if a is None or np.ndim(a) == 0:
+            return None
+        a = np.broadcast_to(np.asarray(a, float), shape)
+        return torch.as_tensor(np.ascontiguousarray(a.transpose(2, 1, 0)), device=dev)
+    fr, fc = field(rho_medium), field(c_medium)
+    if rho_medium is not None and np.ndim(rho_medium) == 0:
+        rho0 = float(rho_medium)
+    if c_medium is not None and np.ndim(c_medium) == 0:
+        c0 = float(c_medium)
+    rho = torch.empty(fortran, dtype=torch.float64, device=dev)
+    c = torch.empty_like(rho)
+    sdf = torch.empty_like(rho) if want_sdf else None
+    ptr = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None
+    _capi.check(_capi.lib().hz_rasterize_materials(
+        shape[0], shape[1], shape[2], float(dx), len(primitives), prims, width, float(rho_obstacle),
+        float(c_obstacle), float(rho0), float(c0), ptr(fr), ptr(fc), ptr(rho), ptr(c), ptr(sdf),
+        ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)), "hz_rasterize_materials")
+    out = [t.cpu().numpy().transpose(2, 1, 0) for t in (rho, c) + ((sdf,) if want_sdf else ())]
+    return tuple(out)
 
 
 # ------------------------------------------------------------- transducers

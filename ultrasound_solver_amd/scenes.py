@@ -100,8 +100,8 @@ def config_heterogeneous(n=128, array_n=16, seed=1, sigma=0.05, corr=5.0, sphere
     rho_med = np.full(shape, RHO0)
     centre = np.array([(n - 1)/2 + offset_cells[0], (n - 1)/2 + offset_cells[1],
                        (n - 1)/2 + offset_cells[2]])*H
-    sdf = physics.sphere_sdf(shape, H, centre, sphere_cells*H)
-    rho, c = physics.obstacle_materials(sdf, H, RHO0, C0, rho_medium=rho_med, c_medium=c_med)
+    rho, c = physics.rasterize_materials(shape, H, [("sphere", centre, sphere_cells*H)], RHO0, C0,
+                                         rho_medium=rho_med, c_medium=c_med)
     bnd = {"-x": "open", "+x": "reflecting", "-y": "open", "+y": "open", "-z": "reflecting", "+z": "open"}
     pml = {"cells": 8, "strength": 2.0, "order": 3}
     pitch, radius, s = fitted_array(n, array_n, 10.5e-3, 5.0e-3, pml["cells"])
@@ -118,8 +118,8 @@ def config_two_layer(n=256, array_n=16, semi_axes_cells=(40, 25, 15)):
     rho_med = np.where(k < n//2, 1.2, 1.0)*np.ones(shape)
     c_med = np.where(k < n//2, 343.0, 360.0)*np.ones(shape)
     centre = np.array([(n - 1)/2]*3)*H
-    sdf = physics.ellipsoid_sdf(shape, H, centre, np.array(semi_axes_cells)*H)
-    rho, c = physics.obstacle_materials(sdf, H, RHO0, C0, rho_medium=rho_med, c_medium=c_med)
+    rho, c = physics.rasterize_materials(shape, H, [("ellipsoid", centre, np.array(semi_axes_cells)*H)], RHO0, C0,
+                                         rho_medium=rho_med, c_medium=c_med)
     pml = {"cells": 8, "strength": 2.0, "order": 3}
     pitch, radius, s = fitted_array(n, array_n, 10.5e-3, 5.0e-3, pml["cells"])
     return Scene("two_layer_ellipsoid_%d" % n, rho, c, dict(OPEN5), pml, _array_centers(shape, array_n, pitch),
@@ -134,15 +134,15 @@ def config_uneven(index, seed=5, array_n=8):
     shape = (n, n, n)
     g = gaussian_random_field(shape, 5.0, seed*1000 + index)
     c_med = C0*(1 + 0.03*g)
-    sdf = np.full(shape, np.inf)
+    prims = []
     for _ in range(int(rng.integers(1, 4))):
         ctr = rng.uniform(0.3, 0.7, 3)*(n - 1)*H
         ctr[2] = rng.uniform(0.45, 0.8)*(n - 1)*H
         if rng.random() < 0.5:
-            sdf = np.minimum(sdf, physics.sphere_sdf(shape, H, ctr, rng.uniform(6, 16)*H))
+            prims.append(("sphere", ctr, rng.uniform(6, 16)*H))
         else:
-            sdf = np.minimum(sdf, physics.box_sdf(shape, H, ctr, rng.uniform(4, 12, 3)*H))
-    rho, c = physics.obstacle_materials(sdf, H, RHO0, C0, rho_medium=np.full(shape, RHO0), c_medium=c_med)
+            prims.append(("box", ctr, rng.uniform(4, 12, 3)*H))
+    rho, c = physics.rasterize_materials(shape, H, prims, RHO0, C0, rho_medium=RHO0, c_medium=c_med)
     z = [2.0*RHO0*C0, 0.5*RHO0*C0]
     bnd = {"-x": "open", "+x": "open", "-y": {"type": "impedance", "impedance": z},
            "+y": {"type": "impedance", "impedance": z}, "-z": "reflecting", "+z": "open"}

@@ -249,13 +249,13 @@ class HelmholtzSolver:
         if kind != "sphere":
             raise ValueError(f"Unsupported obstacle type for this backend: {kind!r} (use material arrays)")
         center = np.asarray(obstacle.get("center", [.04, .04, .02]), float) - self.origin
-        sdf = physics.sphere_sdf(shape, dx, center, float(obstacle.get("radius", .012)))
         width = float(obstacle.get("smoothing_width_cells", 1.2))
         if not np.isfinite(width) or width < 0:
             raise ValueError("smoothing_width_cells must be finite and >= 0")
         mat = obstacle.get("material", {})
-        return physics.obstacle_materials(sdf, dx, rho0, c0, float(mat.get("density", 1250.0)),
-                                          float(mat.get("sound_speed", 2200.0)), width)
+        return physics.rasterize_materials(shape, dx, [("sphere", center, float(obstacle.get("radius", .012)))],
+                                           rho0, c0, float(mat.get("density", 1250.0)),
+                                           float(mat.get("sound_speed", 2200.0)), width)
 
     def assemble_matrix(self, force=False):
         """Build the device operator and source columns; returns the RHS of the
