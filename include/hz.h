@@ -288,6 +288,14 @@ hz_status hz_rasterize_materials(int32_t nx, int32_t ny, int32_t nz, double dx, 
                                  double rho_medium, double c_medium, const double *rho_field, const double *c_field,
                                  double *rho, double *c, double *sdf, void *stream);
 
+/* In place: x (device FP64, nx*ny*nz, Fortran order) correlated along x,
+ * then y, then z with the 2*radius+1 symmetric weights (host array, centre
+ * weights[radius]), periodic boundaries: the Gaussian filter that turns white
+ * noise into the random medium of the benchmark scenes.  No counterpart in
+ * the reference, which takes media as given arrays. */
+hz_status hz_gaussian_filter(int32_t nx, int32_t ny, int32_t nz, int32_t radius, const double *weights, double *x,
+                             void *stream);
+
 /* Bilinear deposition of sinc apertures on the z = 0 plane
  * (sinc_aperture.py:36-72, the masses of WarpAssemblyEngine.source_columns):
  * emitter e has centre (centers[2e], centers[2e+1]) and the q quadrature

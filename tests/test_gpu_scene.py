@@ -58,3 +58,16 @@ def test_box_ellipsoid_and_union_closed_forms():
     _, _, u = physics.rasterize_materials((17, 18, 19), 1.0, prims, 1.2, 343.0, want_sdf=True)
     parts = [physics.rasterize_materials((17, 18, 19), 1.0, [p], 1.2, 343.0, want_sdf=True)[2] for p in prims]
     np.testing.assert_array_equal(u, np.minimum(np.minimum(parts[0], parts[1]), parts[2]))
+
+
+@pytest.mark.parametrize("shape,corr", [((24, 20, 18), 5.0), ((64, 64, 64), 5.0), ((9, 30, 7), 2.5)])
+def test_gaussian_random_field_matches_scipy(shape, corr):
+    """The device filter against scipy.ndimage.gaussian_filter (mode "wrap",
+    truncate 4) on the same seeded noise, including edges shorter than the
+    41-tap kernel (several periodic wraps)."""
+    from scipy.ndimage import gaussian_filter
+    from ultrasound_solver_amd import scenes
+    g = scenes.gaussian_random_field(shape, corr, 7)
+    ref = gaussian_filter(np.random.default_rng(7).standard_normal(shape), corr, mode="wrap")
+    ref = np.clip((ref - ref.mean())/ref.std(), -3.0, 3.0)
+    np.testing.assert_allclose(g, ref, rtol=0, atol=1e-13)

@@ -21,7 +21,7 @@ EXPORTS = ("hz_create", "hz_destroy", "hz_apply", "hz_gmres", "hz_set_sources", 
            "hz_z_transfer", "hz_surface_create", "hz_surface_destroy", "hz_surface_tables",
            "hz_surface_set_sources", "hz_surface_transform", "hz_surface_solve_flux", "hz_surface_planes",
            "hz_surface_sample", "hz_surface_target_response", "hz_deposit_sources",
-           "hz_rasterize_materials")
+           "hz_rasterize_materials", "hz_gaussian_filter")
 PRE_JACOBI, PRE_MULTIGRID = 0, 1
 
 
@@ -115,6 +115,8 @@ def lib():
         handle.hz_rasterize_materials.argtypes = [c_int32, c_int32, c_int32, c_double, c_int32, POINTER(Primitive),
                                                   c_double, c_double, c_double, c_double, c_double, c_void_p,
                                                   c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
+        handle.hz_gaussian_filter.argtypes = [c_int32, c_int32, c_int32, c_int32, POINTER(c_double), c_void_p,
+                                              c_void_p]
         handle.hz_deposit_sources.argtypes = [c_int32, POINTER(c_double), c_int32, POINTER(c_double),
                                               POINTER(c_double), c_double, POINTER(c_int32), c_int32,
                                               POINTER(c_double)]

@@ -2158,3 +2158,32 @@ hz_status hz_rasterize_materials(int32_t nx, int32_t ny, int32_t nz, double dx, 
 }
 
 }  // extern "C"
+
+extern "C" {
+
+hz_status hz_gaussian_filter(int32_t nx, int32_t ny, int32_t nz, int32_t radius, const double *weights, double *x,
+                             void *stream) {
+    return guarded([&] {
+        if (nx < 1 || ny < 1 || nz < 1 || radius < 0) throw BadArgument("empty grid or negative radius");
+        if (!weights || !x) throw BadArgument("null weights or grid");
+        for (int j = 0; j < radius; ++j)
+            if (weights[j] != weights[2 * radius - j]) throw BadArgument("the weights must be symmetric");
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        const int64_t n = (int64_t)nx * ny * nz;
+        DevBuf w, tmp;
+        upload(w, weights, (size_t)2 * radius + 1);
+        tmp.ensure((size_t)n * sizeof(double));
+        double *src = x, *dst = tmp.as<double>();
+        for (int axis = 0; axis < 3; ++axis) {
+            hipLaunchKernelGGL(k_wrap_filter, dim3((unsigned)blocks_for(n, 256, 65536)), dim3(256), 0, st, nx, ny, nz,
+                               axis, (int)radius, (const double *)w.as<double>(), (const double *)src, dst);
+            HZ_LAUNCHED();
+            std::swap(src, dst);
+        }
+        // Three passes leave the result in the scratch buffer.
+        HZ_HIP(hipMemcpyAsync(x, src, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        HZ_HIP(hipStreamSynchronize(st));  // the scratch buffer is freed on return
+    });
+}
+
+}  // extern "C"

@@ -88,4 +88,28 @@ __global__ __launch_bounds__(256) void k_materials(MaterialArgs a) {
     }
 }
 
+// One pass of a periodic ("wrap") correlation of a Fortran-order FP64 grid
+// along `axis` with 2r+1 symmetric weights w[0..2r] (centre w[r]):
+// out = w[r] x_i + sum_{j=r..1} w[r-j] (x_{i-j} + x_{i+j}), indices modulo n
+// (the Gaussian filter of a random medium, applied axis by axis).
+__global__ __launch_bounds__(256) void k_wrap_filter(int nx, int ny, int nz, int axis, int r,
+                                                     const double *__restrict__ w, const double *__restrict__ in,
+                                                     double *__restrict__ out) {
+#pragma clang fp contract(off)
+    const int64_t n = (int64_t)nx * ny * nz;
+    const int len = axis == 0 ? nx : (axis == 1 ? ny : nz);
+    const int64_t stride = axis == 0 ? 1 : (axis == 1 ? (int64_t)nx : (int64_t)nx * ny);
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)((axis == 0 ? p : (axis == 1 ? p / nx : p / ((int64_t)nx * ny))) % len);
+        const int64_t base = p - (int64_t)i * stride;
+        double acc = in[p] * w[r];
+        for (int j = r; j >= 1; --j) {
+            int lo = (i - j) % len, hi = (i + j) % len;
+            if (lo < 0) lo += len;
+            acc = acc + (in[base + lo * stride] + in[base + hi * stride]) * w[r - j];
+        }
+        out[p] = acc;
+    }
+}
+
 }  // namespace hz

@@ -53,10 +53,27 @@ class Scene:
         return len(self.centers)
 
 
-def gaussian_random_field(shape, corr_cells, seed, clip=3.0):
-    from scipy.ndimage import gaussian_filter
+def gaussian_random_field(shape, corr_cells, seed, clip=3.0, truncate=4.0):
+    """White noise (seeded, host) filtered on the device by a periodic
+    Gaussian of `corr_cells` standard deviation, truncated at `truncate`
+    of them (the weights and truncation of scipy.ndimage.gaussian_filter,
+    mode "wrap"), normalised to unit standard deviation and clipped."""
+    import ctypes
+    import torch
+    from . import _capi
     rng = np.random.default_rng(seed)
-    g = gaussian_filter(rng.standard_normal(shape), corr_cells, mode="wrap")
+    noise = rng.standard_normal(shape)
+    radius = int(truncate*float(corr_cells) + 0.5)
+    x = np.arange(-radius, radius + 1)
+    w = np.exp(-0.5/(corr_cells*corr_cells)*x**2)
+    w = np.ascontiguousarray(w/w.sum())
+    dev = torch.as_tensor(np.ascontiguousarray(noise.transpose(2, 1, 0)), device=torch.device("cuda"))
+    _capi.check(_capi.lib().hz_gaussian_filter(shape[0], shape[1], shape[2], radius,
+                                               w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                               ctypes.c_void_p(dev.data_ptr()),
+                                               ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)),
+                "hz_gaussian_filter")
+    g = dev.cpu().numpy().transpose(2, 1, 0)
     g = (g - g.mean())/g.std()
     return np.clip(g, -clip, clip)
 
