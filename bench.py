@@ -39,6 +39,8 @@ def parse():
     p.add_argument("--cpu-iters", type=int, default=60, help="scipy GMRES iterations in the CPU sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--stencil-reps", type=int, default=20)
+    p.add_argument("--stencil-rhs", type=int, default=0,
+                   help="columns per roofline stencil launch (0: the solve's chunk; stencil-only: 32)")
     p.add_argument("--stencil-shape", type=int, default=-1,
                    help="stencil-only: pin the fine apply's tile shape (index) after autotuning")
     p.add_argument("--stencil-only", action="store_true",
@@ -150,17 +152,19 @@ def main():
     plain_rate = plain_solved/plain_elapsed if plain_elapsed > 0 else 0.0
     log(f"plain: {plain_solved} solves in {plain_elapsed:.2f}s -> {plain_rate:.3f}/s; iterations {plain_iters}")
 
-    # Stencil roofline: R A M apply over `chunk` complex64 right-hand sides,
-    # timed with events on the stream the kernel is launched on.
-    k = STENCIL_RHS
+    # Stencil roofline: the Krylov apply of the timed solve (R A with the
+    # multigrid preconditioner, R A M with Jacobi) over the solve's chunk of
+    # complex64 columns, timed with events on the stream it is launched on.
+    k = args.stencil_rhs or args.chunk
+    mode = stencil_mode(args)
     x = torch.randn((k, op.n), dtype=torch.complex64, device="cuda:0")
     for _ in range(3):
-        op.apply(x, mode="RAM", precision="c64")
+        op.apply(x, mode=mode, precision="c64")
     stream = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     for _ in range(args.stencil_reps):
-        op.apply(x, mode="RAM", precision="c64")
+        op.apply(x, mode=mode, precision="c64")
     e1.record(stream)
     torch.cuda.synchronize()
     stencil_s = e0.elapsed_time(e1)/1e3/args.stencil_reps
@@ -186,7 +190,7 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(scene, args, plain_iters)
     zb, ty, rp, nt, flat = HelmholtzOperator.SLAB_SHAPES[tuning["shapes"][0]]
-    traffic = measured_traffic(args, zb, ty, rp, nt, flat)
+    traffic = measured_traffic(args, mode, k, zb, ty, rp, nt, flat)
 
     out = {
         "metric": "RHS solves/s at 128^3 to 1e-6 rel. residual (mixed vs FP64); stencil HBM GB/s",
@@ -204,8 +208,8 @@ def main():
         "refinements_median": float(np.median(refines)) if refines else None,
         "worst_scaled_residual": worst,
         "setup_s": setup_s,
-        "roofline": {"kernel": f"hz::k_slab<1, 0, {zb}, {ty}, {rp}, {nt}, {flat}> (complex64 R A M apply, "
-                               f"{k} columns)",
+        "roofline": {"kernel": f"hz::k_slab<{STENCIL_MODE_ID[mode]}, 0, {zb}, {ty}, {rp}, {nt}, {flat}> "
+                               f"(complex64 {' '.join(mode)} apply, {k} columns)",
                      "bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                      "frac": achieved/8000.0, "traffic": traffic, "alg_bytes_per_launch": alg_bytes,
                      "launch_us": stencil_s*1e6,
@@ -216,17 +220,23 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "stencil_128_k16_traffic.md")
-STENCIL_RHS = 16  # columns per stencil launch in the roofline measurement
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "stencil_128_traffic.md")
+STENCIL_MODE_ID = {"RAM": 1, "RA": 4}
 
 
-def measured_traffic(args, zb, ty, rp, nt, flat):
+def stencil_mode(args):
+    """The Krylov apply of the solve: R A under the multigrid preconditioner
+    (which is applied as P R^-1), R A M with Jacobi."""
+    return "RA" if args.precond == "multigrid" else "RAM"
+
+
+def measured_traffic(args, mode, k, zb, ty, rp, nt, flat):
     """HBM bytes per stencil launch from the FETCH_SIZE/WRITE_SIZE counter runs
-    of the same kernel, tile shape and workload (128^3, 16 columns), scaled by
-    the copy calibration recorded beside them; None for other workloads."""
+    of the same kernel, mode, column count and tile shape on the 128^3 scene,
+    scaled by the copy calibration recorded beside them; None otherwise."""
     if args.n != 128 or not os.path.exists(TRAFFIC_PROFILE):
         return None
-    key = f"traffic_bytes_per_launch[zb={zb},ty={ty},rp={rp},nt={nt},flat={flat}]:"
+    key = f"traffic_bytes_per_launch[mode={mode},k={k},zb={zb},ty={ty},rp={rp},nt={nt},flat={flat}]:"
     with open(TRAFFIC_PROFILE) as f:
         for line in f:
             if line.startswith(key):
@@ -235,29 +245,31 @@ def measured_traffic(args, zb, ty, rp, nt, flat):
 
 
 def stencil_only(op, args, m):
-    """Stencil launches (k complex64 RHS, R A M mode) and a copy of known size,
-    so that hardware byte counters can be scaled.  The copy moves 2 GiB each
+    """Launches of the solve's Krylov apply (k complex64 columns; R A with the
+    multigrid preconditioner) and a copy of known size, so that hardware byte
+    counters can be scaled.  The copy moves 2 GiB each
     way from a source evicted from the 256 MB last-level cache by a 1 GiB
     fill in between."""
     import torch
-    k = STENCIL_RHS
+    k = args.stencil_rhs or 32
+    mode = stencil_mode(args)
     if args.precond == "multigrid":
         op.set_multigrid(shift=args.mg_shift, pre=args.mg_pre, post=args.mg_post, post_coarse=args.mg_post_coarse,
                          weight=args.mg_weight)
-    tuning = op.autotune(m, max_chunk=16, restart=args.mixed_restart, preconditioner=args.precond)
+    tuning = op.autotune(m, max_chunk=k, restart=args.mixed_restart, preconditioner=args.precond)
     if args.stencil_shape >= 0:
         op.set_tile_shapes([args.stencil_shape] + tuning["shapes"][1:])
     print(json.dumps({"autotune": tuning, "stencil_shape": args.stencil_shape}), flush=True)
     x = torch.randn((k, op.n), dtype=torch.complex64, device="cuda:0")
     for _ in range(args.stencil_reps):
-        op.apply(x, mode="RAM", precision="c64")
+        op.apply(x, mode=mode, precision="c64")
     src = torch.ones(512*1024*1024, dtype=torch.float32, device="cuda:0")
     flush = torch.empty(256*1024*1024, dtype=torch.float32, device="cuda:0")
     flush.fill_(2.0)
     dst = torch.empty_like(src)
     dst.copy_(src)
     torch.cuda.synchronize()
-    print(json.dumps({"stencil_launches": args.stencil_reps, "rhs": k, "n": op.n,
+    print(json.dumps({"stencil_launches": args.stencil_reps, "mode": mode, "rhs": k, "n": op.n,
                       "alg_bytes_per_launch": (16*k + 8)*op.n, "copy_bytes_each_way": src.numel()*4}))
 
 
@@ -360,11 +372,11 @@ def bench_large(args):
     log(f"large: {ok}/{k} solves in {solve_s:.2f}s; median iterations {np.median([x['iterations'] for x in st])}")
     x = torch.randn((k, op.n), dtype=torch.complex64, device="cuda:0")
     for _ in range(3):
-        op.apply(x, mode="RAM", precision="c64")
+        op.apply(x, mode="RA", precision="c64")
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(args.stencil_reps):
-        op.apply(x, mode="RAM", precision="c64")
+        op.apply(x, mode="RA", precision="c64")
     e1.record()
     torch.cuda.synchronize()
     st_s = e0.elapsed_time(e1)/1e3/args.stencil_reps

@@ -81,7 +81,10 @@ typedef enum hz_apply_mode {
     HZ_APPLY_A = 0,             /* y = A x                                   */
     HZ_APPLY_RAM = 1,           /* y = R A M x  (row scale R, Jacobi M)      */
     HZ_APPLY_AH = 2,            /* y = A^H x                                 */
-    HZ_APPLY_RAM_H = 3          /* y = R' A^H M' x (R', M' built from A^H)   */
+    HZ_APPLY_RAM_H = 3,         /* y = R' A^H M' x (R', M' built from A^H)   */
+    HZ_APPLY_RA = 4,            /* y = R A x (the Krylov apply of the
+                                   multigrid-preconditioned mixed form)      */
+    HZ_APPLY_RA_H = 5           /* y = R' A^H x                              */
 } hz_apply_mode;
 
 /* Grid, materials and faces.  All pointers are host memory read during

@@ -12,7 +12,7 @@ LIBRARY = Path(__file__).resolve().parent/"libhz.so"
 HZ_OK, HZ_NOT_CONVERGED, HZ_BAD_ARGUMENT, HZ_OUT_OF_MEMORY, HZ_DEVICE_ERROR = range(5)
 FACE_KIND = {"reflecting": 0, "open": 1, "soft": 2, "impedance": 3}
 C128, C64 = 0, 1
-MODE_A, MODE_RAM, MODE_AH, MODE_RAMH = 0, 1, 2, 3
+MODE_A, MODE_RAM, MODE_AH, MODE_RAMH, MODE_RA, MODE_RAH = 0, 1, 2, 3, 4, 5
 
 EXPORTS = ("hz_create", "hz_destroy", "hz_apply", "hz_gmres", "hz_set_sources", "hz_build_rhs",
            "hz_gather_sources", "hz_sample", "hz_device_bytes", "hz_last_error", "hz_set_multigrid",

@@ -1243,7 +1243,7 @@ hz_status hz_apply(hz_operator *op, int32_t precision, int32_t mode, const void 
         check_op(op);
         if (!x || !y) throw BadArgument("x and y must be device pointers");
         if (nrhs < 1 || ld < op->n) throw BadArgument("need nrhs >= 1 and ld >= N");
-        if (mode < MODE_A || mode > MODE_RAMH) throw BadArgument("unknown apply mode");
+        if (mode < MODE_A || mode > MODE_RAH) throw BadArgument("unknown apply mode");
         hipStream_t s = static_cast<hipStream_t>(stream);
         if (precision == HZ_C128)
             apply_mode<double>(op, mode, static_cast<const cx<double> *>(x), ld, static_cast<cx<double> *>(y), ld,

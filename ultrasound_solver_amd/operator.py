@@ -15,7 +15,8 @@ import torch
 
 from . import _capi, physics
 
-MODES = {"A": _capi.MODE_A, "RAM": _capi.MODE_RAM, "AH": _capi.MODE_AH, "RAMH": _capi.MODE_RAMH}
+MODES = {"A": _capi.MODE_A, "RAM": _capi.MODE_RAM, "AH": _capi.MODE_AH, "RAMH": _capi.MODE_RAMH,
+         "RA": _capi.MODE_RA, "RAH": _capi.MODE_RAH}
 PRECISIONS = {"c128": _capi.C128, "c64": _capi.C64}
 
 
