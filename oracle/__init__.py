@@ -22,6 +22,11 @@ reference ``file:line`` it follows.  How each part is pinned:
   solution parity against scipy ``splu`` (complex128) and the reference's
   acceptance invariants.  The CuPy iteration trajectory itself is
   parity unpinned (CuPy is not available anywhere in this build).
+* ``surface``    -- homogeneous surface condensation (axis operators,
+  transfer ratios, Schur complement, modal transforms, target response):
+  planes equal the sparse direct solve of ``helmholtz``'s 3-D operator with
+  ``source``'s flux RHS; per-mode tridiagonal solves; linearity between the
+  target response and the sampled field.
 * ``losses``     -- field_match / focal_pressure / focal_contrast, the
   Gaussian target, background mask and trilinear target response: the
   archived run's stored loss values and target fields; the phase VJP by

@@ -55,7 +55,8 @@ def test_apply_c128_matches_reference_matrix(fix, name):
     minv_h = okr.jacobi_inverse(ah, scale_h)
     refs = {"A": a @ x, "AH": ah @ x,
             "RAM": scale[:, None]*(a @ (minv[:, None]*x)),
-            "RAMH": scale_h[:, None]*(ah @ (minv_h[:, None]*x))}
+            "RAMH": scale_h[:, None]*(ah @ (minv_h[:, None]*x)),
+            "RA": scale[:, None]*(a @ x), "RAH": scale_h[:, None]*(ah @ x)}
     for mode, ref in refs.items():
         got = op.apply(x, mode=mode).cpu().numpy().T
         assert _rel(got, ref) < 1e-13, mode
@@ -73,6 +74,8 @@ def test_apply_c64_matches_reference_matrix(fix, name):
     assert _rel(got, ref) < 2e-6
     got = op.apply(x, mode="A", precision="c64").cpu().numpy().T
     assert _rel(got, a @ x) < 2e-6
+    got = op.apply(x, mode="RA", precision="c64").cpu().numpy().T
+    assert _rel(got, scale[:, None]*(a @ x)) < 2e-6
 
 
 def _scene(n, seed=0, soft=False):
