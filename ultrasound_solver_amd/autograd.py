@@ -139,8 +139,3 @@ def focal_contrast_loss(u, indices, target_weight, sidelobe_weight, temperature,
     return loss + sidelobe_weight*smooth
 
 
-def trilinear_sample(u, rows, weights):
-    """Differentiable trilinear samples of a flat (N,) field: rows/weights (P, 8)."""
-    r = torch.as_tensor(rows, device=u.device)
-    w = torch.as_tensor(weights, device=u.device, dtype=torch.float64)
-    return (u[r]*w).sum(dim=1)

@@ -149,10 +149,6 @@ class PhaseResponseBasis:
             if np.any(dist < cells*cfg.domain.dx):
                 raise ValueError("Target lies in PML; enlarge the physical interior")
 
-    def _weights(self, phases, device=None):
-        ph = torch.as_tensor(np.asarray(phases, dtype=np.float64), device=device)
-        return torch.exp(1j*ph)
-
     def field(self, phases, return_numpy=True):
         self._require_complete()
         phases = np.asarray(phases) if not isinstance(phases, torch.Tensor) else phases
