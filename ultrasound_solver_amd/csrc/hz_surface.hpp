@@ -71,8 +71,7 @@ __global__ __launch_bounds__(256) void k_z_transfer(const cx<double> *__restrict
 //   B(k, j) = B[k*b_sk + j*b_sj],  C(i, j) = C[i + j*ldc] (divided by D(i, j)
 //   at the end when D is given).
 // Batch p uses A + p*a_sp, B + p*b_sp, C + p*c_sp and W + sel[p]*w_sp (no
-// weight for sel[p] < 0; W + p*w_sp without sel).  A 32 x 32 tile of C per
-// 256-thread workgroup, 2 x 2 per thread, K in steps of 16 through LDS.
+// weight for sel[p] < 0; W + p*w_sp without sel).
 struct ZgemmArgs {
     int m, n, k;
     const cx<double> *a;
@@ -87,7 +86,22 @@ struct ZgemmArgs {
     const cx<double> *d;
 };
 
-constexpr int ZG_TILE = 32, ZG_KS = 16;
+// A 64 x 64 tile of C per 256-thread workgroup; thread (tx, ty) = (t % 16,
+// t / 16) owns rows tx + 16u and columns ty + 16v (u, v < 4), so each K step
+// reads 8 complex values from LDS for 16 complex multiply-adds.  K advances
+// by 8; the next K tile is loaded into registers while the current one is
+// multiplied.  Global loads run along whichever operand index has unit
+// stride, so transposed operands stay coalesced.  gfx950's FP64 matrix rate
+// equals its FP64 vector rate, so vector FMAs are used.
+constexpr int ZG_TILE = 64, ZG_KS = 8;
+
+// acc += a b as four fused multiply-adds (two dependent ones per component).
+__device__ __forceinline__ void cmac(cx<double> &acc, cx<double> a, cx<double> b) {
+    acc.re = fma(a.re, b.re, acc.re);
+    acc.re = fma(-a.im, b.im, acc.re);
+    acc.im = fma(a.re, b.im, acc.im);
+    acc.im = fma(a.im, b.re, acc.im);
+}
 
 __global__ __launch_bounds__(256) void k_zgemm(ZgemmArgs g) {
     __shared__ cx<double> sa[ZG_KS][ZG_TILE + 1];
@@ -102,54 +116,72 @@ __global__ __launch_bounds__(256) void k_zgemm(ZgemmArgs g) {
         if (slot >= 0) W = g.w + slot * g.w_sp;
     }
     const int tid = threadIdx.x;
-    const int ti = (tid & 15) * 2, tj = (tid >> 4) * 2;
-    cx<double> acc[2][2];
+    const int tx = tid & 15, ty = tid >> 4;
+    const bool a_rows = g.a_si == 1;  // A loads along i (else along k)
+    const bool b_rows = g.b_sk == 1;  // B loads along k (else along j)
+    // Element e (0, 1) of this thread's share of a K tile: (row, k) of A and (k, col) of B.
+    int ai[2], ak[2], bk[2], bj[2];
 #pragma unroll
-    for (int u = 0; u < 2; ++u)
+    for (int e = 0; e < 2; ++e) {
+        const int idx = tid + e * 256;
+        if (a_rows) ai[e] = idx & 63, ak[e] = idx >> 6;
+        else ak[e] = idx & 7, ai[e] = idx >> 3;
+        if (b_rows) bk[e] = idx & 7, bj[e] = idx >> 3;
+        else bj[e] = idx & 63, bk[e] = idx >> 6;
+    }
+    auto load_a = [&](int k0, int e) {
+        const int gi = i0 + ai[e], gk = k0 + ak[e];
+        cx<double> v = mk(0.0, 0.0);
+        if (gi < g.m && gk < g.k) {
+            const int64_t off = gi * g.a_si + gk * g.a_sk;
+            v = A[off];
+            if (W) v = W[off] * v;
+        }
+        return v;
+    };
+    auto load_b = [&](int k0, int e) {
+        const int gk = k0 + bk[e], gj = j0 + bj[e];
+        return (gk < g.k && gj < g.n) ? B[gk * g.b_sk + gj * g.b_sj] : mk(0.0, 0.0);
+    };
+    cx<double> acc[4][4];
 #pragma unroll
-        for (int v = 0; v < 2; ++v) acc[u][v] = mk(0.0, 0.0);
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = mk(0.0, 0.0);
+    cx<double> ra[2], rb[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) ra[e] = load_a(0, e), rb[e] = load_b(0, e);
     for (int k0 = 0; k0 < g.k; k0 += ZG_KS) {
-        // 512 elements of each tile, two per thread.
+        __syncthreads();  // the previous tile has been read
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
-            const int idx = tid + e * 256;
-            {   // A tile: i fastest across threads
-                const int ii = idx & (ZG_TILE - 1), kk = idx / ZG_TILE;
-                const int gi = i0 + ii, gk = k0 + kk;
-                cx<double> v = mk(0.0, 0.0);
-                if (gi < g.m && gk < g.k) {
-                    const int64_t off = gi * g.a_si + gk * g.a_sk;
-                    v = A[off];
-                    if (W) v = W[off] * v;
-                }
-                sa[kk][ii] = v;
-            }
-            {   // B tile: k fastest across threads
-                const int kk = idx & (ZG_KS - 1), jj = idx / ZG_KS;
-                const int gk = k0 + kk, gj = j0 + jj;
-                cx<double> v = mk(0.0, 0.0);
-                if (gk < g.k && gj < g.n) v = B[gk * g.b_sk + gj * g.b_sj];
-                sb[kk][jj] = v;
-            }
+            sa[ak[e]][ai[e]] = ra[e];
+            sb[bk[e]][bj[e]] = rb[e];
         }
         __syncthreads();
+        if (k0 + ZG_KS < g.k) {
+#pragma unroll
+            for (int e = 0; e < 2; ++e) ra[e] = load_a(k0 + ZG_KS, e), rb[e] = load_b(k0 + ZG_KS, e);
+        }
 #pragma unroll
         for (int kk = 0; kk < ZG_KS; ++kk) {
-            const cx<double> a0 = sa[kk][ti], a1 = sa[kk][ti + 1];
-            const cx<double> b0 = sb[kk][tj], b1 = sb[kk][tj + 1];
-            acc[0][0] = acc[0][0] + a0 * b0;
-            acc[1][0] = acc[1][0] + a1 * b0;
-            acc[0][1] = acc[0][1] + a0 * b1;
-            acc[1][1] = acc[1][1] + a1 * b1;
+            cx<double> a[4], b[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) a[u] = sa[kk][tx + 16 * u];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) b[v] = sb[kk][ty + 16 * v];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) cmac(acc[u][v], a[u], b[v]);
         }
-        __syncthreads();
     }
     cx<double> *C = g.c + p * g.c_sp;
 #pragma unroll
-    for (int u = 0; u < 2; ++u)
+    for (int u = 0; u < 4; ++u)
 #pragma unroll
-        for (int v = 0; v < 2; ++v) {
-            const int gi = i0 + ti + u, gj = j0 + tj + v;
+        for (int v = 0; v < 4; ++v) {
+            const int gi = i0 + tx + 16 * u, gj = j0 + ty + 16 * v;
             if (gi < g.m && gj < g.n) {
                 const int64_t off = gi + gj * g.ldc;
                 cx<double> r = acc[u][v];
