@@ -49,6 +49,8 @@ def parse():
     p.add_argument("--mg-pre", type=int, default=1)
     p.add_argument("--mg-post", type=int, default=1, help="post-sweeps on the finest multigrid level")
     p.add_argument("--mg-post-coarse", type=int, default=4, help="post-sweeps on the coarser levels")
+    p.add_argument("--mg-coarse-sweeps", type=int, default=30, help="sweeps on the coarsest level")
+    p.add_argument("--mg-levels", type=int, default=0, help="multigrid levels (0: down to 5 nodes per edge)")
     p.add_argument("--mg-weight", type=float, default=0.7)
     p.add_argument("--mg-cycle", type=int, default=1, choices=[1, 2], help="1: V-cycle, 2: W below the finest level")
     p.add_argument("--graph", type=int, default=0, help="replay each GMRES cycle from a captured hipGraph")
@@ -98,7 +100,7 @@ def main():
         return
     if args.precond == "multigrid":
         op.set_multigrid(shift=args.mg_shift, pre=args.mg_pre, post=args.mg_post, post_coarse=args.mg_post_coarse,
-                         weight=args.mg_weight,
+                         coarse_sweeps=args.mg_coarse_sweeps, levels=args.mg_levels or None, weight=args.mg_weight,
                          cycle=args.mg_cycle)
     m = scene.num_sources
     tuning = op.autotune(m, max_chunk=args.chunk or 64, restart=args.mixed_restart, preconditioner=args.precond)

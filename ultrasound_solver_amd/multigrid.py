@@ -63,6 +63,11 @@ def _axis_tables(x):
     return w, e
 
 
+# Nodes of the largest level the library solves in LDS, one workgroup per
+# column (its coarsest-level solver); coarsening stops there by default.
+COARSE_MAX_NODES = 1024
+
+
 class Hierarchy:
     def __init__(self, op, levels=None, min_size=5):
         self.levels = []
@@ -85,7 +90,8 @@ class Hierarchy:
             self.levels.append(lev)
             if levels is not None and len(self.levels) == levels:
                 break
-            if min(shape) <= min_size or (levels is None and len(self.levels) >= 8):
+            if min(shape) <= min_size or (levels is None and (len(self.levels) >= 8
+                                                               or int(np.prod(shape)) <= COARSE_MAX_NODES)):
                 break
             mats = [prolongation_1d(n) for n in shape]
             idx = [coarse_nodes(n) for n in shape]
