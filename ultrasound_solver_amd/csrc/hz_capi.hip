@@ -250,8 +250,8 @@ template <int MODE, int EPI>
 void launch_slab(int shape, const OpView<float> &v, const cx<float> *x, int64_t ldx, cx<float> *y, int64_t ldy,
                  int k, const int *live, const double *scale, const cx<float> *b, const cx<float> *dinv, float w,
                  const float *bdiv, hipStream_t s) {
-    const TileMap tm = slab_map(v.nx, v.ny, v.nz, shape);
-    const dim3 grid(tm.nblocks), block(64 * SLAB_SHAPE[shape].ty);
+    const TileMap tm = slab_map(v.nx, v.ny, v.nz, shape, k);
+    const dim3 grid(tm.nblocks * tm.csplit), block(64 * SLAB_SHAPE[shape].ty);
 #define HZ_SLAB_CASE(ID, ZB, TY, RP, NT, FLAT)                                                                     \
     case ID:                                                                                                       \
         static_assert(SLAB_SHAPE[ID].zb == ZB && SLAB_SHAPE[ID].ty == TY && SLAB_SHAPE[ID].rp == RP &&             \

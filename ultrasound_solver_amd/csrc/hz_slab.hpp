@@ -14,6 +14,8 @@
 // (flat tiles of nx < 64 TY nodes: 2 nx / (64 TY) halo loads per node).
 #pragma once
 
+#include <algorithm>
+
 #include "hz_stencil.hpp"
 
 namespace hz {
@@ -35,7 +37,15 @@ constexpr SlabShape SLAB_SHAPE[SLAB_SHAPES] = {
     {2, 4, 2, 0, 0}, {4, 4, 1, 1, 0}, {2, 4, 2, 1, 0}, {4, 4, 1, 1, 1}, {2, 4, 1, 1, 1},
     {2, 4, 2, 1, 1}, {4, 4, 1, 1, 2}, {2, 4, 2, 1, 2}, {4, 8, 1, 1, 2}};
 
-__host__ inline TileMap slab_map(int nx, int ny, int nz, int shape) {
+// Workgroups a slab launch aims for.  Grids with fewer tiles (the coarse
+// multigrid levels, small scenes) split the right-hand sides into groups,
+// one workgroup per tile and group, instead of leaving most CUs idle while
+// each workgroup walks every column.
+#ifndef HZ_SLAB_TARGET_BLOCKS
+#define HZ_SLAB_TARGET_BLOCKS 1024
+#endif
+
+__host__ inline TileMap slab_map(int nx, int ny, int nz, int shape, int k) {
     const SlabShape sh = SLAB_SHAPE[shape];
     TileMap t;
     // Flat tiles: tiles_x = 1 and tiles_y counts 64*ty-node pieces of a plane.
@@ -46,6 +56,8 @@ __host__ inline TileMap slab_map(int nx, int ny, int nz, int shape) {
     t.tiles_z = (nz + sh.zb - 1) / sh.zb;
     t.ntiles = t.tiles_x * t.tiles_y * t.tiles_z;
     t.nblocks = (t.ntiles + 7) / 8 * 8;
+    const int passes = (k + sh.rp - 1) / sh.rp;
+    t.csplit = std::max(1, std::min(passes, (HZ_SLAB_TARGET_BLOCKS + t.ntiles - 1) / t.ntiles));
     return t;
 }
 
@@ -102,7 +114,10 @@ __global__ __launch_bounds__(64 * TY) void k_slab(OpView<float> op, TileMap tm, 
     // throughput.
     constexpr int NT_ = 64 * TY;
     __shared__ cx<float> s_pl[2][RP][ZB][NT_];
-    const int tile = xcd_tile(blockIdx.x, tm.nblocks);
+    // nblocks is a multiple of 8, so a workgroup's XCD (its index modulo 8)
+    // is the same in every column group.
+    const int part = (int)blockIdx.x / tm.nblocks;
+    const int tile = xcd_tile((int)blockIdx.x - part * tm.nblocks, tm.nblocks);
     if (tile >= tm.ntiles) return;  // whole workgroup
     stage_live_columns<float>(nrhs, live, scale, s_act, s_sc, &s_na);
     int tx, ty, tz;
@@ -166,19 +181,23 @@ __global__ __launch_bounds__(64 * TY) void k_slab(OpView<float> op, TileMap tm, 
     const int tid = (int)threadIdx.x;
     const int ystep = FLAT == 1 ? op.nx : 64;  // LDS index distance of a y-step
     const bool lds_ym = tid >= ystep, lds_yp = tid + ystep < NT_;
+    // This workgroup's live columns: group `part` of csplit groups of whole
+    // passes (the last groups may be short or empty).
     const int na = s_na;
+    const int per = ((na + tm.csplit - 1) / tm.csplit + RP - 1) / RP * RP;
+    const int t_begin = part * per, t_end = min(na, t_begin + per);
     int buf = 0;
-    for (int t = 0; t < na; t += RP) {
+    for (int t = t_begin; t < t_end; t += RP) {
         // RP right-hand sides per pass.  Every load of their columns is
         // issued before any arithmetic: planes k0-1 .. k0+ZB (absent planes
         // read a valid node and are zeroed), the halo rows, the x-edge
         // values of lanes 0 and 63, and b for the SMOOTH/RESID epilogues.  A
-        // pass past the last column repeats it and does not store.
+        // pass past the group's last column repeats it and does not store.
         int rr[RP];
         cx<float> v[RP][ZB + 2], ym[RP][ZB], yp[RP][ZB], xe[RP][ZB], bb[RP][ZB];
 #pragma unroll
         for (int u = 0; u < RP; ++u) {
-            rr[u] = s_act[min(t + u, na - 1)];
+            rr[u] = s_act[min(t + u, t_end - 1)];
             const cx<float> *xr = xin + rr[u] * ldx;
 #pragma unroll
             for (int q = 0; q < ZB + 2; ++q) {
@@ -219,7 +238,7 @@ __global__ __launch_bounds__(64 * TY) void k_slab(OpView<float> op, TileMap tm, 
         }
 #pragma unroll
         for (int u = 0; u < RP; ++u) {
-            if (u > 0 && t + u >= na) break;
+            if (u > 0 && t + u >= t_end) break;
             const int r = rr[u];
 #pragma unroll
             for (int q = 0; q < ZB; ++q) {

@@ -24,7 +24,8 @@ enum ApplyMode { MODE_A = 0, MODE_RAM = 1, MODE_AH = 2, MODE_RAMH = 3, MODE_RA =
 struct TileMap {
     int tiles_x, tiles_y, tiles_z, zb;
     int ntiles;
-    int nblocks;  // padded to a multiple of 8
+    int nblocks;     // padded to a multiple of 8
+    int csplit = 1;  // column groups: workgroup b works on group b / nblocks (slab kernel)
 };
 
 __host__ inline TileMap make_tilemap(int nx, int ny, int nz, int zb) {
