@@ -216,14 +216,25 @@ int plane_blocks(int nx, int ny, int nz, int k) {
     return (int)((t + 7) / 8 * 8);
 }
 
+// Slots per thread of the per-coarse-node transfers: as many as the grid
+// allows while it keeps >= 2048 workgroups.
+int transfer_group(const MgLevel &C, int k) {
+    const int per_slot = plane_blocks(C.nx, C.ny, C.nz, 1);
+    const int groups = std::max(1, std::min(k, (2048 + per_slot - 1) / per_slot));
+    return (k + groups - 1) / groups;
+}
+
 void launch_restrict(const MgLevel &L, const MgLevel &C, const cx<float> *r, int k, const int *live, hipStream_t s) {
-    const int nb = plane_blocks(C.nx, C.ny, C.nz, k);
+    // At most 4 slots per thread: 19.9 solves/s on the mixed 128^3 workload
+    // against 19.6 with 16 and 19.1 with 1.
+    const int rs = std::min(4, transfer_group(C, k));
+    const int nb = plane_blocks(C.nx, C.ny, C.nz, (k + rs - 1) / rs);
     if (L.nx % 2 == 0)
         hipLaunchKernelGGL((k_restrict<float, 1>), dim3(nb), dim3(256), 0, s, r, L.nx, L.ny, L.nz,
-                           C.bb.as<cx<float>>(), C.nx, C.ny, C.nz, k, live, L.v.soft, nb);
+                           C.bb.as<cx<float>>(), C.nx, C.ny, C.nz, k, rs, live, L.v.soft, nb);
     else
         hipLaunchKernelGGL((k_restrict<float, 0>), dim3(nb), dim3(256), 0, s, r, L.nx, L.ny, L.nz,
-                           C.bb.as<cx<float>>(), C.nx, C.ny, C.nz, k, live, L.v.soft, nb);
+                           C.bb.as<cx<float>>(), C.nx, C.ny, C.nz, k, rs, live, L.v.soft, nb);
     HZ_LAUNCHED();
 }
 
@@ -232,9 +243,7 @@ void launch_prolong(const MgLevel &L, const MgLevel &C, cx<float> *xf, int k, co
     // One thread per coarse cell and group of rs slots: the slots of a group
     // share the cell's weights and first-sweep factors.  Groups are as large
     // as the grid allows while it still has >= 2048 workgroups.
-    const int per_slot = plane_blocks(C.nx, C.ny, C.nz, 1);
-    const int groups = std::max(1, std::min(k, (2048 + per_slot - 1) / per_slot));
-    const int rs = (k + groups - 1) / groups;
+    const int rs = transfer_group(C, k);
     const int nb = plane_blocks(C.nx, C.ny, C.nz, (k + rs - 1) / rs);
     if (L.nx % 2 == 0)
         hipLaunchKernelGGL((k_prolong<float, 1>), dim3(nb), dim3(256), 0, s, (const cx<float> *)C.xx.as<cx<float>>(),

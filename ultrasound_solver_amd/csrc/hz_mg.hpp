@@ -141,9 +141,10 @@ __device__ __forceinline__ PlaneTile plane_tile(int nx, int ny, int nz, int nslo
     return t;
 }
 
-// Coarse right-hand side bc = P^T rf, one thread per (coarse node, slot);
-// grid: flattened plane tiles of the coarse grid times k slots, padded to a
-// multiple of 8 (nblocks).  Along x the taps are fine 2i-1, 2i, 2i+1 with weights 1/2, 1,
+// Coarse right-hand side bc = P^T rf, one thread per (coarse node, group of
+// rs slots): the node's taps and weights are computed once for the group.
+// Grid: flattened plane tiles of the coarse grid times ceil(k / rs) slot
+// groups, padded to a multiple of 8 (nblocks).  Along x the taps are fine 2i-1, 2i, 2i+1 with weights 1/2, 1,
 // 1/2 (0 where absent or a coarse node); along y and z likewise.  PAIR (even
 // fine x-size): each lane loads its (2i, 2i+1) pair with one 16-byte access
 // and takes 2i-1 from the previous lane (DPP), lane 0 loading it itself:
@@ -154,13 +155,14 @@ __device__ __forceinline__ PlaneTile plane_tile(int nx, int ny, int nz, int nslo
 // residual-scaled value never lands in a solution-scaled identity row.
 template <typename T, int PAIR>
 __global__ __launch_bounds__(256) void k_restrict(const cx<T> *__restrict__ rf, int nxf, int nyf, int nzf,
-                                                  cx<T> *__restrict__ bc, int nxc, int nyc, int nzc, int k,
+                                                  cx<T> *__restrict__ bc, int nxc, int nyc, int nzc, int k, int rs,
                                                   const int *__restrict__ live, int soft, int nblocks) {
-    const PlaneTile t = plane_tile(nxc, nyc, nzc, k, nblocks);
-    // Slot and plane are uniform over the workgroup.
-    if (t.r >= k || (live && !live[t.r])) return;
+    const PlaneTile t = plane_tile(nxc, nyc, nzc, (k + rs - 1) / rs, nblocks);
+    // Slot group and plane are uniform over the workgroup.
+    if (t.r * rs >= k) return;
     const bool valid = t.in_grid;
-    const int ic = t.i, jc = t.j, kc = t.k, r = t.r;
+    const int ic = t.i, jc = t.j, kc = t.k;
+    const int r0 = t.r * rs, r1 = min(k, r0 + rs);
     const int lane = threadIdx.x & 63;
     const int64_t nc = (int64_t)nxc * nyc * nzc;
     const int64_t nf = (int64_t)nxf * nyf * nzf;
@@ -182,39 +184,49 @@ __global__ __launch_bounds__(256) void k_restrict(const cx<T> *__restrict__ rf, 
         oy[d] = min(max(fj + d - 1, 0), nyf - 1);
         oz[d] = min(max(fk + d - 1, 0), nzf - 1);
     }
-    const cx<T> *src = rf + r * nf;
-    cx<T> acc = mk<T>(0, 0);
+    T wt[3][3][3];  // tap weights, zero on fine identity rows
 #pragma unroll
     for (int dz = 0; dz < 3; ++dz)
 #pragma unroll
-        for (int dy = 0; dy < 3; ++dy) {
-            const int64_t row = (int64_t)nxf * (oy[dy] + (int64_t)nyf * oz[dz]);
-            cx<T> v[3];
-            if constexpr (PAIR) {
-                static_assert(sizeof(T) == 4, "paired loads are for complex64");
-                // Pair (2i, 2i+1); the last coarse node of an even grid is fine
-                // node nxf-1, the second element of the pair (nxf-2, nxf-1).
-                const int pidx = min(2 * ic, nxf - 2);
-                const float4 pr = *reinterpret_cast<const float4 *>(src + row + pidx);
-                const bool first = 2 * ic <= nxf - 2;
-                v[1] = first ? mk<float>(pr.x, pr.y) : mk<float>(pr.z, pr.w);
-                v[2] = first ? mk<float>(pr.z, pr.w) : mk<float>(0, 0);
-                // The previous lane holds x-1 of the same row except in lane 0
-                // and at a row start (where the left tap has weight 0).
-                v[0] = mk<float>(lane_prev(pr.z), lane_prev(pr.w));
-                if (lane == 0 || ic == 0) v[0] = src[row + ox[0]];
-            } else {
-#pragma unroll
-                for (int dx = 0; dx < 3; ++dx) v[dx] = src[row + ox[dx]];
-            }
+        for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
             for (int dx = 0; dx < 3; ++dx) {
                 T wv = wz[dz] * wy[dy] * wx[dx];
                 if (soft && on_soft_face(ox[dx], oy[dy], oz[dz], nxf, nyf, nzf, soft)) wv = T(0);
-                acc = acc + wv * v[dx];
+                wt[dz][dy][dx] = wv;
             }
-        }
-    if (valid) bc[r * nc + q] = fixed ? mk<T>(0, 0) : acc;
+    for (int r = r0; r < r1; ++r) {
+        if (live && !live[r]) continue;  // uniform over the workgroup
+        const cx<T> *src = rf + r * nf;
+        cx<T> acc = mk<T>(0, 0);
+#pragma unroll
+        for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy) {
+                const int64_t row = (int64_t)nxf * (oy[dy] + (int64_t)nyf * oz[dz]);
+                cx<T> v[3];
+                if constexpr (PAIR) {
+                    static_assert(sizeof(T) == 4, "paired loads are for complex64");
+                    // Pair (2i, 2i+1); the last coarse node of an even grid is fine
+                    // node nxf-1, the second element of the pair (nxf-2, nxf-1).
+                    const int pidx = min(2 * ic, nxf - 2);
+                    const float4 pr = *reinterpret_cast<const float4 *>(src + row + pidx);
+                    const bool first = 2 * ic <= nxf - 2;
+                    v[1] = first ? mk<float>(pr.x, pr.y) : mk<float>(pr.z, pr.w);
+                    v[2] = first ? mk<float>(pr.z, pr.w) : mk<float>(0, 0);
+                    // The previous lane holds x-1 of the same row except in lane 0
+                    // and at a row start (where the left tap has weight 0).
+                    v[0] = mk<float>(lane_prev(pr.z), lane_prev(pr.w));
+                    if (lane == 0 || ic == 0) v[0] = src[row + ox[0]];
+                } else {
+#pragma unroll
+                    for (int dx = 0; dx < 3; ++dx) v[dx] = src[row + ox[dx]];
+                }
+#pragma unroll
+                for (int dx = 0; dx < 3; ++dx) acc = acc + wt[dz][dy][dx] * v[dx];
+            }
+        if (valid) bc[r * nc + q] = fixed ? mk<T>(0, 0) : acc;
+    }
 }
 
 // Fine nodes of coarse cell c along one axis (fine size n, coarse size m):
