@@ -40,7 +40,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--stencil-reps", type=int, default=20)
     p.add_argument("--stencil-rhs", type=int, default=0,
-                   help="columns per roofline stencil launch (0: the solve's chunk; stencil-only: 32)")
+                   help="columns per roofline stencil launch (0: 32, whatever chunk the solve uses)")
     p.add_argument("--stencil-shape", type=int, default=-1,
                    help="stencil-only: pin the fine apply's tile shape (index) after autotuning")
     p.add_argument("--stencil-only", action="store_true",
@@ -155,9 +155,11 @@ def main():
     log(f"plain: {plain_solved} solves in {plain_elapsed:.2f}s -> {plain_rate:.3f}/s; iterations {plain_iters}")
 
     # Stencil roofline: the Krylov apply of the timed solve (R A with the
-    # multigrid preconditioner, R A M with Jacobi) over the solve's chunk of
-    # complex64 columns, timed with events on the stream it is launched on.
-    k = args.stencil_rhs or args.chunk
+    # multigrid preconditioner, R A M with Jacobi) with the solve's tile shape,
+    # over 32 complex64 columns (a fixed width, so that the figure does not
+    # move with the chunk the autotuner picks), timed with events on the
+    # stream it is launched on.
+    k = args.stencil_rhs or 32
     mode = stencil_mode(args)
     x = torch.randn((k, op.n), dtype=torch.complex64, device="cuda:0")
     for _ in range(3):
