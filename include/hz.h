@@ -217,7 +217,7 @@ hz_status hz_set_multigrid(hz_operator *op, int32_t nlevels, const hz_level_desc
  * most max_chunk and nrhs, workspace within memory_budget bytes, or half
  * of the free device memory when memory_budget <= 0) with the least time
  * per column of one Arnoldi step at j = restart / 2, preconditioner
- * included; a wider chunk must be 2 % faster per column to win.  Writes the
+ * included; of the widths within 2 % of that least time, the widest.  Writes the
  * width to *chunk_out and, when shapes_out is not null, the shape indices
  * (apply, then smoother / residual / first-sweep residual of each level) to
  * shapes_out[0 .. shapes_len).  The workspace used for timing is released. */

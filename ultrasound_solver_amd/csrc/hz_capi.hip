@@ -1646,21 +1646,24 @@ hz_status hz_autotune(hz_operator *op, int32_t nrhs, int32_t max_chunk, int32_t 
             }
         }
         // Chunk width: least time per column of one Arnoldi step at j = restart / 2.
-        int best = cand.front();
-        double best_per = 0;
+        // Widths within 2 % of the least go to the widest of them: the step
+        // timing does not see the per-cycle costs (cycle tops, refinement
+        // residuals, host round trips) that a wider chunk spreads over more
+        // columns.  The memory budget already bounds the candidates.
+        std::vector<double> per_col(cand.size());
+        double least = 0;
         for (size_t ci = 0; ci < cand.size(); ++ci) {
             const int c = cand[ci];
             const float ms = time_launches([&] { arnoldi_step_for_timing(op, c, restart / 2, restart, use_mg, s); },
                                            3, s);
-            const double per = ms / c;
+            per_col[ci] = ms / c;
             if (verbose()) std::fprintf(stderr, "[hz] autotune chunk %d: %.3f ms/step, %.1f us per column\n", c, ms,
-                                        per * 1e3);
-            // A wider chunk must win by more than 2 % (it costs memory).
-            if (ci == 0 || per < 0.98 * best_per) {
-                best = c;
-                best_per = per;
-            }
+                                        per_col[ci] * 1e3);
+            if (ci == 0 || per_col[ci] < least) least = per_col[ci];
         }
+        int best = cand.front();
+        for (size_t ci = 0; ci < cand.size(); ++ci)
+            if (per_col[ci] <= 1.02 * least) best = cand[ci];
         HZ_HIP(hipStreamSynchronize(s));
         release_workspace(op);
         if (chunk_out) *chunk_out = best;
