@@ -41,9 +41,7 @@ constexpr SlabShape SLAB_SHAPE[SLAB_SHAPES] = {
 // multigrid levels, small scenes) split the right-hand sides into groups,
 // one workgroup per tile and group, instead of leaving most CUs idle while
 // each workgroup walks every column.
-#ifndef HZ_SLAB_TARGET_BLOCKS
-#define HZ_SLAB_TARGET_BLOCKS 1024
-#endif
+constexpr int SLAB_TARGET_BLOCKS = 1024;
 
 __host__ inline TileMap slab_map(int nx, int ny, int nz, int shape, int k) {
     const SlabShape sh = SLAB_SHAPE[shape];
@@ -57,7 +55,7 @@ __host__ inline TileMap slab_map(int nx, int ny, int nz, int shape, int k) {
     t.ntiles = t.tiles_x * t.tiles_y * t.tiles_z;
     t.nblocks = (t.ntiles + 7) / 8 * 8;
     const int passes = (k + sh.rp - 1) / sh.rp;
-    t.csplit = std::max(1, std::min(passes, (HZ_SLAB_TARGET_BLOCKS + t.ntiles - 1) / t.ntiles));
+    t.csplit = std::max(1, std::min(passes, (SLAB_TARGET_BLOCKS + t.ntiles - 1) / t.ntiles));
     return t;
 }
 
