@@ -212,7 +212,7 @@ def main():
         "refinements_median": float(np.median(refines)) if refines else None,
         "worst_scaled_residual": worst,
         "setup_s": setup_s,
-        "roofline": {"kernel": f"hz::k_slab<{STENCIL_MODE_ID[mode]}, 0, {zb}, {ty}, {rp}, {nt}, {flat}> "
+        "roofline": {"kernel": f"{slab_kernel_name(mode, zb, ty, rp, nt, flat)} "
                                f"(complex64 {' '.join(mode)} apply, {k} columns)",
                      "bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                      "frac": achieved/8000.0, "traffic": traffic, "alg_bytes_per_launch": alg_bytes,
@@ -226,6 +226,14 @@ def main():
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "stencil_128_traffic.md")
 STENCIL_MODE_ID = {"RAM": 1, "RA": 4}
+
+
+def slab_kernel_name(mode, zb, ty, rp, nt, flat):
+    """The slab-stencil kernel a tile shape launches (flat = 4: the paired
+    kernel, template <MODE, EPI, ZB, TY, NT>)."""
+    if flat == 4:
+        return f"hz::k_slab2<{STENCIL_MODE_ID[mode]}, 0, {zb}, {ty}, {nt}>"
+    return f"hz::k_slab<{STENCIL_MODE_ID[mode]}, 0, {zb}, {ty}, {rp}, {nt}, {flat}>"
 
 
 def stencil_mode(args):

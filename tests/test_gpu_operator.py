@@ -269,6 +269,39 @@ def test_tile_shapes_are_bitwise_identical_and_autotune_picks_one():
     assert torch.equal(sol, ref_x)
 
 
+def test_tile_shapes_bitwise_identical_on_a_wide_even_grid():
+    """An even width above two 128-node tile rows: the paired tiles (two nodes
+    per lane, neighbours across tile edges loaded by lanes 0 and 63, a partial
+    last tile) against every other shape, for all six apply modes and a
+    multigrid-preconditioned solve (whose finest level is even, the coarser
+    ones odd)."""
+    from ultrasound_solver_amd.operator import HelmholtzOperator
+    rng = np.random.default_rng(11)
+    shape3 = (260, 9, 8)
+    rho = 1.2*(1 + 0.1*rng.random(shape3))
+    c = 343.0*(1 + 0.03*rng.standard_normal(shape3))
+    bnd = {"-x": "open", "+x": "soft", "-y": "open", "+y": "reflecting", "-z": "reflecting", "+z": "open"}
+    op = HelmholtzOperator(rho, c, 1.07e-3, 2*np.pi*40e3, bnd, {"cells": 3}, device="cuda:0")
+    op.set_multigrid()
+    nshape = 1 + 3*len(op.hierarchy.levels)
+    x = torch.as_tensor(_rand(op.n, 3, 5).T.copy(), device="cuda:0").to(torch.complex64)
+    b = _point_rhs(op.n, 2, 9)
+    modes = ("A", "RAM", "AH", "RAMH", "RA", "RAH")
+    ref = None
+    for shape in range(len(HelmholtzOperator.SLAB_SHAPES)):
+        op.set_tile_shapes([shape]*nshape)
+        ys = [op.apply(x, mode=m, precision="c64") for m in modes]
+        sol, st = op.gmres(b, rtol=1e-6, precision="c64", chunk=2, maxiter=3000, preconditioner="multigrid")
+        its = [s["iterations"] for s in st]
+        if ref is None:
+            ref = (ys, sol, its)
+            continue
+        for m, y, y0 in zip(modes, ys, ref[0]):
+            assert torch.equal(y, y0), f"shape {shape}, mode {m}: apply differs"
+        assert its == ref[2], (shape, its, ref[2])
+        assert torch.equal(sol, ref[1]), f"shape {shape}: solution differs"
+
+
 @pytest.mark.parametrize("precision,pre", [("c128", "jacobi"), ("c64", "multigrid")])
 def test_graph_replay_matches_eager(precision, pre):
     """A captured cycle body replayed once per restart cycle (eager cycle-top

@@ -259,6 +259,12 @@ template <int MODE, int EPI>
 void launch_slab(int shape, const OpView<float> &v, const cx<float> *x, int64_t ldx, cx<float> *y, int64_t ldy,
                  int k, const int *live, const double *scale, const cx<float> *b, const cx<float> *dinv, float w,
                  const float *bdiv, hipStream_t s) {
+    // Paired tiles move two nodes per 16-byte access: even widths and
+    // 16-byte aligned columns only.
+    auto a16 = [](const void *ptr) { return ((uintptr_t)ptr & 15) == 0; };
+    if (SLAB_SHAPE[shape].flat == 4 &&
+        !(v.nx % 2 == 0 && ldx % 2 == 0 && ldy % 2 == 0 && a16(x) && a16(y) && a16(b)))
+        shape = PAIR_FALLBACK;
     const TileMap tm = slab_map(v.nx, v.ny, v.nz, shape, k);
     const dim3 grid(tm.nblocks * tm.csplit), block(64 * SLAB_SHAPE[shape].ty);
 #define HZ_SLAB_CASE(ID, ZB, TY, RP, NT, FLAT)                                                                     \
@@ -284,6 +290,17 @@ void launch_slab(int shape, const OpView<float> &v, const cx<float> *x, int64_t 
         HZ_SLAB_CASE(11, 4, 4, 1, 1, 2)
         HZ_SLAB_CASE(12, 2, 4, 2, 1, 2)
         HZ_SLAB_CASE(13, 4, 8, 1, 1, 2)
+#define HZ_SLAB2_CASE(ID, ZB, TY, NT)                                                                              \
+    case ID:                                                                                                       \
+        static_assert(SLAB_SHAPE[ID].zb == ZB && SLAB_SHAPE[ID].ty == TY && SLAB_SHAPE[ID].rp == 1 &&              \
+                          SLAB_SHAPE[ID].nt == NT && SLAB_SHAPE[ID].flat == 4,                                     \
+                      "slab shape table");                                                                         \
+        hipLaunchKernelGGL((k_slab2<MODE, EPI, ZB, TY, NT>), grid, block, 0, s, v, tm, x, ldx, y, ldy, k, live,      \
+                           scale, b, dinv, w, bdiv);                                                               \
+        break;
+        HZ_SLAB2_CASE(14, 2, 4, 1)
+        HZ_SLAB2_CASE(15, 2, 8, 1)
+#undef HZ_SLAB2_CASE
         default:
             throw BadArgument("unknown slab tile shape");
     }

@@ -69,6 +69,9 @@ struct OpView {
 
 constexpr int WAVE = 64;
 
+// Two complex64 values moved with one 16-byte access.
+typedef float hz_f4 __attribute__((ext_vector_type(4)));
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll

@@ -261,7 +261,6 @@ __device__ __forceinline__ CellAxis cell_axis(int c, int n, int m) {
 // rows get none.  PAIR (even fine x-size, complex64): the two fine nodes of a
 // cell row, 2i and 2i+1, are read (b, or the iterate) and written with one
 // 16-byte access each.
-typedef float hz_f4 __attribute__((ext_vector_type(4)));
 template <typename T, int PAIR>
 __global__ __launch_bounds__(256) void k_prolong(const cx<T> *__restrict__ xc, int nxc, int nyc, int nzc,
                                                  cx<T> *__restrict__ xf, int nxf, int nyf, int nzf, int k, int rs,

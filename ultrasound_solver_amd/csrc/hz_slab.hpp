@@ -28,14 +28,19 @@ namespace hz {
 // flat = 2: z-fastest tile order (rectangular tiles): the workgroups resident
 // together on one XCD hold whole z-columns of slabs, so each slab's z-halo
 // planes are the neighbouring slabs' own planes, loaded at about the same time.
+// flat = 4: paired tiles (k_slab2): each lane owns two x-adjacent nodes of a
+// 128 x TY tile and moves them with one 16-byte access per plane and row.
+// Even widths and 16-byte aligned vectors only; launch_slab substitutes
+// PAIR_FALLBACK elsewhere.
 struct SlabShape {
     int zb, ty, rp, nt, flat;
 };
-constexpr int SLAB_SHAPES = 14;
+constexpr int SLAB_SHAPES = 16;
 constexpr SlabShape SLAB_SHAPE[SLAB_SHAPES] = {
-    {4, 4, 1, 0, 0}, {2, 4, 1, 0, 0}, {4, 8, 1, 0, 0}, {2, 8, 1, 0, 0}, {4, 2, 1, 0, 0},
-    {2, 4, 2, 0, 0}, {4, 4, 1, 1, 0}, {2, 4, 2, 1, 0}, {4, 4, 1, 1, 1}, {2, 4, 1, 1, 1},
-    {2, 4, 2, 1, 1}, {4, 4, 1, 1, 2}, {2, 4, 2, 1, 2}, {4, 8, 1, 1, 2}};
+    {4, 4, 1, 0, 0}, {2, 4, 1, 0, 0}, {4, 8, 1, 0, 0}, {2, 8, 1, 0, 0}, {4, 2, 1, 0, 0}, {2, 4, 2, 0, 0},
+    {4, 4, 1, 1, 0}, {2, 4, 2, 1, 0}, {4, 4, 1, 1, 1}, {2, 4, 1, 1, 1}, {2, 4, 2, 1, 1}, {4, 4, 1, 1, 2},
+    {2, 4, 2, 1, 2}, {4, 8, 1, 1, 2}, {2, 4, 1, 1, 4}, {2, 8, 1, 1, 4}};
+constexpr int PAIR_FALLBACK = 6;
 
 // Workgroups a slab launch aims for.  Grids with fewer tiles (the coarse
 // multigrid levels, small scenes) split the right-hand sides into groups,
@@ -48,7 +53,7 @@ __host__ inline TileMap slab_map(int nx, int ny, int nz, int shape, int k) {
     TileMap t;
     // Flat tiles: tiles_x = 1 and tiles_y counts 64*ty-node pieces of a plane.
     const bool flat = sh.flat == 1;
-    t.tiles_x = flat ? 1 : (nx + 63) / 64;
+    t.tiles_x = flat ? 1 : (sh.flat == 4 ? (nx + 127) / 128 : (nx + 63) / 64);
     t.tiles_y = flat ? (nx * ny + 64 * sh.ty - 1) / (64 * sh.ty) : (ny + sh.ty - 1) / sh.ty;
     t.zb = sh.zb;
     t.tiles_z = (nz + sh.zb - 1) / sh.zb;
@@ -265,6 +270,160 @@ __global__ __launch_bounds__(64 * TY) void k_slab(OpView<float> op, TileMap tm, 
                     }
                     store_out<NT>(yout + r * ldy + p, out);
                 }
+            }
+        }
+    }
+}
+
+// Paired slab stencil (flat = 4): the arithmetic of k_slab per node, in the
+// same order (bit-identical results), with each lane holding the nodes
+// (2i', 2i'+1), i' = its lane in a 128-wide tile row.  x-neighbours: 2i'-1
+// is the previous lane's second node and 2i'+2 the next lane's first (DPP);
+// lanes 0 and 63 load the one across the tile edge.  One right-hand side per
+// pass.  Requires an even nx and 16-byte aligned columns (even ldx, ldy).
+template <int MODE, int EPI, int ZB, int TY, int NT>
+__global__ __launch_bounds__(64 * TY) void k_slab2(OpView<float> op, TileMap tm, const cx<float> *__restrict__ xin,
+                                               int64_t ldx, cx<float> *__restrict__ yout, int64_t ldy, int nrhs,
+                                               const int *__restrict__ live, const double *__restrict__ scale,
+                                               const cx<float> *__restrict__ b, const cx<float> *__restrict__ dinv,
+                                               float w, const float *__restrict__ bdiv) {
+    __shared__ int s_act[APPLY_MAX_RHS];
+    __shared__ float s_sc[APPLY_MAX_RHS];
+    __shared__ int s_na;
+    constexpr int NT_ = 64 * TY;
+    __shared__ hz_f4 s_pl[2][ZB][NT_];
+    const int part = (int)blockIdx.x / tm.nblocks;
+    const int tile = xcd_tile((int)blockIdx.x - part * tm.nblocks, tm.nblocks);
+    if (tile >= tm.ntiles) return;  // whole workgroup
+    stage_live_columns<float>(nrhs, live, scale, s_act, s_sc, &s_na);
+    const int tx = tile % tm.tiles_x;
+    const int ty = (tile / tm.tiles_x) % tm.tiles_y;
+    const int tz = tile / (tm.tiles_x * tm.tiles_y);
+    const int lane = threadIdx.x & 63;
+    // Lanes past the grid edge run on the last pair (feeding the shifts)
+    // and never store.
+    const int i_raw = tx * 128 + 2 * lane;
+    const int j_raw = ty * TY + (threadIdx.x >> 6);
+    const bool valid = i_raw < op.nx && j_raw < op.ny;
+    const int i = min(i_raw, op.nx - 2);
+    const int j = min(j_raw, op.ny - 1);
+    const int k0 = tz * tm.zb;
+    const int nzl = min(op.nz - k0, ZB);
+    const int64_t sxy = (int64_t)op.nx * op.ny;
+    const int64_t p0 = i + (int64_t)op.nx * (j + (int64_t)op.ny * k0);
+
+    cx<float> cs[ZB][2], c[ZB][2][6], dv[ZB][2];
+    float bs[ZB][2];
+#pragma unroll
+    for (int q = 0; q < ZB; ++q)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int kq = min(k0 + q, op.nz - 1);
+            const int64_t p = i + e + (int64_t)op.nx * (j + (int64_t)op.ny * kq);
+            const Nbr nb = neighbours(i + e, j, kq, op.nx, op.ny, op.nz);
+            row_coefs<float, MODE>(op, i + e, j, kq, p, nb, cs[q][e], c[q][e]);
+            dv[q][e] = mk<float>(0, 0);
+            bs[q][e] = 1.0f;
+            if (EPI == SLAB_SMOOTH) dv[q][e] = (MODE == MODE_AH) ? conj(dinv[p]) : dinv[p];
+            if (EPI != SLAB_APPLY && bdiv) bs[q][e] = 1.0f / bdiv[p];
+            if (EPI == SLAB_RESID0) {
+                cs[q][e] = cs[q][e] * first_sweep_factor<MODE>(dinv, bdiv, w, p);
+#pragma unroll
+                for (int d = 0; d < 6; ++d)
+                    c[q][e][d] = c[q][e][d] * first_sweep_factor<MODE>(dinv, bdiv, w, p + nb.off[d]);
+            }
+        }
+    __syncthreads();  // s_act
+    const bool has_xm = i > 0, has_xp = i + 2 < op.nx;
+    const int64_t oym = j > 0 ? -(int64_t)op.nx : 0;
+    const int64_t oyp = j < op.ny - 1 ? (int64_t)op.nx : 0;
+    const int tid = (int)threadIdx.x;
+    const bool lds_ym = tid >= 64, lds_yp = tid + 64 < NT_;
+    const int na = s_na;
+    const int per = (na + tm.csplit - 1) / tm.csplit;
+    const int t_begin = part * per, t_end = min(na, t_begin + per);
+    const hz_f4 zero4 = {0.0f, 0.0f, 0.0f, 0.0f};
+    int buf = 0;
+    for (int t = t_begin; t < t_end; ++t) {
+        const int r = s_act[t];
+        const cx<float> *xr = xin + r * ldx;
+        hz_f4 v[ZB + 2], ym[ZB], yp[ZB], bb[ZB];
+        cx<float> xe[ZB];
+#pragma unroll
+        for (int q = 0; q < ZB + 2; ++q) {
+            const int kq = k0 + q - 1;
+            const int kc = min(max(kq, 0), op.nz - 1);
+            v[q] = *reinterpret_cast<const hz_f4 *>(xr + p0 + (int64_t)(kc - k0) * sxy);
+            if (kq < 0 || kq >= op.nz) v[q] = zero4;
+        }
+#pragma unroll
+        for (int q = 0; q < ZB; ++q) {
+            const int kc = min(k0 + q, op.nz - 1);
+            const int64_t p = p0 + (int64_t)(kc - k0) * sxy;
+            ym[q] = zero4;
+            yp[q] = zero4;
+            if (!lds_ym) ym[q] = *reinterpret_cast<const hz_f4 *>(xr + p + oym);
+            if (!lds_yp) yp[q] = *reinterpret_cast<const hz_f4 *>(xr + p + oyp);
+            xe[q] = mk<float>(0, 0);
+            if (lane == 0 && has_xm) xe[q] = xr[p - 1];
+            if (lane == 63 && has_xp) xe[q] = xr[p + 2];
+            bb[q] = zero4;
+            if (EPI == SLAB_SMOOTH || EPI == SLAB_RESID) bb[q] = *reinterpret_cast<const hz_f4 *>(b + r * ldy + p);
+        }
+#pragma unroll
+        for (int q = 0; q < ZB; ++q) s_pl[buf][q][tid] = v[q + 1];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < ZB; ++q) {
+            if (lds_ym) ym[q] = s_pl[buf][q][tid - 64];
+            if (lds_yp) yp[q] = s_pl[buf][q][tid + 64];
+        }
+        buf ^= 1;
+#pragma unroll
+        for (int q = 0; q < ZB; ++q) {
+            const hz_f4 c4 = v[q + 1];
+            cx<float> xl = mk<float>(lane_prev(c4.z), lane_prev(c4.w));
+            cx<float> xn = mk<float>(lane_next(c4.x), lane_next(c4.y));
+            if (lane == 0) xl = xe[q];
+            if (lane == 63) xn = xe[q];
+            cx<float> out[2];
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const cx<float> xc = e ? mk<float>(c4.z, c4.w) : mk<float>(c4.x, c4.y);
+                const cx<float> xm = e ? mk<float>(c4.x, c4.y) : xl;
+                const cx<float> xp = e ? xn : mk<float>(c4.z, c4.w);
+                const cx<float> ymv = e ? mk<float>(ym[q].z, ym[q].w) : mk<float>(ym[q].x, ym[q].y);
+                const cx<float> ypv = e ? mk<float>(yp[q].z, yp[q].w) : mk<float>(yp[q].x, yp[q].y);
+                const cx<float> zm = e ? mk<float>(v[q].z, v[q].w) : mk<float>(v[q].x, v[q].y);
+                const cx<float> zp = e ? mk<float>(v[q + 2].z, v[q + 2].w) : mk<float>(v[q + 2].x, v[q + 2].y);
+                cx<float> acc = cs[q][e] * xc;
+                acc = acc + c[q][e][0] * xm;
+                acc = acc + c[q][e][1] * xp;
+                acc = acc + c[q][e][2] * ymv;
+                acc = acc + c[q][e][3] * ypv;
+                acc = acc + c[q][e][4] * zm;
+                acc = acc + c[q][e][5] * zp;
+                if (EPI == SLAB_APPLY) {
+                    out[e] = s_sc[t] * acc;
+                } else if (EPI == SLAB_RESID0) {
+                    out[e] = bs[q][e] * xc - acc;
+                } else {
+                    const cx<float> bv = e ? mk<float>(bb[q].z, bb[q].w) : mk<float>(bb[q].x, bb[q].y);
+                    const cx<float> ev = bs[q][e] * bv - acc;
+                    out[e] = (EPI == SLAB_SMOOTH) ? xc + w * (dv[q][e] * ev) : ev;
+                }
+            }
+            if (valid && q < nzl) {
+                hz_f4 o;
+                o.x = out[0].re;
+                o.y = out[0].im;
+                o.z = out[1].re;
+                o.w = out[1].im;
+                hz_f4 *dst = reinterpret_cast<hz_f4 *>(yout + r * ldy + p0 + (int64_t)q * sxy);
+                if (NT)
+                    __builtin_nontemporal_store(o, dst);
+                else
+                    *dst = o;
             }
         }
     }

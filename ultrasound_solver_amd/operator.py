@@ -221,10 +221,12 @@ class HelmholtzOperator:
 
     #: Slab tile shapes by index (z-planes per thread, tile rows, right-hand
     #: sides per pass, non-temporal stores, tile order: 0 rectangular, 1 flat,
-    #: 2 rectangular z-fastest), as in the library's table.
+    #: 2 rectangular z-fastest, 4 paired: two nodes per lane, even widths), as in the
+    #: library's table.
     SLAB_SHAPES = ((4, 4, 1, 0, 0), (2, 4, 1, 0, 0), (4, 8, 1, 0, 0), (2, 8, 1, 0, 0), (4, 2, 1, 0, 0),
                    (2, 4, 2, 0, 0), (4, 4, 1, 1, 0), (2, 4, 2, 1, 0), (4, 4, 1, 1, 1), (2, 4, 1, 1, 1),
-                   (2, 4, 2, 1, 1), (4, 4, 1, 1, 2), (2, 4, 2, 1, 2), (4, 8, 1, 1, 2))
+                   (2, 4, 2, 1, 1), (4, 4, 1, 1, 2), (2, 4, 2, 1, 2), (4, 8, 1, 1, 2), (2, 4, 1, 1, 4),
+                   (2, 8, 1, 1, 4))
 
     def material_vjp(self, u, lam):
         """dL/drho, dL/dc (grid-shaped, FP64, on the device) for
