@@ -141,6 +141,7 @@ struct hz_operator {
     // Cached GMRES workspace (grow-only).
     DevBuf V, X, Bh, dpart, npart, cpart, red, Hm, gc, gs, g, scales, coef, resid, tol, steps, live, member;
     DevBuf Hu, coef2, alpha, gram;  // two-step Arnoldi blocks
+    DevBuf Z;                       // multigrid form: preconditioned basis slots, restart x k x n
     DevBuf rhs32;
     std::unique_ptr<Multigrid> mg;
     // Captured GMRES cycle bodies, keyed by everything their launches bake in.
@@ -485,8 +486,8 @@ std::vector<uintptr_t> graph_signature(const hz_operator *op, size_t elem, int m
                                        bool use_mg) {
     std::vector<uintptr_t> sig = {elem, (uintptr_t)mode, (uintptr_t)k, (uintptr_t)restart, (uintptr_t)use_mg,
                                   (uintptr_t)op->slab_shape};
-    for (const DevBuf *b : {&op->V, &op->X, &op->dpart, &op->npart, &op->Hm, &op->gc, &op->gs, &op->g, &op->scales,
-                            &op->coef, &op->resid, &op->tol, &op->steps, &op->live, &op->member})
+    for (const DevBuf *b : {&op->V, &op->X, &op->Z, &op->dpart, &op->npart, &op->Hm, &op->gc, &op->gs, &op->g,
+                            &op->scales, &op->coef, &op->resid, &op->tol, &op->steps, &op->live, &op->member})
         sig.push_back((uintptr_t)b->p);
     if (use_mg)
         for (const auto &L : op->mg->levels) {
@@ -539,7 +540,8 @@ struct ChunkState {
 //   ref_semantics: full `restart` steps per cycle, convergence tested only at
 //   the cycle top (the reference/CuPy loop); otherwise a slot leaves its cycle
 //   as soon as the Givens estimate meets its tolerance.
-// On return X holds y for every slot that was active on entry.
+// On return X holds y (M y in the multigrid form) for every slot that was
+// active on entry.
 // Preconditioner of the inner solve: Jacobi (operator R A M, the reference's
 // choice) or shifted-Laplacian multigrid P (operator R A P).
 enum Precond { PRE_JACOBI = 0, PRE_MULTIGRID = 1 };
@@ -558,6 +560,17 @@ void gmres_cycles(hz_operator *op, int mode_ram, int k, int restart, int maxiter
     cx<T> *V = op->V.as<cx<T>>();
     cx<T> *X = op->X.as<cx<T>>();
     cx<T> *Bh = op->Bh.as<cx<T>>();
+    // Multigrid form: every preconditioned vector the Arnoldi steps compute is
+    // kept (slot i of Z), and X holds M y itself, updated from Z at the end
+    // of a cycle.  The cycle top and the caller then need no V-cycle of their
+    // own on X.
+    cx<float> *Z = nullptr;
+    if constexpr (std::is_same<T, float>::value) {
+        if (use_mg) {
+            op->Z.ensure((size_t)restart * k * n * sizeof(cx<float>));
+            Z = op->Z.as<cx<float>>();
+        }
+    }
     GmresState st;
     st.H = op->Hm.as<cx<double>>();
     st.gc = op->gc.as<double>();
@@ -586,16 +599,9 @@ void gmres_cycles(hz_operator *op, int mode_ram, int k, int restart, int maxiter
     while (true) {
         for (int r = 0; r < k; ++r) live_h[r] = cs.active[r];
         HZ_HIP(hipMemcpyAsync(st.live, live_h.data(), k * sizeof(int), hipMemcpyHostToDevice, s));
-        // Cycle top: V_0 = Bh - R A M X (true residual of the preconditioned system).
-        const cx<T> *src = X;
-        if constexpr (std::is_same<T, float>::value) {
-            if (use_mg) {
-                cx<float> *z = op->mg->levels[0]->xx.as<cx<float>>();
-                precondition(op, adjoint, X, z, k, st.live, s);
-                src = z;
-            }
-        }
-        residual_mode<T>(op, mode_op, src, Bh, V, k, st.live, op->npart.as<double>(), s);
+        // Cycle top: V_0 = Bh - R A M y (true residual of the preconditioned
+        // system); X is y, or M y itself in the multigrid form.
+        residual_mode<T>(op, mode_op, X, Bh, V, k, st.live, op->npart.as<double>(), s);
         hipLaunchKernelGGL(k_cycle_start, dim3(k), dim3(256), 0, s, st, op->npart.as<double>(),
                            (int)op->tm.nblocks, maxv, restart);
         HZ_LAUNCHED();
@@ -638,14 +644,15 @@ void gmres_cycles(hz_operator *op, int mode_ram, int k, int restart, int maxiter
                 cx<float> *w1 = V + (int64_t)(j + 1) * k * n;
                 cx<float> *w2 = V + (int64_t)(j + 2) * k * n;
                 const cx<float> *in = V + (int64_t)j * k * n;
-                cx<float> *z = use_mg ? op->mg->levels[0]->xx.as<cx<float>>() : nullptr;
                 if (use_mg) {
+                    cx<float> *z = Z + (int64_t)j * k * n;  // M V_j
                     precondition(op, adjoint, in, z, k, st.live, cs_stream);
                     in = z;
                 }
                 apply_mode<float>(op, mode_op, in, n, w1, n, k, st.live, st.scales + (int64_t)j * k, cs_stream);
                 in = w1;
                 if (use_mg) {
+                    cx<float> *z = Z + (int64_t)(j + 1) * k * n;  // M w1, before w1 is orthogonalised
                     precondition(op, adjoint, w1, z, k, st.live, cs_stream);
                     in = z;
                 }
@@ -700,7 +707,7 @@ void gmres_cycles(hz_operator *op, int mode_ram, int k, int restart, int maxiter
                 const cx<T> *in = vj;
                 if constexpr (std::is_same<T, float>::value) {
                     if (use_mg) {
-                        cx<float> *z = op->mg->levels[0]->xx.as<cx<float>>();
+                        cx<float> *z = Z + (int64_t)j * k * n;  // M V_j
                         precondition(op, adjoint, vj, z, k, st.live, cs_stream);
                         in = z;
                     }
@@ -718,10 +725,14 @@ void gmres_cycles(hz_operator *op, int mode_ram, int k, int restart, int maxiter
                                    (int)nblk, j, maxv, restart);
                 HZ_LAUNCHED();
             }
-            hipLaunchKernelGGL(k_trisolve, dim3(k), dim3(64), 0, cs_stream, st, maxv, restart, member);
+            hipLaunchKernelGGL(k_trisolve, dim3(k), dim3(64), 0, cs_stream, st, maxv, restart, member,
+                               (int)(use_mg && two));
             HZ_LAUNCHED();
-            launch_update<T>(dim3(nblk, k), cs_stream, V, k, n, 0, st.steps, X, n, st.coef, maxv, member, nullptr,
-                             chunk);
+            const cx<T> *basis = V;
+            if constexpr (std::is_same<T, float>::value)
+                if (use_mg) basis = Z;
+            launch_update<T>(dim3(nblk, k), cs_stream, basis, k, n, 0, st.steps, X, n, st.coef, maxv, member,
+                             nullptr, chunk);
         };
         if (use_graph) {
             hipGraphExec_t exec =
@@ -936,11 +947,9 @@ void solve_mixed(hz_operator *op, const hz_gmres_opts &o, const cx<double> *b, i
         }
         HZ_HIP(hipMemcpyAsync(op->member.p, member.data(), k * sizeof(int), hipMemcpyHostToDevice, s));
         if (use_mg) {
-            // x += P X: one more V-cycle on the accumulated Krylov combination.
-            cx<float> *z = op->mg->levels[0]->xx.as<cx<float>>();
-            precondition(op, o.adjoint != 0, op->X.as<cx<float>>(), z, k, op->member.as<int>(), s);
-            hipLaunchKernelGGL(k_accumulate, dim3(vec_blocks(n), k), dim3(256), 0, s, (const cx<float> *)z, n, x, ldx,
-                               (const int *)op->member.as<int>());
+            // x += M y: X already holds it.
+            hipLaunchKernelGGL(k_accumulate, dim3(vec_blocks(n), k), dim3(256), 0, s,
+                               (const cx<float> *)op->X.as<cx<float>>(), n, x, ldx, (const int *)op->member.as<int>());
         } else {
             hipLaunchKernelGGL((k_precond_out<float>), dim3(vec_blocks(n), k), dim3(256), 0, s,
                                (const cx<float> *)op->X.as<cx<float>>(), n, m, x, ldx, n,
@@ -1046,8 +1055,10 @@ int pick_shape(F &&launch, hipStream_t s, int preferred = 0) {
 // multigrid level vectors.
 size_t chunk_bytes(const hz_operator *op, int k, int restart) {
     size_t nodes = (size_t)op->n * (restart + 1 + 3);
-    if (op->mg)
+    if (op->mg) {
+        nodes += (size_t)op->n * restart;  // preconditioned basis slots
         for (const auto &L : op->mg->levels) nodes += 3 * (size_t)L->n;
+    }
     return nodes * k * sizeof(cx<float>);
 }
 
@@ -1057,7 +1068,7 @@ void drop_graphs(hz_operator *op) {
 }
 
 void release_workspace(hz_operator *op) {
-    for (DevBuf *b : {&op->V, &op->X, &op->Bh, &op->rhs32}) b->release();
+    for (DevBuf *b : {&op->V, &op->X, &op->Z, &op->Bh, &op->rhs32}) b->release();
     if (op->mg) {
         for (auto &L : op->mg->levels) {
             L->bb.release();
@@ -1790,7 +1801,7 @@ int64_t hz_device_bytes(const hz_operator *op) {
     if (!op) return 0;
     int64_t total = 0;
     const DevBuf *bufs[] = {&op->rho64, &op->kap64, &op->rho32, &op->kap32, &op->rsf64, &op->mf64, &op->rsa64,
-                            &op->ma64, &op->rsf32, &op->mf32, &op->rsa32, &op->ma32, &op->V, &op->X, &op->Bh,
+                            &op->ma64, &op->rsf32, &op->mf32, &op->rsa32, &op->ma32, &op->V, &op->X, &op->Z, &op->Bh,
                             &op->dpart, &op->npart, &op->cpart, &op->rhs32};
     for (const DevBuf *bf : bufs) total += (int64_t)bf->bytes;
     if (op->mg)

@@ -603,7 +603,15 @@ __global__ __launch_bounds__(256) void k_block_finish(GmresState st, const doubl
 
 // Back substitution on the rotated Hessenberg matrix; coef[r][i] = -y_i s_i so
 // that k_update adds sum_i y_i v_i to the iterate.
-__global__ void k_trisolve(GmresState st, int maxv, int restart, const int *__restrict__ member) {
+// blocks_z: the update runs over the stored preconditioned slots Z_i instead
+// (the multigrid form with two-step blocks).  Z_i = M V_i for a block's first
+// slot i, M q_i = s_i Z_i; its second slot holds Z_{i} = M w1 of the raw
+// block vector w1 = A M q_{i-1}, so that
+//   M q_i = (Z_i - sum_{l < i} Hu(l, i-1) M q_l) / Hu(i, i-1)
+// (the linearity the block already uses for column i of H).  The
+// coefficients of sum_i y_i M q_i on the Z_i follow from the highest slot
+// down.
+__global__ void k_trisolve(GmresState st, int maxv, int restart, const int *__restrict__ member, int blocks_z) {
     const int r = blockIdx.x;
     if (threadIdx.x != 0 || !member[r]) return;
     const int m = st.steps[r];
@@ -616,7 +624,21 @@ __global__ void k_trisolve(GmresState st, int maxv, int restart, const int *__re
         const cx<double> d = col[i * restart + i];
         y[i] = (cabs(d) > 0) ? cdiv(acc, d) : mk<double>(0, 0);
     }
-    for (int i = 0; i < m; ++i) st.coef[r * maxv + i] = (-st.scales[(int64_t)i * gridDim.x + r]) * y[i];
+    if (!blocks_z) {
+        for (int i = 0; i < m; ++i) st.coef[r * maxv + i] = (-st.scales[(int64_t)i * gridDim.x + r]) * y[i];
+        return;
+    }
+    const cx<double> *hu = st.Hu + (int64_t)r * (restart + 1) * restart;
+    for (int i = m - 1; i >= 0; --i) {
+        if (i & 1) {
+            const double r11 = hu[i * restart + i - 1].re;
+            const cx<double> c = r11 > 0 ? (1.0 / r11) * y[i] : mk<double>(0, 0);
+            for (int l = 0; l < i; ++l) y[l] = y[l] - c * hu[l * restart + i - 1];
+            st.coef[r * maxv + i] = -1.0 * c;
+        } else {
+            st.coef[r * maxv + i] = (-st.scales[(int64_t)i * gridDim.x + r]) * y[i];
+        }
+    }
 }
 
 // ---------------------------------------------------------------- elementwise
