@@ -255,8 +255,9 @@ hz_status hz_basis_field_match(const void *G, int64_t ldg, int64_t nrows, int32_
 hz_status hz_material_vjp(hz_operator *op, const void *u, const void *lambda, int32_t nrhs, int64_t ld,
                           double *g_rho, double *g_c, void *stream);
 
-/* Free the cached solver workspace (Krylov basis, multigrid level vectors,
- * captured graphs); the next solve allocates what it needs.  Waits for the
+/* Free the cached solver workspace (Krylov basis, the multigrid form's
+ * preconditioned basis, multigrid level vectors, captured graphs); the next
+ * solve allocates what it needs.  Waits for the
  * device first.  No counterpart in the reference (CuPy's pool frees on its
  * own schedule). */
 hz_status hz_release_workspace(hz_operator *op);
