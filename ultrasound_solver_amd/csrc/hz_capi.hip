@@ -659,40 +659,34 @@ void gmres_cycles(hz_operator *op, int mode_ram, int k, int restart, int maxiter
                 apply_mode<float>(op, mode_op, in, n, w2, n, k, st.live, nullptr, cs_stream);
                 const int groups = (j + 1 + DOT_GROUP / 2 - 1) / (DOT_GROUP / 2);
                 const bool pr = paired<float>(n);
+                double *gram = op->gram.as<double>();
                 if (pr)
                     hipLaunchKernelGGL((k_dots2<float, 2>), dim3(nblk, k, groups), dim3(256), 0, cs_stream,
                                        (const cx<float> *)V, k, n, j + 1, (const cx<float> *)w1,
-                                       (const cx<float> *)w2, st.live, op->dpart.as<double2>(), maxv, chunk);
+                                       (const cx<float> *)w2, st.live, op->dpart.as<double2>(), maxv, chunk, gram);
                 else
                     hipLaunchKernelGGL((k_dots2<float, 1>), dim3(nblk, k, groups), dim3(256), 0, cs_stream,
                                        (const cx<float> *)V, k, n, j + 1, (const cx<float> *)w1,
-                                       (const cx<float> *)w2, st.live, op->dpart.as<double2>(), maxv, chunk);
+                                       (const cx<float> *)w2, st.live, op->dpart.as<double2>(), maxv, chunk, gram);
                 HZ_LAUNCHED();
                 hipLaunchKernelGGL(k_reduce_dots2, dim3(j + 1, k), dim3(256), 0, cs_stream, op->dpart.as<double2>(),
                                    (int)nblk, maxv, j, st, restart);
                 HZ_LAUNCHED();
+                hipLaunchKernelGGL(k_block_alpha, dim3(k), dim3(256), 0, cs_stream, st, (const double *)gram,
+                                   (int)nblk, j, maxv, restart);
+                HZ_LAUNCHED();
                 if (pr)
                     hipLaunchKernelGGL((k_update2<float, 2>), dim3(nblk, k), dim3(256), 0, cs_stream,
                                        (const cx<float> *)V, k, n, j + 1, w1, w2, (const cx<double> *)st.coef,
-                                       (const cx<double> *)st.coef2, maxv, st.live, op->gram.as<double>(), chunk);
+                                       (const cx<double> *)st.coef2, maxv, st.live, gram, chunk,
+                                       (const cx<double> *)st.alpha);
                 else
                     hipLaunchKernelGGL((k_update2<float, 1>), dim3(nblk, k), dim3(256), 0, cs_stream,
                                        (const cx<float> *)V, k, n, j + 1, w1, w2, (const cx<double> *)st.coef,
-                                       (const cx<double> *)st.coef2, maxv, st.live, op->gram.as<double>(), chunk);
+                                       (const cx<double> *)st.coef2, maxv, st.live, gram, chunk,
+                                       (const cx<double> *)st.alpha);
                 HZ_LAUNCHED();
-                hipLaunchKernelGGL(k_block_mid, dim3(k), dim3(256), 0, cs_stream, st, op->gram.as<double>(),
-                                   (int)nblk, j, restart);
-                HZ_LAUNCHED();
-                if (pr)
-                    hipLaunchKernelGGL((k_axpy_norm<float, 2>), dim3(nblk, k), dim3(256), 0, cs_stream,
-                                       (const cx<float> *)w1, w2, n, (const cx<double> *)st.alpha, st.live,
-                                       op->npart.as<double>(), chunk);
-                else
-                    hipLaunchKernelGGL((k_axpy_norm<float, 1>), dim3(nblk, k), dim3(256), 0, cs_stream,
-                                       (const cx<float> *)w1, w2, n, (const cx<double> *)st.alpha, st.live,
-                                       op->npart.as<double>(), chunk);
-                HZ_LAUNCHED();
-                hipLaunchKernelGGL(k_block_finish, dim3(k), dim3(256), 0, cs_stream, st, op->npart.as<double>(),
+                hipLaunchKernelGGL(k_block_finish, dim3(k), dim3(256), 0, cs_stream, st, (const double *)gram,
                                    (int)nblk, j, restart);
                 HZ_LAUNCHED();
             }

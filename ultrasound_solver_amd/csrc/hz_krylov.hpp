@@ -301,17 +301,22 @@ __global__ __launch_bounds__(256) void k_arnoldi_finish(GmresState st, const dou
 //   W1 = w1 - sum_i h1_i q_i,          h1_i = q_i^H w1
 //   W2 = w2 - sum_i h2_i q_i,          h2_i = q_i^H w2
 //   q_{j+1} = W1 / r11,  r11 = |W1|
-//   q_{j+2} = (W2 - r12 q_{j+1}) / r22, r12 = q_{j+1}^H W2   (one axpy pass)
+//   q_{j+2} = (W2 - alpha W1) / r22, alpha = W1^H W2 / |W1|^2, r12 = alpha r11
+// alpha comes from the dot pass (|W1|^2 = |w1|^2 - |h1|^2, W1^H W2 =
+// w1^H w2 - h1^H h2), so the update pass writes W2 - alpha W1 directly.
 // Column j of H is (h1, r11); column j+1 follows from linearity:
 //   A M q_{j+1} = (w2 - sum_{i<=j} h1_i A M q_i) / r11,
 // with A M q_i = sum_l Hu(l, i) q_l the unrotated columns already built.
 
-// Dot partials of both targets: partial[((t*k + r)*maxv + i)*nblk + blk].
+// Dot partials of both targets: partial[((t*k + r)*maxv + i)*nblk + blk];
+// the first group of basis vectors also writes partials of |w1|^2 and
+// w1^H w2 (wgram[(r*nblk + blk)*4 + 0..2]), from which the intra-block
+// coefficient follows before the update pass.
 template <typename T, int VW>
 __global__ __launch_bounds__(256) void k_dots2(const cx<T> *__restrict__ V, int k, int64_t n, int jp1,
                                                const cx<T> *__restrict__ w1, const cx<T> *__restrict__ w2,
                                                const int *__restrict__ live, double2 *__restrict__ partial, int maxv,
-                                               int64_t chunk) {
+                                               int64_t chunk, double *__restrict__ wgram) {
     constexpr int G = DOT_GROUP / 2;
     __shared__ double red[4];
     const int r = blockIdx.y;
@@ -326,12 +331,19 @@ __global__ __launch_bounds__(256) void k_dots2(const cx<T> *__restrict__ V, int 
     T are[G], aim[G], bre[G], bim[G];
 #pragma unroll
     for (int q = 0; q < G; ++q) are[q] = aim[q] = bre[q] = bim[q] = 0;
+    const bool first = blockIdx.z == 0;
+    T waa = 0, wabr = 0, wabi = 0;
     if constexpr (VW == 2) {
         static_assert(sizeof(T) == 4, "paired loads are for complex64");
         const float4 *a4 = reinterpret_cast<const float4 *>(ar);
         const float4 *b4 = reinterpret_cast<const float4 *>(br);
         for (int64_t p = (p0 >> 1) + threadIdx.x; p < (p1 >> 1); p += 256) {
             const float4 av = a4[p], bv = b4[p];
+            if (first) {
+                waa += av.x * av.x + av.y * av.y + av.z * av.z + av.w * av.w;
+                wabr += av.x * bv.x + av.y * bv.y + av.z * bv.z + av.w * bv.w;
+                wabi += av.x * bv.y - av.y * bv.x + av.z * bv.w - av.w * bv.z;
+            }
 #pragma unroll
             for (int q = 0; q < G; ++q) {
                 if (q < cnt) {
@@ -346,6 +358,11 @@ __global__ __launch_bounds__(256) void k_dots2(const cx<T> *__restrict__ V, int 
     } else {
         for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
             const cx<T> av = ar[p], bv = br[p];
+            if (first) {
+                waa += norm2(av);
+                wabr += av.re * bv.re + av.im * bv.im;
+                wabi += av.re * bv.im - av.im * bv.re;
+            }
 #pragma unroll
             for (int q = 0; q < G; ++q) {
                 if (q < cnt) {
@@ -371,6 +388,51 @@ __global__ __launch_bounds__(256) void k_dots2(const cx<T> *__restrict__ V, int 
             }
         }
     }
+    if (first) {
+        const double s0 = block_combine_256((double)wave_sum(waa), red);
+        const double s1 = block_combine_256((double)wave_sum(wabr), red);
+        const double s2 = block_combine_256((double)wave_sum(wabi), red);
+        if (threadIdx.x == 0) {
+            double *o = wgram + ((int64_t)r * gridDim.x + blockIdx.x) * 4;
+            o[0] = s0;
+            o[1] = s1;
+            o[2] = s2;
+        }
+    }
+}
+
+// The intra-block coefficient before the update pass (grid (k)): with
+// W1 = w1 - sum h1_i q_i and W2 = w2 - sum h2_i q_i over orthonormal q_i,
+// |W1|^2 = |w1|^2 - |h1|^2 and W1^H W2 = w1^H w2 - h1^H h2, so
+// alpha = W1^H W2 / |W1|^2, and the update pass can write
+// W2 - alpha W1 = (w2 - alpha w1) - sum (coef2_i - alpha coef_i) V_i
+// directly (coef2 is rewritten here).  The norms that scale the two new
+// vectors are measured in that pass, not predicted.
+__global__ __launch_bounds__(256) void k_block_alpha(GmresState st, const double *__restrict__ wgram, int nblk, int j,
+                                                     int maxv, int restart) {
+    __shared__ double red[4];
+    const int r = blockIdx.x;
+    if (!st.live[r]) return;
+    double v[3] = {0, 0, 0};
+    for (int b = threadIdx.x; b < nblk; b += 256) {
+        const double *o = wgram + ((int64_t)r * nblk + b) * 4;
+        v[0] += o[0];
+        v[1] += o[1];
+        v[2] += o[2];
+    }
+    for (int q = 0; q < 3; ++q) v[q] = block_sum_256(v[q], red);
+    if (threadIdx.x != 0) return;
+    const cx<double> *hu = st.Hu + (int64_t)r * (restart + 1) * restart;
+    double w11 = v[0];
+    cx<double> w12 = mk<double>(v[1], v[2]);
+    for (int i = 0; i <= j; ++i) {
+        const cx<double> h1 = hu[i * restart + j], h2 = hu[i * restart + j + 1];
+        w11 -= norm2(h1);
+        w12 = w12 - conj(h1) * h2;
+    }
+    const cx<double> al = w11 > 0 ? (1.0 / w11) * w12 : mk<double>(0, 0);
+    st.alpha[r] = al;
+    for (int i = 0; i <= j; ++i) st.coef2[r * maxv + i] = st.coef2[r * maxv + i] - al * st.coef[r * maxv + i];
 }
 
 // Combine both targets' partials (grid (jp1, k)): Hu(i, j) = h1_i = s_i d1_i,
@@ -403,15 +465,16 @@ __global__ __launch_bounds__(256) void k_reduce_dots2(const double2 *__restrict_
     }
 }
 
-// W1 = w1 - sum coef V_i, W2 = w2 - sum coef2 V_i in place, with FP64 partials
-// of |W1|^2, Re/Im W1^H W2 and |W2|^2 (gram[(r*nblk + blk)*4 + 0..3]).
+// W1 = w1 - sum coef V_i, W2 = w2 - alpha w1 - sum coef2 V_i in place (coef2
+// already carries -alpha coef), with FP64 partials of |W1|^2, Re/Im W1^H W2
+// and |W2|^2 (gram[(r*nblk + blk)*4 + 0..3]).
 template <typename T, int VW>
 __global__ __launch_bounds__(256) void k_update2(const cx<T> *__restrict__ V, int k, int64_t n, int cnt,
                                                  cx<T> *__restrict__ w1, cx<T> *__restrict__ w2,
                                                  const cx<double> *__restrict__ coef,
                                                  const cx<double> *__restrict__ coef2, int maxv,
                                                  const int *__restrict__ live, double *__restrict__ gram,
-                                                 int64_t chunk) {
+                                                 int64_t chunk, const cx<double> *__restrict__ alpha) {
     __shared__ cx<T> cf[64], cf2[64];
     __shared__ double red[4];
     const int r = blockIdx.y;
@@ -425,6 +488,7 @@ __global__ __launch_bounds__(256) void k_update2(const cx<T> *__restrict__ V, in
     const int64_t p1 = min(n, p0 + chunk);
     cx<T> *ar = w1 + (int64_t)r * n;
     cx<T> *br = w2 + (int64_t)r * n;
+    const cx<T> al = cast<T>(alpha[r]);
     T g11 = 0, g22 = 0, g12r = 0, g12i = 0;
     if constexpr (VW == 2) {
         static_assert(sizeof(T) == 4, "paired loads are for complex64");
@@ -432,6 +496,10 @@ __global__ __launch_bounds__(256) void k_update2(const cx<T> *__restrict__ V, in
         float4 *b4 = reinterpret_cast<float4 *>(br);
         for (int64_t p = (p0 >> 1) + threadIdx.x; p < (p1 >> 1); p += 256) {
             float4 a = a4[p], b = b4[p];
+            b.x -= al.re * a.x - al.im * a.y;
+            b.y -= al.re * a.y + al.im * a.x;
+            b.z -= al.re * a.z - al.im * a.w;
+            b.w -= al.re * a.w + al.im * a.z;
             int q = 0;
             for (; q + 4 <= cnt; q += 4) {
                 float4 v[4];
@@ -473,6 +541,7 @@ __global__ __launch_bounds__(256) void k_update2(const cx<T> *__restrict__ V, in
     } else {
         for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
             cx<T> a = ar[p], b = br[p];
+            b = b - al * a;
             for (int q = 0; q < cnt; ++q) {
                 const cx<T> v = V[((int64_t)q * k + r) * n + p];
                 a = a - cf[q] * v;
@@ -499,89 +568,32 @@ __global__ __launch_bounds__(256) void k_update2(const cx<T> *__restrict__ V, in
     }
 }
 
-// After the update pass (grid (k)): r11 = |W1|, r12 = q_{j+1}^H W2, the
-// intra-block coefficient alpha = r12 / r11 for W2 -= alpha W1, and the
-// scale of slot j+1.  r11 and r12 are parked in Hu(j+1, j), Hu(j+1, j+1).
-__global__ __launch_bounds__(256) void k_block_mid(GmresState st, const double *__restrict__ gram, int nblk, int j,
-                                                   int restart) {
+// Finish the block (grid (k)) from the update pass's Gram partials:
+// r11 = |W1| and r22 = |W2 - alpha W1| as measured, r12 = alpha r11 (w2 =
+// sum h2_i q_i + alpha r11 q_{j+1} + r22 q_{j+2} exactly, by construction),
+// the scales of slots j+1 and j+2, column j+1 of Hu by linearity, then the
+// Givens updates of columns j and j+1 (the second only if the RHS is still
+// live after the first).
+__global__ __launch_bounds__(256) void k_block_finish(GmresState st, const double *__restrict__ gram, int nblk,
+                                                      int j, int restart) {
     __shared__ double red[4];
     const int r = blockIdx.x;
     if (!st.live[r]) return;
-    double v[3] = {0, 0, 0};
+    double v[2] = {0, 0};
     for (int b = threadIdx.x; b < nblk; b += 256) {
         const double *o = gram + ((int64_t)r * nblk + b) * 4;
         v[0] += o[0];
-        v[1] += o[1];
-        v[2] += o[2];
+        v[1] += o[3];
     }
-    for (int q = 0; q < 3; ++q) v[q] = block_sum_256(v[q], red);
+    for (int q = 0; q < 2; ++q) v[q] = block_sum_256(v[q], red);
     if (threadIdx.x != 0) return;
-    const double r11 = sqrt(v[0]);
-    const cx<double> r12 = r11 > 0 ? (1.0 / r11) * mk<double>(v[1], v[2]) : mk<double>(0, 0);
-    cx<double> *hu = st.Hu + (int64_t)r * (restart + 1) * restart;
-    hu[(j + 1) * restart + j] = mk<double>(r11, 0);
-    hu[(j + 1) * restart + j + 1] = r12;
-    st.alpha[r] = r11 > 0 ? (1.0 / r11) * r12 : mk<double>(0, 0);
+    const double r11 = sqrt(v[0]), r22 = sqrt(v[1]);
+    const cx<double> r12 = r11 * st.alpha[r];
     st.scales[(int64_t)(j + 1) * gridDim.x + r] = r11 > 0 ? 1.0 / r11 : 0.0;
-}
-
-// W2 -= alpha W1 in place with FP64 partials of |W2|^2.  Grid (nblk, k).
-template <typename T, int VW>
-__global__ __launch_bounds__(256) void k_axpy_norm(const cx<T> *__restrict__ w1, cx<T> *__restrict__ w2, int64_t n,
-                                                   const cx<double> *__restrict__ alpha,
-                                                   const int *__restrict__ live, double *__restrict__ norm_partial,
-                                                   int64_t chunk) {
-    __shared__ double red[4];
-    const int r = blockIdx.y;
-    if (!live[r]) return;
-    const cx<T> al = cast<T>(alpha[r]);
-    const int64_t p0 = blockIdx.x * chunk;
-    const int64_t p1 = min(n, p0 + chunk);
-    const cx<T> *ar = w1 + (int64_t)r * n;
-    cx<T> *br = w2 + (int64_t)r * n;
-    T nrm = 0;
-    if constexpr (VW == 2) {
-        const float4 *a4 = reinterpret_cast<const float4 *>(ar);
-        float4 *b4 = reinterpret_cast<float4 *>(br);
-        for (int64_t p = (p0 >> 1) + threadIdx.x; p < (p1 >> 1); p += 256) {
-            const float4 a = a4[p];
-            float4 b = b4[p];
-            b.x -= al.re * a.x - al.im * a.y;
-            b.y -= al.re * a.y + al.im * a.x;
-            b.z -= al.re * a.z - al.im * a.w;
-            b.w -= al.re * a.w + al.im * a.z;
-            b4[p] = b;
-            nrm += b.x * b.x + b.y * b.y + b.z * b.z + b.w * b.w;
-        }
-    } else {
-        for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
-            const cx<T> b = br[p] - al * ar[p];
-            br[p] = b;
-            nrm += norm2(b);
-        }
-    }
-    const double s = block_combine_256((double)wave_sum(nrm), red);
-    if (threadIdx.x == 0) norm_partial[(int64_t)r * gridDim.x + blockIdx.x] = s;
-}
-
-// Finish the block (grid (k)): r22 = |W2 - alpha W1|, scale of slot j+2,
-// column j+1 of Hu by linearity, then the Givens updates of columns j and
-// j+1 (the second only if the RHS is still live after the first).
-__global__ __launch_bounds__(256) void k_block_finish(GmresState st, const double *__restrict__ norm_partial,
-                                                      int nblk, int j, int restart) {
-    __shared__ double red[4];
-    const int r = blockIdx.x;
-    if (!st.live[r]) return;
-    double s = 0;
-    for (int b = threadIdx.x; b < nblk; b += 256) s += norm_partial[(int64_t)r * nblk + b];
-    s = block_sum_256(s, red);
-    if (threadIdx.x != 0) return;
-    const double r22 = sqrt(s);
     st.scales[(int64_t)(j + 2) * gridDim.x + r] = r22 > 0 ? 1.0 / r22 : 0.0;
     cx<double> *hu = st.Hu + (int64_t)r * (restart + 1) * restart;
     cx<double> *h = st.H + (int64_t)r * (restart + 1) * restart;
-    const double r11 = hu[(j + 1) * restart + j].re;
-    const cx<double> r12 = hu[(j + 1) * restart + j + 1];
+    hu[(j + 1) * restart + j] = mk<double>(r11, 0);
     // Coefficients of w2 in q_0 .. q_{j+2}: (h2_0 .. h2_j, r12, r22); subtract
     // h1_i * Hu(:, i) for i <= j (column j itself is (h1, r11)).
     cx<double> col[66];
