@@ -48,9 +48,9 @@ def parse():
     p.add_argument("--mg-shift", type=float, default=1.2)
     p.add_argument("--mg-pre", type=int, default=1)
     p.add_argument("--mg-post", type=int, default=1, help="post-sweeps on the finest multigrid level")
-    p.add_argument("--mg-post-coarse", type=int, default=4, help="post-sweeps on the coarser levels")
+    p.add_argument("--mg-post-coarse", type=int, default=5, help="post-sweeps on the coarser levels")
     p.add_argument("--mg-coarse-sweeps", type=int, default=30, help="sweeps on the coarsest level")
-    p.add_argument("--mg-levels", type=int, default=0, help="multigrid levels (0: down to 5 nodes per edge)")
+    p.add_argument("--mg-levels", type=int, default=0, help="multigrid levels (0: the library's default, down to the first level of <= 1024 nodes)")
     p.add_argument("--mg-weight", type=float, default=0.7)
     p.add_argument("--mg-cycle", type=int, default=1, choices=[1, 2], help="1: V-cycle, 2: W below the finest level")
     p.add_argument("--graph", type=int, default=0, help="replay each GMRES cycle from a captured hipGraph")
@@ -99,9 +99,7 @@ def main():
         stencil_only(op, args, scene.num_sources)
         return
     if args.precond == "multigrid":
-        op.set_multigrid(shift=args.mg_shift, pre=args.mg_pre, post=args.mg_post, post_coarse=args.mg_post_coarse,
-                         coarse_sweeps=args.mg_coarse_sweeps, levels=args.mg_levels or None, weight=args.mg_weight,
-                         cycle=args.mg_cycle)
+        op.set_multigrid(**_mg_args(args))
     m = scene.num_sources
     tuning = op.autotune(m, max_chunk=args.chunk or 64, restart=args.mixed_restart, preconditioner=args.precond)
     if not args.chunk:
@@ -266,8 +264,7 @@ def stencil_only(op, args, m):
     k = args.stencil_rhs or 32
     mode = stencil_mode(args)
     if args.precond == "multigrid":
-        op.set_multigrid(shift=args.mg_shift, pre=args.mg_pre, post=args.mg_post, post_coarse=args.mg_post_coarse,
-                         weight=args.mg_weight)
+        op.set_multigrid(**_mg_args(args))
     tuning = op.autotune(m, max_chunk=k, restart=args.mixed_restart, preconditioner=args.precond)
     if args.stencil_shape >= 0:
         op.set_tile_shapes([args.stencil_shape] + tuning["shapes"][1:])
@@ -293,6 +290,13 @@ def _unit_rhs(op, m, start, k):
     return op.build_rhs(w)
 
 
+def _mg_args(args):
+    """The multigrid settings of the command line, for every workload."""
+    return dict(shift=args.mg_shift, pre=args.mg_pre, post=args.mg_post, post_coarse=args.mg_post_coarse,
+                weight=args.mg_weight, coarse_sweeps=args.mg_coarse_sweeps,
+                levels=args.mg_levels or None, cycle=args.mg_cycle)
+
+
 def _scene_operator(scene, multigrid=True, **mg):
     from ultrasound_solver_amd import scenes
     from ultrasound_solver_amd.operator import HelmholtzOperator
@@ -316,7 +320,7 @@ def bench_uneven(args):
     sc, ops = [], []
     for i in range(args.scenes):
         sc.append(scenes.config_uneven(i))
-        ops.append(_scene_operator(sc[-1], shift=args.mg_shift))
+        ops.append(_scene_operator(sc[-1], **_mg_args(args)))
         if i % 8 == 7:
             log(f"uneven: {i + 1} scenes set up")
     bs = [_unit_rhs(op, s.num_sources, 0, args.scene_rhs) for op, s in zip(ops, sc)]
@@ -364,7 +368,7 @@ def bench_large(args):
     from ultrasound_solver_amd import scenes
     t0 = time.perf_counter()
     scene = scenes.config_two_layer(256)
-    op = _scene_operator(scene, shift=args.mg_shift)
+    op = _scene_operator(scene, **_mg_args(args))
     m = scene.num_sources
     tuning = op.autotune(m, max_chunk=16, restart=args.mixed_restart, preconditioner="multigrid")
     torch.cuda.synchronize()

@@ -122,7 +122,7 @@ class Hierarchy:
             out.append(np.ascontiguousarray((1j*op.omega*area*adm).ravel(order="F"), dtype=np.complex128))
         return out
 
-    def install(self, op, shift=1.2, pre=1, post=1, coarse_sweeps=30, weight=0.7, cycle=1, post_coarse=4):
+    def install(self, op, shift=1.2, pre=1, post=1, coarse_sweeps=30, weight=0.7, cycle=1, post_coarse=5):
         keep = []
 
         def ptr(a):

@@ -255,7 +255,7 @@ class HelmholtzOperator:
         _capi.check(_capi.lib().hz_set_tile_shapes(self.handle, arr, len(shapes)), "hz_set_tile_shapes")
 
     def set_multigrid(self, shift=1.2, pre=1, post=1, coarse_sweeps=30, weight=0.7, levels=None, cycle=1,
-                      post_coarse=4):
+                      post_coarse=5):
         """Install the shifted-Laplacian multigrid preconditioner (mixed form):
         `post` damped-Jacobi sweeps after the correction on the finest level,
         `post_coarse` on the levels below it (coarse sweeps cost an eighth per
