@@ -301,6 +301,7 @@ void launch_slab(int shape, const OpView<float> &v, const cx<float> *x, int64_t 
         break;
         HZ_SLAB2_CASE(14, 2, 4, 1)
         HZ_SLAB2_CASE(15, 2, 8, 1)
+        HZ_SLAB2_CASE(16, 4, 4, 1)
 #undef HZ_SLAB2_CASE
         default:
             throw BadArgument("unknown slab tile shape");

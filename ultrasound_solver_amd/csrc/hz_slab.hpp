@@ -35,11 +35,11 @@ namespace hz {
 struct SlabShape {
     int zb, ty, rp, nt, flat;
 };
-constexpr int SLAB_SHAPES = 16;
+constexpr int SLAB_SHAPES = 17;
 constexpr SlabShape SLAB_SHAPE[SLAB_SHAPES] = {
     {4, 4, 1, 0, 0}, {2, 4, 1, 0, 0}, {4, 8, 1, 0, 0}, {2, 8, 1, 0, 0}, {4, 2, 1, 0, 0}, {2, 4, 2, 0, 0},
     {4, 4, 1, 1, 0}, {2, 4, 2, 1, 0}, {4, 4, 1, 1, 1}, {2, 4, 1, 1, 1}, {2, 4, 2, 1, 1}, {4, 4, 1, 1, 2},
-    {2, 4, 2, 1, 2}, {4, 8, 1, 1, 2}, {2, 4, 1, 1, 4}, {2, 8, 1, 1, 4}};
+    {2, 4, 2, 1, 2}, {4, 8, 1, 1, 2}, {2, 4, 1, 1, 4}, {2, 8, 1, 1, 4}, {4, 4, 1, 1, 4}};
 constexpr int PAIR_FALLBACK = 6;
 
 // Workgroups a slab launch aims for.  Grids with fewer tiles (the coarse

@@ -226,7 +226,7 @@ class HelmholtzOperator:
     SLAB_SHAPES = ((4, 4, 1, 0, 0), (2, 4, 1, 0, 0), (4, 8, 1, 0, 0), (2, 8, 1, 0, 0), (4, 2, 1, 0, 0),
                    (2, 4, 2, 0, 0), (4, 4, 1, 1, 0), (2, 4, 2, 1, 0), (4, 4, 1, 1, 1), (2, 4, 1, 1, 1),
                    (2, 4, 2, 1, 1), (4, 4, 1, 1, 2), (2, 4, 2, 1, 2), (4, 8, 1, 1, 2), (2, 4, 1, 1, 4),
-                   (2, 8, 1, 1, 4))
+                   (2, 8, 1, 1, 4), (4, 4, 1, 1, 4))
 
     def material_vjp(self, u, lam):
         """dL/drho, dL/dc (grid-shaped, FP64, on the device) for
