@@ -76,6 +76,14 @@ __device__ __forceinline__ cx<float> first_sweep_factor(const cx<float> *__restr
     return (bdiv ? w / bdiv[p] : w) * d;
 }
 
+// A coefficient times a first-sweep factor, rounded the same way in every
+// kernel (no fused multiply-adds), so that the folded coefficients of the
+// RESID0 epilogue agree bit for bit across tile shapes.
+__device__ __forceinline__ cx<float> fold(cx<float> c, cx<float> f) {
+#pragma clang fp contract(off)
+    return mk<float>(c.re * f.re - c.im * f.im, c.re * f.im + c.im * f.re);
+}
+
 // APPLY : y = s_r * op(x)
 // SMOOTH: y = x + w * dinv * (b / bdiv - op(x))     (damped Jacobi)
 // RESID : y = b / bdiv - op(x)
@@ -172,9 +180,9 @@ __global__ __launch_bounds__(64 * TY) void k_slab(OpView<float> op, TileMap tm, 
         if (EPI == SLAB_SMOOTH) dv[q] = (MODE == MODE_AH) ? conj(dinv[p]) : dinv[p];
         if (EPI != SLAB_APPLY && bdiv) bs[q] = 1.0f / bdiv[p];
         if (EPI == SLAB_RESID0) {
-            cs[q] = cs[q] * first_sweep_factor<MODE>(dinv, bdiv, w, p);
+            cs[q] = fold(cs[q], first_sweep_factor<MODE>(dinv, bdiv, w, p));
 #pragma unroll
-            for (int e = 0; e < 6; ++e) c[q][e] = c[q][e] * first_sweep_factor<MODE>(dinv, bdiv, w, p + nb.off[e]);
+            for (int e = 0; e < 6; ++e) c[q][e] = fold(c[q][e], first_sweep_factor<MODE>(dinv, bdiv, w, p + nb.off[e]));
         }
     }
     __syncthreads();  // s_act: staged while the coefficients were computed
@@ -326,13 +334,47 @@ __global__ __launch_bounds__(64 * TY) void k_slab2(OpView<float> op, TileMap tm,
             bs[q][e] = 1.0f;
             if (EPI == SLAB_SMOOTH) dv[q][e] = (MODE == MODE_AH) ? conj(dinv[p]) : dinv[p];
             if (EPI != SLAB_APPLY && bdiv) bs[q][e] = 1.0f / bdiv[p];
-            if (EPI == SLAB_RESID0) {
-                cs[q][e] = cs[q][e] * first_sweep_factor<MODE>(dinv, bdiv, w, p);
-#pragma unroll
-                for (int d = 0; d < 6; ++d)
-                    c[q][e][d] = c[q][e][d] * first_sweep_factor<MODE>(dinv, bdiv, w, p + nb.off[d]);
-            }
         }
+    if constexpr (EPI == SLAB_RESID0) {
+        // First-sweep factors, one per distinct node around the pair's
+        // column: the window planes k0-1 .. k0+ZB of both nodes, the rows
+        // j -+ 1 and the x-neighbours across the pair, each folded into every
+        // coefficient that reaches it (as k_slab does per neighbour; an absent
+        // neighbour's coefficient is zero, so its clamped stand-in is moot).
+        cx<float> fw[ZB + 2][2], fym[ZB][2], fyp[ZB][2], fxm[ZB], fxp[ZB];
+        const int jm = max(j - 1, 0), jp = min(j + 1, op.ny - 1);
+        const int im = max(i - 1, 0), ip = min(i + 2, op.nx - 1);
+        auto at = [&](int ii, int jj, int kk) { return ii + (int64_t)op.nx * (jj + (int64_t)op.ny * kk); };
+#pragma unroll
+        for (int q = 0; q < ZB + 2; ++q) {
+            const int kc = min(max(k0 + q - 1, 0), op.nz - 1);
+#pragma unroll
+            for (int e = 0; e < 2; ++e) fw[q][e] = first_sweep_factor<MODE>(dinv, bdiv, w, at(i + e, j, kc));
+        }
+#pragma unroll
+        for (int q = 0; q < ZB; ++q) {
+            const int kc = min(k0 + q, op.nz - 1);
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                fym[q][e] = first_sweep_factor<MODE>(dinv, bdiv, w, at(i + e, jm, kc));
+                fyp[q][e] = first_sweep_factor<MODE>(dinv, bdiv, w, at(i + e, jp, kc));
+            }
+            fxm[q] = first_sweep_factor<MODE>(dinv, bdiv, w, at(im, j, kc));
+            fxp[q] = first_sweep_factor<MODE>(dinv, bdiv, w, at(ip, j, kc));
+        }
+#pragma unroll
+        for (int q = 0; q < ZB; ++q)
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                cs[q][e] = fold(cs[q][e], fw[q + 1][e]);
+                c[q][e][0] = fold(c[q][e][0], e ? fw[q + 1][0] : fxm[q]);
+                c[q][e][1] = fold(c[q][e][1], e ? fxp[q] : fw[q + 1][1]);
+                c[q][e][2] = fold(c[q][e][2], fym[q][e]);
+                c[q][e][3] = fold(c[q][e][3], fyp[q][e]);
+                c[q][e][4] = fold(c[q][e][4], fw[q][e]);
+                c[q][e][5] = fold(c[q][e][5], fw[q + 2][e]);
+            }
+    }
     __syncthreads();  // s_act
     const bool has_xm = i > 0, has_xp = i + 2 < op.nx;
     const int64_t oym = j > 0 ? -(int64_t)op.nx : 0;
