@@ -246,6 +246,50 @@ hz_status hz_basis_field_match(const void *G, int64_t ldg, int64_t nrows, int32_
                                const double *target, const double *weight, double eps, double *loss,
                                void *g, void *u, void *stream);
 
+/* Fused loss evaluations on G, one pass each (phase_adjoint_optimizer.py:
+ * 163-364; m <= 512): u = G w, the loss into *loss (device double), its
+ * cotangent c per row and g = G^H c (m complex128); u is written when not
+ * null.  The per-wave partials live in the per-device workspace (see above).
+ *   HZ_LOSS_FIELD_MATCH  field_match (:163-177): target and weight per row.
+ *   HZ_LOSS_ENERGY       the background term of focal_pressure (:244-258):
+ *                        loss 0.5 scale sum |u|^2, c = scale u, with
+ *                        scale = background_weight / N.
+ *   HZ_LOSS_SOFTMAX      the sidelobe term of focal_contrast (:312-338): loss
+ *                        scale (max + T (log sum exp((|u| - max)/T) - log count_a))
+ *                        over rows with mask != 0, c = scale p u/(|u| + eps) with
+ *                        p the softmax weights; scale = sidelobe weight,
+ *                        T = temperature, count_a = number of masked rows.
+ *   HZ_LOSS_RATIO        the focal contrast ratio mean_A |u|^2 / mean_B |u|^2
+ *                        over rows of class mask == 1 (count_a rows) and
+ *                        mask == 2 (count_b rows), c = 2u/(count_a mean_B) on A and
+ *                        -2 mean_A u/(count_b mean_B^2) on B.  No reference
+ *                        counterpart: the ratio form of a focal_contrast loss
+ *                        as a 160^3 gradient workload states it.
+ * The focal term -w mean_t |u_t| is not included: it needs only the target
+ * rows (or exact samples) and is added by the caller
+ * (src/algorithm/baselines/common.py:189-195). */
+typedef enum hz_basis_loss_kind {
+    HZ_LOSS_FIELD_MATCH = 0,
+    HZ_LOSS_ENERGY = 1,
+    HZ_LOSS_SOFTMAX = 2,
+    HZ_LOSS_RATIO = 3
+} hz_basis_loss_kind;
+
+typedef struct hz_basis_loss_desc {
+    int32_t kind;            /* hz_basis_loss_kind */
+    const double *target;    /* FIELD_MATCH: target amplitude per row (device) */
+    const double *weight;    /* FIELD_MATCH: weight per row (device) */
+    const uint8_t *mask;     /* SOFTMAX: nonzero on background rows; RATIO: 1 / 2 classes (device) */
+    double eps;              /* amplitude regulariser (1e-12 in the reference) */
+    double scale;            /* ENERGY: background_weight / N; SOFTMAX: sidelobe weight */
+    double temperature;      /* SOFTMAX */
+    int64_t count_a;         /* SOFTMAX: masked rows; RATIO: class-1 rows */
+    int64_t count_b;         /* RATIO: class-2 rows */
+} hz_basis_loss_desc;
+
+hz_status hz_basis_loss(const void *G, int64_t ldg, int64_t nrows, int32_t m, const void *w,
+                        const hz_basis_loss_desc *desc, double *loss, void *g, void *u, void *stream);
+
 /* Material gradient, replacing ConservativeHelmholtz.material_vjp
  * (src/algorithm/physics/conservative_helmholtz.py:151-175): for
  * dL = -Re(lambda^H dA u) writes dL/drho and dL/dc per node (FP64, N each)

@@ -4,7 +4,9 @@ Follows ``src/algorithm/training/phase_adjoint_optimizer.py``
 (Gaussian target fields :78-139, nearest grid index :142-144,
 ``AmplitudeFieldLoss.evaluate`` :163-177, ``_focal_term`` :210-223,
 ``FocalPressureLoss.evaluate`` :244-258, ``FocalContrastLoss.evaluate``
-:312-338, background mask :384-414),
+:312-338, background mask :384-414), the ratio form of focal contrast that
+the 160^3 gradient workload states (no reference code; pinned by its
+piecewise-constant closed form, scale invariance and central differences),
 ``src/algorithm/training/phase_response_basis.py`` (trilinear
 ``target_response`` :139-175, ``field`` :177-206, ``phase_vjp`` :208-253),
 ``src/algorithm/baselines/common.py`` (focal gradient :189-195, metrics
@@ -105,6 +107,39 @@ def focal_contrast(field, indices, target_weight, sidelobe_weight, temperature, 
         return loss, None
     cot[mask] += sidelobe_weight*prob*field[mask]/(bg + EPS)
     return loss, cot
+
+
+def ratio_masks(shape, box, points, radius):
+    """(ball, rest) of the focal contrast ratio: nodes within `radius` of a
+    target (grid from 0, as ``background_mask``) and the nodes of the upper
+    half z >= Lz/2 outside every ball."""
+    axes = [np.linspace(0.0, box[a], shape[a]) for a in range(3)]
+    gx, gy, gz = np.meshgrid(*axes, indexing="ij")
+    ball = np.zeros(shape, dtype=bool)
+    for p in np.asarray(points, dtype=float).reshape(-1, 3):
+        ball |= (gx - p[0])**2 + (gy - p[1])**2 + (gz - p[2])**2 <= radius*radius
+    upper = np.zeros(shape, dtype=bool)
+    upper[:, :, shape[2]//2:] = True
+    return ball, upper & ~ball
+
+
+def focal_contrast_ratio(field, ball, rest, with_cotangent=True):
+    """The focal contrast of the 160^3 gradient workload as a ratio: mean |u|^2
+    over the target ball divided by mean |u|^2 over the rest of the upper half.
+    No reference code exists for this form (its focal_contrast is the
+    smooth-max form above); with a = mean_ball |u|^2, b = mean_rest |u|^2 and
+    d|u|^2 = Re((2u)^H du): c = 2u/(n_ball b) on the ball,
+    -2 a u/(n_rest b^2) on the rest."""
+    p2 = np.abs(field)**2
+    na, nb = int(np.count_nonzero(ball)), int(np.count_nonzero(rest))
+    a = float(np.sum(p2[ball]))/na
+    b = float(np.sum(p2[rest]))/nb
+    if not with_cotangent:
+        return a/b, None
+    cot = np.zeros(field.shape, dtype=np.complex128)
+    cot[ball] = 2.0*field[ball]/(na*b)
+    cot[rest] = -2.0*a*field[rest]/(nb*b*b)
+    return a/b, cot
 
 
 def trilinear_rows(points, shape, origin, dx):

@@ -33,6 +33,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_capi.GmresStats) == 4*4 + 8*3 + 4*2
     assert ctypes.sizeof(_capi.SurfaceDesc) == 4*3 + 4 + 8 + 8*8  # 4 bytes of padding before dx
     assert ctypes.sizeof(_capi.Primitive) == 4 + 4 + 8*6  # 4 bytes of padding before center
+    assert ctypes.sizeof(_capi.BasisLossDesc) == 4 + 4 + 8*3 + 8*3 + 8*2  # 4 bytes of padding after kind
 
 
 def test_last_error_is_empty_string_initially():

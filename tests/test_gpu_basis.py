@@ -88,3 +88,93 @@ def test_basis_autograd_vjp_matches_oracle_and_fd():
     fd = olo.central_difference(
         lambda p: olo.field_match(olo.basis_field(g, p, shape), target, weight, False)[0], p0, 1e-6, count=m)
     assert max(olo.scaled_gradient_error(a, b) for a, b in zip(grad, fd)) < 1e-6
+
+
+def _fused_case(kind, n, m, seed, temperature=2.0):
+    """(g, ph, u, kwargs for basis.fused_loss, oracle loss, oracle G^H c, the
+    bound |G|^T |c| that scales the rounding of each entry of G^H c)."""
+    g, gd = _basis(n, m, seed)
+    rng = np.random.default_rng(seed + 1)
+    ph = rng.uniform(0, 2*np.pi, m)
+    u = g @ np.exp(1j*ph)
+    origin = [(0,)]  # a focal index with weight 0: only the background term remains
+    if kind == "energy":
+        beta = 0.7
+        ref_loss, cot = olo.focal_pressure(u, origin, 0.0, beta)
+        kw = dict(scale=beta/n)
+    elif kind == "softmax":
+        beta = 1.3
+        mask = rng.random(n) < 0.7
+        mask[rng.integers(0, n)] = True
+        ref_loss, cot = olo.focal_contrast(u, origin, 0.0, beta, temperature, mask)
+        kw = dict(mask=torch.as_tensor(mask.astype(np.uint8), device="cuda:0"), scale=beta,
+                  temperature=temperature, count_a=int(mask.sum()))
+    else:
+        cls = rng.integers(0, 3, n)
+        cls[0], cls[-1] = 1, 2
+        ref_loss, cot = olo.focal_contrast_ratio(u, cls == 1, cls == 2)
+        kw = dict(mask=torch.as_tensor(cls.astype(np.uint8), device="cuda:0"),
+                  count_a=int((cls == 1).sum()), count_b=int((cls == 2).sum()))
+    return gd, ph, u, kw, ref_loss, g.conj().T @ cot, np.abs(g).T @ np.abs(cot)
+
+
+def _close(got, want, bound, tol):
+    """Entries of G^H c within tol of the magnitudes summed into them (the
+    ratio loss with one emitter has G^H c = 0 exactly: scale invariance)."""
+    return np.max(np.abs(got - want)) <= tol*np.max(bound)
+
+
+@pytest.mark.parametrize("n,m", [(7001, 1), (6000, 144), (3001, 512), (5, 33)])
+@pytest.mark.parametrize("kind", ["energy", "softmax", "ratio"])
+def test_fused_losses_match_oracle(kind, n, m):
+    """hz_basis_loss: the focal_pressure background, the focal_contrast
+    smooth-max and the contrast ratio, loss and G^H c to 1e-12 of the oracle,
+    from one row up to the widest row (512 emitters) and fewer rows than waves."""
+    from ultrasound_solver_amd import basis
+    gd, ph, u_ref, kw, ref_loss, ref_g, bound = _fused_case(kind, n, m, 11)
+    w = torch.as_tensor(np.exp(1j*ph), device="cuda:0")
+    loss, g, u = basis.fused_loss(gd, w, kind, return_field=True, **kw)
+    assert float(loss) == pytest.approx(ref_loss, rel=1e-12)
+    assert _close(g.cpu().numpy(), ref_g, bound, 1e-12)
+    assert _rel(u.cpu().numpy(), u_ref) < 1e-12
+    loss2, g2 = basis.fused_loss(gd, w, kind, **kw)
+    assert torch.equal(g2, g) and float(loss2) == float(loss)
+
+
+@pytest.mark.parametrize("temperature", [0.05, 40.0])
+def test_fused_softmax_running_maximum(temperature):
+    """A sharp and a flat softmax (T = 0.05 underflows most weights to 0)."""
+    from ultrasound_solver_amd import basis
+    gd, ph, _, kw, ref_loss, ref_g, bound = _fused_case("softmax", 20011, 96, 13, temperature)
+    loss, g = basis.fused_loss(gd, torch.as_tensor(np.exp(1j*ph), device="cuda:0"), "softmax", **kw)
+    assert float(loss) == pytest.approx(ref_loss, rel=1e-12)
+    assert _close(g.cpu().numpy(), ref_g, bound, 1e-12)
+
+
+def test_fused_softmax_single_masked_row():
+    """One background row r: the smooth max is |u_r| exactly and c_r = beta u_r/(|u_r| + eps)."""
+    from ultrasound_solver_amd import basis
+    g, gd = _basis(4097, 20, 17)
+    ph = np.random.default_rng(18).uniform(0, 2*np.pi, 20)
+    u = g @ np.exp(1j*ph)
+    r = 3001
+    mask = np.zeros(4097, dtype=np.uint8)
+    mask[r] = 1
+    loss, gg = basis.fused_loss(gd, torch.as_tensor(np.exp(1j*ph), device="cuda:0"), "softmax",
+                                mask=torch.as_tensor(mask, device="cuda:0"), scale=0.5, temperature=3.0, count_a=1)
+    assert float(loss) == pytest.approx(0.5*abs(u[r]), rel=1e-13)
+    want = g[r].conj()*(0.5*u[r]/(abs(u[r]) + olo.EPS))
+    assert _rel(gg.cpu().numpy(), want) < 1e-12
+
+
+def test_fused_loss_rejects_bad_descriptors():
+    from ultrasound_solver_amd import basis
+    _, gd = _basis(100, 8, 19)
+    w = torch.ones(8, dtype=torch.complex128, device="cuda:0")
+    with pytest.raises(RuntimeError, match="mask"):
+        basis.fused_loss(gd, w, "softmax", temperature=1.0, count_a=3)
+    mask = torch.ones(100, dtype=torch.uint8, device="cuda:0")
+    with pytest.raises(RuntimeError, match="temperature"):
+        basis.fused_loss(gd, w, "softmax", mask=mask, temperature=0.0, count_a=100)
+    with pytest.raises(RuntimeError, match="count"):
+        basis.fused_loss(gd, w, "ratio", mask=mask, count_a=60, count_b=60)

@@ -197,3 +197,71 @@ def test_batched_phases_give_per_row_gradients():
         (hza.phase_field(p1, eng).abs()**2).sum().backward()
         np.testing.assert_allclose(phs.grad[r].cpu().numpy(), p1.grad.cpu().numpy(), rtol=1e-6, atol=1e-9)
     s.close()
+
+
+def _oracle_objective(s, cfg, loss, g_ref, ph):
+    """(loss, phase gradient) of PhaseProblem.evaluate (common.py:275-343) from
+    the oracle's basis: exact focal samples through the trilinear target
+    response, the grid cotangent's basis VJP plus the focal gradient."""
+    shape = tuple(cfg.domain.grid_size)
+    box = cfg.domain.box_size
+    pts = np.asarray(cfg.targets)
+    idx = [olo.nearest_index(p, cfg.domain.origin_m, cfg.domain.dx) for p in pts]
+    u = olo.basis_field(g_ref, ph, shape)
+    if loss == "field_match":
+        from ultrasound_solver_amd import training
+        tgt = training.AmplitudeTarget.from_config(cfg)
+        val, cot = olo.field_match(u, tgt.target, tgt.weight)
+        return val, olo.phase_vjp(g_ref, cot, ph)
+    if loss == "focal_contrast_ratio":
+        ball, rest = olo.ratio_masks(shape, box, pts, cfg.training.target_radius)
+        val, cot = olo.focal_contrast_ratio(u, ball, rest)
+        return val, olo.phase_vjp(g_ref, cot, ph)
+    resp = olo.target_response(g_ref, pts, shape, cfg.domain.origin_m, cfg.domain.dx)
+    focal = resp @ np.exp(1j*ph)
+    tr = cfg.training
+    if loss == "focal_pressure":
+        val, cot = olo.focal_pressure(u, idx, tr.target_weight, tr.background_weight, focal_field=focal)
+    else:
+        mask = olo.background_mask(shape, box, pts, tr.target_radius, tr.source_exclusion_layers)
+        val, cot = olo.focal_contrast(u, idx, tr.target_weight, tr.background_weight, tr.sidelobe_temperature,
+                                      mask, focal_field=focal)
+    grad = olo.phase_vjp(g_ref, cot, ph) + olo.focal_gradient(resp, focal, ph, tr.target_weight)
+    return val, grad
+
+
+@pytest.mark.parametrize("route", ["basis", "adjoint"])
+@pytest.mark.parametrize("loss", ["field_match", "focal_pressure", "focal_contrast", "focal_contrast_ratio"])
+def test_phase_objective_matches_oracle_and_gradient_check(tmp_path, loss, route):
+    """PhaseObjective.evaluate (common.py:275-343) through the device basis
+    (fused loss kernels) and through forward + adjoint solves: loss and phase
+    gradient against the oracle's basis composition, including the focal
+    gradient of exact samples (common.py:189-195); then the reference's
+    central-difference check (adjoint/__init__.py:44-58) on 4 emitters."""
+    from ultrasound_solver_amd.solvers import HelmholtzDirectSolver
+    from ultrasound_solver_amd.training import PhaseObjective
+    cfg = small_cfg(tmp_path, precision="mixed", pre="multigrid", loss=loss)
+    s = HelmholtzDirectSolver(cfg)
+    if route == "basis":
+        obj = PhaseObjective(s, basis=s.build_phase_response_basis())
+    else:
+        obj = PhaseObjective(s, engine_kwargs=dict(rtol=1e-10, preconditioner="multigrid"))
+    a, b = oracle_system(s)
+    g_ref = okr.direct_solve(a, b.toarray())
+    ph = np.random.default_rng(5).uniform(0, 2*np.pi, 4)
+    val, grad = obj.evaluate(ph, gradient=True)
+    ref_val, ref_grad = _oracle_objective(s, cfg, loss, g_ref, ph)
+    assert val == pytest.approx(ref_val, rel=1e-6)
+    for i in range(4):
+        assert olo.scaled_gradient_error(grad[i], ref_grad[i]) < 1e-5, (i, grad, ref_grad)
+    # The reference's default step (gradient_check_step 1e-3).  Through the
+    # basis the loss is exact up to rounding; through solves to 1e-10 the
+    # difference quotient carries ~1e-10 |L| / 1e-3 of noise (|L| ~ 1e9 for
+    # field_match), hence the workload's 1e-4 bar there.
+    checks = obj.gradient_check(ph, step=1e-3, count=4)
+    assert [c["index"] for c in checks] == [0, 1, 2, 3]
+    for c in checks:
+        assert c["scaled_error"] < (1e-5 if route == "basis" else 1e-4), checks
+    assert obj.field_evaluations == 10 and obj.vjp_evaluations == 2
+    assert obj.history[-1]["stage"] == "gradient_check"
+    s.close()

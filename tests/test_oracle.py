@@ -323,7 +323,7 @@ def test_worked_cotangent_example():
     assert np.real(np.conj(c[0])*(0.01 + 0.02j)) == pytest.approx(-0.022, abs=1e-12)
 
 
-@pytest.mark.parametrize("kind", ["field_match", "focal_pressure", "focal_contrast"])
+@pytest.mark.parametrize("kind", ["field_match", "focal_pressure", "focal_contrast", "focal_contrast_ratio"])
 def test_phase_vjp_matches_central_differences(kind):
     shape = (7, 6, 5)
     n, m = int(np.prod(shape)), 9
@@ -341,6 +341,8 @@ def test_phase_vjp_matches_central_differences(kind):
             return losses.field_match(u, target, weight, cot)
         if kind == "focal_pressure":
             return losses.focal_pressure(u, idx, 1.0, 0.3, cot)
+        if kind == "focal_contrast_ratio":
+            return losses.focal_contrast_ratio(u, ~mask, mask, cot)
         return losses.focal_contrast(u, idx, 1.0, 1.3, 2.0, mask, cot)
 
     ph = rng.uniform(0, 2*np.pi, m)

@@ -17,12 +17,13 @@ MODE_A, MODE_RAM, MODE_AH, MODE_RAMH, MODE_RA, MODE_RAH = 0, 1, 2, 3, 4, 5
 EXPORTS = ("hz_create", "hz_destroy", "hz_apply", "hz_gmres", "hz_set_sources", "hz_build_rhs",
            "hz_gather_sources", "hz_sample", "hz_device_bytes", "hz_last_error", "hz_set_multigrid",
            "hz_autotune", "hz_set_tile_shapes", "hz_gmres_batch", "hz_release_workspace",
-           "hz_material_vjp", "hz_basis_field", "hz_basis_adjoint", "hz_basis_field_match",
+           "hz_material_vjp", "hz_basis_field", "hz_basis_adjoint", "hz_basis_field_match", "hz_basis_loss",
            "hz_z_transfer", "hz_surface_create", "hz_surface_destroy", "hz_surface_tables",
            "hz_surface_set_sources", "hz_surface_transform", "hz_surface_solve_flux", "hz_surface_planes",
            "hz_surface_sample", "hz_surface_target_response", "hz_deposit_sources",
            "hz_rasterize_materials", "hz_gaussian_filter")
 PRE_JACOBI, PRE_MULTIGRID = 0, 1
+LOSS_FIELD_MATCH, LOSS_ENERGY, LOSS_SOFTMAX, LOSS_RATIO = range(4)
 
 
 class GridDesc(ctypes.Structure):
@@ -69,6 +70,12 @@ class GmresStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class BasisLossDesc(ctypes.Structure):
+    _fields_ = [("kind", c_int32), ("target", c_void_p), ("weight", c_void_p), ("mask", c_void_p),
+                ("eps", c_double), ("scale", c_double), ("temperature", c_double),
+                ("count_a", c_int64), ("count_b", c_int64)]
+
+
 class HzError(RuntimeError):
     def __init__(self, status, message):
         super().__init__(message)
@@ -109,6 +116,8 @@ def lib():
         handle.hz_basis_adjoint.argtypes = [c_void_p, c_int64, c_int64, c_int32, c_void_p, c_void_p, c_void_p]
         handle.hz_basis_field_match.argtypes = [c_void_p, c_int64, c_int64, c_int32, c_void_p, c_void_p, c_void_p,
                                                 c_double, c_void_p, c_void_p, c_void_p, c_void_p]
+        handle.hz_basis_loss.argtypes = [c_void_p, c_int64, c_int64, c_int32, c_void_p, POINTER(BasisLossDesc),
+                                         c_void_p, c_void_p, c_void_p, c_void_p]
         handle.hz_material_vjp.argtypes = [c_void_p, c_void_p, c_void_p, c_int32, c_int64, c_void_p, c_void_p,
                                            c_void_p]
         handle.hz_set_tile_shapes.argtypes = [c_void_p, POINTER(c_int32), c_int32]

@@ -139,3 +139,9 @@ def focal_contrast_loss(u, indices, target_weight, sidelobe_weight, temperature,
     return loss + sidelobe_weight*smooth
 
 
+
+
+def focal_contrast_ratio_loss(u, ball, rest):
+    """mean |u|^2 over the ball / mean |u|^2 over the rest (boolean masks)."""
+    p2 = u.real**2 + u.imag**2
+    return p2[ball].mean()/p2[rest].mean()

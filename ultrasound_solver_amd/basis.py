@@ -1,7 +1,7 @@
 """Device response-basis operations on a phase response basis G held as an
 (N, M) complex128 torch tensor (rows = nodes in Fortran order), through the
 library's one-pass kernels (phase_response_basis.py:177-253,
-phase_adjoint_optimizer.py:163-200)."""
+phase_adjoint_optimizer.py:163-364)."""
 import ctypes
 
 import torch
@@ -73,4 +73,49 @@ def field_match(G, phases, target, weight, eps=EPS, return_field=False):
         ctypes.c_void_p(wt.data_ptr()), float(eps), ctypes.c_void_p(loss.data_ptr()), ctypes.c_void_p(g.data_ptr()),
         ctypes.c_void_p(u.data_ptr() if u is not None else 0), _stream()), "hz_basis_field_match")
     out = (loss[0], phase_gradient(g, w))
+    return out + (u,) if return_field else out
+
+
+FUSED_MAX_EMITTERS = 512
+LOSS_KINDS = {"field_match": _capi.LOSS_FIELD_MATCH, "energy": _capi.LOSS_ENERGY,
+              "softmax": _capi.LOSS_SOFTMAX, "ratio": _capi.LOSS_RATIO}
+
+
+def fused_loss(G, w, kind, target=None, weight=None, mask=None, eps=EPS, scale=0.0, temperature=1.0,
+               count_a=0, count_b=0, return_field=False):
+    """One pass over G for a loss of the field u = G w (hz_basis_loss):
+    returns (loss as a 0-d device tensor, g = G^H c (M,) device[, u]).
+
+    kind: "field_match" (target, weight per row), "energy" (0.5 scale sum
+    |u|^2), "softmax" (scale * smooth-max of |u| over rows with mask != 0 at
+    `temperature`; count_a masked rows) or "ratio" (mean |u|^2 over mask == 1
+    divided by mean over mask == 2; count_a, count_b rows).  Per-row arrays
+    are device tensors in the row order of G; mask is uint8."""
+    n, m, ld = _check(G)
+    if m > FUSED_MAX_EMITTERS:
+        raise ValueError(f"the fused loss kernels take at most {FUSED_MAX_EMITTERS} emitters")
+    w = _vec(w, m, G)
+    desc = _capi.BasisLossDesc()
+    desc.kind = LOSS_KINDS[kind]
+    keep = []
+    for name, arr, dtype in (("target", target, torch.float64), ("weight", weight, torch.float64),
+                             ("mask", mask, torch.uint8)):
+        if arr is None:
+            continue
+        a = torch.as_tensor(arr, device=G.device).to(dtype).contiguous().reshape(-1)
+        if a.numel() != n:
+            raise ValueError(f"{name} needs one entry per basis row")
+        keep.append(a)
+        setattr(desc, name, a.data_ptr())
+    desc.eps, desc.scale, desc.temperature = float(eps), float(scale), float(temperature)
+    desc.count_a, desc.count_b = int(count_a), int(count_b)
+    loss = torch.empty(1, dtype=torch.float64, device=G.device)
+    g = torch.empty(m, dtype=torch.complex128, device=G.device)
+    u = torch.empty(n, dtype=torch.complex128, device=G.device) if return_field else None
+    _capi.check(_capi.lib().hz_basis_loss(ctypes.c_void_p(G.data_ptr()), ld, n, m, ctypes.c_void_p(w.data_ptr()),
+                                          ctypes.byref(desc), ctypes.c_void_p(loss.data_ptr()),
+                                          ctypes.c_void_p(g.data_ptr()),
+                                          ctypes.c_void_p(u.data_ptr() if u is not None else 0), _stream()),
+                "hz_basis_loss")
+    out = (loss[0], g)
     return out + (u,) if return_field else out

@@ -1732,27 +1732,133 @@ hz_status hz_basis_adjoint(const void *G, int64_t ldg, int64_t nrows, int32_t m,
     });
 }
 
+}  // extern "C"
+
+namespace {
+
+// Waves of a fused loss launch: 4 per workgroup, at most 1024 workgroups,
+// and at most BASIS_LOSS_PARTIAL_BYTES of per-wave partials.
+int64_t basis_loss_waves(int64_t nrows, int m, int nacc) {
+    const int64_t cap = BASIS_LOSS_PARTIAL_BYTES / ((int64_t)nacc * m * (int64_t)sizeof(cx<double>));
+    int64_t nwaves = std::min<int64_t>({(int64_t)4096, std::max<int64_t>(4, cap), nrows + 3});
+    return std::max<int64_t>(4, nwaves / 4 * 4);
+}
+
+BasisLossArgs basis_loss_args(const hz_basis_loss_desc *d, int64_t nrows) {
+    if (!d) throw BadArgument("the loss descriptor is required");
+    BasisLossArgs a{};
+    a.target = d->target;
+    a.weight = d->weight;
+    a.mask = d->mask;
+    a.eps = d->eps;
+    a.scale = d->scale;
+    a.temperature = d->temperature;
+    a.count_a = (double)d->count_a;
+    a.count_b = (double)d->count_b;
+    switch (d->kind) {
+    case HZ_LOSS_FIELD_MATCH:
+        if (!a.target || !a.weight) throw BadArgument("field_match needs target and weight");
+        break;
+    case HZ_LOSS_ENERGY:
+        break;
+    case HZ_LOSS_SOFTMAX:
+        if (!a.mask) throw BadArgument("the smooth-max loss needs a row mask");
+        if (!(a.temperature > 0)) throw BadArgument("the smooth-max temperature must be positive");
+        if (d->count_a < 1 || d->count_a > nrows) throw BadArgument("count_a must be the number of masked rows");
+        break;
+    case HZ_LOSS_RATIO:
+        if (!a.mask) throw BadArgument("the ratio loss needs a row class mask");
+        if (d->count_a < 1 || d->count_b < 1 || d->count_a + d->count_b > nrows)
+            throw BadArgument("count_a and count_b must be the sizes of row classes 1 and 2");
+        break;
+    default:
+        throw BadArgument("unknown loss kind");
+    }
+    return a;
+}
+
+template <int KIND>
+void launch_basis_loss_kind(const cx<double> *G, int64_t ldg, int64_t nrows, int m, const cx<double> *w,
+                            const BasisLossArgs &a, int64_t nwaves, cx<double> *part, double *scal, double *loss,
+                            cx<double> *g, cx<double> *u, hipStream_t s) {
+    const int64_t rows = (nrows + nwaves - 1) / nwaves;
+    hipLaunchKernelGGL(k_basis_loss<KIND>, dim3((unsigned)(nwaves / 4)), dim3(256), 0, s, G, ldg, nrows, m, w, a, rows,
+                       part, scal, u);
+    HZ_LAUNCHED();
+    hipLaunchKernelGGL(k_basis_loss_combine<KIND>, dim3((m + 255) / 256), dim3(256), 0, s,
+                       (const cx<double> *)part, (const double *)scal, nwaves, m, a, g, loss);
+    HZ_LAUNCHED();
+}
+
+// One fused loss evaluation with its partials at `scratch_base` (which holds
+// at least basis_loss_bytes(...) bytes).
+void launch_basis_loss(const void *G, int64_t ldg, int64_t nrows, int32_t m, const void *w,
+                       const BasisLossArgs &a, int kind, void *scratch_base, double *loss, void *g, void *u,
+                       hipStream_t s) {
+    const int nacc = kind == HZ_LOSS_RATIO ? 2 : 1;
+    const int64_t nwaves = basis_loss_waves(nrows, m, nacc);
+    cx<double> *part = static_cast<cx<double> *>(scratch_base);
+    double *scal = reinterpret_cast<double *>(part + (int64_t)nacc * nwaves * m);
+    auto Gc = static_cast<const cx<double> *>(G);
+    auto wc = static_cast<const cx<double> *>(w);
+    auto gc = static_cast<cx<double> *>(g);
+    auto uc = static_cast<cx<double> *>(u);
+    switch (kind) {
+    case HZ_LOSS_FIELD_MATCH:
+        launch_basis_loss_kind<LOSS_FIELD_MATCH>(Gc, ldg, nrows, m, wc, a, nwaves, part, scal, loss, gc, uc, s);
+        break;
+    case HZ_LOSS_ENERGY:
+        launch_basis_loss_kind<LOSS_ENERGY>(Gc, ldg, nrows, m, wc, a, nwaves, part, scal, loss, gc, uc, s);
+        break;
+    case HZ_LOSS_SOFTMAX:
+        launch_basis_loss_kind<LOSS_SOFTMAX>(Gc, ldg, nrows, m, wc, a, nwaves, part, scal, loss, gc, uc, s);
+        break;
+    default:
+        launch_basis_loss_kind<LOSS_RATIO>(Gc, ldg, nrows, m, wc, a, nwaves, part, scal, loss, gc, uc, s);
+        break;
+    }
+}
+
+size_t basis_loss_bytes(int64_t nrows, int32_t m, int kind) {
+    const int nacc = kind == HZ_LOSS_RATIO ? 2 : 1;
+    const int64_t nwaves = basis_loss_waves(nrows, m, nacc);
+    return (size_t)nacc * nwaves * m * sizeof(cx<double>) + (size_t)nwaves * LOSS_SCALARS * sizeof(double);
+}
+
+void check_basis_loss_args(const void *G, int64_t ldg, int64_t nrows, int32_t m, const void *w, double *loss,
+                           void *g) {
+    check_basis_args(G, ldg, nrows, m);
+    if (m > BASIS_FUSED_MAX_M) throw BadArgument("the fused loss kernels take at most 512 emitters");
+    if (!w || !loss || !g) throw BadArgument("w, loss and g must be device pointers");
+}
+
+}  // namespace
+
+extern "C" {
+
 hz_status hz_basis_field_match(const void *G, int64_t ldg, int64_t nrows, int32_t m, const void *w,
                                const double *target, const double *weight, double eps, double *loss, void *g, void *u,
                                void *stream) {
     return guarded([&] {
-        check_basis_args(G, ldg, nrows, m);
-        if (m > BASIS_FUSED_MAX_M) throw BadArgument("the fused field_match kernel takes at most 512 emitters");
-        if (!w || !target || !weight || !loss || !g) throw BadArgument("w, target, weight, loss and g are required");
-        hipStream_t s = static_cast<hipStream_t>(stream);
-        const int nb = (int)std::min<int64_t>(1024, (nrows + 3) / 4);
-        const int64_t nwaves = (int64_t)nb * 4;
-        const int64_t rows = (nrows + nwaves - 1) / nwaves;
-        DevBuf &ws = basis_workspace((size_t)nwaves * m * sizeof(cx<double>) + (size_t)nwaves * sizeof(double));
-        cx<double> *part = ws.as<cx<double>>();
-        double *lpart = reinterpret_cast<double *>(part + nwaves * m);
-        hipLaunchKernelGGL(k_basis_field_match, dim3(nb), dim3(256), 0, s, static_cast<const cx<double> *>(G), ldg,
-                           nrows, (int)m, static_cast<const cx<double> *>(w), target, weight, eps, rows, part, lpart,
-                           static_cast<cx<double> *>(u));
-        HZ_LAUNCHED();
-        hipLaunchKernelGGL(k_basis_combine, dim3((m + 255) / 256), dim3(256), 0, s, (const cx<double> *)part,
-                           (int)nwaves, (int)m, static_cast<cx<double> *>(g), (const double *)lpart, loss);
-        HZ_LAUNCHED();
+        check_basis_loss_args(G, ldg, nrows, m, w, loss, g);
+        hz_basis_loss_desc d{};
+        d.kind = HZ_LOSS_FIELD_MATCH;
+        d.target = target;
+        d.weight = weight;
+        d.eps = eps;
+        const BasisLossArgs a = basis_loss_args(&d, nrows);
+        DevBuf &ws = basis_workspace(basis_loss_bytes(nrows, m, d.kind));
+        launch_basis_loss(G, ldg, nrows, m, w, a, d.kind, ws.p, loss, g, u, static_cast<hipStream_t>(stream));
+    });
+}
+
+hz_status hz_basis_loss(const void *G, int64_t ldg, int64_t nrows, int32_t m, const void *w,
+                        const hz_basis_loss_desc *desc, double *loss, void *g, void *u, void *stream) {
+    return guarded([&] {
+        check_basis_loss_args(G, ldg, nrows, m, w, loss, g);
+        const BasisLossArgs a = basis_loss_args(desc, nrows);
+        DevBuf &ws = basis_workspace(basis_loss_bytes(nrows, m, desc->kind));
+        launch_basis_loss(G, ldg, nrows, m, w, a, desc->kind, ws.p, loss, g, u, static_cast<hipStream_t>(stream));
     });
 }
 

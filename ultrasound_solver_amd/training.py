@@ -9,8 +9,12 @@
 * Losses -- src/algorithm/training/phase_adjoint_optimizer.py:78-414:
   ``AmplitudeTarget``/Gaussian targets, ``AmplitudeFieldLoss``
   (field_match), ``FocalPressureLoss``, ``FocalContrastLoss`` and the
-  background mask, each with a host (numpy) and a device (torch) evaluation
-  returning (loss, cotangent) with dL = Re(c^H du).
+  background mask, plus ``FocalContrastRatioLoss`` (the ratio form of focal
+  contrast), each with a host (numpy) and a device (torch) evaluation
+  returning (loss, cotangent) with dL = Re(c^H du), and a one-pass
+  evaluation on a device basis returning (loss, G^H c).
+* ``PhaseObjective`` -- ``PhaseProblem.evaluate`` (src/algorithm/baselines/
+  common.py:172-343) on the device.
 """
 import hashlib
 import json
@@ -121,15 +125,35 @@ class PhaseResponseBasis:
         self.basis_gpu = None
 
     def target_response(self, points):
+        """(P, M) trilinear rows of G at the points: the 8 corner rows of each
+        point are read (from the device basis, or from the stored file) and
+        weighted on the device (phase_response_basis.py:139-175)."""
         self._require_complete()
         cfg = self.solver.cfg
         rows, wts = trilinear_rows(points, cfg.domain.grid_size, cfg.domain.origin_m, cfg.domain.dx,
                                    cfg.domain.box_size)
         self._check_outside_pml(np.asarray(points, float))
-        out = np.zeros((len(rows), self.num_transducers), dtype=np.complex128)
-        for c in range(8):
-            out += wts[:, c, None]*self.basis[rows[:, c]]
-        return out
+        dev = self._device()
+        if self.basis_gpu is not None:
+            corner = self.basis_gpu.index_select(0, torch.as_tensor(rows.ravel(), device=dev))
+        else:
+            corner = torch.as_tensor(np.asarray(self.basis[rows.ravel()]), device=dev)
+        corner = corner.reshape(len(rows), 8, self.num_transducers)
+        out = (torch.as_tensor(wts, device=dev)[:, :, None]*corner).sum(dim=1)
+        return out.cpu().numpy()
+
+    def _device(self):
+        if self.basis_gpu is not None:
+            return self.basis_gpu.device
+        op = getattr(self.solver, "operator", None)
+        return torch.device(op.device if op is not None else "cuda")
+
+    def _row_chunks(self):
+        """(start, device rows) of a basis that stays in its file: chunks of
+        voxel_chunk_size rows are uploaded in turn (phase_response_basis.py:373-410)."""
+        dev = self._device()
+        for s in range(0, self.solver.total_dofs, self.voxel_chunk_size):
+            yield s, torch.as_tensor(np.asarray(self.basis[s:s + self.voxel_chunk_size]), device=dev)
 
     def _check_outside_pml(self, points):
         from .physics import FACES, boundary_type
@@ -150,51 +174,50 @@ class PhaseResponseBasis:
                 raise ValueError("Target lies in PML; enlarge the physical interior")
 
     def field(self, phases, return_numpy=True):
+        """u = G e^{i phi} on the device (phase_response_basis.py:177-206);
+        a basis not moved to the device is streamed to it in row chunks."""
         self._require_complete()
         phases = np.asarray(phases) if not isinstance(phases, torch.Tensor) else phases
         if tuple(phases.shape) != (self.num_transducers,):
             raise ValueError(f"phases must have shape ({self.num_transducers},), got {tuple(phases.shape)}")
+        from . import basis as dev_basis
         d = self.solver.cfg.domain
+        dev = self._device()
+        w = torch.exp(1j*torch.as_tensor(phases, dtype=torch.float64, device=dev))
         if self.basis_gpu is not None:
-            from . import basis as dev_basis
-            ph = torch.as_tensor(phases, dtype=torch.float64, device=self.basis_gpu.device)
-            u = dev_basis.field(self.basis_gpu, torch.exp(1j*ph))
-            if return_numpy:
-                return u.cpu().numpy().reshape((d.nx, d.ny, d.nz), order="F")
-            # (nx, ny, nz) view of the Fortran-ordered vector
-            return u.reshape(d.nz, d.ny, d.nx).permute(2, 1, 0)
-        w = np.exp(1j*np.asarray(phases, dtype=np.float64))
-        out = np.empty(self.solver.total_dofs, dtype=np.complex128)
-        for s in range(0, self.solver.total_dofs, self.voxel_chunk_size):
-            out[s:s + self.voxel_chunk_size] = self.basis[s:s + self.voxel_chunk_size] @ w
-        return out.reshape((d.nx, d.ny, d.nz), order="F")
+            u = dev_basis.field(self.basis_gpu, w)
+        else:
+            u = torch.empty(self.solver.total_dofs, dtype=torch.complex128, device=dev)
+            for s, rows in self._row_chunks():
+                u[s:s + rows.shape[0]] = dev_basis.field(rows, w)
+        if return_numpy:
+            return u.cpu().numpy().reshape((d.nx, d.ny, d.nz), order="F")
+        # (nx, ny, nz) view of the Fortran-ordered vector
+        return u.reshape(d.nz, d.ny, d.nx).permute(2, 1, 0)
 
     def phase_vjp(self, field_cotangent, phases, return_numpy=True):
-        """dL/dphi for dL = Re(c^H du) (phase_response_basis.py:208-253)."""
+        """dL/dphi = Re(conj(G^H c) i e^{i phi}) for dL = Re(c^H du), on the
+        device (phase_response_basis.py:208-253)."""
         self._require_complete()
-        if self.basis_gpu is not None:
-            dev = self.basis_gpu.device
-            c = field_cotangent
-            if isinstance(c, torch.Tensor):
-                c = c.to(dev, torch.complex128)
-                c = c.permute(*reversed(range(c.dim()))).reshape(-1) if c.dim() == 3 else c.reshape(-1)
-            else:
-                c = torch.as_tensor(np.asarray(field_cotangent, np.complex128).reshape(-1, order="F"), device=dev)
-            if c.numel() != self.solver.total_dofs:
-                raise ValueError("field_cotangent has the wrong size")
-            from . import basis as dev_basis
-            resp = dev_basis.adjoint(self.basis_gpu, c)
-            w = torch.exp(1j*torch.as_tensor(np.asarray(phases, np.float64), device=dev))
-            g = dev_basis.phase_gradient(resp, w)
-            return g.cpu().numpy() if return_numpy else g
-        c = np.asarray(field_cotangent, dtype=np.complex128).reshape(-1, order="F")
-        if c.size != self.solver.total_dofs:
+        from . import basis as dev_basis
+        dev = self._device()
+        c = field_cotangent
+        if isinstance(c, torch.Tensor):
+            c = c.to(dev, torch.complex128)
+            c = c.permute(*reversed(range(c.dim()))).reshape(-1) if c.dim() == 3 else c.reshape(-1)
+        else:
+            c = torch.as_tensor(np.asarray(field_cotangent, np.complex128).reshape(-1, order="F"), device=dev)
+        if c.numel() != self.solver.total_dofs:
             raise ValueError("field_cotangent has the wrong size")
-        resp = np.zeros(self.num_transducers, dtype=np.complex128)
-        for s in range(0, self.solver.total_dofs, self.voxel_chunk_size):
-            resp += c[s:s + self.voxel_chunk_size].conj() @ self.basis[s:s + self.voxel_chunk_size]
-        resp = resp.conj()
-        return np.real(np.conj(resp)*(1j*np.exp(1j*np.asarray(phases, np.float64))))
+        if self.basis_gpu is not None:
+            resp = dev_basis.adjoint(self.basis_gpu, c)
+        else:
+            resp = torch.zeros(self.num_transducers, dtype=torch.complex128, device=dev)
+            for s, rows in self._row_chunks():
+                resp += dev_basis.adjoint(rows, c[s:s + rows.shape[0]].contiguous())
+        w = torch.exp(1j*torch.as_tensor(np.asarray(phases, np.float64), device=dev))
+        g = dev_basis.phase_gradient(resp, w)
+        return g.cpu().numpy() if return_numpy else g
 
     def close(self):
         self.release_gpu()
@@ -377,6 +400,34 @@ def _focal_term(field, indices, weight, with_cotangent, focal_field):
     return loss, cot
 
 
+def _fortran_rows(indices, shape):
+    """Basis rows (Fortran node order) of (i, j, k) grid indices."""
+    idx = np.asarray(indices, dtype=np.int64).reshape(-1, 3)
+    return idx[:, 0] + shape[0]*(idx[:, 1] + shape[1]*idx[:, 2])
+
+
+def _basis_focal(G, rows, w, weight, eps, focal_field=None):
+    """Focal term -w mean |u_t| on a device basis: with exact samples the grid
+    part of the gradient is zero (their gradient is added by the caller, as
+    common.py:189-195 does); otherwise u_t = G[rows] w and
+    G^H c = G[rows]^H (-w/K u_t/(|u_t| + eps)), repeated rows accumulating."""
+    if focal_field is not None:
+        return -weight*focal_field.abs().mean(), None
+    gt = G.index_select(0, rows)
+    vals = gt @ w
+    amp = vals.abs()
+    return -weight*amp.mean(), gt.conj().transpose(0, 1) @ ((-weight/len(rows))*vals/(amp + eps))
+
+
+def _device_rows(cache, key, G, make):
+    """Per-device cache of a loss's row data (masks, row indices, weights)."""
+    hit = cache.get(key)
+    if hit is None or hit.device != G.device:
+        hit = torch.as_tensor(make(), device=G.device)
+        cache[key] = hit
+    return hit
+
+
 class AmplitudeFieldLoss:
     def __init__(self, amplitude_target, eps=EPS):
         self.target = amplitude_target.target
@@ -408,6 +459,14 @@ class AmplitudeFieldLoss:
             return loss, None
         return loss, weight*weight*res*field/(amp + self.eps)
 
+    def evaluate_basis(self, G, w, with_gradient=True, focal_field=None):
+        """(loss, G^H c) in one pass over the device basis G (hz_basis_loss)."""
+        from . import basis as dev_basis
+        tgt = _device_rows(self._dev, "target_rows", G, lambda: self.target.ravel(order="F"))
+        wt = _device_rows(self._dev, "weight_rows", G, lambda: self.weight.ravel(order="F"))
+        loss, g = dev_basis.fused_loss(G, w, "field_match", target=tgt, weight=wt, eps=self.eps)
+        return loss, (g if with_gradient else None)
+
 
 class FocalPressureLoss:
     def __init__(self, cfg, amplitude_target, eps=EPS):
@@ -418,6 +477,7 @@ class FocalPressureLoss:
         self.target_indices = [nearest_grid_index(cfg, p) for p in cfg.targets]
         self.target_weight = float(cfg.training.target_weight)
         self.background_weight = float(cfg.training.background_weight)
+        self._dev = {}
 
     def evaluate(self, field, with_cotangent=True, focal_field=None):
         amp = np.abs(field)
@@ -436,6 +496,21 @@ class FocalPressureLoss:
             if with_cotangent:
                 cot = cot + self.background_weight/field.numel()*field
         return loss, cot
+
+    def evaluate_basis(self, G, w, with_gradient=True, focal_field=None):
+        """(loss, G^H c) on the device basis G: the background energy in one
+        fused pass (hz_basis_loss), the focal term on the target rows."""
+        from . import basis as dev_basis
+        shape = tuple(self.cfg.domain.grid_size)
+        rows = _device_rows(self._dev, "rows", G, lambda: _fortran_rows(self.target_indices, shape))
+        loss, g = _basis_focal(G, rows, w, self.target_weight, self.eps, focal_field)
+        if self.background_weight > 0:
+            lb, gb = dev_basis.fused_loss(G, w, "energy", scale=self.background_weight/G.shape[0])
+            loss = loss + lb
+            g = gb if g is None else g + gb
+        if with_gradient and g is None:
+            g = torch.zeros(G.shape[1], dtype=torch.complex128, device=G.device)
+        return loss, (g if with_gradient else None)
 
 
 class FocalContrastLoss:
@@ -456,6 +531,7 @@ class FocalContrastLoss:
         if not np.any(self.background_mask):
             raise ValueError("focal_contrast background mask is empty")
         self._mask_dev = None
+        self._dev = {}
 
     def evaluate(self, field, with_cotangent=True, focal_field=None):
         amp = np.abs(field)
@@ -486,8 +562,105 @@ class FocalContrastLoss:
         cot[mask] += self.sidelobe_weight*prob*field[mask]/(bg + self.eps)
         return loss, cot
 
+    def evaluate_basis(self, G, w, with_gradient=True, focal_field=None):
+        """(loss, G^H c) on the device basis G: the smooth-max sidelobe term in
+        one fused pass (hz_basis_loss, running-maximum softmax), the focal term
+        on the target rows."""
+        from . import basis as dev_basis
+        shape = tuple(self.cfg.domain.grid_size)
+        rows = _device_rows(self._dev, "rows", G, lambda: _fortran_rows(self.target_indices, shape))
+        mask = _device_rows(self._dev, "mask_rows", G,
+                            lambda: self.background_mask.ravel(order="F").astype(np.uint8))
+        loss, g = _basis_focal(G, rows, w, self.target_weight, self.eps, focal_field)
+        ls, gs = dev_basis.fused_loss(G, w, "softmax", mask=mask, eps=self.eps, scale=self.sidelobe_weight,
+                                      temperature=self.temperature,
+                                      count_a=int(np.count_nonzero(self.background_mask)))
+        g = gs if g is None else g + gs
+        return loss + ls, (g if with_gradient else None)
+
+
+def build_ratio_masks(cfg, target_points, target_radius):
+    """(ball, rest) of the focal contrast ratio: nodes within target_radius of
+    a target (the grid from 0 of the reference's background mask,
+    phase_adjoint_optimizer.py:384-414) and the nodes of the upper half
+    z >= Lz/2 outside every ball."""
+    if target_radius <= 0:
+        raise ValueError("training.target_radius must be positive")
+    d = cfg.domain
+    shape = (d.nx, d.ny, d.nz)
+    axes = [np.linspace(0.0, d.box_size[a], shape[a]) for a in range(3)]
+    ball = np.zeros(shape, dtype=bool)
+    for p in np.asarray(target_points, float).reshape(-1, 3):
+        r2 = ((axes[0][:, None, None] - p[0])**2 + (axes[1][None, :, None] - p[1])**2
+              + (axes[2][None, None, :] - p[2])**2)
+        ball |= r2 <= target_radius*target_radius
+    rest = np.zeros(shape, dtype=bool)
+    rest[:, :, d.nz//2:] = True
+    rest &= ~ball
+    return ball, rest
+
+
+class FocalContrastRatioLoss:
+    """Focal contrast as a ratio, mean |u|^2 over the target ball over mean
+    |u|^2 over the rest of the upper half-domain (loss_type
+    "focal_contrast_ratio"; larger is better, so an optimiser minimises its
+    negative).  The reference's focal_contrast is the smooth-max form
+    (FocalContrastLoss); this is the form a 160^3 gradient workload states.
+    Cotangent (dL = Re(c^H du)): 2u/(n_ball mean_rest) on the ball,
+    -2 mean_ball u/(n_rest mean_rest^2) on the rest."""
+
+    def __init__(self, cfg, amplitude_target=None, eps=EPS):
+        tr = cfg.training
+        self.cfg = cfg
+        self.eps = float(eps)
+        self.target_weight = float(tr.target_weight)
+        self.ball, self.rest = build_ratio_masks(cfg, np.asarray(cfg.targets, float), float(tr.target_radius))
+        self.n_ball, self.n_rest = int(self.ball.sum()), int(self.rest.sum())
+        if not self.n_ball or not self.n_rest:
+            raise ValueError("focal_contrast_ratio needs nonempty ball and rest regions")
+        self._dev = {}
+
+    def _masks(self, like):
+        if "ball" not in self._dev or self._dev["ball"].device != like.device:
+            self._dev["ball"] = torch.as_tensor(self.ball, device=like.device)
+            self._dev["rest"] = torch.as_tensor(self.rest, device=like.device)
+        return self._dev["ball"], self._dev["rest"]
+
+    def evaluate(self, field, with_cotangent=True, focal_field=None):
+        p2 = np.abs(field)**2
+        na, nb = self.n_ball, self.n_rest
+        a, b = float(p2[self.ball].sum())/na, float(p2[self.rest].sum())/nb
+        if not with_cotangent:
+            return a/b, None
+        cot = np.zeros(field.shape, dtype=np.complex128)
+        cot[self.ball] = 2.0*field[self.ball]/(na*b)
+        cot[self.rest] = -2.0*a*field[self.rest]/(nb*b*b)
+        return a/b, cot
+
+    def evaluate_device(self, field, with_cotangent=True, focal_field=None):
+        ball, rest = self._masks(field)
+        p2 = field.real**2 + field.imag**2
+        na, nb = self.n_ball, self.n_rest
+        a, b = p2[ball].sum()/na, p2[rest].sum()/nb
+        if not with_cotangent:
+            return a/b, None
+        cot = torch.zeros_like(field, dtype=torch.complex128)
+        cot[ball] = 2.0*field[ball]/(na*b)
+        cot[rest] = -2.0*a*field[rest]/(nb*b*b)
+        return a/b, cot
+
+    def evaluate_basis(self, G, w, with_gradient=True, focal_field=None):
+        """(loss, G^H c) in one fused pass over the device basis G (hz_basis_loss)."""
+        from . import basis as dev_basis
+        cls = _device_rows(self._dev, "class_rows", G, lambda: (self.ball.ravel(order="F").astype(np.uint8)
+                                                                + 2*self.rest.ravel(order="F").astype(np.uint8)))
+        loss, g = dev_basis.fused_loss(G, w, "ratio", mask=cls, count_a=self.n_ball, count_b=self.n_rest)
+        return loss, (g if with_gradient else None)
+
 
 def make_loss(cfg, target=None):
+    if cfg.training.loss_type == "focal_contrast_ratio":
+        return FocalContrastRatioLoss(cfg)
     target = target or AmplitudeTarget.from_config(cfg)
     kind = cfg.training.loss_type
     if kind == "field_match":
@@ -496,10 +669,28 @@ def make_loss(cfg, target=None):
         return FocalPressureLoss(cfg, target)
     if kind == "focal_contrast":
         return FocalContrastLoss(cfg, target)
+    if kind == "focal_contrast_ratio":
+        return FocalContrastRatioLoss(cfg, target)
     raise ValueError("Unknown training.loss_type")
 
 
 # --------------------------------------------------------------- the objective
+def focal_gradient(response, focal, phases, target_weight, eps=EPS):
+    """Phase gradient of the focal term -w mean_t |f_t| evaluated on exact
+    samples f = R e^{i phi} of a (P, M) target response R
+    (src/algorithm/baselines/common.py:189-195): with c_t = -w/P f_t/(|f_t| + eps),
+    dL/dphi = Re(conj(R^H c) * i e^{i phi}).  numpy arrays or torch tensors."""
+    if isinstance(focal, torch.Tensor):
+        ph = torch.as_tensor(phases, dtype=torch.float64, device=focal.device)
+        cot = (-target_weight/focal.numel())*focal/(focal.abs() + eps)
+        adj = response.conj().transpose(0, 1) @ cot
+        return torch.real(adj.conj()*1j*torch.exp(1j*ph))
+    focal = np.asarray(focal)
+    cot = -target_weight/len(focal)*focal/(np.abs(focal) + eps)
+    adj = np.asarray(response).conj().T @ cot
+    return np.real(adj.conj()*1j*np.exp(1j*np.asarray(phases, np.float64)))
+
+
 def wrap_phases(phases):
     phases = np.asarray(phases, dtype=np.float64)
     if not np.isfinite(phases).all():
@@ -511,16 +702,19 @@ class PhaseObjective:
     """Loss and phase gradient of one scene, evaluated on the device.
 
     Mirrors ``PhaseProblem.evaluate`` (src/algorithm/baselines/common.py:275-343):
-    phases wrapped mod 2 pi, field u(phi), loss with the exact focal samples
-    (trilinear target response) when targets exist, gradient = phase VJP of the
-    grid cotangent plus the focal gradient (common.py:189-195), finiteness
-    checks and an evaluation history.  The field comes from the basis G when
-    one is given (``basis.field``/``phase_vjp``), otherwise from one forward
-    solve and the gradient from one adjoint solve (A^H lambda = c,
-    G^H c = a * B^T lambda) without forming G.
+    phases wrapped mod 2 pi, the loss with the exact focal samples (trilinear
+    target response) when targets exist, gradient = phase VJP of the grid
+    cotangent plus the focal gradient (common.py:189-195), finiteness checks
+    and an evaluation history.  Nothing of field size leaves the device: with
+    a basis G (moved to the device if needed) each evaluation is one fused
+    pass over G that yields the loss and G^H c together
+    (``evaluate_basis`` of the loss); without one, a forward solve gives u,
+    the torch loss its cotangent c, and an adjoint solve G^H c
+    (A^H lambda = c, G^H c = a * B^T lambda) without forming G.  Per
+    evaluation only the loss and the M phase gradients reach the host.
     """
 
-    def __init__(self, solver, basis=None, engine_kwargs=None):
+    def __init__(self, solver, basis=None, engine_kwargs=None, device="cuda"):
         self.solver = solver
         self.cfg = solver.cfg
         self.basis = basis
@@ -530,62 +724,68 @@ class PhaseObjective:
         self.history = []
         self.field_evaluations = 0
         self.vjp_evaluations = 0
-        if basis is None:
+        self.engine = None
+        if basis is not None:
+            if basis.basis_gpu is None:
+                basis.to_gpu(device)
+            self.device = basis.basis_gpu.device
+        else:
             from .autograd import PhaseEngine
             solver.assemble_matrix()
             kw = dict(precision="c64", preconditioner="multigrid" if solver.operator.__dict__.get("hierarchy")
                       else "jacobi", rtol=1e-8, maxiter=100000, chunk=8)
             kw.update(engine_kwargs or {})
             self.engine = PhaseEngine(solver.operator, solver.transducers.amplitudes, **kw)
-        if self.cfg.targets and self.cfg.training.loss_type != "field_match":
-            self.response = (basis.target_response(np.asarray(self.cfg.targets)) if basis is not None
-                             else solver.target_response(np.asarray(self.cfg.targets)))
+            self.device = torch.device(solver.operator.device)
+        kind = self.cfg.training.loss_type
+        if self.cfg.targets and kind not in ("field_match", "focal_contrast_ratio"):
+            pts = np.asarray(self.cfg.targets)
+            resp = basis.target_response(pts) if basis is not None else solver.target_response(pts)
+            self.response = torch.as_tensor(np.asarray(resp, np.complex128), device=self.device)
         else:
             self.response = None
 
-    def _field(self, phases):
+    def _loss_and_adjoint(self, ph, w, gradient, focal):
+        """(loss 0-d tensor, G^H c or None) for phases ph, w = e^{i ph}, on the device."""
+        kw = {"focal_field": focal} if focal is not None else {}
         if self.basis is not None:
-            return self.basis.field(phases)
-        w = torch.as_tensor(phases, device=self.solver.operator.device)
-        u = self.engine.forward(w.reshape(1, -1))[0]
-        return u.cpu().numpy().reshape(self.shape, order="F")
-
-    def _vjp(self, cot, phases):
-        if self.basis is not None:
-            return self.basis.phase_vjp(cot, phases)
-        c = torch.as_tensor(np.asarray(cot, np.complex128).reshape(1, -1, order="F"), device=self.solver.operator.device)
-        resp = self.engine.response_adjoint(c)[0].cpu().numpy()
-        return np.real(np.conj(resp)*(1j*np.exp(1j*phases)))
+            return self.loss_model.evaluate_basis(self.basis.basis_gpu, w, gradient, **kw)
+        from .autograd import grid_view
+        u = self.engine.forward(ph[None])[0]
+        if not bool(torch.isfinite(torch.view_as_real(u)).all()):
+            raise FloatingPointError("Non-finite field or loss")
+        loss, cot = self.loss_model.evaluate_device(grid_view(u, self.shape), with_cotangent=gradient, **kw)
+        if not gradient:
+            return loss, None
+        c = cot.permute(2, 1, 0).reshape(1, -1)
+        return loss, self.engine.response_adjoint(c)[0]
 
     def evaluate(self, phases, gradient=False, stage="optimization"):
         phases = wrap_phases(phases)
         if phases.shape != (self.size,):
             raise ValueError(f"Expected {self.size} phases")
         self.field_evaluations += 1
-        u = self._field(phases)
-        focal = self.response @ np.exp(1j*phases) if self.response is not None else None
-        kw = {"focal_field": focal} if focal is not None else {}
-        if isinstance(self.loss_model, AmplitudeFieldLoss):
-            loss, cot = self.loss_model.evaluate(u, with_cotangent=gradient)
-        else:
-            loss, cot = self.loss_model.evaluate(u, with_cotangent=gradient, **kw)
-        if not np.isfinite(loss) or not np.isfinite(u).all():
-            raise FloatingPointError("Non-finite field or loss")
+        ph = torch.as_tensor(phases, device=self.device)
+        w = torch.exp(1j*ph)
+        focal = self.response @ w if self.response is not None else None
+        loss_t, g = self._loss_and_adjoint(ph, w, gradient, focal)
         grad = None
         if gradient:
             self.vjp_evaluations += 1
-            grad = self._vjp(cot, phases)
+            gt = torch.real(torch.conj(g)*(1j*w))
             if focal is not None:
                 lm = self.loss_model
-                fc = -lm.target_weight/len(focal)*focal/(np.abs(focal) + lm.eps)
-                adj = self.response.conj().T @ fc
-                grad = grad + np.real(adj.conj()*1j*np.exp(1j*phases))
-            if not np.isfinite(grad).all():
-                raise FloatingPointError("Non-finite phase gradient")
+                gt = gt + focal_gradient(self.response, focal, ph, lm.target_weight, lm.eps)
+            grad = gt.cpu().numpy()
+        loss = float(loss_t)
+        if not np.isfinite(loss):
+            raise FloatingPointError("Non-finite field or loss")
+        if grad is not None and not np.isfinite(grad).all():
+            raise FloatingPointError("Non-finite phase gradient")
         self.history.append({"evaluation": self.field_evaluations, "vjp_evaluations": self.vjp_evaluations,
-                             "loss": float(loss), "stage": stage,
+                             "loss": loss, "stage": stage,
                              "gradient_norm": float(np.linalg.norm(grad)) if grad is not None else None})
-        return float(loss), grad
+        return loss, grad
 
     def gradient_check(self, phases, step=1.0e-3, count=8):
         """Central differences on the first min(count, M) emitters with the
