@@ -290,6 +290,18 @@ typedef struct hz_basis_loss_desc {
 hz_status hz_basis_loss(const void *G, int64_t ldg, int64_t nrows, int32_t m, const void *w,
                         const hz_basis_loss_desc *desc, double *loss, void *g, void *u, void *stream);
 
+/* hz_basis_loss without any allocation: the same kernels, launch shape and
+ * results (bit for bit), with the per-wave partials in a static device array
+ * of 8 MiB + 64 KiB that each GPU holds from the first call to process end.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving or optimiser loops,
+ * harnesses that forbid hipMalloc).  Limits as hz_basis_loss (m <= 512;
+ * arguments whose partials would exceed the array are rejected with
+ * HZ_BAD_ARGUMENT).  Every call on one GPU shares that GPU's array: calls
+ * that overlap on one device (two streams, two threads) race. */
+hz_status hz_basis_loss_static(const void *G, int64_t ldg, int64_t nrows, int32_t m, const void *w,
+                               const hz_basis_loss_desc *desc, double *loss, void *g, void *u, void *stream);
+
 /* Material gradient, replacing ConservativeHelmholtz.material_vjp
  * (src/algorithm/physics/conservative_helmholtz.py:151-175): for
  * dL = -Re(lambda^H dA u) writes dL/drho and dL/dc per node (FP64, N each)

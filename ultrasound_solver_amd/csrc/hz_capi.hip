@@ -1862,6 +1862,55 @@ hz_status hz_basis_loss(const void *G, int64_t ldg, int64_t nrows, int32_t m, co
     });
 }
 
+}  // extern "C"
+
+// Scratch of hz_basis_loss_static: the per-wave partials of one fused loss
+// launch (at most BASIS_LOSS_PARTIAL_BYTES of accumulators plus two doubles
+// for each of at most 4096 waves), in device memory that exists from module
+// load to process end, one copy per GPU.  Its host-side name is not a device
+// address: host code uses only the pointer hipGetSymbolAddress returns.
+constexpr size_t STATIC_SCRATCH_BYTES = (size_t)BASIS_LOSS_PARTIAL_BYTES + 4096 * LOSS_SCALARS * sizeof(double);
+static __device__ __align__(256) unsigned char scratch[STATIC_SCRATCH_BYTES];
+
+namespace {
+
+// Device address of `scratch` on the current device, looked up once per
+// device (each GPU holds its own copy at its own address).
+void *static_scratch() {
+    static std::mutex mu;
+    static std::map<int, void *> ptrs;
+    int dev = 0;
+    HZ_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = ptrs.find(dev);
+    if (it != ptrs.end()) return it->second;
+    void *ptr = nullptr;
+    HZ_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw HipFailure(hipErrorInvalidSymbol, "hipGetSymbolAddress returned a null scratch address");
+    ptrs[dev] = ptr;
+    return ptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+hz_status hz_basis_loss_static(const void *G, int64_t ldg, int64_t nrows, int32_t m, const void *w,
+                               const hz_basis_loss_desc *desc, double *loss, void *g, void *u, void *stream) {
+    return guarded([&] {
+        check_basis_loss_args(G, ldg, nrows, m, w, loss, g);
+        const BasisLossArgs a = basis_loss_args(desc, nrows);
+        const size_t need = basis_loss_bytes(nrows, m, desc->kind);
+        if (need > STATIC_SCRATCH_BYTES)
+            throw BadArgument("the partial sums need " + std::to_string(need) + " bytes, more than the " +
+                              std::to_string(STATIC_SCRATCH_BYTES) + "-byte static scratch");
+        // Every launch writes each partial and wave scalar it later reads, so
+        // nothing in the scratch needs clearing between calls.
+        launch_basis_loss(G, ldg, nrows, m, w, a, desc->kind, static_scratch(), loss, g, u,
+                          static_cast<hipStream_t>(stream));
+    });
+}
+
 hz_status hz_material_vjp(hz_operator *op, const void *u, const void *lambda, int32_t nrhs, int64_t ld, double *g_rho,
                           double *g_c, void *stream) {
     return guarded([&] {
