@@ -171,7 +171,6 @@ def archive_fixture(reference):
             "first_loss": np.float64(log[0]["loss"]),
             "geometric_metrics": np.asarray(str(data["geometric_metrics"])),
         }
-    (HERE/"archive_config.yaml").write_text((folder/"config.yaml").read_text())
     np.savez_compressed(HERE/"archive_fixture.npz", **out)
 
 

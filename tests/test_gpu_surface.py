@@ -65,13 +65,28 @@ def test_z_transfer_entry_point_matches_the_recurrence():
 
 
 def test_tables_and_transforms(pair):
+    """Transfer ratios and Schur complement per mode; the device's
+    eigenvector tables diagonalise the oracle's axis operators; the
+    transform kernels apply exactly those tables.  Eigenvectors are fixed
+    only up to a scale per mode (LAPACK's normalisation can pick another
+    phase for an operator that differs in its last bits), so modal
+    coefficients are compared through the device's own tables."""
     dev, ref = pair
     close(dev.h.cpu().numpy(), ref.h, 1e-12)
     close(dev.schur.cpu().numpy(), ref.schur, 1e-12)
+    vx, ix, vy, iy = (t.cpu().numpy() for t in (dev.vx, dev.ix, dev.vy, dev.iy))
+    for axis, v, inv in ((0, vx, ix), (1, vy, iy)):
+        a = ref.axes[axis]
+        lam = np.diag(inv @ a @ v)
+        scale = np.abs(a).max()*np.linalg.cond(v)
+        assert np.abs(inv @ v - np.eye(len(v))).max() < 1e-12*np.linalg.cond(v)
+        assert np.abs(a @ v - v*lam[None, :]).max() < 1e-12*scale
+        assert np.abs(inv @ a @ v - np.diag(lam)).max() < 1e-12*scale
     rng = np.random.default_rng(1)
     planes = rng.standard_normal((3, *SHAPE[:2])) + 1j*rng.standard_normal((3, *SHAPE[:2]))
-    close(dev.transform(planes[0]).cpu().numpy(), ref.transform(planes[0]))
-    close(dev.inverse_transform(planes).cpu().numpy(), np.stack([ref.inverse_transform(p) for p in planes]))
+    close(dev.transform(planes[0]).cpu().numpy(), ix @ planes[0] @ iy.T)
+    close(dev.inverse_transform(planes).cpu().numpy(), np.stack([vx @ p @ vy.T for p in planes]))
+    close(dev.inverse_transform(dev.transform(planes[1])).cpu().numpy(), planes[1])
 
 
 def test_solve_flux_planes_and_samples(pair):
@@ -82,7 +97,9 @@ def test_solve_flux_planes_and_samples(pair):
     ref.solve_flux(m, phases, amps)
     assert diag["iterations"] == 0 and len(diag["eigenvector_condition"]) == 2
     close(dev.surface_rhs.cpu().numpy(), ref.surface_rhs, 1e-13)
-    close(dev.surface_modes.cpu().numpy(), ref.surface_modes)
+    # Modal coefficients depend on the eigenvector scaling: compare the plane
+    # they synthesise instead.
+    close(dev.inverse_transform(dev.surface_modes).cpu().numpy(), ref.surface)
     for z in range(SHAPE[2]):
         close(dev.plane(z), ref.plane(z))
     close(dev.sample(POINTS), ref.sample(POINTS))

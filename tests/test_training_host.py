@@ -199,29 +199,69 @@ def _homogeneous_cfg(tmp_path):
 
 def test_target_points_and_response_cache(tmp_path):
     """Target validation (inside the domain, outside the PML) and the response
-    cache file: round trip, and a fingerprint that changes with the scene."""
+    cache file: round trip, and a key that changes with the scene."""
+    from ultrasound_solver_amd.response_cache import ResponseCache
     from ultrasound_solver_amd.solvers import HelmholtzSolver
     cfg = _homogeneous_cfg(tmp_path)
     s = HelmholtzSolver(cfg)
     h = cfg.domain.dx
     with pytest.raises(ValueError, match="inside the PDE domain"):
-        s._local_target_points(np.array([[0.0, 0.0, -h]]))
+        s._local_points(np.array([[0.0, 0.0, -h]]))
     with pytest.raises(ValueError, match="lies in PML"):
-        s._local_target_points(np.array([[2*h, 12*h, 5*h]]))
+        s._local_points(np.array([[2*h, 12*h, 5*h]]))
+    with pytest.raises(ValueError, match="lies in PML"):
+        s._local_points(np.array([[12*h, 21*h, 5*h]]))
     pts = np.array([[12*h, 12*h, 8*h], [10*h, 13*h, 3*h]])
-    assert np.allclose(s._local_target_points(pts), pts)
-    path, cached = s._read_target_response_cache()
-    assert cached is None and path == tmp_path/"responses.npz"
+    assert np.allclose(s._local_points(pts), pts)
+    cache = s._response_cache()
+    assert cache.path == tmp_path/"responses.npz" and len(cache.points) == 0
+    assert list(cache.lookup(pts)) == [-1, -1]
     resp = np.arange(8).reshape(2, 4)*(1 + 2j)
-    s._write_target_response_cache(path, pts, resp)
-    _, (p2, r2) = s._read_target_response_cache()
-    np.testing.assert_array_equal(p2, pts)
-    np.testing.assert_array_equal(r2, resp)
+    cache.store(pts, resp)
+    again = s._response_cache()
+    np.testing.assert_array_equal(again.points, pts)
+    np.testing.assert_array_equal(again.rows, resp)
+    assert list(again.lookup(pts[::-1] + 5e-13)) == [1, 0]
     s._ensure_target_surface_solver = None  # every point is cached
     np.testing.assert_array_equal(s.target_response(pts[::-1]), resp[::-1])
     cfg2 = _homogeneous_cfg(tmp_path)
     cfg2.boundary_conditions["+x"] = "open"
-    assert HelmholtzSolver(cfg2)._read_target_response_cache()[1] is None
+    assert len(HelmholtzSolver(cfg2)._response_cache().points) == 0
+    # A file with another key or malformed arrays reads as empty.
+    np.savez_compressed(tmp_path/"other.npz", fingerprint=np.asarray("x"), points_m=pts, response=resp)
+    assert len(ResponseCache(tmp_path/"other.npz", "y", 4).points) == 0
+    np.savez_compressed(tmp_path/"bad.npz", fingerprint=np.asarray("y"), points_m=pts, response=resp[:, :3])
+    assert len(ResponseCache(tmp_path/"bad.npz", "y", 4).points) == 0
+
+
+def test_scene_fingerprint_tracks_materials_and_files(tmp_path):
+    from ultrasound_solver_amd.solvers import HelmholtzSolver
+    cfg = _homogeneous_cfg(tmp_path)
+    s = HelmholtzSolver(cfg)
+    base = s.scene_fingerprint()
+    assert HelmholtzSolver(_homogeneous_cfg(tmp_path)).scene_fingerprint() == base
+    rho = np.full(s.shape, 1.2)
+    c = np.full(s.shape, 343.0)
+    s.set_material_fields(rho, c)
+    with_arrays = s.scene_fingerprint()
+    assert with_arrays != base
+    c[3, 4, 5] = 344.0
+    s.set_material_fields(rho, c)
+    assert s.scene_fingerprint() not in (base, with_arrays)
+    with pytest.raises(ValueError, match="finite and positive"):
+        s.set_material_fields(rho, -c)
+    path = tmp_path/"materials.npz"
+    np.savez(path, density=rho, sound_speed=c)
+    cfg3 = _homogeneous_cfg(tmp_path)
+    cfg3.physics.material_field_file = str(path)
+    s3 = HelmholtzSolver(cfg3)
+    first = s3.scene_fingerprint()
+    assert first == s3.scene_fingerprint()
+    np.savez(path, density=rho, sound_speed=c + 1.0)
+    assert s3.scene_fingerprint() != first
+    cfg.domain.grid_size = [27]*3
+    with pytest.raises(ValueError, match="construct a new solver"):
+        s.scene_fingerprint()
 
 
 def test_focal_contrast_ratio_loss_matches_oracle():

@@ -65,9 +65,45 @@ def axis_tables(shape, boundaries, pml=None):
     return weights, stretch
 
 
+def interior_bounds(box_size, dx, boundaries, pml=None):
+    """(lo, hi): per axis, the local coordinates between which a point is
+    outside every PML layer (a face with PML moves its bound inward by
+    cells * dx; helmholtz_solver.py:105-123 and phase_response_basis.py:153-164
+    reject target points beyond it)."""
+    cells, _, _, faces = pml_settings(pml, boundaries)
+    box = np.asarray(box_size, dtype=float)
+    lo, hi = np.zeros(3), box.copy()
+    if cells:
+        for f in faces:
+            axis = "xyz".index(f[1])
+            if f[0] == "-":
+                lo[axis] = cells*dx
+            else:
+                hi[axis] = box[axis] - cells*dx
+    return lo, hi
+
+
 def pml_node_mask(stretch, shape):
     sx, sy, sz = (np.imag(s) for s in stretch)
     return (sx[:, None, None] + sy[None, :, None] + sz[None, None, :]) > 0
+
+
+def face_admittance(spec, rho, c):
+    """Admittance Y of a face: 1/(rho c) for an open face (rho, c scalars or
+    the face's node arrays), 1/Z for an impedance face given as
+    {type: impedance, impedance: [Re Z, Im Z]} in Pa s/m, None otherwise
+    (conservative_helmholtz.py:89-103)."""
+    kind = boundary_type(spec)
+    if kind == "open":
+        return 1.0/(rho*c)
+    if kind != "impedance":
+        return None
+    if not isinstance(spec, dict):
+        raise ValueError("impedance requires {type: impedance, impedance: [real, imag]}")
+    z = np.asarray(spec.get("impedance"), dtype=float)
+    if z.shape != (2,) or not np.isfinite(z).all() or z[0] < 0 or not np.any(z):
+        raise ValueError("impedance must be finite [real>=0, imag], nonzero, in Pa s/m")
+    return 1.0/complex(z[0], z[1])
 
 
 def face_tables(rho, c, dx, omega, boundaries, weights, stretch):
@@ -89,16 +125,7 @@ def face_tables(rho, c, dx, omega, boundaries, weights, stretch):
         sel = [slice(None)]*3
         sel[axis] = 0 if index % 2 == 0 else -1
         sel = tuple(sel)
-        if kind == "open":
-            admittance = 1.0/(rho[sel]*c[sel])
-        else:
-            if not isinstance(spec, dict):
-                raise ValueError("impedance requires {type: impedance, impedance: [real, imag]}")
-            z = np.asarray(spec.get("impedance"), dtype=float)
-            if z.shape != (2,) or not np.isfinite(z).all() or z[0] < 0 or not np.any(z):
-                raise ValueError("impedance must be finite [real>=0, imag], nonzero, in Pa s/m")
-            admittance = 1.0/complex(z[0], z[1])
-        term = 1j*omega*area*admittance
+        term = 1j*omega*area*face_admittance(spec, rho[sel], c[sel])
         tables.append(np.ascontiguousarray(term.ravel(order="F"), dtype=np.complex128))
     return kinds, tables
 

@@ -28,6 +28,7 @@ plain form's, 30 as in the reference).
 """
 import hashlib
 import json
+import os
 import time
 import warnings
 from dataclasses import asdict
@@ -185,37 +186,38 @@ class HelmholtzSolver:
         return self._obs_sdf
 
     def set_material_fields(self, density, sound_speed):
-        arrays = tuple(np.array(a, dtype=np.float64, copy=True) for a in (density, sound_speed))
-        if any(a.shape != self.shape or not np.isfinite(a).all() or np.any(a <= 0) for a in arrays):
-            raise ValueError("Material fields must match the grid and be finite and positive")
+        """Per-node density and sound speed (helmholtz_solver.py:238-247), copied;
+        their digest is taken here, once, for the scene fingerprint."""
+        fields = []
+        for a in (density, sound_speed):
+            a = np.array(a, dtype=np.float64, copy=True)
+            if a.shape != self.shape or not np.all(np.isfinite(a) & (a > 0)):
+                raise ValueError("Material fields must match the grid and be finite and positive")
+            fields.append(a)
         if self.cfg.obstacles or self.cfg.physics.material_field_file:
             raise ValueError("Choose one material input: arrays, NPZ file, or obstacle")
-        self._material_fields = arrays
-        digest = hashlib.sha256()
-        for a in arrays:
-            digest.update(a.tobytes())
-        self._material_digest = digest.hexdigest()
+        self._material_fields = tuple(fields)
+        self._material_digest = _digest_bytes(*(a.tobytes() for a in fields))
         self.invalidate_factorization()
 
     def scene_fingerprint(self):
-        """sha256 of the matrix-defining inputs (helmholtz_solver.py:249-271); the
-        material arrays are hashed once when set rather than on every call."""
+        """Key of everything that defines A and B (helmholtz_solver.py:249-271):
+        one digest per part -- the configuration blocks, the source layout,
+        the content of a material file and the material arrays -- combined.
+        A file's digest is reused while its size, inode and modification time
+        stay the same, so repeated solves do not re-read it."""
         if asdict(self.cfg.domain) != self._grid_definition or asdict(self.cfg.specs) != self._source_definition:
             raise ValueError("Grid or source specification changed; construct a new solver")
-        digest = hashlib.sha256(json.dumps({
-            "model": self.MODEL_VERSION, "physics": asdict(self.cfg.physics),
-            "domain": asdict(self.cfg.domain), "specs": asdict(self.cfg.specs),
-            "origin_m": self.origin.tolist(), "centers": (self.transducers.centers - self.origin).tolist(),
-            "boundaries": self.cfg.boundary_conditions, "obstacles": self.cfg.obstacles,
-            "pml": self.cfg.solver.get("pml", {}),
-        }, sort_keys=True).encode())
-        if self.cfg.physics.material_field_file:
-            with open(self.cfg.physics.material_field_file, "rb") as fh:
-                for chunk in iter(lambda: fh.read(1 << 20), b""):
-                    digest.update(chunk)
+        cfg = self.cfg
+        parts = [self.MODEL_VERSION,
+                 _digest_json([asdict(cfg.physics), asdict(cfg.domain), asdict(cfg.specs)]),
+                 _digest_json([cfg.boundary_conditions, cfg.obstacles, cfg.solver.get("pml", {})]),
+                 _digest_json([self.origin.tolist(), (self.transducers.centers - self.origin).tolist()])]
+        if cfg.physics.material_field_file:
+            parts.append(_file_digest(cfg.physics.material_field_file))
         if self._material_digest is not None:
-            digest.update(self._material_digest.encode())
-        return digest.hexdigest()
+            parts.append(self._material_digest)
+        return _digest_bytes(*(p.encode() for p in parts))
 
     def source_diagnostics(self):
         centers = (self.transducers.centers - self.origin)[:, :2]
@@ -415,71 +417,32 @@ class HelmholtzSolver:
             self.shape, d.dx, self.cfg.frequency, self.cfg.physics.sound_speed, self.cfg.physics.medium_density,
             self.cfg.boundary_conditions, pml=self.cfg.solver.get("pml", {}), device=self.device)
 
-    def _local_target_points(self, points):
-        """Physical points -> local coordinates, rejected outside the domain or
-        inside the PML (helmholtz_solver.py:105-123)."""
+    def _local_points(self, points):
+        """Physical target points -> coordinates relative to the domain origin;
+        a point outside the domain, or inside a PML layer (1e-12 m of slack),
+        is rejected (helmholtz_solver.py:105-123)."""
         points = np.asarray(points, dtype=float)
+        box = np.asarray(self.cfg.domain.box_size, dtype=float)
         local = points - self.origin
-        extent = np.asarray(self.cfg.domain.box_size, dtype=float)
-        if (points.ndim != 2 or points.shape[1] != 3 or not np.isfinite(points).all()
-                or np.any(local < -1e-12) or np.any(local > extent + 1e-12)):
+        if not (points.ndim == 2 and points.shape[1] == 3 and np.isfinite(points).all()
+                and np.all(local >= -1e-12) and np.all(local <= box + 1e-12)):
             raise ValueError("Targets must be finite physical coordinates inside the PDE domain")
-        pml = self.cfg.solver.get("pml", {})
-        cells = int(pml.get("cells", 0))
-        if cells:
-            faces = pml.get("faces", [f for f in physics.FACES
-                                      if physics.boundary_type(self.cfg.boundary_conditions[f]) == "open"])
-            thickness = cells*self.cfg.domain.dx
-            for face in faces:
-                axis = "xyz".index(face[1])
-                distance = local[:, axis] if face[0] == "-" else extent[axis] - local[:, axis]
-                if np.any(distance < thickness - 1e-12):
-                    raise ValueError("Target lies in PML; enlarge the physical interior")
+        lo, hi = physics.interior_bounds(box, self.cfg.domain.dx, self.cfg.boundary_conditions,
+                                         self.cfg.solver.get("pml", {}))
+        if np.any(local < lo - 1e-12) or np.any(local > hi + 1e-12):
+            raise ValueError("Target lies in PML; enlarge the physical interior")
         return local
 
-    def _target_response_cache_fingerprint(self):
-        """Scene fingerprint plus the implementation that computes responses
-        (helmholtz_solver.py:125-144): this module, the surface solver, the
-        host physics and the device library."""
-        from pathlib import Path
-        here = Path(__file__).resolve().parent
-        digest = hashlib.sha256()
-        for path in (here/"solvers.py", here/"surface.py", here/"physics.py", here/"csrc"/"hz_surface.hpp"):
-            digest.update(path.read_bytes())
-        payload = {"model": self.MODEL_VERSION, "scene": self.scene_fingerprint(),
-                   "origin_m": self.origin.tolist(), "implementation_sha256": digest.hexdigest()}
-        return hashlib.sha256(json.dumps(payload, sort_keys=True, separators=(",", ":")).encode()).hexdigest()
-
-    def _read_target_response_cache(self):
-        """(path, (points, response) or None) (helmholtz_solver.py:146-166)."""
-        from pathlib import Path
-        filename = str(self.cfg.solver.get("target_response_cache", "")).strip()
-        path = Path(filename) if filename else None
-        if path is None or not path.is_file():
-            return path, None
-        try:
-            with np.load(path, allow_pickle=False) as data:
-                if str(data["fingerprint"]) != self._target_response_cache_fingerprint():
-                    return path, None
-                points = np.asarray(data["points_m"], dtype=float)
-                response = np.asarray(data["response"], dtype=complex)
-        except (KeyError, OSError, ValueError):
-            return path, None
-        if (points.ndim != 2 or points.shape[1] != 3
-                or response.shape != (len(points), self.transducers.num_transducers)
-                or not np.isfinite(points).all() or not np.isfinite(response).all()):
-            return path, None
-        return path, (points, response)
-
-    def _write_target_response_cache(self, path, points, response):
-        """Atomic rewrite of the cache file (helmholtz_solver.py:168-177)."""
-        if path is None:
-            return
-        path.parent.mkdir(parents=True, exist_ok=True)
-        temporary = path.with_suffix(".tmp.npz")
-        np.savez_compressed(temporary, fingerprint=self._target_response_cache_fingerprint(),
-                            points_m=points, response=response)
-        temporary.replace(path)
+    def _response_cache(self):
+        """The target_response_cache file for this scene.  Its key covers the
+        scene fingerprint, the origin and the code that computes responses
+        (this module, the surface solver, the host tables and the device
+        surface kernels), so an edit to any of them starts a fresh cache."""
+        from .response_cache import ResponseCache
+        key = _digest_bytes(self.MODEL_VERSION.encode(), self.scene_fingerprint().encode(),
+                            np.asarray(self.origin, np.float64).tobytes(), _implementation_digest().encode())
+        name = str(self.cfg.solver.get("target_response_cache", "")).strip()
+        return ResponseCache(name or None, key, self.transducers.num_transducers)
 
     def _adjoint_target_response(self, local):
         """Rows of G at local points by reciprocity, one batched adjoint solve
@@ -495,55 +458,51 @@ class HelmholtzSolver:
         lam = np.conj(self.iterative_solver.solve(self.operator, t, adjoint=True).cpu().numpy().T)
         return (self.source_columns.T @ lam).T*self.transducers.amplitudes[None, :]
 
+    def _compute_responses(self, local):
+        if self._homogeneous():
+            self._ensure_target_surface_solver()
+            return self._target_surface_solver.flux_target_response(self._target_source_masses, local)
+        return self._adjoint_target_response(local)
+
     def target_response(self, points):
-        """(P, M) responses of the corrected discrete PDE at physical points
-        (helmholtz_solver.py:179-208): cached rows are reused, the rest come
-        from the exact homogeneous surface condensation on the device.  A
-        heterogeneous scene (obstacle, material arrays or file), where the
-        reference raises, is served by the adjoint solve instead."""
+        """(P, M) responses of the discrete PDE at physical points
+        (helmholtz_solver.py:179-208).  Points found in the response cache
+        file are served from it; the others are computed together -- by the
+        exact homogeneous surface condensation on the device, or, for a
+        heterogeneous scene (where the reference raises), by one batched
+        adjoint solve -- and appended to the file."""
         points = np.asarray(points, dtype=float)
-        local = self._local_target_points(points)
-        path, cached = self._read_target_response_cache()
-        channels = self.transducers.num_transducers
-        result = np.empty((len(points), channels), complex)
-        if cached is None:
-            cached_points, cached_response = np.empty((0, 3), float), np.empty((0, channels), complex)
-        else:
-            cached_points, cached_response = cached
-        missing = []
-        for index, point in enumerate(points):
-            matches = np.flatnonzero(np.all(np.isclose(cached_points, point, rtol=0, atol=1e-12), axis=1))
-            if matches.size:
-                result[index] = cached_response[matches[0]]
-            else:
-                missing.append(index)
-        if missing:
-            if self._homogeneous():
-                self._ensure_target_surface_solver()
-                computed = self._target_surface_solver.flux_target_response(self._target_source_masses,
-                                                                            local[missing])
-            else:
-                computed = self._adjoint_target_response(local[missing])
-            result[missing] = computed
-            self._write_target_response_cache(path, np.vstack((cached_points, points[missing])),
-                                              np.vstack((cached_response, computed)))
-        return result
+        local = self._local_points(points)
+        cache = self._response_cache()
+        hit = cache.lookup(points)
+        out = np.empty((len(points), self.transducers.num_transducers), dtype=np.complex128)
+        out[hit >= 0] = cache.rows[hit[hit >= 0]]
+        todo = np.flatnonzero(hit < 0)
+        if todo.size:
+            fresh = self._compute_responses(local[todo])
+            out[todo] = fresh
+            cache.store(points[todo], fresh)
+        return out
+
+    def _phase_vector(self, phases):
+        values = self.transducers.phases if phases is None else np.asarray(phases, dtype=float)
+        if values.shape != (self.transducers.num_transducers,) or not np.isfinite(values).all():
+            raise ValueError("phases must match the transducer array and be finite")
+        return values
 
     def field_at(self, points, phases=None):
-        values = self.transducers.phases if phases is None else np.asarray(phases, dtype=float)
-        if values.shape != (self.transducers.num_transducers,) or not np.isfinite(values).all():
-            raise ValueError("phases must match the transducer array and be finite")
-        return self.target_response(points) @ np.exp(1j*values)
+        """Field at physical points for the given (or current) phases, from the
+        target responses (helmholtz_solver.py:210-214)."""
+        return self.target_response(points) @ np.exp(1j*self._phase_vector(phases))
 
     def field_at_direct(self, points, phases=None):
-        """Surface solve for the phases, then trilinear samples
-        (helmholtz_solver.py:216-222)."""
-        values = self.transducers.phases if phases is None else np.asarray(phases, dtype=float)
-        if values.shape != (self.transducers.num_transducers,) or not np.isfinite(values).all():
-            raise ValueError("phases must match the transducer array and be finite")
+        """Field at physical points from one surface solve with these phases
+        and trilinear samples of its planes (helmholtz_solver.py:216-222)."""
+        values = self._phase_vector(phases)
+        local = self._local_points(points)
         self._ensure_target_surface_solver()
         self._target_surface_solver.solve_flux(self._target_source_masses, values)
-        return self._target_surface_solver.sample(self._local_target_points(points))
+        return self._target_surface_solver.sample(local)
 
     def build_phase_response_basis(self):
         from .training import PhaseResponseBasis
@@ -553,3 +512,38 @@ class HelmholtzSolver:
 
 
 HelmholtzDirectSolver = HelmholtzSolver
+
+
+def _digest_bytes(*chunks):
+    h = hashlib.sha256()
+    for c in chunks:
+        h.update(len(c).to_bytes(8, "little"))
+        h.update(c)
+    return h.hexdigest()
+
+
+def _digest_json(obj):
+    return _digest_bytes(json.dumps(obj, sort_keys=True, separators=(",", ":")).encode())
+
+
+_FILE_DIGESTS = {}
+
+
+def _file_digest(name):
+    """sha256 of a file's content, memoised by (path, inode, size, mtime)."""
+    st = os.stat(name)
+    key = (os.path.abspath(name), st.st_ino, st.st_size, st.st_mtime_ns)
+    if key not in _FILE_DIGESTS:
+        h = hashlib.sha256()
+        with open(name, "rb") as fh:
+            for block in iter(lambda: fh.read(1 << 20), b""):
+                h.update(block)
+        _FILE_DIGESTS[key] = h.hexdigest()
+    return _FILE_DIGESTS[key]
+
+
+def _implementation_digest():
+    """Digest of the sources that compute target responses."""
+    here = os.path.dirname(os.path.abspath(__file__))
+    names = ("solvers.py", "surface.py", "physics.py", "response_cache.py", os.path.join("csrc", "hz_surface.hpp"))
+    return _digest_bytes(*(_file_digest(os.path.join(here, n)).encode() for n in names))

@@ -4,8 +4,7 @@ reference's ``HomogeneousSurfaceSolver``
 
 In a homogeneous medium over a reflecting -z baffle the finite-volume
 operator is the Kronecker sum of three 1-D axis operators.  The x and y
-operators are diagonalised on the host (LAPACK, as the reference does,
-:20-38); every (x, y) mode then owns a tridiagonal z system whose solution is
+operators are diagonalised on the host (LAPACK); every (x, y) mode then owns a tridiagonal z system whose solution is
 u[j] = H[j-1] u[0], with the transfer ratios H from a backward continued
 fraction and u[0] from the surface Schur complement.  The device holds the
 eigenvector matrices, H and the Schur complement (``hz_surface_*`` in
@@ -20,62 +19,70 @@ test oracle), and ``transform`` / ``inverse_transform`` / ``surface_rhs`` /
 tensors where the reference returns CuPy arrays.
 """
 import ctypes
+import functools
 import hashlib
-from collections import OrderedDict
 
 import numpy as np
 import scipy.linalg as la
+import scipy.sparse as sps
 import torch
 
 from . import _capi, physics
 
 FACES = physics.FACES
-_EIGEN_CACHE = OrderedDict()
 
 
-def _cached_eigendecomposition(matrix):
-    """(eigenvalues, eigenvectors, inverse, 1-norm condition), the last four
-    distinct axis operators kept (homogeneous_surface.py:20-38)."""
-    m = np.ascontiguousarray(matrix)
-    key = (m.shape, m.dtype.str, hashlib.sha256(m.view(np.uint8)).digest())
-    hit = _EIGEN_CACHE.get(key)
-    if hit is not None:
-        _EIGEN_CACHE.move_to_end(key)
-        return hit
-    values, vectors = la.eig(m)
-    inverse = la.inv(vectors)
-    result = (values, vectors, inverse, float(la.norm(vectors, 1)*la.norm(inverse, 1)))
-    _EIGEN_CACHE[key] = result
-    while len(_EIGEN_CACHE) > 4:
-        _EIGEN_CACHE.popitem(last=False)
-    return result
+def axis_operators(shape, dx, omega, density, sound_speed, boundaries, pml):
+    """The three 1-D operators whose Kronecker sum, plus k^2, is the
+    homogeneous finite-volume operator divided by (1/rho) prod_a w_a s_a, and
+    each axis' PML stretch s.
 
-
-def axis_matrix(n, dx, omega, density, sound_speed, faces, pml, axis):
-    """The 1-D operator of one axis, divided by its control-volume weights,
-    and that axis' PML stretch (homogeneous_surface.py:85-110): the axis'
-    two faces become -x/+x of a thin n x 2 x 2 grid so that the production
-    PML and face validation apply."""
-    translated = {"-x": faces.get(FACES[2*axis], "reflecting"), "+x": faces.get(FACES[2*axis + 1], "reflecting")}
-    if any(physics.boundary_type(v) not in {"open", "reflecting", "impedance"} for v in translated.values()):
+    Along axis a: L u_i = (1/(w_i s_i)) sum_{edges i-j} e_ij (u_j - u_i) with
+    e_ij = 1/(0.5 (s_i + s_j) h^2), plus i omega rho Y/(h w_i s_i) at an open
+    (Y = 1/(rho c)) or impedance (Y = 1/Z) end.  The weights w, stretches s
+    and admittances come from the production tables (physics.axis_tables,
+    physics.face_admittance) of the full grid, so the PML and face checks of
+    the 3-D operator apply.  Soft faces have no such separable form."""
+    kinds = [physics.boundary_type(boundaries.get(f, "reflecting")) for f in FACES]
+    if any(k not in ("open", "reflecting", "impedance") for k in kinds):
         raise ValueError("Surface diagnostic supports open/reflecting/impedance faces")
-    pml = dict(pml or {})
-    original = pml.get("faces", [f for f in FACES if physics.boundary_type(faces.get(f, "reflecting")) == "open"])
-    local = {**pml, "faces": [("-x", "+x")[i] for i in range(2) if FACES[2*axis + i] in original]}
-    weights, stretch = physics.axis_tables((n, 2, 2), translated, local)
-    w, s = weights[0], stretch[0]
-    edge = 1.0/(0.5*(s[:-1] + s[1:])*dx*dx)
-    diagonal = np.zeros(n, complex)
-    diagonal[:-1] -= edge
-    diagonal[1:] -= edge
-    matrix = np.diag(diagonal) + np.diag(edge, 1) + np.diag(edge, -1)
-    matrix /= (w*s)[:, None]
-    for i, face in ((0, "-x"), (-1, "+x")):
-        kind = physics.boundary_type(translated[face])
-        if kind in {"open", "impedance"}:
-            z = density*sound_speed if kind == "open" else complex(*translated[face]["impedance"])
-            matrix[i, i] += 1j*omega*density/(z*dx*w[i]*s[i])
-    return matrix, s
+    weights, stretch = physics.axis_tables(shape, boundaries, pml or {})
+    out = []
+    for axis in range(3):
+        w, s = weights[axis], stretch[axis]
+        e = 1.0/(0.5*(s[:-1] + s[1:])*dx*dx)
+        diag = -np.concatenate((e, [0.0])) - np.concatenate(([0.0], e))
+        for end, face in ((0, FACES[2*axis]), (-1, FACES[2*axis + 1])):
+            y = physics.face_admittance(boundaries.get(face, "reflecting"), density, sound_speed)
+            if y is not None:
+                diag[end] += 1j*omega*density*y/dx
+        band = sps.diags([e, diag, e], [-1, 0, 1], shape=(len(w), len(w))).toarray()
+        out.append((band/(w*s)[:, None], s))
+    return out
+
+
+@functools.lru_cache(maxsize=8)
+def _axis_modes(shape, dx, omega, density, sound_speed, faces_key, pml_key, axis):
+    """Eigen-decomposition (values, vectors, inverse, 1-norm condition number
+    of the vectors) of one axis operator, cached by the definition of the
+    axis, so that the x and y axes of a square grid, and repeated solvers of
+    one scene, decompose once."""
+    boundaries = {f: (dict(v) if isinstance(v, tuple) else v) for f, v in faces_key}
+    matrix = axis_operators(shape, dx, omega, density, sound_speed, boundaries, dict(pml_key))[axis][0]
+    values, vectors = la.eig(matrix)
+    inverse = la.inv(vectors)
+    return values, vectors, inverse, float(la.norm(vectors, 1)*la.norm(inverse, 1))
+
+
+def _freeze(mapping):
+    """Hashable form of a boundary or PML mapping (dict values become sorted tuples)."""
+    def item(v):
+        if isinstance(v, dict):
+            return tuple(sorted((k, item(x)) for k, x in v.items()))
+        if isinstance(v, list):
+            return tuple(item(x) for x in v)
+        return v
+    return tuple(sorted((k, item(v)) for k, v in (mapping or {}).items()))
 
 
 def _fortran(a):
@@ -106,13 +113,14 @@ class HomogeneousSurfaceSolver:
         self.device = torch.device(device)
         self.boundaries = boundaries.copy()
         omega = 2*np.pi*frequency
-        axes = [axis_matrix(n, dx, omega, density, sound_speed, boundaries, pml or {}, a)
-                for a, n in enumerate(self.shape)]
+        axes = axis_operators(self.shape, self.dx, omega, density, sound_speed, boundaries, pml or {})
         self.axes = [a[0] for a in axes]
         self.surface_pml = (axes[0][1].imag[:, None] + axes[1][1].imag[None, :] + axes[2][1].imag[0]) > 0
         self.surface_rhs = self.surface_modes = self.surface = None
-        ex, vx, ix, cond_x = _cached_eigendecomposition(self.axes[0])
-        ey, vy, iy, cond_y = _cached_eigendecomposition(self.axes[1])
+        key = (self.shape, self.dx, float(omega), float(density), float(sound_speed), _freeze(boundaries),
+               _freeze(pml))
+        ex, vx, ix, cond_x = _axis_modes(*key, 0)
+        ey, vy, iy, cond_y = _axis_modes(*key, 1)
         self.eigen_condition = [cond_x, cond_y]
         shift = ex[:, None] + ey[None, :] + (omega/sound_speed)**2
         z = self.axes[2]

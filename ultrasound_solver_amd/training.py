@@ -156,22 +156,13 @@ class PhaseResponseBasis:
             yield s, torch.as_tensor(np.asarray(self.basis[s:s + self.voxel_chunk_size]), device=dev)
 
     def _check_outside_pml(self, points):
-        from .physics import FACES, boundary_type
+        from .physics import interior_bounds
         cfg = self.solver.cfg
-        pml = cfg.solver.get("pml", {}) or {}
-        cells = int(pml.get("cells", 0))
-        if not cells:
-            return
-        faces = pml.get("faces", [f for f in FACES
-                                  if boundary_type(cfg.boundary_conditions.get(f, "reflecting")) == "open"])
-        origin = np.asarray(cfg.domain.origin_m, float)
-        upper = np.asarray(cfg.domain.box_size, float)
-        for face in faces:
-            axis = "xyz".index(face[1])
-            local = points[:, axis] - origin[axis]
-            dist = local if face[0] == "-" else upper[axis] - local
-            if np.any(dist < cells*cfg.domain.dx):
-                raise ValueError("Target lies in PML; enlarge the physical interior")
+        lo, hi = interior_bounds(cfg.domain.box_size, cfg.domain.dx, cfg.boundary_conditions,
+                                 cfg.solver.get("pml", {}))
+        local = points - np.asarray(cfg.domain.origin_m, float)
+        if np.any(local < lo) or np.any(local > hi):
+            raise ValueError("Target lies in PML; enlarge the physical interior")
 
     def field(self, phases, return_numpy=True):
         """u = G e^{i phi} on the device (phase_response_basis.py:177-206);
