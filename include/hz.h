@@ -118,6 +118,10 @@ typedef struct hz_gmres_opts {
     int32_t arnoldi_block;      /* 1: one Arnoldi step at a time; 2: two
                                    basis vectors per Gram-Schmidt pass
                                    (mixed form only)                         */
+    double relative_rtol;       /* 0: the reference's acceptance only; > 0:
+                                   also ||b-Ax||/||b|| <= relative_rtol in
+                                   FP64, which the mixed form's refinement
+                                   then aims for as well                     */
 } hz_gmres_opts;
 
 typedef struct hz_gmres_stats {
@@ -129,7 +133,8 @@ typedef struct hz_gmres_stats {
     double relative_residual;   /* ||Ax-b|| / ||b||       (FP64)             */
     double backward_error;      /* max_i |Ax-b|_i / (|A||x|+|b|)_i           */
     int32_t finite;             /* all entries of x finite                   */
-    int32_t converged;          /* reference acceptance test passed          */
+    int32_t converged;          /* reference acceptance test passed (and the
+                                   relative residual, when relative_rtol > 0) */
 } hz_gmres_stats;
 
 /* One level of the shifted-Laplacian multigrid preconditioner: the same

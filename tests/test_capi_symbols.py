@@ -28,7 +28,7 @@ def test_struct_layouts_match_header():
     from ultrasound_solver_amd import _capi
     # int32 x3, pad, 2 doubles, 2 ptrs, 3+3 ptrs, int32 x6, 6 ptrs
     assert ctypes.sizeof(_capi.GridDesc) == 16 + 16 + 16 + 48 + 24 + 48
-    assert ctypes.sizeof(_capi.GmresOpts) == 4*2 + 8*2 + 4*4 + 8 + 4*3 + 4  # 4 bytes of tail padding
+    assert ctypes.sizeof(_capi.GmresOpts) == 4*2 + 8*2 + 4*4 + 8 + 4*3 + 4 + 8  # 4 bytes of padding before relative_rtol
     assert ctypes.sizeof(_capi.LevelDesc) == 16 + 16 + 24*3 + 48
     assert ctypes.sizeof(_capi.GmresStats) == 4*4 + 8*3 + 4*2
     assert ctypes.sizeof(_capi.SurfaceDesc) == 4*3 + 4 + 8 + 8*8  # 4 bytes of padding before dx

@@ -149,6 +149,29 @@ def test_mixed_gmres_refines_to_fp64_accuracy(soft):
         assert okr.scaled_residual(a, got, b[:, r]) <= 1e-8
 
 
+@pytest.mark.parametrize("preconditioner", ["jacobi", "multigrid"])
+def test_mixed_gmres_relative_residual_target(preconditioner):
+    """relative_rtol: the mixed form refines until ||b - A x||/||b|| meets it
+    as well (FP64, recomputed here on the oracle matrix); a target the
+    refinement budget cannot reach is reported as not converged."""
+    op, a = _solver_pair(14, seed=3)
+    if preconditioner == "multigrid":
+        op.set_multigrid()
+    b = _point_rhs(op.n, 3, 11)
+    x, st = op.gmres(b, restart=10, maxiter=6000, rtol=1e-6, precision="c64", chunk=3,
+                     preconditioner=preconditioner, relative_rtol=1e-10)
+    for r in range(3):
+        got = x[r].cpu().numpy()
+        rel = np.linalg.norm(b[:, r] - a @ got)/np.linalg.norm(b[:, r])
+        assert st[r]["converged"] == 1, st[r]
+        assert rel <= 1e-10
+        assert st[r]["relative_residual"] == pytest.approx(rel, rel=1e-6, abs=1e-16)
+        assert okr.scaled_residual(a, got, b[:, r]) <= 1e-6
+    _, st = op.gmres(b[:, :1], restart=10, maxiter=6000, rtol=1e-6, precision="c64", chunk=1,
+                     preconditioner=preconditioner, relative_rtol=1e-10, max_refinements=0)
+    assert st[0]["converged"] == 0 and st[0]["relative_residual"] > 1e-10
+
+
 @pytest.mark.parametrize("precision", ["c128", "c64"])
 def test_adjoint_solve_matches_direct(precision):
     op, a = _solver_pair(12, seed=2, soft=True)

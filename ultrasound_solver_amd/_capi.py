@@ -39,7 +39,8 @@ class GmresOpts(ctypes.Structure):
     _fields_ = [("restart", c_int32), ("maxiter", c_int32), ("rtol", c_double),
                 ("backward_error_tol", c_double), ("precision", c_int32), ("adjoint", c_int32),
                 ("chunk", c_int32), ("max_refinements", c_int32), ("inner_rtol", c_double),
-                ("use_graph", c_int32), ("preconditioner", c_int32), ("arnoldi_block", c_int32)]
+                ("use_graph", c_int32), ("preconditioner", c_int32), ("arnoldi_block", c_int32),
+                ("relative_rtol", c_double)]
 
 
 class LevelDesc(ctypes.Structure):

@@ -37,7 +37,7 @@ def _device_tensor(x, dtype, device):
 
 def gmres_options(restart=30, maxiter=3000, rtol=1.0e-6, backward_error_tol=1.0e-4, precision="c128",
                   adjoint=False, chunk=1, max_refinements=10, inner_rtol=1.0e-5, preconditioner="jacobi",
-                  use_graph=False, arnoldi_block=None):
+                  use_graph=False, arnoldi_block=None, relative_rtol=0.0):
     if arnoldi_block is None:
         arnoldi_block = 2 if precision == "c64" else 1
     return _capi.GmresOpts(restart=int(restart), maxiter=int(maxiter), rtol=float(rtol),
@@ -47,7 +47,7 @@ def gmres_options(restart=30, maxiter=3000, rtol=1.0e-6, backward_error_tol=1.0e
                            inner_rtol=float(inner_rtol), use_graph=int(bool(use_graph)),
                            preconditioner={"jacobi": _capi.PRE_JACOBI,
                                            "multigrid": _capi.PRE_MULTIGRID}[preconditioner],
-                           arnoldi_block=int(arnoldi_block))
+                           arnoldi_block=int(arnoldi_block), relative_rtol=float(relative_rtol))
 
 
 def gmres_batch(operators, rhs, workers=4, **options):
@@ -179,18 +179,21 @@ class HelmholtzOperator:
 
     def gmres(self, b, restart=30, maxiter=3000, rtol=1.0e-6, backward_error_tol=1.0e-4,
               precision="c128", adjoint=False, chunk=1, max_refinements=10, inner_rtol=1.0e-5,
-              preconditioner="jacobi", raise_on_failure=False, use_graph=False, arnoldi_block=None):
+              preconditioner="jacobi", raise_on_failure=False, use_graph=False, arnoldi_block=None,
+              relative_rtol=0.0):
         """Solve A x = b (or A^H x = b) for each column; returns (x, stats).
         arnoldi_block: basis vectors per Gram-Schmidt pass, 2 by default in
         the mixed form (same iterates up to rounding, 43 % less basis traffic
-        per cycle), 1 in the plain form (the reference's Arnoldi)."""
+        per cycle), 1 in the plain form (the reference's Arnoldi).
+        relative_rtol > 0 also requires ||b - A x|| / ||b|| <= relative_rtol
+        (FP64) for "converged", and the mixed form refines until it holds."""
         if arnoldi_block is None:
             arnoldi_block = 2 if precision == "c64" else 1
         b = self.as_columns(b, torch.complex128)
         k = b.shape[0]
         x = torch.empty_like(b)
         opts = gmres_options(restart, maxiter, rtol, backward_error_tol, precision, adjoint, chunk,
-                             max_refinements, inner_rtol, preconditioner, use_graph, arnoldi_block)
+                             max_refinements, inner_rtol, preconditioner, use_graph, arnoldi_block, relative_rtol)
         stats = (_capi.GmresStats*k)()
         status = _capi.lib().hz_gmres(self.handle, ctypes.byref(opts), ctypes.c_void_p(b.data_ptr()),
                                       ctypes.c_void_p(x.data_ptr()), k, self.n, stats, _stream())
