@@ -397,6 +397,37 @@ def test_material_vjp_matches_reference_fixture(fix, name):
                                atol=1e-12*np.abs(ref_c).max())
 
 
+@pytest.mark.parametrize("guard", [None, "1"])
+@pytest.mark.parametrize("pre", ["jacobi", "multigrid"])
+def test_two_step_blocks_keep_one_step_iterations(monkeypatch, pre, guard):
+    """Even grid width (the paired 16-byte kernels): two-step blocks take
+    within 5 % of the one-step iterations per column and solve to the direct
+    solution.  guard "1" (HZ_BLOCK_GUARD) sends every block through the
+    measured fix-up, which orthogonalises W2 against W1 with their measured
+    inner product instead of the predicted coefficient."""
+    from ultrasound_solver_amd.operator import HelmholtzOperator
+    if guard is not None:
+        monkeypatch.setenv("HZ_BLOCK_GUARD", guard)
+    n = 22
+    rho, c, bnd, pml = _scene(n, seed=8)
+    op = HelmholtzOperator(rho, c, 1.07e-3, 2*np.pi*40e3, bnd, pml, device="cuda:0")
+    if pre == "multigrid":
+        op.set_multigrid()
+    a = ohz.OracleOperator(rho, c, 1.07e-3, 2*np.pi*40e3, bnd, pml).matrix()
+    b = _point_rhs(op.n, 4, 31)
+    direct = okr.direct_solve(a, b)
+    kw = dict(restart=10, rtol=1e-8, precision="c64", chunk=4, maxiter=20000, preconditioner=pre)
+    x2, st2 = op.gmres(b, arnoldi_block=2, **kw)
+    x1, st1 = op.gmres(b, arnoldi_block=1, **kw)
+    it1, it2 = [s["iterations"] for s in st1], [s["iterations"] for s in st2]
+    print(f"{pre} guard {guard}: one-step {it1}, two-step {it2}", flush=True)
+    for r in range(4):
+        assert st2[r]["converged"] == 1, st2[r]
+        got = x2[r].cpu().numpy()
+        assert np.linalg.norm(got - direct[:, r])/np.linalg.norm(direct[:, r]) < 1e-5
+        assert abs(it2[r] - it1[r]) <= max(2, 0.05*it1[r]), (it1, it2)
+
+
 @pytest.mark.parametrize("pre,adjoint", [("jacobi", False), ("multigrid", False), ("multigrid", True)])
 def test_two_step_arnoldi_blocks_solve_to_direct(pre, adjoint):
     """Mixed-form GMRES with two basis vectors per Gram-Schmidt pass reaches

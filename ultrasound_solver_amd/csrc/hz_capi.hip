@@ -43,6 +43,16 @@ bool verbose() {
     return on;
 }
 
+// Threshold of the two-step blocks' predicted intra-block coefficient (see
+// k_block_alpha); HZ_BLOCK_GUARD overrides it per solve (1 sends every
+// block through the measured fix-up, for tests).
+double block_guard() {
+    const char *v = std::getenv("HZ_BLOCK_GUARD");
+    if (!v || !v[0]) return BLOCK_GUARD;
+    const double g = std::atof(v);
+    return g > 0 ? g : BLOCK_GUARD;
+}
+
 struct HipFailure : std::runtime_error {
     hipError_t code;
     HipFailure(hipError_t c, const std::string &what) : std::runtime_error(what), code(c) {}
@@ -140,7 +150,7 @@ struct hz_operator {
     DevBuf src_col_node, src_col_ptr, src_col_val;
     // Cached GMRES workspace (grow-only).
     DevBuf V, X, Bh, dpart, npart, cpart, red, Hm, gc, gs, g, scales, coef, resid, tol, steps, live, member;
-    DevBuf Hu, coef2, alpha, gram;  // two-step Arnoldi blocks
+    DevBuf Hu, coef2, alpha, gram, bfix;  // two-step Arnoldi blocks
     DevBuf Z;                       // multigrid form: preconditioned basis slots, restart x k x n
     DevBuf rhs32;
     std::unique_ptr<Multigrid> mg;
@@ -586,6 +596,8 @@ void gmres_cycles(hz_operator *op, int mode_ram, int k, int restart, int maxiter
     st.Hu = op->Hu.as<cx<double>>();
     st.coef2 = op->coef2.as<cx<double>>();
     st.alpha = op->alpha.as<cx<double>>();
+    st.fix = op->bfix.as<int>();
+    st.guard = block_guard();
     int *member = op->member.as<int>();
 
     hipLaunchKernelGGL((k_zero<T>), dim3(vec_blocks(n), k), dim3(256), 0, s, X, n, n);
@@ -687,6 +699,12 @@ void gmres_cycles(hz_operator *op, int mode_ram, int k, int restart, int maxiter
                                        (const cx<double> *)st.coef2, maxv, st.live, gram, chunk,
                                        (const cx<double> *)st.alpha);
                 HZ_LAUNCHED();
+                hipLaunchKernelGGL(k_block_fixcoef, dim3(k), dim3(256), 0, cs_stream, st, (const double *)gram,
+                                   (int)nblk);
+                HZ_LAUNCHED();
+                hipLaunchKernelGGL((k_block_fixup<float>), dim3(nblk, k), dim3(256), 0, cs_stream, st, n,
+                                   (const cx<float> *)w1, w2, gram, chunk);
+                HZ_LAUNCHED();
                 hipLaunchKernelGGL(k_block_finish, dim3(k), dim3(256), 0, cs_stream, st, (const double *)gram,
                                    (int)nblk, j, restart);
                 HZ_LAUNCHED();
@@ -767,7 +785,8 @@ void reserve_workspace(hz_operator *op, int k, int restart, size_t elem) {
     op->coef.ensure((size_t)k * maxv * sizeof(cx<double>));
     op->coef2.ensure((size_t)k * maxv * sizeof(cx<double>));
     op->Hu.ensure((size_t)k * (restart + 1) * restart * sizeof(cx<double>));
-    op->alpha.ensure((size_t)k * sizeof(cx<double>));
+    op->alpha.ensure((size_t)2 * k * sizeof(cx<double>));
+    op->bfix.ensure((size_t)2 * k * sizeof(int));
     op->gram.ensure((size_t)k * nblk * 4 * sizeof(double));
     op->resid.ensure((size_t)k * sizeof(double));
     op->tol.ensure((size_t)k * sizeof(double));
@@ -886,6 +905,11 @@ void solve_mixed(hz_operator *op, const hz_gmres_opts &o, const cx<double> *b, i
     const cx<double> *m = o.adjoint ? op->ma64.as<cx<double>>() : op->mf64.as<cx<double>>();
     hipLaunchKernelGGL((k_zero<double>), dim3(vec_blocks(n), k), dim3(256), 0, s, x, ldx, n);
     HZ_LAUNCHED();
+    {
+        // Per-RHS count of two-step blocks that took the measured coefficient.
+        const std::vector<int> zeros((size_t)2 * k, 0);
+        HZ_HIP(hipMemcpyAsync(op->bfix.p, zeros.data(), zeros.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    }
     ChunkState cs;
     cs.k = k;
     cs.active.assign(k, 1);
@@ -960,6 +984,15 @@ void solve_mixed(hz_operator *op, const hz_gmres_opts &o, const cx<double> *b, i
                                (const int *)op->member.as<int>(), 1);
         }
         HZ_LAUNCHED();
+    }
+    if (verbose() && o.arnoldi_block == 2) {
+        std::vector<int> fixes((size_t)2 * k);
+        HZ_HIP(hipMemcpyAsync(fixes.data(), op->bfix.p, fixes.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+        HZ_HIP(hipStreamSynchronize(s));
+        for (int r = 0; r < k; ++r)
+            if (fixes[k + r])
+                std::fprintf(stderr, "[hz] rhs %d: %d two-step blocks took the measured coefficient\n", r,
+                             fixes[k + r]);
     }
     for (int r = 0; r < k; ++r) {
         hz_gmres_stats &st = stats[r];

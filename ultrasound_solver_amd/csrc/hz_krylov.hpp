@@ -198,7 +198,9 @@ struct GmresState {
     cx<double> *H;     // [k][restart+1][restart], Givens-rotated in place
     cx<double> *Hu;    // [k][restart+1][restart], unrotated (two-step blocks only)
     cx<double> *coef2; // second-target update coefficients [k][maxv] (two-step blocks)
-    cx<double> *alpha; // per-RHS intra-block coefficient (two-step blocks)
+    cx<double> *alpha; // per-RHS intra-block coefficients [2][k]: predicted, measured (two-step blocks)
+    int *fix;          // [2][k]: the block of RHS r takes the measured coefficient; blocks that did so
+    double guard;      // predict the intra-block coefficient only while |W1|^2 > guard |w1|^2
     double *gc;        // Givens cosines [k][restart]
     cx<double> *gs;    // Givens sines   [k][restart]
     cx<double> *g;     // rotated RHS    [k][restart+1]
@@ -332,7 +334,9 @@ __global__ __launch_bounds__(256) void k_dots2(const cx<T> *__restrict__ V, int 
 #pragma unroll
     for (int q = 0; q < G; ++q) are[q] = aim[q] = bre[q] = bim[q] = 0;
     const bool first = blockIdx.z == 0;
-    T waa = 0, wabr = 0, wabi = 0;
+    // |w1|^2 and w1^H w2 feed a difference (|W1|^2 = |w1|^2 - |h1|^2) that
+    // can cancel: they are summed in FP64 from the first product on.
+    double waa = 0, wabr = 0, wabi = 0;
     if constexpr (VW == 2) {
         static_assert(sizeof(T) == 4, "paired loads are for complex64");
         const float4 *a4 = reinterpret_cast<const float4 *>(ar);
@@ -340,9 +344,11 @@ __global__ __launch_bounds__(256) void k_dots2(const cx<T> *__restrict__ V, int 
         for (int64_t p = (p0 >> 1) + threadIdx.x; p < (p1 >> 1); p += 256) {
             const float4 av = a4[p], bv = b4[p];
             if (first) {
-                waa += av.x * av.x + av.y * av.y + av.z * av.z + av.w * av.w;
-                wabr += av.x * bv.x + av.y * bv.y + av.z * bv.z + av.w * bv.w;
-                wabi += av.x * bv.y - av.y * bv.x + av.z * bv.w - av.w * bv.z;
+                const double ax = av.x, ay = av.y, az = av.z, aw = av.w;
+                const double bx = bv.x, by = bv.y, bz = bv.z, bw = bv.w;
+                waa += ax * ax + ay * ay + az * az + aw * aw;
+                wabr += ax * bx + ay * by + az * bz + aw * bw;
+                wabi += ax * by - ay * bx + az * bw - aw * bz;
             }
 #pragma unroll
             for (int q = 0; q < G; ++q) {
@@ -359,9 +365,10 @@ __global__ __launch_bounds__(256) void k_dots2(const cx<T> *__restrict__ V, int 
         for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
             const cx<T> av = ar[p], bv = br[p];
             if (first) {
-                waa += norm2(av);
-                wabr += av.re * bv.re + av.im * bv.im;
-                wabi += av.re * bv.im - av.im * bv.re;
+                const cx<double> a = cast<double>(av), b = cast<double>(bv);
+                waa += norm2(a);
+                wabr += a.re * b.re + a.im * b.im;
+                wabi += a.re * b.im - a.im * b.re;
             }
 #pragma unroll
             for (int q = 0; q < G; ++q) {
@@ -389,9 +396,9 @@ __global__ __launch_bounds__(256) void k_dots2(const cx<T> *__restrict__ V, int 
         }
     }
     if (first) {
-        const double s0 = block_combine_256((double)wave_sum(waa), red);
-        const double s1 = block_combine_256((double)wave_sum(wabr), red);
-        const double s2 = block_combine_256((double)wave_sum(wabi), red);
+        const double s0 = block_combine_256(wave_sum(waa), red);
+        const double s1 = block_combine_256(wave_sum(wabr), red);
+        const double s2 = block_combine_256(wave_sum(wabi), red);
         if (threadIdx.x == 0) {
             double *o = wgram + ((int64_t)r * gridDim.x + blockIdx.x) * 4;
             o[0] = s0;
@@ -408,6 +415,8 @@ __global__ __launch_bounds__(256) void k_dots2(const cx<T> *__restrict__ V, int 
 // W2 - alpha W1 = (w2 - alpha w1) - sum (coef2_i - alpha coef_i) V_i
 // directly (coef2 is rewritten here).  The norms that scale the two new
 // vectors are measured in that pass, not predicted.
+constexpr double BLOCK_GUARD = 1.0e-2;  // default of GmresState::guard
+
 __global__ __launch_bounds__(256) void k_block_alpha(GmresState st, const double *__restrict__ wgram, int nblk, int j,
                                                      int maxv, int restart) {
     __shared__ double red[4];
@@ -430,8 +439,15 @@ __global__ __launch_bounds__(256) void k_block_alpha(GmresState st, const double
         w11 -= norm2(h1);
         w12 = w12 - conj(h1) * h2;
     }
-    const cx<double> al = w11 > 0 ? (1.0 / w11) * w12 : mk<double>(0, 0);
+    // When W1 keeps less than st.guard of |w1|^2 the prediction cancels
+    // (the dot products come from complex64 vectors): no intra-block
+    // coefficient is predicted, and the fix-up pass orthogonalises W2
+    // against W1 with their measured inner product instead.
+    const bool guard = !(w11 > st.guard * v[0]);
+    const cx<double> al = guard ? mk<double>(0, 0) : (1.0 / w11) * w12;
     st.alpha[r] = al;
+    st.fix[r] = guard ? 1 : 0;
+    if (guard) st.fix[gridDim.x + r] += 1;
     for (int i = 0; i <= j; ++i) st.coef2[r * maxv + i] = st.coef2[r * maxv + i] - al * st.coef[r * maxv + i];
 }
 
@@ -568,8 +584,52 @@ __global__ __launch_bounds__(256) void k_update2(const cx<T> *__restrict__ V, in
     }
 }
 
+// Blocks whose coefficient was not predicted (k_block_alpha's guard):
+// alpha'[r] = W1^H W2 / |W1|^2 from the update pass's Gram partials (grid (k)).
+__global__ __launch_bounds__(256) void k_block_fixcoef(GmresState st, const double *__restrict__ gram, int nblk) {
+    __shared__ double red[4];
+    const int r = blockIdx.x;
+    const int k = gridDim.x;
+    if (!st.live[r] || !st.fix[r]) return;
+    double v[3] = {0, 0, 0};
+    for (int b = threadIdx.x; b < nblk; b += 256) {
+        const double *o = gram + ((int64_t)r * nblk + b) * 4;
+        v[0] += o[0];
+        v[1] += o[1];
+        v[2] += o[2];
+    }
+    for (int q = 0; q < 3; ++q) v[q] = block_sum_256(v[q], red);
+    if (threadIdx.x == 0) st.alpha[k + r] = v[0] > 0 ? (1.0 / v[0]) * mk<double>(v[1], v[2]) : mk<double>(0, 0);
+}
+
+// W2 -= alpha' W1 in place for those blocks, and |W2|^2 measured again into
+// gram[(r*nblk + blk)*4 + 3] (grid (nblk, k)).
+template <typename T>
+__global__ __launch_bounds__(256) void k_block_fixup(GmresState st, int64_t n, const cx<T> *__restrict__ w1,
+                                                     cx<T> *__restrict__ w2, double *__restrict__ gram,
+                                                     int64_t chunk) {
+    __shared__ double red[4];
+    const int r = blockIdx.y;
+    const int k = gridDim.y;
+    if (!st.live[r] || !st.fix[r]) return;
+    const cx<T> al = cast<T>(st.alpha[k + r]);
+    const cx<T> *a = w1 + (int64_t)r * n;
+    cx<T> *b = w2 + (int64_t)r * n;
+    const int64_t p0 = blockIdx.x * chunk;
+    const int64_t p1 = min(n, p0 + chunk);
+    T g22 = 0;
+    for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
+        const cx<T> v = b[p] - al * a[p];
+        b[p] = v;
+        g22 += norm2(v);
+    }
+    const double s = block_combine_256(wave_sum((double)g22), red);
+    if (threadIdx.x == 0) gram[((int64_t)r * gridDim.x + blockIdx.x) * 4 + 3] = s;
+}
+
 // Finish the block (grid (k)) from the update pass's Gram partials:
-// r11 = |W1| and r22 = |W2 - alpha W1| as measured, r12 = alpha r11 (w2 =
+// r11 = |W1| and r22 = |W2 - alpha W1| as measured (alpha the predicted
+// coefficient plus, after a fix-up, the measured one), r12 = alpha r11 (w2 =
 // sum h2_i q_i + alpha r11 q_{j+1} + r22 q_{j+2} exactly, by construction),
 // the scales of slots j+1 and j+2, column j+1 of Hu by linearity, then the
 // Givens updates of columns j and j+1 (the second only if the RHS is still
@@ -588,7 +648,8 @@ __global__ __launch_bounds__(256) void k_block_finish(GmresState st, const doubl
     for (int q = 0; q < 2; ++q) v[q] = block_sum_256(v[q], red);
     if (threadIdx.x != 0) return;
     const double r11 = sqrt(v[0]), r22 = sqrt(v[1]);
-    const cx<double> r12 = r11 * st.alpha[r];
+    const cx<double> al = st.fix[r] ? st.alpha[r] + st.alpha[gridDim.x + r] : st.alpha[r];
+    const cx<double> r12 = r11 * al;
     st.scales[(int64_t)(j + 1) * gridDim.x + r] = r11 > 0 ? 1.0 / r11 : 0.0;
     st.scales[(int64_t)(j + 2) * gridDim.x + r] = r22 > 0 ? 1.0 / r22 : 0.0;
     cx<double> *hu = st.Hu + (int64_t)r * (restart + 1) * restart;
