@@ -265,7 +265,7 @@ def roofline_launches(op, args, shape=ROOFLINE_SHAPE):
     zb, ty, rp, nt, flat = op.SLAB_SHAPES[shape]
     return {"kernel": f"{slab_kernel_name(mode, zb, ty, rp, nt, flat)} (complex64 {' '.join(mode)} apply, {k} columns)",
             "bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s", "frac": achieved/8000.0,
-            "traffic": measured_traffic(args, mode, k, zb, ty, rp, nt, flat), "alg_bytes_per_launch": alg_bytes,
+            "traffic": measured_traffic(args, mode, k, shape), "alg_bytes_per_launch": alg_bytes,
             "launch_us": stencil_s*1e6, "launches": args.stencil_reps,
             "copy_reference": {"what": f"device copy of the same {k} complex64 columns",
                                "achieved": copy_gbps, "frac": copy_gbps/8000.0}}
@@ -329,7 +329,7 @@ def adjoint_gradient(op, scene, args):
                     f"{args.precond} GMRES({args.mixed_restart}) c64 + FP64 refinement, 1 phase vector"}
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "stencil_128_traffic.md")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "stencil_128_roofline.md")
 STENCIL_MODE_ID = {"RAM": 1, "RA": 4}
 
 
@@ -347,13 +347,14 @@ def stencil_mode(args):
     return "RA" if args.precond == "multigrid" else "RAM"
 
 
-def measured_traffic(args, mode, k, zb, ty, rp, nt, flat):
-    """HBM bytes per stencil launch from the FETCH_SIZE/WRITE_SIZE counter runs
-    of the same kernel, mode, column count and tile shape on the 128^3 scene,
-    scaled by the copy calibration recorded beside them; None otherwise."""
-    if args.n != 128 or not os.path.exists(TRAFFIC_PROFILE):
+def measured_traffic(args, mode, k, shape):
+    """HBM bytes per launch of the roofline kernel (tile shape, k columns of
+    the R A apply on the 128^3 scene) from the FETCH_SIZE / WRITE_SIZE
+    counter runs summarised beside its kernel trace, scaled by the copy
+    calibration recorded there; None for another kernel or grid."""
+    if args.n != 128 or mode != "RA" or not os.path.exists(TRAFFIC_PROFILE):
         return None
-    key = f"traffic_bytes_per_launch[mode={mode},k={k},zb={zb},ty={ty},rp={rp},nt={nt},flat={flat}]:"
+    key = f"traffic_bytes_per_launch[shape={shape},k={k}]:"
     with open(TRAFFIC_PROFILE) as f:
         for line in f:
             if line.startswith(key):
