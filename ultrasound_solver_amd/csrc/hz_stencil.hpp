@@ -81,6 +81,13 @@ __device__ __forceinline__ Nbr neighbours(int i, int j, int k, int nx, int ny, i
 // neighbour loads are issued unconditionally (an absent neighbour aliases P
 // and is weighted by zero) so that they overlap instead of serialising
 // behind per-neighbour branches.
+// 1 / (a + b) for the edge factor of two positive densities: exact
+// division in FP64 (the plain form reproduces the reference's matrix), the
+// hardware reciprocal (1 ulp) in complex64 arithmetic, where six correctly
+// rounded divisions per node were the bulk of the coefficient setup.
+__device__ __forceinline__ double inv_sum(double a, double b) { return 1.0 / (a + b); }
+__device__ __forceinline__ float inv_sum(float a, float b) { return __builtin_amdgcn_rcpf(a + b); }
+
 template <typename T, bool WANT_DIAG>
 __device__ __forceinline__ void edge_coefs(const OpView<T> &op, int i, int j, int k, int64_t p,
                                            const Nbr &nb, cx<T> c[6], cx<T> *diag) {
@@ -93,12 +100,12 @@ __device__ __forceinline__ void edge_coefs(const OpView<T> &op, int i, int j, in
     const cx<T> q2 = op.q[1][j > 0 ? j - 1 : 0], q3 = op.q[1][j < op.ny - 1 ? j : op.ny - 2];
     const cx<T> q4 = op.q[2][k > 0 ? k - 1 : 0], q5 = op.q[2][k < op.nz - 1 ? k : op.nz - 2];
     const cx<T> ayz = ay * az, axz = ax * az, axy = ax * ay;
-    c[0] = ((nb.has[0] ? T(1) : T(0)) / (rp + rq[0])) * (ayz * q0);
-    c[1] = ((nb.has[1] ? T(1) : T(0)) / (rp + rq[1])) * (ayz * q1);
-    c[2] = ((nb.has[2] ? T(1) : T(0)) / (rp + rq[2])) * (axz * q2);
-    c[3] = ((nb.has[3] ? T(1) : T(0)) / (rp + rq[3])) * (axz * q3);
-    c[4] = ((nb.has[4] ? T(1) : T(0)) / (rp + rq[4])) * (axy * q4);
-    c[5] = ((nb.has[5] ? T(1) : T(0)) / (rp + rq[5])) * (axy * q5);
+    c[0] = (nb.has[0] ? inv_sum(rp, rq[0]) : T(0)) * (ayz * q0);
+    c[1] = (nb.has[1] ? inv_sum(rp, rq[1]) : T(0)) * (ayz * q1);
+    c[2] = (nb.has[2] ? inv_sum(rp, rq[2]) : T(0)) * (axz * q2);
+    c[3] = (nb.has[3] ? inv_sum(rp, rq[3]) : T(0)) * (axz * q3);
+    c[4] = (nb.has[4] ? inv_sum(rp, rq[4]) : T(0)) * (axy * q4);
+    c[5] = (nb.has[5] ? inv_sum(rp, rq[5]) : T(0)) * (axy * q5);
     if (WANT_DIAG) {
         const cx<T> mw = axy * az;
         const T kp = op.kappa[p];
