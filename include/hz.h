@@ -316,6 +316,18 @@ hz_status hz_basis_loss_static(const void *G, int64_t ldg, int64_t nrows, int32_
 hz_status hz_material_vjp(hz_operator *op, const void *u, const void *lambda, int32_t nrhs, int64_t ld,
                           double *g_rho, double *g_c, void *stream);
 
+/* The multigrid transfers between level `level` (0 = the operator's grid)
+ * and level + 1 of the installed hierarchy, on nrhs complex64 vectors stored
+ * one after another (device pointers, fine vectors 16-byte aligned):
+ *   direction 0: out = P^T in   (restriction, fine -> coarse),
+ *   direction 1: out += P in    (prolongation, coarse -> fine),
+ * with P the trilinear prolongation of the even-node coarsening; identity
+ * rows (pressure-release faces) take no part, as in the V-cycle.  No
+ * counterpart in the reference: it exposes the preconditioner's transfers
+ * to tests and to callers composing their own cycles. */
+hz_status hz_mg_transfer(hz_operator *op, int32_t level, int32_t direction, const void *in, void *out, int32_t nrhs,
+                         void *stream);
+
 /* Free the cached solver workspace (Krylov basis, the multigrid form's
  * preconditioned basis, multigrid level vectors, captured graphs); the next
  * solve allocates what it needs.  Waits for the

@@ -235,33 +235,34 @@ int transfer_group(const MgLevel &C, int k) {
     return (k + groups - 1) / groups;
 }
 
-void launch_restrict(const MgLevel &L, const MgLevel &C, const cx<float> *r, int k, const int *live, hipStream_t s) {
-    // At most 4 slots per thread: 19.9 solves/s on the mixed 128^3 workload
-    // against 19.6 with 16 and 19.1 with 1.
-    const int rs = std::min(4, transfer_group(C, k));
-    const int nb = plane_blocks(C.nx, C.ny, C.nz, (k + rs - 1) / rs);
+void launch_restrict(const MgLevel &L, const MgLevel &C, const cx<float> *r, cx<float> *bc, int k, const int *live,
+                     hipStream_t s) {
+    // z-marching restriction: one workgroup per 256 coarse nodes of a plane,
+    // chunk of RZ_CHUNK coarse planes and slot.
+    const int64_t t = (int64_t)((C.nx * C.ny + 255) / 256) * ((C.nz + RZ_CHUNK - 1) / RZ_CHUNK) * k;
+    const int nb = (int)((t + 7) / 8 * 8);
     if (L.nx % 2 == 0)
-        hipLaunchKernelGGL((k_restrict<float, 1>), dim3(nb), dim3(256), 0, s, r, L.nx, L.ny, L.nz,
-                           C.bb.as<cx<float>>(), C.nx, C.ny, C.nz, k, rs, live, L.v.soft, nb);
+        hipLaunchKernelGGL((k_restrict_z<float, 1>), dim3(nb), dim3(256), 0, s, r, L.nx, L.ny, L.nz, bc, C.nx, C.ny,
+                           C.nz, k, live, L.v.soft, nb);
     else
-        hipLaunchKernelGGL((k_restrict<float, 0>), dim3(nb), dim3(256), 0, s, r, L.nx, L.ny, L.nz,
-                           C.bb.as<cx<float>>(), C.nx, C.ny, C.nz, k, rs, live, L.v.soft, nb);
+        hipLaunchKernelGGL((k_restrict_z<float, 0>), dim3(nb), dim3(256), 0, s, r, L.nx, L.ny, L.nz, bc, C.nx, C.ny,
+                           C.nz, k, live, L.v.soft, nb);
     HZ_LAUNCHED();
 }
 
-void launch_prolong(const MgLevel &L, const MgLevel &C, cx<float> *xf, int k, const int *live, const cx<float> *b,
-                    const cx<float> *dinv, float w, const float *bdiv, int conj_d, hipStream_t s) {
+void launch_prolong(const MgLevel &L, const MgLevel &C, const cx<float> *xc, cx<float> *xf, int k, const int *live,
+                    const cx<float> *b, const cx<float> *dinv, float w, const float *bdiv, int conj_d, hipStream_t s) {
     // One thread per coarse cell and group of rs slots: the slots of a group
     // share the cell's weights and first-sweep factors.  Groups are as large
     // as the grid allows while it still has >= 2048 workgroups.
     const int rs = transfer_group(C, k);
     const int nb = plane_blocks(C.nx, C.ny, C.nz, (k + rs - 1) / rs);
     if (L.nx % 2 == 0)
-        hipLaunchKernelGGL((k_prolong<float, 1>), dim3(nb), dim3(256), 0, s, (const cx<float> *)C.xx.as<cx<float>>(),
-                           C.nx, C.ny, C.nz, xf, L.nx, L.ny, L.nz, k, rs, live, L.v.soft, b, dinv, w, bdiv, conj_d, nb);
+        hipLaunchKernelGGL((k_prolong<float, 1>), dim3(nb), dim3(256), 0, s, xc, C.nx, C.ny, C.nz, xf, L.nx, L.ny,
+                           L.nz, k, rs, live, L.v.soft, b, dinv, w, bdiv, conj_d, nb);
     else
-        hipLaunchKernelGGL((k_prolong<float, 0>), dim3(nb), dim3(256), 0, s, (const cx<float> *)C.xx.as<cx<float>>(),
-                           C.nx, C.ny, C.nz, xf, L.nx, L.ny, L.nz, k, rs, live, L.v.soft, b, dinv, w, bdiv, conj_d, nb);
+        hipLaunchKernelGGL((k_prolong<float, 0>), dim3(nb), dim3(256), 0, s, xc, C.nx, C.ny, C.nz, xf, L.nx, L.ny,
+                           L.nz, k, rs, live, L.v.soft, b, dinv, w, bdiv, conj_d, nb);
     HZ_LAUNCHED();
 }
 
@@ -425,14 +426,14 @@ void vcycle(Multigrid &mg, int l, const cx<float> *b, cx<float> *x, int k, const
         cx<float> *r = L.tt.as<cx<float>>();
         launch_slab<MODE, SLAB_RESID0>(L.shape[2], L.v, b, L.n, r, L.n, k, live, nullptr, nullptr, dinv, mg.weight, bdiv,
                                        s);
-        launch_restrict(L, C, r, k, live, s);
+        launch_restrict(L, C, r, C.bb.as<cx<float>>(), k, live, s);
         vcycle<MODE>(mg, l + 1, C.bb.as<cx<float>>(), C.xx.as<cx<float>>(), k, live, s);
         // Post-sweeps ping-pong between x and the level scratch; start where
         // the last one lands in x.
         const int post = l == 0 ? mg.post : mg.post_coarse;
         cx<float> *cur = (post % 2 == 0) ? x : L.tt.as<cx<float>>();
         cx<float> *alt = (cur == x) ? L.tt.as<cx<float>>() : x;
-        launch_prolong(L, C, cur, k, live, b, dinv, mg.weight, bdiv, (int)(MODE == MODE_AH), s);
+        launch_prolong(L, C, C.xx.as<cx<float>>(), cur, k, live, b, dinv, mg.weight, bdiv, (int)(MODE == MODE_AH), s);
         for (int it = 0; it < post; ++it) {
             level_smooth<MODE>(L, cur, b, alt, mg.weight, k, live, bdiv, s);
             std::swap(cur, alt);
@@ -442,9 +443,9 @@ void vcycle(Multigrid &mg, int l, const cx<float> *b, cx<float> *x, int k, const
             cx<float> *t = L.tt.as<cx<float>>();
             launch_slab<MODE, SLAB_RESID>(L.shape[1], L.v, x, L.n, t, L.n, k, live, nullptr, b, nullptr, 0.0f, bdiv,
                                           s);
-            launch_restrict(L, C, t, k, live, s);
+            launch_restrict(L, C, t, C.bb.as<cx<float>>(), k, live, s);
             vcycle<MODE>(mg, l + 1, C.bb.as<cx<float>>(), C.xx.as<cx<float>>(), k, live, s);
-            launch_prolong(L, C, x, k, live, nullptr, dinv, mg.weight, bdiv, 0, s);
+            launch_prolong(L, C, C.xx.as<cx<float>>(), x, k, live, nullptr, dinv, mg.weight, bdiv, 0, s);
             cx<float> *c2 = x, *a2 = t;
             for (int it = 0; it < post; ++it) {
                 level_smooth<MODE>(L, c2, b, a2, mg.weight, k, live, bdiv, s);
@@ -471,9 +472,9 @@ void vcycle(Multigrid &mg, int l, const cx<float> *b, cx<float> *x, int k, const
     if (!coarsest) {
         MgLevel &C = *mg.levels[l + 1];
         launch_slab<MODE, SLAB_RESID>(L.shape[1], L.v, cur, L.n, alt, L.n, k, live, nullptr, b, nullptr, 0.0f, bdiv, s);
-        launch_restrict(L, C, alt, k, live, s);
+        launch_restrict(L, C, alt, C.bb.as<cx<float>>(), k, live, s);
         vcycle<MODE>(mg, l + 1, C.bb.as<cx<float>>(), C.xx.as<cx<float>>(), k, live, s);
-        launch_prolong(L, C, cur, k, live, nullptr, dinv, mg.weight, bdiv, 0, s);
+        launch_prolong(L, C, C.xx.as<cx<float>>(), cur, k, live, nullptr, dinv, mg.weight, bdiv, 0, s);
         for (int it = 0; it < post; ++it) {
             level_smooth<MODE>(L, cur, b, alt, mg.weight, k, live, bdiv, s);
             std::swap(cur, alt);
@@ -1995,6 +1996,29 @@ hz_status hz_release_workspace(hz_operator *op) {
         HZ_HIP(hipDeviceSynchronize());
         drop_graphs(op);
         release_workspace(op);
+    });
+}
+
+hz_status hz_mg_transfer(hz_operator *op, int32_t level, int32_t direction, const void *in, void *out, int32_t nrhs,
+                         void *stream) {
+    return guarded([&] {
+        check_op(op);
+        if (!op->mg) throw BadArgument("no multigrid hierarchy set (hz_set_multigrid)");
+        const int levels = (int)op->mg->levels.size();
+        if (level < 0 || level + 1 >= levels) throw BadArgument("level must name a level with a coarser one below it");
+        if (direction != 0 && direction != 1) throw BadArgument("direction must be 0 (restrict) or 1 (prolong)");
+        if (!in || !out || nrhs < 1) throw BadArgument("in, out and nrhs >= 1 are required");
+        const MgLevel &L = *op->mg->levels[level], &C = *op->mg->levels[level + 1];
+        auto a16 = [](const void *ptr) { return ((uintptr_t)ptr & 15) == 0; };
+        if (L.nx % 2 == 0 && !a16(direction == 0 ? in : out))
+            throw BadArgument("even-width fine vectors must be 16-byte aligned");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        if (direction == 0)
+            launch_restrict(L, C, static_cast<const cx<float> *>(in), static_cast<cx<float> *>(out), nrhs, nullptr,
+                            s);
+        else
+            launch_prolong(L, C, static_cast<const cx<float> *>(in), static_cast<cx<float> *>(out), nrhs, nullptr,
+                           nullptr, nullptr, 0.0f, nullptr, 0, s);
     });
 }
 

@@ -141,92 +141,142 @@ __device__ __forceinline__ PlaneTile plane_tile(int nx, int ny, int nz, int nslo
     return t;
 }
 
-// Coarse right-hand side bc = P^T rf, one thread per (coarse node, group of
-// rs slots): the node's taps and weights are computed once for the group.
-// Grid: flattened plane tiles of the coarse grid times ceil(k / rs) slot
-// groups, padded to a multiple of 8 (nblocks).  Along x the taps are fine 2i-1, 2i, 2i+1 with weights 1/2, 1,
-// 1/2 (0 where absent or a coarse node); along y and z likewise.  PAIR (even
-// fine x-size): each lane loads its (2i, 2i+1) pair with one 16-byte access
-// and takes 2i-1 from the previous lane (DPP), lane 0 loading it itself:
-// 9 loads per slot instead of 27.  Lanes past the plane's end run on clamped
-// indices (they feed the shifts) and do not store.  Identity rows
-// (pressure-release faces) take no part in the coarse correction: their
-// fine residuals are not restricted and coarse identity rows get zero, so a
-// residual-scaled value never lands in a solution-scaled identity row.
+// Coarse right-hand side bc = P^T rf marching in z (the restriction of the
+// multigrid form): a workgroup owns 256 consecutive coarse nodes of an
+// xy-plane (flattened, like plane_tile) and a chunk of RZ_CHUNK coarse planes
+// of one slot.  Along x the taps of coarse node i are fine 2i-1, 2i, 2i+1
+// with weights 1/2, 1, 1/2 (0 where absent or a coarse node), along y
+// likewise.  Each fine plane's xy-restricted value
+//   s(f) = sum_{dy, dx} wy(dy) wx(dx) rf(2i + dx, 2j + dy, f)
+// is computed once per chunk and
+// added to the coarse planes it feeds: weight 1 to the plane whose coarse
+// node it is, 1/2 to each neighbouring coarse plane otherwise.  Fine planes
+// go in pairs; the fine data are read about once (a thread per coarse node
+// loading its nine fine rows reads them 2.25 times on average), and the
+// 128^3 -> 65^3 restriction of 64 columns takes 323 us against 420 us for
+// that per-node form.  Chunks of 4,
+// 16 or all planes, plain workgroup order instead of the XCD remap, loads
+// issued without the DPP shift, and four planes per step were all measured
+// and no faster.  Identity rows (pressure-release faces) take no part in the
+// coarse correction: their fine residuals are not restricted and coarse
+// identity rows get zero, so a residual-scaled value never lands in a
+// solution-scaled identity row.  Grid: (pieces of a coarse plane) x chunks x slots, padded to
+// a multiple of 8 (nblocks).
+constexpr int RZ_CHUNK = 8;
+
 template <typename T, int PAIR>
-__global__ __launch_bounds__(256) void k_restrict(const cx<T> *__restrict__ rf, int nxf, int nyf, int nzf,
-                                                  cx<T> *__restrict__ bc, int nxc, int nyc, int nzc, int k, int rs,
-                                                  const int *__restrict__ live, int soft, int nblocks) {
-    const PlaneTile t = plane_tile(nxc, nyc, nzc, (k + rs - 1) / rs, nblocks);
-    // Slot group and plane are uniform over the workgroup.
-    if (t.r * rs >= k) return;
-    const bool valid = t.in_grid;
-    const int ic = t.i, jc = t.j, kc = t.k;
-    const int r0 = t.r * rs, r1 = min(k, r0 + rs);
+__global__ __launch_bounds__(256) void k_restrict_z(const cx<T> *__restrict__ rf, int nxf, int nyf, int nzf,
+                                                    cx<T> *__restrict__ bc, int nxc, int nyc, int nzc, int k,
+                                                    const int *__restrict__ live, int soft, int nblocks) {
+    const int tile = xcd_tile(blockIdx.x, nblocks);
+    const int nxy = nxc * nyc;
+    const int pieces = (nxy + 255) >> 8;
+    const int chunks = (nzc + RZ_CHUNK - 1) / RZ_CHUNK;
+    const int r = tile / (pieces * chunks);
+    if (r >= k) return;  // padding: whole workgroup
+    if (live && !live[r]) return;
+    const int rest = tile - r * pieces * chunks;
+    const int chunk = rest / pieces;
+    const int pxy_raw = (rest - chunk * pieces) * 256 + (int)threadIdx.x;
+    const bool valid = pxy_raw < nxy;
+    const int pxy = min(pxy_raw, nxy - 1);
+    const int jc = pxy / nxc, ic = pxy - jc * nxc;
     const int lane = threadIdx.x & 63;
     const int64_t nc = (int64_t)nxc * nyc * nzc;
     const int64_t nf = (int64_t)nxf * nyf * nzf;
-    const int64_t q = ic + (int64_t)nxc * (jc + (int64_t)nyc * kc);
-    const bool fixed = soft && on_soft_face(ic, jc, kc, nxc, nyc, nzc, soft);
-    const int fi = fine_of(ic, nxf), fj = fine_of(jc, nyf), fk = fine_of(kc, nzf);
-    T wx[3], wy[3], wz[3];
-    int ox[3], oy[3], oz[3];
-    wx[1] = wy[1] = wz[1] = 1;
+    const int fi = fine_of(ic, nxf), fj = fine_of(jc, nyf);
+    T wx[3], wy[3];
+    int ox[3], oy[3];
+    wx[1] = wy[1] = 1;
     wx[0] = (fi - 1 >= 0 && !coarse_node(fi - 1, nxf)) ? T(0.5) : T(0);
     wx[2] = (fi + 1 < nxf && !coarse_node(fi + 1, nxf)) ? T(0.5) : T(0);
     wy[0] = (fj - 1 >= 0 && !coarse_node(fj - 1, nyf)) ? T(0.5) : T(0);
     wy[2] = (fj + 1 < nyf && !coarse_node(fj + 1, nyf)) ? T(0.5) : T(0);
-    wz[0] = (fk - 1 >= 0 && !coarse_node(fk - 1, nzf)) ? T(0.5) : T(0);
-    wz[2] = (fk + 1 < nzf && !coarse_node(fk + 1, nzf)) ? T(0.5) : T(0);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         ox[d] = min(max(fi + d - 1, 0), nxf - 1);
         oy[d] = min(max(fj + d - 1, 0), nyf - 1);
-        oz[d] = min(max(fk + d - 1, 0), nzf - 1);
     }
-    T wt[3][3][3];  // tap weights, zero on fine identity rows
+    T wt[3][3];  // xy tap weights, zero on fine identity rows of the x and y faces
 #pragma unroll
-    for (int dz = 0; dz < 3; ++dz)
+    for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+            T wv = wy[dy] * wx[dx];
+            if ((soft & 15) && on_soft_face(ox[dx], oy[dy], 1, nxf, nyf, 3, soft & 15)) wv = T(0);
+            wt[dy][dx] = wv;
+        }
+    const cx<T> *src = rf + r * nf;
+    // Row values (x taps 2i-1, 2i, 2i+1) of fine row y = oy[dy], plane f:
+    // the pair (2i, 2i+1) with one 16-byte access, 2i-1 from the previous
+    // lane (DPP) or, in lane 0 and at a row start, loaded.
+    auto load_rows = [&](int f, cx<T> (&v)[3][3]) {
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy) {
+            const int64_t row = (int64_t)nxf * (oy[dy] + (int64_t)nyf * f);
+            if constexpr (PAIR) {
+                static_assert(sizeof(T) == 4, "paired loads are for complex64");
+                const int pidx = min(2 * ic, nxf - 2);
+                const float4 pr = *reinterpret_cast<const float4 *>(src + row + pidx);
+                const bool first = 2 * ic <= nxf - 2;
+                v[dy][1] = first ? mk<float>(pr.x, pr.y) : mk<float>(pr.z, pr.w);
+                v[dy][2] = first ? mk<float>(pr.z, pr.w) : mk<float>(0, 0);
+                v[dy][0] = mk<float>(lane_prev(pr.z), lane_prev(pr.w));
+                if (lane == 0 || ic == 0) v[dy][0] = src[row + ox[0]];
+            } else {
+#pragma unroll
+                for (int dx = 0; dx < 3; ++dx) v[dy][dx] = src[row + ox[dx]];
+            }
+        }
+    };
+    auto plane_sum = [&](const cx<T> (&v)[3][3]) {
+        cx<T> acc = mk<T>(0, 0);
 #pragma unroll
         for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
-            for (int dx = 0; dx < 3; ++dx) {
-                T wv = wz[dz] * wy[dy] * wx[dx];
-                if (soft && on_soft_face(ox[dx], oy[dy], oz[dz], nxf, nyf, nzf, soft)) wv = T(0);
-                wt[dz][dy][dx] = wv;
-            }
-    for (int r = r0; r < r1; ++r) {
-        if (live && !live[r]) continue;  // uniform over the workgroup
-        const cx<T> *src = rf + r * nf;
-        cx<T> acc = mk<T>(0, 0);
+            for (int dx = 0; dx < 3; ++dx) acc = acc + wt[dy][dx] * v[dy][dx];
+        return acc;
+    };
+    const int kc0 = chunk * RZ_CHUNK, kc1 = min(nzc, kc0 + RZ_CHUNK);
+    const int f_lo = max(fine_of(kc0, nzf) - 1, 0);
+    const int f_hi = min(fine_of(kc1 - 1, nzf) + 1, nzf - 1);
+    // Accumulators of coarse planes ka and ka + 1; a plane is written once no
+    // later fine plane reaches it.
+    int ka = kc0;
+    cx<T> acc_a = mk<T>(0, 0), acc_b = mk<T>(0, 0);
+    auto flush = [&]() {
+        const bool fixed = soft && on_soft_face(ic, jc, ka, nxc, nyc, nzc, soft);
+        if (valid) bc[r * nc + ic + (int64_t)nxc * (jc + (int64_t)nyc * ka)] = fixed ? mk<T>(0, 0) : acc_a;
+        acc_a = acc_b;
+        acc_b = mk<T>(0, 0);
+        ++ka;
+    };
+    auto feed = [&](int f, cx<T> sv) {
+        if ((soft & 48) && on_soft_face(1, 1, f, 3, 3, nzf, soft & 48)) return;  // identity plane
+        const bool own = coarse_node(f, nzf);
+        const int t0 = own ? coarse_of(f, nzf, nzc) : coarse_of(f - 1, nzf, nzc);
+        const int t1 = own ? -1 : coarse_of(f + 1, nzf, nzc);
+        const T w = own ? T(1) : T(0.5);
 #pragma unroll
-        for (int dz = 0; dz < 3; ++dz)
-#pragma unroll
-            for (int dy = 0; dy < 3; ++dy) {
-                const int64_t row = (int64_t)nxf * (oy[dy] + (int64_t)nyf * oz[dz]);
-                cx<T> v[3];
-                if constexpr (PAIR) {
-                    static_assert(sizeof(T) == 4, "paired loads are for complex64");
-                    // Pair (2i, 2i+1); the last coarse node of an even grid is fine
-                    // node nxf-1, the second element of the pair (nxf-2, nxf-1).
-                    const int pidx = min(2 * ic, nxf - 2);
-                    const float4 pr = *reinterpret_cast<const float4 *>(src + row + pidx);
-                    const bool first = 2 * ic <= nxf - 2;
-                    v[1] = first ? mk<float>(pr.x, pr.y) : mk<float>(pr.z, pr.w);
-                    v[2] = first ? mk<float>(pr.z, pr.w) : mk<float>(0, 0);
-                    // The previous lane holds x-1 of the same row except in lane 0
-                    // and at a row start (where the left tap has weight 0).
-                    v[0] = mk<float>(lane_prev(pr.z), lane_prev(pr.w));
-                    if (lane == 0 || ic == 0) v[0] = src[row + ox[0]];
-                } else {
-#pragma unroll
-                    for (int dx = 0; dx < 3; ++dx) v[dx] = src[row + ox[dx]];
-                }
-#pragma unroll
-                for (int dx = 0; dx < 3; ++dx) acc = acc + wt[dz][dy][dx] * v[dx];
-            }
-        if (valid) bc[r * nc + q] = fixed ? mk<T>(0, 0) : acc;
+        for (int q = 0; q < 2; ++q) {
+            const int kt = q ? t1 : t0;
+            if (kt < kc0 || kt >= kc1) continue;
+            while (kt > ka + 1) flush();
+            if (kt == ka)
+                acc_a = acc_a + w * sv;
+            else
+                acc_b = acc_b + w * sv;
+        }
+    };
+    for (int f = f_lo; f <= f_hi; f += 2) {
+        cx<T> v0[3][3], v1[3][3];
+        const bool two = f + 1 <= f_hi;  // uniform over the workgroup
+        load_rows(f, v0);
+        if (two) load_rows(f + 1, v1);
+        feed(f, plane_sum(v0));
+        if (two) feed(f + 1, plane_sum(v1));
     }
+    while (ka < kc1) flush();
 }
 
 // Fine nodes of coarse cell c along one axis (fine size n, coarse size m):

@@ -252,6 +252,24 @@ class HelmholtzOperator:
         """Free the cached solver workspace (the next solve re-allocates)."""
         _capi.check(_capi.lib().hz_release_workspace(self.handle), "hz_release_workspace")
 
+    def mg_transfer(self, level, x, prolong=False, out=None):
+        """The multigrid transfer between `level` and level + 1 (hz_mg_transfer)
+        on (k, n) complex64 device vectors: P^T x (fine -> coarse), or
+        out + P x (coarse -> fine; out defaults to zeros)."""
+        if self.hierarchy is None:
+            raise RuntimeError("set_multigrid() first")
+        shapes = [lev["shape"] for lev in self.hierarchy.levels]
+        n_in = int(np.prod(shapes[level + 1 if prolong else level]))
+        n_out = int(np.prod(shapes[level if prolong else level + 1]))
+        x = x.to(torch.complex64).contiguous()
+        if x.dim() != 2 or x.shape[1] != n_in:
+            raise ValueError(f"expected (k, {n_in}) vectors")
+        if out is None:
+            out = torch.zeros((x.shape[0], n_out), dtype=torch.complex64, device=x.device)
+        _capi.check(_capi.lib().hz_mg_transfer(self.handle, int(level), int(bool(prolong)), ctypes.c_void_p(x.data_ptr()),
+                                               ctypes.c_void_p(out.data_ptr()), x.shape[0], _stream()), "hz_mg_transfer")
+        return out
+
     def set_tile_shapes(self, shapes):
         """Pin the slab tile shapes (the layout `autotune` reports)."""
         arr = (ctypes.c_int32*len(shapes))(*[int(v) for v in shapes])
