@@ -253,9 +253,11 @@ void launch_restrict(const MgLevel &L, const MgLevel &C, const cx<float> *r, cx<
 void launch_prolong(const MgLevel &L, const MgLevel &C, const cx<float> *xc, cx<float> *xf, int k, const int *live,
                     const cx<float> *b, const cx<float> *dinv, float w, const float *bdiv, int conj_d, hipStream_t s) {
     // One thread per coarse cell and group of rs slots: the slots of a group
-    // share the cell's weights and first-sweep factors.  Groups are as large
-    // as the grid allows while it still has >= 2048 workgroups.
-    const int rs = transfer_group(C, k);
+    // share the cell's weights and first-sweep factors.  At most 8 slots per
+    // group (while the grid keeps >= 2048 workgroups): the 128^3 -> 65^3
+    // prolongation of 64 columns inside a solve takes 531 us with groups of
+    // 8, 574 us with 4 and 583 us with 32 (the grid's limit there).
+    const int rs = std::min(8, transfer_group(C, k));
     const int nb = plane_blocks(C.nx, C.ny, C.nz, (k + rs - 1) / rs);
     if (L.nx % 2 == 0)
         hipLaunchKernelGGL((k_prolong<float, 1>), dim3(nb), dim3(256), 0, s, xc, C.nx, C.ny, C.nz, xf, L.nx, L.ny,
