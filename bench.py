@@ -60,9 +60,10 @@ def parse():
     p.add_argument("--graph", type=int, default=0, help="replay each GMRES cycle from a captured hipGraph")
     p.add_argument("--arnoldi-block", type=int, default=2, choices=[1, 2],
                    help="mixed form: basis vectors per Gram-Schmidt pass")
-    p.add_argument("--workload", choices=["g128", "uneven", "large", "surface"], default="g128",
+    p.add_argument("--workload", choices=["g128", "uneven", "large", "surface", "online"], default="g128",
                    help="g128: G columns of the heterogeneous 128^3 scene (the headline); uneven: the "
-                        "32-scene batch; large: the 256^3 two-layer scene")
+                        "32-scene batch; large: the 256^3 two-layer scene; surface: homogeneous target responses; "
+                        "online: PhaseObjective evaluations on a device basis of the archived 41^3 x 144 shape")
     p.add_argument("--scenes", type=int, default=32, help="uneven: scenes in the batch")
     p.add_argument("--scene-rhs", type=int, default=16, help="uneven: right-hand sides per scene")
     p.add_argument("--workers", type=int, default=4, help="uneven: concurrent scene streams")
@@ -98,6 +99,8 @@ def main():
         return bench_large(args)
     if args.workload == "surface":
         return bench_surface(args)
+    if args.workload == "online":
+        return bench_online(args)
     t0 = time.perf_counter()
     scene = scenes.config_heterogeneous(n=args.n)
     op = HelmholtzOperator(scene.rho, scene.c, scene.dx, scene.omega, scene.boundaries, scene.pml,
@@ -129,6 +132,11 @@ def main():
                   relative_rtol=args.rtol)
     mixed = dict(common, restart=args.mixed_restart, precision="c64", chunk=args.chunk, preconditioner=args.precond,
                  use_graph=bool(args.graph), arnoldi_block=args.arnoldi_block)
+    # G is kept as its rows on a 9 x 9 x 9 node patch between the array and
+    # the obstacle (trilinear samples at the nodes), the form the
+    # optimisers consume; each step samples its chunk of columns.
+    prows, pw = patch_rows(scene.shape, (args.n//2, args.n//2, int(0.4*args.n)), 9)
+    g_patch = torch.zeros((m, len(prows)), dtype=torch.complex128, device="cuda:0")
     for s in range(args.warmup):
         b = rhs_block(s*args.chunk, args.chunk)
         _, st = op.gmres(b, **mixed)
@@ -137,8 +145,11 @@ def main():
     stats = []
     t_start = time.perf_counter()
     for s in range(args.steps):
-        b = rhs_block((args.warmup + s)*args.chunk, args.chunk)
-        _, st = op.gmres(b, **mixed)
+        start = (args.warmup + s)*args.chunk
+        b = rhs_block(start, args.chunk)
+        x, st = op.gmres(b, **mixed)
+        cols = (start + torch.arange(args.chunk, device="cuda:0")) % m
+        g_patch[cols] = op.sample(prows, pw, x)
         stats += st
         log(f"step {s}: converged {sum(x['converged'] for x in st)}/{len(st)}")
     torch.cuda.synchronize()
@@ -215,6 +226,9 @@ def main():
         "worst_scaled_residual": max(x["scaled_residual"] for x in stats) if stats else None,
         "worst_relative_residual": max(x["relative_residual"] for x in stats) if stats else None,
         "worst_backward_error": max(x["backward_error"] for x in stats) if stats else None,
+        "g_patch": {"rows": len(prows), "columns_sampled": args.steps*args.chunk,
+                    "what": "trilinear samples of each solved column at a 9x9x9 node patch centred at "
+                            f"({args.n//2}, {args.n//2}, {int(0.4*args.n)}), inside the timed steps"},
         "adjoint_gradient": gradient,
         "g_plus_gradient": None if g_cols is None or gradient is None else {
             "mixed_s": g_cols + gradient["seconds"],
@@ -583,6 +597,97 @@ def bench_surface(args, points=64):
         "roofline": {"kernel": "hz::k_zgemm", "bound": "fp64", "achieved": flops/launch_s/1e12, "peak": 78.6,
                      "unit": "TFLOP/s", "frac": flops/launch_s/1e12/78.6, "launch_us": launch_s*1e6},
         "cpu_baseline": cpu}), flush=True)
+
+
+def patch_rows(shape, centre, width):
+    """(rows (P, 8), weights (P, 8)) sampling the width^3 nodes centred at
+    `centre` (node indices): each point is a node, so one weight is 1."""
+    nx, ny, nz = shape
+    h = width//2
+    ii, jj, kk = np.meshgrid(*[np.arange(c - h, c - h + width) for c in centre], indexing="ij")
+    node = (ii + nx*(jj + ny*kk)).ravel(order="F").astype(np.int64)
+    rows = np.repeat(node[:, None], 8, axis=1)
+    w = np.zeros(rows.shape)
+    w[:, 0] = 1.0
+    return rows, w
+
+
+class _DeviceBasis:
+    """A phase response basis already on the device, for PhaseObjective:
+    rows = nodes (Fortran order), trilinear target rows from it."""
+
+    def __init__(self, g, cfg):
+        self.basis_gpu = g
+        self.cfg = cfg
+
+    def target_response(self, points):
+        from ultrasound_solver_amd import training
+        d = self.cfg.domain
+        rows, w = training.trilinear_rows(points, d.grid_size, d.origin_m, d.dx, d.box_size)
+        import torch
+        corner = self.basis_gpu.index_select(0, torch.as_tensor(rows.ravel(), device=self.basis_gpu.device))
+        corner = corner.reshape(len(rows), 8, -1)
+        return (torch.as_tensor(w, device=corner.device)[:, :, None]*corner).sum(dim=1).cpu().numpy()
+
+
+def bench_online(args, evaluations=2000):
+    """PhaseObjective.evaluate (loss and phase gradient, one fused pass over
+    G) on a device basis the shape of the archived 41^3 run with a 12 x 12
+    array (68,921 x 144 complex128): the inner loop of every phase optimiser
+    (src/algorithm/baselines/common.py:275-343).  The archived scene itself
+    has 2.9 points per wavelength, below the solver's 6, so G holds seeded
+    random values; the cost of an evaluation depends only on the shape.
+    Losses as the archived run configures them (focal_contrast, T = 5,
+    beta = 1.3, target ball 8 mm, 4 source layers excluded), plus
+    focal_pressure and field_match on the same basis."""
+    import torch
+    from types import SimpleNamespace
+    from ultrasound_solver_amd import training
+    from ultrasound_solver_amd.config import SimulationConfig
+    raw = {"mode": "phase_optimization",
+           "physics": {"sound_speed": 343.0, "medium_density": 1.21},
+           "transducer_specs": {"frequency": 40000.0, "diameter": 0.01, "pitch": 0.01, "array_n": 12},
+           "domain": {"box_size": [0.12]*3, "grid_size": [41]*3},
+           "boundary_conditions": {"-x": "open", "+x": "reflecting", "-y": "open", "+y": "open",
+                                   "-z": "reflecting", "+z": "open"},
+           "targets": [{"point": [0.09, 0.09, 0.09]}],
+           "training": {"loss_type": "focal_contrast", "target_peak_pressure": 600.0, "target_sigma": 0.006,
+                        "background_weight": 1.3, "target_radius": 0.008, "source_exclusion_layers": 4,
+                        "sidelobe_temperature": 5.0},
+           "solver": {"backend": "gpu_iterative"}}
+    n, m = 41**3, 144
+    gen = torch.Generator(device="cuda:0").manual_seed(12)
+    g = torch.complex(torch.randn((n, m), generator=gen, device="cuda:0", dtype=torch.float64),
+                      torch.randn((n, m), generator=gen, device="cuda:0", dtype=torch.float64))*50
+    rng = np.random.default_rng(13)
+    out = {}
+    for loss in ("focal_contrast", "focal_pressure", "field_match"):
+        cfg = SimulationConfig.from_dict(dict(raw, training=dict(raw["training"], loss_type=loss)))
+        solver = SimpleNamespace(cfg=cfg, transducers=SimpleNamespace(num_transducers=m))
+        obj = training.PhaseObjective(solver, basis=_DeviceBasis(g, cfg))
+        phases = rng.uniform(0, 2*np.pi, (evaluations, m))
+        for q in range(20):
+            obj.evaluate(phases[q], gradient=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for q in range(evaluations):
+            obj.evaluate(phases[q], gradient=True)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        out[loss] = {"evaluations_per_s": evaluations/sec, "ms_per_evaluation": sec/evaluations*1e3}
+        log(f"online {loss}: {evaluations/sec:.0f} evaluations/s")
+    basis_bytes = n*m*16
+    print(json.dumps({
+        "metric": "phase-optimiser evaluations/s (loss + phase gradient) on a 41^3 x 144 device basis",
+        "value": out["focal_contrast"]["evaluations_per_s"], "unit": "evaluations/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "c128", "data": "synthetic basis of the archived run's shape",
+        "config": {"workload": f"PhaseObjective.evaluate(gradient=True), {evaluations} seeded phase vectors, "
+                               "archived focal_contrast settings; G read once per evaluation"},
+        "losses": out, "basis_bytes": basis_bytes,
+        "basis_read_GBps": basis_bytes*out["focal_contrast"]["evaluations_per_s"]/1e9,
+        "context": "the reference's archived run: 1,701 field + VJP evaluations in 12.54 s on its own GPU "
+                   "(research_archive/algorithm_comparison/results/phase_oblique_reflecting_x_12x12)"}),
+        flush=True)
 
 
 def _threads():
