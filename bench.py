@@ -76,6 +76,7 @@ def parse():
                    help="only the roofline launches: the solve's Krylov apply over 32 columns with the tile "
                         "shape pinned, no autotune and no solve (for profiler runs)")
     p.add_argument("--cpu-samples", type=int, default=3, help="timed CPU GMRES samples per thread setting")
+    p.add_argument("--evaluations", type=int, default=2000, help="online: timed evaluations per loss")
     return p.parse_args()
 
 
@@ -100,7 +101,7 @@ def main():
     if args.workload == "surface":
         return bench_surface(args)
     if args.workload == "online":
-        return bench_online(args)
+        return bench_online(args, args.evaluations)
     t0 = time.perf_counter()
     scene = scenes.config_heterogeneous(n=args.n)
     op = HelmholtzOperator(scene.rho, scene.c, scene.dx, scene.omega, scene.boundaries, scene.pml,

@@ -1803,13 +1803,18 @@ hz_status hz_basis_adjoint(const void *G, int64_t ldg, int64_t nrows, int32_t m,
 
 namespace {
 
-// Waves of a fused loss launch: 4 per workgroup, at most 1024 workgroups,
-// and at most BASIS_LOSS_PARTIAL_BYTES of per-wave partials.
-int64_t basis_loss_waves(int64_t nrows, int m, int nacc) {
-    const int64_t cap = BASIS_LOSS_PARTIAL_BYTES / ((int64_t)nacc * m * (int64_t)sizeof(cx<double>));
-    int64_t nwaves = std::min<int64_t>({(int64_t)4096, std::max<int64_t>(4, cap), nrows + 3});
-    return std::max<int64_t>(4, nwaves / 4 * 4);
+// Workgroups of a fused loss launch (4 waves each): at most 1024, and their
+// partials plus the folded slices within BASIS_LOSS_PARTIAL_BYTES.
+int64_t basis_loss_blocks(int64_t nrows, int m, int nacc) {
+    const int64_t per = (int64_t)nacc * m * (int64_t)sizeof(cx<double>);
+    const int64_t cap = (BASIS_LOSS_PARTIAL_BYTES - LOSS_FOLD_SLICES * per) / per;
+    return std::max<int64_t>(1, std::min<int64_t>({(int64_t)1024, cap, (nrows + 3) / 4}));
 }
+
+// Slices the workgroup partials fold into (0: combined directly); the
+// combine takes at most LOSS_FOLD_SLICES partials and a slice at most 64
+// (1024 workgroups / 64 slices = 16).
+int64_t basis_loss_slices(int64_t nblk) { return nblk > LOSS_FOLD_SLICES ? LOSS_FOLD_SLICES : 0; }
 
 BasisLossArgs basis_loss_args(const hz_basis_loss_desc *d, int64_t nrows) {
     if (!d) throw BadArgument("the loss descriptor is required");
@@ -1844,16 +1849,43 @@ BasisLossArgs basis_loss_args(const hz_basis_loss_desc *d, int64_t nrows) {
     return a;
 }
 
-template <int KIND>
-void launch_basis_loss_kind(const cx<double> *G, int64_t ldg, int64_t nrows, int m, const cx<double> *w,
-                            const BasisLossArgs &a, int64_t nwaves, cx<double> *part, double *scal, double *loss,
-                            cx<double> *g, cx<double> *u, hipStream_t s) {
-    const int64_t rows = (nrows + nwaves - 1) / nwaves;
-    hipLaunchKernelGGL(k_basis_loss<KIND>, dim3((unsigned)(nwaves / 4)), dim3(256), 0, s, G, ldg, nrows, m, w, a, rows,
+template <int KIND, int PER>
+void launch_basis_loss_per(const cx<double> *G, int64_t ldg, int64_t nrows, int m, const cx<double> *w,
+                           const BasisLossArgs &a, int64_t nblk, cx<double> *part, double *scal, cx<double> *u,
+                           hipStream_t s) {
+    const int64_t rows = (nrows + 4 * nblk - 1) / (4 * nblk);
+    hipLaunchKernelGGL((k_basis_loss<KIND, PER>), dim3((unsigned)nblk), dim3(256), 0, s, G, ldg, nrows, m, w, a, rows,
                        part, scal, u);
     HZ_LAUNCHED();
-    hipLaunchKernelGGL(k_basis_loss_combine<KIND>, dim3((m + 255) / 256), dim3(256), 0, s,
-                       (const cx<double> *)part, (const double *)scal, nwaves, m, a, g, loss);
+}
+
+// Scratch layout: workgroup partials, folded partials, workgroup scalars,
+// folded scalars.
+template <int KIND>
+void launch_basis_loss_kind(const cx<double> *G, int64_t ldg, int64_t nrows, int m, const cx<double> *w,
+                            const BasisLossArgs &a, void *scratch_base, double *loss, cx<double> *g, cx<double> *u,
+                            hipStream_t s) {
+    constexpr int nacc = KIND == LOSS_RATIO ? 2 : 1;
+    const int64_t nblk = basis_loss_blocks(nrows, m, nacc);
+    const int64_t nsl = basis_loss_slices(nblk);
+    cx<double> *part = static_cast<cx<double> *>(scratch_base);
+    cx<double> *fold = part + (int64_t)nacc * nblk * m;
+    double *scal = reinterpret_cast<double *>(fold + (int64_t)nacc * nsl * m);
+    double *fscal = scal + nblk * LOSS_SCALARS;
+    if (m <= 128) launch_basis_loss_per<KIND, 2>(G, ldg, nrows, m, w, a, nblk, part, scal, u, s);
+    else if (m <= 256) launch_basis_loss_per<KIND, 4>(G, ldg, nrows, m, w, a, nblk, part, scal, u, s);
+    else launch_basis_loss_per<KIND, 8>(G, ldg, nrows, m, w, a, nblk, part, scal, u, s);
+    int64_t nparts = nblk;
+    if (nsl) {
+        hipLaunchKernelGGL(k_basis_loss_fold<KIND>, dim3((unsigned)((m + 63) / 64), (unsigned)nsl), dim3(64), 0, s,
+                           (const cx<double> *)part, (const double *)scal, nblk, m, a.temperature, fold, fscal, nsl);
+        HZ_LAUNCHED();
+        part = fold;
+        scal = fscal;
+        nparts = nsl;
+    }
+    hipLaunchKernelGGL(k_basis_loss_combine<KIND>, dim3((m + 63) / 64), dim3(256), 0, s,
+                       (const cx<double> *)part, (const double *)scal, nparts, m, a, g, loss);
     HZ_LAUNCHED();
 }
 
@@ -1862,34 +1894,31 @@ void launch_basis_loss_kind(const cx<double> *G, int64_t ldg, int64_t nrows, int
 void launch_basis_loss(const void *G, int64_t ldg, int64_t nrows, int32_t m, const void *w,
                        const BasisLossArgs &a, int kind, void *scratch_base, double *loss, void *g, void *u,
                        hipStream_t s) {
-    const int nacc = kind == HZ_LOSS_RATIO ? 2 : 1;
-    const int64_t nwaves = basis_loss_waves(nrows, m, nacc);
-    cx<double> *part = static_cast<cx<double> *>(scratch_base);
-    double *scal = reinterpret_cast<double *>(part + (int64_t)nacc * nwaves * m);
     auto Gc = static_cast<const cx<double> *>(G);
     auto wc = static_cast<const cx<double> *>(w);
     auto gc = static_cast<cx<double> *>(g);
     auto uc = static_cast<cx<double> *>(u);
     switch (kind) {
     case HZ_LOSS_FIELD_MATCH:
-        launch_basis_loss_kind<LOSS_FIELD_MATCH>(Gc, ldg, nrows, m, wc, a, nwaves, part, scal, loss, gc, uc, s);
+        launch_basis_loss_kind<LOSS_FIELD_MATCH>(Gc, ldg, nrows, m, wc, a, scratch_base, loss, gc, uc, s);
         break;
     case HZ_LOSS_ENERGY:
-        launch_basis_loss_kind<LOSS_ENERGY>(Gc, ldg, nrows, m, wc, a, nwaves, part, scal, loss, gc, uc, s);
+        launch_basis_loss_kind<LOSS_ENERGY>(Gc, ldg, nrows, m, wc, a, scratch_base, loss, gc, uc, s);
         break;
     case HZ_LOSS_SOFTMAX:
-        launch_basis_loss_kind<LOSS_SOFTMAX>(Gc, ldg, nrows, m, wc, a, nwaves, part, scal, loss, gc, uc, s);
+        launch_basis_loss_kind<LOSS_SOFTMAX>(Gc, ldg, nrows, m, wc, a, scratch_base, loss, gc, uc, s);
         break;
     default:
-        launch_basis_loss_kind<LOSS_RATIO>(Gc, ldg, nrows, m, wc, a, nwaves, part, scal, loss, gc, uc, s);
+        launch_basis_loss_kind<LOSS_RATIO>(Gc, ldg, nrows, m, wc, a, scratch_base, loss, gc, uc, s);
         break;
     }
 }
 
 size_t basis_loss_bytes(int64_t nrows, int32_t m, int kind) {
     const int nacc = kind == HZ_LOSS_RATIO ? 2 : 1;
-    const int64_t nwaves = basis_loss_waves(nrows, m, nacc);
-    return (size_t)nacc * nwaves * m * sizeof(cx<double>) + (size_t)nwaves * LOSS_SCALARS * sizeof(double);
+    const int64_t nblk = basis_loss_blocks(nrows, m, nacc);
+    const int64_t nsl = basis_loss_slices(nblk);
+    return (size_t)nacc * (nblk + nsl) * m * sizeof(cx<double>) + (size_t)(nblk + nsl) * LOSS_SCALARS * sizeof(double);
 }
 
 void check_basis_loss_args(const void *G, int64_t ldg, int64_t nrows, int32_t m, const void *w, double *loss,
@@ -1931,10 +1960,10 @@ hz_status hz_basis_loss(const void *G, int64_t ldg, int64_t nrows, int32_t m, co
 
 }  // extern "C"
 
-// Scratch of hz_basis_loss_static: the per-wave partials of one fused loss
-// launch (at most BASIS_LOSS_PARTIAL_BYTES of accumulators plus two doubles
-// for each of at most 4096 waves), in device memory that exists from module
-// load to process end, one copy per GPU.  Its host-side name is not a device
+// Scratch of hz_basis_loss_static: the per-workgroup and folded partials of
+// one fused loss launch (at most BASIS_LOSS_PARTIAL_BYTES of accumulators plus
+// two doubles for each of at most 1024 + 64 partials; 4096 reserved), in
+// device memory that exists from module load to process end, one copy per GPU.  Its host-side name is not a device
 // address: host code uses only the pointer hipGetSymbolAddress returns.
 constexpr size_t STATIC_SCRATCH_BYTES = (size_t)BASIS_LOSS_PARTIAL_BYTES + 4096 * LOSS_SCALARS * sizeof(double);
 static __device__ __align__(256) unsigned char scratch[STATIC_SCRATCH_BYTES];
@@ -1971,7 +2000,7 @@ hz_status hz_basis_loss_static(const void *G, int64_t ldg, int64_t nrows, int32_
         if (need > STATIC_SCRATCH_BYTES)
             throw BadArgument("the partial sums need " + std::to_string(need) + " bytes, more than the " +
                               std::to_string(STATIC_SCRATCH_BYTES) + "-byte static scratch");
-        // Every launch writes each partial and wave scalar it later reads, so
+        // Every launch writes each partial and partial scalar it later reads, so
         // nothing in the scratch needs clearing between calls.
         launch_basis_loss(G, ldg, nrows, m, w, a, desc->kind, static_scratch(), loss, g, u,
                           static_cast<hipStream_t>(stream));

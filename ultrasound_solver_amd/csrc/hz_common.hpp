@@ -113,6 +113,35 @@ __device__ __forceinline__ double block_sum_256(double v, double *scratch) {
     return r;
 }
 
+// A double moved across lanes by one DPP pattern (as two 32-bit halves).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ double read_lane_f64(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over the wave, the same value in every lane, in a fixed order: pairs
+// (quad_perm 1,0,3,2), quads (quad_perm 2,3,0,1), half rows
+// (row_half_mirror) and rows of 16 (row_mirror) by DPP -- each step adds two
+// values in both lanes that hold them, so all lanes of a row agree -- then
+// the four row sums ((r0 + r1) + r2) + r3 read as scalars.
+__device__ __forceinline__ double wave_sum_uniform(double v) {
+    v += dpp_f64<0xB1>(v);
+    v += dpp_f64<0x4E>(v);
+    v += dpp_f64<0x141>(v);
+    v += dpp_f64<0x140>(v);
+    return ((read_lane_f64(v, 0) + read_lane_f64(v, 16)) + read_lane_f64(v, 32)) + read_lane_f64(v, 48);
+}
+
 // Value of lane l-1 (wave_shr:1) / lane l+1 (wave_shl:1); lanes at the wave
 // edge receive 0 and are fixed up by an explicit load.
 __device__ __forceinline__ float lane_prev(float v) { return __builtin_amdgcn_update_dpp(0.0f, v, 0x138, 0xf, 0xf, false); }
