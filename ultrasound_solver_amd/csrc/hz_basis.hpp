@@ -38,55 +38,9 @@ __global__ __launch_bounds__(256) void k_basis_field(const cx<double> *__restric
             re += g.re * x.re - g.im * x.im;
             im += g.re * x.im + g.im * x.re;
         }
-        re = wave_sum(re);
-        im = wave_sum(im);
+        re = wave_sum_uniform(re);
+        im = wave_sum_uniform(im);
         if (lane == 0) u[i] = mk<double>(re, im);
-    }
-}
-
-// partial[blk * m + q] = sum over the workgroup's rows of conj(G[i, q]) c[i];
-// each thread owns columns tid, tid + 256, ...  A fixed row partition and a
-// fixed-order combine (k_basis_combine) keep the result reproducible.
-__global__ __launch_bounds__(256) void k_basis_adjoint(const cx<double> *__restrict__ G, int64_t ldg, int64_t nrows,
-                                                       int m, const cx<double> *__restrict__ c, int64_t rows_per_block,
-                                                       cx<double> *__restrict__ partial) {
-    constexpr int PER = BASIS_MAX_M / 256;
-    cx<double> acc[PER];
-#pragma unroll
-    for (int q = 0; q < PER; ++q) acc[q] = mk<double>(0, 0);
-    const int64_t i0 = blockIdx.x * rows_per_block;
-    const int64_t i1 = min(nrows, i0 + rows_per_block);
-    for (int64_t i = i0; i < i1; ++i) {
-        const cx<double> ci = c[i];
-        const cx<double> *row = G + i * ldg;
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const int col = threadIdx.x + 256 * q;
-            if (col < m) acc[q] = acc[q] + conj(row[col]) * ci;
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const int col = threadIdx.x + 256 * q;
-        if (col < m) partial[blockIdx.x * (int64_t)m + col] = acc[q];
-    }
-}
-
-// out[q] = sum_blk partial[blk * m + q] (+ the loss partials when asked).
-__global__ __launch_bounds__(256) void k_basis_combine(const cx<double> *__restrict__ partial, int nblk, int m,
-                                                       cx<double> *__restrict__ out,
-                                                       const double *__restrict__ loss_partial,
-                                                       double *__restrict__ loss) {
-    const int q = blockIdx.x * 256 + threadIdx.x;
-    if (q < m) {
-        cx<double> s = mk<double>(0, 0);
-        for (int b = 0; b < nblk; ++b) s = s + partial[(int64_t)b * m + q];
-        out[q] = s;
-    }
-    if (loss_partial && blockIdx.x == 0 && threadIdx.x == 0) {
-        double s = 0;
-        for (int b = 0; b < nblk; ++b) s += loss_partial[b];
-        *loss = s;
     }
 }
 
@@ -113,7 +67,9 @@ __global__ __launch_bounds__(256) void k_basis_combine(const cx<double> *__restr
 //   LOSS_RATIO        mean_A |u|^2 / mean_B |u|^2 over two row classes
 //                     (mask 1 = A, 2 = B): A and B sums of |u|^2 and of
 //                     conj(G) u; c follows at the combine
-enum BasisLoss { LOSS_FIELD_MATCH = 0, LOSS_ENERGY = 1, LOSS_SOFTMAX = 2, LOSS_RATIO = 3 };
+//   LOSS_ADJOINT      no loss: G^H c for a given cotangent c (no forward
+//                     product), the phase VJP of phase_response_basis.py:208-253
+enum BasisLoss { LOSS_FIELD_MATCH = 0, LOSS_ENERGY = 1, LOSS_SOFTMAX = 2, LOSS_RATIO = 3, LOSS_ADJOINT = 4 };
 
 constexpr int LOSS_SCALARS = 2;
 // Slices the fold reduces the workgroup partials to before the final combine.
@@ -123,6 +79,7 @@ struct BasisLossArgs {
     const double *target;
     const double *weight;
     const unsigned char *mask;
+    const cx<double> *cot;  // LOSS_ADJOINT
     double eps, scale, temperature;
     double count_a, count_b;
 };
@@ -145,7 +102,8 @@ __global__ __launch_bounds__(256) void k_basis_loss(const cx<double> *__restrict
     __shared__ cx<double> ws[PER * 64];
     __shared__ cx<double> red[NACC][PER * 64];
     __shared__ double rs[2][4];
-    for (int q = threadIdx.x; q < PER * 64; q += 256) ws[q] = q < m ? w[q] : mk<double>(0, 0);
+    if (KIND != LOSS_ADJOINT)
+        for (int q = threadIdx.x; q < PER * 64; q += 256) ws[q] = q < m ? w[q] : mk<double>(0, 0);
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
@@ -172,21 +130,25 @@ __global__ __launch_bounds__(256) void k_basis_loss(const cx<double> *__restrict
     for (int64_t i = i0; i < i1; ++i) {
         if (i + 1 < i1) load_row(gn, i + 1);
         double re = 0, im = 0;
+        if (KIND != LOSS_ADJOINT) {
 #pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const cx<double> x = ws[lane + 64 * q];
-            re += g[q].re * x.re - g[q].im * x.im;
-            im += g[q].re * x.im + g[q].im * x.re;
+            for (int q = 0; q < PER; ++q) {
+                const cx<double> x = ws[lane + 64 * q];
+                re += g[q].re * x.re - g[q].im * x.im;
+                im += g[q].re * x.im + g[q].im * x.re;
+            }
+            re = wave_sum_uniform(re);
+            im = wave_sum_uniform(im);
         }
-        re = wave_sum_uniform(re);
-        im = wave_sum_uniform(im);
         const cx<double> ui = mk<double>(re, im);
-        if (u_out && lane == 0) u_out[i] = ui;
+        if (KIND != LOSS_ADJOINT && u_out && lane == 0) u_out[i] = ui;
         const double p2 = re * re + im * im;
         int slot = 0;
         bool take = true;
         cx<double> ci;
-        if (KIND == LOSS_FIELD_MATCH) {
+        if (KIND == LOSS_ADJOINT) {
+            ci = a.cot[i];
+        } else if (KIND == LOSS_FIELD_MATCH) {
             const double amp = sqrt(p2);
             const double res = amp - a.target[i];
             const double wt = a.weight[i];
@@ -336,7 +298,8 @@ __global__ __launch_bounds__(64) void k_basis_loss_fold(const cx<double> *__rest
     }
 }
 
-// g[q] = G^H c from nparts <= 64 partials and the loss into *loss.  Grid
+// g[q] = G^H c from nparts <= 64 partials and the loss into *loss (none
+// when loss is null).  Grid
 // ceil(m / 64), 256 threads: wave w sums partials w, w + 4, ... of 64
 // columns, the four wave sums are added in order in LDS.
 template <int KIND>
@@ -371,7 +334,7 @@ __global__ __launch_bounds__(256) void k_basis_loss_combine(const cx<double> *__
     } else if (KIND == LOSS_SOFTMAX) {
         lval = a.scale * (top + a.temperature * (log(t1) - log(a.count_a)));
         f0 = a.scale / t1;
-    } else {
+    } else if (KIND == LOSS_RATIO) {
         const double ma = t0 / a.count_a, mb = t1 / a.count_b;
         lval = ma / mb;
         f0 = 2.0 / (a.count_a * mb);
@@ -397,7 +360,7 @@ __global__ __launch_bounds__(256) void k_basis_loss_combine(const cx<double> *__
         for (int h = 0; h < NACC; ++h) sum[h] = ((red[h][0][lane] + red[h][1][lane]) + red[h][2][lane]) + red[h][3][lane];
         g[q] = KIND == LOSS_RATIO ? f0 * sum[0] + f1 * sum[NACC - 1] : f0 * sum[0];
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) *loss = lval;
+    if (loss && blockIdx.x == 0 && threadIdx.x == 0) *loss = lval;
 }
 
 }  // namespace hz

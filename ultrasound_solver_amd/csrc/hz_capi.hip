@@ -1780,25 +1780,6 @@ hz_status hz_basis_field(const void *G, int64_t ldg, int64_t nrows, int32_t m, c
     });
 }
 
-hz_status hz_basis_adjoint(const void *G, int64_t ldg, int64_t nrows, int32_t m, const void *c, void *g,
-                           void *stream) {
-    return guarded([&] {
-        check_basis_args(G, ldg, nrows, m);
-        if (!c || !g) throw BadArgument("c and g must be device pointers");
-        hipStream_t s = static_cast<hipStream_t>(stream);
-        const int nb = (int)std::min<int64_t>(1024, nrows);
-        const int64_t rows = (nrows + nb - 1) / nb;
-        DevBuf &ws = basis_workspace((size_t)nb * m * sizeof(cx<double>));
-        hipLaunchKernelGGL(k_basis_adjoint, dim3(nb), dim3(256), 0, s, static_cast<const cx<double> *>(G), ldg, nrows,
-                           (int)m, static_cast<const cx<double> *>(c), rows, ws.as<cx<double>>());
-        HZ_LAUNCHED();
-        hipLaunchKernelGGL(k_basis_combine, dim3((m + 255) / 256), dim3(256), 0, s,
-                           (const cx<double> *)ws.as<cx<double>>(), nb, (int)m, static_cast<cx<double> *>(g),
-                           (const double *)nullptr, (double *)nullptr);
-        HZ_LAUNCHED();
-    });
-}
-
 }  // extern "C"
 
 namespace {
@@ -1921,6 +1902,12 @@ size_t basis_loss_bytes(int64_t nrows, int32_t m, int kind) {
     return (size_t)nacc * (nblk + nsl) * m * sizeof(cx<double>) + (size_t)(nblk + nsl) * LOSS_SCALARS * sizeof(double);
 }
 
+// G^H c for a given cotangent (no loss) through the fused kernels.
+void launch_basis_adjoint(const cx<double> *G, int64_t ldg, int64_t nrows, int m, const BasisLossArgs &a,
+                          void *scratch_base, cx<double> *g, hipStream_t s) {
+    launch_basis_loss_kind<LOSS_ADJOINT>(G, ldg, nrows, m, nullptr, a, scratch_base, nullptr, g, nullptr, s);
+}
+
 void check_basis_loss_args(const void *G, int64_t ldg, int64_t nrows, int32_t m, const void *w, double *loss,
                            void *g) {
     check_basis_args(G, ldg, nrows, m);
@@ -1931,6 +1918,24 @@ void check_basis_loss_args(const void *G, int64_t ldg, int64_t nrows, int32_t m,
 }  // namespace
 
 extern "C" {
+
+hz_status hz_basis_adjoint(const void *G, int64_t ldg, int64_t nrows, int32_t m, const void *c, void *g,
+                           void *stream) {
+    return guarded([&] {
+        check_basis_args(G, ldg, nrows, m);
+        if (!c || !g) throw BadArgument("c and g must be device pointers");
+        BasisLossArgs a{};
+        a.cot = static_cast<const cx<double> *>(c);
+        // Columns in slices of at most BASIS_FUSED_MAX_M (each reads its own
+        // columns of G once); the slices share the workspace in stream order.
+        const int32_t width = std::min<int32_t>(m, BASIS_FUSED_MAX_M);
+        DevBuf &ws = basis_workspace(basis_loss_bytes(nrows, width, HZ_LOSS_ENERGY));
+        for (int32_t c0 = 0; c0 < m; c0 += BASIS_FUSED_MAX_M)
+            launch_basis_adjoint(static_cast<const cx<double> *>(G) + c0, ldg, nrows,
+                                 std::min<int32_t>(BASIS_FUSED_MAX_M, m - c0), a, ws.p,
+                                 static_cast<cx<double> *>(g) + c0, static_cast<hipStream_t>(stream));
+    });
+}
 
 hz_status hz_basis_field_match(const void *G, int64_t ldg, int64_t nrows, int32_t m, const void *w,
                                const double *target, const double *weight, double eps, double *loss, void *g, void *u,
