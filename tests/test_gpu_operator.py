@@ -76,6 +76,14 @@ def test_apply_c64_matches_reference_matrix(fix, name):
     assert _rel(got, a @ x) < 2e-6
     got = op.apply(x, mode="RA", precision="c64").cpu().numpy().T
     assert _rel(got, scale[:, None]*(a @ x)) < 2e-6
+    # The adjoint modes against the conjugate transpose of the same matrix.
+    ah = okr.adjoint_matrix(a)
+    scale_h = okr.row_scale(ah)
+    minv_h = okr.jacobi_inverse(ah, scale_h)
+    refs = {"AH": ah @ x, "RAMH": scale_h[:, None]*(ah @ (minv_h[:, None]*x)), "RAH": scale_h[:, None]*(ah @ x)}
+    for mode, ref in refs.items():
+        got = op.apply(x, mode=mode, precision="c64").cpu().numpy().T
+        assert _rel(got, ref) < 2e-6, mode
 
 
 def _scene(n, seed=0, soft=False):

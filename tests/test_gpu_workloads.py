@@ -107,8 +107,10 @@ def test_heterogeneous_128_mixed_multigrid_residuals():
 
 def test_adjoint_gradient_160_fd_and_fp64_reference():
     """160^3 air, 16x16 array, reference focal_contrast loss on a 5-cell ball
-    around a seeded target at z ~ 0.7 Lz; 4 seeded phase vectors batched;
-    central differences on 8 emitters drawn with seed 4."""
+    around a seeded target at z ~ 0.7 Lz, and the ratio form (mean |u|^2 over
+    the ball / mean |u|^2 over the rest of the upper half) from the same
+    solves; 4 seeded phase vectors batched; central differences on 8 emitters
+    drawn with seed 4."""
     from ultrasound_solver_amd import autograd as hza, scenes
     n = 160
     sc = scenes.config_small(n, array_n=16)
@@ -127,14 +129,24 @@ def test_adjoint_gradient_160_fd_and_fp64_reference():
     def loss_of(u):
         return hza.focal_contrast_loss(hza.grid_view(u, sc.shape), idx, 1.0, 1.3, 5.0, mask_t)
 
+    ball_t = torch.as_tensor(ball.ravel(order="F"), device="cuda:0")
+    rest_t = torch.as_tensor(mask.ravel(order="F"), device="cuda:0")
+
+    def ratio_of(u):
+        return hza.focal_contrast_ratio_loss(u, ball_t, rest_t)
+
     phs = rng.uniform(0, 2*np.pi, (4, sc.num_sources))
     ph = torch.tensor(phs, device="cuda:0", requires_grad=True)
     t0 = time.perf_counter()
     u = hza.phase_field(ph, eng)
     total = sum(loss_of(u[r]) for r in range(4))
-    total.backward()
+    total.backward(retain_graph=True)
     grad = ph.grad.cpu().numpy()
-    print(f"160^3: 4 forward + adjoint solves (c64 multigrid, 1e-10) in {time.perf_counter() - t0:.1f}s", flush=True)
+    ph.grad = None
+    sum(ratio_of(u[r]) for r in range(4)).backward()
+    grad_ratio = ph.grad.cpu().numpy()
+    print(f"160^3: 4 forward + 2 x 4 adjoint solves (c64 multigrid, 1e-10) in {time.perf_counter() - t0:.1f}s",
+          flush=True)
     # FD on 8 random emitters (seed 4) of the first phase vector, batched.
     pick = np.random.default_rng(4).choice(sc.num_sources, 8, replace=False)
     batch = []
@@ -146,11 +158,15 @@ def test_adjoint_gradient_160_fd_and_fp64_reference():
     with torch.no_grad():
         uf = hza.phase_field(torch.tensor(np.array(batch), device="cuda:0"), eng)
         vals = [float(loss_of(uf[q])) for q in range(len(batch))]
+        vals_ratio = [float(ratio_of(uf[q])) for q in range(len(batch))]
     print(f"160^3: 16 finite-difference solves done at {time.perf_counter() - t0:.1f}s", flush=True)
     for q, e in enumerate(pick):
         fd = (vals[2*q] - vals[2*q + 1])/2e-4
-        print(f"emitter {e}: adjoint {grad[0, e]:.10g} central difference {fd:.10g}", flush=True)
+        fd_ratio = (vals_ratio[2*q] - vals_ratio[2*q + 1])/2e-4
+        print(f"emitter {e}: adjoint {grad[0, e]:.10g} central difference {fd:.10g}; "
+              f"ratio {grad_ratio[0, e]:.10g} / {fd_ratio:.10g}", flush=True)
         assert abs(grad[0, e] - fd)/max(abs(grad[0, e]), abs(fd), 1.0) < 1e-4
+        assert abs(grad_ratio[0, e] - fd_ratio)/max(abs(grad_ratio[0, e]), abs(fd_ratio), 1.0) < 1e-4
     # FP64 plain-form (reference semantics) gradient of the first vector.
     p64 = torch.tensor(phs[:1], device="cuda:0", requires_grad=True)
     loss_of(hza.phase_field(p64, eng64)[0]).backward()
