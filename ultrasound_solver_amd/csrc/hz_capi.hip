@@ -180,6 +180,11 @@ template <>
 const OpView<float> &view<float>(const hz_operator *op) { return op->v32; }
 
 int64_t vec_blocks(int64_t n) { return blocks_for(n, 256 * 16, 1024); }
+// Workgroups per column of the Krylov vector passes over k columns: those of
+// vec_blocks, except that a single column gets twice as many (at 2^21
+// complex128 elements a 16-vector dot pass streams 6.1 TB/s on 1024
+// workgroups and 4.0 TB/s on 512).
+int64_t vec_blocks_cols(int64_t n, int k) { return k == 1 ? blocks_for(n, 256 * 8, 1024) : vec_blocks(n); }
 // Elements per workgroup of the vector kernels, even so that paired loads
 // never straddle two workgroups.
 int64_t vec_chunk(int64_t n, int64_t nblk) { return ((n + nblk - 1) / nblk + 1) & ~int64_t(1); }
@@ -337,8 +342,10 @@ void launch_apply(const hz_operator *op, const cx<T> *x, int64_t ldx, cx<T> *y, 
             launch_slab<MODE, SLAB_APPLY>(op->slab_shape, op->v32, xs, ldx, ys, ldy, kk, lv, sc, nullptr, nullptr,
                                           0.0f, nullptr, s);
         } else {
-            hipLaunchKernelGGL((k_apply<T, MODE>), dim3(op->tm.nblocks), dim3(256), 0, s, view<T>(op), op->tm, xs, ldx,
-                               ys, ldy, kk, lv, sc);
+            // complex128: the same slab layout with FP64 row coefficients
+            const TileMap tm = slab64_map(op->nx, op->ny, op->nz, kk);
+            hipLaunchKernelGGL((k_slab64<MODE, SLAB64_ZB, SLAB64_TY>), dim3(tm.nblocks * tm.csplit),
+                               dim3(64 * SLAB64_TY), 0, s, view<T>(op), tm, xs, ldx, ys, ldy, kk, lv, sc);
         }
         HZ_LAUNCHED();
     }
@@ -569,7 +576,7 @@ void gmres_cycles(hz_operator *op, int mode_ram, int k, int restart, int maxiter
     const int mode_op = use_mg ? (adjoint ? MODE_RAH : MODE_RA) : mode_ram;
     const int64_t n = op->n;
     const int maxv = restart + 1;
-    const int64_t nblk = vec_blocks(n);
+    const int64_t nblk = vec_blocks_cols(n, k);
     const int64_t chunk = vec_chunk(n, nblk);
     cx<T> *V = op->V.as<cx<T>>();
     cx<T> *X = op->X.as<cx<T>>();
@@ -772,7 +779,7 @@ void gmres_cycles(hz_operator *op, int mode_ram, int k, int restart, int maxiter
 void reserve_workspace(hz_operator *op, int k, int restart, size_t elem) {
     const int64_t n = op->n;
     const int maxv = restart + 1;
-    const int64_t nblk = vec_blocks(n);
+    const int64_t nblk = vec_blocks_cols(n, k);
     op->V.ensure((size_t)maxv * k * n * elem);
     op->X.ensure((size_t)k * n * elem);
     op->Bh.ensure((size_t)k * n * elem);
@@ -848,7 +855,7 @@ void solve_plain(hz_operator *op, const hz_gmres_opts &o, const cx<double> *b, i
     reserve_workspace(op, k, o.restart, sizeof(cx<double>));
     const double *rs = o.adjoint ? op->rsa64.as<double>() : op->rsf64.as<double>();
     const cx<double> *m = o.adjoint ? op->ma64.as<cx<double>>() : op->mf64.as<cx<double>>();
-    const int64_t nblk = vec_blocks(n);
+    const int64_t nblk = vec_blocks_cols(n, k);
     const int64_t chunk = vec_chunk(n, nblk);
     double *part = op->npart.as<double>();
     hipLaunchKernelGGL((k_scale_rhs<double>), dim3(nblk, k), dim3(256), 0, s, b, ldb, rs, op->Bh.as<cx<double>>(),
@@ -1136,7 +1143,7 @@ void release_workspace(hz_operator *op) {
 void arnoldi_step_for_timing(hz_operator *op, int k, int j, int restart, bool use_mg, hipStream_t s) {
     const int64_t n = op->n;
     const int maxv = restart + 1;
-    const int64_t nblk = vec_blocks(n);
+    const int64_t nblk = vec_blocks_cols(n, k);
     const int64_t chunk = vec_chunk(n, nblk);
     cx<float> *V = op->V.as<cx<float>>();
     const cx<float> *in = V + (int64_t)j * k * n;

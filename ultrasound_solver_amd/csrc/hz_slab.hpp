@@ -471,4 +471,139 @@ __global__ __launch_bounds__(64 * TY) void k_slab2(OpView<float> op, TileMap tm,
     }
 }
 
+// FP64 apply of the plain form on the slab layout: y[:, r] = s_r op(x[:, r])
+// with k_apply's per-node arithmetic in k_apply's order (bit-identical
+// results).  A workgroup of 64 x TY threads owns a 64 x TY (x, y) tile and
+// ZB z-planes; the row coefficients of the thread's ZB nodes are computed
+// once; per right-hand side the thread loads planes k0-1 .. k0+ZB of its own
+// column (the z-neighbours stay in registers), takes the y-neighbours from
+// the block's planes in LDS (halo rows loaded by the first and last tile
+// rows) and the x-neighbours from the adjacent lanes by DPP (lanes 0 and 63
+// load the one across the tile edge): about (ZB + 2) / ZB + 2 / TY loads of
+// x per node and column instead of 7.
+__device__ __forceinline__ double lane_prev_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x138, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double lane_next_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x130, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+constexpr int SLAB64_ZB = 2, SLAB64_TY = 4;
+
+__host__ inline TileMap slab64_map(int nx, int ny, int nz, int k) {
+    TileMap t;
+    t.tiles_x = (nx + 63) / 64;
+    t.tiles_y = (ny + SLAB64_TY - 1) / SLAB64_TY;
+    t.zb = SLAB64_ZB;
+    t.tiles_z = (nz + SLAB64_ZB - 1) / SLAB64_ZB;
+    t.ntiles = t.tiles_x * t.tiles_y * t.tiles_z;
+    t.nblocks = (t.ntiles + 7) / 8 * 8;
+    t.csplit = std::max(1, std::min(k, (SLAB_TARGET_BLOCKS + t.ntiles - 1) / t.ntiles));
+    return t;
+}
+
+template <int MODE, int ZB, int TY>
+__global__ __launch_bounds__(64 * TY) void k_slab64(OpView<double> op, TileMap tm, const cx<double> *__restrict__ xin,
+                                                int64_t ldx, cx<double> *__restrict__ yout, int64_t ldy, int nrhs,
+                                                const int *__restrict__ live, const double *__restrict__ scale) {
+    __shared__ int s_act[APPLY_MAX_RHS];
+    __shared__ double s_sc[APPLY_MAX_RHS];
+    __shared__ int s_na;
+    constexpr int NT_ = 64 * TY;
+    __shared__ cx<double> s_pl[2][ZB][NT_];
+    const int part = (int)blockIdx.x / tm.nblocks;
+    const int tile = xcd_tile((int)blockIdx.x - part * tm.nblocks, tm.nblocks);
+    if (tile >= tm.ntiles) return;  // whole workgroup
+    stage_live_columns<double>(nrhs, live, scale, s_act, s_sc, &s_na);
+    const int tx = tile % tm.tiles_x;
+    const int ty = (tile / tm.tiles_x) % tm.tiles_y;
+    const int tz = tile / (tm.tiles_x * tm.tiles_y);
+    const int lane = threadIdx.x & 63;
+    // Lanes past the grid edge run on a clamped node (feeding the shifts)
+    // and never store.
+    const int i_raw = tx * 64 + lane;
+    const int j_raw = ty * TY + (int)(threadIdx.x >> 6);
+    const bool valid = i_raw < op.nx && j_raw < op.ny;
+    const int i = min(i_raw, op.nx - 1);
+    const int j = min(j_raw, op.ny - 1);
+    const int k0 = tz * tm.zb;
+    const int nzl = min(op.nz - k0, ZB);
+    const int64_t sxy = (int64_t)op.nx * op.ny;
+    const int64_t p0 = i + (int64_t)op.nx * (j + (int64_t)op.ny * k0);
+
+    cx<double> cs[ZB], c[ZB][6];
+#pragma unroll
+    for (int q = 0; q < ZB; ++q) {
+        const int kq = min(k0 + q, op.nz - 1);
+        const int64_t p = i + (int64_t)op.nx * (j + (int64_t)op.ny * kq);
+        const Nbr nb = neighbours(i, j, kq, op.nx, op.ny, op.nz);
+        row_coefs<double, MODE>(op, i, j, kq, p, nb, cs[q], c[q]);
+    }
+    __syncthreads();  // s_act
+    const bool has_xm = i > 0, has_xp = i < op.nx - 1;
+    const int64_t oym = j > 0 ? -(int64_t)op.nx : 0;
+    const int64_t oyp = j < op.ny - 1 ? (int64_t)op.nx : 0;
+    const int tid = (int)threadIdx.x;
+    const bool lds_ym = tid >= 64, lds_yp = tid + 64 < NT_;
+    const int na = s_na;
+    const int per = (na + tm.csplit - 1) / tm.csplit;
+    const int t_begin = part * per, t_end = min(na, t_begin + per);
+    int buf = 0;
+    for (int t = t_begin; t < t_end; ++t) {
+        const int r = s_act[t];
+        const cx<double> *xr = xin + r * ldx;
+        cx<double> v[ZB + 2], ym[ZB], yp[ZB], xe[ZB];
+#pragma unroll
+        for (int q = 0; q < ZB + 2; ++q) {
+            const int kq = k0 + q - 1;
+            const int kc = min(max(kq, 0), op.nz - 1);
+            v[q] = xr[p0 + (int64_t)(kc - k0) * sxy];
+            if (kq < 0 || kq >= op.nz) v[q] = mk<double>(0, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < ZB; ++q) {
+            const int kc = min(k0 + q, op.nz - 1);
+            const int64_t p = p0 + (int64_t)(kc - k0) * sxy;
+            ym[q] = mk<double>(0, 0);
+            yp[q] = mk<double>(0, 0);
+            if (!lds_ym) ym[q] = xr[p + oym];
+            if (!lds_yp) yp[q] = xr[p + oyp];
+            xe[q] = mk<double>(0, 0);
+            if (lane == 0 && has_xm) xe[q] = xr[p - 1];
+            if (lane == 63 && has_xp) xe[q] = xr[p + 1];
+        }
+#pragma unroll
+        for (int q = 0; q < ZB; ++q) s_pl[buf][q][tid] = v[q + 1];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < ZB; ++q) {
+            if (lds_ym) ym[q] = s_pl[buf][q][tid - 64];
+            if (lds_yp) yp[q] = s_pl[buf][q][tid + 64];
+        }
+        buf ^= 1;
+#pragma unroll
+        for (int q = 0; q < ZB; ++q) {
+            const cx<double> xc = v[q + 1];
+            cx<double> xl = mk<double>(lane_prev_f64(xc.re), lane_prev_f64(xc.im));
+            cx<double> xn = mk<double>(lane_next_f64(xc.re), lane_next_f64(xc.im));
+            if (lane == 0) xl = xe[q];
+            if (lane == 63) xn = xe[q];
+            cx<double> acc = cs[q] * xc;
+            acc = acc + c[q][0] * xl;
+            acc = acc + c[q][1] * xn;
+            acc = acc + c[q][2] * ym[q];
+            acc = acc + c[q][3] * yp[q];
+            acc = acc + c[q][4] * v[q];
+            acc = acc + c[q][5] * v[q + 2];
+            if (valid && q < nzl) yout[r * ldy + p0 + (int64_t)q * sxy] = s_sc[t] * acc;
+        }
+    }
+}
+
 }  // namespace hz
