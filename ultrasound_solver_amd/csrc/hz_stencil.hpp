@@ -209,67 +209,6 @@ __device__ __forceinline__ void stage_live_columns(int nrhs, const int *__restri
     if (lane == 0) *s_na = base;
 }
 
-// y[:, r] = s_r * op(x[:, r]) for active r.  One coefficient evaluation per
-// node serves every right-hand side; the active list and scales are staged in
-// LDS and the right-hand sides are processed in pairs to keep 14 loads in flight.
-template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_apply(OpView<T> op, TileMap tm, const cx<T> *__restrict__ x,
-                                               int64_t ldx, cx<T> *__restrict__ y, int64_t ldy, int nrhs,
-                                               const int *__restrict__ live, const double *__restrict__ scale) {
-    __shared__ int s_act[APPLY_MAX_RHS];
-    __shared__ T s_sc[APPLY_MAX_RHS];
-    __shared__ int s_na;
-    stage_live_columns<T>(nrhs, live, scale, s_act, s_sc, &s_na);
-    __syncthreads();
-    const int tile = xcd_tile(blockIdx.x, tm.nblocks);
-    if (tile >= tm.ntiles) return;
-    const int tx = tile % tm.tiles_x;
-    const int ty = (tile / tm.tiles_x) % tm.tiles_y;
-    const int tz = tile / (tm.tiles_x * tm.tiles_y);
-    const int i = tx * 64 + (threadIdx.x & 63);
-    const int j = ty * 4 + (threadIdx.x >> 6);
-    if (i >= op.nx || j >= op.ny) return;
-    const int na = s_na;
-    const int k0 = tz * tm.zb;
-    const int k1 = min(op.nz, k0 + tm.zb);
-    for (int k = k0; k < k1; ++k) {
-        const int64_t p = i + (int64_t)op.nx * (j + (int64_t)op.ny * k);
-        const Nbr nb = neighbours(i, j, k, op.nx, op.ny, op.nz);
-        cx<T> cs, c[6];
-        row_coefs<T, MODE>(op, i, j, k, p, nb, cs, c);
-        int t = 0;
-        for (; t + 2 <= na; t += 2) {
-            const int r0 = s_act[t], r1 = s_act[t + 1];
-            const cx<T> *x0 = x + r0 * ldx + p;
-            const cx<T> *x1 = x + r1 * ldx + p;
-            cx<T> v0[7], v1[7];
-            v0[6] = x0[0];
-            v1[6] = x1[0];
-#pragma unroll
-            for (int n = 0; n < 6; ++n) {
-                v0[n] = x0[nb.off[n]];
-                v1[n] = x1[nb.off[n]];
-            }
-            cx<T> a0 = cs * v0[6], a1 = cs * v1[6];
-#pragma unroll
-            for (int n = 0; n < 6; ++n) {
-                a0 = a0 + c[n] * v0[n];
-                a1 = a1 + c[n] * v1[n];
-            }
-            y[r0 * ldy + p] = s_sc[t] * a0;
-            y[r1 * ldy + p] = s_sc[t + 1] * a1;
-        }
-        if (t < na) {
-            const int r0 = s_act[t];
-            const cx<T> *x0 = x + r0 * ldx + p;
-            cx<T> a0 = cs * x0[0];
-#pragma unroll
-            for (int n = 0; n < 6; ++n) a0 = a0 + c[n] * x0[nb.off[n]];
-            y[r0 * ldy + p] = s_sc[t] * a0;
-        }
-    }
-}
-
 // out[:, r] = b[:, r] - op(x[:, r]) with per-workgroup FP64 partials of
 // |out|^2; one right-hand side per grid row (blockIdx.y).
 template <typename T, int MODE>
