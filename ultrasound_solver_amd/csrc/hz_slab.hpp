@@ -473,8 +473,8 @@ __global__ __launch_bounds__(64 * TY) void k_slab2(OpView<float> op, TileMap tm,
 
 // FP64 apply of the plain form on the slab layout: y[:, r] = s_r op(x[:, r])
 // with the row coefficients of row_coefs and the neighbour terms added in
-// the order x-, x+, y-, y+, z-, z+ after the centre (the order of the
-// reference's row, and of k_residual and k_check).  A workgroup of 64 x TY threads owns a 64 x TY (x, y) tile and
+// the order x-, x+, y-, y+, z-, z+ after the centre, the order of
+// k_residual.  A workgroup of 64 x TY threads owns a 64 x TY (x, y) tile and
 // ZB z-planes; the row coefficients of the thread's ZB nodes are computed
 // once; per right-hand side the thread loads planes k0-1 .. k0+ZB of its own
 // column (the z-neighbours stay in registers), takes the y-neighbours from
