@@ -140,11 +140,16 @@ typedef struct hz_gmres_stats {
 /* One level of the shifted-Laplacian multigrid preconditioner: the same
  * operator rediscretised on a coarser node set.  Tables are normalised by
  * the fine spacing dx: weight = dual-cell width / dx, edge = edge length / dx
- * (length n_a - 1), face terms i*omega*(w_b w_c s_b s_c / dx)*Y.            */
+ * (length n_a - 1), face terms i*omega*(w_b w_c s_b s_c / dx)*Y.
+ * rho and kappa NULL on every level: the library derives each level's
+ * fields on the device, 1/rho and kappa of the level above (the operator's
+ * own at level 0) by normalised full weighting, and the term of an open face
+ * whose face_term is NULL from the derived fields (Y = sqrt(kappa / rho));
+ * the host then supplies only the axis tables and the other face terms.    */
 typedef struct hz_level_desc {
     int32_t nx, ny, nz;
-    const double *rho;          /* n_l densities                             */
-    const double *kappa;        /* n_l compressibilities                     */
+    const double *rho;          /* n_l densities (host), or NULL             */
+    const double *kappa;        /* n_l compressibilities (host), or NULL     */
     const double *weight[3];
     const double *stretch[3];   /* complex                                   */
     const double *edge[3];
@@ -327,6 +332,17 @@ hz_status hz_material_vjp(hz_operator *op, const void *u, const void *lambda, in
  * to tests and to callers composing their own cycles. */
 hz_status hz_mg_transfer(hz_operator *op, int32_t level, int32_t direction, const void *in, void *out, int32_t nrhs,
                          void *stream);
+
+/* Copies level `level` (0 = the operator's grid) of the installed multigrid
+ * hierarchy into caller device buffers: rho and kappa (n_l floats each,
+ * Fortran order) and, for each face f that has a term, face[f] (complex64,
+ * n_b x n_c with the lower-numbered in-plane axis fastest; face or face[f]
+ * NULL skips it).  *face_mask receives bit f for each face that has a term.
+ * No counterpart in the reference (it has no multigrid): for tests and
+ * inspection of the hierarchy, such as comparing a device-derived level
+ * with a host-built one. */
+hz_status hz_mg_level(hz_operator *op, int32_t level, float *rho, float *kappa, void *const *face,
+                      int32_t *face_mask, void *stream);
 
 /* Free the cached solver workspace (Krylov basis, the multigrid form's
  * preconditioned basis, multigrid level vectors, captured graphs); the next

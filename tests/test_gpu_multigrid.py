@@ -56,3 +56,48 @@ def test_transfers_match_kronecker_prolongation(shape, soft):
                              out=torch.as_tensor(base, device="cuda:0")).cpu().numpy()
         want = base + (df @ p @ dc @ xc.astype(np.complex128).T).T
         assert np.abs(got - want).max() <= 2e-6*np.abs(want).max(), (shape, level, "prolong")
+
+
+@pytest.mark.parametrize("shape,bnd_over", [
+    ((22, 17, 13), {}),
+    ((21, 16, 19), {"+x": "soft", "-z": {"type": "impedance", "impedance": [400.0, 100.0]}}),
+    ((40, 33, 27), {"+y": "reflecting", "-x": {"type": "impedance", "impedance": [600.0, -150.0]}})])
+def test_device_derived_levels_match_host_and_definition(shape, bnd_over):
+    """The hierarchy derived on the device (set_multigrid(device_fields=True))
+    against the host-built one: every level's rho, kappa and face terms
+    within complex64/float32 rounding; and the coarse fields against the
+    definition, 1/rho and kappa by normalised full weighting (P^T f) / (P^T 1)
+    with the Kronecker-product P of the level above."""
+    from ultrasound_solver_amd import multigrid
+    from ultrasound_solver_amd.operator import HelmholtzOperator
+    rng = np.random.default_rng(sum(shape) + 7)
+    rho = 1.2*(1 + 0.4*rng.random(shape))
+    c = 343.0*(1 + 0.05*rng.standard_normal(shape))
+    bnd = {"-x": "open", "+x": "reflecting", "-y": "open", "+y": "open", "-z": "reflecting", "+z": "open"}
+    bnd.update(bnd_over)
+    op = HelmholtzOperator(rho, c, 1.07e-3, 2*np.pi*40e3, bnd, {"cells": 3}, device="cuda:0")
+    op.set_multigrid(device_fields=False)
+    host = [op.mg_level(lv) for lv in range(len(op.hierarchy.levels))]
+    host_fields = [(lev["inv_rho"], lev["kappa"]) for lev in op.hierarchy.levels]
+    op.set_multigrid(device_fields=True)
+    shapes = [lev["shape"] for lev in op.hierarchy.levels]
+    assert len(shapes) == len(host) >= 2
+    for lv in range(len(shapes)):
+        dev = op.mg_level(lv)
+        assert sorted(dev["faces"]) == sorted(host[lv]["faces"]), lv
+        for key in ("rho", "kappa"):
+            got, want = dev[key].cpu().numpy(), host[lv][key].cpu().numpy()
+            assert np.max(np.abs(got - want)/np.abs(want)) <= 1.2e-7, (lv, key)
+        for f, t in host[lv]["faces"].items():
+            got, want = dev["faces"][f].cpu().numpy(), t.cpu().numpy()
+            assert np.max(np.abs(got - want)) <= 2.4e-7*np.max(np.abs(want)), (lv, f)
+        if lv == 0:
+            continue
+        fine = shapes[lv - 1]
+        px, py, pz = (sp.csr_matrix(multigrid.prolongation_1d(n)) for n in fine)
+        pt = sp.kron(pz, sp.kron(py, px)).tocsr().T
+        norm = pt @ np.ones(pt.shape[1])
+        inv_f, kap_f = (a.ravel(order="F") for a in host_fields[lv - 1])
+        want_inv, want_kap = (pt @ inv_f)/norm, (pt @ kap_f)/norm
+        assert np.max(np.abs(dev["rho"].cpu().numpy()*want_inv - 1.0)) <= 2.4e-7, lv
+        assert np.max(np.abs(dev["kappa"].cpu().numpy()/want_kap - 1.0)) <= 1.2e-7, lv

@@ -18,7 +18,7 @@ EXPORTS = ("hz_create", "hz_destroy", "hz_apply", "hz_gmres", "hz_set_sources", 
            "hz_gather_sources", "hz_sample", "hz_device_bytes", "hz_last_error", "hz_set_multigrid",
            "hz_autotune", "hz_set_tile_shapes", "hz_gmres_batch", "hz_release_workspace",
            "hz_material_vjp", "hz_basis_field", "hz_basis_adjoint", "hz_basis_field_match", "hz_basis_loss",
-           "hz_basis_loss_static", "hz_mg_transfer",
+           "hz_basis_loss_static", "hz_mg_transfer", "hz_mg_level",
            "hz_z_transfer", "hz_surface_create", "hz_surface_destroy", "hz_surface_tables",
            "hz_surface_set_sources", "hz_surface_transform", "hz_surface_solve_flux", "hz_surface_planes",
            "hz_surface_sample", "hz_surface_target_response", "hz_deposit_sources",
@@ -122,6 +122,8 @@ def lib():
                                          c_void_p, c_void_p, c_void_p, c_void_p]
         handle.hz_basis_loss_static.argtypes = handle.hz_basis_loss.argtypes
         handle.hz_mg_transfer.argtypes = [c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_int32, c_void_p]
+        handle.hz_mg_level.argtypes = [c_void_p, c_int32, c_void_p, c_void_p, POINTER(c_void_p), POINTER(c_int32),
+                                       c_void_p]
         handle.hz_material_vjp.argtypes = [c_void_p, c_void_p, c_void_p, c_int32, c_int64, c_void_p, c_void_p,
                                            c_void_p]
         handle.hz_set_tile_shapes.argtypes = [c_void_p, POINTER(c_int32), c_int32]

@@ -1470,9 +1470,17 @@ hz_status hz_set_multigrid(hz_operator *op, int32_t nlevels, const hz_level_desc
         mg->weight = (float)weight;
         mg->shift = shift;
         const double h2 = op->dx * op->dx;
+        // Levels without host fields are derived on the device: 1/rho and
+        // kappa of the level above (the operator's own at level 0) by
+        // normalised full weighting, kept in FP64 until the level's
+        // single-precision copies are written.
+        const bool derive = !levels[0].rho && !levels[0].kappa;
+        DevBuf cur_inv, cur_kap, nxt_inv, nxt_kap, t_inv, t_kap, t_w[2], t_s[2];
         for (int l = 0; l < nlevels; ++l) {
             const hz_level_desc &d = levels[l];
             const int dims[3] = {d.nx, d.ny, d.nz};
+            if (derive != (!d.rho && !d.kappa))
+                throw BadArgument("either every level gives rho and kappa or none does");
             if (l == 0 && (d.nx != op->nx || d.ny != op->ny || d.nz != op->nz))
                 throw BadArgument("level 0 must be the operator's grid");
             if (l > 0) {
@@ -1482,17 +1490,53 @@ hz_status hz_set_multigrid(hz_operator *op, int32_t nlevels, const hz_level_desc
                     if (dims[a] != expect) throw BadArgument("level sizes must follow even-node coarsening");
                 }
             }
-            if (d.nx < 2 || d.ny < 2 || d.nz < 2 || !d.rho || !d.kappa) throw BadArgument("bad level grid");
+            if (d.nx < 2 || d.ny < 2 || d.nz < 2) throw BadArgument("bad level grid");
             auto L = std::make_unique<MgLevel>();
             L->nx = d.nx;
             L->ny = d.ny;
             L->nz = d.nz;
             L->n = (int64_t)d.nx * d.ny * d.nz;
-            std::vector<float> f32((size_t)L->n);
-            for (int64_t p = 0; p < L->n; ++p) f32[p] = (float)d.rho[p];
-            upload(L->rho, f32.data(), L->n);
-            for (int64_t p = 0; p < L->n; ++p) f32[p] = (float)d.kappa[p];
-            upload(L->kap, f32.data(), L->n);
+            if (derive) {
+                const unsigned nb = (unsigned)((L->n + 255) / 256);
+                if (l == 0) {
+                    cur_inv.ensure(L->n * sizeof(double));
+                    cur_kap.ensure(L->n * sizeof(double));
+                    hipLaunchKernelGGL(k_mg_fields0, dim3(nb), dim3(256), 0, 0, op->rho64.as<double>(),
+                                       op->kap64.as<double>(), L->n, cur_inv.as<double>(), cur_kap.as<double>());
+                    HZ_LAUNCHED();
+                } else {
+                    // x, then y, then z: (px, py, pz) -> (nx, py, pz) -> (nx, ny, pz) -> (nx, ny, nz).
+                    const int px = levels[l - 1].nx, py = levels[l - 1].ny, pz = levels[l - 1].nz;
+                    const int64_t n1 = (int64_t)d.nx * py * pz, n2 = (int64_t)d.nx * d.ny * pz;
+                    t_inv.ensure(n1 * sizeof(double));
+                    t_kap.ensure(n1 * sizeof(double));
+                    nxt_inv.ensure(n2 * sizeof(double));
+                    nxt_kap.ensure(n2 * sizeof(double));
+                    hipLaunchKernelGGL(k_mg_weight_axis, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, 0,
+                                       cur_inv.as<double>(), cur_kap.as<double>(), px, py, pz, 0, d.nx,
+                                       t_inv.as<double>(), t_kap.as<double>());
+                    HZ_LAUNCHED();
+                    hipLaunchKernelGGL(k_mg_weight_axis, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, 0,
+                                       t_inv.as<double>(), t_kap.as<double>(), d.nx, py, pz, 1, d.ny,
+                                       nxt_inv.as<double>(), nxt_kap.as<double>());
+                    HZ_LAUNCHED();
+                    hipLaunchKernelGGL(k_mg_weight_axis, dim3(nb), dim3(256), 0, 0, nxt_inv.as<double>(),
+                                       nxt_kap.as<double>(), d.nx, d.ny, pz, 2, d.nz, cur_inv.as<double>(),
+                                       cur_kap.as<double>());
+                    HZ_LAUNCHED();
+                }
+                L->rho.ensure(L->n * sizeof(float));
+                L->kap.ensure(L->n * sizeof(float));
+                hipLaunchKernelGGL(k_mg_level_fields, dim3(nb), dim3(256), 0, 0, cur_inv.as<double>(),
+                                   cur_kap.as<double>(), L->n, L->rho.as<float>(), L->kap.as<float>());
+                HZ_LAUNCHED();
+            } else {
+                std::vector<float> f32((size_t)L->n);
+                for (int64_t p = 0; p < L->n; ++p) f32[p] = (float)d.rho[p];
+                upload(L->rho, f32.data(), L->n);
+                for (int64_t p = 0; p < L->n; ++p) f32[p] = (float)d.kappa[p];
+                upload(L->kap, f32.data(), L->n);
+            }
             for (int a = 0; a < 3; ++a) {
                 const int na = dims[a];
                 if (!d.weight[a] || !d.stretch[a] || !d.edge[a]) throw BadArgument("level tables are required");
@@ -1508,10 +1552,27 @@ hz_status hz_set_multigrid(hz_operator *op, int32_t nlevels, const hz_level_desc
                 upload(L->q[a], qv.data(), qv.size());
             }
             for (int f = 0; f < 6; ++f) {
-                if (!d.face_term[f]) continue;
                 const int ax = f / 2;
                 const int b0 = ax == 0 ? 1 : 0, c0 = ax == 2 ? 1 : 2;
                 const int64_t cnt = (int64_t)dims[b0] * dims[c0];
+                if (!d.face_term[f] && derive && (op->open_mask >> f & 1)) {
+                    // Open face of a derived level: its term from the derived fields.
+                    const int ab[2] = {b0, c0};
+                    for (int u = 0; u < 2; ++u) {
+                        upload(t_w[u], d.weight[ab[u]], (size_t)dims[ab[u]]);
+                        upload(t_s[u], reinterpret_cast<const cx<double> *>(d.stretch[ab[u]]), (size_t)dims[ab[u]]);
+                    }
+                    L->face[f].ensure(cnt * sizeof(cx<float>));
+                    hipLaunchKernelGGL(k_mg_open_face, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, 0,
+                                       cur_inv.as<double>(), cur_kap.as<double>(), d.nx, d.ny, d.nz, f,
+                                       t_w[0].as<double>(), t_s[0].as<cx<double>>(), t_w[1].as<double>(),
+                                       t_s[1].as<cx<double>>(), op->omega, op->dx, L->face[f].as<cx<float>>());
+                    HZ_LAUNCHED();
+                    // The next face reuses t_w / t_s: wait for this launch.
+                    HZ_HIP(hipDeviceSynchronize());
+                    continue;
+                }
+                if (!d.face_term[f]) continue;
                 std::vector<cx<float>> t32(cnt);
                 for (int64_t e = 0; e < cnt; ++e)
                     t32[e] = mk<float>((float)d.face_term[f][2 * e], (float)d.face_term[f][2 * e + 1]);
@@ -2062,6 +2123,31 @@ hz_status hz_mg_transfer(hz_operator *op, int32_t level, int32_t direction, cons
         else
             launch_prolong(L, C, static_cast<const cx<float> *>(in), static_cast<cx<float> *>(out), nrhs, nullptr,
                            nullptr, nullptr, 0.0f, nullptr, 0, s);
+    });
+}
+
+hz_status hz_mg_level(hz_operator *op, int32_t level, float *rho, float *kappa, void *const *face,
+                      int32_t *face_mask, void *stream) {
+    return guarded([&] {
+        check_op(op);
+        if (!op->mg) throw BadArgument("no multigrid hierarchy is installed");
+        if (level < 0 || level >= (int)op->mg->levels.size()) throw BadArgument("level out of range");
+        if (!face_mask) throw BadArgument("face_mask is required");
+        const MgLevel &L = *op->mg->levels[level];
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        if (rho) HZ_HIP(hipMemcpyAsync(rho, L.rho.p, L.n * sizeof(float), hipMemcpyDeviceToDevice, s));
+        if (kappa) HZ_HIP(hipMemcpyAsync(kappa, L.kap.p, L.n * sizeof(float), hipMemcpyDeviceToDevice, s));
+        const int dims[3] = {L.nx, L.ny, L.nz};
+        int mask = 0;
+        for (int f = 0; f < 6; ++f) {
+            if (!L.face[f].p) continue;
+            mask |= 1 << f;
+            const int ax = f / 2;
+            const int64_t cnt = (int64_t)dims[ax == 0 ? 1 : 0] * dims[ax == 2 ? 1 : 2];
+            if (face && face[f])
+                HZ_HIP(hipMemcpyAsync(face[f], L.face[f].p, cnt * sizeof(cx<float>), hipMemcpyDeviceToDevice, s));
+        }
+        *face_mask = mask;
     });
 }
 

@@ -424,4 +424,103 @@ __global__ __launch_bounds__(256) void k_accumulate(const cx<float> *__restrict_
         x[r * ldx + p] = x[r * ldx + p] + cast<double>(z[r * n + p]);
 }
 
+// ---------------------------------------------------------------- hierarchy
+// Coarse-level material fields derived on the device.  Even-node coarsening
+// along an axis of n fine nodes keeps F(I) = min(2I, n - 1); a fine node that
+// is not kept (odd, below n - 1) gives weight 1/2 to both kept neighbours.
+// Normalised full weighting of a nodal field along one axis:
+//   out[I] = sum_f (P[f, I] / sum_g P[g, I]) in[f],  f = F-1, F, F+1 in order,
+// for two fields at once (1/rho and kappa).  Input grid (nx, ny, nz), output
+// the same with `nc` nodes along `axis`.
+__device__ __forceinline__ bool mg_kept(int f, int n) { return (f & 1) == 0 || f == n - 1; }
+
+__global__ __launch_bounds__(256) void k_mg_weight_axis(const double *__restrict__ a_in,
+                                                        const double *__restrict__ b_in, int nx, int ny, int nz,
+                                                        int axis, int nc, double *__restrict__ a_out,
+                                                        double *__restrict__ b_out) {
+    const int dims[3] = {nx, ny, nz};
+    int od[3] = {nx, ny, nz};
+    od[axis] = nc;
+    const int64_t nout = (int64_t)od[0] * od[1] * od[2];
+    const int64_t q = blockIdx.x * 256ll + threadIdx.x;
+    if (q >= nout) return;
+    int idx[3];
+    idx[0] = (int)(q % od[0]);
+    idx[1] = (int)((q / od[0]) % od[1]);
+    idx[2] = (int)(q / ((int64_t)od[0] * od[1]));
+    const int n = dims[axis];
+    const int F = min(2 * idx[axis], n - 1);
+    const int64_t stride = axis == 0 ? 1 : (axis == 1 ? (int64_t)nx : (int64_t)nx * ny);
+    int src[3] = {idx[0], idx[1], idx[2]};
+    src[axis] = F;
+    const int64_t pF = src[0] + (int64_t)nx * (src[1] + (int64_t)ny * src[2]);
+    const bool lo = F > 0 && !mg_kept(F - 1, n);
+    const bool hi = F + 1 < n && !mg_kept(F + 1, n);
+    const double total = 1.0 + (lo ? 0.5 : 0.0) + (hi ? 0.5 : 0.0);
+    const double wc = 1.0 / total, wh = 0.5 / total;
+    double a = 0.0, b = 0.0;
+    if (lo) {
+        a += wh * a_in[pF - stride];
+        b += wh * b_in[pF - stride];
+    }
+    a += wc * a_in[pF];
+    b += wc * b_in[pF];
+    if (hi) {
+        a += wh * a_in[pF + stride];
+        b += wh * b_in[pF + stride];
+    }
+    a_out[q] = a;
+    b_out[q] = b;
+}
+
+// inv_rho = 1 / rho and a copy of kappa (the finest level's FP64 fields).
+__global__ __launch_bounds__(256) void k_mg_fields0(const double *__restrict__ rho, const double *__restrict__ kappa,
+                                                    int64_t n, double *__restrict__ inv_rho,
+                                                    double *__restrict__ kap) {
+    const int64_t p = blockIdx.x * 256ll + threadIdx.x;
+    if (p >= n) return;
+    inv_rho[p] = 1.0 / rho[p];
+    kap[p] = kappa[p];
+}
+
+// The level's single-precision fields: rho = 1 / inv_rho, kappa.
+__global__ __launch_bounds__(256) void k_mg_level_fields(const double *__restrict__ inv_rho,
+                                                         const double *__restrict__ kappa, int64_t n,
+                                                         float *__restrict__ rho32, float *__restrict__ kap32) {
+    const int64_t p = blockIdx.x * 256ll + threadIdx.x;
+    if (p >= n) return;
+    rho32[p] = (float)(1.0 / inv_rho[p]);
+    kap32[p] = (float)kappa[p];
+}
+
+// Open-face term of a derived level on face f (axis f / 2, first or last
+// plane): i omega (w_b w_c s_b s_c / dx) sqrt(kappa / rho) per face node
+// e = ib + n_b ic, with the dual widths w and stretches s of the two in-plane
+// axes (b < c), associated as the host-built hierarchy does.
+__global__ __launch_bounds__(256) void k_mg_open_face(const double *__restrict__ inv_rho,
+                                                      const double *__restrict__ kappa, int nx, int ny, int nz,
+                                                      int f, const double *__restrict__ wb,
+                                                      const cx<double> *__restrict__ sb,
+                                                      const double *__restrict__ wc,
+                                                      const cx<double> *__restrict__ sc, double omega, double dx,
+                                                      cx<float> *__restrict__ out) {
+    const int dims[3] = {nx, ny, nz};
+    const int ax = f / 2;
+    const int b0 = ax == 0 ? 1 : 0, c0 = ax == 2 ? 1 : 2;
+    const int64_t cnt = (int64_t)dims[b0] * dims[c0];
+    const int64_t e = blockIdx.x * 256ll + threadIdx.x;
+    if (e >= cnt) return;
+    const int ib = (int)(e % dims[b0]), ic = (int)(e / dims[b0]);
+    int idx[3];
+    idx[ax] = (f & 1) ? dims[ax] - 1 : 0;
+    idx[b0] = ib;
+    idx[c0] = ic;
+    const int64_t p = idx[0] + (int64_t)nx * (idx[1] + (int64_t)ny * idx[2]);
+    const double adm = sqrt(kappa[p] * inv_rho[p]);
+    const cx<double> a = ((wb[ib] * wc[ic]) * sb[ib]) * sc[ic];
+    const cx<double> area = mk<double>(a.re / dx, a.im / dx);
+    const cx<double> t = mk<double>(-omega * area.im, omega * area.re);
+    out[e] = cast<float>(mk<double>(t.re * adm, t.im * adm));
+}
+
 }  // namespace hz

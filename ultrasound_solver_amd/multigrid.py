@@ -11,6 +11,11 @@ Each level is the finite-volume operator rediscretised on its node set:
 * 1/rho and kappa = 1/(rho c^2) restricted by normalised full weighting;
 * open faces get i omega area sqrt(kappa/rho), impedance faces i omega area / Z.
 
+By default (device_fields) the library derives the 3-D fields and the open
+faces' terms on the device from the operator's own fields; the host builds
+only the O(n) axis tables and the O(n^2) impedance terms.  The host
+restriction below is the same definition, kept for checking the device one.
+
 The library shifts every level's mass term by omega^2 -> omega^2 (1 + i beta).
 None of this exists in the reference (its only preconditioner is Jacobi,
 src/algorithm/solvers/iterative_solver.py:46-56); it is an opt-in
@@ -69,13 +74,16 @@ COARSE_MAX_NODES = 1024
 
 
 class Hierarchy:
-    def __init__(self, op, levels=None, min_size=5):
+    def __init__(self, op, levels=None, min_size=5, device_fields=True):
         self.levels = []
+        self.device_fields = bool(device_fields)
         shape = op.shape
         x = [np.arange(n, dtype=float) for n in shape]
         sigma = [np.imag(s) for s in op.stretch]
-        inv_rho = 1.0/op.rho
-        kappa = 1.0/(op.rho*op.c**2)
+        inv_rho = kappa = None
+        if not self.device_fields:
+            inv_rho = 1.0/op.rho
+            kappa = 1.0/(op.rho*op.c**2)
         kinds = op.kinds
         while True:
             lev = {"shape": shape, "inv_rho": inv_rho, "kappa": kappa}
@@ -93,10 +101,11 @@ class Hierarchy:
             if min(shape) <= min_size or (levels is None and (len(self.levels) >= 8
                                                                or int(np.prod(shape)) <= COARSE_MAX_NODES)):
                 break
-            mats = [prolongation_1d(n) for n in shape]
             idx = [coarse_nodes(n) for n in shape]
-            inv_rho = _restrict_field(inv_rho, mats)
-            kappa = _restrict_field(kappa, mats)
+            if not self.device_fields:
+                mats = [prolongation_1d(n) for n in shape]
+                inv_rho = _restrict_field(inv_rho, mats)
+                kappa = _restrict_field(kappa, mats)
             x = [x[a][idx[a]] for a in range(3)]
             sigma = [sigma[a][idx[a]] for a in range(3)]
             shape = tuple(len(i) for i in idx)
@@ -115,6 +124,9 @@ class Hierarchy:
             sel[axis] = 0 if index % 2 == 0 else -1
             sel = tuple(sel)
             if kind == "open":
+                if self.device_fields:
+                    out.append(None)  # derived on the device
+                    continue
                 adm = np.sqrt(lev["kappa"][sel]*lev["inv_rho"][sel])
             else:
                 z = np.asarray(op.boundaries[physics.FACES[index]]["impedance"], float)
@@ -131,10 +143,14 @@ class Hierarchy:
             return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
         descs = (_capi.LevelDesc*len(self.levels))()
+        null = ctypes.POINTER(ctypes.c_double)()
         for d, lev in zip(descs, self.levels):
             d.nx, d.ny, d.nz = lev["shape"]
-            d.rho = ptr((1.0/lev["inv_rho"]).ravel(order="F"))
-            d.kappa = ptr(lev["kappa"].ravel(order="F"))
+            if self.device_fields:
+                d.rho = d.kappa = null
+            else:
+                d.rho = ptr((1.0/lev["inv_rho"]).ravel(order="F"))
+                d.kappa = ptr(lev["kappa"].ravel(order="F"))
             for a in range(3):
                 d.weight[a] = ptr(lev["weight"][a])
                 d.stretch[a] = ptr(lev["stretch"][a].astype(np.complex128).view(np.float64))

@@ -270,20 +270,43 @@ class HelmholtzOperator:
                                                ctypes.c_void_p(out.data_ptr()), x.shape[0], _stream()), "hz_mg_transfer")
         return out
 
+    def mg_level(self, level):
+        """Level `level` of the installed hierarchy as device tensors (hz_mg_level):
+        {"rho", "kappa": (n_l,) float32, "faces": {f: (n_b * n_c,) complex64}}."""
+        if self.hierarchy is None:
+            raise RuntimeError("set_multigrid() first")
+        shape = self.hierarchy.levels[level]["shape"]
+        n = int(np.prod(shape))
+        rho = torch.empty(n, dtype=torch.float32, device=self.device)
+        kappa = torch.empty(n, dtype=torch.float32, device=self.device)
+        faces = {}
+        for f in range(6):
+            ax = f//2
+            b, c = [a for a in range(3) if a != ax]
+            faces[f] = torch.empty(shape[b]*shape[c], dtype=torch.complex64, device=self.device)
+        ptrs = (ctypes.c_void_p*6)(*[faces[f].data_ptr() for f in range(6)])
+        mask = ctypes.c_int32()
+        _capi.check(_capi.lib().hz_mg_level(self.handle, int(level), ctypes.c_void_p(rho.data_ptr()),
+                                            ctypes.c_void_p(kappa.data_ptr()), ptrs, ctypes.byref(mask), _stream()),
+                    "hz_mg_level")
+        return {"rho": rho, "kappa": kappa, "faces": {f: t for f, t in faces.items() if mask.value >> f & 1}}
+
     def set_tile_shapes(self, shapes):
         """Pin the slab tile shapes (the layout `autotune` reports)."""
         arr = (ctypes.c_int32*len(shapes))(*[int(v) for v in shapes])
         _capi.check(_capi.lib().hz_set_tile_shapes(self.handle, arr, len(shapes)), "hz_set_tile_shapes")
 
     def set_multigrid(self, shift=1.2, pre=1, post=1, coarse_sweeps=30, weight=0.7, levels=None, cycle=1,
-                      post_coarse=5):
+                      post_coarse=5, device_fields=True):
         """Install the shifted-Laplacian multigrid preconditioner (mixed form):
         `post` damped-Jacobi sweeps after the correction on the finest level,
         `post_coarse` on the levels below it (coarse sweeps cost an eighth per
-        level); cycle 2 makes it a W-cycle below the finest level."""
+        level); cycle 2 makes it a W-cycle below the finest level.  With
+        device_fields the coarse materials and open-face terms are derived on
+        the device; otherwise on the host (same definition)."""
         from .multigrid import Hierarchy
-        self.hierarchy = Hierarchy(self, levels=levels).install(self, shift, pre, post, coarse_sweeps, weight, cycle,
-                                                                post_coarse)
+        self.hierarchy = Hierarchy(self, levels=levels, device_fields=device_fields).install(
+            self, shift, pre, post, coarse_sweeps, weight, cycle, post_coarse)
         return self.hierarchy
 
     # -- sources ----------------------------------------------------------
