@@ -71,3 +71,17 @@ def test_surface_axis_matrices_match_the_oracle():
         np.testing.assert_allclose(got, want, rtol=1e-14, atol=0)
     with pytest.raises(ValueError, match="open/reflecting/impedance"):
         surface.axis_operators((10, 4, 4), 1e-3, 1.0, 1.2, 343.0, {"-x": "soft"}, {})
+
+
+def test_iterative_solver_options_and_their_checks():
+    """The backend's optional solver keys and their validation (no device
+    work: GpuIterativeSolver only stores them)."""
+    from ultrasound_solver_amd.solvers import GpuIterativeSolver
+    s = GpuIterativeSolver(precision="mixed", preconditioner="multigrid", chunk="auto", mixed_restart=12,
+                           relative_rtol=1e-6)
+    assert (s.chunk, s.mixed_restart, s.restart, s.relative_rtol) == (0, 12, 30, 1e-6)
+    assert GpuIterativeSolver().relative_rtol == 0.0
+    for bad, msg in ((dict(relative_rtol=-1.0), "relative_rtol"), (dict(precision="fp32"), "gmres_precision"),
+                     (dict(preconditioner="multigrid"), "mixed form"), (dict(mixed_restart=0), "restart")):
+        with pytest.raises(ValueError, match=msg):
+            GpuIterativeSolver(**bad)

@@ -24,7 +24,9 @@ Extra optional ``solver:`` keys: ``gmres_precision`` ("fp64" plain form,
 "auto" for the autotuner's choice),
 ``gmres_max_refinements``, ``gmres_inner_rtol``, ``gmres_mixed_restart`` (restart
 length of the mixed form's inner GMRES, 10 by default; ``gmres_restart`` is the
-plain form's, 30 as in the reference).
+plain form's, 30 as in the reference), ``gmres_relative_rtol`` (> 0: a column
+also needs FP64 ||b - Ax|| / ||b|| at most this to count as converged; 0, the
+default, keeps the reference's acceptance test alone).
 """
 import hashlib
 import json
@@ -73,7 +75,7 @@ class GpuIterativeSolver:
 
     def __init__(self, restart=30, maxiter=3000, rtol=1.0e-6, backward_error_tol=1.0e-4,
                  precision="fp64", chunk=8, preconditioner="jacobi", max_refinements=10, inner_rtol=1.0e-5,
-                 verbose=True, mixed_restart=10):
+                 verbose=True, mixed_restart=10, relative_rtol=0.0):
         self.restart = int(restart)
         # The mixed form's inner complex64 GMRES restarts every `mixed_restart`
         # steps: with the multigrid preconditioner the step count barely
@@ -97,6 +99,9 @@ class GpuIterativeSolver:
         self.preconditioner = preconditioner
         self.max_refinements = int(max_refinements)
         self.inner_rtol = float(inner_rtol)
+        self.relative_rtol = float(relative_rtol)
+        if not (self.relative_rtol >= 0.0):
+            raise ValueError("gmres_relative_rtol must be >= 0")
         self.verbose = verbose
         self.last_iterations = 0
         self.last_scaled_residual = None
@@ -122,7 +127,7 @@ class GpuIterativeSolver:
             backward_error_tol=self.backward_error_tol,
             precision="c128" if self.precision == "fp64" else "c64", adjoint=adjoint,
             chunk=chunk, max_refinements=self.max_refinements,
-            inner_rtol=self.inner_rtol, preconditioner=self.preconditioner)
+            inner_rtol=self.inner_rtol, preconditioner=self.preconditioner, relative_rtol=self.relative_rtol)
         self.last_stats = stats
         self.last_iterations = max(s["iterations"] for s in stats)
         self.last_scaled_residual = max(s["scaled_residual"] for s in stats)
@@ -178,7 +183,8 @@ class HelmholtzSolver:
             preconditioner=str(s.get("gmres_preconditioner", "jacobi")),
             max_refinements=int(s.get("gmres_max_refinements", 10)),
             inner_rtol=float(s.get("gmres_inner_rtol", 1.0e-5)),
-            mixed_restart=int(s.get("gmres_mixed_restart", 10)))
+            mixed_restart=int(s.get("gmres_mixed_restart", 10)),
+            relative_rtol=float(s.get("gmres_relative_rtol", 0.0)))
 
     # -- scene ------------------------------------------------------------
     @property
