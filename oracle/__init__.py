@@ -31,4 +31,9 @@ reference ``file:line`` it follows.  How each part is pinned:
   Gaussian target, background mask and trilinear target response: the
   archived run's stored loss values and target fields; the phase VJP by
   central finite differences and the worked example u = 3+4i.
+  focal_pressure against the reference's derived closed form
+  (docs/GRADIENT_DERIVATION.html:914-930) on a constant field and a field
+  with one nonzero node; focal_gradient by a two-emitter closed form; the
+  ratio form of focal contrast (no reference code) by its value on
+  piecewise-constant fields, scale invariance and central differences.
 """
