@@ -185,6 +185,30 @@ hz_status hz_gmres_batch(int32_t nscenes, hz_operator *const *ops, const hz_gmre
                          const void *const *b, void *const *x, const int32_t *nrhs, const int64_t *ld,
                          hz_gmres_stats *const *stats, int32_t workers, void *stream);
 
+/* Several independent scenes in one lockstep solve (mixed form, HZ_C64):
+ * arguments as hz_gmres_batch.  Every scene's chunk of columns iterates in
+ * the same GMRES cycles, and each kernel of a cycle runs once for all the
+ * scenes still iterating: one grid over every scene's workgroups, each
+ * reading its scene's arguments from a per-scene descriptor table (up to 32
+ * scenes per launch; more take several).  With opts->use_graph each cycle
+ * body of the whole batch is one graph launch.  Columns are taken in rounds
+ * of opts->chunk per scene; the workspaces of all scenes are held together
+ * during the call and released on return.  Per-scene results are those of
+ * hz_gmres on the scene alone.  No counterpart in the reference: its solver
+ * handles one scene per process (src/algorithm/solvers/helmholtz_solver.py:
+ * 589-604). */
+hz_status hz_gmres_fused(int32_t nscenes, hz_operator *const *ops, const hz_gmres_opts *opts,
+                         const void *const *b, void *const *x, const int32_t *nrhs, const int64_t *ld,
+                         hz_gmres_stats *const *stats, void *stream);
+
+/* Launch counters of the solver since the last reset, process-wide:
+ * counts[0] graph launches, counts[1] fused multi-scene kernel launches,
+ * counts[2] single-scene kernel launches (launches recorded into a graph
+ * count once per graph launch, as counts[0]).  reset != 0 zeroes them after
+ * reading.  Instrumentation of hz_gmres and hz_gmres_fused; no counterpart
+ * in the reference. */
+hz_status hz_launch_counts(int64_t *counts, int32_t reset);
+
 hz_status hz_set_sources(hz_operator *op, int32_t num_columns, int64_t nnz,
                          const int64_t *node, const int32_t *column, const double *value);
 

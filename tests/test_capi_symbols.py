@@ -47,6 +47,10 @@ def test_bad_arguments_are_rejected_without_touching_the_device():
     status = _capi.lib().hz_create(None, ctypes.byref(handle))
     assert status == _capi.HZ_BAD_ARGUMENT
     assert b"null" in _capi.lib().hz_last_error()
+    # The fused multi-scene solve validates its arrays before any device call.
+    status = _capi.lib().hz_gmres_fused(0, None, None, None, None, None, None, None, None)
+    assert status == _capi.HZ_BAD_ARGUMENT
+    assert b"nscenes" in _capi.lib().hz_last_error()
 
 
 def test_operator_exposes_every_entry_point():
@@ -56,3 +60,5 @@ def test_operator_exposes_every_entry_point():
                  "set_multigrid", "set_sources", "build_rhs", "gather_sources", "sample", "device_bytes"):
         assert callable(getattr(operator.HelmholtzOperator, name)), name
     assert callable(operator.gmres_batch)
+    assert callable(operator.gmres_fused)
+    assert callable(operator.launch_counts)

@@ -1,0 +1,133 @@
+"""Lockstep multi-scene solves (hz_gmres_fused).
+
+Every kernel of a GMRES cycle runs once for all the scenes still iterating:
+one grid over all their workgroups, each workgroup reading its scene's
+arguments from a per-scene table.  Per scene the operations are those of
+the scene's own solve, and their rounding is set by the source (the library
+contracts multiply-adds only within one expression, and the hot loops spell
+theirs out as FMAs), so the fields, iteration counts and refinement counts
+must equal hz_gmres on each scene alone bit for bit: with and without graph
+capture, for both preconditioners, for scenes whose multigrid hierarchies
+have different depths and for scenes with different column counts (the last
+round holds fewer scenes).  The converged fields are also checked against
+the CPU oracle: FP64 residuals of the oracle's complex128 matrix.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from oracle import helmholtz as ohz, krylov as okr
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def small_scene(n, seed):
+    rng = np.random.default_rng(seed)
+    rho = np.full((n, n, n), 1.2)
+    rho[n//3:n//2, n//3:n//2, n//2:] = 1250.0
+    c = 343.0*(1 + 0.03*rng.standard_normal((n, n, n)))
+    bnd = {"-x": "open", "+x": "reflecting", "-y": "open", "+y": {"type": "impedance", "impedance": [823.2, 205.8]},
+           "+z": "open"}
+    return rho, c, bnd, {"cells": 3}
+
+
+def make_op(n, seed, multigrid):
+    from ultrasound_solver_amd.operator import HelmholtzOperator
+    rho, c, bnd, pml = small_scene(n, seed)
+    op = HelmholtzOperator(rho, c, 1.07e-3, 2*np.pi*40e3, bnd, pml, device="cuda:0")
+    if multigrid:
+        op.set_multigrid()
+    return op
+
+
+def rhs(n, k, seed):
+    rng = np.random.default_rng(100 + seed)
+    b = np.zeros((k, n**3), dtype=complex)
+    for r in range(k):
+        b[r, rng.integers(0, n**3, 4)] = rng.standard_normal(4) + 1j*rng.standard_normal(4)
+    return torch.tensor(b, device="cuda:0")
+
+
+def same_results(fused, single):
+    for (xf, sf), (xs, ss) in zip(fused, single):
+        assert torch.equal(xf, xs)
+        for a, b in zip(sf, ss):
+            assert (a["iterations"], a["cycles"], a["refinements"], a["converged"]) == \
+                   (b["iterations"], b["cycles"], b["refinements"], b["converged"])
+            assert a["relative_residual"] == b["relative_residual"]
+
+
+@pytest.mark.parametrize("preconditioner", ["jacobi", "multigrid"])
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_fused_small_scenes_equal_their_own_solves(preconditioner, use_graph):
+    """Three scenes of 14^3, 19^3 and 24^3 (odd and even widths; 2, 3 and 3
+    multigrid levels) with 3, 2 and 1 columns in chunks of 2."""
+    from ultrasound_solver_amd.operator import gmres_fused
+    sizes, ks = (14, 19, 24), (3, 2, 1)
+    ops = [make_op(n, i, preconditioner == "multigrid") for i, n in enumerate(sizes)]
+    bs = [rhs(n, k, i) for i, (n, k) in enumerate(zip(sizes, ks))]
+    opts = dict(rtol=1e-8, precision="c64", chunk=2, maxiter=20000, restart=10, preconditioner=preconditioner,
+                use_graph=use_graph, relative_rtol=1e-8)
+    print(f"fused small: {preconditioner}, graph {use_graph}", flush=True)
+    single = [op.gmres(b, **opts) for op, b in zip(ops, bs)]
+    fused = gmres_fused(ops, bs, **opts)
+    same_results(fused, single)
+    for (x, st), op, b, n, i in zip(fused, ops, bs, sizes, range(3)):
+        assert all(v["converged"] for v in st)
+        rho, c, bnd, pml = small_scene(n, i)
+        a = ohz.OracleOperator(rho, c, 1.07e-3, 2*np.pi*40e3, bnd, pml).matrix()
+        xh, bh = x.cpu().numpy(), b.cpu().numpy()
+        for r in range(x.shape[0]):
+            assert np.linalg.norm(bh[r] - a @ xh[r])/np.linalg.norm(bh[r]) <= 1e-8
+            assert okr.scaled_residual(a, xh[r], bh[r]) <= 1e-8
+
+
+def test_fused_uneven_scenes_equal_their_own_solves():
+    """Scenes 0, 1 and 5 of the uneven batch (128^3, 64^3 and 96^3; 5, 4 and
+    5 multigrid levels), two columns each to 1e-8, cycle bodies captured as
+    graphs: the fused solve equals each scene's own hz_gmres bit for bit."""
+    from ultrasound_solver_amd import scenes
+    from ultrasound_solver_amd.operator import gmres_fused
+    from test_gpu_workloads import build, unit_columns, _fp64_relative_residuals
+    t0 = time.perf_counter()
+    sc = [scenes.config_uneven(i) for i in (0, 1, 5)]
+    assert [s.shape[0] for s in sc] == [128, 64, 96]
+    ops = [build(s, multigrid=True) for s in sc]
+    bs = [unit_columns(op, [0, 63]) for op in ops]
+    opts = dict(rtol=1e-8, precision="c64", chunk=2, maxiter=60000, preconditioner="multigrid", use_graph=True,
+                relative_rtol=1e-8)
+    single = [op.gmres(b, **opts) for op, b in zip(ops, bs)]
+    print(f"uneven 3: single solves at {time.perf_counter() - t0:.1f}s", flush=True)
+    fused = gmres_fused(ops, bs, **opts)
+    print(f"uneven 3: fused solve at {time.perf_counter() - t0:.1f}s", flush=True)
+    same_results(fused, single)
+    for (x, st), op, b in zip(fused, ops, bs):
+        rel = _fp64_relative_residuals(op, x, b)
+        assert all(r <= 1e-8 for r in rel)
+
+
+def test_fused_batch_of_all_32_scenes():
+    """All 32 scenes of the uneven batch in one hz_gmres_fused call (each
+    cycle one graph launch for the whole batch), two columns each: every
+    column meets the reference acceptance and the FP64 relative residual
+    1e-6, and an independent FP64 residual matches the reported one."""
+    from ultrasound_solver_amd import scenes
+    from ultrasound_solver_amd.operator import gmres_fused
+    from test_gpu_workloads import build, unit_columns, _fp64_relative_residuals
+    t0 = time.perf_counter()
+    sc = [scenes.config_uneven(i) for i in range(32)]
+    ops = [build(s, multigrid=True) for s in sc]
+    bs = [unit_columns(op, [0, 63]) for op in ops]
+    print(f"fused 32: set up in {time.perf_counter() - t0:.1f}s", flush=True)
+    out = gmres_fused(ops, bs, rtol=1e-6, precision="c64", chunk=2, maxiter=60000, preconditioner="multigrid",
+                      use_graph=True, relative_rtol=1e-6)
+    print(f"fused 32: solved at {time.perf_counter() - t0:.1f}s", flush=True)
+    for (x, st), op, b, s in zip(out, ops, bs, sc):
+        assert all(v["converged"] for v in st), (s.name, st)
+        rel = _fp64_relative_residuals(op, x, b)
+        for r in range(2):
+            assert st[r]["scaled_residual"] <= 1e-6 and rel[r] <= 1e-6
+            assert rel[r] == pytest.approx(st[r]["relative_residual"], rel=1e-6, abs=1e-14)
+        op.release_workspace()

@@ -16,7 +16,8 @@ MODE_A, MODE_RAM, MODE_AH, MODE_RAMH, MODE_RA, MODE_RAH = 0, 1, 2, 3, 4, 5
 
 EXPORTS = ("hz_create", "hz_destroy", "hz_apply", "hz_gmres", "hz_set_sources", "hz_build_rhs",
            "hz_gather_sources", "hz_sample", "hz_device_bytes", "hz_last_error", "hz_set_multigrid",
-           "hz_autotune", "hz_set_tile_shapes", "hz_gmres_batch", "hz_release_workspace",
+           "hz_autotune", "hz_set_tile_shapes", "hz_gmres_batch", "hz_gmres_fused", "hz_launch_counts",
+           "hz_release_workspace",
            "hz_material_vjp", "hz_basis_field", "hz_basis_adjoint", "hz_basis_field_match", "hz_basis_loss",
            "hz_basis_loss_static", "hz_mg_transfer", "hz_mg_level",
            "hz_z_transfer", "hz_surface_create", "hz_surface_destroy", "hz_surface_tables",
@@ -113,6 +114,10 @@ def lib():
         handle.hz_gmres_batch.argtypes = [c_int32, POINTER(c_void_p), POINTER(GmresOpts), POINTER(c_void_p),
                                           POINTER(c_void_p), POINTER(c_int32), POINTER(c_int64),
                                           POINTER(POINTER(GmresStats)), c_int32, c_void_p]
+        handle.hz_gmres_fused.argtypes = [c_int32, POINTER(c_void_p), POINTER(GmresOpts), POINTER(c_void_p),
+                                          POINTER(c_void_p), POINTER(c_int32), POINTER(c_int64),
+                                          POINTER(POINTER(GmresStats)), c_void_p]
+        handle.hz_launch_counts.argtypes = [POINTER(c_int64), c_int32]
         handle.hz_release_workspace.argtypes = [c_void_p]
         handle.hz_basis_field.argtypes = [c_void_p, c_int64, c_int64, c_int32, c_void_p, c_void_p, c_void_p]
         handle.hz_basis_adjoint.argtypes = [c_void_p, c_int64, c_int64, c_int32, c_void_p, c_void_p, c_void_p]

@@ -15,7 +15,12 @@ LIBRARY = HERE/"libhz.so"
 ARCH = os.environ.get("HZ_OFFLOAD_ARCH", "gfx950")
 
 # Packed-FP32 instructions are kept out of the generated code on purpose.
-FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
+# Multiply-adds are contracted into FMAs only within one source expression
+# (-ffp-contract=on; the hot loops spell theirs out), so that the rounding of
+# every kernel is set by its source, not by how the compiler schedules it:
+# a tile shape, a fused multi-scene launch and a single-scene launch of the
+# same row round it identically.
+FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}", "-ffp-contract=on",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-command-line-argument"]
 

@@ -127,6 +127,65 @@ struct Multigrid {
     int kcap = 0;
 };
 
+// Per-scene argument tables of fused launches (see Fuser): written on the
+// host by a recording pass over a launch sequence, then uploaded once and
+// read by every launch (or graph replay) of that sequence.
+struct FusePlan {
+    std::vector<char> host;
+    std::vector<size_t> offs;  // table of each fused launch, in launch order
+    DevBuf dev;
+    size_t next = 0;
+    bool recording = true;
+    void upload() {
+        if (!host.empty()) {
+            dev.ensure(host.size());
+            HZ_HIP(hipMemcpy(dev.p, host.data(), host.size(), hipMemcpyHostToDevice));
+        }
+        recording = false;
+        next = 0;
+    }
+};
+
+// Launch sequences kept for replay, keyed by everything their launches bake
+// in: captured graphs (one per GMRES cycle body) and the fused launches'
+// tables.  Bounded: the oldest entry goes first.
+struct LaunchCache {
+    struct Entry {
+        std::vector<uintptr_t> sig;
+        hipGraphExec_t exec = nullptr;
+        std::unique_ptr<FusePlan> plan;
+    };
+    std::vector<std::unique_ptr<Entry>> entries;
+    hipStream_t capture = nullptr;  // non-default stream used only for capture
+    size_t limit = 8;
+    LaunchCache() = default;
+    LaunchCache(const LaunchCache &) = delete;
+    LaunchCache &operator=(const LaunchCache &) = delete;
+    ~LaunchCache() {
+        clear();
+        if (capture) (void)hipStreamDestroy(capture);
+    }
+    void clear() {
+        for (auto &e : entries)
+            if (e->exec) (void)hipGraphExecDestroy(e->exec);
+        entries.clear();
+    }
+    Entry *find(const std::vector<uintptr_t> &sig) {
+        for (auto &e : entries)
+            if (e->sig == sig) return e.get();
+        return nullptr;
+    }
+    Entry *add(std::vector<uintptr_t> sig) {
+        if (entries.size() >= limit) {
+            if (entries.front()->exec) (void)hipGraphExecDestroy(entries.front()->exec);
+            entries.erase(entries.begin());
+        }
+        entries.push_back(std::make_unique<Entry>());
+        entries.back()->sig = std::move(sig);
+        return entries.back().get();
+    }
+};
+
 }  // namespace
 
 struct hz_operator {
@@ -154,20 +213,10 @@ struct hz_operator {
     DevBuf Z;                       // multigrid form: preconditioned basis slots, restart x k x n
     DevBuf rhs32;
     std::unique_ptr<Multigrid> mg;
-    // Captured GMRES cycle bodies, keyed by everything their launches bake in.
-    struct Graph {
-        std::vector<uintptr_t> sig;
-        hipGraphExec_t exec = nullptr;
-    };
-    std::vector<Graph> graphs;
-    hipStream_t capture = nullptr;  // non-default stream used only for capture
+    LaunchCache graphs;  // captured GMRES cycle bodies of this operator's solves
     hz_operator() = default;
     hz_operator(const hz_operator &) = delete;
     hz_operator &operator=(const hz_operator &) = delete;
-    ~hz_operator() {
-        for (Graph &g : graphs) (void)hipGraphExecDestroy(g.exec);
-        if (capture) (void)hipStreamDestroy(capture);
-    }
 };
 
 namespace {
@@ -194,35 +243,159 @@ bool paired(int64_t n) {
     return std::is_same<T, float>::value && n % 2 == 0;
 }
 
+// ------------------------------------------------------------- fused launches
+// The solver's host code sends its kernel launches to a Fuser.  Without a
+// plan it launches each kernel at once (one scene).  In a lockstep batch it
+// collects the launches the scenes make at one point of the sequence and
+// flush() issues them: one k_multi launch per distinct kernel (hz_multi.hpp),
+// or the kernel's own launch where a single scene makes it.  A scene that
+// launches again before a flush flushes first, so a fused launch holds at
+// most one launch per scene and every scene's own launch order is kept.
+// Launch counters (hz_launch_counts): graph launches, fused (k_multi)
+// kernel launches and single-scene kernel launches made on a stream; a
+// sequence being captured counts once per graph launch instead.
+std::atomic<int64_t> g_launches[3];
+
+template <class K>
+void launch_multi(const void *table, const MultiGrid &mg, unsigned blocks, hipStream_t s) {
+    hipLaunchKernelGGL((k_multi<K>), dim3(blocks), dim3(K::THREADS), 0, s,
+                       static_cast<const typename K::Args *>(table), mg);
+}
+template <class K>
+void launch_single(dim3 grid, hipStream_t s, const void *args) {
+    K::single(grid, s, *static_cast<const typename K::Args *>(args));
+}
+
+class Fuser {
+  public:
+    // counted: launches count in the launch counters (not under capture).
+    explicit Fuser(hipStream_t s, FusePlan *plan = nullptr, bool counted = true)
+        : s_(s), plan_(plan), counted_(counted) {}
+    Fuser(const Fuser &) = delete;
+    Fuser &operator=(const Fuser &) = delete;
+    hipStream_t stream() const { return s_; }
+
+    template <class K>
+    void add(int scene, dim3 grid, const typename K::Args &a) {
+        if (!plan_) {
+            K::single(grid, s_, a);
+            HZ_LAUNCHED();
+            if (counted_) ++g_launches[2];
+            return;
+        }
+        static_assert(std::is_trivially_copyable<typename K::Args>::value, "launch arguments are copied as bytes");
+        if (std::find(pending_.begin(), pending_.end(), scene) != pending_.end()) flush();
+        pending_.push_back(scene);
+        const void *key = reinterpret_cast<const void *>(&launch_multi<K>);
+        Group *g = nullptr;
+        for (Group &x : groups_)
+            if (x.key == key) g = &x;
+        if (!g) {
+            groups_.push_back(Group{key, sizeof(typename K::Args), &launch_multi<K>, &launch_single<K>, {}, {}});
+            g = &groups_.back();
+        }
+        const char *p = reinterpret_cast<const char *>(&a);
+        g->args.insert(g->args.end(), p, p + sizeof(a));
+        g->grids.push_back(grid);
+    }
+
+    // Device-to-device copy, in launch order.
+    void copy(void *dst, const void *src, size_t bytes) {
+        flush();
+        if (plan_ && plan_->recording) return;
+        HZ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s_));
+    }
+
+    void flush() {
+        pending_.clear();
+        if (groups_.empty()) return;
+        std::vector<Group> gs;
+        gs.swap(groups_);
+        const bool rec = plan_->recording;
+        for (Group &g : gs) {
+            const int count = (int)g.grids.size();
+            if (count == 1) {
+                if (!rec) {
+                    g.one(g.grids[0], s_, g.args.data());
+                    HZ_LAUNCHED();
+                    if (counted_) ++g_launches[2];
+                }
+                continue;
+            }
+            for (int c0 = 0; c0 < count; c0 += MULTI_MAX) {
+                const int cnt = std::min(MULTI_MAX, count - c0);
+                MultiGrid mg{};
+                mg.count = cnt;
+                int64_t total = 0;
+                for (int i = 0; i < cnt; ++i) {
+                    const dim3 gd = g.grids[c0 + i];
+                    mg.first[i] = (int)total;
+                    mg.gx[i] = (int)gd.x;
+                    mg.gy[i] = (int)gd.y;
+                    mg.gz[i] = (int)gd.z;
+                    total += ((int64_t)gd.x * gd.y * gd.z + 7) / 8 * 8;
+                }
+                if (total >= (int64_t(1) << 31)) throw BadArgument("fused launch too large");
+                for (int i = cnt; i <= MULTI_MAX; ++i) mg.first[i] = (int)total;
+                const size_t bytes = (size_t)cnt * g.size;
+                if (rec) {
+                    const size_t off = (plan_->host.size() + 255) / 256 * 256;
+                    plan_->host.resize(off + bytes);
+                    std::memcpy(plan_->host.data() + off, g.args.data() + (size_t)c0 * g.size, bytes);
+                    plan_->offs.push_back(off);
+                } else {
+                    if (plan_->next >= plan_->offs.size())
+                        throw std::logic_error("fused launch sequence differs from its recording");
+                    g.multi(plan_->dev.as<char>() + plan_->offs[plan_->next++], mg, (unsigned)total, s_);
+                    HZ_LAUNCHED();
+                    if (counted_) ++g_launches[1];
+                }
+            }
+        }
+    }
+
+  private:
+    struct Group {
+        const void *key;
+        size_t size;
+        void (*multi)(const void *, const MultiGrid &, unsigned, hipStream_t);
+        void (*one)(dim3, hipStream_t, const void *);
+        std::vector<char> args;
+        std::vector<dim3> grids;
+    };
+    hipStream_t s_;
+    FusePlan *plan_;
+    bool counted_;
+    std::vector<Group> groups_;
+    std::vector<int> pending_;
+};
+
+// ---------------------------------------------------------------- launchers
+// Each takes the Fuser and the scene index of the launch.
 template <typename T>
-void launch_dots(dim3 grid, hipStream_t s, const cx<T> *V, int k, int64_t n, int jp1, const cx<T> *w,
+void launch_dots(Fuser &f, int sc, dim3 grid, const cx<T> *V, int k, int64_t n, int jp1, const cx<T> *w,
                  const int *live, double2 *part, int maxv, int64_t chunk) {
     if constexpr (std::is_same<T, float>::value) {
         if (paired<T>(n)) {
-            hipLaunchKernelGGL((k_dots<T, 2>), grid, dim3(256), 0, s, V, k, n, jp1, w, live, part, maxv, chunk);
-            HZ_LAUNCHED();
+            f.add<KDots<T, 2>>(sc, grid, {V, k, n, jp1, w, live, part, maxv, chunk});
             return;
         }
     }
-    hipLaunchKernelGGL((k_dots<T, 1>), grid, dim3(256), 0, s, V, k, n, jp1, w, live, part, maxv, chunk);
-    HZ_LAUNCHED();
+    f.add<KDots<T, 1>>(sc, grid, {V, k, n, jp1, w, live, part, maxv, chunk});
 }
 
 template <typename T>
-void launch_update(dim3 grid, hipStream_t s, const cx<T> *V, int k, int64_t n, int cnt_all, const int *cnt_rhs,
+void launch_update(Fuser &f, int sc, dim3 grid, const cx<T> *V, int k, int64_t n, int cnt_all, const int *cnt_rhs,
                    cx<T> *target, int64_t ldt, const cx<double> *coef, int maxv, const int *live, double *norm_partial,
                    int64_t chunk) {
     if constexpr (std::is_same<T, float>::value) {
         if (paired<T>(n) && ldt % 2 == 0) {
-            hipLaunchKernelGGL((k_update<T, 2>), grid, dim3(256), 0, s, V, k, n, cnt_all, cnt_rhs, target, ldt, coef,
-                               maxv, live, norm_partial, chunk);
-            HZ_LAUNCHED();
+            f.add<KUpdate<T, 2>>(sc, grid, {V, k, n, cnt_all, cnt_rhs, target, ldt, coef, maxv, live, norm_partial,
+                                            chunk});
             return;
         }
     }
-    hipLaunchKernelGGL((k_update<T, 1>), grid, dim3(256), 0, s, V, k, n, cnt_all, cnt_rhs, target, ldt, coef, maxv,
-                       live, norm_partial, chunk);
-    HZ_LAUNCHED();
+    f.add<KUpdate<T, 1>>(sc, grid, {V, k, n, cnt_all, cnt_rhs, target, ldt, coef, maxv, live, norm_partial, chunk});
 }
 
 // Workgroups of the per-node multigrid transfers: one per 256 nodes of an
@@ -240,23 +413,21 @@ int transfer_group(const MgLevel &C, int k) {
     return (k + groups - 1) / groups;
 }
 
-void launch_restrict(const MgLevel &L, const MgLevel &C, const cx<float> *r, cx<float> *bc, int k, const int *live,
-                     hipStream_t s) {
+void launch_restrict(Fuser &f, int sc, const MgLevel &L, const MgLevel &C, const cx<float> *r, cx<float> *bc, int k,
+                     const int *live) {
     // z-marching restriction: one workgroup per 256 coarse nodes of a plane,
     // chunk of RZ_CHUNK coarse planes and slot.
     const int64_t t = (int64_t)((C.nx * C.ny + 255) / 256) * ((C.nz + RZ_CHUNK - 1) / RZ_CHUNK) * k;
     const int nb = (int)((t + 7) / 8 * 8);
     if (L.nx % 2 == 0)
-        hipLaunchKernelGGL((k_restrict_z<float, 1>), dim3(nb), dim3(256), 0, s, r, L.nx, L.ny, L.nz, bc, C.nx, C.ny,
-                           C.nz, k, live, L.v.soft, nb);
+        f.add<KRestrictZ<float, 1>>(sc, dim3(nb), {r, L.nx, L.ny, L.nz, bc, C.nx, C.ny, C.nz, k, live, L.v.soft, nb});
     else
-        hipLaunchKernelGGL((k_restrict_z<float, 0>), dim3(nb), dim3(256), 0, s, r, L.nx, L.ny, L.nz, bc, C.nx, C.ny,
-                           C.nz, k, live, L.v.soft, nb);
-    HZ_LAUNCHED();
+        f.add<KRestrictZ<float, 0>>(sc, dim3(nb), {r, L.nx, L.ny, L.nz, bc, C.nx, C.ny, C.nz, k, live, L.v.soft, nb});
 }
 
-void launch_prolong(const MgLevel &L, const MgLevel &C, const cx<float> *xc, cx<float> *xf, int k, const int *live,
-                    const cx<float> *b, const cx<float> *dinv, float w, const float *bdiv, int conj_d, hipStream_t s) {
+void launch_prolong(Fuser &f, int sc, const MgLevel &L, const MgLevel &C, const cx<float> *xc, cx<float> *xf, int k,
+                    const int *live, const cx<float> *b, const cx<float> *dinv, float w, const float *bdiv,
+                    int conj_d) {
     // One thread per coarse cell and group of rs slots: the slots of a group
     // share the cell's weights and first-sweep factors.  At most 8 slots per
     // group (while the grid keeps >= 2048 workgroups): the 128^3 -> 65^3
@@ -265,19 +436,18 @@ void launch_prolong(const MgLevel &L, const MgLevel &C, const cx<float> *xc, cx<
     const int rs = std::min(8, transfer_group(C, k));
     const int nb = plane_blocks(C.nx, C.ny, C.nz, (k + rs - 1) / rs);
     if (L.nx % 2 == 0)
-        hipLaunchKernelGGL((k_prolong<float, 1>), dim3(nb), dim3(256), 0, s, xc, C.nx, C.ny, C.nz, xf, L.nx, L.ny,
-                           L.nz, k, rs, live, L.v.soft, b, dinv, w, bdiv, conj_d, nb);
+        f.add<KProlong<float, 1>>(sc, dim3(nb), {xc, C.nx, C.ny, C.nz, xf, L.nx, L.ny, L.nz, k, rs, live, L.v.soft, b,
+                                                 dinv, w, bdiv, conj_d, nb});
     else
-        hipLaunchKernelGGL((k_prolong<float, 0>), dim3(nb), dim3(256), 0, s, xc, C.nx, C.ny, C.nz, xf, L.nx, L.ny,
-                           L.nz, k, rs, live, L.v.soft, b, dinv, w, bdiv, conj_d, nb);
-    HZ_LAUNCHED();
+        f.add<KProlong<float, 0>>(sc, dim3(nb), {xc, C.nx, C.ny, C.nz, xf, L.nx, L.ny, L.nz, k, rs, live, L.v.soft, b,
+                                                 dinv, w, bdiv, conj_d, nb});
 }
 
 // One slab-stencil launch with tile shape `shape` (index into SLAB_SHAPE).
 template <int MODE, int EPI>
-void launch_slab(int shape, const OpView<float> &v, const cx<float> *x, int64_t ldx, cx<float> *y, int64_t ldy,
-                 int k, const int *live, const double *scale, const cx<float> *b, const cx<float> *dinv, float w,
-                 const float *bdiv, hipStream_t s) {
+void launch_slab(Fuser &f, int sc, int shape, const OpView<float> &v, const cx<float> *x, int64_t ldx, cx<float> *y,
+                 int64_t ldy, int k, const int *live, const double *scale, const cx<float> *b, const cx<float> *dinv,
+                 float w, const float *bdiv) {
     // Paired tiles move two nodes per 16-byte access: even widths and
     // 16-byte aligned columns only.
     auto a16 = [](const void *ptr) { return ((uintptr_t)ptr & 15) == 0; };
@@ -285,14 +455,14 @@ void launch_slab(int shape, const OpView<float> &v, const cx<float> *x, int64_t 
         !(v.nx % 2 == 0 && ldx % 2 == 0 && ldy % 2 == 0 && a16(x) && a16(y) && a16(b)))
         shape = PAIR_FALLBACK;
     const TileMap tm = slab_map(v.nx, v.ny, v.nz, shape, k);
-    const dim3 grid(tm.nblocks * tm.csplit), block(64 * SLAB_SHAPE[shape].ty);
+    const dim3 grid(tm.nblocks * tm.csplit);
 #define HZ_SLAB_CASE(ID, ZB, TY, RP, NT, FLAT)                                                                     \
     case ID:                                                                                                       \
         static_assert(SLAB_SHAPE[ID].zb == ZB && SLAB_SHAPE[ID].ty == TY && SLAB_SHAPE[ID].rp == RP &&             \
                           SLAB_SHAPE[ID].nt == NT && SLAB_SHAPE[ID].flat == FLAT,                                  \
                       "slab shape table");                                                                         \
-        hipLaunchKernelGGL((k_slab<MODE, EPI, ZB, TY, RP, NT, FLAT>), grid, block, 0, s, v, tm, x, ldx, y, ldy, k,   \
-                           live, scale, b, dinv, w, bdiv);                                                         \
+        f.add<KSlab<MODE, EPI, ZB, TY, RP, NT, FLAT>>(sc, grid, {v, tm, x, ldx, y, ldy, k, live, scale, b, dinv, w, \
+                                                                 bdiv});                                           \
         break;
     switch (shape) {
         HZ_SLAB_CASE(0, 4, 4, 1, 0, 0)
@@ -314,8 +484,7 @@ void launch_slab(int shape, const OpView<float> &v, const cx<float> *x, int64_t 
         static_assert(SLAB_SHAPE[ID].zb == ZB && SLAB_SHAPE[ID].ty == TY && SLAB_SHAPE[ID].rp == 1 &&              \
                           SLAB_SHAPE[ID].nt == NT && SLAB_SHAPE[ID].flat == 4,                                     \
                       "slab shape table");                                                                         \
-        hipLaunchKernelGGL((k_slab2<MODE, EPI, ZB, TY, NT>), grid, block, 0, s, v, tm, x, ldx, y, ldy, k, live,      \
-                           scale, b, dinv, w, bdiv);                                                               \
+        f.add<KSlab2<MODE, EPI, ZB, TY, NT>>(sc, grid, {v, tm, x, ldx, y, ldy, k, live, scale, b, dinv, w, bdiv});  \
         break;
         HZ_SLAB2_CASE(14, 2, 4, 1)
         HZ_SLAB2_CASE(15, 2, 8, 1)
@@ -325,71 +494,56 @@ void launch_slab(int shape, const OpView<float> &v, const cx<float> *x, int64_t 
             throw BadArgument("unknown slab tile shape");
     }
 #undef HZ_SLAB_CASE
-    HZ_LAUNCHED();
 }
 
 template <typename T, int MODE>
-void launch_apply(const hz_operator *op, const cx<T> *x, int64_t ldx, cx<T> *y, int64_t ldy, int nrhs,
-                  const int *live, const double *scale, hipStream_t s) {
+void launch_apply(Fuser &f, int sc, const hz_operator *op, const cx<T> *x, int64_t ldx, cx<T> *y, int64_t ldy,
+                  int nrhs, const int *live, const double *scale) {
     for (int r0 = 0; r0 < nrhs; r0 += APPLY_MAX_RHS) {
         const int kk = std::min(APPLY_MAX_RHS, nrhs - r0);
         const cx<T> *xs = x + (int64_t)r0 * ldx;
         cx<T> *ys = y + (int64_t)r0 * ldy;
         const int *lv = live ? live + r0 : nullptr;
-        const double *sc = scale ? scale + r0 : nullptr;
+        const double *sc_r = scale ? scale + r0 : nullptr;
         if constexpr (std::is_same<T, float>::value) {
             // complex64: the 2.5-D slab kernel (register z-window, DPP x-neighbours)
-            launch_slab<MODE, SLAB_APPLY>(op->slab_shape, op->v32, xs, ldx, ys, ldy, kk, lv, sc, nullptr, nullptr,
-                                          0.0f, nullptr, s);
+            launch_slab<MODE, SLAB_APPLY>(f, sc, op->slab_shape, op->v32, xs, ldx, ys, ldy, kk, lv, sc_r, nullptr,
+                                          nullptr, 0.0f, nullptr);
         } else {
             // complex128: the same slab layout with FP64 row coefficients
             const TileMap tm = slab64_map(op->nx, op->ny, op->nz, kk);
-            hipLaunchKernelGGL((k_slab64<MODE, SLAB64_ZB, SLAB64_TY>), dim3(tm.nblocks * tm.csplit),
-                               dim3(64 * SLAB64_TY), 0, s, view<T>(op), tm, xs, ldx, ys, ldy, kk, lv, sc);
+            f.add<KSlab64<MODE, SLAB64_ZB, SLAB64_TY>>(sc, dim3(tm.nblocks * tm.csplit),
+                                                       {view<T>(op), tm, xs, ldx, ys, ldy, kk, lv, sc_r});
         }
-        HZ_LAUNCHED();
     }
 }
 
 template <typename T>
-void apply_mode(const hz_operator *op, int mode, const cx<T> *x, int64_t ldx, cx<T> *y, int64_t ldy, int nrhs,
-                const int *live, const double *scale, hipStream_t s) {
+void apply_mode(Fuser &f, int sc, const hz_operator *op, int mode, const cx<T> *x, int64_t ldx, cx<T> *y,
+                int64_t ldy, int nrhs, const int *live, const double *scale) {
     switch (mode) {
-        case MODE_A: launch_apply<T, MODE_A>(op, x, ldx, y, ldy, nrhs, live, scale, s); break;
-        case MODE_RAM: launch_apply<T, MODE_RAM>(op, x, ldx, y, ldy, nrhs, live, scale, s); break;
-        case MODE_AH: launch_apply<T, MODE_AH>(op, x, ldx, y, ldy, nrhs, live, scale, s); break;
-        case MODE_RAMH: launch_apply<T, MODE_RAMH>(op, x, ldx, y, ldy, nrhs, live, scale, s); break;
-        case MODE_RA: launch_apply<T, MODE_RA>(op, x, ldx, y, ldy, nrhs, live, scale, s); break;
-        case MODE_RAH: launch_apply<T, MODE_RAH>(op, x, ldx, y, ldy, nrhs, live, scale, s); break;
+        case MODE_A: launch_apply<T, MODE_A>(f, sc, op, x, ldx, y, ldy, nrhs, live, scale); break;
+        case MODE_RAM: launch_apply<T, MODE_RAM>(f, sc, op, x, ldx, y, ldy, nrhs, live, scale); break;
+        case MODE_AH: launch_apply<T, MODE_AH>(f, sc, op, x, ldx, y, ldy, nrhs, live, scale); break;
+        case MODE_RAMH: launch_apply<T, MODE_RAMH>(f, sc, op, x, ldx, y, ldy, nrhs, live, scale); break;
+        case MODE_RA: launch_apply<T, MODE_RA>(f, sc, op, x, ldx, y, ldy, nrhs, live, scale); break;
+        case MODE_RAH: launch_apply<T, MODE_RAH>(f, sc, op, x, ldx, y, ldy, nrhs, live, scale); break;
         default: throw BadArgument("unknown apply mode");
     }
 }
 
 template <typename T>
-void residual_mode(const hz_operator *op, int mode, const cx<T> *x, const cx<T> *b, cx<T> *out, int k,
-                   const int *live, double *partial, hipStream_t s) {
+void residual_mode(Fuser &f, int sc, const hz_operator *op, int mode, const cx<T> *x, const cx<T> *b, cx<T> *out,
+                   int k, const int *live, double *partial) {
     const dim3 grid(op->tm.nblocks, k);
     const int64_t n = op->n;
     switch (mode) {
-        case MODE_RAM:
-            hipLaunchKernelGGL((k_residual<T, MODE_RAM>), grid, dim3(256), 0, s, view<T>(op), op->tm, x, n, b, n,
-                               out, n, live, partial);
-            break;
-        case MODE_RAMH:
-            hipLaunchKernelGGL((k_residual<T, MODE_RAMH>), grid, dim3(256), 0, s, view<T>(op), op->tm, x, n, b, n,
-                               out, n, live, partial);
-            break;
-        case MODE_RA:
-            hipLaunchKernelGGL((k_residual<T, MODE_RA>), grid, dim3(256), 0, s, view<T>(op), op->tm, x, n, b, n,
-                               out, n, live, partial);
-            break;
-        case MODE_RAH:
-            hipLaunchKernelGGL((k_residual<T, MODE_RAH>), grid, dim3(256), 0, s, view<T>(op), op->tm, x, n, b, n,
-                               out, n, live, partial);
-            break;
+        case MODE_RAM: f.add<KResidual<T, MODE_RAM>>(sc, grid, {view<T>(op), op->tm, x, n, b, n, out, n, live, partial}); break;
+        case MODE_RAMH: f.add<KResidual<T, MODE_RAMH>>(sc, grid, {view<T>(op), op->tm, x, n, b, n, out, n, live, partial}); break;
+        case MODE_RA: f.add<KResidual<T, MODE_RA>>(sc, grid, {view<T>(op), op->tm, x, n, b, n, out, n, live, partial}); break;
+        case MODE_RAH: f.add<KResidual<T, MODE_RAH>>(sc, grid, {view<T>(op), op->tm, x, n, b, n, out, n, live, partial}); break;
         default: throw BadArgument("unknown residual mode");
     }
-    HZ_LAUNCHED();
 }
 
 // ------------------------------------------------------------------ multigrid
@@ -405,100 +559,188 @@ void mg_reserve(Multigrid &mg, int k) {
 }
 
 template <int MODE>
-void level_smooth(MgLevel &L, const cx<float> *x, const cx<float> *b, cx<float> *out, float w, int k,
-                  const int *live, const float *bdiv, hipStream_t s) {
-    launch_slab<MODE, SLAB_SMOOTH>(L.shape[0], L.v, x, L.n, out, L.n, k, live, nullptr, b, L.dinv.as<cx<float>>(), w,
-                                   bdiv, s);
+void level_smooth(Fuser &f, int sc, MgLevel &L, const cx<float> *x, const cx<float> *b, cx<float> *out, float w,
+                  int k, const int *live, const float *bdiv) {
+    launch_slab<MODE, SLAB_SMOOTH>(f, sc, L.shape[0], L.v, x, L.n, out, L.n, k, live, nullptr, b,
+                                   L.dinv.as<cx<float>>(), w, bdiv);
 }
 
-// x = V-cycle(b / bdiv) on level l; x and b hold k slots of length n_l.  The
-// result is left in `x` (which must not alias b).  bdiv (finest level only)
-// is the Krylov system's row scale.
+// One scene's V-cycle: x = V-cycle(b / bdiv) on level l of mg; x and b hold k
+// slots of length n_l (x must not alias b).  bdiv (finest level only) is the
+// Krylov system's row scale.
+struct MgTask {
+    int scene;
+    Multigrid *mg;
+    const cx<float> *b;
+    cx<float> *x;
+    int k;
+    const int *live;
+    const float *bdiv;
+};
+
+// The V-cycles of several scenes on level l, in lockstep: every launch
+// position is one flush of the Fuser, so the scenes' launches there fuse.
+// A scene whose hierarchy ends above level l+1 takes its coarsest-level
+// path while the others descend.
+struct VcState {
+    MgTask t;
+    MgLevel *L, *C;
+    bool coarsest, direct, pre1;
+    int post, first;
+    cx<float> *cur, *alt;
+};
+
 template <int MODE>
-void vcycle(Multigrid &mg, int l, const cx<float> *b, cx<float> *x, int k, const int *live, hipStream_t s,
-            const float *bdiv = nullptr) {
-    MgLevel &L = *mg.levels[l];
-    const bool coarsest = (l + 1 == (int)mg.levels.size());
-    const int64_t n = L.n;
-    if (coarsest && n <= COARSE_MAX_NODES && !bdiv) {
-        hipLaunchKernelGGL((k_coarse_jacobi<float, MODE>), dim3(k), dim3(256), 0, s, L.v, b, x,
-                           (const cx<float> *)L.dinv.as<cx<float>>(), mg.weight, mg.coarse_sweeps, live);
-        HZ_LAUNCHED();
-        return;
+void vcycle(Fuser &f, std::vector<MgTask> &tasks, int l) {
+    using State = VcState;
+    std::vector<State> st;
+    std::vector<MgTask> down;
+    for (const MgTask &t : tasks) {
+        State s{};
+        s.t = t;
+        Multigrid &mg = *t.mg;
+        s.L = mg.levels[l].get();
+        s.coarsest = (l + 1 == (int)mg.levels.size());
+        s.C = s.coarsest ? nullptr : mg.levels[l + 1].get();
+        s.direct = s.coarsest && s.L->n <= COARSE_MAX_NODES && !t.bdiv;
+        s.pre1 = !s.coarsest && mg.pre == 1;
+        s.post = l == 0 ? mg.post : mg.post_coarse;
+        st.push_back(s);
     }
-    const cx<float> *dinv = L.dinv.as<cx<float>>();
-    if (!coarsest && mg.pre == 1) {
-        // One pre-sweep: its iterate f b (f = w dinv / bdiv) is never stored.
-        // The residual b/bdiv - A (f b) comes from b in one stencil pass and
-        // the prolongation rebuilds f b + P e from b.
-        MgLevel &C = *mg.levels[l + 1];
-        cx<float> *r = L.tt.as<cx<float>>();
-        launch_slab<MODE, SLAB_RESID0>(L.shape[2], L.v, b, L.n, r, L.n, k, live, nullptr, nullptr, dinv, mg.weight, bdiv,
-                                       s);
-        launch_restrict(L, C, r, C.bb.as<cx<float>>(), k, live, s);
-        vcycle<MODE>(mg, l + 1, C.bb.as<cx<float>>(), C.xx.as<cx<float>>(), k, live, s);
-        // Post-sweeps ping-pong between x and the level scratch; start where
-        // the last one lands in x.
-        const int post = l == 0 ? mg.post : mg.post_coarse;
-        cx<float> *cur = (post % 2 == 0) ? x : L.tt.as<cx<float>>();
-        cx<float> *alt = (cur == x) ? L.tt.as<cx<float>>() : x;
-        launch_prolong(L, C, C.xx.as<cx<float>>(), cur, k, live, b, dinv, mg.weight, bdiv, (int)(MODE == MODE_AH), s);
-        for (int it = 0; it < post; ++it) {
-            level_smooth<MODE>(L, cur, b, alt, mg.weight, k, live, bdiv, s);
-            std::swap(cur, alt);
+    // Pre-smoothing: the coarsest level's direct solve; one pre-sweep folded
+    // into the residual (its iterate f b, f = w dinv / bdiv, is never
+    // stored: the residual b/bdiv - A (f b) comes from b in one stencil pass
+    // and the prolongation rebuilds f b + P e from b); or explicit Jacobi
+    // sweeps, ping-ponging between x and the level scratch so that the last
+    // one lands in x.
+    for (State &s : st) {
+        MgLevel &L = *s.L;
+        const MgTask &t = s.t;
+        const cx<float> *dinv = L.dinv.as<cx<float>>();
+        if (s.direct) {
+            f.add<KCoarseJacobi<float, MODE>>(t.scene, dim3(t.k), {L.v, t.b, t.x, dinv, t.mg->weight,
+                                                                   t.mg->coarse_sweeps, t.live});
+        } else if (s.pre1) {
+            launch_slab<MODE, SLAB_RESID0>(f, t.scene, L.shape[2], L.v, t.b, L.n, L.tt.as<cx<float>>(), L.n, t.k,
+                                           t.live, nullptr, nullptr, dinv, t.mg->weight, t.bdiv);
+        } else {
+            const int swaps = s.coarsest ? t.mg->coarse_sweeps - 1 : (t.mg->pre - 1) + s.post;
+            s.cur = (swaps % 2 == 0) ? t.x : L.tt.as<cx<float>>();
+            s.alt = (s.cur == t.x) ? L.tt.as<cx<float>>() : t.x;
+            s.first = s.coarsest ? t.mg->coarse_sweeps : t.mg->pre;
+            f.add<KLevelJacobi0<float, MODE>>(t.scene, dim3(blocks_for(L.n, 256 * 8, 2048), t.k),
+                                              {t.b, s.cur, dinv, t.mg->weight, L.n, t.live, t.bdiv});
         }
-        // W-cycle below the finest level: further coarse-grid corrections of x.
-        for (int visit = 1; l >= 1 && visit < mg.cycle; ++visit) {
-            cx<float> *t = L.tt.as<cx<float>>();
-            launch_slab<MODE, SLAB_RESID>(L.shape[1], L.v, x, L.n, t, L.n, k, live, nullptr, b, nullptr, 0.0f, bdiv,
-                                          s);
-            launch_restrict(L, C, t, C.bb.as<cx<float>>(), k, live, s);
-            vcycle<MODE>(mg, l + 1, C.bb.as<cx<float>>(), C.xx.as<cx<float>>(), k, live, s);
-            launch_prolong(L, C, C.xx.as<cx<float>>(), x, k, live, nullptr, dinv, mg.weight, bdiv, 0, s);
-            cx<float> *c2 = x, *a2 = t;
-            for (int it = 0; it < post; ++it) {
-                level_smooth<MODE>(L, c2, b, a2, mg.weight, k, live, bdiv, s);
-                std::swap(c2, a2);
+    }
+    f.flush();
+    for (int it = 1;; ++it) {
+        bool any = false;
+        for (State &s : st) {
+            if (s.direct || s.pre1 || it >= s.first) continue;
+            level_smooth<MODE>(f, s.t.scene, *s.L, s.cur, s.t.b, s.alt, s.t.mg->weight, s.t.k, s.t.live, s.t.bdiv);
+            std::swap(s.cur, s.alt);
+            any = true;
+        }
+        f.flush();
+        if (!any) break;
+    }
+    // Coarse-grid correction.
+    for (State &s : st) {
+        if (s.coarsest || s.pre1) continue;
+        launch_slab<MODE, SLAB_RESID>(f, s.t.scene, s.L->shape[1], s.L->v, s.cur, s.L->n, s.alt, s.L->n, s.t.k,
+                                      s.t.live, nullptr, s.t.b, nullptr, 0.0f, s.t.bdiv);
+    }
+    f.flush();
+    for (State &s : st) {
+        if (s.coarsest) continue;
+        const cx<float> *r = s.pre1 ? s.L->tt.as<cx<float>>() : s.alt;
+        launch_restrict(f, s.t.scene, *s.L, *s.C, r, s.C->bb.as<cx<float>>(), s.t.k, s.t.live);
+        down.push_back({s.t.scene, s.t.mg, s.C->bb.as<cx<float>>(), s.C->xx.as<cx<float>>(), s.t.k, s.t.live,
+                        nullptr});
+    }
+    f.flush();
+    if (!down.empty()) vcycle<MODE>(f, down, l + 1);
+    for (State &s : st) {
+        if (s.coarsest) continue;
+        const cx<float> *dinv = s.L->dinv.as<cx<float>>();
+        if (s.pre1) {
+            // Post-sweeps start where the last one lands in x.
+            s.cur = (s.post % 2 == 0) ? s.t.x : s.L->tt.as<cx<float>>();
+            s.alt = (s.cur == s.t.x) ? s.L->tt.as<cx<float>>() : s.t.x;
+            launch_prolong(f, s.t.scene, *s.L, *s.C, s.C->xx.as<cx<float>>(), s.cur, s.t.k, s.t.live, s.t.b, dinv,
+                           s.t.mg->weight, s.t.bdiv, (int)(MODE == MODE_AH));
+        } else {
+            launch_prolong(f, s.t.scene, *s.L, *s.C, s.C->xx.as<cx<float>>(), s.cur, s.t.k, s.t.live, nullptr, dinv,
+                           s.t.mg->weight, s.t.bdiv, 0);
+        }
+    }
+    f.flush();
+    for (int it = 0;; ++it) {
+        bool any = false;
+        for (State &s : st) {
+            if (s.coarsest || it >= s.post) continue;
+            level_smooth<MODE>(f, s.t.scene, *s.L, s.cur, s.t.b, s.alt, s.t.mg->weight, s.t.k, s.t.live, s.t.bdiv);
+            std::swap(s.cur, s.alt);
+            any = true;
+        }
+        f.flush();
+        if (!any) break;
+    }
+    // W-cycle below the finest level: further coarse-grid corrections of x
+    // (one-pre-sweep levels).
+    for (int visit = 1;; ++visit) {
+        std::vector<State *> w;
+        for (State &s : st)
+            if (l >= 1 && s.pre1 && visit < s.t.mg->cycle) w.push_back(&s);
+        if (w.empty()) break;
+        for (State *s : w)
+            launch_slab<MODE, SLAB_RESID>(f, s->t.scene, s->L->shape[1], s->L->v, s->t.x, s->L->n,
+                                          s->L->tt.as<cx<float>>(), s->L->n, s->t.k, s->t.live, nullptr, s->t.b,
+                                          nullptr, 0.0f, s->t.bdiv);
+        f.flush();
+        std::vector<MgTask> again;
+        for (State *s : w) {
+            launch_restrict(f, s->t.scene, *s->L, *s->C, s->L->tt.as<cx<float>>(), s->C->bb.as<cx<float>>(), s->t.k,
+                            s->t.live);
+            again.push_back({s->t.scene, s->t.mg, s->C->bb.as<cx<float>>(), s->C->xx.as<cx<float>>(), s->t.k,
+                             s->t.live, nullptr});
+        }
+        f.flush();
+        vcycle<MODE>(f, again, l + 1);
+        for (State *s : w) {
+            launch_prolong(f, s->t.scene, *s->L, *s->C, s->C->xx.as<cx<float>>(), s->t.x, s->t.k, s->t.live, nullptr,
+                           s->L->dinv.as<cx<float>>(), s->t.mg->weight, s->t.bdiv, 0);
+            s->cur = s->t.x;
+            s->alt = s->L->tt.as<cx<float>>();
+        }
+        f.flush();
+        for (int it = 0;; ++it) {
+            bool any = false;
+            for (State *s : w) {
+                if (it >= s->post) continue;
+                level_smooth<MODE>(f, s->t.scene, *s->L, s->cur, s->t.b, s->alt, s->t.mg->weight, s->t.k, s->t.live,
+                                   s->t.bdiv);
+                std::swap(s->cur, s->alt);
+                any = true;
             }
-            if (c2 != x) HZ_HIP(hipMemcpyAsync(x, c2, (size_t)k * n * sizeof(cx<float>), hipMemcpyDeviceToDevice, s));
+            f.flush();
+            if (!any) break;
         }
-        return;
+        for (State *s : w)
+            if (s->cur != s->t.x) f.copy(s->t.x, s->cur, (size_t)s->t.k * s->L->n * sizeof(cx<float>));
     }
-    // Jacobi sweeps ping-pong between x and the level scratch; start in the
-    // buffer that makes the last sweep land in x (no trailing copy).
-    const int post = l == 0 ? mg.post : mg.post_coarse;
-    const int swaps = coarsest ? mg.coarse_sweeps - 1 : (mg.pre - 1) + post;
-    cx<float> *cur = (swaps % 2 == 0) ? x : L.tt.as<cx<float>>();
-    cx<float> *alt = (cur == x) ? L.tt.as<cx<float>>() : x;
-    hipLaunchKernelGGL((k_level_jacobi0<float, MODE>), dim3(blocks_for(n, 256 * 8, 2048), k), dim3(256), 0, s, b,
-                       cur, dinv, mg.weight, n, live, bdiv);
-    HZ_LAUNCHED();
-    const int first = coarsest ? mg.coarse_sweeps : mg.pre;
-    for (int it = 1; it < first; ++it) {
-        level_smooth<MODE>(L, cur, b, alt, mg.weight, k, live, bdiv, s);
-        std::swap(cur, alt);
-    }
-    if (!coarsest) {
-        MgLevel &C = *mg.levels[l + 1];
-        launch_slab<MODE, SLAB_RESID>(L.shape[1], L.v, cur, L.n, alt, L.n, k, live, nullptr, b, nullptr, 0.0f, bdiv, s);
-        launch_restrict(L, C, alt, C.bb.as<cx<float>>(), k, live, s);
-        vcycle<MODE>(mg, l + 1, C.bb.as<cx<float>>(), C.xx.as<cx<float>>(), k, live, s);
-        launch_prolong(L, C, C.xx.as<cx<float>>(), cur, k, live, nullptr, dinv, mg.weight, bdiv, 0, s);
-        for (int it = 0; it < post; ++it) {
-            level_smooth<MODE>(L, cur, b, alt, mg.weight, k, live, bdiv, s);
-            std::swap(cur, alt);
-        }
-    }
-    if (cur != x) HZ_HIP(hipMemcpyAsync(x, cur, (size_t)k * n * sizeof(cx<float>), hipMemcpyDeviceToDevice, s));
+    // Explicit-sweep levels end in their scratch when the sweep count is odd.
+    for (State &s : st)
+        if (!s.direct && !s.pre1 && s.cur != s.t.x)
+            f.copy(s.t.x, s.cur, (size_t)s.t.k * s.L->n * sizeof(cx<float>));
 }
 
-// z = P R^-1 v: one V-cycle on the unscaled right-hand side.
-void precondition(hz_operator *op, bool adjoint, const cx<float> *b, cx<float> *x, int k, const int *live,
-                  hipStream_t s) {
+// z = P R^-1 v: one V-cycle of each scene on its unscaled right-hand side.
+void precondition(Fuser &f, std::vector<MgTask> &tasks, bool adjoint) {
     if (adjoint)
-        vcycle<MODE_AH>(*op->mg, 0, b, x, k, live, s, op->v32.rs_a);
+        vcycle<MODE_AH>(f, tasks, 0);
     else
-        vcycle<MODE_A>(*op->mg, 0, b, x, k, live, s, op->v32.rs_f);
+        vcycle<MODE_A>(f, tasks, 0);
 }
 
 // Everything a captured cycle bakes into its launches: precision, mode, chunk
@@ -508,7 +750,8 @@ std::vector<uintptr_t> graph_signature(const hz_operator *op, size_t elem, int m
     std::vector<uintptr_t> sig = {elem, (uintptr_t)mode, (uintptr_t)k, (uintptr_t)restart, (uintptr_t)use_mg,
                                   (uintptr_t)op->slab_shape};
     for (const DevBuf *b : {&op->V, &op->X, &op->Z, &op->dpart, &op->npart, &op->Hm, &op->gc, &op->gs, &op->g,
-                            &op->scales, &op->coef, &op->resid, &op->tol, &op->steps, &op->live, &op->member})
+                            &op->scales, &op->coef, &op->resid, &op->tol, &op->steps, &op->live, &op->member,
+                            &op->Hu, &op->coef2, &op->alpha, &op->bfix, &op->gram, &op->Bh})
         sig.push_back((uintptr_t)b->p);
     if (use_mg)
         for (const auto &L : op->mg->levels) {
@@ -518,34 +761,61 @@ std::vector<uintptr_t> graph_signature(const hz_operator *op, size_t elem, int m
     return sig;
 }
 
-// Executable graph of one cycle body, captured on first use for `sig`.  A
-// capture records launches only (no memset nodes); the graph is replayed on
-// the caller's stream.
+// Runs the launch sequence `body(Fuser &)` on stream s.  Without fusion or
+// capture it launches as it goes.  Otherwise the sequence is kept in `lc`
+// under `sig`: fused sequences are recorded once (their tables uploaded),
+// captured ones become a graph; later runs replay them.  A capture records
+// launches only (no memset nodes).
 template <typename F>
-hipGraphExec_t cycle_graph(hz_operator *op, std::vector<uintptr_t> sig, F &&body) {
-    for (auto &g : op->graphs)
-        if (g.sig == sig) return g.exec;
-    if (!op->capture) HZ_HIP(hipStreamCreateWithFlags(&op->capture, hipStreamNonBlocking));
-    HZ_HIP(hipStreamBeginCapture(op->capture, hipStreamCaptureModeThreadLocal));
-    hipGraph_t graph = nullptr;
-    try {
-        body(op->capture);
-    } catch (...) {
-        (void)hipStreamEndCapture(op->capture, &graph);
-        if (graph) (void)hipGraphDestroy(graph);
-        throw;
+void run_sequence(LaunchCache &lc, std::vector<uintptr_t> sig, bool fused, bool graph, hipStream_t s, F &&body) {
+    if (!fused && !graph) {
+        Fuser f(s);
+        body(f);
+        return;
     }
-    HZ_HIP(hipStreamEndCapture(op->capture, &graph));
-    hipGraphExec_t exec = nullptr;
-    const hipError_t err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    HZ_HIP(err);
-    if (op->graphs.size() >= 8) {  // bounded cache: drop the oldest
-        (void)hipGraphExecDestroy(op->graphs.front().exec);
-        op->graphs.erase(op->graphs.begin());
+    LaunchCache::Entry *e = lc.find(sig);
+    if (!e) {
+        e = lc.add(std::move(sig));
+        if (fused) {
+            e->plan = std::make_unique<FusePlan>();
+            Fuser rec(s, e->plan.get());
+            body(rec);
+            rec.flush();
+            e->plan->upload();
+        }
+        if (graph) {
+            if (!lc.capture) HZ_HIP(hipStreamCreateWithFlags(&lc.capture, hipStreamNonBlocking));
+            HZ_HIP(hipStreamBeginCapture(lc.capture, hipStreamCaptureModeThreadLocal));
+            hipGraph_t g = nullptr;
+            try {
+                if (e->plan) e->plan->next = 0;
+                Fuser f(lc.capture, e->plan.get(), false);
+                body(f);
+                f.flush();
+            } catch (...) {
+                (void)hipStreamEndCapture(lc.capture, &g);
+                if (g) (void)hipGraphDestroy(g);
+                lc.entries.pop_back();
+                throw;
+            }
+            HZ_HIP(hipStreamEndCapture(lc.capture, &g));
+            hipGraphExec_t exec = nullptr;
+            const hipError_t err = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (err != hipSuccess) lc.entries.pop_back();
+            HZ_HIP(err);
+            e->exec = exec;
+        }
     }
-    op->graphs.push_back({std::move(sig), exec});
-    return exec;
+    if (e->exec) {
+        HZ_HIP(hipGraphLaunch(e->exec, s));
+        ++g_launches[0];
+        return;
+    }
+    e->plan->next = 0;
+    Fuser f(s, e->plan.get());
+    body(f);
+    f.flush();
 }
 
 // Per-RHS bookkeeping of one chunk.
@@ -557,41 +827,21 @@ struct ChunkState {
     std::vector<double> last_resid;
 };
 
-// Restarted GMRES on k slots of  (R A M) y = Bh  in precision T, y in X.
-//   ref_semantics: full `restart` steps per cycle, convergence tested only at
-//   the cycle top (the reference/CuPy loop); otherwise a slot leaves its cycle
-//   as soon as the Givens estimate meets its tolerance.
-// On return X holds y (M y in the multigrid form) for every slot that was
-// active on entry.
 // Preconditioner of the inner solve: Jacobi (operator R A M, the reference's
 // choice) or shifted-Laplacian multigrid P (operator R A P).
 enum Precond { PRE_JACOBI = 0, PRE_MULTIGRID = 1 };
 
-template <typename T>
-void gmres_cycles(hz_operator *op, int mode_ram, int k, int restart, int maxiter, bool ref_semantics,
-                  const std::vector<double> &atol, ChunkState &cs, hipStream_t s, int precond = PRE_JACOBI,
-                  bool use_graph = false, int block = 1) {
-    const bool use_mg = precond == PRE_MULTIGRID;
-    const bool adjoint = (mode_ram == MODE_RAMH);
-    const int mode_op = use_mg ? (adjoint ? MODE_RAH : MODE_RA) : mode_ram;
-    const int64_t n = op->n;
-    const int maxv = restart + 1;
-    const int64_t nblk = vec_blocks_cols(n, k);
-    const int64_t chunk = vec_chunk(n, nblk);
-    cx<T> *V = op->V.as<cx<T>>();
-    cx<T> *X = op->X.as<cx<T>>();
-    cx<T> *Bh = op->Bh.as<cx<T>>();
-    // Multigrid form: every preconditioned vector the Arnoldi steps compute is
-    // kept (slot i of Z), and X holds M y itself, updated from Z at the end
-    // of a cycle.  The cycle top and the caller then need no V-cycle of their
-    // own on X.
-    cx<float> *Z = nullptr;
-    if constexpr (std::is_same<T, float>::value) {
-        if (use_mg) {
-            op->Z.ensure((size_t)restart * k * n * sizeof(cx<float>));
-            Z = op->Z.as<cx<float>>();
-        }
-    }
+// One scene's chunk in a GMRES run: its operator (whose workspace holds the
+// chunk), its k columns' progress and their absolute tolerances.
+struct Member {
+    hz_operator *op = nullptr;
+    int k = 0;
+    ChunkState *cs = nullptr;
+    std::vector<double> atol;
+};
+
+// Device view of an operator's GMRES state.
+GmresState gmres_state(hz_operator *op) {
     GmresState st;
     st.H = op->Hm.as<cx<double>>();
     st.gc = op->gc.as<double>();
@@ -608,170 +858,379 @@ void gmres_cycles(hz_operator *op, int mode_ram, int k, int restart, int maxiter
     st.alpha = op->alpha.as<cx<double>>();
     st.fix = op->bfix.as<int>();
     st.guard = block_guard();
-    int *member = op->member.as<int>();
+    return st;
+}
 
-    hipLaunchKernelGGL((k_zero<T>), dim3(vec_blocks(n), k), dim3(256), 0, s, X, n, n);
-    HZ_LAUNCHED();
-    std::vector<double> early(k, -1.0), resid(k);
-    if (!ref_semantics)
-        for (int r = 0; r < k; ++r) early[r] = atol[r];
-    HZ_HIP(hipMemcpyAsync(st.tol, early.data(), k * sizeof(double), hipMemcpyHostToDevice, s));
+// Restarted GMRES on the k slots of every member, (R A M) y = Bh in
+// precision T with y in the member's X.  With several members (scenes) the
+// cycles run in lockstep: every launch position fuses across the scenes that
+// are still iterating, and each cycle body is one graph launch for all of
+// them.
+//   ref_semantics: full `restart` steps per cycle, convergence tested only at
+//   the cycle top (the reference/CuPy loop); otherwise a slot leaves its cycle
+//   as soon as the Givens estimate meets its tolerance.
+// On return X holds y (M y in the multigrid form) for every slot that was
+// active on entry.
+template <typename T>
+void gmres_cycles(std::vector<Member> &ms, int mode_ram, int restart, int maxiter, bool ref_semantics,
+                  hipStream_t s, int precond = PRE_JACOBI, bool use_graph = false, int block = 1,
+                  LaunchCache *batch_cache = nullptr) {
+    const int S = (int)ms.size();
+    const bool fused = S > 1;
+    const bool use_mg = precond == PRE_MULTIGRID;
+    const bool adjoint = (mode_ram == MODE_RAMH);
+    const int mode_op = use_mg ? (adjoint ? MODE_RAH : MODE_RA) : mode_ram;
+    const int maxv = restart + 1;
+    const bool two = block == 2 && std::is_same<T, float>::value;
+    struct Run {
+        hz_operator *op;
+        int k;
+        int64_t n, nblk, chunk;
+        cx<T> *V, *X, *Bh;
+        cx<float> *Z;
+        GmresState st;
+        int *member;
+        double2 *dpart;  // dot-product partials
+        double *npart;   // norm partials
+        double *gram;    // two-step block Gram partials
+        std::vector<double> early, resid, prev_top;
+        std::vector<int> live_h, early_exit, steps_h;
+        bool on;  // has slots in the current cycle
+    };
+    std::vector<Run> rs(S);
+    for (int i = 0; i < S; ++i) {
+        Run &r = rs[i];
+        hz_operator *op = ms[i].op;
+        r.op = op;
+        r.k = ms[i].k;
+        r.n = op->n;
+        r.nblk = vec_blocks_cols(r.n, r.k);
+        r.chunk = vec_chunk(r.n, r.nblk);
+        r.V = op->V.as<cx<T>>();
+        r.X = op->X.as<cx<T>>();
+        r.Bh = op->Bh.as<cx<T>>();
+        // Multigrid form: every preconditioned vector the Arnoldi steps
+        // compute is kept (slot i of Z), and X holds M y itself, updated from
+        // Z at the end of a cycle.  The cycle top and the caller then need no
+        // V-cycle of their own on X.
+        r.Z = nullptr;
+        if constexpr (std::is_same<T, float>::value) {
+            if (use_mg) {
+                op->Z.ensure((size_t)restart * r.k * r.n * sizeof(cx<float>));
+                r.Z = op->Z.as<cx<float>>();
+            }
+        }
+        r.st = gmres_state(op);
+        r.member = op->member.as<int>();
+        r.dpart = op->dpart.as<double2>();
+        r.npart = op->npart.as<double>();
+        r.gram = op->gram.as<double>();
+        r.early.assign(r.k, -1.0);
+        if (!ref_semantics)
+            for (int c = 0; c < r.k; ++c) r.early[c] = ms[i].atol[c];
+        r.resid.assign(r.k, 0.0);
+        r.prev_top.assign(r.k, 0.0);
+        r.live_h.assign(r.k, 0);
+        r.early_exit.assign(r.k, 0);
+        r.steps_h.assign(r.k, 0);
+        r.on = true;
+    }
+    LaunchCache local;
+    local.limit = 64;
+    LaunchCache &lc = fused ? (batch_cache ? *batch_cache : local) : ms[0].op->graphs;
 
-    std::vector<int> live_h(k), early_exit(k, 0);
-    std::vector<double> prev_top(k, 0.0);
+    {
+        Fuser f(s);
+        for (int i = 0; i < S; ++i)
+            f.add<KZero<T>>(i, dim3(vec_blocks(rs[i].n), rs[i].k), {rs[i].X, rs[i].n, rs[i].n});
+    }
+    for (Run &r : rs)
+        HZ_HIP(hipMemcpyAsync(r.st.tol, r.early.data(), r.k * sizeof(double), hipMemcpyHostToDevice, s));
+
+    // Scenes in the current cycle, as a signature prefix (tag: 0 cycle top,
+    // 1 cycle body).
+    auto members_sig = [&](int tag) {
+        std::vector<uintptr_t> sig = {(uintptr_t)tag, (uintptr_t)S};
+        for (int i = 0; i < S; ++i)
+            if (rs[i].on) sig.push_back((uintptr_t)i);
+        return sig;
+    };
     while (true) {
-        for (int r = 0; r < k; ++r) live_h[r] = cs.active[r];
-        HZ_HIP(hipMemcpyAsync(st.live, live_h.data(), k * sizeof(int), hipMemcpyHostToDevice, s));
+        for (int i = 0; i < S; ++i) {
+            Run &r = rs[i];
+            if (!r.on) continue;
+            for (int c = 0; c < r.k; ++c) r.live_h[c] = ms[i].cs->active[c];
+            HZ_HIP(hipMemcpyAsync(r.st.live, r.live_h.data(), r.k * sizeof(int), hipMemcpyHostToDevice, s));
+        }
         // Cycle top: V_0 = Bh - R A M y (true residual of the preconditioned
         // system); X is y, or M y itself in the multigrid form.
-        residual_mode<T>(op, mode_op, X, Bh, V, k, st.live, op->npart.as<double>(), s);
-        hipLaunchKernelGGL(k_cycle_start, dim3(k), dim3(256), 0, s, st, op->npart.as<double>(),
-                           (int)op->tm.nblocks, maxv, restart);
-        HZ_LAUNCHED();
-        HZ_HIP(hipMemcpyAsync(resid.data(), st.resid, k * sizeof(double), hipMemcpyDeviceToHost, s));
-        HZ_HIP(hipStreamSynchronize(s));
-        bool any = false;
-        for (int r = 0; r < k; ++r) {
-            if (!cs.active[r]) continue;
-            if (verbose())
-                std::fprintf(stderr, "[hz] %s cycle %d rhs %d: residual %.6e (target %.6e)\n",
-                             sizeof(T) == 8 ? "c128" : "c64", cs.cycles[r], r, resid[r], atol[r]);
-            cs.last_resid[r] = resid[r];
-            if (!(resid[r] > atol[r]) || cs.iterations[r] >= maxiter || !std::isfinite(resid[r])) {
-                cs.active[r] = 0;
-                continue;
-            }
-            // Working-precision floor: the last cycle stopped early on its
-            // Givens estimate, yet the true residual did not even halve.  More
-            // cycles cannot help; the caller's FP64 refinement takes over.
-            if (early_exit[r] && resid[r] > 0.5 * prev_top[r]) {
-                cs.active[r] = 0;
-                continue;
-            }
-            prev_top[r] = resid[r];
-            any = true;
+        {
+            std::vector<uintptr_t> sig = members_sig(0);
+            for (int i = 0; i < S; ++i)
+                if (rs[i].on) {
+                    const auto gs = graph_signature(rs[i].op, sizeof(T), mode_op, rs[i].k, restart, false);
+                    sig.insert(sig.end(), gs.begin(), gs.end());
+                }
+            run_sequence(lc, std::move(sig), fused, false, s, [&](Fuser &f) {
+                for (int i = 0; i < S; ++i) {
+                    Run &r = rs[i];
+                    if (r.on)
+                        residual_mode<T>(f, i, r.op, mode_op, r.X, r.Bh, r.V, r.k, r.st.live,
+                                         r.npart);
+                }
+                f.flush();
+                for (int i = 0; i < S; ++i) {
+                    Run &r = rs[i];
+                    if (r.on)
+                        f.add<KCycleStart>(i, dim3(r.k), {r.st, r.npart, (int)r.op->tm.nblocks, maxv,
+                                                          restart});
+                }
+                f.flush();
+            });
         }
-        if (!any) break;
-        for (int r = 0; r < k; ++r) live_h[r] = cs.active[r];
-        HZ_HIP(hipMemcpyAsync(st.live, live_h.data(), k * sizeof(int), hipMemcpyHostToDevice, s));
-        HZ_HIP(hipMemcpyAsync(member, live_h.data(), k * sizeof(int), hipMemcpyHostToDevice, s));
+        for (Run &r : rs)
+            if (r.on)
+                HZ_HIP(hipMemcpyAsync(r.resid.data(), r.st.resid, r.k * sizeof(double), hipMemcpyDeviceToHost, s));
+        HZ_HIP(hipStreamSynchronize(s));
+        bool any_scene = false;
+        for (int i = 0; i < S; ++i) {
+            Run &r = rs[i];
+            if (!r.on) continue;
+            ChunkState &cs = *ms[i].cs;
+            const std::vector<double> &atol = ms[i].atol;
+            bool any = false;
+            for (int c = 0; c < r.k; ++c) {
+                if (!cs.active[c]) continue;
+                if (verbose())
+                    std::fprintf(stderr, "[hz] %s cycle %d scene %d rhs %d: residual %.6e (target %.6e)\n",
+                                 sizeof(T) == 8 ? "c128" : "c64", cs.cycles[c], i, c, r.resid[c], atol[c]);
+                cs.last_resid[c] = r.resid[c];
+                if (!(r.resid[c] > atol[c]) || cs.iterations[c] >= maxiter || !std::isfinite(r.resid[c])) {
+                    cs.active[c] = 0;
+                    continue;
+                }
+                // Working-precision floor: the last cycle stopped early on its
+                // Givens estimate, yet the true residual did not even halve.
+                // More cycles cannot help; the caller's FP64 refinement takes
+                // over.
+                if (r.early_exit[c] && r.resid[c] > 0.5 * r.prev_top[c]) {
+                    cs.active[c] = 0;
+                    continue;
+                }
+                r.prev_top[c] = r.resid[c];
+                any = true;
+            }
+            r.on = any;
+            any_scene = any_scene || any;
+            if (!any) continue;
+            for (int c = 0; c < r.k; ++c) r.live_h[c] = cs.active[c];
+            HZ_HIP(hipMemcpyAsync(r.st.live, r.live_h.data(), r.k * sizeof(int), hipMemcpyHostToDevice, s));
+            HZ_HIP(hipMemcpyAsync(r.member, r.live_h.data(), r.k * sizeof(int), hipMemcpyHostToDevice, s));
+        }
+        if (!any_scene) break;
         // The cycle body: `restart` Arnoldi steps, the triangular solve and
         // the iterate update.  Slots that leave early are skipped inside the
         // kernels (device live flags), so the launch sequence is the same
         // every cycle and can be replayed from a captured graph.
         // Two-step blocks (complex64 only): w1 = A M q_j, w2 = A M w1, one dot
         // pass and one update pass over V_0..V_j for both, an intra-block axpy.
-        const bool two = block == 2 && std::is_same<T, float>::value;
-        auto block_step = [&](int j, hipStream_t cs_stream) {
+        // The V-cycles of basis slot j: Z_j = M V_j.
+        auto mg_tasks = [&](int j) {
+            std::vector<MgTask> t;
             if constexpr (std::is_same<T, float>::value) {
-                cx<float> *w1 = V + (int64_t)(j + 1) * k * n;
-                cx<float> *w2 = V + (int64_t)(j + 2) * k * n;
-                const cx<float> *in = V + (int64_t)j * k * n;
-                if (use_mg) {
-                    cx<float> *z = Z + (int64_t)j * k * n;  // M V_j
-                    precondition(op, adjoint, in, z, k, st.live, cs_stream);
-                    in = z;
+                for (int i = 0; i < S; ++i) {
+                    Run &r = rs[i];
+                    if (!r.on) continue;
+                    t.push_back({i, r.op->mg.get(), r.V + (int64_t)j * r.k * r.n, r.Z + (int64_t)j * r.k * r.n, r.k,
+                                 r.st.live, adjoint ? r.op->v32.rs_a : r.op->v32.rs_f});
                 }
-                apply_mode<float>(op, mode_op, in, n, w1, n, k, st.live, st.scales + (int64_t)j * k, cs_stream);
-                in = w1;
+            }
+            return t;
+        };
+        auto block_step = [&](Fuser &f, int j) {
+            if constexpr (std::is_same<T, float>::value) {
+                // w1 = A M V_j into slot j+1.
                 if (use_mg) {
-                    cx<float> *z = Z + (int64_t)(j + 1) * k * n;  // M w1, before w1 is orthogonalised
-                    precondition(op, adjoint, w1, z, k, st.live, cs_stream);
-                    in = z;
+                    auto t = mg_tasks(j);
+                    precondition(f, t, adjoint);
                 }
-                apply_mode<float>(op, mode_op, in, n, w2, n, k, st.live, nullptr, cs_stream);
+                for (int i = 0; i < S; ++i) {
+                    Run &r = rs[i];
+                    if (!r.on) continue;
+                    const cx<float> *in = use_mg ? r.Z + (int64_t)j * r.k * r.n : r.V + (int64_t)j * r.k * r.n;
+                    apply_mode<float>(f, i, r.op, mode_op, in, r.n, r.V + (int64_t)(j + 1) * r.k * r.n, r.n, r.k,
+                                      r.st.live, r.st.scales + (int64_t)j * r.k);
+                }
+                f.flush();
+                // w2 = A M w1 into slot j+2 (M w1 kept in Z_{j+1}, before w1
+                // is orthogonalised).
+                if (use_mg) {
+                    auto t = mg_tasks(j + 1);
+                    precondition(f, t, adjoint);
+                }
+                for (int i = 0; i < S; ++i) {
+                    Run &r = rs[i];
+                    if (!r.on) continue;
+                    const cx<float> *in =
+                        use_mg ? r.Z + (int64_t)(j + 1) * r.k * r.n : r.V + (int64_t)(j + 1) * r.k * r.n;
+                    apply_mode<float>(f, i, r.op, mode_op, in, r.n, r.V + (int64_t)(j + 2) * r.k * r.n, r.n, r.k,
+                                      r.st.live, nullptr);
+                }
+                f.flush();
                 const int groups = (j + 1 + DOT_GROUP / 2 - 1) / (DOT_GROUP / 2);
-                const bool pr = paired<float>(n);
-                double *gram = op->gram.as<double>();
-                if (pr)
-                    hipLaunchKernelGGL((k_dots2<float, 2>), dim3(nblk, k, groups), dim3(256), 0, cs_stream,
-                                       (const cx<float> *)V, k, n, j + 1, (const cx<float> *)w1,
-                                       (const cx<float> *)w2, st.live, op->dpart.as<double2>(), maxv, chunk, gram);
-                else
-                    hipLaunchKernelGGL((k_dots2<float, 1>), dim3(nblk, k, groups), dim3(256), 0, cs_stream,
-                                       (const cx<float> *)V, k, n, j + 1, (const cx<float> *)w1,
-                                       (const cx<float> *)w2, st.live, op->dpart.as<double2>(), maxv, chunk, gram);
-                HZ_LAUNCHED();
-                hipLaunchKernelGGL(k_reduce_dots2, dim3(j + 1, k), dim3(256), 0, cs_stream, op->dpart.as<double2>(),
-                                   (int)nblk, maxv, j, st, restart);
-                HZ_LAUNCHED();
-                hipLaunchKernelGGL(k_block_alpha, dim3(k), dim3(256), 0, cs_stream, st, (const double *)gram,
-                                   (int)nblk, j, maxv, restart);
-                HZ_LAUNCHED();
-                if (pr)
-                    hipLaunchKernelGGL((k_update2<float, 2>), dim3(nblk, k), dim3(256), 0, cs_stream,
-                                       (const cx<float> *)V, k, n, j + 1, w1, w2, (const cx<double> *)st.coef,
-                                       (const cx<double> *)st.coef2, maxv, st.live, gram, chunk,
-                                       (const cx<double> *)st.alpha);
-                else
-                    hipLaunchKernelGGL((k_update2<float, 1>), dim3(nblk, k), dim3(256), 0, cs_stream,
-                                       (const cx<float> *)V, k, n, j + 1, w1, w2, (const cx<double> *)st.coef,
-                                       (const cx<double> *)st.coef2, maxv, st.live, gram, chunk,
-                                       (const cx<double> *)st.alpha);
-                HZ_LAUNCHED();
-                hipLaunchKernelGGL(k_block_fixcoef, dim3(k), dim3(256), 0, cs_stream, st, (const double *)gram,
-                                   (int)nblk);
-                HZ_LAUNCHED();
-                hipLaunchKernelGGL((k_block_fixup<float>), dim3(nblk, k), dim3(256), 0, cs_stream, st, n,
-                                   (const cx<float> *)w1, w2, gram, chunk);
-                HZ_LAUNCHED();
-                hipLaunchKernelGGL(k_block_finish, dim3(k), dim3(256), 0, cs_stream, st, (const double *)gram,
-                                   (int)nblk, j, restart);
-                HZ_LAUNCHED();
+                for (int i = 0; i < S; ++i) {
+                    Run &r = rs[i];
+                    if (!r.on) continue;
+                    const cx<float> *w1 = r.V + (int64_t)(j + 1) * r.k * r.n;
+                    const cx<float> *w2 = r.V + (int64_t)(j + 2) * r.k * r.n;
+                    const dim3 grid(r.nblk, r.k, groups);
+                    typename KDots2<float, 2>::Args a{r.V, r.k, r.n, j + 1, w1, w2, r.st.live,
+                                                      r.dpart, maxv, r.chunk,
+                                                      r.gram};
+                    if (paired<float>(r.n))
+                        f.add<KDots2<float, 2>>(i, grid, a);
+                    else
+                        f.add<KDots2<float, 1>>(i, grid, {a.V, a.k, a.n, a.jp1, a.w1, a.w2, a.live, a.partial, a.maxv,
+                                                          a.chunk, a.wgram});
+                }
+                f.flush();
+                for (int i = 0; i < S; ++i)
+                    if (rs[i].on)
+                        f.add<KReduceDots2>(i, dim3(j + 1, rs[i].k), {rs[i].dpart,
+                                                                      (int)rs[i].nblk, maxv, j, rs[i].st, restart});
+                f.flush();
+                for (int i = 0; i < S; ++i)
+                    if (rs[i].on)
+                        f.add<KBlockAlpha>(i, dim3(rs[i].k), {rs[i].st, rs[i].gram, (int)rs[i].nblk,
+                                                              j, maxv, restart});
+                f.flush();
+                for (int i = 0; i < S; ++i) {
+                    Run &r = rs[i];
+                    if (!r.on) continue;
+                    cx<float> *w1 = r.V + (int64_t)(j + 1) * r.k * r.n;
+                    cx<float> *w2 = r.V + (int64_t)(j + 2) * r.k * r.n;
+                    const dim3 grid(r.nblk, r.k);
+                    typename KUpdate2<float, 2>::Args a{r.V, r.k, r.n, j + 1, w1, w2, r.st.coef, r.st.coef2, maxv,
+                                                        r.st.live, r.gram, r.chunk, r.st.alpha};
+                    if (paired<float>(r.n))
+                        f.add<KUpdate2<float, 2>>(i, grid, a);
+                    else
+                        f.add<KUpdate2<float, 1>>(i, grid, {a.V, a.k, a.n, a.cnt, a.w1, a.w2, a.coef, a.coef2, a.maxv,
+                                                            a.live, a.gram, a.chunk, a.alpha});
+                }
+                f.flush();
+                for (int i = 0; i < S; ++i)
+                    if (rs[i].on)
+                        f.add<KBlockFixcoef>(i, dim3(rs[i].k), {rs[i].st, rs[i].gram,
+                                                                (int)rs[i].nblk});
+                f.flush();
+                for (int i = 0; i < S; ++i) {
+                    Run &r = rs[i];
+                    if (r.on)
+                        f.add<KBlockFixup<float>>(i, dim3(r.nblk, r.k), {r.st, r.n, r.V + (int64_t)(j + 1) * r.k * r.n,
+                                                                         r.V + (int64_t)(j + 2) * r.k * r.n,
+                                                                         r.gram, r.chunk});
+                }
+                f.flush();
+                for (int i = 0; i < S; ++i)
+                    if (rs[i].on)
+                        f.add<KBlockFinish>(i, dim3(rs[i].k), {rs[i].st, rs[i].gram,
+                                                               (int)rs[i].nblk, j, restart});
+                f.flush();
             }
         };
-        auto body = [&](hipStream_t cs_stream) {
+        auto body = [&](Fuser &f) {
             int j0 = 0;
             if (two)
-                for (; j0 + 1 < restart; j0 += 2) block_step(j0, cs_stream);
+                for (; j0 + 1 < restart; j0 += 2) block_step(f, j0);
             for (int j = j0; j < restart; ++j) {
-                const cx<T> *vj = V + (int64_t)j * k * n;
-                cx<T> *w = V + (int64_t)(j + 1) * k * n;
-                const cx<T> *in = vj;
                 if constexpr (std::is_same<T, float>::value) {
                     if (use_mg) {
-                        cx<float> *z = Z + (int64_t)j * k * n;  // M V_j
-                        precondition(op, adjoint, vj, z, k, st.live, cs_stream);
-                        in = z;
+                        auto t = mg_tasks(j);
+                        precondition(f, t, adjoint);
                     }
                 }
-                apply_mode<T>(op, mode_op, in, n, w, n, k, st.live, st.scales + (int64_t)j * k, cs_stream);
+                for (int i = 0; i < S; ++i) {
+                    Run &r = rs[i];
+                    if (!r.on) continue;
+                    const cx<T> *in = r.V + (int64_t)j * r.k * r.n;
+                    if constexpr (std::is_same<T, float>::value)
+                        if (use_mg) in = r.Z + (int64_t)j * r.k * r.n;
+                    apply_mode<T>(f, i, r.op, mode_op, in, r.n, r.V + (int64_t)(j + 1) * r.k * r.n, r.n, r.k,
+                                  r.st.live, r.st.scales + (int64_t)j * r.k);
+                }
+                f.flush();
                 const int groups = (j + 1 + DOT_GROUP - 1) / DOT_GROUP;
-                launch_dots<T>(dim3(nblk, k, groups), cs_stream, V, k, n, j + 1, w, st.live, op->dpart.as<double2>(),
-                               maxv, chunk);
-                hipLaunchKernelGGL(k_reduce_dots, dim3(j + 1, k), dim3(256), 0, cs_stream, op->dpart.as<double2>(),
-                                   (int)nblk, maxv, j, st.scales, st.live, st.H, st.coef, restart);
-                HZ_LAUNCHED();
-                launch_update<T>(dim3(nblk, k), cs_stream, V, k, n, j + 1, nullptr, w, n, st.coef, maxv, st.live,
-                                 op->npart.as<double>(), chunk);
-                hipLaunchKernelGGL(k_arnoldi_finish, dim3(k), dim3(256), 0, cs_stream, st, op->npart.as<double>(),
-                                   (int)nblk, j, maxv, restart);
-                HZ_LAUNCHED();
+                for (int i = 0; i < S; ++i) {
+                    Run &r = rs[i];
+                    if (r.on)
+                        launch_dots<T>(f, i, dim3(r.nblk, r.k, groups), r.V, r.k, r.n, j + 1,
+                                       r.V + (int64_t)(j + 1) * r.k * r.n, r.st.live, r.dpart, maxv,
+                                       r.chunk);
+                }
+                f.flush();
+                for (int i = 0; i < S; ++i) {
+                    Run &r = rs[i];
+                    if (r.on)
+                        f.add<KReduceDots>(i, dim3(j + 1, r.k), {r.dpart, (int)r.nblk, maxv, j,
+                                                                 r.st.scales, r.st.live, r.st.H, r.st.coef, restart});
+                }
+                f.flush();
+                for (int i = 0; i < S; ++i) {
+                    Run &r = rs[i];
+                    if (r.on)
+                        launch_update<T>(f, i, dim3(r.nblk, r.k), r.V, r.k, r.n, j + 1, nullptr,
+                                         r.V + (int64_t)(j + 1) * r.k * r.n, r.n, r.st.coef, maxv, r.st.live,
+                                         r.npart, r.chunk);
+                }
+                f.flush();
+                for (int i = 0; i < S; ++i) {
+                    Run &r = rs[i];
+                    if (r.on)
+                        f.add<KArnoldiFinish>(i, dim3(r.k), {r.st, r.npart, (int)r.nblk, j, maxv,
+                                                             restart});
+                }
+                f.flush();
             }
-            hipLaunchKernelGGL(k_trisolve, dim3(k), dim3(64), 0, cs_stream, st, maxv, restart, member,
-                               (int)(use_mg && two));
-            HZ_LAUNCHED();
-            const cx<T> *basis = V;
-            if constexpr (std::is_same<T, float>::value)
-                if (use_mg) basis = Z;
-            launch_update<T>(dim3(nblk, k), cs_stream, basis, k, n, 0, st.steps, X, n, st.coef, maxv, member,
-                             nullptr, chunk);
+            for (int i = 0; i < S; ++i)
+                if (rs[i].on)
+                    f.add<KTrisolve>(i, dim3(rs[i].k), {rs[i].st, maxv, restart, rs[i].member, (int)(use_mg && two)});
+            f.flush();
+            for (int i = 0; i < S; ++i) {
+                Run &r = rs[i];
+                if (!r.on) continue;
+                const cx<T> *basis = r.V;
+                if constexpr (std::is_same<T, float>::value)
+                    if (use_mg) basis = r.Z;
+                launch_update<T>(f, i, dim3(r.nblk, r.k), basis, r.k, r.n, 0, r.st.steps, r.X, r.n, r.st.coef, maxv,
+                                 r.member, nullptr, r.chunk);
+            }
+            f.flush();
         };
-        if (use_graph) {
-            hipGraphExec_t exec =
-                cycle_graph(op, graph_signature(op, sizeof(T), mode_op + 16 * (two ? 1 : 0), k, restart, use_mg), body);
-            HZ_HIP(hipGraphLaunch(exec, s));
-        } else {
-            body(s);
+        {
+            std::vector<uintptr_t> sig = members_sig(1);
+            for (int i = 0; i < S; ++i)
+                if (rs[i].on) {
+                    const auto gs = graph_signature(rs[i].op, sizeof(T), mode_op + 16 * (two ? 1 : 0), rs[i].k,
+                                                    restart, use_mg);
+                    sig.insert(sig.end(), gs.begin(), gs.end());
+                }
+            run_sequence(lc, std::move(sig), fused, use_graph, s, body);
         }
-        std::vector<int> steps_h(k);
-        HZ_HIP(hipMemcpyAsync(steps_h.data(), st.steps, k * sizeof(int), hipMemcpyDeviceToHost, s));
+        for (Run &r : rs)
+            if (r.on)
+                HZ_HIP(hipMemcpyAsync(r.steps_h.data(), r.st.steps, r.k * sizeof(int), hipMemcpyDeviceToHost, s));
         HZ_HIP(hipStreamSynchronize(s));
-        for (int r = 0; r < k; ++r) {
-            if (!cs.active[r]) continue;
-            cs.iterations[r] += ref_semantics ? restart : steps_h[r];
-            cs.cycles[r] += 1;
-            early_exit[r] = !ref_semantics && steps_h[r] < restart;
+        for (int i = 0; i < S; ++i) {
+            Run &r = rs[i];
+            if (!r.on) continue;
+            ChunkState &cs = *ms[i].cs;
+            for (int c = 0; c < r.k; ++c) {
+                if (!cs.active[c]) continue;
+                cs.iterations[c] += ref_semantics ? restart : r.steps_h[c];
+                cs.cycles[c] += 1;
+                r.early_exit[c] = !ref_semantics && r.steps_h[c] < restart;
+            }
         }
     }
 }
@@ -810,27 +1269,37 @@ struct CheckResult {
 };
 
 // FP64 acceptance check of x (physical) against b for k columns; optionally
-// writes the scaled refinement residual into op->rhs32.
-std::vector<CheckResult> check(hz_operator *op, int adjoint, const cx<double> *x, int64_t ldx,
-                               const cx<double> *b, int64_t ldb, int k, cx<float> *rhs_out, hipStream_t s) {
+// writes the scaled refinement residual into op->rhs32.  Launches only: the
+// sums land in op->red (read by check_read after a stream synchronisation).
+void check_launch(Fuser &f, int sc, hz_operator *op, int adjoint, const cx<double> *x, int64_t ldx,
+                  const cx<double> *b, int64_t ldb, int k, cx<float> *rhs_out) {
     const dim3 grid(op->tm.nblocks, k);
     double *part = op->cpart.as<double>();
     if (adjoint)
-        hipLaunchKernelGGL(k_check<MODE_AH>, grid, dim3(256), 0, s, op->v64, op->tm, x, ldx, b, ldb,
-                           (const int *)nullptr, part, rhs_out, op->n);
+        f.add<KCheck<MODE_AH>>(sc, grid, {op->v64, op->tm, x, ldx, b, ldb, nullptr, part, rhs_out, op->n});
     else
-        hipLaunchKernelGGL(k_check<MODE_A>, grid, dim3(256), 0, s, op->v64, op->tm, x, ldx, b, ldb,
-                           (const int *)nullptr, part, rhs_out, op->n);
-    HZ_LAUNCHED();
-    hipLaunchKernelGGL(k_reduce_rows, dim3(6, k), dim3(256), 0, s, part, (int)op->tm.nblocks, 6, 1 << 4,
-                       op->red.as<double>());
-    HZ_LAUNCHED();
-    std::vector<double> h((size_t)k * 6);
-    HZ_HIP(hipMemcpyAsync(h.data(), op->red.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-    HZ_HIP(hipStreamSynchronize(s));
+        f.add<KCheck<MODE_A>>(sc, grid, {op->v64, op->tm, x, ldx, b, ldb, nullptr, part, rhs_out, op->n});
+}
+
+void check_reduce(Fuser &f, int sc, hz_operator *op, int k) {
+    f.add<KReduceRows>(sc, dim3(6, k), {op->cpart.as<double>(), (int)op->tm.nblocks, 6, 1 << 4, op->red.as<double>()});
+}
+
+std::vector<CheckResult> check_read(const std::vector<double> &h, int k) {
     std::vector<CheckResult> out(k);
     for (int r = 0; r < k; ++r) out[r] = {h[r * 6], h[r * 6 + 1], h[r * 6 + 2], h[r * 6 + 3], h[r * 6 + 4], h[r * 6 + 5]};
     return out;
+}
+
+std::vector<CheckResult> check(hz_operator *op, int adjoint, const cx<double> *x, int64_t ldx,
+                               const cx<double> *b, int64_t ldb, int k, cx<float> *rhs_out, hipStream_t s) {
+    Fuser f(s);
+    check_launch(f, 0, op, adjoint, x, ldx, b, ldb, k, rhs_out);
+    check_reduce(f, 0, op, k);
+    std::vector<double> h((size_t)k * 6);
+    HZ_HIP(hipMemcpyAsync(h.data(), op->red.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HZ_HIP(hipStreamSynchronize(s));
+    return check_read(h, k);
 }
 
 void fill_stats(hz_gmres_stats &st, const CheckResult &c, const hz_gmres_opts &o) {
@@ -858,11 +1327,11 @@ void solve_plain(hz_operator *op, const hz_gmres_opts &o, const cx<double> *b, i
     const int64_t nblk = vec_blocks_cols(n, k);
     const int64_t chunk = vec_chunk(n, nblk);
     double *part = op->npart.as<double>();
-    hipLaunchKernelGGL((k_scale_rhs<double>), dim3(nblk, k), dim3(256), 0, s, b, ldb, rs, op->Bh.as<cx<double>>(),
-                       n, n, part, chunk);
-    HZ_LAUNCHED();
-    hipLaunchKernelGGL(k_reduce_rows, dim3(2, k), dim3(256), 0, s, part, (int)nblk, 2, 0, op->red.as<double>());
-    HZ_LAUNCHED();
+    {
+        Fuser f(s);
+        f.add<KScaleRhs<double>>(0, dim3(nblk, k), {b, ldb, rs, op->Bh.as<cx<double>>(), n, n, part, chunk});
+        f.add<KReduceRows>(0, dim3(2, k), {part, (int)nblk, 2, 0, op->red.as<double>()});
+    }
     std::vector<double> norms((size_t)2 * k);
     HZ_HIP(hipMemcpyAsync(norms.data(), op->red.p, norms.size() * sizeof(double), hipMemcpyDeviceToHost, s));
     HZ_HIP(hipStreamSynchronize(s));
@@ -882,12 +1351,19 @@ void solve_plain(hz_operator *op, const hz_gmres_opts &o, const cx<double> *b, i
             cs.active[r] = 0;
         }
     }
-    gmres_cycles<double>(op, o.adjoint ? MODE_RAMH : MODE_RAM, k, o.restart, o.maxiter, true, atol, cs, s,
-                         PRE_JACOBI, o.use_graph != 0);
+    std::vector<Member> ms(1);
+    ms[0].op = op;
+    ms[0].k = k;
+    ms[0].cs = &cs;
+    ms[0].atol = atol;
+    gmres_cycles<double>(ms, o.adjoint ? MODE_RAMH : MODE_RAM, o.restart, o.maxiter, true, s, PRE_JACOBI,
+                         o.use_graph != 0);
     // x = M y for every column (zero right-hand sides give y = 0).
-    hipLaunchKernelGGL((k_precond_out<double>), dim3(vec_blocks(n), k), dim3(256), 0, s,
-                       (const cx<double> *)op->X.as<cx<double>>(), n, m, x, ldx, n, (const int *)nullptr, 0);
-    HZ_LAUNCHED();
+    {
+        Fuser f(s);
+        f.add<KPrecondOut<double>>(0, dim3(vec_blocks(n), k), {(const cx<double> *)op->X.as<cx<double>>(), n, m, x,
+                                                               ldx, n, nullptr, 0});
+    }
     const auto chk = check(op, o.adjoint, x, ldx, b, ldb, k, nullptr, s);
     for (int r = 0; r < k; ++r) {
         hz_gmres_stats &st = stats[r];
@@ -906,130 +1382,182 @@ void solve_plain(hz_operator *op, const hz_gmres_opts &o, const cx<double> *b, i
     }
 }
 
-// Mixed form: complex64 GMRES inner solves under FP64 iterative refinement.
-void solve_mixed(hz_operator *op, const hz_gmres_opts &o, const cx<double> *b, int64_t ldb, cx<double> *x,
-                 int64_t ldx, int k, hz_gmres_stats *stats, hipStream_t s) {
-    const int64_t n = op->n;
-    reserve_workspace(op, k, o.restart, sizeof(cx<float>));
-    op->rhs32.ensure((size_t)k * n * sizeof(cx<float>));
-    const cx<double> *m = o.adjoint ? op->ma64.as<cx<double>>() : op->mf64.as<cx<double>>();
-    hipLaunchKernelGGL((k_zero<double>), dim3(vec_blocks(n), k), dim3(256), 0, s, x, ldx, n);
-    HZ_LAUNCHED();
-    {
-        // Per-RHS count of two-step blocks that took the measured coefficient.
-        const std::vector<int> zeros((size_t)2 * k, 0);
-        HZ_HIP(hipMemcpyAsync(op->bfix.p, zeros.data(), zeros.size() * sizeof(int), hipMemcpyHostToDevice, s));
-    }
+// One scene's chunk of a mixed-form solve: FP64 right-hand sides and
+// solution (caller's device buffers), its stats and its refinement progress.
+struct MixedJob {
+    hz_operator *op = nullptr;
+    const cx<double> *b = nullptr;
+    cx<double> *x = nullptr;
+    int64_t ldb = 0, ldx = 0;
+    int k = 0;
+    hz_gmres_stats *stats = nullptr;
     ChunkState cs;
-    cs.k = k;
-    cs.active.assign(k, 1);
-    cs.iterations.assign(k, 0);
-    cs.cycles.assign(k, 0);
-    cs.last_resid.assign(k, 0.0);
-    std::vector<int> done(k, 0), refinements(k, 0), zero_rhs(k, 0), member(k, 0);
+    std::vector<int> done, refinements, zero_rhs, member;
     std::vector<CheckResult> last;
+};
+
+// Mixed form: complex64 GMRES inner solves under FP64 iterative refinement,
+// for one chunk of each job's scene.  Several jobs refine in lockstep: each
+// round checks every scene, then runs one lockstep inner solve over the
+// columns that still need one (gmres_cycles with several members).
+void solve_mixed(std::vector<MixedJob> &jobs, const hz_gmres_opts &o, hipStream_t s, LaunchCache *batch_cache) {
+    const int S = (int)jobs.size();
+    const bool fused = S > 1;
+    const bool use_mg = o.preconditioner == PRE_MULTIGRID;
+    {
+        Fuser f(s);
+        for (int i = 0; i < S; ++i) {
+            MixedJob &jb = jobs[i];
+            hz_operator *op = jb.op;
+            reserve_workspace(op, jb.k, o.restart, sizeof(cx<float>));
+            op->rhs32.ensure((size_t)jb.k * op->n * sizeof(cx<float>));
+            if (use_mg) mg_reserve(*op->mg, jb.k);
+            f.add<KZero<double>>(i, dim3(vec_blocks(op->n), jb.k), {jb.x, jb.ldx, op->n});
+            // Per-RHS count of two-step blocks that took the measured coefficient.
+            const std::vector<int> zeros((size_t)2 * jb.k, 0);
+            HZ_HIP(hipMemcpyAsync(op->bfix.p, zeros.data(), zeros.size() * sizeof(int), hipMemcpyHostToDevice, s));
+            HZ_HIP(hipStreamSynchronize(s));
+            jb.cs.k = jb.k;
+            jb.cs.active.assign(jb.k, 1);
+            jb.cs.iterations.assign(jb.k, 0);
+            jb.cs.cycles.assign(jb.k, 0);
+            jb.cs.last_resid.assign(jb.k, 0.0);
+            jb.done.assign(jb.k, 0);
+            jb.refinements.assign(jb.k, 0);
+            jb.zero_rhs.assign(jb.k, 0);
+            jb.member.assign(jb.k, 0);
+        }
+    }
+    std::vector<std::vector<double>> sums(S);
     for (int ref = 0;; ++ref) {
-        last = check(op, o.adjoint, x, ldx, b, ldb, k, op->rhs32.as<cx<float>>(), s);
-        std::vector<double> atol(k, 0.0);
-        bool any = false;
-        for (int r = 0; r < k; ++r) {
-            if (done[r]) continue;
-            const CheckResult &c = last[r];
-            if (c.ub == 0.0) {
-                zero_rhs[r] = 1;
-                done[r] = 1;
+        // FP64 check of every scene (refinement residual into rhs32).
+        {
+            Fuser f(s);
+            for (int i = 0; i < S; ++i)
+                check_launch(f, i, jobs[i].op, o.adjoint, jobs[i].x, jobs[i].ldx, jobs[i].b, jobs[i].ldb, jobs[i].k,
+                             jobs[i].op->rhs32.as<cx<float>>());
+            for (int i = 0; i < S; ++i) check_reduce(f, i, jobs[i].op, jobs[i].k);
+            for (int i = 0; i < S; ++i) {
+                sums[i].resize((size_t)jobs[i].k * 6);
+                HZ_HIP(hipMemcpyAsync(sums[i].data(), jobs[i].op->red.p, sums[i].size() * sizeof(double),
+                                      hipMemcpyDeviceToHost, s));
+            }
+            HZ_HIP(hipStreamSynchronize(s));
+        }
+        std::vector<Member> ms;
+        std::vector<int> job_of;
+        for (int i = 0; i < S; ++i) {
+            MixedJob &jb = jobs[i];
+            jb.last = check_read(sums[i], jb.k);
+            std::vector<double> atol(jb.k, 0.0);
+            bool any = false;
+            for (int r = 0; r < jb.k; ++r) {
+                if (jb.done[r]) continue;
+                const CheckResult &c = jb.last[r];
+                if (c.ub == 0.0) {
+                    jb.zero_rhs[r] = 1;
+                    jb.done[r] = 1;
+                    continue;
+                }
+                const double scaled = c.sb > 0 ? std::sqrt(c.se / c.sb) : 0.0;
+                const double rel = std::sqrt(c.ue / c.ub);
+                const bool rel_ok = !(o.relative_rtol > 0 && rel > o.relative_rtol);
+                const bool ok = c.bad == 0 && scaled <= o.rtol && c.bwd <= o.backward_error_tol && rel_ok;
+                if (verbose())
+                    std::fprintf(stderr,
+                                 "[hz] refinement %d scene %d rhs %d: scaled residual %.3e relative residual %.3e "
+                                 "backward error %.3e%s\n",
+                                 ref, i, r, scaled, rel, c.bwd, ok ? " (accepted)" : "");
+                if (ok || ref >= o.max_refinements || jb.cs.iterations[r] >= o.maxiter || c.bad != 0) {
+                    jb.done[r] = 1;
+                    continue;
+                }
+                // Aim below the scaled-residual target.  When only the
+                // componentwise backward error fails, ask for the reduction it
+                // needs with a factor of 2 to spare (at least 2x, at most
+                // 100x): it typically misses by less than 3x, and every decade
+                // asked for costs a fresh Krylov space dozens of steps.
+                // A relative-residual miss (when asked for) likewise asks for
+                // the reduction it needs, in the scaled norm the inner solve
+                // measures.
+                auto reduce = [&](double need) {
+                    return std::sqrt(c.se) * std::min(0.5, std::max(0.01, 0.5 * need));
+                };
+                double goal = scaled > o.rtol                   ? 0.5 * o.rtol * std::sqrt(c.sb)
+                              : c.bwd > o.backward_error_tol ? reduce(o.backward_error_tol / c.bwd)
+                                                                : HUGE_VAL;
+                if (!rel_ok) goal = std::min(goal, reduce(o.relative_rtol / rel));
+                atol[r] = std::max(goal, o.inner_rtol * std::sqrt(c.se));
+                any = true;
+            }
+            if (!any) continue;
+            // The inner right-hand side is the scaled residual written by the check.
+            HZ_HIP(hipMemcpyAsync(jb.op->Bh.p, jb.op->rhs32.p, (size_t)jb.k * jb.op->n * sizeof(cx<float>),
+                                  hipMemcpyDeviceToDevice, s));
+            for (int r = 0; r < jb.k; ++r) jb.cs.active[r] = jb.member[r] = jb.done[r] ? 0 : 1;
+            Member m;
+            m.op = jb.op;
+            m.k = jb.k;
+            m.cs = &jb.cs;  // the iteration budget is cumulative over refinements
+            m.atol = atol;
+            ms.push_back(std::move(m));
+            job_of.push_back(i);
+        }
+        if (ms.empty()) break;
+        gmres_cycles<float>(ms, o.adjoint ? MODE_RAMH : MODE_RAM, o.restart, o.maxiter, false, s, o.preconditioner,
+                            o.use_graph != 0, o.arnoldi_block, fused ? batch_cache : nullptr);
+        Fuser f(s);
+        for (size_t q = 0; q < ms.size(); ++q) {
+            MixedJob &jb = jobs[job_of[q]];
+            for (int r = 0; r < jb.k; ++r)
+                if (jb.member[r]) jb.refinements[r] += 1;
+            hz_operator *op = jb.op;
+            HZ_HIP(hipMemcpyAsync(op->member.p, jb.member.data(), jb.k * sizeof(int), hipMemcpyHostToDevice, s));
+            if (use_mg) {
+                // x += M y: X already holds it.
+                f.add<KAccumulate>(job_of[q], dim3(vec_blocks(op->n), jb.k),
+                                   {(const cx<float> *)op->X.as<cx<float>>(), op->n, jb.x, jb.ldx,
+                                    (const int *)op->member.as<int>()});
+            } else {
+                const cx<double> *m = o.adjoint ? op->ma64.as<cx<double>>() : op->mf64.as<cx<double>>();
+                f.add<KPrecondOut<float>>(job_of[q], dim3(vec_blocks(op->n), jb.k),
+                                          {(const cx<float> *)op->X.as<cx<float>>(), op->n, m, jb.x, jb.ldx, op->n,
+                                           (const int *)op->member.as<int>(), 1});
+            }
+        }
+    }
+    for (int i = 0; i < S; ++i) {
+        MixedJob &jb = jobs[i];
+        if (verbose() && o.arnoldi_block == 2) {
+            std::vector<int> fixes((size_t)2 * jb.k);
+            HZ_HIP(hipMemcpyAsync(fixes.data(), jb.op->bfix.p, fixes.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+            HZ_HIP(hipStreamSynchronize(s));
+            for (int r = 0; r < jb.k; ++r)
+                if (fixes[jb.k + r])
+                    std::fprintf(stderr, "[hz] scene %d rhs %d: %d two-step blocks took the measured coefficient\n", i,
+                                 r, fixes[jb.k + r]);
+        }
+        for (int r = 0; r < jb.k; ++r) {
+            hz_gmres_stats &st = jb.stats[r];
+            std::memset(&st, 0, sizeof(st));
+            if (jb.zero_rhs[r]) {
+                st.finite = 1;
+                st.converged = 1;
                 continue;
             }
-            const double scaled = c.sb > 0 ? std::sqrt(c.se / c.sb) : 0.0;
-            const double rel = std::sqrt(c.ue / c.ub);
-            const bool rel_ok = !(o.relative_rtol > 0 && rel > o.relative_rtol);
-            const bool ok = c.bad == 0 && scaled <= o.rtol && c.bwd <= o.backward_error_tol && rel_ok;
-            if (verbose())
-                std::fprintf(stderr,
-                             "[hz] refinement %d rhs %d: scaled residual %.3e relative residual %.3e backward error "
-                             "%.3e%s\n",
-                             ref, r, scaled, rel, c.bwd, ok ? " (accepted)" : "");
-            if (ok || ref >= o.max_refinements || cs.iterations[r] >= o.maxiter || c.bad != 0) {
-                done[r] = 1;
-                continue;
+            st.iterations = jb.cs.iterations[r];
+            st.cycles = jb.cs.cycles[r];
+            st.refinements = jb.refinements[r];
+            fill_stats(st, jb.last[r], o);
+            if (!st.converged) {
+                st.info = std::max(1, jb.cs.iterations[r]);
+                st.converged = 0;
             }
-            // Aim below the scaled-residual target.  When only the componentwise
-            // backward error fails, ask for the reduction it needs with a factor
-            // of 2 to spare (at least 2x, at most 100x): it typically misses by
-            // less than 3x, and every decade asked for costs a fresh Krylov
-            // space dozens of steps.
-            // A relative-residual miss (when asked for) likewise asks for the
-            // reduction it needs, in the scaled norm the inner solve measures.
-            auto reduce = [&](double need) { return std::sqrt(c.se) * std::min(0.5, std::max(0.01, 0.5 * need)); };
-            double goal = scaled > o.rtol                   ? 0.5 * o.rtol * std::sqrt(c.sb)
-                          : c.bwd > o.backward_error_tol ? reduce(o.backward_error_tol / c.bwd)
-                                                            : HUGE_VAL;
-            if (!rel_ok) goal = std::min(goal, reduce(o.relative_rtol / rel));
-            atol[r] = std::max(goal, o.inner_rtol * std::sqrt(c.se));
-            any = true;
-        }
-        if (!any) break;
-        // The inner right-hand side is the scaled residual written by the check.
-        HZ_HIP(hipMemcpyAsync(op->Bh.p, op->rhs32.p, (size_t)k * n * sizeof(cx<float>), hipMemcpyDeviceToDevice, s));
-        for (int r = 0; r < k; ++r) cs.active[r] = member[r] = done[r] ? 0 : 1;
-        // The iteration budget is cumulative over refinements (cs.iterations carries over).
-        ChunkState inner = cs;
-        const bool use_mg = o.preconditioner == PRE_MULTIGRID;
-        if (use_mg) mg_reserve(*op->mg, k);
-        gmres_cycles<float>(op, o.adjoint ? MODE_RAMH : MODE_RAM, k, o.restart, o.maxiter, false, atol, inner, s,
-                            o.preconditioner, o.use_graph != 0, o.arnoldi_block);
-        for (int r = 0; r < k; ++r) {
-            cs.iterations[r] = inner.iterations[r];
-            cs.cycles[r] = inner.cycles[r];
-            if (member[r]) refinements[r] += 1;
-        }
-        HZ_HIP(hipMemcpyAsync(op->member.p, member.data(), k * sizeof(int), hipMemcpyHostToDevice, s));
-        if (use_mg) {
-            // x += M y: X already holds it.
-            hipLaunchKernelGGL(k_accumulate, dim3(vec_blocks(n), k), dim3(256), 0, s,
-                               (const cx<float> *)op->X.as<cx<float>>(), n, x, ldx, (const int *)op->member.as<int>());
-        } else {
-            hipLaunchKernelGGL((k_precond_out<float>), dim3(vec_blocks(n), k), dim3(256), 0, s,
-                               (const cx<float> *)op->X.as<cx<float>>(), n, m, x, ldx, n,
-                               (const int *)op->member.as<int>(), 1);
-        }
-        HZ_LAUNCHED();
-    }
-    if (verbose() && o.arnoldi_block == 2) {
-        std::vector<int> fixes((size_t)2 * k);
-        HZ_HIP(hipMemcpyAsync(fixes.data(), op->bfix.p, fixes.size() * sizeof(int), hipMemcpyDeviceToHost, s));
-        HZ_HIP(hipStreamSynchronize(s));
-        for (int r = 0; r < k; ++r)
-            if (fixes[k + r])
-                std::fprintf(stderr, "[hz] rhs %d: %d two-step blocks took the measured coefficient\n", r,
-                             fixes[k + r]);
-    }
-    for (int r = 0; r < k; ++r) {
-        hz_gmres_stats &st = stats[r];
-        std::memset(&st, 0, sizeof(st));
-        if (zero_rhs[r]) {
-            st.finite = 1;
-            st.converged = 1;
-            continue;
-        }
-        st.iterations = cs.iterations[r];
-        st.cycles = cs.cycles[r];
-        st.refinements = refinements[r];
-        fill_stats(st, last[r], o);
-        if (!st.converged) {
-            st.info = std::max(1, cs.iterations[r]);
-            st.converged = 0;
         }
     }
 }
 
-// Validated hz_gmres on one stream (throws on bad arguments or HIP errors).
-void gmres_run(hz_operator *op, const hz_gmres_opts *opts, const void *b, void *x, int nrhs, int64_t ld,
-               hz_gmres_stats *stats, hipStream_t s) {
-    if (!op) throw BadArgument("null operator handle");
-    if (!opts || !b || !x || !stats) throw BadArgument("opts, b, x and stats are required");
-    const hz_gmres_opts &o = *opts;
-    if (nrhs < 1 || ld < op->n) throw BadArgument("need nrhs >= 1 and ld >= N");
+void check_opts(const hz_operator *op, const hz_gmres_opts &o) {
     if (o.restart < 1 || o.restart > 63) throw BadArgument("restart must be in [1, 63]");
     if (o.maxiter < 0) throw BadArgument("maxiter must be >= 0");
     if (!(o.rtol > 0) || !(o.backward_error_tol > 0)) throw BadArgument("tolerances must be positive");
@@ -1043,16 +1571,38 @@ void gmres_run(hz_operator *op, const hz_gmres_opts *opts, const void *b, void *
     } else if (o.preconditioner != PRE_JACOBI) {
         throw BadArgument("unknown preconditioner");
     }
-    // Right-hand sides per batch; the stencil kernels stage at most APPLY_MAX_RHS in LDS.
-    const int chunk = std::max(1, std::min<int>(std::min<int>(nrhs, APPLY_MAX_RHS), o.chunk > 0 ? o.chunk : 1));
+}
+
+// Right-hand sides per batch; the stencil kernels stage at most APPLY_MAX_RHS in LDS.
+int chunk_width(const hz_gmres_opts &o, int nrhs) {
+    return std::max(1, std::min<int>(std::min<int>(nrhs, APPLY_MAX_RHS), o.chunk > 0 ? o.chunk : 1));
+}
+
+// Validated hz_gmres on one stream (throws on bad arguments or HIP errors).
+void gmres_run(hz_operator *op, const hz_gmres_opts *opts, const void *b, void *x, int nrhs, int64_t ld,
+               hz_gmres_stats *stats, hipStream_t s) {
+    if (!op) throw BadArgument("null operator handle");
+    if (!opts || !b || !x || !stats) throw BadArgument("opts, b, x and stats are required");
+    const hz_gmres_opts &o = *opts;
+    if (nrhs < 1 || ld < op->n) throw BadArgument("need nrhs >= 1 and ld >= N");
+    check_opts(op, o);
+    const int chunk = chunk_width(o, nrhs);
     for (int c0 = 0; c0 < nrhs; c0 += chunk) {
         const int k = std::min(chunk, nrhs - c0);
         const cx<double> *bc = static_cast<const cx<double> *>(b) + (int64_t)c0 * ld;
         cx<double> *xc = static_cast<cx<double> *>(x) + (int64_t)c0 * ld;
-        if (o.precision == HZ_C128)
+        if (o.precision == HZ_C128) {
             solve_plain(op, o, bc, ld, xc, ld, k, stats + c0, s);
-        else
-            solve_mixed(op, o, bc, ld, xc, ld, k, stats + c0, s);
+        } else {
+            std::vector<MixedJob> jobs(1);
+            jobs[0].op = op;
+            jobs[0].b = bc;
+            jobs[0].x = xc;
+            jobs[0].ldb = jobs[0].ldx = ld;
+            jobs[0].k = k;
+            jobs[0].stats = stats + c0;
+            solve_mixed(jobs, o, s, nullptr);
+        }
     }
 }
 
@@ -1121,10 +1671,7 @@ size_t chunk_bytes(const hz_operator *op, int k, int restart) {
     return nodes * k * sizeof(cx<float>);
 }
 
-void drop_graphs(hz_operator *op) {
-    for (auto &g : op->graphs) (void)hipGraphExecDestroy(g.exec);
-    op->graphs.clear();
-}
+void drop_graphs(hz_operator *op) { op->graphs.clear(); }
 
 void release_workspace(hz_operator *op) {
     for (DevBuf *b : {&op->V, &op->X, &op->Z, &op->Bh, &op->rhs32}) b->release();
@@ -1149,15 +1696,17 @@ void arnoldi_step_for_timing(hz_operator *op, int k, int j, int restart, bool us
     const cx<float> *in = V + (int64_t)j * k * n;
     cx<float> *w = V + (int64_t)(j + 1) * k * n;
     const int *live = op->live.as<int>();
+    Fuser f(s);
     if (use_mg) {
         cx<float> *z = op->mg->levels[0]->xx.as<cx<float>>();
-        precondition(op, false, in, z, k, live, s);
+        std::vector<MgTask> t = {{0, op->mg.get(), in, z, k, live, op->v32.rs_f}};
+        precondition(f, t, false);
         in = z;
     }
-    apply_mode<float>(op, use_mg ? MODE_RA : MODE_RAM, in, n, w, n, k, live, op->scales.as<double>(), s);
+    apply_mode<float>(f, 0, op, use_mg ? MODE_RA : MODE_RAM, in, n, w, n, k, live, op->scales.as<double>());
     const int groups = (j + 1 + DOT_GROUP - 1) / DOT_GROUP;
-    launch_dots<float>(dim3(nblk, k, groups), s, V, k, n, j + 1, w, live, op->dpart.as<double2>(), maxv, chunk);
-    launch_update<float>(dim3(nblk, k), s, V, k, n, j + 1, nullptr, w, n, op->coef.as<cx<double>>(), maxv, live,
+    launch_dots<float>(f, 0, dim3(nblk, k, groups), V, k, n, j + 1, w, live, op->dpart.as<double2>(), maxv, chunk);
+    launch_update<float>(f, 0, dim3(nblk, k), V, k, n, j + 1, nullptr, w, n, op->coef.as<cx<double>>(), maxv, live,
                          op->npart.as<double>(), chunk);
 }
 
@@ -1348,13 +1897,13 @@ hz_status hz_apply(hz_operator *op, int32_t precision, int32_t mode, const void 
         if (!x || !y) throw BadArgument("x and y must be device pointers");
         if (nrhs < 1 || ld < op->n) throw BadArgument("need nrhs >= 1 and ld >= N");
         if (mode < MODE_A || mode > MODE_RAH) throw BadArgument("unknown apply mode");
-        hipStream_t s = static_cast<hipStream_t>(stream);
+        Fuser f(static_cast<hipStream_t>(stream));
         if (precision == HZ_C128)
-            apply_mode<double>(op, mode, static_cast<const cx<double> *>(x), ld, static_cast<cx<double> *>(y), ld,
-                               nrhs, nullptr, nullptr, s);
+            apply_mode<double>(f, 0, op, mode, static_cast<const cx<double> *>(x), ld, static_cast<cx<double> *>(y),
+                               ld, nrhs, nullptr, nullptr);
         else if (precision == HZ_C64)
-            apply_mode<float>(op, mode, static_cast<const cx<float> *>(x), ld, static_cast<cx<float> *>(y), ld,
-                              nrhs, nullptr, nullptr, s);
+            apply_mode<float>(f, 0, op, mode, static_cast<const cx<float> *>(x), ld, static_cast<cx<float> *>(y), ld,
+                              nrhs, nullptr, nullptr);
         else
             throw BadArgument("unknown precision");
     });
@@ -1450,6 +1999,69 @@ hz_status hz_gmres_batch(int32_t nscenes, hz_operator *const *ops, const hz_gmre
     for (int sc = 0; sc < nscenes; ++sc)
         for (int r = 0; r < nrhs[sc]; ++r)
             if (!stats[sc][r].converged) return HZ_NOT_CONVERGED;
+    return HZ_OK;
+}
+
+hz_status hz_gmres_fused(int32_t nscenes, hz_operator *const *ops, const hz_gmres_opts *opts, const void *const *b,
+                         void *const *x, const int32_t *nrhs, const int64_t *ld, hz_gmres_stats *const *stats,
+                         void *stream) {
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const hz_status st = guarded([&] {
+        if (nscenes < 1 || !ops || !opts || !b || !x || !nrhs || !ld || !stats)
+            throw BadArgument("need nscenes >= 1 and every per-scene array");
+        const hz_gmres_opts &o = *opts;
+        if (o.precision != HZ_C64) throw BadArgument("fused multi-scene solves run in the mixed form (HZ_C64)");
+        int most = 0;
+        for (int i = 0; i < nscenes; ++i) {
+            check_op(ops[i]);
+            for (int j = 0; j < i; ++j)
+                if (ops[j] == ops[i]) throw BadArgument("an operator appears twice in one batch");
+            if (!b[i] || !x[i] || !stats[i]) throw BadArgument("b, x and stats are required for every scene");
+            if (nrhs[i] < 1 || ld[i] < ops[i]->n) throw BadArgument("need nrhs >= 1 and ld >= N for every scene");
+            check_opts(ops[i], o);
+            most = std::max<int>(most, nrhs[i]);
+        }
+        // Round q solves columns [q c, (q + 1) c) of every scene that has
+        // them, all scenes in lockstep; the launch sequences (fused tables,
+        // captured cycle bodies) are kept across the round's refinements.
+        const int chunk = chunk_width(o, most);
+        LaunchCache cache;
+        cache.limit = 64;
+        for (int c0 = 0; c0 < most; c0 += chunk) {
+            std::vector<MixedJob> jobs;
+            for (int i = 0; i < nscenes; ++i) {
+                if (c0 >= nrhs[i]) continue;
+                MixedJob jb;
+                jb.op = ops[i];
+                jb.b = static_cast<const cx<double> *>(b[i]) + (int64_t)c0 * ld[i];
+                jb.x = static_cast<cx<double> *>(x[i]) + (int64_t)c0 * ld[i];
+                jb.ldb = jb.ldx = ld[i];
+                jb.k = std::min(chunk, nrhs[i] - c0);
+                jb.stats = stats[i] + c0;
+                jobs.push_back(std::move(jb));
+            }
+            solve_mixed(jobs, o, s, &cache);
+        }
+        HZ_HIP(hipStreamSynchronize(s));
+        cache.clear();
+        for (int i = 0; i < nscenes; ++i) {
+            drop_graphs(ops[i]);
+            release_workspace(ops[i]);
+        }
+    });
+    if (st != HZ_OK) return st;
+    for (int sc = 0; sc < nscenes; ++sc)
+        for (int r = 0; r < nrhs[sc]; ++r)
+            if (!stats[sc][r].converged) return HZ_NOT_CONVERGED;
+    return HZ_OK;
+}
+
+hz_status hz_launch_counts(int64_t *counts, int32_t reset) {
+    if (!counts) {
+        g_error = "counts is null";
+        return HZ_BAD_ARGUMENT;
+    }
+    for (int q = 0; q < 3; ++q) counts[q] = reset ? g_launches[q].exchange(0) : g_launches[q].load();
     return HZ_OK;
 }
 
@@ -1714,6 +2326,7 @@ hz_status hz_autotune(hz_operator *op, int32_t nrhs, int32_t max_chunk, int32_t 
         if (use_mg && !op->mg) throw BadArgument("multigrid preconditioner requested but no hierarchy installed");
         if (!use_mg && preconditioner != PRE_JACOBI) throw BadArgument("unknown preconditioner");
         hipStream_t s = static_cast<hipStream_t>(stream);
+        Fuser fs(s);
         const int64_t n = op->n;
         size_t free_b = 0, total_b = 0;
         HZ_HIP(hipMemGetInfo(&free_b, &total_b));
@@ -1759,18 +2372,18 @@ hz_status hz_autotune(hz_operator *op, int32_t nrhs, int32_t max_chunk, int32_t 
         const int mode = use_mg ? MODE_RA : MODE_RAM;
         warm_up(
             [&] {
-                launch_slab<MODE_A, SLAB_APPLY>(op->slab_shape, op->v32, x, n, y, n, kt, live,
-                                                op->scales.as<double>(), nullptr, nullptr, 0.0f, nullptr, s);
+                launch_slab<MODE_A, SLAB_APPLY>(fs, 0, op->slab_shape, op->v32, x, n, y, n, kt, live,
+                                                op->scales.as<double>(), nullptr, nullptr, 0.0f, nullptr);
             },
             0.3, s);
         op->slab_shape = pick_shape(
             [&](int sh) {
                 if (mode == MODE_RA)
-                    launch_slab<MODE_RA, SLAB_APPLY>(sh, op->v32, x, n, y, n, kt, live, op->scales.as<double>(),
-                                                     nullptr, nullptr, 0.0f, nullptr, s);
+                    launch_slab<MODE_RA, SLAB_APPLY>(fs, 0, sh, op->v32, x, n, y, n, kt, live, op->scales.as<double>(),
+                                                     nullptr, nullptr, 0.0f, nullptr);
                 else
-                    launch_slab<MODE_RAM, SLAB_APPLY>(sh, op->v32, x, n, y, n, kt, live, op->scales.as<double>(),
-                                                      nullptr, nullptr, 0.0f, nullptr, s);
+                    launch_slab<MODE_RAM, SLAB_APPLY>(fs, 0, sh, op->v32, x, n, y, n, kt, live, op->scales.as<double>(),
+                                                      nullptr, nullptr, 0.0f, nullptr);
             },
             s);
         // Multigrid level kernels (forward operator; the adjoint uses the same shapes).
@@ -1785,20 +2398,20 @@ hz_status hz_autotune(hz_operator *op, int32_t nrhs, int32_t max_chunk, int32_t 
                 const float *bd = l == 0 ? op->v32.rs_f : nullptr;
                 L.shape[0] = pick_shape(
                     [&](int sh) {
-                        launch_slab<MODE_A, SLAB_SMOOTH>(sh, L.v, lx, L.n, lt, L.n, kt, live, nullptr, lb, dv,
-                                                         mg.weight, bd, s);
+                        launch_slab<MODE_A, SLAB_SMOOTH>(fs, 0, sh, L.v, lx, L.n, lt, L.n, kt, live, nullptr, lb, dv,
+                                                         mg.weight, bd);
                     },
                     s);
                 L.shape[1] = pick_shape(
                     [&](int sh) {
-                        launch_slab<MODE_A, SLAB_RESID>(sh, L.v, lx, L.n, lt, L.n, kt, live, nullptr, lb, nullptr, 0.0f,
-                                                        bd, s);
+                        launch_slab<MODE_A, SLAB_RESID>(fs, 0, sh, L.v, lx, L.n, lt, L.n, kt, live, nullptr, lb, nullptr, 0.0f,
+                                                        bd);
                     },
                     s);
                 L.shape[2] = pick_shape(
                     [&](int sh) {
-                        launch_slab<MODE_A, SLAB_RESID0>(sh, L.v, lb, L.n, lt, L.n, kt, live, nullptr, nullptr, dv,
-                                                         mg.weight, bd, s);
+                        launch_slab<MODE_A, SLAB_RESID0>(fs, 0, sh, L.v, lb, L.n, lt, L.n, kt, live, nullptr, nullptr, dv,
+                                                         mg.weight, bd);
                     },
                     s, 1);
             }
@@ -2116,13 +2729,13 @@ hz_status hz_mg_transfer(hz_operator *op, int32_t level, int32_t direction, cons
         auto a16 = [](const void *ptr) { return ((uintptr_t)ptr & 15) == 0; };
         if (L.nx % 2 == 0 && !a16(direction == 0 ? in : out))
             throw BadArgument("even-width fine vectors must be 16-byte aligned");
-        hipStream_t s = static_cast<hipStream_t>(stream);
+        Fuser f(static_cast<hipStream_t>(stream));
         if (direction == 0)
-            launch_restrict(L, C, static_cast<const cx<float> *>(in), static_cast<cx<float> *>(out), nrhs, nullptr,
-                            s);
+            launch_restrict(f, 0, L, C, static_cast<const cx<float> *>(in), static_cast<cx<float> *>(out), nrhs,
+                            nullptr);
         else
-            launch_prolong(L, C, static_cast<const cx<float> *>(in), static_cast<cx<float> *>(out), nrhs, nullptr,
-                           nullptr, nullptr, 0.0f, nullptr, 0, s);
+            launch_prolong(f, 0, L, C, static_cast<const cx<float> *>(in), static_cast<cx<float> *>(out), nrhs,
+                           nullptr, nullptr, nullptr, 0.0f, nullptr, 0);
     });
 }
 

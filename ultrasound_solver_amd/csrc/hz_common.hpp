@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "hz_multi.hpp"
+
 namespace hz {
 
 template <typename T>
@@ -47,6 +49,31 @@ __host__ __device__ __forceinline__ cx<T> crecip(cx<T> b) {
 }
 template <typename To, typename From>
 __host__ __device__ __forceinline__ cx<To> cast(cx<From> a) { return {(To)a.re, (To)a.im}; }
+
+// Multiply-adds of the hot loops as FMAs in a fixed order.  The library is
+// compiled with expression-level contraction only, so these (and not the
+// compiler's scheduling) set the rounding of a stencil row or a Krylov
+// update wherever it is evaluated.
+// acc + a b
+template <typename T>
+__device__ __forceinline__ cx<T> cfma(cx<T> a, cx<T> b, cx<T> acc) {
+    return {fma(-a.im, b.im, fma(a.re, b.re, acc.re)), fma(a.im, b.re, fma(a.re, b.im, acc.im))};
+}
+// acc - a b
+template <typename T>
+__device__ __forceinline__ cx<T> cfnma(cx<T> a, cx<T> b, cx<T> acc) {
+    return {fma(a.im, b.im, fma(-a.re, b.re, acc.re)), fma(-a.im, b.re, fma(-a.re, b.im, acc.im))};
+}
+// acc + s x (s real)
+template <typename T>
+__device__ __forceinline__ cx<T> sfma(T s, cx<T> x, cx<T> acc) {
+    return {fma(s, x.re, acc.re), fma(s, x.im, acc.im)};
+}
+// s x - acc (s real)
+template <typename T>
+__device__ __forceinline__ cx<T> sfms(T s, cx<T> x, cx<T> acc) {
+    return {fma(s, x.re, -acc.re), fma(s, x.im, -acc.im)};
+}
 
 // Per-precision operator view passed by value to every kernel.
 template <typename T>

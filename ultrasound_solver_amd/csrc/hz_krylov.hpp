@@ -23,16 +23,16 @@ constexpr int DOT_GROUP = 16;  // basis vectors per dot-product pass
 // VW = 2 (complex64 only; n and chunk even): each lane loads two complex
 // values per 16-byte access.
 template <typename T, int VW>
-__global__ __launch_bounds__(256) void k_dots(const cx<T> *__restrict__ V, int k, int64_t n, int jp1,
-                                              const cx<T> *__restrict__ w, const int *__restrict__ live,
-                                              double2 *__restrict__ partial, int maxv, int64_t chunk) {
+__device__ __forceinline__ void dots_body(const cx<T> *__restrict__ V, int k, int64_t n, int jp1,
+        const cx<T> *__restrict__ w, const int *__restrict__ live, double2 *__restrict__ partial, int maxv,
+        int64_t chunk, const Blk &bk) {
     __shared__ double red[4];
-    const int r = blockIdx.y;
+    const int r = bk.y;
     if (!live[r]) return;
-    const int g0 = blockIdx.z * DOT_GROUP;
+    const int g0 = bk.z * DOT_GROUP;
     const int cnt = min(DOT_GROUP, jp1 - g0);
     if (cnt <= 0) return;
-    const int64_t p0 = blockIdx.x * chunk;
+    const int64_t p0 = bk.x * chunk;
     const int64_t p1 = min(n, p0 + chunk);
     const cx<T> *wr = w + (int64_t)r * n;
     T are[DOT_GROUP], aim[DOT_GROUP];
@@ -72,20 +72,49 @@ __global__ __launch_bounds__(256) void k_dots(const cx<T> *__restrict__ V, int k
             const double sre = block_combine_256((double)wave_sum(are[q]), red);
             const double sim = block_combine_256((double)wave_sum(aim[q]), red);
             if (threadIdx.x == 0)
-                partial[((int64_t)r * maxv + g0 + q) * gridDim.x + blockIdx.x] = make_double2(sre, sim);
+                partial[((int64_t)r * maxv + g0 + q) * bk.gx + bk.x] = make_double2(sre, sim);
         }
     }
 }
 
+template <typename T, int VW>
+__global__ __launch_bounds__(256) void k_dots(const cx<T> *__restrict__ V, int k, int64_t n, int jp1,
+        const cx<T> *__restrict__ w, const int *__restrict__ live, double2 *__restrict__ partial, int maxv,
+        int64_t chunk) {
+    dots_body<T, VW>(V, k, n, jp1, w, live, partial, maxv, chunk, blk_here());
+}
+
+template <typename T, int VW>
+struct KDots {
+    static constexpr int THREADS = 256;
+    struct Args {
+        const cx<T> *V;
+        int k;
+        int64_t n;
+        int jp1;
+        const cx<T> *w;
+        const int *live;
+        double2 *partial;
+        int maxv;
+        int64_t chunk;
+    };
+    __device__ __forceinline__ static void run(const Args &a, const Blk &b) {
+        dots_body<T, VW>(a.V, a.k, a.n, a.jp1, a.w, a.live, a.partial, a.maxv, a.chunk, b);
+    }
+    static void single(dim3 g, hipStream_t s, const Args &a) {
+        hipLaunchKernelGGL((k_dots<T, VW>), g, dim3(THREADS), 0, s, a.V, a.k, a.n, a.jp1, a.w, a.live, a.partial,
+                a.maxv, a.chunk);
+    }
+};
+
 // Combine dot partials: d_i = sum_blk partial; H[r](i, j) = s_i d_i and the
 // update coefficient coef[r][i] = s_i^2 d_i.  Grid (jp1, k).
-__global__ __launch_bounds__(256) void k_reduce_dots(const double2 *__restrict__ partial, int nblk, int maxv,
-                                                     int j, const double *__restrict__ scales,
-                                                     const int *__restrict__ live, cx<double> *__restrict__ H,
-                                                     cx<double> *__restrict__ coef, int restart) {
+__device__ __forceinline__ void reduce_dots_body(const double2 *__restrict__ partial, int nblk, int maxv, int j,
+        const double *__restrict__ scales, const int *__restrict__ live, cx<double> *__restrict__ H,
+        cx<double> *__restrict__ coef, int restart, const Blk &bk) {
     __shared__ double red[4];
-    const int i = blockIdx.x;
-    const int r = blockIdx.y;
+    const int i = bk.x;
+    const int r = bk.y;
     if (!live[r]) return;
     const double2 *src = partial + ((int64_t)r * maxv + i) * nblk;
     double re = 0, im = 0;
@@ -96,30 +125,56 @@ __global__ __launch_bounds__(256) void k_reduce_dots(const double2 *__restrict__
     re = block_sum_256(re, red);
     im = block_sum_256(im, red);
     if (threadIdx.x == 0) {
-        const double s = scales[(int64_t)i * gridDim.y + r];
+        const double s = scales[(int64_t)i * bk.gy + r];
         const cx<double> d = mk<double>(re, im);
         H[((int64_t)r * (restart + 1) + i) * restart + j] = s * d;
         coef[r * maxv + i] = (s * s) * d;
     }
 }
 
+__global__ __launch_bounds__(256) void k_reduce_dots(const double2 *__restrict__ partial, int nblk, int maxv, int j,
+        const double *__restrict__ scales, const int *__restrict__ live, cx<double> *__restrict__ H,
+        cx<double> *__restrict__ coef, int restart) {
+    reduce_dots_body(partial, nblk, maxv, j, scales, live, H, coef, restart, blk_here());
+}
+
+struct KReduceDots {
+    static constexpr int THREADS = 256;
+    struct Args {
+        const double2 *partial;
+        int nblk;
+        int maxv;
+        int j;
+        const double *scales;
+        const int *live;
+        cx<double> *H;
+        cx<double> *coef;
+        int restart;
+    };
+    __device__ __forceinline__ static void run(const Args &a, const Blk &b) {
+        reduce_dots_body(a.partial, a.nblk, a.maxv, a.j, a.scales, a.live, a.H, a.coef, a.restart, b);
+    }
+    static void single(dim3 g, hipStream_t s, const Args &a) {
+        hipLaunchKernelGGL((k_reduce_dots), g, dim3(THREADS), 0, s, a.partial, a.nblk, a.maxv, a.j, a.scales, a.live,
+                a.H, a.coef, a.restart);
+    }
+};
+
 // target[r] -= sum_{i<cnt} coef[r][i] * V_i[r]; optional FP64 partial of
 // |target|^2 per workgroup (norm_partial[r*nblk + blk]).  Grid (nblk, k).
 // VW = 2 as in k_dots (target leading dimension even as well).
 template <typename T, int VW>
-__global__ __launch_bounds__(256) void k_update(const cx<T> *__restrict__ V, int k, int64_t n, int cnt_all,
-                                                const int *__restrict__ cnt_rhs, cx<T> *__restrict__ target,
-                                                int64_t ldt, const cx<double> *__restrict__ coef, int maxv,
-                                                const int *__restrict__ live, double *__restrict__ norm_partial,
-                                                int64_t chunk) {
+__device__ __forceinline__ void update_body(const cx<T> *__restrict__ V, int k, int64_t n, int cnt_all,
+        const int *__restrict__ cnt_rhs, cx<T> *__restrict__ target, int64_t ldt, const cx<double> *__restrict__ coef,
+        int maxv, const int *__restrict__ live, double *__restrict__ norm_partial, int64_t chunk, const Blk &bk) {
     __shared__ cx<T> cf[64];
     __shared__ double red[4];
-    const int r = blockIdx.y;
+    const int r = bk.y;
     if (!live[r]) return;
     const int cnt = cnt_rhs ? cnt_rhs[r] : cnt_all;
     for (int q = threadIdx.x; q < cnt; q += 256) cf[q] = cast<T>(coef[r * maxv + q]);
     __syncthreads();
-    const int64_t p0 = blockIdx.x * chunk;
+    const int64_t p0 = bk.x * chunk;
     const int64_t p1 = min(n, p0 + chunk);
     cx<T> *tr = target + (int64_t)r * ldt;
     T nrm = 0;
@@ -137,10 +192,10 @@ __global__ __launch_bounds__(256) void k_update(const cx<T> *__restrict__ V, int
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
                     const cx<float> c = cf[q + u];
-                    acc.x -= c.re * v[u].x - c.im * v[u].y;
-                    acc.y -= c.re * v[u].y + c.im * v[u].x;
-                    acc.z -= c.re * v[u].z - c.im * v[u].w;
-                    acc.w -= c.re * v[u].w + c.im * v[u].z;
+                    acc.x = fmaf(c.im, v[u].y, fmaf(-c.re, v[u].x, acc.x));
+                    acc.y = fmaf(-c.im, v[u].x, fmaf(-c.re, v[u].y, acc.y));
+                    acc.z = fmaf(c.im, v[u].w, fmaf(-c.re, v[u].z, acc.z));
+                    acc.w = fmaf(-c.im, v[u].z, fmaf(-c.re, v[u].w, acc.w));
                 }
             }
             for (; q + 4 <= cnt; q += 4) {
@@ -151,19 +206,19 @@ __global__ __launch_bounds__(256) void k_update(const cx<T> *__restrict__ V, int
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const cx<float> c = cf[q + u];
-                    acc.x -= c.re * v[u].x - c.im * v[u].y;
-                    acc.y -= c.re * v[u].y + c.im * v[u].x;
-                    acc.z -= c.re * v[u].z - c.im * v[u].w;
-                    acc.w -= c.re * v[u].w + c.im * v[u].z;
+                    acc.x = fmaf(c.im, v[u].y, fmaf(-c.re, v[u].x, acc.x));
+                    acc.y = fmaf(-c.im, v[u].x, fmaf(-c.re, v[u].y, acc.y));
+                    acc.z = fmaf(c.im, v[u].w, fmaf(-c.re, v[u].z, acc.z));
+                    acc.w = fmaf(-c.im, v[u].z, fmaf(-c.re, v[u].w, acc.w));
                 }
             }
             for (; q < cnt; ++q) {
                 const float4 v = reinterpret_cast<const float4 *>(V + ((int64_t)q * k + r) * n)[p];
                 const cx<float> c = cf[q];
-                acc.x -= c.re * v.x - c.im * v.y;
-                acc.y -= c.re * v.y + c.im * v.x;
-                acc.z -= c.re * v.z - c.im * v.w;
-                acc.w -= c.re * v.w + c.im * v.z;
+                acc.x = fmaf(c.im, v.y, fmaf(-c.re, v.x, acc.x));
+                acc.y = fmaf(-c.im, v.x, fmaf(-c.re, v.y, acc.y));
+                acc.z = fmaf(c.im, v.w, fmaf(-c.re, v.z, acc.z));
+                acc.w = fmaf(-c.im, v.z, fmaf(-c.re, v.w, acc.w));
             }
             t4[p] = acc;
             nrm += acc.x * acc.x + acc.y * acc.y + acc.z * acc.z + acc.w * acc.w;
@@ -177,21 +232,55 @@ __global__ __launch_bounds__(256) void k_update(const cx<T> *__restrict__ V, int
                 const cx<T> v1 = V[((int64_t)(q + 1) * k + r) * n + p];
                 const cx<T> v2 = V[((int64_t)(q + 2) * k + r) * n + p];
                 const cx<T> v3 = V[((int64_t)(q + 3) * k + r) * n + p];
-                acc = acc - cf[q] * v0;
-                acc = acc - cf[q + 1] * v1;
-                acc = acc - cf[q + 2] * v2;
-                acc = acc - cf[q + 3] * v3;
+                acc = cfnma(cf[q], v0, acc);
+                acc = cfnma(cf[q + 1], v1, acc);
+                acc = cfnma(cf[q + 2], v2, acc);
+                acc = cfnma(cf[q + 3], v3, acc);
             }
-            for (; q < cnt; ++q) acc = acc - cf[q] * V[((int64_t)q * k + r) * n + p];
+            for (; q < cnt; ++q) acc = cfnma(cf[q], V[((int64_t)q * k + r) * n + p], acc);
             tr[p] = acc;
             nrm += norm2(acc);
         }
     }
     if (norm_partial) {
         const double s = block_combine_256((double)wave_sum(nrm), red);
-        if (threadIdx.x == 0) norm_partial[(int64_t)r * gridDim.x + blockIdx.x] = s;
+        if (threadIdx.x == 0) norm_partial[(int64_t)r * bk.gx + bk.x] = s;
     }
 }
+
+template <typename T, int VW>
+__global__ __launch_bounds__(256) void k_update(const cx<T> *__restrict__ V, int k, int64_t n, int cnt_all,
+        const int *__restrict__ cnt_rhs, cx<T> *__restrict__ target, int64_t ldt, const cx<double> *__restrict__ coef,
+        int maxv, const int *__restrict__ live, double *__restrict__ norm_partial, int64_t chunk) {
+    update_body<T, VW>(V, k, n, cnt_all, cnt_rhs, target, ldt, coef, maxv, live, norm_partial, chunk, blk_here());
+}
+
+template <typename T, int VW>
+struct KUpdate {
+    static constexpr int THREADS = 256;
+    struct Args {
+        const cx<T> *V;
+        int k;
+        int64_t n;
+        int cnt_all;
+        const int *cnt_rhs;
+        cx<T> *target;
+        int64_t ldt;
+        const cx<double> *coef;
+        int maxv;
+        const int *live;
+        double *norm_partial;
+        int64_t chunk;
+    };
+    __device__ __forceinline__ static void run(const Args &a, const Blk &b) {
+        update_body<T, VW>(a.V, a.k, a.n, a.cnt_all, a.cnt_rhs, a.target, a.ldt, a.coef, a.maxv, a.live,
+                a.norm_partial, a.chunk, b);
+    }
+    static void single(dim3 g, hipStream_t s, const Args &a) {
+        hipLaunchKernelGGL((k_update<T, VW>), g, dim3(THREADS), 0, s, a.V, a.k, a.n, a.cnt_all, a.cnt_rhs, a.target,
+                a.ldt, a.coef, a.maxv, a.live, a.norm_partial, a.chunk);
+    }
+};
 
 // Per-RHS GMRES state, FP64, on the device.
 struct GmresState {
@@ -213,10 +302,10 @@ struct GmresState {
 };
 
 // After the cycle-top residual: beta0 = ||r||, s_0 = 1/beta0, g = beta0 e_1.
-__global__ __launch_bounds__(256) void k_cycle_start(GmresState st, const double *__restrict__ partial, int nblk,
-                                                     int maxv, int restart) {
+__device__ __forceinline__ void cycle_start_body(GmresState st, const double *__restrict__ partial, int nblk,
+        int maxv, int restart, const Blk &bk) {
     __shared__ double red[4];
-    const int r = blockIdx.x;
+    const int r = bk.x;
     if (!st.live[r]) return;
     double s = 0;
     for (int b = threadIdx.x; b < nblk; b += 256) s += partial[(int64_t)r * nblk + b];
@@ -230,6 +319,28 @@ __global__ __launch_bounds__(256) void k_cycle_start(GmresState st, const double
         st.steps[r] = 0;
     }
 }
+
+__global__ __launch_bounds__(256) void k_cycle_start(GmresState st, const double *__restrict__ partial, int nblk,
+        int maxv, int restart) {
+    cycle_start_body(st, partial, nblk, maxv, restart, blk_here());
+}
+
+struct KCycleStart {
+    static constexpr int THREADS = 256;
+    struct Args {
+        GmresState st;
+        const double *partial;
+        int nblk;
+        int maxv;
+        int restart;
+    };
+    __device__ __forceinline__ static void run(const Args &a, const Blk &b) {
+        cycle_start_body(a.st, a.partial, a.nblk, a.maxv, a.restart, b);
+    }
+    static void single(dim3 g, hipStream_t s, const Args &a) {
+        hipLaunchKernelGGL((k_cycle_start), g, dim3(THREADS), 0, s, a.st, a.partial, a.nblk, a.maxv, a.restart);
+    }
+};
 
 // Givens update of column j of H (rotated storage) whose subdiagonal entry
 // is beta (real, >= 0): apply the previous rotations, make the new one,
@@ -282,19 +393,43 @@ __device__ bool givens_column(GmresState &st, int r, int j, double beta, int res
 
 // Finish Arnoldi step j: beta = ||w||, H(j+1, j) = beta, Givens update,
 // residual estimate |g_{j+1}|, early exit when it meets st.tol[r].
-__global__ __launch_bounds__(256) void k_arnoldi_finish(GmresState st, const double *__restrict__ norm_partial,
-                                                        int nblk, int j, int maxv, int restart) {
+__device__ __forceinline__ void arnoldi_finish_body(GmresState st, const double *__restrict__ norm_partial, int nblk,
+        int j, int maxv, int restart, const Blk &bk) {
     __shared__ double red[4];
-    const int r = blockIdx.x;
+    const int r = bk.x;
     if (!st.live[r]) return;
     double s = 0;
     for (int b = threadIdx.x; b < nblk; b += 256) s += norm_partial[(int64_t)r * nblk + b];
     s = block_sum_256(s, red);
     if (threadIdx.x != 0) return;
     const double beta = sqrt(s);
-    st.scales[(int64_t)(j + 1) * gridDim.x + r] = beta > 0 ? 1.0 / beta : 0.0;
+    st.scales[(int64_t)(j + 1) * bk.gx + r] = beta > 0 ? 1.0 / beta : 0.0;
     givens_column(st, r, j, beta, restart);
 }
+
+__global__ __launch_bounds__(256) void k_arnoldi_finish(GmresState st, const double *__restrict__ norm_partial,
+        int nblk, int j, int maxv, int restart) {
+    arnoldi_finish_body(st, norm_partial, nblk, j, maxv, restart, blk_here());
+}
+
+struct KArnoldiFinish {
+    static constexpr int THREADS = 256;
+    struct Args {
+        GmresState st;
+        const double *norm_partial;
+        int nblk;
+        int j;
+        int maxv;
+        int restart;
+    };
+    __device__ __forceinline__ static void run(const Args &a, const Blk &b) {
+        arnoldi_finish_body(a.st, a.norm_partial, a.nblk, a.j, a.maxv, a.restart, b);
+    }
+    static void single(dim3 g, hipStream_t s, const Args &a) {
+        hipLaunchKernelGGL((k_arnoldi_finish), g, dim3(THREADS), 0, s, a.st, a.norm_partial, a.nblk, a.j, a.maxv,
+                a.restart);
+    }
+};
 
 // ------------------------------------------------------- two-step Arnoldi blocks
 // A block at basis index j generates w1 = A M q_j (slot j+1) and
@@ -315,25 +450,24 @@ __global__ __launch_bounds__(256) void k_arnoldi_finish(GmresState st, const dou
 // w1^H w2 (wgram[(r*nblk + blk)*4 + 0..2]), from which the intra-block
 // coefficient follows before the update pass.
 template <typename T, int VW>
-__global__ __launch_bounds__(256) void k_dots2(const cx<T> *__restrict__ V, int k, int64_t n, int jp1,
-                                               const cx<T> *__restrict__ w1, const cx<T> *__restrict__ w2,
-                                               const int *__restrict__ live, double2 *__restrict__ partial, int maxv,
-                                               int64_t chunk, double *__restrict__ wgram) {
+__device__ __forceinline__ void dots2_body(const cx<T> *__restrict__ V, int k, int64_t n, int jp1,
+        const cx<T> *__restrict__ w1, const cx<T> *__restrict__ w2, const int *__restrict__ live,
+        double2 *__restrict__ partial, int maxv, int64_t chunk, double *__restrict__ wgram, const Blk &bk) {
     constexpr int G = DOT_GROUP / 2;
     __shared__ double red[4];
-    const int r = blockIdx.y;
+    const int r = bk.y;
     if (!live[r]) return;
-    const int g0 = blockIdx.z * G;
+    const int g0 = bk.z * G;
     const int cnt = min(G, jp1 - g0);
     if (cnt <= 0) return;
-    const int64_t p0 = blockIdx.x * chunk;
+    const int64_t p0 = bk.x * chunk;
     const int64_t p1 = min(n, p0 + chunk);
     const cx<T> *ar = w1 + (int64_t)r * n;
     const cx<T> *br = w2 + (int64_t)r * n;
     T are[G], aim[G], bre[G], bim[G];
 #pragma unroll
     for (int q = 0; q < G; ++q) are[q] = aim[q] = bre[q] = bim[q] = 0;
-    const bool first = blockIdx.z == 0;
+    const bool first = bk.z == 0;
     // |w1|^2 and w1^H w2 feed a difference (|W1|^2 = |w1|^2 - |h1|^2) that
     // can cancel: they are summed in FP64 from the first product on.
     double waa = 0, wabr = 0, wabi = 0;
@@ -390,8 +524,8 @@ __global__ __launch_bounds__(256) void k_dots2(const cx<T> *__restrict__ V, int 
             const double s2 = block_combine_256((double)wave_sum(bre[q]), red);
             const double s3 = block_combine_256((double)wave_sum(bim[q]), red);
             if (threadIdx.x == 0) {
-                partial[(((int64_t)0 * k + r) * maxv + g0 + q) * gridDim.x + blockIdx.x] = make_double2(s0, s1);
-                partial[(((int64_t)1 * k + r) * maxv + g0 + q) * gridDim.x + blockIdx.x] = make_double2(s2, s3);
+                partial[(((int64_t)0 * k + r) * maxv + g0 + q) * bk.gx + bk.x] = make_double2(s0, s1);
+                partial[(((int64_t)1 * k + r) * maxv + g0 + q) * bk.gx + bk.x] = make_double2(s2, s3);
             }
         }
     }
@@ -400,13 +534,45 @@ __global__ __launch_bounds__(256) void k_dots2(const cx<T> *__restrict__ V, int 
         const double s1 = block_combine_256(wave_sum(wabr), red);
         const double s2 = block_combine_256(wave_sum(wabi), red);
         if (threadIdx.x == 0) {
-            double *o = wgram + ((int64_t)r * gridDim.x + blockIdx.x) * 4;
+            double *o = wgram + ((int64_t)r * bk.gx + bk.x) * 4;
             o[0] = s0;
             o[1] = s1;
             o[2] = s2;
         }
     }
 }
+
+template <typename T, int VW>
+__global__ __launch_bounds__(256) void k_dots2(const cx<T> *__restrict__ V, int k, int64_t n, int jp1,
+        const cx<T> *__restrict__ w1, const cx<T> *__restrict__ w2, const int *__restrict__ live,
+        double2 *__restrict__ partial, int maxv, int64_t chunk, double *__restrict__ wgram) {
+    dots2_body<T, VW>(V, k, n, jp1, w1, w2, live, partial, maxv, chunk, wgram, blk_here());
+}
+
+template <typename T, int VW>
+struct KDots2 {
+    static constexpr int THREADS = 256;
+    struct Args {
+        const cx<T> *V;
+        int k;
+        int64_t n;
+        int jp1;
+        const cx<T> *w1;
+        const cx<T> *w2;
+        const int *live;
+        double2 *partial;
+        int maxv;
+        int64_t chunk;
+        double *wgram;
+    };
+    __device__ __forceinline__ static void run(const Args &a, const Blk &b) {
+        dots2_body<T, VW>(a.V, a.k, a.n, a.jp1, a.w1, a.w2, a.live, a.partial, a.maxv, a.chunk, a.wgram, b);
+    }
+    static void single(dim3 g, hipStream_t s, const Args &a) {
+        hipLaunchKernelGGL((k_dots2<T, VW>), g, dim3(THREADS), 0, s, a.V, a.k, a.n, a.jp1, a.w1, a.w2, a.live,
+                a.partial, a.maxv, a.chunk, a.wgram);
+    }
+};
 
 // The intra-block coefficient before the update pass (grid (k)): with
 // W1 = w1 - sum h1_i q_i and W2 = w2 - sum h2_i q_i over orthonormal q_i,
@@ -417,10 +583,10 @@ __global__ __launch_bounds__(256) void k_dots2(const cx<T> *__restrict__ V, int 
 // vectors are measured in that pass, not predicted.
 constexpr double BLOCK_GUARD = 1.0e-2;  // default of GmresState::guard
 
-__global__ __launch_bounds__(256) void k_block_alpha(GmresState st, const double *__restrict__ wgram, int nblk, int j,
-                                                     int maxv, int restart) {
+__device__ __forceinline__ void block_alpha_body(GmresState st, const double *__restrict__ wgram, int nblk, int j,
+        int maxv, int restart, const Blk &bk) {
     __shared__ double red[4];
-    const int r = blockIdx.x;
+    const int r = bk.x;
     if (!st.live[r]) return;
     double v[3] = {0, 0, 0};
     for (int b = threadIdx.x; b < nblk; b += 256) {
@@ -447,19 +613,42 @@ __global__ __launch_bounds__(256) void k_block_alpha(GmresState st, const double
     const cx<double> al = guard ? mk<double>(0, 0) : (1.0 / w11) * w12;
     st.alpha[r] = al;
     st.fix[r] = guard ? 1 : 0;
-    if (guard) st.fix[gridDim.x + r] += 1;
+    if (guard) st.fix[bk.gx + r] += 1;
     for (int i = 0; i <= j; ++i) st.coef2[r * maxv + i] = st.coef2[r * maxv + i] - al * st.coef[r * maxv + i];
 }
+
+__global__ __launch_bounds__(256) void k_block_alpha(GmresState st, const double *__restrict__ wgram, int nblk, int j,
+        int maxv, int restart) {
+    block_alpha_body(st, wgram, nblk, j, maxv, restart, blk_here());
+}
+
+struct KBlockAlpha {
+    static constexpr int THREADS = 256;
+    struct Args {
+        GmresState st;
+        const double *wgram;
+        int nblk;
+        int j;
+        int maxv;
+        int restart;
+    };
+    __device__ __forceinline__ static void run(const Args &a, const Blk &b) {
+        block_alpha_body(a.st, a.wgram, a.nblk, a.j, a.maxv, a.restart, b);
+    }
+    static void single(dim3 g, hipStream_t s, const Args &a) {
+        hipLaunchKernelGGL((k_block_alpha), g, dim3(THREADS), 0, s, a.st, a.wgram, a.nblk, a.j, a.maxv, a.restart);
+    }
+};
 
 // Combine both targets' partials (grid (jp1, k)): Hu(i, j) = h1_i = s_i d1_i,
 // Hu(i, j+1) = h2_i (held there until the block finishes), and the update
 // coefficients coef = s_i^2 d1_i, coef2 = s_i^2 d2_i.
-__global__ __launch_bounds__(256) void k_reduce_dots2(const double2 *__restrict__ partial, int nblk, int maxv,
-                                                      int j, GmresState st, int restart) {
+__device__ __forceinline__ void reduce_dots2_body(const double2 *__restrict__ partial, int nblk, int maxv, int j,
+        GmresState st, int restart, const Blk &bk) {
     __shared__ double red[4];
-    const int i = blockIdx.x;
-    const int r = blockIdx.y;
-    const int k = gridDim.y;
+    const int i = bk.x;
+    const int r = bk.y;
+    const int k = bk.gy;
     if (!st.live[r]) return;
     double v[4] = {0, 0, 0, 0};
     for (int t = 0; t < 2; ++t) {
@@ -481,26 +670,47 @@ __global__ __launch_bounds__(256) void k_reduce_dots2(const double2 *__restrict_
     }
 }
 
+__global__ __launch_bounds__(256) void k_reduce_dots2(const double2 *__restrict__ partial, int nblk, int maxv, int j,
+        GmresState st, int restart) {
+    reduce_dots2_body(partial, nblk, maxv, j, st, restart, blk_here());
+}
+
+struct KReduceDots2 {
+    static constexpr int THREADS = 256;
+    struct Args {
+        const double2 *partial;
+        int nblk;
+        int maxv;
+        int j;
+        GmresState st;
+        int restart;
+    };
+    __device__ __forceinline__ static void run(const Args &a, const Blk &b) {
+        reduce_dots2_body(a.partial, a.nblk, a.maxv, a.j, a.st, a.restart, b);
+    }
+    static void single(dim3 g, hipStream_t s, const Args &a) {
+        hipLaunchKernelGGL((k_reduce_dots2), g, dim3(THREADS), 0, s, a.partial, a.nblk, a.maxv, a.j, a.st, a.restart);
+    }
+};
+
 // W1 = w1 - sum coef V_i, W2 = w2 - alpha w1 - sum coef2 V_i in place (coef2
 // already carries -alpha coef), with FP64 partials of |W1|^2, Re/Im W1^H W2
 // and |W2|^2 (gram[(r*nblk + blk)*4 + 0..3]).
 template <typename T, int VW>
-__global__ __launch_bounds__(256) void k_update2(const cx<T> *__restrict__ V, int k, int64_t n, int cnt,
-                                                 cx<T> *__restrict__ w1, cx<T> *__restrict__ w2,
-                                                 const cx<double> *__restrict__ coef,
-                                                 const cx<double> *__restrict__ coef2, int maxv,
-                                                 const int *__restrict__ live, double *__restrict__ gram,
-                                                 int64_t chunk, const cx<double> *__restrict__ alpha) {
+__device__ __forceinline__ void update2_body(const cx<T> *__restrict__ V, int k, int64_t n, int cnt,
+        cx<T> *__restrict__ w1, cx<T> *__restrict__ w2, const cx<double> *__restrict__ coef,
+        const cx<double> *__restrict__ coef2, int maxv, const int *__restrict__ live, double *__restrict__ gram,
+        int64_t chunk, const cx<double> *__restrict__ alpha, const Blk &bk) {
     __shared__ cx<T> cf[64], cf2[64];
     __shared__ double red[4];
-    const int r = blockIdx.y;
+    const int r = bk.y;
     if (!live[r]) return;
     for (int q = threadIdx.x; q < cnt; q += 256) {
         cf[q] = cast<T>(coef[r * maxv + q]);
         cf2[q] = cast<T>(coef2[r * maxv + q]);
     }
     __syncthreads();
-    const int64_t p0 = blockIdx.x * chunk;
+    const int64_t p0 = bk.x * chunk;
     const int64_t p1 = min(n, p0 + chunk);
     cx<T> *ar = w1 + (int64_t)r * n;
     cx<T> *br = w2 + (int64_t)r * n;
@@ -512,10 +722,10 @@ __global__ __launch_bounds__(256) void k_update2(const cx<T> *__restrict__ V, in
         float4 *b4 = reinterpret_cast<float4 *>(br);
         for (int64_t p = (p0 >> 1) + threadIdx.x; p < (p1 >> 1); p += 256) {
             float4 a = a4[p], b = b4[p];
-            b.x -= al.re * a.x - al.im * a.y;
-            b.y -= al.re * a.y + al.im * a.x;
-            b.z -= al.re * a.z - al.im * a.w;
-            b.w -= al.re * a.w + al.im * a.z;
+            b.x = fmaf(al.im, a.y, fmaf(-al.re, a.x, b.x));
+            b.y = fmaf(-al.im, a.x, fmaf(-al.re, a.y, b.y));
+            b.z = fmaf(al.im, a.w, fmaf(-al.re, a.z, b.z));
+            b.w = fmaf(-al.im, a.z, fmaf(-al.re, a.w, b.w));
             int q = 0;
             for (; q + 4 <= cnt; q += 4) {
                 float4 v[4];
@@ -525,27 +735,27 @@ __global__ __launch_bounds__(256) void k_update2(const cx<T> *__restrict__ V, in
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const cx<float> c = cf[q + u], d = cf2[q + u];
-                    a.x -= c.re * v[u].x - c.im * v[u].y;
-                    a.y -= c.re * v[u].y + c.im * v[u].x;
-                    a.z -= c.re * v[u].z - c.im * v[u].w;
-                    a.w -= c.re * v[u].w + c.im * v[u].z;
-                    b.x -= d.re * v[u].x - d.im * v[u].y;
-                    b.y -= d.re * v[u].y + d.im * v[u].x;
-                    b.z -= d.re * v[u].z - d.im * v[u].w;
-                    b.w -= d.re * v[u].w + d.im * v[u].z;
+                    a.x = fmaf(c.im, v[u].y, fmaf(-c.re, v[u].x, a.x));
+                    a.y = fmaf(-c.im, v[u].x, fmaf(-c.re, v[u].y, a.y));
+                    a.z = fmaf(c.im, v[u].w, fmaf(-c.re, v[u].z, a.z));
+                    a.w = fmaf(-c.im, v[u].z, fmaf(-c.re, v[u].w, a.w));
+                    b.x = fmaf(d.im, v[u].y, fmaf(-d.re, v[u].x, b.x));
+                    b.y = fmaf(-d.im, v[u].x, fmaf(-d.re, v[u].y, b.y));
+                    b.z = fmaf(d.im, v[u].w, fmaf(-d.re, v[u].z, b.z));
+                    b.w = fmaf(-d.im, v[u].z, fmaf(-d.re, v[u].w, b.w));
                 }
             }
             for (; q < cnt; ++q) {
                 const float4 v = reinterpret_cast<const float4 *>(V + ((int64_t)q * k + r) * n)[p];
                 const cx<float> c = cf[q], d = cf2[q];
-                a.x -= c.re * v.x - c.im * v.y;
-                a.y -= c.re * v.y + c.im * v.x;
-                a.z -= c.re * v.z - c.im * v.w;
-                a.w -= c.re * v.w + c.im * v.z;
-                b.x -= d.re * v.x - d.im * v.y;
-                b.y -= d.re * v.y + d.im * v.x;
-                b.z -= d.re * v.z - d.im * v.w;
-                b.w -= d.re * v.w + d.im * v.z;
+                a.x = fmaf(c.im, v.y, fmaf(-c.re, v.x, a.x));
+                a.y = fmaf(-c.im, v.x, fmaf(-c.re, v.y, a.y));
+                a.z = fmaf(c.im, v.w, fmaf(-c.re, v.z, a.z));
+                a.w = fmaf(-c.im, v.z, fmaf(-c.re, v.w, a.w));
+                b.x = fmaf(d.im, v.y, fmaf(-d.re, v.x, b.x));
+                b.y = fmaf(-d.im, v.x, fmaf(-d.re, v.y, b.y));
+                b.z = fmaf(d.im, v.w, fmaf(-d.re, v.z, b.z));
+                b.w = fmaf(-d.im, v.z, fmaf(-d.re, v.w, b.w));
             }
             a4[p] = a;
             b4[p] = b;
@@ -576,7 +786,7 @@ __global__ __launch_bounds__(256) void k_update2(const cx<T> *__restrict__ V, in
     const double s2 = block_combine_256((double)wave_sum(g12i), red);
     const double s3 = block_combine_256((double)wave_sum(g22), red);
     if (threadIdx.x == 0) {
-        double *o = gram + ((int64_t)r * gridDim.x + blockIdx.x) * 4;
+        double *o = gram + ((int64_t)r * bk.gx + bk.x) * 4;
         o[0] = s0;
         o[1] = s1;
         o[2] = s2;
@@ -584,12 +794,49 @@ __global__ __launch_bounds__(256) void k_update2(const cx<T> *__restrict__ V, in
     }
 }
 
+template <typename T, int VW>
+__global__ __launch_bounds__(256) void k_update2(const cx<T> *__restrict__ V, int k, int64_t n, int cnt,
+        cx<T> *__restrict__ w1, cx<T> *__restrict__ w2, const cx<double> *__restrict__ coef,
+        const cx<double> *__restrict__ coef2, int maxv, const int *__restrict__ live, double *__restrict__ gram,
+        int64_t chunk, const cx<double> *__restrict__ alpha) {
+    update2_body<T, VW>(V, k, n, cnt, w1, w2, coef, coef2, maxv, live, gram, chunk, alpha, blk_here());
+}
+
+template <typename T, int VW>
+struct KUpdate2 {
+    static constexpr int THREADS = 256;
+    struct Args {
+        const cx<T> *V;
+        int k;
+        int64_t n;
+        int cnt;
+        cx<T> *w1;
+        cx<T> *w2;
+        const cx<double> *coef;
+        const cx<double> *coef2;
+        int maxv;
+        const int *live;
+        double *gram;
+        int64_t chunk;
+        const cx<double> *alpha;
+    };
+    __device__ __forceinline__ static void run(const Args &a, const Blk &b) {
+        update2_body<T, VW>(a.V, a.k, a.n, a.cnt, a.w1, a.w2, a.coef, a.coef2, a.maxv, a.live, a.gram, a.chunk,
+                a.alpha, b);
+    }
+    static void single(dim3 g, hipStream_t s, const Args &a) {
+        hipLaunchKernelGGL((k_update2<T, VW>), g, dim3(THREADS), 0, s, a.V, a.k, a.n, a.cnt, a.w1, a.w2, a.coef,
+                a.coef2, a.maxv, a.live, a.gram, a.chunk, a.alpha);
+    }
+};
+
 // Blocks whose coefficient was not predicted (k_block_alpha's guard):
 // alpha'[r] = W1^H W2 / |W1|^2 from the update pass's Gram partials (grid (k)).
-__global__ __launch_bounds__(256) void k_block_fixcoef(GmresState st, const double *__restrict__ gram, int nblk) {
+__device__ __forceinline__ void block_fixcoef_body(GmresState st, const double *__restrict__ gram, int nblk,
+        const Blk &bk) {
     __shared__ double red[4];
-    const int r = blockIdx.x;
-    const int k = gridDim.x;
+    const int r = bk.x;
+    const int k = bk.gx;
     if (!st.live[r] || !st.fix[r]) return;
     double v[3] = {0, 0, 0};
     for (int b = threadIdx.x; b < nblk; b += 256) {
@@ -602,20 +849,38 @@ __global__ __launch_bounds__(256) void k_block_fixcoef(GmresState st, const doub
     if (threadIdx.x == 0) st.alpha[k + r] = v[0] > 0 ? (1.0 / v[0]) * mk<double>(v[1], v[2]) : mk<double>(0, 0);
 }
 
+__global__ __launch_bounds__(256) void k_block_fixcoef(GmresState st, const double *__restrict__ gram, int nblk) {
+    block_fixcoef_body(st, gram, nblk, blk_here());
+}
+
+struct KBlockFixcoef {
+    static constexpr int THREADS = 256;
+    struct Args {
+        GmresState st;
+        const double *gram;
+        int nblk;
+    };
+    __device__ __forceinline__ static void run(const Args &a, const Blk &b) {
+        block_fixcoef_body(a.st, a.gram, a.nblk, b);
+    }
+    static void single(dim3 g, hipStream_t s, const Args &a) {
+        hipLaunchKernelGGL((k_block_fixcoef), g, dim3(THREADS), 0, s, a.st, a.gram, a.nblk);
+    }
+};
+
 // W2 -= alpha' W1 in place for those blocks, and |W2|^2 measured again into
 // gram[(r*nblk + blk)*4 + 3] (grid (nblk, k)).
 template <typename T>
-__global__ __launch_bounds__(256) void k_block_fixup(GmresState st, int64_t n, const cx<T> *__restrict__ w1,
-                                                     cx<T> *__restrict__ w2, double *__restrict__ gram,
-                                                     int64_t chunk) {
+__device__ __forceinline__ void block_fixup_body(GmresState st, int64_t n, const cx<T> *__restrict__ w1,
+        cx<T> *__restrict__ w2, double *__restrict__ gram, int64_t chunk, const Blk &bk) {
     __shared__ double red[4];
-    const int r = blockIdx.y;
-    const int k = gridDim.y;
+    const int r = bk.y;
+    const int k = bk.gy;
     if (!st.live[r] || !st.fix[r]) return;
     const cx<T> al = cast<T>(st.alpha[k + r]);
     const cx<T> *a = w1 + (int64_t)r * n;
     cx<T> *b = w2 + (int64_t)r * n;
-    const int64_t p0 = blockIdx.x * chunk;
+    const int64_t p0 = bk.x * chunk;
     const int64_t p1 = min(n, p0 + chunk);
     T g22 = 0;
     for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
@@ -624,8 +889,33 @@ __global__ __launch_bounds__(256) void k_block_fixup(GmresState st, int64_t n, c
         g22 += norm2(v);
     }
     const double s = block_combine_256(wave_sum((double)g22), red);
-    if (threadIdx.x == 0) gram[((int64_t)r * gridDim.x + blockIdx.x) * 4 + 3] = s;
+    if (threadIdx.x == 0) gram[((int64_t)r * bk.gx + bk.x) * 4 + 3] = s;
 }
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_block_fixup(GmresState st, int64_t n, const cx<T> *__restrict__ w1,
+        cx<T> *__restrict__ w2, double *__restrict__ gram, int64_t chunk) {
+    block_fixup_body<T>(st, n, w1, w2, gram, chunk, blk_here());
+}
+
+template <typename T>
+struct KBlockFixup {
+    static constexpr int THREADS = 256;
+    struct Args {
+        GmresState st;
+        int64_t n;
+        const cx<T> *w1;
+        cx<T> *w2;
+        double *gram;
+        int64_t chunk;
+    };
+    __device__ __forceinline__ static void run(const Args &a, const Blk &b) {
+        block_fixup_body<T>(a.st, a.n, a.w1, a.w2, a.gram, a.chunk, b);
+    }
+    static void single(dim3 g, hipStream_t s, const Args &a) {
+        hipLaunchKernelGGL((k_block_fixup<T>), g, dim3(THREADS), 0, s, a.st, a.n, a.w1, a.w2, a.gram, a.chunk);
+    }
+};
 
 // Finish the block (grid (k)) from the update pass's Gram partials:
 // r11 = |W1| and r22 = |W2 - alpha W1| as measured (alpha the predicted
@@ -634,10 +924,10 @@ __global__ __launch_bounds__(256) void k_block_fixup(GmresState st, int64_t n, c
 // the scales of slots j+1 and j+2, column j+1 of Hu by linearity, then the
 // Givens updates of columns j and j+1 (the second only if the RHS is still
 // live after the first).
-__global__ __launch_bounds__(256) void k_block_finish(GmresState st, const double *__restrict__ gram, int nblk,
-                                                      int j, int restart) {
+__device__ __forceinline__ void block_finish_body(GmresState st, const double *__restrict__ gram, int nblk, int j,
+        int restart, const Blk &bk) {
     __shared__ double red[4];
-    const int r = blockIdx.x;
+    const int r = bk.x;
     if (!st.live[r]) return;
     double v[2] = {0, 0};
     for (int b = threadIdx.x; b < nblk; b += 256) {
@@ -648,10 +938,10 @@ __global__ __launch_bounds__(256) void k_block_finish(GmresState st, const doubl
     for (int q = 0; q < 2; ++q) v[q] = block_sum_256(v[q], red);
     if (threadIdx.x != 0) return;
     const double r11 = sqrt(v[0]), r22 = sqrt(v[1]);
-    const cx<double> al = st.fix[r] ? st.alpha[r] + st.alpha[gridDim.x + r] : st.alpha[r];
+    const cx<double> al = st.fix[r] ? st.alpha[r] + st.alpha[bk.gx + r] : st.alpha[r];
     const cx<double> r12 = r11 * al;
-    st.scales[(int64_t)(j + 1) * gridDim.x + r] = r11 > 0 ? 1.0 / r11 : 0.0;
-    st.scales[(int64_t)(j + 2) * gridDim.x + r] = r22 > 0 ? 1.0 / r22 : 0.0;
+    st.scales[(int64_t)(j + 1) * bk.gx + r] = r11 > 0 ? 1.0 / r11 : 0.0;
+    st.scales[(int64_t)(j + 2) * bk.gx + r] = r22 > 0 ? 1.0 / r22 : 0.0;
     cx<double> *hu = st.Hu + (int64_t)r * (restart + 1) * restart;
     cx<double> *h = st.H + (int64_t)r * (restart + 1) * restart;
     hu[(j + 1) * restart + j] = mk<double>(r11, 0);
@@ -674,6 +964,28 @@ __global__ __launch_bounds__(256) void k_block_finish(GmresState st, const doubl
     givens_column(st, r, j + 1, hu[(j + 2) * restart + j + 1].re, restart);
 }
 
+__global__ __launch_bounds__(256) void k_block_finish(GmresState st, const double *__restrict__ gram, int nblk, int j,
+        int restart) {
+    block_finish_body(st, gram, nblk, j, restart, blk_here());
+}
+
+struct KBlockFinish {
+    static constexpr int THREADS = 256;
+    struct Args {
+        GmresState st;
+        const double *gram;
+        int nblk;
+        int j;
+        int restart;
+    };
+    __device__ __forceinline__ static void run(const Args &a, const Blk &b) {
+        block_finish_body(a.st, a.gram, a.nblk, a.j, a.restart, b);
+    }
+    static void single(dim3 g, hipStream_t s, const Args &a) {
+        hipLaunchKernelGGL((k_block_finish), g, dim3(THREADS), 0, s, a.st, a.gram, a.nblk, a.j, a.restart);
+    }
+};
+
 // Back substitution on the rotated Hessenberg matrix; coef[r][i] = -y_i s_i so
 // that k_update adds sum_i y_i v_i to the iterate.
 // blocks_z: the update runs over the stored preconditioned slots Z_i instead
@@ -684,8 +996,9 @@ __global__ __launch_bounds__(256) void k_block_finish(GmresState st, const doubl
 // (the linearity the block already uses for column i of H).  The
 // coefficients of sum_i y_i M q_i on the Z_i follow from the highest slot
 // down.
-__global__ void k_trisolve(GmresState st, int maxv, int restart, const int *__restrict__ member, int blocks_z) {
-    const int r = blockIdx.x;
+__device__ __forceinline__ void trisolve_body(GmresState st, int maxv, int restart, const int *__restrict__ member,
+        int blocks_z, const Blk &bk) {
+    const int r = bk.x;
     if (threadIdx.x != 0 || !member[r]) return;
     const int m = st.steps[r];
     const cx<double> *col = st.H + (int64_t)r * (restart + 1) * restart;
@@ -698,7 +1011,7 @@ __global__ void k_trisolve(GmresState st, int maxv, int restart, const int *__re
         y[i] = (cabs(d) > 0) ? cdiv(acc, d) : mk<double>(0, 0);
     }
     if (!blocks_z) {
-        for (int i = 0; i < m; ++i) st.coef[r * maxv + i] = (-st.scales[(int64_t)i * gridDim.x + r]) * y[i];
+        for (int i = 0; i < m; ++i) st.coef[r * maxv + i] = (-st.scales[(int64_t)i * bk.gx + r]) * y[i];
         return;
     }
     const cx<double> *hu = st.Hu + (int64_t)r * (restart + 1) * restart;
@@ -709,22 +1022,43 @@ __global__ void k_trisolve(GmresState st, int maxv, int restart, const int *__re
             for (int l = 0; l < i; ++l) y[l] = y[l] - c * hu[l * restart + i - 1];
             st.coef[r * maxv + i] = -1.0 * c;
         } else {
-            st.coef[r * maxv + i] = (-st.scales[(int64_t)i * gridDim.x + r]) * y[i];
+            st.coef[r * maxv + i] = (-st.scales[(int64_t)i * bk.gx + r]) * y[i];
         }
     }
 }
+
+__global__ __launch_bounds__(64) void k_trisolve(GmresState st, int maxv, int restart, const int *__restrict__ member,
+        int blocks_z) {
+    trisolve_body(st, maxv, restart, member, blocks_z, blk_here());
+}
+
+struct KTrisolve {
+    static constexpr int THREADS = 64;
+    struct Args {
+        GmresState st;
+        int maxv;
+        int restart;
+        const int *member;
+        int blocks_z;
+    };
+    __device__ __forceinline__ static void run(const Args &a, const Blk &b) {
+        trisolve_body(a.st, a.maxv, a.restart, a.member, a.blocks_z, b);
+    }
+    static void single(dim3 g, hipStream_t s, const Args &a) {
+        hipLaunchKernelGGL((k_trisolve), g, dim3(THREADS), 0, s, a.st, a.maxv, a.restart, a.member, a.blocks_z);
+    }
+};
 
 // ---------------------------------------------------------------- elementwise
 // out[r] = cast(rs * b[r]) (scaled right-hand side) with FP64 partials of
 // |rs b|^2 and |b|^2 (partial[(r*nblk+blk)*2 + 0/1]).
 template <typename T>
-__global__ __launch_bounds__(256) void k_scale_rhs(const cx<double> *__restrict__ b, int64_t ldb,
-                                                   const double *__restrict__ rs, cx<T> *__restrict__ out,
-                                                   int64_t ldo, int64_t n, double *__restrict__ partial,
-                                                   int64_t chunk) {
+__device__ __forceinline__ void scale_rhs_body(const cx<double> *__restrict__ b, int64_t ldb,
+        const double *__restrict__ rs, cx<T> *__restrict__ out, int64_t ldo, int64_t n, double *__restrict__ partial,
+        int64_t chunk, const Blk &bk) {
     __shared__ double red[4];
-    const int r = blockIdx.y;
-    const int64_t p0 = blockIdx.x * chunk;
+    const int r = bk.y;
+    const int64_t p0 = bk.x * chunk;
     const int64_t p1 = min(n, p0 + chunk);
     double s1 = 0, s2 = 0;
     for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
@@ -737,31 +1071,109 @@ __global__ __launch_bounds__(256) void k_scale_rhs(const cx<double> *__restrict_
     s1 = block_sum_256(s1, red);
     s2 = block_sum_256(s2, red);
     if (threadIdx.x == 0) {
-        partial[((int64_t)r * gridDim.x + blockIdx.x) * 2 + 0] = s1;
-        partial[((int64_t)r * gridDim.x + blockIdx.x) * 2 + 1] = s2;
+        partial[((int64_t)r * bk.gx + bk.x) * 2 + 0] = s1;
+        partial[((int64_t)r * bk.gx + bk.x) * 2 + 1] = s2;
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_scale_rhs(const cx<double> *__restrict__ b, int64_t ldb,
+        const double *__restrict__ rs, cx<T> *__restrict__ out, int64_t ldo, int64_t n, double *__restrict__ partial,
+        int64_t chunk) {
+    scale_rhs_body<T>(b, ldb, rs, out, ldo, n, partial, chunk, blk_here());
+}
+
+template <typename T>
+struct KScaleRhs {
+    static constexpr int THREADS = 256;
+    struct Args {
+        const cx<double> *b;
+        int64_t ldb;
+        const double *rs;
+        cx<T> *out;
+        int64_t ldo;
+        int64_t n;
+        double *partial;
+        int64_t chunk;
+    };
+    __device__ __forceinline__ static void run(const Args &a, const Blk &b) {
+        scale_rhs_body<T>(a.b, a.ldb, a.rs, a.out, a.ldo, a.n, a.partial, a.chunk, b);
+    }
+    static void single(dim3 g, hipStream_t s, const Args &a) {
+        hipLaunchKernelGGL((k_scale_rhs<T>), g, dim3(THREADS), 0, s, a.b, a.ldb, a.rs, a.out, a.ldo, a.n, a.partial,
+                a.chunk);
+    }
+};
+
 // x[r] (+)= m * cast<double>(y[r]): apply the Jacobi factor to the iterate.
 template <typename T>
-__global__ __launch_bounds__(256) void k_precond_out(const cx<T> *__restrict__ y, int64_t ldy,
-                                                     const cx<double> *__restrict__ m, cx<double> *__restrict__ x,
-                                                     int64_t ldx, int64_t n, const int *__restrict__ member,
-                                                     int accumulate) {
-    const int r = blockIdx.y;
+__device__ __forceinline__ void precond_out_body(const cx<T> *__restrict__ y, int64_t ldy,
+        const cx<double> *__restrict__ m, cx<double> *__restrict__ x, int64_t ldx, int64_t n,
+        const int *__restrict__ member, int accumulate, const Blk &bk) {
+    const int r = bk.y;
     if (member && !member[r]) return;
-    for (int64_t p = blockIdx.x * 256 + threadIdx.x; p < n; p += (int64_t)gridDim.x * 256) {
+    for (int64_t p = bk.x * 256 + threadIdx.x; p < n; p += (int64_t)bk.gx * 256) {
         const cx<double> v = m[p] * cast<double>(y[r * ldy + p]);
         x[r * ldx + p] = accumulate ? x[r * ldx + p] + v : v;
     }
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_zero(cx<T> *__restrict__ x, int64_t ldx, int64_t n) {
-    const int r = blockIdx.y;
-    for (int64_t p = blockIdx.x * 256 + threadIdx.x; p < n; p += (int64_t)gridDim.x * 256)
+__global__ __launch_bounds__(256) void k_precond_out(const cx<T> *__restrict__ y, int64_t ldy,
+        const cx<double> *__restrict__ m, cx<double> *__restrict__ x, int64_t ldx, int64_t n,
+        const int *__restrict__ member, int accumulate) {
+    precond_out_body<T>(y, ldy, m, x, ldx, n, member, accumulate, blk_here());
+}
+
+template <typename T>
+struct KPrecondOut {
+    static constexpr int THREADS = 256;
+    struct Args {
+        const cx<T> *y;
+        int64_t ldy;
+        const cx<double> *m;
+        cx<double> *x;
+        int64_t ldx;
+        int64_t n;
+        const int *member;
+        int accumulate;
+    };
+    __device__ __forceinline__ static void run(const Args &a, const Blk &b) {
+        precond_out_body<T>(a.y, a.ldy, a.m, a.x, a.ldx, a.n, a.member, a.accumulate, b);
+    }
+    static void single(dim3 g, hipStream_t s, const Args &a) {
+        hipLaunchKernelGGL((k_precond_out<T>), g, dim3(THREADS), 0, s, a.y, a.ldy, a.m, a.x, a.ldx, a.n, a.member,
+                a.accumulate);
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ void zero_body(cx<T> *__restrict__ x, int64_t ldx, int64_t n, const Blk &bk) {
+    const int r = bk.y;
+    for (int64_t p = bk.x * 256 + threadIdx.x; p < n; p += (int64_t)bk.gx * 256)
         x[r * ldx + p] = mk<T>(0, 0);
 }
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_zero(cx<T> *__restrict__ x, int64_t ldx, int64_t n) {
+    zero_body<T>(x, ldx, n, blk_here());
+}
+
+template <typename T>
+struct KZero {
+    static constexpr int THREADS = 256;
+    struct Args {
+        cx<T> *x;
+        int64_t ldx;
+        int64_t n;
+    };
+    __device__ __forceinline__ static void run(const Args &a, const Blk &b) {
+        zero_body<T>(a.x, a.ldx, a.n, b);
+    }
+    static void single(dim3 g, hipStream_t s, const Args &a) {
+        hipLaunchKernelGGL((k_zero<T>), g, dim3(THREADS), 0, s, a.x, a.ldx, a.n);
+    }
+};
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_fill(cx<T> *__restrict__ x, int64_t ldx, int64_t n, cx<T> value) {
@@ -772,11 +1184,11 @@ __global__ __launch_bounds__(256) void k_fill(cx<T> *__restrict__ x, int64_t ldx
 // Sum (or max, where bit q of max_mask is set) of nblk partials for each of
 // `width` values per RHS: out[r*width + q] = reduce_b partial[(r*nblk + b)*width + q].
 // Grid (width, k).
-__global__ __launch_bounds__(256) void k_reduce_rows(const double *__restrict__ partial, int nblk, int width,
-                                                     int max_mask, double *__restrict__ out) {
+__device__ __forceinline__ void reduce_rows_body(const double *__restrict__ partial, int nblk, int width,
+        int max_mask, double *__restrict__ out, const Blk &bk) {
     __shared__ double red[4];
-    const int q = blockIdx.x;
-    const int r = blockIdx.y;
+    const int q = bk.x;
+    const int r = bk.y;
     const bool is_max = (max_mask >> q) & 1;
     double acc = 0;
     for (int b = threadIdx.x; b < nblk; b += 256) {
@@ -793,6 +1205,28 @@ __global__ __launch_bounds__(256) void k_reduce_rows(const double *__restrict__ 
         if (threadIdx.x == 0) out[r * width + q] = acc;
     }
 }
+
+__global__ __launch_bounds__(256) void k_reduce_rows(const double *__restrict__ partial, int nblk, int width,
+        int max_mask, double *__restrict__ out) {
+    reduce_rows_body(partial, nblk, width, max_mask, out, blk_here());
+}
+
+struct KReduceRows {
+    static constexpr int THREADS = 256;
+    struct Args {
+        const double *partial;
+        int nblk;
+        int width;
+        int max_mask;
+        double *out;
+    };
+    __device__ __forceinline__ static void run(const Args &a, const Blk &b) {
+        reduce_rows_body(a.partial, a.nblk, a.width, a.max_mask, a.out, b);
+    }
+    static void single(dim3 g, hipStream_t s, const Args &a) {
+        hipLaunchKernelGGL((k_reduce_rows), g, dim3(THREADS), 0, s, a.partial, a.nblk, a.width, a.max_mask, a.out);
+    }
+};
 
 // ------------------------------------------------------------ sources / sampling
 // b[node, r] = sum over the node's entries of value * w[column, r]: the

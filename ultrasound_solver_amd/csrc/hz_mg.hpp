@@ -48,17 +48,44 @@ __global__ __launch_bounds__(256) void k_level_dinv(OpView<T> op, cx<T> *__restr
 
 // First sweep from a zero guess: xo = w * dinv * b.
 template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_level_jacobi0(const cx<T> *__restrict__ b, cx<T> *__restrict__ xo,
-                                                       const cx<T> *__restrict__ dinv, T w, int64_t n,
-                                                       const int *__restrict__ live, const T *__restrict__ bdiv) {
-    const int r = blockIdx.y;
+__device__ __forceinline__ void level_jacobi0_body(const cx<T> *__restrict__ b, cx<T> *__restrict__ xo,
+        const cx<T> *__restrict__ dinv, T w, int64_t n, const int *__restrict__ live, const T *__restrict__ bdiv,
+        const Blk &bk) {
+    const int r = bk.y;
     if (live && !live[r]) return;
-    for (int64_t p = blockIdx.x * 256 + threadIdx.x; p < n; p += (int64_t)gridDim.x * 256) {
+    for (int64_t p = bk.x * 256 + threadIdx.x; p < n; p += (int64_t)bk.gx * 256) {
         const cx<T> dv = (MODE == MODE_AH) ? conj(dinv[p]) : dinv[p];
         const T ws = bdiv ? w / bdiv[p] : w;
         xo[r * n + p] = ws * (dv * b[r * n + p]);
     }
 }
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_level_jacobi0(const cx<T> *__restrict__ b, cx<T> *__restrict__ xo,
+        const cx<T> *__restrict__ dinv, T w, int64_t n, const int *__restrict__ live, const T *__restrict__ bdiv) {
+    level_jacobi0_body<T, MODE>(b, xo, dinv, w, n, live, bdiv, blk_here());
+}
+
+template <typename T, int MODE>
+struct KLevelJacobi0 {
+    static constexpr int THREADS = 256;
+    struct Args {
+        const cx<T> *b;
+        cx<T> *xo;
+        const cx<T> *dinv;
+        T w;
+        int64_t n;
+        const int *live;
+        const T *bdiv;
+    };
+    __device__ __forceinline__ static void run(const Args &a, const Blk &b) {
+        level_jacobi0_body<T, MODE>(a.b, a.xo, a.dinv, a.w, a.n, a.live, a.bdiv, b);
+    }
+    static void single(dim3 g, hipStream_t s, const Args &a) {
+        hipLaunchKernelGGL((k_level_jacobi0<T, MODE>), g, dim3(THREADS), 0, s, a.b, a.xo, a.dinv, a.w, a.n, a.live,
+                a.bdiv);
+    }
+};
 
 constexpr int COARSE_MAX_NODES = 1024;
 
@@ -66,12 +93,11 @@ constexpr int COARSE_MAX_NODES = 1024;
 // workgroup per right-hand side keeps the iterate in LDS (ping-pong) and
 // each thread holds the coefficients of its <= 4 nodes in registers.
 template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_coarse_jacobi(OpView<T> op, const cx<T> *__restrict__ b,
-                                                       cx<T> *__restrict__ x, const cx<T> *__restrict__ dinv, T w,
-                                                       int sweeps, const int *__restrict__ live) {
+__device__ __forceinline__ void coarse_jacobi_body(const OpView<T> &op, const cx<T> *__restrict__ b, cx<T> *__restrict__ x,
+        const cx<T> *__restrict__ dinv, T w, int sweeps, const int *__restrict__ live, const Blk &bk) {
     constexpr int PER = COARSE_MAX_NODES / 256;
     __shared__ cx<T> xs[2][COARSE_MAX_NODES];
-    const int r = blockIdx.x;
+    const int r = bk.x;
     if (live && !live[r]) return;
     const int n = (int)op.n;
     cx<T> cs[PER], c[PER][6], dv[PER], bv[PER];
@@ -100,8 +126,8 @@ __global__ __launch_bounds__(256) void k_coarse_jacobi(OpView<T> op, const cx<T>
                 const cx<T> x0 = xs[cur][p];
                 cx<T> acc = cs[q] * x0;
 #pragma unroll
-                for (int e = 0; e < 6; ++e) acc = acc + c[q][e] * xs[cur][p + off[q][e]];
-                xs[cur ^ 1][p] = x0 + w * (dv[q] * (bv[q] - acc));
+                for (int e = 0; e < 6; ++e) acc = cfma(c[q][e], xs[cur][p + off[q][e]], acc);
+                xs[cur ^ 1][p] = sfma(w, dv[q] * (bv[q] - acc), x0);
             }
         }
         cur ^= 1;
@@ -114,6 +140,33 @@ __global__ __launch_bounds__(256) void k_coarse_jacobi(OpView<T> op, const cx<T>
     }
 }
 
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_coarse_jacobi(OpView<T> op, const cx<T> *__restrict__ b,
+        cx<T> *__restrict__ x, const cx<T> *__restrict__ dinv, T w, int sweeps, const int *__restrict__ live) {
+    coarse_jacobi_body<T, MODE>(op, b, x, dinv, w, sweeps, live, blk_here());
+}
+
+template <typename T, int MODE>
+struct KCoarseJacobi {
+    static constexpr int THREADS = 256;
+    struct Args {
+        OpView<T> op;
+        const cx<T> *b;
+        cx<T> *x;
+        const cx<T> *dinv;
+        T w;
+        int sweeps;
+        const int *live;
+    };
+    __device__ __forceinline__ static void run(const Args &a, const Blk &b) {
+        coarse_jacobi_body<T, MODE>(a.op, a.b, a.x, a.dinv, a.w, a.sweeps, a.live, b);
+    }
+    static void single(dim3 g, hipStream_t s, const Args &a) {
+        hipLaunchKernelGGL((k_coarse_jacobi<T, MODE>), g, dim3(THREADS), 0, s, a.op, a.b, a.x, a.dinv, a.w, a.sweeps,
+                a.live);
+    }
+};
+
 // Workgroup -> (256 consecutive nodes of one xy-plane, z-plane, slot) of an
 // nx x ny x nz grid with k slots.  Flattening the plane leaves no idle lanes
 // on awkward widths (the coarse levels are 65, 33, 17 ... wide).  The slot is
@@ -124,8 +177,8 @@ struct PlaneTile {
     int i, j, k, r;
     bool in_grid;
 };
-__device__ __forceinline__ PlaneTile plane_tile(int nx, int ny, int nz, int nslots, int nblocks) {
-    const int tile = xcd_tile(blockIdx.x, nblocks);
+__device__ __forceinline__ PlaneTile plane_tile(int bid, int nx, int ny, int nz, int nslots, int nblocks) {
+    const int tile = xcd_tile(bid, nblocks);
     const int nxy = nx * ny;
     const int per_plane = (nxy + 255) >> 8;
     const int per_slot = per_plane * nz;
@@ -165,10 +218,10 @@ __device__ __forceinline__ PlaneTile plane_tile(int nx, int ny, int nz, int nslo
 constexpr int RZ_CHUNK = 8;
 
 template <typename T, int PAIR>
-__global__ __launch_bounds__(256) void k_restrict_z(const cx<T> *__restrict__ rf, int nxf, int nyf, int nzf,
-                                                    cx<T> *__restrict__ bc, int nxc, int nyc, int nzc, int k,
-                                                    const int *__restrict__ live, int soft, int nblocks) {
-    const int tile = xcd_tile(blockIdx.x, nblocks);
+__device__ __forceinline__ void restrict_z_body(const cx<T> *__restrict__ rf, int nxf, int nyf, int nzf,
+        cx<T> *__restrict__ bc, int nxc, int nyc, int nzc, int k, const int *__restrict__ live, int soft, int nblocks,
+        const Blk &bk) {
+    const int tile = xcd_tile(bk.x, nblocks);
     const int nxy = nxc * nyc;
     const int pieces = (nxy + 255) >> 8;
     const int chunks = (nzc + RZ_CHUNK - 1) / RZ_CHUNK;
@@ -234,7 +287,7 @@ __global__ __launch_bounds__(256) void k_restrict_z(const cx<T> *__restrict__ rf
 #pragma unroll
         for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
-            for (int dx = 0; dx < 3; ++dx) acc = acc + wt[dy][dx] * v[dy][dx];
+            for (int dx = 0; dx < 3; ++dx) acc = sfma(wt[dy][dx], v[dy][dx], acc);
         return acc;
     };
     const int kc0 = chunk * RZ_CHUNK, kc1 = min(nzc, kc0 + RZ_CHUNK);
@@ -279,6 +332,39 @@ __global__ __launch_bounds__(256) void k_restrict_z(const cx<T> *__restrict__ rf
     while (ka < kc1) flush();
 }
 
+template <typename T, int PAIR>
+__global__ __launch_bounds__(256) void k_restrict_z(const cx<T> *__restrict__ rf, int nxf, int nyf, int nzf,
+        cx<T> *__restrict__ bc, int nxc, int nyc, int nzc, int k, const int *__restrict__ live, int soft, int nblocks) {
+    restrict_z_body<T, PAIR>(rf, nxf, nyf, nzf, bc, nxc, nyc, nzc, k, live, soft, nblocks, blk_here());
+}
+
+template <typename T, int PAIR>
+struct KRestrictZ {
+    static constexpr int THREADS = 256;
+    struct Args {
+        const cx<T> *rf;
+        int nxf;
+        int nyf;
+        int nzf;
+        cx<T> *bc;
+        int nxc;
+        int nyc;
+        int nzc;
+        int k;
+        const int *live;
+        int soft;
+        int nblocks;
+    };
+    __device__ __forceinline__ static void run(const Args &a, const Blk &b) {
+        restrict_z_body<T, PAIR>(a.rf, a.nxf, a.nyf, a.nzf, a.bc, a.nxc, a.nyc, a.nzc, a.k, a.live, a.soft, a.nblocks,
+                b);
+    }
+    static void single(dim3 g, hipStream_t s, const Args &a) {
+        hipLaunchKernelGGL((k_restrict_z<T, PAIR>), g, dim3(THREADS), 0, s, a.rf, a.nxf, a.nyf, a.nzf, a.bc, a.nxc,
+                a.nyc, a.nzc, a.k, a.live, a.soft, a.nblocks);
+    }
+};
+
 // Fine nodes of coarse cell c along one axis (fine size n, coarse size m):
 // cell c < m-1 spans fine [fine_of(c), fine_of(c+1)) (one or two nodes), the
 // last cell is the last fine node.  The first node of a cell is the coarse
@@ -312,12 +398,11 @@ __device__ __forceinline__ CellAxis cell_axis(int c, int n, int m) {
 // cell row, 2i and 2i+1, are read (b, or the iterate) and written with one
 // 16-byte access each.
 template <typename T, int PAIR>
-__global__ __launch_bounds__(256) void k_prolong(const cx<T> *__restrict__ xc, int nxc, int nyc, int nzc,
-                                                 cx<T> *__restrict__ xf, int nxf, int nyf, int nzf, int k, int rs,
-                                                 const int *__restrict__ live, int soft,
-                                                 const cx<T> *__restrict__ b, const cx<T> *__restrict__ dinv, T w,
-                                                 const T *__restrict__ bdiv, int conj_d, int nblocks) {
-    const PlaneTile t = plane_tile(nxc, nyc, nzc, (k + rs - 1) / rs, nblocks);
+__device__ __forceinline__ void prolong_body(const cx<T> *__restrict__ xc, int nxc, int nyc, int nzc,
+        cx<T> *__restrict__ xf, int nxf, int nyf, int nzf, int k, int rs, const int *__restrict__ live, int soft,
+        const cx<T> *__restrict__ b, const cx<T> *__restrict__ dinv, T w, const T *__restrict__ bdiv, int conj_d,
+        int nblocks, const Blk &bk) {
+    const PlaneTile t = plane_tile(bk.x, nxc, nyc, nzc, (k + rs - 1) / rs, nblocks);
     if (!t.in_grid) return;
     const int r0 = t.r * rs, r1 = min(k, r0 + rs);
     const int64_t nf = (int64_t)nxf * nyf * nzf;
@@ -379,7 +464,7 @@ __global__ __launch_bounds__(256) void k_prolong(const cx<T> *__restrict__ xc, i
 #pragma unroll
                         for (int a = 0; a < 2; ++a) {
                             const T wt = wgt(dz, c) * cw[2][c] * wgt(dy, bq) * cw[1][bq] * wgt(dx, a) * cw[0][a];
-                            acc = acc + wt * e[c][bq][a];
+                            acc = sfma(wt, e[c][bq][a], acc);
                         }
             }
             return acc;
@@ -414,15 +499,77 @@ __global__ __launch_bounds__(256) void k_prolong(const cx<T> *__restrict__ xc, i
     }
 }
 
+template <typename T, int PAIR>
+__global__ __launch_bounds__(256) void k_prolong(const cx<T> *__restrict__ xc, int nxc, int nyc, int nzc,
+        cx<T> *__restrict__ xf, int nxf, int nyf, int nzf, int k, int rs, const int *__restrict__ live, int soft,
+        const cx<T> *__restrict__ b, const cx<T> *__restrict__ dinv, T w, const T *__restrict__ bdiv, int conj_d,
+        int nblocks) {
+    prolong_body<T, PAIR>(xc, nxc, nyc, nzc, xf, nxf, nyf, nzf, k, rs, live, soft, b, dinv, w, bdiv, conj_d, nblocks, blk_here());
+}
+
+template <typename T, int PAIR>
+struct KProlong {
+    static constexpr int THREADS = 256;
+    struct Args {
+        const cx<T> *xc;
+        int nxc;
+        int nyc;
+        int nzc;
+        cx<T> *xf;
+        int nxf;
+        int nyf;
+        int nzf;
+        int k;
+        int rs;
+        const int *live;
+        int soft;
+        const cx<T> *b;
+        const cx<T> *dinv;
+        T w;
+        const T *bdiv;
+        int conj_d;
+        int nblocks;
+    };
+    __device__ __forceinline__ static void run(const Args &a, const Blk &b) {
+        prolong_body<T, PAIR>(a.xc, a.nxc, a.nyc, a.nzc, a.xf, a.nxf, a.nyf, a.nzf, a.k, a.rs, a.live, a.soft, a.b,
+                a.dinv, a.w, a.bdiv, a.conj_d, a.nblocks, b);
+    }
+    static void single(dim3 g, hipStream_t s, const Args &a) {
+        hipLaunchKernelGGL((k_prolong<T, PAIR>), g, dim3(THREADS), 0, s, a.xc, a.nxc, a.nyc, a.nzc, a.xf, a.nxf,
+                a.nyf, a.nzf, a.k, a.rs, a.live, a.soft, a.b, a.dinv, a.w, a.bdiv, a.conj_d, a.nblocks);
+    }
+};
+
 // x64 += cast<double>(z): apply a complex64 correction to the FP64 iterate.
-__global__ __launch_bounds__(256) void k_accumulate(const cx<float> *__restrict__ z, int64_t n,
-                                                    cx<double> *__restrict__ x, int64_t ldx,
-                                                    const int *__restrict__ member) {
-    const int r = blockIdx.y;
+__device__ __forceinline__ void accumulate_body(const cx<float> *__restrict__ z, int64_t n,
+        cx<double> *__restrict__ x, int64_t ldx, const int *__restrict__ member, const Blk &bk) {
+    const int r = bk.y;
     if (member && !member[r]) return;
-    for (int64_t p = blockIdx.x * 256 + threadIdx.x; p < n; p += (int64_t)gridDim.x * 256)
+    for (int64_t p = bk.x * 256 + threadIdx.x; p < n; p += (int64_t)bk.gx * 256)
         x[r * ldx + p] = x[r * ldx + p] + cast<double>(z[r * n + p]);
 }
+
+__global__ __launch_bounds__(256) void k_accumulate(const cx<float> *__restrict__ z, int64_t n,
+        cx<double> *__restrict__ x, int64_t ldx, const int *__restrict__ member) {
+    accumulate_body(z, n, x, ldx, member, blk_here());
+}
+
+struct KAccumulate {
+    static constexpr int THREADS = 256;
+    struct Args {
+        const cx<float> *z;
+        int64_t n;
+        cx<double> *x;
+        int64_t ldx;
+        const int *member;
+    };
+    __device__ __forceinline__ static void run(const Args &a, const Blk &b) {
+        accumulate_body(a.z, a.n, a.x, a.ldx, a.member, b);
+    }
+    static void single(dim3 g, hipStream_t s, const Args &a) {
+        hipLaunchKernelGGL((k_accumulate), g, dim3(THREADS), 0, s, a.z, a.n, a.x, a.ldx, a.member);
+    }
+};
 
 // ---------------------------------------------------------------- hierarchy
 // Coarse-level material fields derived on the device.  Even-node coarsening

@@ -108,11 +108,10 @@ __device__ __forceinline__ void store_out(cx<float> *dst, cx<float> v) {
 }
 
 template <int MODE, int EPI, int ZB, int TY, int RP, int NT, int FLAT>
-__global__ __launch_bounds__(64 * TY) void k_slab(OpView<float> op, TileMap tm, const cx<float> *__restrict__ xin,
-                                              int64_t ldx, cx<float> *__restrict__ yout, int64_t ldy, int nrhs,
-                                              const int *__restrict__ live, const double *__restrict__ scale,
-                                              const cx<float> *__restrict__ b, const cx<float> *__restrict__ dinv,
-                                              float w, const float *__restrict__ bdiv) {
+__device__ __forceinline__ void slab_body(const OpView<float> &op, const TileMap &tm, const cx<float> *__restrict__ xin,
+        int64_t ldx, cx<float> *__restrict__ yout, int64_t ldy, int nrhs, const int *__restrict__ live,
+        const double *__restrict__ scale, const cx<float> *__restrict__ b, const cx<float> *__restrict__ dinv,
+        float w, const float *__restrict__ bdiv, const Blk &bk) {
     __shared__ int s_act[APPLY_MAX_RHS];
     __shared__ float s_sc[APPLY_MAX_RHS];
     __shared__ int s_na;
@@ -127,8 +126,8 @@ __global__ __launch_bounds__(64 * TY) void k_slab(OpView<float> op, TileMap tm, 
     __shared__ cx<float> s_pl[2][RP][ZB][NT_];
     // nblocks is a multiple of 8, so a workgroup's XCD (its index modulo 8)
     // is the same in every column group.
-    const int part = (int)blockIdx.x / tm.nblocks;
-    const int tile = xcd_tile((int)blockIdx.x - part * tm.nblocks, tm.nblocks);
+    const int part = (int)bk.x / tm.nblocks;
+    const int tile = xcd_tile((int)bk.x - part * tm.nblocks, tm.nblocks);
     if (tile >= tm.ntiles) return;  // whole workgroup
     stage_live_columns<float>(nrhs, live, scale, s_act, s_sc, &s_na);
     int tx, ty, tz;
@@ -259,22 +258,22 @@ __global__ __launch_bounds__(64 * TY) void k_slab(OpView<float> op, TileMap tm, 
                 if (lane == 0) xl = xe[u][q];
                 if (lane == 63) xn = xe[u][q];
                 cx<float> acc = cs[q] * xc;
-                acc = acc + c[q][0] * xl;
-                acc = acc + c[q][1] * xn;
-                acc = acc + c[q][2] * ym[u][q];
-                acc = acc + c[q][3] * yp[u][q];
-                acc = acc + c[q][4] * v[u][q];
-                acc = acc + c[q][5] * v[u][q + 2];
+                acc = cfma(c[q][0], xl, acc);
+                acc = cfma(c[q][1], xn, acc);
+                acc = cfma(c[q][2], ym[u][q], acc);
+                acc = cfma(c[q][3], yp[u][q], acc);
+                acc = cfma(c[q][4], v[u][q], acc);
+                acc = cfma(c[q][5], v[u][q + 2], acc);
                 if (valid && q < nzl) {
                     const int64_t p = p0 + (int64_t)q * sxy;
                     cx<float> out;
                     if (EPI == SLAB_APPLY) {
                         out = s_sc[t + u] * acc;
                     } else if (EPI == SLAB_RESID0) {
-                        out = bs[q] * xc - acc;
+                        out = sfms(bs[q], xc, acc);
                     } else {
-                        const cx<float> e = bs[q] * bb[u][q] - acc;
-                        out = (EPI == SLAB_SMOOTH) ? xc + w * (dv[q] * e) : e;
+                        const cx<float> e = sfms(bs[q], bb[u][q], acc);
+                        out = (EPI == SLAB_SMOOTH) ? sfma(w, dv[q] * e, xc) : e;
                     }
                     store_out<NT>(yout + r * ldy + p, out);
                 }
@@ -283,6 +282,42 @@ __global__ __launch_bounds__(64 * TY) void k_slab(OpView<float> op, TileMap tm, 
     }
 }
 
+template <int MODE, int EPI, int ZB, int TY, int RP, int NT, int FLAT>
+__global__ __launch_bounds__(64 * TY) void k_slab(OpView<float> op, TileMap tm, const cx<float> *__restrict__ xin,
+        int64_t ldx, cx<float> *__restrict__ yout, int64_t ldy, int nrhs, const int *__restrict__ live,
+        const double *__restrict__ scale, const cx<float> *__restrict__ b, const cx<float> *__restrict__ dinv,
+        float w, const float *__restrict__ bdiv) {
+    slab_body<MODE, EPI, ZB, TY, RP, NT, FLAT>(op, tm, xin, ldx, yout, ldy, nrhs, live, scale, b, dinv, w, bdiv, blk_here());
+}
+
+template <int MODE, int EPI, int ZB, int TY, int RP, int NT, int FLAT>
+struct KSlab {
+    static constexpr int THREADS = 64 * TY;
+    struct Args {
+        OpView<float> op;
+        TileMap tm;
+        const cx<float> *xin;
+        int64_t ldx;
+        cx<float> *yout;
+        int64_t ldy;
+        int nrhs;
+        const int *live;
+        const double *scale;
+        const cx<float> *b;
+        const cx<float> *dinv;
+        float w;
+        const float *bdiv;
+    };
+    __device__ __forceinline__ static void run(const Args &a, const Blk &b) {
+        slab_body<MODE, EPI, ZB, TY, RP, NT, FLAT>(a.op, a.tm, a.xin, a.ldx, a.yout, a.ldy, a.nrhs, a.live, a.scale,
+                a.b, a.dinv, a.w, a.bdiv, b);
+    }
+    static void single(dim3 g, hipStream_t s, const Args &a) {
+        hipLaunchKernelGGL((k_slab<MODE, EPI, ZB, TY, RP, NT, FLAT>), g, dim3(THREADS), 0, s, a.op, a.tm, a.xin,
+                a.ldx, a.yout, a.ldy, a.nrhs, a.live, a.scale, a.b, a.dinv, a.w, a.bdiv);
+    }
+};
+
 // Paired slab stencil (flat = 4): the arithmetic of k_slab per node, in the
 // same order (bit-identical results), with each lane holding the nodes
 // (2i', 2i'+1), i' = its lane in a 128-wide tile row.  x-neighbours: 2i'-1
@@ -290,18 +325,17 @@ __global__ __launch_bounds__(64 * TY) void k_slab(OpView<float> op, TileMap tm, 
 // lanes 0 and 63 load the one across the tile edge.  One right-hand side per
 // pass.  Requires an even nx and 16-byte aligned columns (even ldx, ldy).
 template <int MODE, int EPI, int ZB, int TY, int NT>
-__global__ __launch_bounds__(64 * TY) void k_slab2(OpView<float> op, TileMap tm, const cx<float> *__restrict__ xin,
-                                               int64_t ldx, cx<float> *__restrict__ yout, int64_t ldy, int nrhs,
-                                               const int *__restrict__ live, const double *__restrict__ scale,
-                                               const cx<float> *__restrict__ b, const cx<float> *__restrict__ dinv,
-                                               float w, const float *__restrict__ bdiv) {
+__device__ __forceinline__ void slab2_body(const OpView<float> &op, const TileMap &tm, const cx<float> *__restrict__ xin,
+        int64_t ldx, cx<float> *__restrict__ yout, int64_t ldy, int nrhs, const int *__restrict__ live,
+        const double *__restrict__ scale, const cx<float> *__restrict__ b, const cx<float> *__restrict__ dinv,
+        float w, const float *__restrict__ bdiv, const Blk &bk) {
     __shared__ int s_act[APPLY_MAX_RHS];
     __shared__ float s_sc[APPLY_MAX_RHS];
     __shared__ int s_na;
     constexpr int NT_ = 64 * TY;
     __shared__ hz_f4 s_pl[2][ZB][NT_];
-    const int part = (int)blockIdx.x / tm.nblocks;
-    const int tile = xcd_tile((int)blockIdx.x - part * tm.nblocks, tm.nblocks);
+    const int part = (int)bk.x / tm.nblocks;
+    const int tile = xcd_tile((int)bk.x - part * tm.nblocks, tm.nblocks);
     if (tile >= tm.ntiles) return;  // whole workgroup
     stage_live_columns<float>(nrhs, live, scale, s_act, s_sc, &s_na);
     const int tx = tile % tm.tiles_x;
@@ -439,20 +473,20 @@ __global__ __launch_bounds__(64 * TY) void k_slab2(OpView<float> op, TileMap tm,
                 const cx<float> zm = e ? mk<float>(v[q].z, v[q].w) : mk<float>(v[q].x, v[q].y);
                 const cx<float> zp = e ? mk<float>(v[q + 2].z, v[q + 2].w) : mk<float>(v[q + 2].x, v[q + 2].y);
                 cx<float> acc = cs[q][e] * xc;
-                acc = acc + c[q][e][0] * xm;
-                acc = acc + c[q][e][1] * xp;
-                acc = acc + c[q][e][2] * ymv;
-                acc = acc + c[q][e][3] * ypv;
-                acc = acc + c[q][e][4] * zm;
-                acc = acc + c[q][e][5] * zp;
+                acc = cfma(c[q][e][0], xm, acc);
+                acc = cfma(c[q][e][1], xp, acc);
+                acc = cfma(c[q][e][2], ymv, acc);
+                acc = cfma(c[q][e][3], ypv, acc);
+                acc = cfma(c[q][e][4], zm, acc);
+                acc = cfma(c[q][e][5], zp, acc);
                 if (EPI == SLAB_APPLY) {
                     out[e] = s_sc[t] * acc;
                 } else if (EPI == SLAB_RESID0) {
-                    out[e] = bs[q][e] * xc - acc;
+                    out[e] = sfms(bs[q][e], xc, acc);
                 } else {
                     const cx<float> bv = e ? mk<float>(bb[q].z, bb[q].w) : mk<float>(bb[q].x, bb[q].y);
-                    const cx<float> ev = bs[q][e] * bv - acc;
-                    out[e] = (EPI == SLAB_SMOOTH) ? xc + w * (dv[q][e] * ev) : ev;
+                    const cx<float> ev = sfms(bs[q][e], bv, acc);
+                    out[e] = (EPI == SLAB_SMOOTH) ? sfma(w, dv[q][e] * ev, xc) : ev;
                 }
             }
             if (valid && q < nzl) {
@@ -470,6 +504,42 @@ __global__ __launch_bounds__(64 * TY) void k_slab2(OpView<float> op, TileMap tm,
         }
     }
 }
+
+template <int MODE, int EPI, int ZB, int TY, int NT>
+__global__ __launch_bounds__(64 * TY) void k_slab2(OpView<float> op, TileMap tm, const cx<float> *__restrict__ xin,
+        int64_t ldx, cx<float> *__restrict__ yout, int64_t ldy, int nrhs, const int *__restrict__ live,
+        const double *__restrict__ scale, const cx<float> *__restrict__ b, const cx<float> *__restrict__ dinv,
+        float w, const float *__restrict__ bdiv) {
+    slab2_body<MODE, EPI, ZB, TY, NT>(op, tm, xin, ldx, yout, ldy, nrhs, live, scale, b, dinv, w, bdiv, blk_here());
+}
+
+template <int MODE, int EPI, int ZB, int TY, int NT>
+struct KSlab2 {
+    static constexpr int THREADS = 64 * TY;
+    struct Args {
+        OpView<float> op;
+        TileMap tm;
+        const cx<float> *xin;
+        int64_t ldx;
+        cx<float> *yout;
+        int64_t ldy;
+        int nrhs;
+        const int *live;
+        const double *scale;
+        const cx<float> *b;
+        const cx<float> *dinv;
+        float w;
+        const float *bdiv;
+    };
+    __device__ __forceinline__ static void run(const Args &a, const Blk &b) {
+        slab2_body<MODE, EPI, ZB, TY, NT>(a.op, a.tm, a.xin, a.ldx, a.yout, a.ldy, a.nrhs, a.live, a.scale, a.b,
+                a.dinv, a.w, a.bdiv, b);
+    }
+    static void single(dim3 g, hipStream_t s, const Args &a) {
+        hipLaunchKernelGGL((k_slab2<MODE, EPI, ZB, TY, NT>), g, dim3(THREADS), 0, s, a.op, a.tm, a.xin, a.ldx, a.yout,
+                a.ldy, a.nrhs, a.live, a.scale, a.b, a.dinv, a.w, a.bdiv);
+    }
+};
 
 // FP64 apply of the plain form on the slab layout: y[:, r] = s_r op(x[:, r])
 // with the row coefficients of row_coefs and the neighbour terms added in
@@ -510,16 +580,16 @@ __host__ inline TileMap slab64_map(int nx, int ny, int nz, int k) {
 }
 
 template <int MODE, int ZB, int TY>
-__global__ __launch_bounds__(64 * TY) void k_slab64(OpView<double> op, TileMap tm, const cx<double> *__restrict__ xin,
-                                                int64_t ldx, cx<double> *__restrict__ yout, int64_t ldy, int nrhs,
-                                                const int *__restrict__ live, const double *__restrict__ scale) {
+__device__ __forceinline__ void slab64_body(const OpView<double> &op, const TileMap &tm, const cx<double> *__restrict__ xin,
+        int64_t ldx, cx<double> *__restrict__ yout, int64_t ldy, int nrhs, const int *__restrict__ live,
+        const double *__restrict__ scale, const Blk &bk) {
     __shared__ int s_act[APPLY_MAX_RHS];
     __shared__ double s_sc[APPLY_MAX_RHS];
     __shared__ int s_na;
     constexpr int NT_ = 64 * TY;
     __shared__ cx<double> s_pl[2][ZB][NT_];
-    const int part = (int)blockIdx.x / tm.nblocks;
-    const int tile = xcd_tile((int)blockIdx.x - part * tm.nblocks, tm.nblocks);
+    const int part = (int)bk.x / tm.nblocks;
+    const int tile = xcd_tile((int)bk.x - part * tm.nblocks, tm.nblocks);
     if (tile >= tm.ntiles) return;  // whole workgroup
     stage_live_columns<double>(nrhs, live, scale, s_act, s_sc, &s_na);
     const int tx = tile % tm.tiles_x;
@@ -596,15 +666,45 @@ __global__ __launch_bounds__(64 * TY) void k_slab64(OpView<double> op, TileMap t
             if (lane == 0) xl = xe[q];
             if (lane == 63) xn = xe[q];
             cx<double> acc = cs[q] * xc;
-            acc = acc + c[q][0] * xl;
-            acc = acc + c[q][1] * xn;
-            acc = acc + c[q][2] * ym[q];
-            acc = acc + c[q][3] * yp[q];
-            acc = acc + c[q][4] * v[q];
-            acc = acc + c[q][5] * v[q + 2];
+            acc = cfma(c[q][0], xl, acc);
+            acc = cfma(c[q][1], xn, acc);
+            acc = cfma(c[q][2], ym[q], acc);
+            acc = cfma(c[q][3], yp[q], acc);
+            acc = cfma(c[q][4], v[q], acc);
+            acc = cfma(c[q][5], v[q + 2], acc);
             if (valid && q < nzl) yout[r * ldy + p0 + (int64_t)q * sxy] = s_sc[t] * acc;
         }
     }
 }
+
+template <int MODE, int ZB, int TY>
+__global__ __launch_bounds__(64 * TY) void k_slab64(OpView<double> op, TileMap tm, const cx<double> *__restrict__ xin,
+        int64_t ldx, cx<double> *__restrict__ yout, int64_t ldy, int nrhs, const int *__restrict__ live,
+        const double *__restrict__ scale) {
+    slab64_body<MODE, ZB, TY>(op, tm, xin, ldx, yout, ldy, nrhs, live, scale, blk_here());
+}
+
+template <int MODE, int ZB, int TY>
+struct KSlab64 {
+    static constexpr int THREADS = 64 * TY;
+    struct Args {
+        OpView<double> op;
+        TileMap tm;
+        const cx<double> *xin;
+        int64_t ldx;
+        cx<double> *yout;
+        int64_t ldy;
+        int nrhs;
+        const int *live;
+        const double *scale;
+    };
+    __device__ __forceinline__ static void run(const Args &a, const Blk &b) {
+        slab64_body<MODE, ZB, TY>(a.op, a.tm, a.xin, a.ldx, a.yout, a.ldy, a.nrhs, a.live, a.scale, b);
+    }
+    static void single(dim3 g, hipStream_t s, const Args &a) {
+        hipLaunchKernelGGL((k_slab64<MODE, ZB, TY>), g, dim3(THREADS), 0, s, a.op, a.tm, a.xin, a.ldx, a.yout, a.ldy,
+                a.nrhs, a.live, a.scale);
+    }
+};
 
 }  // namespace hz

@@ -212,14 +212,13 @@ __device__ __forceinline__ void stage_live_columns(int nrhs, const int *__restri
 // out[:, r] = b[:, r] - op(x[:, r]) with per-workgroup FP64 partials of
 // |out|^2; one right-hand side per grid row (blockIdx.y).
 template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_residual(OpView<T> op, TileMap tm, const cx<T> *__restrict__ x,
-                                                  int64_t ldx, const cx<T> *__restrict__ b, int64_t ldb,
-                                                  cx<T> *__restrict__ out, int64_t ldo,
-                                                  const int *__restrict__ live, double *__restrict__ partial) {
+__device__ __forceinline__ void residual_body(const OpView<T> &op, const TileMap &tm, const cx<T> *__restrict__ x, int64_t ldx,
+        const cx<T> *__restrict__ b, int64_t ldb, cx<T> *__restrict__ out, int64_t ldo, const int *__restrict__ live,
+        double *__restrict__ partial, const Blk &bk) {
     __shared__ double red[4];
-    const int r = blockIdx.y;
+    const int r = bk.y;
     if (live && !live[r]) return;
-    const int tile = xcd_tile(blockIdx.x, tm.nblocks);
+    const int tile = xcd_tile(bk.x, tm.nblocks);
     double acc2 = 0.0;
     if (tile < tm.ntiles) {
         const int tx = tile % tm.tiles_x;
@@ -239,7 +238,7 @@ __global__ __launch_bounds__(256) void k_residual(OpView<T> op, TileMap tm, cons
                 const cx<T> *xr = x + r * ldx + p;
                 cx<T> acc = cs * xr[0];
 #pragma unroll
-                for (int n = 0; n < 6; ++n) acc = acc + c[n] * xr[nb.off[n]];
+                for (int n = 0; n < 6; ++n) acc = cfma(c[n], xr[nb.off[n]], acc);
                 const cx<T> e = b[r * ldb + p] - acc;
                 out[r * ldo + p] = e;
                 loc += norm2(e);
@@ -248,8 +247,39 @@ __global__ __launch_bounds__(256) void k_residual(OpView<T> op, TileMap tm, cons
         }
     }
     const double s = block_sum_256(acc2, red);
-    if (threadIdx.x == 0) partial[(int64_t)r * gridDim.x + blockIdx.x] = s;
+    if (threadIdx.x == 0) partial[(int64_t)r * bk.gx + bk.x] = s;
 }
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_residual(OpView<T> op, TileMap tm, const cx<T> *__restrict__ x, int64_t ldx,
+        const cx<T> *__restrict__ b, int64_t ldb, cx<T> *__restrict__ out, int64_t ldo, const int *__restrict__ live,
+        double *__restrict__ partial) {
+    residual_body<T, MODE>(op, tm, x, ldx, b, ldb, out, ldo, live, partial, blk_here());
+}
+
+template <typename T, int MODE>
+struct KResidual {
+    static constexpr int THREADS = 256;
+    struct Args {
+        OpView<T> op;
+        TileMap tm;
+        const cx<T> *x;
+        int64_t ldx;
+        const cx<T> *b;
+        int64_t ldb;
+        cx<T> *out;
+        int64_t ldo;
+        const int *live;
+        double *partial;
+    };
+    __device__ __forceinline__ static void run(const Args &a, const Blk &b) {
+        residual_body<T, MODE>(a.op, a.tm, a.x, a.ldx, a.b, a.ldb, a.out, a.ldo, a.live, a.partial, b);
+    }
+    static void single(dim3 g, hipStream_t s, const Args &a) {
+        hipLaunchKernelGGL((k_residual<T, MODE>), g, dim3(THREADS), 0, s, a.op, a.tm, a.x, a.ldx, a.b, a.ldb, a.out,
+                a.ldo, a.live, a.partial);
+    }
+};
 
 // Acceptance check of the reference (iterative_solver.py:70-97) in FP64 for
 // x (the physical solution): e = op(x) - b, t = |op| |x| + |b|.  Partials per
@@ -257,14 +287,13 @@ __global__ __launch_bounds__(256) void k_residual(OpView<T> op, TileMap tm, cons
 // When `rhs_out` is set it also writes the scaled refinement residual
 // -R e, rounded to complex64.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_check(OpView<double> op, TileMap tm, const cx<double> *__restrict__ x,
-                                               int64_t ldx, const cx<double> *__restrict__ b, int64_t ldb,
-                                               const int *__restrict__ live, double *__restrict__ partial,
-                                               cx<float> *__restrict__ rhs_out, int64_t ldr) {
+__device__ __forceinline__ void check_body(const OpView<double> &op, const TileMap &tm, const cx<double> *__restrict__ x,
+        int64_t ldx, const cx<double> *__restrict__ b, int64_t ldb, const int *__restrict__ live,
+        double *__restrict__ partial, cx<float> *__restrict__ rhs_out, int64_t ldr, const Blk &bk) {
     __shared__ double red[4];
-    const int r = blockIdx.y;
+    const int r = bk.y;
     if (live && !live[r]) return;
-    const int tile = xcd_tile(blockIdx.x, tm.nblocks);
+    const int tile = xcd_tile(bk.x, tm.nblocks);
     double se = 0, sb = 0, ue = 0, ub = 0, mx = 0, bad = 0;
     if (tile < tm.ntiles) {
         const int tx = tile % tm.tiles_x;
@@ -288,7 +317,7 @@ __global__ __launch_bounds__(256) void k_check(OpView<double> op, TileMap tm, co
 #pragma unroll
                 for (int n = 0; n < 6; ++n) {
                     const cx<double> xq = xr[nb.off[n]];
-                    acc = acc + c[n] * xq;
+                    acc = cfma(c[n], xq, acc);
                     t += cabs(c[n]) * cabs(xq);
                 }
                 const cx<double> bp = b[r * ldb + p];
@@ -306,7 +335,7 @@ __global__ __launch_bounds__(256) void k_check(OpView<double> op, TileMap tm, co
             }
         }
     }
-    double *dst = partial + ((int64_t)r * gridDim.x + blockIdx.x) * 6;
+    double *dst = partial + ((int64_t)r * bk.gx + bk.x) * 6;
     double v;
     v = block_sum_256(se, red);
     if (threadIdx.x == 0) dst[0] = v;
@@ -326,6 +355,37 @@ __global__ __launch_bounds__(256) void k_check(OpView<double> op, TileMap tm, co
     v = block_sum_256(bad, red);
     if (threadIdx.x == 0) dst[5] = v;
 }
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_check(OpView<double> op, TileMap tm, const cx<double> *__restrict__ x,
+        int64_t ldx, const cx<double> *__restrict__ b, int64_t ldb, const int *__restrict__ live,
+        double *__restrict__ partial, cx<float> *__restrict__ rhs_out, int64_t ldr) {
+    check_body<MODE>(op, tm, x, ldx, b, ldb, live, partial, rhs_out, ldr, blk_here());
+}
+
+template <int MODE>
+struct KCheck {
+    static constexpr int THREADS = 256;
+    struct Args {
+        OpView<double> op;
+        TileMap tm;
+        const cx<double> *x;
+        int64_t ldx;
+        const cx<double> *b;
+        int64_t ldb;
+        const int *live;
+        double *partial;
+        cx<float> *rhs_out;
+        int64_t ldr;
+    };
+    __device__ __forceinline__ static void run(const Args &a, const Blk &b) {
+        check_body<MODE>(a.op, a.tm, a.x, a.ldx, a.b, a.ldb, a.live, a.partial, a.rhs_out, a.ldr, b);
+    }
+    static void single(dim3 g, hipStream_t s, const Args &a) {
+        hipLaunchKernelGGL((k_check<MODE>), g, dim3(THREADS), 0, s, a.op, a.tm, a.x, a.ldx, a.b, a.ldb, a.live,
+                a.partial, a.rhs_out, a.ldr);
+    }
+};
 
 // Row scales and Jacobi inverses of A and A^H (iterative_solver.py:37-56),
 // computed once in FP64 and rounded for the FP32 view.

@@ -76,6 +76,42 @@ def gmres_batch(operators, rhs, workers=4, **options):
     return [(x, [s.as_dict() for s in st]) for x, st in zip(xs, stats)]
 
 
+def launch_counts(reset=False):
+    """Solver launches since the last reset: {"graphs", "fused", "single"}
+    (graph launches, fused multi-scene kernel launches, single-scene kernel
+    launches; launches inside a graph count once, as its graph launch)."""
+    out = (ctypes.c_int64*3)()
+    _capi.check(_capi.lib().hz_launch_counts(out, int(bool(reset))), "hz_launch_counts")
+    return {"graphs": out[0], "fused": out[1], "single": out[2]}
+
+
+def gmres_fused(operators, rhs, **options):
+    """Solve several scenes in lockstep (one hz_gmres_fused call, mixed form).
+
+    Every GMRES cycle runs each of its kernels once for all the scenes still
+    iterating: one grid over all their workgroups, each reading its scene's
+    arguments from a per-scene table (with use_graph=True each cycle body is one
+    graph launch).  Arguments and return value as gmres_batch; the results
+    per scene are those of HelmholtzOperator.gmres on the scene alone."""
+    ns = len(operators)
+    if ns != len(rhs) or ns == 0:
+        raise ValueError("need one right-hand side per operator")
+    bs = [op.as_columns(b, torch.complex128) for op, b in zip(operators, rhs)]
+    xs = [torch.empty_like(b) for b in bs]
+    stats = [(_capi.GmresStats*b.shape[0])() for b in bs]
+    opts = gmres_options(**options)
+    vp = ctypes.c_void_p
+    status = _capi.lib().hz_gmres_fused(
+        ns, (vp*ns)(*[op.handle for op in operators]), ctypes.byref(opts),
+        (vp*ns)(*[b.data_ptr() for b in bs]), (vp*ns)(*[x.data_ptr() for x in xs]),
+        (ctypes.c_int32*ns)(*[b.shape[0] for b in bs]), (ctypes.c_int64*ns)(*[op.n for op in operators]),
+        (ctypes.POINTER(_capi.GmresStats)*ns)(*[ctypes.cast(st, ctypes.POINTER(_capi.GmresStats)) for st in stats]),
+        _stream())
+    if status not in (_capi.HZ_OK, _capi.HZ_NOT_CONVERGED):
+        _capi.check(status, "hz_gmres_fused")
+    return [(x, [s.as_dict() for s in st]) for x, st in zip(xs, stats)]
+
+
 class HelmholtzOperator:
     """A = div(rho^-1 grad) + omega^2/(rho c^2) with PML and face rows, on the GPU."""
 
