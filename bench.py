@@ -434,10 +434,17 @@ def bench_uneven(args):
         if i % 8 == 7:
             log(f"uneven: {i + 1} scenes set up")
     bs = [_unit_rhs(op, s.num_sources, 0, args.scene_rhs) for op, s in zip(ops, sc)]
+    # Tile shapes: autotuned on the largest scene and used by every scene
+    # (the fused solve runs all scenes with the largest scene's shapes).
+    lead = max(range(len(ops)), key=lambda i: ops[i].n)
+    tuned = ops[lead].autotune(args.scene_rhs, max_chunk=args.chunk or 16, restart=args.mixed_restart,
+                               preconditioner="multigrid")
+    for op in ops:
+        op.set_tile_shapes(tuned["shapes"][:1 + 3*len(op.hierarchy.levels)])
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
     edges = [s.shape[0] for s in sc]
-    log(f"uneven setup {setup_s:.1f}s: edges {edges}")
+    log(f"uneven setup {setup_s:.1f}s (shapes {tuned['shapes']}): edges {edges}")
     kw = dict(restart=args.mixed_restart, maxiter=args.maxiter, rtol=args.rtol, backward_error_tol=1e-4,
               precision="c64", chunk=args.chunk or 16, preconditioner="multigrid", use_graph=bool(args.graph))
     # Warm-up: one small batch (kernel loading, graph capture paths).

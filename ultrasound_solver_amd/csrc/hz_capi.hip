@@ -3,6 +3,7 @@
 // complex64 inner solves under FP64 iterative refinement), the source
 // columns and trilinear sampling.  Declarations: include/hz.h.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -2020,6 +2021,47 @@ hz_status hz_gmres_fused(int32_t nscenes, hz_operator *const *ops, const hz_gmre
             if (nrhs[i] < 1 || ld[i] < ops[i]->n) throw BadArgument("need nrhs >= 1 and ld >= N for every scene");
             check_opts(ops[i], o);
             most = std::max<int>(most, nrhs[i]);
+        }
+        // Every scene runs with the tile shapes of the largest scene that has
+        // the level (results do not depend on the tile shape), so that the
+        // scenes' launches at a position are one kernel and fuse; each
+        // scene's own shapes are restored on return.
+        struct Saved {
+            int slab;
+            std::vector<std::array<int, 3>> lv;
+        };
+        std::vector<Saved> saved(nscenes);
+        auto lead_of = [&](int level) {
+            int lead = -1;
+            for (int i = 0; i < nscenes; ++i) {
+                const int nl = ops[i]->mg ? (int)ops[i]->mg->levels.size() : 0;
+                if ((level < 0 || level < nl) && (lead < 0 || ops[i]->n > ops[lead]->n)) lead = i;
+            }
+            return lead;
+        };
+        for (int i = 0; i < nscenes; ++i) {
+            saved[i].slab = ops[i]->slab_shape;
+            if (ops[i]->mg)
+                for (auto &L : ops[i]->mg->levels) saved[i].lv.push_back({L->shape[0], L->shape[1], L->shape[2]});
+        }
+        struct Restore {
+            hz_operator *const *ops;
+            std::vector<Saved> &saved;
+            ~Restore() {
+                for (size_t i = 0; i < saved.size(); ++i) {
+                    ops[i]->slab_shape = saved[i].slab;
+                    for (size_t l = 0; l < saved[i].lv.size(); ++l)
+                        for (int e = 0; e < 3; ++e) ops[i]->mg->levels[l]->shape[e] = saved[i].lv[l][e];
+                }
+            }
+        } restore{ops, saved};
+        const int lead = lead_of(-1);
+        for (int i = 0; i < nscenes; ++i) {
+            ops[i]->slab_shape = saved[lead].slab;
+            for (size_t l = 0; l < saved[i].lv.size(); ++l) {
+                const int src = lead_of((int)l);
+                for (int e = 0; e < 3; ++e) ops[i]->mg->levels[l]->shape[e] = saved[src].lv[l][e];
+            }
         }
         // Round q solves columns [q c, (q + 1) c) of every scene that has
         // them, all scenes in lockstep; the launch sequences (fused tables,
