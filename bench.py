@@ -67,6 +67,8 @@ def parse():
     p.add_argument("--scenes", type=int, default=32, help="uneven: scenes in the batch")
     p.add_argument("--scene-rhs", type=int, default=16, help="uneven: right-hand sides per scene")
     p.add_argument("--workers", type=int, default=4, help="uneven: concurrent scene streams")
+    p.add_argument("--uneven-legs", choices=["all", "fused"], default="all",
+                   help="uneven: all legs (scene by scene, streams, fused) or the fused solve alone")
     p.add_argument("--fused-graph", type=int, default=1,
                    help="uneven: capture each lockstep cycle body of the fused solve as a graph")
     p.add_argument("--precond", choices=["multigrid", "jacobi"], default="multigrid",
@@ -447,27 +449,30 @@ def bench_uneven(args):
     log(f"uneven setup {setup_s:.1f}s (shapes {tuned['shapes']}): edges {edges}")
     kw = dict(restart=args.mixed_restart, maxiter=args.maxiter, rtol=args.rtol, backward_error_tol=1e-4,
               precision="c64", chunk=args.chunk or 16, preconditioner="multigrid", use_graph=bool(args.graph))
-    # Warm-up: one small batch (kernel loading, graph capture paths).
-    gmres_batch(ops[:2], [b[:1] for b in bs[:2]], workers=2, **kw)
-    torch.cuda.synchronize()
-    ts = time.perf_counter()
-    seq_ok = 0
-    for op, b in zip(ops, bs):
-        _, st = op.gmres(b, **kw)
-        seq_ok += sum(x["converged"] for x in st)
-        op.release_workspace()
-    torch.cuda.synchronize()
-    seq_s = time.perf_counter() - ts
-    log(f"uneven sequential: {seq_ok} solves in {seq_s:.2f}s")
-    tb = time.perf_counter()
-    out = gmres_batch(ops, bs, workers=args.workers, **kw)
-    torch.cuda.synchronize()
-    batch_s = time.perf_counter() - tb
-    ok = sum(x["converged"] for _, st in out for x in st)
-    iters = [x["iterations"] for _, st in out for x in st]
-    log(f"uneven batch ({args.workers} streams): {ok} solves in {batch_s:.2f}s")
-    for op in ops:
-        op.release_workspace()
+    legs = {}
+    if args.uneven_legs == "all":
+        # Warm-up: one small batch (kernel loading, graph capture paths).
+        gmres_batch(ops[:2], [b[:1] for b in bs[:2]], workers=2, **kw)
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        seq_ok = 0
+        for op, b in zip(ops, bs):
+            _, st = op.gmres(b, **kw)
+            seq_ok += sum(x["converged"] for x in st)
+            op.release_workspace()
+        torch.cuda.synchronize()
+        seq_s = time.perf_counter() - ts
+        log(f"uneven sequential: {seq_ok} solves in {seq_s:.2f}s")
+        tb = time.perf_counter()
+        out = gmres_batch(ops, bs, workers=args.workers, **kw)
+        torch.cuda.synchronize()
+        batch_s = time.perf_counter() - tb
+        ok = sum(x["converged"] for _, st in out for x in st)
+        log(f"uneven batch ({args.workers} streams): {ok} solves in {batch_s:.2f}s")
+        legs = {"streams": {"value": ok/batch_s, "unit": "solves/s", "workers": args.workers},
+                "sequential": {"value": seq_ok/seq_s, "unit": "solves/s"}}
+        for op in ops:
+            op.release_workspace()
     # Lockstep: one fused solve of every scene, each cycle body one graph
     # launch (warm-up on two scenes first: kernel loading).
     fkw = dict(kw, use_graph=bool(args.fused_graph))
@@ -482,6 +487,7 @@ def bench_uneven(args):
     fok = sum(x["converged"] for _, st in fout for x in st)
     fworst = max(x["relative_residual"] for _, st in fout for x in st)
     cycles = [x["cycles"] for _, st in fout for x in st]
+    iters = [x["iterations"] for _, st in fout for x in st]
     log(f"uneven fused: {fok} solves in {fused_s:.2f}s; launches {counts}")
     print(json.dumps({
         "metric": "uneven 32-scene batch: RHS solves/s to 1e-6 rel. residual (mixed form)",
@@ -496,9 +502,7 @@ def bench_uneven(args):
                   "single_kernel_launches": counts["single"], "max_scene_cycles": max(cycles),
                   "what": "one graph launch per lockstep cycle body for all scenes still iterating; the "
                           "cycle tops and FP64 refinement checks are the fused and single launches"},
-        "streams": {"value": ok/batch_s, "unit": "solves/s", "workers": args.workers},
-        "sequential": {"value": seq_ok/seq_s, "unit": "solves/s"},
-        "iterations_median": float(np.median(iters)), "setup_s": setup_s}), flush=True)
+        **legs, "iterations_median": float(np.median(iters)), "setup_s": setup_s}), flush=True)
 
 
 def bench_large(args):
