@@ -538,11 +538,15 @@ void residual_mode(Fuser &f, int sc, const hz_operator *op, int mode, const cx<T
                    int k, const int *live, double *partial) {
     const dim3 grid(op->tm.nblocks, k);
     const int64_t n = op->n;
+    auto go = [&](auto mode_c) {
+        f.add<KResidual<T, decltype(mode_c)::value>>(sc, grid,
+                                                      {view<T>(op), op->tm, x, n, b, n, out, n, live, partial});
+    };
     switch (mode) {
-        case MODE_RAM: f.add<KResidual<T, MODE_RAM>>(sc, grid, {view<T>(op), op->tm, x, n, b, n, out, n, live, partial}); break;
-        case MODE_RAMH: f.add<KResidual<T, MODE_RAMH>>(sc, grid, {view<T>(op), op->tm, x, n, b, n, out, n, live, partial}); break;
-        case MODE_RA: f.add<KResidual<T, MODE_RA>>(sc, grid, {view<T>(op), op->tm, x, n, b, n, out, n, live, partial}); break;
-        case MODE_RAH: f.add<KResidual<T, MODE_RAH>>(sc, grid, {view<T>(op), op->tm, x, n, b, n, out, n, live, partial}); break;
+        case MODE_RAM: go(std::integral_constant<int, MODE_RAM>()); break;
+        case MODE_RAMH: go(std::integral_constant<int, MODE_RAMH>()); break;
+        case MODE_RA: go(std::integral_constant<int, MODE_RA>()); break;
+        case MODE_RAH: go(std::integral_constant<int, MODE_RAH>()); break;
         default: throw BadArgument("unknown residual mode");
     }
 }
@@ -1288,7 +1292,10 @@ void check_reduce(Fuser &f, int sc, hz_operator *op, int k) {
 
 std::vector<CheckResult> check_read(const std::vector<double> &h, int k) {
     std::vector<CheckResult> out(k);
-    for (int r = 0; r < k; ++r) out[r] = {h[r * 6], h[r * 6 + 1], h[r * 6 + 2], h[r * 6 + 3], h[r * 6 + 4], h[r * 6 + 5]};
+    for (int r = 0; r < k; ++r) {
+        const double *v = h.data() + (size_t)r * 6;
+        out[r] = {v[0], v[1], v[2], v[3], v[4], v[5]};
+    }
     return out;
 }
 
@@ -2446,14 +2453,14 @@ hz_status hz_autotune(hz_operator *op, int32_t nrhs, int32_t max_chunk, int32_t 
                     s);
                 L.shape[1] = pick_shape(
                     [&](int sh) {
-                        launch_slab<MODE_A, SLAB_RESID>(fs, 0, sh, L.v, lx, L.n, lt, L.n, kt, live, nullptr, lb, nullptr, 0.0f,
-                                                        bd);
+                        launch_slab<MODE_A, SLAB_RESID>(fs, 0, sh, L.v, lx, L.n, lt, L.n, kt, live, nullptr, lb,
+                                                        nullptr, 0.0f, bd);
                     },
                     s);
                 L.shape[2] = pick_shape(
                     [&](int sh) {
-                        launch_slab<MODE_A, SLAB_RESID0>(fs, 0, sh, L.v, lb, L.n, lt, L.n, kt, live, nullptr, nullptr, dv,
-                                                         mg.weight, bd);
+                        launch_slab<MODE_A, SLAB_RESID0>(fs, 0, sh, L.v, lb, L.n, lt, L.n, kt, live, nullptr,
+                                                         nullptr, dv, mg.weight, bd);
                     },
                     s, 1);
             }

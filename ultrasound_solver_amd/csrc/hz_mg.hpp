@@ -93,8 +93,9 @@ constexpr int COARSE_MAX_NODES = 1024;
 // workgroup per right-hand side keeps the iterate in LDS (ping-pong) and
 // each thread holds the coefficients of its <= 4 nodes in registers.
 template <typename T, int MODE>
-__device__ __forceinline__ void coarse_jacobi_body(const OpView<T> &op, const cx<T> *__restrict__ b, cx<T> *__restrict__ x,
-        const cx<T> *__restrict__ dinv, T w, int sweeps, const int *__restrict__ live, const Blk &bk) {
+__device__ __forceinline__ void coarse_jacobi_body(const OpView<T> &op, const cx<T> *__restrict__ b,
+        cx<T> *__restrict__ x, const cx<T> *__restrict__ dinv, T w, int sweeps, const int *__restrict__ live,
+        const Blk &bk) {
     constexpr int PER = COARSE_MAX_NODES / 256;
     __shared__ cx<T> xs[2][COARSE_MAX_NODES];
     const int r = bk.x;
@@ -504,7 +505,8 @@ __global__ __launch_bounds__(256) void k_prolong(const cx<T> *__restrict__ xc, i
         cx<T> *__restrict__ xf, int nxf, int nyf, int nzf, int k, int rs, const int *__restrict__ live, int soft,
         const cx<T> *__restrict__ b, const cx<T> *__restrict__ dinv, T w, const T *__restrict__ bdiv, int conj_d,
         int nblocks) {
-    prolong_body<T, PAIR>(xc, nxc, nyc, nzc, xf, nxf, nyf, nzf, k, rs, live, soft, b, dinv, w, bdiv, conj_d, nblocks, blk_here());
+    prolong_body<T, PAIR>(xc, nxc, nyc, nzc, xf, nxf, nyf, nzf, k, rs, live, soft, b, dinv, w, bdiv, conj_d, nblocks,
+                          blk_here());
 }
 
 template <typename T, int PAIR>

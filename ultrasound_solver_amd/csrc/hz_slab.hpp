@@ -108,10 +108,10 @@ __device__ __forceinline__ void store_out(cx<float> *dst, cx<float> v) {
 }
 
 template <int MODE, int EPI, int ZB, int TY, int RP, int NT, int FLAT>
-__device__ __forceinline__ void slab_body(const OpView<float> &op, const TileMap &tm, const cx<float> *__restrict__ xin,
-        int64_t ldx, cx<float> *__restrict__ yout, int64_t ldy, int nrhs, const int *__restrict__ live,
-        const double *__restrict__ scale, const cx<float> *__restrict__ b, const cx<float> *__restrict__ dinv,
-        float w, const float *__restrict__ bdiv, const Blk &bk) {
+__device__ __forceinline__ void slab_body(const OpView<float> &op, const TileMap &tm,
+        const cx<float> *__restrict__ xin, int64_t ldx, cx<float> *__restrict__ yout, int64_t ldy, int nrhs,
+        const int *__restrict__ live, const double *__restrict__ scale, const cx<float> *__restrict__ b,
+        const cx<float> *__restrict__ dinv, float w, const float *__restrict__ bdiv, const Blk &bk) {
     __shared__ int s_act[APPLY_MAX_RHS];
     __shared__ float s_sc[APPLY_MAX_RHS];
     __shared__ int s_na;
@@ -287,7 +287,8 @@ __global__ __launch_bounds__(64 * TY) void k_slab(OpView<float> op, TileMap tm, 
         int64_t ldx, cx<float> *__restrict__ yout, int64_t ldy, int nrhs, const int *__restrict__ live,
         const double *__restrict__ scale, const cx<float> *__restrict__ b, const cx<float> *__restrict__ dinv,
         float w, const float *__restrict__ bdiv) {
-    slab_body<MODE, EPI, ZB, TY, RP, NT, FLAT>(op, tm, xin, ldx, yout, ldy, nrhs, live, scale, b, dinv, w, bdiv, blk_here());
+    slab_body<MODE, EPI, ZB, TY, RP, NT, FLAT>(op, tm, xin, ldx, yout, ldy, nrhs, live, scale, b, dinv, w, bdiv,
+                                               blk_here());
 }
 
 template <int MODE, int EPI, int ZB, int TY, int RP, int NT, int FLAT>
@@ -325,10 +326,10 @@ struct KSlab {
 // lanes 0 and 63 load the one across the tile edge.  One right-hand side per
 // pass.  Requires an even nx and 16-byte aligned columns (even ldx, ldy).
 template <int MODE, int EPI, int ZB, int TY, int NT>
-__device__ __forceinline__ void slab2_body(const OpView<float> &op, const TileMap &tm, const cx<float> *__restrict__ xin,
-        int64_t ldx, cx<float> *__restrict__ yout, int64_t ldy, int nrhs, const int *__restrict__ live,
-        const double *__restrict__ scale, const cx<float> *__restrict__ b, const cx<float> *__restrict__ dinv,
-        float w, const float *__restrict__ bdiv, const Blk &bk) {
+__device__ __forceinline__ void slab2_body(const OpView<float> &op, const TileMap &tm,
+        const cx<float> *__restrict__ xin, int64_t ldx, cx<float> *__restrict__ yout, int64_t ldy, int nrhs,
+        const int *__restrict__ live, const double *__restrict__ scale, const cx<float> *__restrict__ b,
+        const cx<float> *__restrict__ dinv, float w, const float *__restrict__ bdiv, const Blk &bk) {
     __shared__ int s_act[APPLY_MAX_RHS];
     __shared__ float s_sc[APPLY_MAX_RHS];
     __shared__ int s_na;
@@ -580,9 +581,9 @@ __host__ inline TileMap slab64_map(int nx, int ny, int nz, int k) {
 }
 
 template <int MODE, int ZB, int TY>
-__device__ __forceinline__ void slab64_body(const OpView<double> &op, const TileMap &tm, const cx<double> *__restrict__ xin,
-        int64_t ldx, cx<double> *__restrict__ yout, int64_t ldy, int nrhs, const int *__restrict__ live,
-        const double *__restrict__ scale, const Blk &bk) {
+__device__ __forceinline__ void slab64_body(const OpView<double> &op, const TileMap &tm,
+        const cx<double> *__restrict__ xin, int64_t ldx, cx<double> *__restrict__ yout, int64_t ldy, int nrhs,
+        const int *__restrict__ live, const double *__restrict__ scale, const Blk &bk) {
     __shared__ int s_act[APPLY_MAX_RHS];
     __shared__ double s_sc[APPLY_MAX_RHS];
     __shared__ int s_na;

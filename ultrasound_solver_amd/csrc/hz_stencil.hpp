@@ -212,9 +212,9 @@ __device__ __forceinline__ void stage_live_columns(int nrhs, const int *__restri
 // out[:, r] = b[:, r] - op(x[:, r]) with per-workgroup FP64 partials of
 // |out|^2; one right-hand side per grid row (blockIdx.y).
 template <typename T, int MODE>
-__device__ __forceinline__ void residual_body(const OpView<T> &op, const TileMap &tm, const cx<T> *__restrict__ x, int64_t ldx,
-        const cx<T> *__restrict__ b, int64_t ldb, cx<T> *__restrict__ out, int64_t ldo, const int *__restrict__ live,
-        double *__restrict__ partial, const Blk &bk) {
+__device__ __forceinline__ void residual_body(const OpView<T> &op, const TileMap &tm, const cx<T> *__restrict__ x,
+        int64_t ldx, const cx<T> *__restrict__ b, int64_t ldb, cx<T> *__restrict__ out, int64_t ldo,
+        const int *__restrict__ live, double *__restrict__ partial, const Blk &bk) {
     __shared__ double red[4];
     const int r = bk.y;
     if (live && !live[r]) return;
@@ -287,9 +287,10 @@ struct KResidual {
 // When `rhs_out` is set it also writes the scaled refinement residual
 // -R e, rounded to complex64.
 template <int MODE>
-__device__ __forceinline__ void check_body(const OpView<double> &op, const TileMap &tm, const cx<double> *__restrict__ x,
-        int64_t ldx, const cx<double> *__restrict__ b, int64_t ldb, const int *__restrict__ live,
-        double *__restrict__ partial, cx<float> *__restrict__ rhs_out, int64_t ldr, const Blk &bk) {
+__device__ __forceinline__ void check_body(const OpView<double> &op, const TileMap &tm,
+        const cx<double> *__restrict__ x, int64_t ldx, const cx<double> *__restrict__ b, int64_t ldb,
+        const int *__restrict__ live, double *__restrict__ partial, cx<float> *__restrict__ rhs_out, int64_t ldr,
+        const Blk &bk) {
     __shared__ double red[4];
     const int r = bk.y;
     if (live && !live[r]) return;
