@@ -192,8 +192,9 @@ hz_status hz_gmres_batch(int32_t nscenes, hz_operator *const *ops, const hz_gmre
  * reading its scene's arguments from a per-scene descriptor table (up to 32
  * scenes per launch; more take several).  With opts->use_graph each cycle
  * body of the whole batch is one graph launch.  Columns are taken in rounds
- * of opts->chunk per scene; the workspaces of all scenes are held together
- * during the call and released on return.  Every scene runs with the slab
+ * of opts->chunk per scene (fewer when the scenes' workspaces together would
+ * not fit in 90 % of the free device memory); the workspaces of all scenes
+ * are held together during the call and released on return.  Every scene runs with the slab
  * tile shapes of the largest scene that has the level (so that the scenes'
  * launches are one kernel); each scene's own shapes are restored on return.  Per-scene results are those of
  * hz_gmres on the scene alone.  No counterpart in the reference: its solver

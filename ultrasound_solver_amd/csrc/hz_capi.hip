@@ -2073,7 +2073,20 @@ hz_status hz_gmres_fused(int32_t nscenes, hz_operator *const *ops, const hz_gmre
         // Round q solves columns [q c, (q + 1) c) of every scene that has
         // them, all scenes in lockstep; the launch sequences (fused tables,
         // captured cycle bodies) are kept across the round's refinements.
-        const int chunk = chunk_width(o, most);
+        // The round width is opts->chunk, halved while the scenes' complex64
+        // workspaces together would not fit in 90 % of the free device
+        // memory.
+        int chunk = chunk_width(o, most);
+        {
+            size_t free_b = 0, total_b = 0;
+            HZ_HIP(hipMemGetInfo(&free_b, &total_b));
+            auto need = [&](int c) {
+                size_t bytes = 0;
+                for (int i = 0; i < nscenes; ++i) bytes += chunk_bytes(ops[i], std::min(c, (int)nrhs[i]), o.restart);
+                return bytes;
+            };
+            while (chunk > 1 && need(chunk) > free_b / 10 * 9) chunk = (chunk + 1) / 2;
+        }
         LaunchCache cache;
         cache.limit = 64;
         for (int c0 = 0; c0 < most; c0 += chunk) {
