@@ -84,6 +84,33 @@ def test_fused_small_scenes_equal_their_own_solves(preconditioner, use_graph):
             assert okr.scaled_residual(a, xh[r], bh[r]) <= 1e-8
 
 
+@pytest.mark.parametrize("preconditioner", ["jacobi", "multigrid"])
+def test_fused_adjoint_with_a_zero_column_equals_their_own_solves(preconditioner):
+    """The adjoint solves A^H x = b of three scenes, one of whose columns is
+    zero (it converges at once with x = 0), fused and graph-captured, equal
+    each scene's own adjoint solve bit for bit; the fields meet the FP64
+    residual of A^H from the oracle's matrix."""
+    from ultrasound_solver_amd.operator import gmres_fused
+    sizes, ks = (14, 19, 24), (2, 3, 2)
+    ops = [make_op(n, 10 + i, preconditioner == "multigrid") for i, n in enumerate(sizes)]
+    bs = [rhs(n, k, 10 + i) for i, (n, k) in enumerate(zip(sizes, ks))]
+    bs[1][1].zero_()
+    opts = dict(rtol=1e-8, precision="c64", chunk=2, maxiter=20000, restart=10, preconditioner=preconditioner,
+                use_graph=True, relative_rtol=1e-8, adjoint=True)
+    single = [op.gmres(b, **opts) for op, b in zip(ops, bs)]
+    fused = gmres_fused(ops, bs, **opts)
+    same_results(fused, single)
+    assert float(torch.linalg.vector_norm(fused[1][0][1])) == 0.0
+    for (x, st), b, n, i in zip(fused, bs, sizes, range(3)):
+        rho, c, bnd, pml = small_scene(n, 10 + i)
+        ah = ohz.OracleOperator(rho, c, 1.07e-3, 2*np.pi*40e3, bnd, pml).matrix().conj().T.tocsr()
+        xh, bh = x.cpu().numpy(), b.cpu().numpy()
+        for r in range(x.shape[0]):
+            if not np.any(bh[r]):
+                continue
+            assert np.linalg.norm(bh[r] - ah @ xh[r])/np.linalg.norm(bh[r]) <= 1e-8
+
+
 def test_fused_uneven_scenes_equal_their_own_solves():
     """Scenes 0, 1 and 5 of the uneven batch (128^3, 64^3 and 96^3; 5, 4 and
     5 multigrid levels), two columns each to 1e-8, cycle bodies captured as

@@ -333,6 +333,51 @@ def test_tile_shapes_bitwise_identical_on_a_wide_even_grid():
         assert torch.equal(sol, ref[1]), f"shape {shape}: solution differs"
 
 
+def test_xcd_block_tiles_bitwise_identical():
+    """A 64 x 32 x 24 grid whose tile counts divide into XCD blocks (the 8
+    XCDs take 4 x 2 blocks of the tile grid in y and z for the rectangular
+    and paired shapes and for the FP64 slab; flat and z-fastest shapes keep
+    contiguous ranges): every shape's applies and multigrid solve equal those
+    of a flat shape bit for bit, and the FP64 applies match the oracle's
+    matrix."""
+    from ultrasound_solver_amd.operator import HelmholtzOperator
+    rng = np.random.default_rng(13)
+    shape3 = (64, 32, 24)
+    rho = 1.2*(1 + 0.1*rng.random(shape3))
+    c = 343.0*(1 + 0.03*rng.standard_normal(shape3))
+    bnd = {"-x": "open", "+x": "soft", "-y": "open", "+y": {"type": "impedance", "impedance": [823.2, 205.8]},
+           "-z": "reflecting", "+z": "open"}
+    dx, omega = 1.07e-3, 2*np.pi*40e3
+    op = HelmholtzOperator(rho, c, dx, omega, bnd, {"cells": 3}, device="cuda:0")
+    op.set_multigrid()
+    nshape = 1 + 3*len(op.hierarchy.levels)
+    x = torch.as_tensor(_rand(op.n, 3, 23).T.copy(), device="cuda:0").to(torch.complex64)
+    b = _point_rhs(op.n, 2, 29)
+    modes = ("A", "RAM", "AH", "RAMH", "RA", "RAH")
+    flat = 8  # flat tiles: never XCD blocks
+    assert HelmholtzOperator.SLAB_SHAPES[flat][4] == 1
+    ref = None
+    for shape in [flat] + [s for s in range(len(HelmholtzOperator.SLAB_SHAPES)) if s != flat]:
+        op.set_tile_shapes([shape]*nshape)
+        ys = [op.apply(x, mode=m, precision="c64") for m in modes]
+        sol, st = op.gmres(b, rtol=1e-6, precision="c64", chunk=2, maxiter=3000, preconditioner="multigrid")
+        its = [s["iterations"] for s in st]
+        if ref is None:
+            ref = (ys, sol, its)
+            continue
+        for m, y, y0 in zip(modes, ys, ref[0]):
+            assert torch.equal(y, y0), f"shape {shape}, mode {m}: apply differs"
+        assert its == ref[2], (shape, its, ref[2])
+        assert torch.equal(sol, ref[1]), f"shape {shape}: solution differs"
+    a = ohz.OracleOperator(rho, c, dx, omega, bnd, {"cells": 3}).matrix()
+    ah = okr.adjoint_matrix(a)
+    x128 = _rand(op.n, 3, 31)
+    xt = torch.as_tensor(x128.T.copy(), device="cuda:0")
+    for mode, m in (("A", a), ("AH", ah)):
+        got = op.apply(xt, mode=mode, precision="c128").cpu().numpy().T
+        assert _rel(got, m @ x128) <= 1e-13, mode
+
+
 @pytest.mark.parametrize("precision,pre", [("c128", "jacobi"), ("c64", "multigrid")])
 def test_graph_replay_matches_eager(precision, pre):
     """A captured cycle body replayed once per restart cycle (eager cycle-top

@@ -61,6 +61,7 @@ __host__ inline TileMap slab_map(int nx, int ny, int nz, int shape, int k) {
     t.nblocks = (t.ntiles + 7) / 8 * 8;
     const int passes = (k + sh.rp - 1) / sh.rp;
     t.csplit = std::max(1, std::min(passes, (SLAB_TARGET_BLOCKS + t.ntiles - 1) / t.ntiles));
+    if (sh.flat == 0 || sh.flat == 4) xcd_blocks(t, ny, nz);
     return t;
 }
 
@@ -127,19 +128,17 @@ __device__ __forceinline__ void slab_body(const OpView<float> &op, const TileMap
     // nblocks is a multiple of 8, so a workgroup's XCD (its index modulo 8)
     // is the same in every column group.
     const int part = (int)bk.x / tm.nblocks;
-    const int tile = xcd_tile((int)bk.x - part * tm.nblocks, tm.nblocks);
-    if (tile >= tm.ntiles) return;  // whole workgroup
-    stage_live_columns<float>(nrhs, live, scale, s_act, s_sc, &s_na);
     int tx, ty, tz;
     if (FLAT == 2) {  // z-fastest order
+        const int tile = xcd_tile((int)bk.x - part * tm.nblocks, tm.nblocks);
+        if (tile >= tm.ntiles) return;  // whole workgroup
         tz = tile % tm.tiles_z;
         tx = (tile / tm.tiles_z) % tm.tiles_x;
         ty = tile / (tm.tiles_z * tm.tiles_x);
-    } else {
-        tx = tile % tm.tiles_x;
-        ty = (tile / tm.tiles_x) % tm.tiles_y;
-        tz = tile / (tm.tiles_x * tm.tiles_y);
+    } else if (!slab_tile(tm, (int)bk.x - part * tm.nblocks, tx, ty, tz)) {
+        return;  // whole workgroup
     }
+    stage_live_columns<float>(nrhs, live, scale, s_act, s_sc, &s_na);
     const int lane = threadIdx.x & 63;
     // Lanes past the grid edge keep running (they feed the DPP shifts) on a
     // clamped node and never store.  In a flat tile the previous/next lane is
@@ -336,12 +335,9 @@ __device__ __forceinline__ void slab2_body(const OpView<float> &op, const TileMa
     constexpr int NT_ = 64 * TY;
     __shared__ hz_f4 s_pl[2][ZB][NT_];
     const int part = (int)bk.x / tm.nblocks;
-    const int tile = xcd_tile((int)bk.x - part * tm.nblocks, tm.nblocks);
-    if (tile >= tm.ntiles) return;  // whole workgroup
+    int tx, ty, tz;
+    if (!slab_tile(tm, (int)bk.x - part * tm.nblocks, tx, ty, tz)) return;  // whole workgroup
     stage_live_columns<float>(nrhs, live, scale, s_act, s_sc, &s_na);
-    const int tx = tile % tm.tiles_x;
-    const int ty = (tile / tm.tiles_x) % tm.tiles_y;
-    const int tz = tile / (tm.tiles_x * tm.tiles_y);
     const int lane = threadIdx.x & 63;
     // Lanes past the grid edge run on the last pair (feeding the shifts)
     // and never store.
@@ -577,6 +573,7 @@ __host__ inline TileMap slab64_map(int nx, int ny, int nz, int k) {
     t.ntiles = t.tiles_x * t.tiles_y * t.tiles_z;
     t.nblocks = (t.ntiles + 7) / 8 * 8;
     t.csplit = std::max(1, std::min(k, (SLAB_TARGET_BLOCKS + t.ntiles - 1) / t.ntiles));
+    xcd_blocks(t, ny, nz);
     return t;
 }
 
@@ -590,12 +587,9 @@ __device__ __forceinline__ void slab64_body(const OpView<double> &op, const Tile
     constexpr int NT_ = 64 * TY;
     __shared__ cx<double> s_pl[2][ZB][NT_];
     const int part = (int)bk.x / tm.nblocks;
-    const int tile = xcd_tile((int)bk.x - part * tm.nblocks, tm.nblocks);
-    if (tile >= tm.ntiles) return;  // whole workgroup
+    int tx, ty, tz;
+    if (!slab_tile(tm, (int)bk.x - part * tm.nblocks, tx, ty, tz)) return;  // whole workgroup
     stage_live_columns<double>(nrhs, live, scale, s_act, s_sc, &s_na);
-    const int tx = tile % tm.tiles_x;
-    const int ty = (tile / tm.tiles_x) % tm.tiles_y;
-    const int tz = tile / (tm.tiles_x * tm.tiles_y);
     const int lane = threadIdx.x & 63;
     // Lanes past the grid edge run on a clamped node (feeding the shifts)
     // and never store.
