@@ -79,6 +79,10 @@ def parse():
     p.add_argument("--roofline-only", action="store_true",
                    help="only the roofline launches: the solve's Krylov apply over 32 columns with the tile "
                         "shape pinned, no autotune and no solve (for profiler runs)")
+    p.add_argument("--transfers-only", action="store_true",
+                   help="only the finest-level multigrid transfers (restriction and prolongation of 64 "
+                        "columns, hz_mg_transfer), timed with events, then the counter-calibration copy "
+                        "(for profiler runs)")
     p.add_argument("--cpu-samples", type=int, default=3, help="timed CPU GMRES samples per thread setting")
     p.add_argument("--evaluations", type=int, default=2000, help="online: timed evaluations per loss")
     return p.parse_args()
@@ -116,6 +120,9 @@ def main():
     op.set_sources(node, col, val, scene.num_sources)
     if args.roofline_only:
         roofline_only(op, args)
+        return
+    if args.transfers_only:
+        transfers_only(op, args)
         return
     if args.precond == "multigrid":
         op.set_multigrid(**_mg_args(args))
@@ -310,14 +317,61 @@ def roofline_only(op, args):
     shape = ROOFLINE_SHAPE if args.stencil_shape < 0 else args.stencil_shape
     op.set_tile_shapes([shape])
     roof = roofline_launches(op, args, shape)
+    copied = calibration_copy()
+    print(json.dumps({"roofline": roof, "shape": shape, "calibration_copy_bytes_each_way": copied}), flush=True)
+
+
+def calibration_copy():
+    """A device copy of 2 GiB each way from a source evicted from the 256 MB
+    last-level cache by a 1 GiB fill: the known byte count that scales the
+    hardware byte counters of a profiler run."""
+    import torch
     src = torch.ones(512*1024*1024, dtype=torch.float32, device="cuda:0")
     flush = torch.empty(256*1024*1024, dtype=torch.float32, device="cuda:0")
     flush.fill_(2.0)
     dst = torch.empty_like(src)
     dst.copy_(src)
     torch.cuda.synchronize()
-    print(json.dumps({"roofline": roof, "shape": shape, "calibration_copy_bytes_each_way": src.numel()*4}),
-          flush=True)
+    return src.numel()*4
+
+
+def transfers_only(op, args, k=64):
+    """The finest-level multigrid transfers of the 128^3 scene alone:
+    `--stencil-reps` restrictions P^T r (128^3 -> 65^3) and prolongations
+    x += P e of k complex64 columns through hz_mg_transfer, timed with events
+    after half a second of stencil launches (loaded clocks), then the
+    calibration copy.  Algorithmic bytes: restriction 8 k (N_f + N_c),
+    prolongation 8 k (2 N_f + N_c)."""
+    import torch
+    op.set_multigrid(**_mg_args(args))
+    nf, nc = (int(np.prod(lev["shape"])) for lev in op.hierarchy.levels[:2])
+    g = torch.Generator(device="cuda:0")
+    g.manual_seed(5)
+    xf = torch.randn((k, nf), dtype=torch.complex64, device="cuda:0", generator=g)
+    xc = torch.randn((k, nc), dtype=torch.complex64, device="cuda:0", generator=g)
+    bc = torch.empty_like(xc)
+    t_warm = time.perf_counter()
+    while time.perf_counter() - t_warm < 0.5:
+        op.apply(xf[:32], mode="A", precision="c64")
+    stream = torch.cuda.current_stream()
+    out = {}
+    for name, launch, alg in (("restrict", lambda: op.mg_transfer(0, xf, out=bc), 8*k*(nf + nc)),
+                              ("prolong", lambda: op.mg_transfer(0, xc, prolong=True, out=xf), 8*k*(2*nf + nc))):
+        for _ in range(3):
+            launch()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.stencil_reps):
+            launch()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1)*1e3/args.stencil_reps
+        out[name] = {"launch_us": us, "alg_bytes_per_launch": alg, "achieved": alg/us/1e3, "peak": 8000.0,
+                     "unit": "GB/s", "frac": alg/us/1e3/8000.0, "launches": args.stencil_reps}
+        log(f"{name} k={k}: {us:.1f} us/launch, {alg/us/1e3:.0f} GB/s algorithmic")
+    copied = calibration_copy()
+    print(json.dumps({"transfers": out, "columns": k, "fine": nf, "coarse": nc,
+                      "calibration_copy_bytes_each_way": copied}), flush=True)
 
 
 def adjoint_gradient(op, scene, args):
