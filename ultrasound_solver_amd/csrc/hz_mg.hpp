@@ -455,15 +455,18 @@ __device__ __forceinline__ void prolong_body(const cx<T> *__restrict__ xc, int n
             for (int bq = 0; bq < 2; ++bq)
 #pragma unroll
                 for (int a = 0; a < 2; ++a) e[c][bq][a] = src[corner[c][bq][a]];
+        // A node at offset 0 along an axis takes weight 0 from that axis's
+        // second corner: those terms are left out (for a finite e, adding
+        // 0 * e changes no value), the others are added in the same order.
         auto correct = [&](int q, cx<T> acc) {
             const int dz = q >> 2, dy = (q >> 1) & 1, dx = q & 1;
             if (!fixed[q]) {
 #pragma unroll
-                for (int c = 0; c < 2; ++c)
+                for (int c = 0; c <= dz; ++c)
 #pragma unroll
-                    for (int bq = 0; bq < 2; ++bq)
+                    for (int bq = 0; bq <= dy; ++bq)
 #pragma unroll
-                        for (int a = 0; a < 2; ++a) {
+                        for (int a = 0; a <= dx; ++a) {
                             const T wt = wgt(dz, c) * cw[2][c] * wgt(dy, bq) * cw[1][bq] * wgt(dx, a) * cw[0][a];
                             acc = sfma(wt, e[c][bq][a], acc);
                         }
@@ -474,7 +477,9 @@ __device__ __forceinline__ void prolong_body(const cx<T> *__restrict__ xc, int n
         for (int row = 0; row < 4; ++row) {
             const int q0 = 2 * row, q1 = q0 + 1;
             if (PAIR && here[q1]) {  // here[q1] implies here[q0]; pf[q0] is even
-                const hz_f4 in = *reinterpret_cast<const hz_f4 *>((b ? b : xf) + r * nf + pf[q0]);
+                // Each input pair is read once per launch: streamed past the caches.
+                const hz_f4 in =
+                        __builtin_nontemporal_load(reinterpret_cast<const hz_f4 *>((b ? b : xf) + r * nf + pf[q0]));
                 cx<T> a0 = mk<T>(in.x, in.y), a1 = mk<T>(in.z, in.w);
                 if (b) {
                     a0 = f[q0] * a0;
