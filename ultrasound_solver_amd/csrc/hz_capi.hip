@@ -244,6 +244,14 @@ bool paired(int64_t n) {
     return std::is_same<T, float>::value && n % 2 == 0;
 }
 
+// Vector width of the Gram-Schmidt passes over k columns of n (see
+// STREAM_MIN_BYTES): 3 streamed pairs, 2 cached pairs, 1 single values.
+template <typename T>
+int vec_width(int64_t n, int k) {
+    if (!paired<T>(n)) return 1;
+    return (int64_t)k * n * (int64_t)sizeof(cx<T>) >= STREAM_MIN_BYTES ? 3 : 2;
+}
+
 // ------------------------------------------------------------- fused launches
 // The solver's host code sends its kernel launches to a Fuser.  Without a
 // plan it launches each kernel at once (one scene).  In a lockstep batch it
@@ -377,7 +385,12 @@ template <typename T>
 void launch_dots(Fuser &f, int sc, dim3 grid, const cx<T> *V, int k, int64_t n, int jp1, const cx<T> *w,
                  const int *live, double2 *part, int maxv, int64_t chunk) {
     if constexpr (std::is_same<T, float>::value) {
-        if (paired<T>(n)) {
+        const int vw = vec_width<T>(n, k);
+        if (vw == 3) {
+            f.add<KDots<T, 3>>(sc, grid, {V, k, n, jp1, w, live, part, maxv, chunk});
+            return;
+        }
+        if (vw == 2) {
             f.add<KDots<T, 2>>(sc, grid, {V, k, n, jp1, w, live, part, maxv, chunk});
             return;
         }
@@ -390,7 +403,13 @@ void launch_update(Fuser &f, int sc, dim3 grid, const cx<T> *V, int k, int64_t n
                    cx<T> *target, int64_t ldt, const cx<double> *coef, int maxv, const int *live, double *norm_partial,
                    int64_t chunk) {
     if constexpr (std::is_same<T, float>::value) {
-        if (paired<T>(n) && ldt % 2 == 0) {
+        const int vw = ldt % 2 == 0 ? vec_width<T>(n, k) : 1;
+        if (vw == 3) {
+            f.add<KUpdate<T, 3>>(sc, grid, {V, k, n, cnt_all, cnt_rhs, target, ldt, coef, maxv, live, norm_partial,
+                                            chunk});
+            return;
+        }
+        if (vw == 2) {
             f.add<KUpdate<T, 2>>(sc, grid, {V, k, n, cnt_all, cnt_rhs, target, ldt, coef, maxv, live, norm_partial,
                                             chunk});
             return;
@@ -1092,7 +1111,11 @@ void gmres_cycles(std::vector<Member> &ms, int mode_ram, int restart, int maxite
                     typename KDots2<float, 2>::Args a{r.V, r.k, r.n, j + 1, w1, w2, r.st.live,
                                                       r.dpart, maxv, r.chunk,
                                                       r.gram};
-                    if (paired<float>(r.n))
+                    const int vw = vec_width<float>(r.n, r.k);
+                    if (vw == 3)
+                        f.add<KDots2<float, 3>>(i, grid, {a.V, a.k, a.n, a.jp1, a.w1, a.w2, a.live, a.partial, a.maxv,
+                                                          a.chunk, a.wgram});
+                    else if (vw == 2)
                         f.add<KDots2<float, 2>>(i, grid, a);
                     else
                         f.add<KDots2<float, 1>>(i, grid, {a.V, a.k, a.n, a.jp1, a.w1, a.w2, a.live, a.partial, a.maxv,
@@ -1117,7 +1140,11 @@ void gmres_cycles(std::vector<Member> &ms, int mode_ram, int restart, int maxite
                     const dim3 grid(r.nblk, r.k);
                     typename KUpdate2<float, 2>::Args a{r.V, r.k, r.n, j + 1, w1, w2, r.st.coef, r.st.coef2, maxv,
                                                         r.st.live, r.gram, r.chunk, r.st.alpha};
-                    if (paired<float>(r.n))
+                    const int vw = vec_width<float>(r.n, r.k);
+                    if (vw == 3)
+                        f.add<KUpdate2<float, 3>>(i, grid, {a.V, a.k, a.n, a.cnt, a.w1, a.w2, a.coef, a.coef2,
+                                                            a.maxv, a.live, a.gram, a.chunk, a.alpha});
+                    else if (vw == 2)
                         f.add<KUpdate2<float, 2>>(i, grid, a);
                     else
                         f.add<KUpdate2<float, 1>>(i, grid, {a.V, a.k, a.n, a.cnt, a.w1, a.w2, a.coef, a.coef2, a.maxv,

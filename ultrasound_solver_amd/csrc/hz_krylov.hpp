@@ -18,9 +18,37 @@ namespace hz {
 
 constexpr int DOT_GROUP = 16;  // basis vectors per dot-product pass
 
+// Vector width VW of the Gram-Schmidt passes: 1 one complex per access, 2
+// two (16-byte accesses, complex64 with even N), 3 the same with
+// non-temporal accesses.  The host picks 3 where a basis vector (k columns
+// of N) is at least STREAM_MIN_BYTES: each element is read or written once
+// per launch and the vectors are far larger than the caches (128^3 x 64
+// columns: the two-step update pass 853 -> 802 ms and the dot pass 598 ->
+// 569 ms per solve of 64 columns).  Smaller vectors keep cached accesses, so
+// that what one kernel writes can still be read from the last-level cache by
+// the next (the 32-scene batch of 64^3-128^3 scenes at 16 columns ran 5 %
+// slower streamed).
+constexpr int64_t STREAM_MIN_BYTES = (int64_t)512 << 20;
+template <int VW>
+__device__ __forceinline__ float4 ld_vec(const float4 *p) {
+    if constexpr (VW == 3) {
+        const hz_f4 t = __builtin_nontemporal_load(reinterpret_cast<const hz_f4 *>(p));
+        return make_float4(t.x, t.y, t.z, t.w);
+    } else {
+        return *p;
+    }
+}
+template <int VW>
+__device__ __forceinline__ void st_vec(float4 *p, const float4 &v) {
+    if constexpr (VW == 3)
+        __builtin_nontemporal_store(hz_f4{v.x, v.y, v.z, v.w}, reinterpret_cast<hz_f4 *>(p));
+    else
+        *p = v;
+}
+
 // partial[((r*maxv + i)*nblk + blk)] = sum over the workgroup's chunk of
 // conj(V_i[r]) * w[r], for i in the group [g*16, min(g*16+16, jp1)).
-// VW = 2 (complex64 only; n and chunk even): each lane loads two complex
+// VW >= 2 (complex64 only; n and chunk even): each lane loads two complex
 // values per 16-byte access.
 template <typename T, int VW>
 __device__ __forceinline__ void dots_body(const cx<T> *__restrict__ V, int k, int64_t n, int jp1,
@@ -38,15 +66,16 @@ __device__ __forceinline__ void dots_body(const cx<T> *__restrict__ V, int k, in
     T are[DOT_GROUP], aim[DOT_GROUP];
 #pragma unroll
     for (int q = 0; q < DOT_GROUP; ++q) are[q] = aim[q] = 0;
-    if constexpr (VW == 2) {
+    if constexpr (VW >= 2) {
         static_assert(sizeof(T) == 4, "paired loads are for complex64");
         const float4 *w4 = reinterpret_cast<const float4 *>(wr);
         for (int64_t p = (p0 >> 1) + threadIdx.x; p < (p1 >> 1); p += 256) {
-            const float4 wv = w4[p];
+            const float4 wv = ld_vec<VW>(w4 + p);
 #pragma unroll
             for (int q = 0; q < DOT_GROUP; ++q) {
                 if (q < cnt) {
-                    const float4 v = reinterpret_cast<const float4 *>(V + ((int64_t)(g0 + q) * k + r) * n)[p];
+                    const float4 v =
+                            ld_vec<VW>(reinterpret_cast<const float4 *>(V + ((int64_t)(g0 + q) * k + r) * n) + p);
                     are[q] += v.x * wv.x + v.y * wv.y + v.z * wv.z + v.w * wv.w;
                     aim[q] += v.x * wv.y - v.y * wv.x + v.z * wv.w - v.w * wv.z;
                 }
@@ -162,7 +191,7 @@ struct KReduceDots {
 
 // target[r] -= sum_{i<cnt} coef[r][i] * V_i[r]; optional FP64 partial of
 // |target|^2 per workgroup (norm_partial[r*nblk + blk]).  Grid (nblk, k).
-// VW = 2 as in k_dots (target leading dimension even as well).
+// VW >= 2 as in k_dots (target leading dimension even as well).
 template <typename T, int VW>
 __device__ __forceinline__ void update_body(const cx<T> *__restrict__ V, int k, int64_t n, int cnt_all,
         const int *__restrict__ cnt_rhs, cx<T> *__restrict__ target, int64_t ldt, const cx<double> *__restrict__ coef,
@@ -178,17 +207,17 @@ __device__ __forceinline__ void update_body(const cx<T> *__restrict__ V, int k, 
     const int64_t p1 = min(n, p0 + chunk);
     cx<T> *tr = target + (int64_t)r * ldt;
     T nrm = 0;
-    if constexpr (VW == 2) {
+    if constexpr (VW >= 2) {
         static_assert(sizeof(T) == 4, "paired loads are for complex64");
         float4 *t4 = reinterpret_cast<float4 *>(tr);
         for (int64_t p = (p0 >> 1) + threadIdx.x; p < (p1 >> 1); p += 256) {
-            float4 acc = t4[p];
+            float4 acc = ld_vec<VW>(t4 + p);
             int q = 0;
             for (; q + 8 <= cnt; q += 8) {
                 float4 v[8];
 #pragma unroll
                 for (int u = 0; u < 8; ++u)
-                    v[u] = reinterpret_cast<const float4 *>(V + ((int64_t)(q + u) * k + r) * n)[p];
+                    v[u] = ld_vec<VW>(reinterpret_cast<const float4 *>(V + ((int64_t)(q + u) * k + r) * n) + p);
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
                     const cx<float> c = cf[q + u];
@@ -202,7 +231,7 @@ __device__ __forceinline__ void update_body(const cx<T> *__restrict__ V, int k, 
                 float4 v[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
-                    v[u] = reinterpret_cast<const float4 *>(V + ((int64_t)(q + u) * k + r) * n)[p];
+                    v[u] = ld_vec<VW>(reinterpret_cast<const float4 *>(V + ((int64_t)(q + u) * k + r) * n) + p);
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const cx<float> c = cf[q + u];
@@ -213,14 +242,14 @@ __device__ __forceinline__ void update_body(const cx<T> *__restrict__ V, int k, 
                 }
             }
             for (; q < cnt; ++q) {
-                const float4 v = reinterpret_cast<const float4 *>(V + ((int64_t)q * k + r) * n)[p];
+                const float4 v = ld_vec<VW>(reinterpret_cast<const float4 *>(V + ((int64_t)q * k + r) * n) + p);
                 const cx<float> c = cf[q];
                 acc.x = fmaf(c.im, v.y, fmaf(-c.re, v.x, acc.x));
                 acc.y = fmaf(-c.im, v.x, fmaf(-c.re, v.y, acc.y));
                 acc.z = fmaf(c.im, v.w, fmaf(-c.re, v.z, acc.z));
                 acc.w = fmaf(-c.im, v.z, fmaf(-c.re, v.w, acc.w));
             }
-            t4[p] = acc;
+            st_vec<VW>(t4 + p, acc);
             nrm += acc.x * acc.x + acc.y * acc.y + acc.z * acc.z + acc.w * acc.w;
         }
     } else {
@@ -471,12 +500,12 @@ __device__ __forceinline__ void dots2_body(const cx<T> *__restrict__ V, int k, i
     // |w1|^2 and w1^H w2 feed a difference (|W1|^2 = |w1|^2 - |h1|^2) that
     // can cancel: they are summed in FP64 from the first product on.
     double waa = 0, wabr = 0, wabi = 0;
-    if constexpr (VW == 2) {
+    if constexpr (VW >= 2) {
         static_assert(sizeof(T) == 4, "paired loads are for complex64");
         const float4 *a4 = reinterpret_cast<const float4 *>(ar);
         const float4 *b4 = reinterpret_cast<const float4 *>(br);
         for (int64_t p = (p0 >> 1) + threadIdx.x; p < (p1 >> 1); p += 256) {
-            const float4 av = a4[p], bv = b4[p];
+            const float4 av = ld_vec<VW>(a4 + p), bv = ld_vec<VW>(b4 + p);
             if (first) {
                 const double ax = av.x, ay = av.y, az = av.z, aw = av.w;
                 const double bx = bv.x, by = bv.y, bz = bv.z, bw = bv.w;
@@ -487,7 +516,8 @@ __device__ __forceinline__ void dots2_body(const cx<T> *__restrict__ V, int k, i
 #pragma unroll
             for (int q = 0; q < G; ++q) {
                 if (q < cnt) {
-                    const float4 v = reinterpret_cast<const float4 *>(V + ((int64_t)(g0 + q) * k + r) * n)[p];
+                    const float4 v =
+                            ld_vec<VW>(reinterpret_cast<const float4 *>(V + ((int64_t)(g0 + q) * k + r) * n) + p);
                     are[q] += v.x * av.x + v.y * av.y + v.z * av.z + v.w * av.w;
                     aim[q] += v.x * av.y - v.y * av.x + v.z * av.w - v.w * av.z;
                     bre[q] += v.x * bv.x + v.y * bv.y + v.z * bv.z + v.w * bv.w;
@@ -716,12 +746,12 @@ __device__ __forceinline__ void update2_body(const cx<T> *__restrict__ V, int k,
     cx<T> *br = w2 + (int64_t)r * n;
     const cx<T> al = cast<T>(alpha[r]);
     T g11 = 0, g22 = 0, g12r = 0, g12i = 0;
-    if constexpr (VW == 2) {
+    if constexpr (VW >= 2) {
         static_assert(sizeof(T) == 4, "paired loads are for complex64");
         float4 *a4 = reinterpret_cast<float4 *>(ar);
         float4 *b4 = reinterpret_cast<float4 *>(br);
         for (int64_t p = (p0 >> 1) + threadIdx.x; p < (p1 >> 1); p += 256) {
-            float4 a = a4[p], b = b4[p];
+            float4 a = ld_vec<VW>(a4 + p), b = ld_vec<VW>(b4 + p);
             b.x = fmaf(al.im, a.y, fmaf(-al.re, a.x, b.x));
             b.y = fmaf(-al.im, a.x, fmaf(-al.re, a.y, b.y));
             b.z = fmaf(al.im, a.w, fmaf(-al.re, a.z, b.z));
@@ -731,7 +761,7 @@ __device__ __forceinline__ void update2_body(const cx<T> *__restrict__ V, int k,
                 float4 v[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
-                    v[u] = reinterpret_cast<const float4 *>(V + ((int64_t)(q + u) * k + r) * n)[p];
+                    v[u] = ld_vec<VW>(reinterpret_cast<const float4 *>(V + ((int64_t)(q + u) * k + r) * n) + p);
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const cx<float> c = cf[q + u], d = cf2[q + u];
@@ -746,7 +776,7 @@ __device__ __forceinline__ void update2_body(const cx<T> *__restrict__ V, int k,
                 }
             }
             for (; q < cnt; ++q) {
-                const float4 v = reinterpret_cast<const float4 *>(V + ((int64_t)q * k + r) * n)[p];
+                const float4 v = ld_vec<VW>(reinterpret_cast<const float4 *>(V + ((int64_t)q * k + r) * n) + p);
                 const cx<float> c = cf[q], d = cf2[q];
                 a.x = fmaf(c.im, v.y, fmaf(-c.re, v.x, a.x));
                 a.y = fmaf(-c.im, v.x, fmaf(-c.re, v.y, a.y));
@@ -757,8 +787,8 @@ __device__ __forceinline__ void update2_body(const cx<T> *__restrict__ V, int k,
                 b.z = fmaf(d.im, v.w, fmaf(-d.re, v.z, b.z));
                 b.w = fmaf(-d.im, v.z, fmaf(-d.re, v.w, b.w));
             }
-            a4[p] = a;
-            b4[p] = b;
+            st_vec<VW>(a4 + p, a);
+            st_vec<VW>(b4 + p, b);
             g11 += a.x * a.x + a.y * a.y + a.z * a.z + a.w * a.w;
             g22 += b.x * b.x + b.y * b.y + b.z * b.z + b.w * b.w;
             g12r += a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w;
