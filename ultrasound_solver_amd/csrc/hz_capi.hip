@@ -557,9 +557,16 @@ void residual_mode(Fuser &f, int sc, const hz_operator *op, int mode, const cx<T
                    int k, const int *live, double *partial) {
     const dim3 grid(op->tm.nblocks, k);
     const int64_t n = op->n;
+    // complex64: the slab stencil's residual epilogue (b - op(x), the row in
+    // the order of k_residual), whose norm partials col_norm_partials adds;
+    // complex128: k_residual with its partials over op->tm.
     auto go = [&](auto mode_c) {
-        f.add<KResidual<T, decltype(mode_c)::value>>(sc, grid,
-                                                      {view<T>(op), op->tm, x, n, b, n, out, n, live, partial});
+        constexpr int M = decltype(mode_c)::value;
+        if constexpr (std::is_same<T, float>::value)
+            launch_slab<M, SLAB_RESID>(f, sc, op->slab_shape, op->v32, x, n, out, n, k, live, nullptr, b, nullptr,
+                                       0.0f, nullptr);
+        else
+            f.add<KResidual<T, M>>(sc, grid, {view<T>(op), op->tm, x, n, b, n, out, n, live, partial});
     };
     switch (mode) {
         case MODE_RAM: go(std::integral_constant<int, MODE_RAM>()); break;
@@ -568,6 +575,20 @@ void residual_mode(Fuser &f, int sc, const hz_operator *op, int mode, const cx<T
         case MODE_RAH: go(std::integral_constant<int, MODE_RAH>()); break;
         default: throw BadArgument("unknown residual mode");
     }
+}
+
+// FP64 norm partials of k complex64 columns of n (leading dimension n) over
+// nblk chunks of `chunk` elements: partial[r * nblk + blk].
+void col_norm_partials(Fuser &f, int sc, const cx<float> *v, int64_t n, int k, const int *live, double *partial,
+                       int64_t nblk, int64_t chunk) {
+    const dim3 grid((unsigned)nblk, k);
+    const int vw = vec_width<float>(n, k);
+    if (vw == 3)
+        f.add<KColNorms<3>>(sc, grid, {v, n, n, live, partial, chunk});
+    else if (vw == 2)
+        f.add<KColNorms<2>>(sc, grid, {v, n, n, live, partial, chunk});
+    else
+        f.add<KColNorms<1>>(sc, grid, {v, n, n, live, partial, chunk});
 }
 
 // ------------------------------------------------------------------ multigrid
@@ -1003,10 +1024,22 @@ void gmres_cycles(std::vector<Member> &ms, int mode_ram, int restart, int maxite
                                          r.npart);
                 }
                 f.flush();
+                // complex64 residual norms over the vector kernels' chunks.
+                constexpr bool c64 = std::is_same<T, float>::value;
+                if constexpr (c64) {
+                    for (int i = 0; i < S; ++i) {
+                        Run &r = rs[i];
+                        if (r.on)
+                            col_norm_partials(f, i, reinterpret_cast<const cx<float> *>(r.V), r.n, r.k,
+                                              r.st.live, r.npart, r.nblk, r.chunk);
+                    }
+                    f.flush();
+                }
                 for (int i = 0; i < S; ++i) {
                     Run &r = rs[i];
                     if (r.on)
-                        f.add<KCycleStart>(i, dim3(r.k), {r.st, r.npart, (int)r.op->tm.nblocks, maxv,
+                        f.add<KCycleStart>(i, dim3(r.k), {r.st, r.npart,
+                                                          (int)(c64 ? r.nblk : r.op->tm.nblocks), maxv,
                                                           restart});
                 }
                 f.flush();

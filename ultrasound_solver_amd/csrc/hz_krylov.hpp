@@ -371,6 +371,60 @@ struct KCycleStart {
     }
 };
 
+// partial[r * nblk + blk] = FP64 sum of |v[:, r]|^2 over the workgroup's
+// chunk (nblk = gridDim.x; FP32 per thread over its few elements): the norm
+// of the complex64 true residual at the cycle top, which the slab stencil
+// writes without reducing it (a reduction in the stencil would follow the
+// tile shape and round differently per shape).  Grid (nblk, k); VW as in
+// k_dots.
+template <int VW>
+__device__ __forceinline__ void col_norms_body(const cx<float> *__restrict__ v, int64_t n, int64_t ld,
+        const int *__restrict__ live, double *__restrict__ partial, int64_t chunk, const Blk &bk) {
+    __shared__ double red[4];
+    const int r = bk.y;
+    if (live && !live[r]) return;
+    const int64_t p0 = bk.x * chunk;
+    const int64_t p1 = min(n, p0 + chunk);
+    const cx<float> *vr = v + (int64_t)r * ld;
+    float s = 0;
+    if constexpr (VW >= 2) {
+        const float4 *v4 = reinterpret_cast<const float4 *>(vr);
+        for (int64_t p = (p0 >> 1) + threadIdx.x; p < (p1 >> 1); p += 256) {
+            const float4 a = ld_vec<VW>(v4 + p);
+            s += a.x * a.x + a.y * a.y + a.z * a.z + a.w * a.w;
+        }
+    } else {
+        for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) s += norm2(vr[p]);
+    }
+    const double t = block_sum_256((double)s, red);
+    if (threadIdx.x == 0) partial[(int64_t)r * bk.gx + bk.x] = t;
+}
+
+template <int VW>
+__global__ __launch_bounds__(256) void k_col_norms(const cx<float> *__restrict__ v, int64_t n, int64_t ld,
+        const int *__restrict__ live, double *__restrict__ partial, int64_t chunk) {
+    col_norms_body<VW>(v, n, ld, live, partial, chunk, blk_here());
+}
+
+template <int VW>
+struct KColNorms {
+    static constexpr int THREADS = 256;
+    struct Args {
+        const cx<float> *v;
+        int64_t n;
+        int64_t ld;
+        const int *live;
+        double *partial;
+        int64_t chunk;
+    };
+    __device__ __forceinline__ static void run(const Args &a, const Blk &b) {
+        col_norms_body<VW>(a.v, a.n, a.ld, a.live, a.partial, a.chunk, b);
+    }
+    static void single(dim3 g, hipStream_t s, const Args &a) {
+        hipLaunchKernelGGL((k_col_norms<VW>), g, dim3(THREADS), 0, s, a.v, a.n, a.ld, a.live, a.partial, a.chunk);
+    }
+};
+
 // Givens update of column j of H (rotated storage) whose subdiagonal entry
 // is beta (real, >= 0): apply the previous rotations, make the new one,
 // rotate g, record the residual estimate |g_{j+1}| and the step count, and
