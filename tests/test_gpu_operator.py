@@ -333,16 +333,16 @@ def test_tile_shapes_bitwise_identical_on_a_wide_even_grid():
         assert torch.equal(sol, ref[1]), f"shape {shape}: solution differs"
 
 
-def test_xcd_block_tiles_bitwise_identical():
-    """A 64 x 32 x 24 grid whose tile counts divide into XCD blocks (the 8
-    XCDs take 4 x 2 blocks of the tile grid in y and z for the rectangular
-    and paired shapes and for the FP64 slab; flat and z-fastest shapes keep
-    contiguous ranges): every shape's applies and multigrid solve equal those
-    of a flat shape bit for bit, and the FP64 applies match the oracle's
-    matrix."""
+@pytest.mark.parametrize("shape3", [(64, 32, 24), (256, 16, 16)])
+def test_xcd_block_tiles_bitwise_identical(shape3):
+    """Grids whose tile counts divide into XCD blocks (the 8 XCDs take 4 x 2
+    blocks of the tile grid in y and z for most rectangular and paired shapes
+    and for the FP64 slab, with one tile column in x at 64 x 32 x 24 and two
+    to four at 256 x 16 x 16; flat and z-fastest shapes keep contiguous
+    ranges): every shape's applies and multigrid solve equal those of a flat
+    shape bit for bit, and the FP64 applies match the oracle's matrix."""
     from ultrasound_solver_amd.operator import HelmholtzOperator
     rng = np.random.default_rng(13)
-    shape3 = (64, 32, 24)
     rho = 1.2*(1 + 0.1*rng.random(shape3))
     c = 343.0*(1 + 0.03*rng.standard_normal(shape3))
     bnd = {"-x": "open", "+x": "soft", "-y": "open", "+y": {"type": "impedance", "impedance": [823.2, 205.8]},
