@@ -758,27 +758,32 @@ def bench_online(args, evaluations=2000):
     for loss in ("focal_contrast", "focal_pressure", "field_match"):
         cfg = SimulationConfig.from_dict(dict(raw, training=dict(raw["training"], loss_type=loss)))
         solver = SimpleNamespace(cfg=cfg, transducers=SimpleNamespace(num_transducers=m))
-        obj = training.PhaseObjective(solver, basis=_DeviceBasis(g, cfg))
         phases = rng.uniform(0, 2*np.pi, (evaluations, m))
-        for q in range(20):
-            obj.evaluate(phases[q], gradient=True)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for q in range(evaluations):
-            obj.evaluate(phases[q], gradient=True)
-        torch.cuda.synchronize()
-        sec = time.perf_counter() - t0
-        out[loss] = {"evaluations_per_s": evaluations/sec, "ms_per_evaluation": sec/evaluations*1e3}
-        log(f"online {loss}: {evaluations/sec:.0f} evaluations/s")
+        for graph in (False, True):
+            obj = training.PhaseObjective(solver, basis=_DeviceBasis(g, cfg), use_graph=graph)
+            for q in range(20):
+                obj.evaluate(phases[q], gradient=True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for q in range(evaluations):
+                obj.evaluate(phases[q], gradient=True)
+            torch.cuda.synchronize()
+            sec = time.perf_counter() - t0
+            leg = "graph" if graph else "launches"
+            out.setdefault(loss, {})[leg] = {"evaluations_per_s": evaluations/sec,
+                                             "ms_per_evaluation": sec/evaluations*1e3}
+            log(f"online {loss} ({leg}): {evaluations/sec:.0f} evaluations/s")
     basis_bytes = n*m*16
     print(json.dumps({
         "metric": "phase-optimiser evaluations/s (loss + phase gradient) on a 41^3 x 144 device basis",
-        "value": out["focal_contrast"]["evaluations_per_s"], "unit": "evaluations/s", "n_gpus": 1,
+        "value": out["focal_contrast"]["graph"]["evaluations_per_s"], "unit": "evaluations/s", "n_gpus": 1,
         "higher_is_better": True, "dtype": "c128", "data": "synthetic basis of the archived run's shape",
         "config": {"workload": f"PhaseObjective.evaluate(gradient=True), {evaluations} seeded phase vectors, "
-                               "archived focal_contrast settings; G read once per evaluation"},
+                               "archived focal_contrast settings; G read once per evaluation; value: each "
+                               "evaluation one graph replay (use_graph=True), 'launches': the same step "
+                               "launched op by op"},
         "losses": out, "basis_bytes": basis_bytes,
-        "basis_read_GBps": basis_bytes*out["focal_contrast"]["evaluations_per_s"]/1e9,
+        "basis_read_GBps": basis_bytes*out["focal_contrast"]["graph"]["evaluations_per_s"]/1e9,
         "context": "the reference's archived run: 1,701 field + VJP evaluations in 12.54 s on its own GPU "
                    "(research_archive/algorithm_comparison/results/phase_oblique_reflecting_x_12x12)"}),
         flush=True)

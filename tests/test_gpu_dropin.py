@@ -265,3 +265,35 @@ def test_phase_objective_matches_oracle_and_gradient_check(tmp_path, loss, route
     assert obj.field_evaluations == 10 and obj.vjp_evaluations == 2
     assert obj.history[-1]["stage"] == "gradient_check"
     s.close()
+
+
+@pytest.mark.parametrize("loss", ["field_match", "focal_pressure", "focal_contrast", "focal_contrast_ratio"])
+def test_phase_objective_graph_replay_equals_launches(tmp_path, loss):
+    """PhaseObjective(use_graph=True): the device step (e^{i phi}, exact focal
+    samples, the allocation-free fused pass, the phase gradient) captured
+    once and replayed per evaluation gives the loss and gradient of the same
+    step launched op by op, bit for bit, on every replay: replays with and
+    without gradient alternate with evaluations of the launched objective
+    and host reads in between."""
+    from ultrasound_solver_amd.solvers import HelmholtzDirectSolver
+    from ultrasound_solver_amd.training import PhaseObjective
+    cfg = small_cfg(tmp_path, precision="mixed", pre="multigrid", loss=loss)
+    s = HelmholtzDirectSolver(cfg)
+    basis = s.build_phase_response_basis()
+    eager = PhaseObjective(s, basis=basis)
+    graph = PhaseObjective(s, basis=basis, use_graph=True)
+    rng = np.random.default_rng(21)
+    for q in range(6):
+        ph = rng.uniform(0, 2*np.pi, 4)
+        gradient = q % 3 != 2
+        ve, ge = eager.evaluate(ph, gradient=gradient)
+        vg, gg = graph.evaluate(ph, gradient=gradient)
+        assert vg == ve, (q, vg, ve)
+        if gradient:
+            assert np.array_equal(gg, ge), (q, gg, ge)
+        else:
+            assert gg is None and ge is None
+        torch.cuda.synchronize()
+    assert len(graph._graphs) == 2
+    assert graph.field_evaluations == 6 and graph.vjp_evaluations == 4
+    s.close()

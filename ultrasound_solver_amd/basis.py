@@ -82,7 +82,7 @@ LOSS_KINDS = {"field_match": _capi.LOSS_FIELD_MATCH, "energy": _capi.LOSS_ENERGY
 
 
 def fused_loss(G, w, kind, target=None, weight=None, mask=None, eps=EPS, scale=0.0, temperature=1.0,
-               count_a=0, count_b=0, return_field=False):
+               count_a=0, count_b=0, return_field=False, static=False):
     """One pass over G for a loss of the field u = G w (hz_basis_loss):
     returns (loss as a 0-d device tensor, g = G^H c (M,) device[, u]).
 
@@ -90,7 +90,10 @@ def fused_loss(G, w, kind, target=None, weight=None, mask=None, eps=EPS, scale=0
     |u|^2), "softmax" (scale * smooth-max of |u| over rows with mask != 0 at
     `temperature`; count_a masked rows) or "ratio" (mean |u|^2 over mask == 1
     divided by mean over mask == 2; count_a, count_b rows).  Per-row arrays
-    are device tensors in the row order of G; mask is uint8."""
+    are device tensors in the row order of G; mask is uint8.  static=True
+    calls hz_basis_loss_static, which allocates nothing (its partial sums sit
+    in a static device array shared by every call on the GPU), so that the
+    call can be captured into a graph."""
     n, m, ld = _check(G)
     if m > FUSED_MAX_EMITTERS:
         raise ValueError(f"the fused loss kernels take at most {FUSED_MAX_EMITTERS} emitters")
@@ -112,10 +115,10 @@ def fused_loss(G, w, kind, target=None, weight=None, mask=None, eps=EPS, scale=0
     loss = torch.empty(1, dtype=torch.float64, device=G.device)
     g = torch.empty(m, dtype=torch.complex128, device=G.device)
     u = torch.empty(n, dtype=torch.complex128, device=G.device) if return_field else None
-    _capi.check(_capi.lib().hz_basis_loss(ctypes.c_void_p(G.data_ptr()), ld, n, m, ctypes.c_void_p(w.data_ptr()),
-                                          ctypes.byref(desc), ctypes.c_void_p(loss.data_ptr()),
-                                          ctypes.c_void_p(g.data_ptr()),
-                                          ctypes.c_void_p(u.data_ptr() if u is not None else 0), _stream()),
-                "hz_basis_loss")
+    name = "hz_basis_loss_static" if static else "hz_basis_loss"
+    _capi.check(getattr(_capi.lib(), name)(ctypes.c_void_p(G.data_ptr()), ld, n, m, ctypes.c_void_p(w.data_ptr()),
+                                           ctypes.byref(desc), ctypes.c_void_p(loss.data_ptr()),
+                                           ctypes.c_void_p(g.data_ptr()),
+                                           ctypes.c_void_p(u.data_ptr() if u is not None else 0), _stream()), name)
     out = (loss[0], g)
     return out + (u,) if return_field else out

@@ -450,12 +450,12 @@ class AmplitudeFieldLoss:
             return loss, None
         return loss, weight*weight*res*field/(amp + self.eps)
 
-    def evaluate_basis(self, G, w, with_gradient=True, focal_field=None):
+    def evaluate_basis(self, G, w, with_gradient=True, focal_field=None, static=False):
         """(loss, G^H c) in one pass over the device basis G (hz_basis_loss)."""
         from . import basis as dev_basis
         tgt = _device_rows(self._dev, "target_rows", G, lambda: self.target.ravel(order="F"))
         wt = _device_rows(self._dev, "weight_rows", G, lambda: self.weight.ravel(order="F"))
-        loss, g = dev_basis.fused_loss(G, w, "field_match", target=tgt, weight=wt, eps=self.eps)
+        loss, g = dev_basis.fused_loss(G, w, "field_match", target=tgt, weight=wt, eps=self.eps, static=static)
         return loss, (g if with_gradient else None)
 
 
@@ -488,7 +488,7 @@ class FocalPressureLoss:
                 cot = cot + self.background_weight/field.numel()*field
         return loss, cot
 
-    def evaluate_basis(self, G, w, with_gradient=True, focal_field=None):
+    def evaluate_basis(self, G, w, with_gradient=True, focal_field=None, static=False):
         """(loss, G^H c) on the device basis G: the background energy in one
         fused pass (hz_basis_loss), the focal term on the target rows."""
         from . import basis as dev_basis
@@ -496,7 +496,7 @@ class FocalPressureLoss:
         rows = _device_rows(self._dev, "rows", G, lambda: _fortran_rows(self.target_indices, shape))
         loss, g = _basis_focal(G, rows, w, self.target_weight, self.eps, focal_field)
         if self.background_weight > 0:
-            lb, gb = dev_basis.fused_loss(G, w, "energy", scale=self.background_weight/G.shape[0])
+            lb, gb = dev_basis.fused_loss(G, w, "energy", scale=self.background_weight/G.shape[0], static=static)
             loss = loss + lb
             g = gb if g is None else g + gb
         if with_gradient and g is None:
@@ -553,7 +553,7 @@ class FocalContrastLoss:
         cot[mask] += self.sidelobe_weight*prob*field[mask]/(bg + self.eps)
         return loss, cot
 
-    def evaluate_basis(self, G, w, with_gradient=True, focal_field=None):
+    def evaluate_basis(self, G, w, with_gradient=True, focal_field=None, static=False):
         """(loss, G^H c) on the device basis G: the smooth-max sidelobe term in
         one fused pass (hz_basis_loss, running-maximum softmax), the focal term
         on the target rows."""
@@ -565,7 +565,7 @@ class FocalContrastLoss:
         loss, g = _basis_focal(G, rows, w, self.target_weight, self.eps, focal_field)
         ls, gs = dev_basis.fused_loss(G, w, "softmax", mask=mask, eps=self.eps, scale=self.sidelobe_weight,
                                       temperature=self.temperature,
-                                      count_a=int(np.count_nonzero(self.background_mask)))
+                                      count_a=int(np.count_nonzero(self.background_mask)), static=static)
         g = gs if g is None else g + gs
         return loss + ls, (g if with_gradient else None)
 
@@ -640,12 +640,13 @@ class FocalContrastRatioLoss:
         cot[rest] = -2.0*a*field[rest]/(nb*b*b)
         return a/b, cot
 
-    def evaluate_basis(self, G, w, with_gradient=True, focal_field=None):
+    def evaluate_basis(self, G, w, with_gradient=True, focal_field=None, static=False):
         """(loss, G^H c) in one fused pass over the device basis G (hz_basis_loss)."""
         from . import basis as dev_basis
         cls = _device_rows(self._dev, "class_rows", G, lambda: (self.ball.ravel(order="F").astype(np.uint8)
                                                                 + 2*self.rest.ravel(order="F").astype(np.uint8)))
-        loss, g = dev_basis.fused_loss(G, w, "ratio", mask=cls, count_a=self.n_ball, count_b=self.n_rest)
+        loss, g = dev_basis.fused_loss(G, w, "ratio", mask=cls, count_a=self.n_ball, count_b=self.n_rest,
+                                       static=static)
         return loss, (g if with_gradient else None)
 
 
@@ -703,10 +704,20 @@ class PhaseObjective:
     the torch loss its cotangent c, and an adjoint solve G^H c
     (A^H lambda = c, G^H c = a * B^T lambda) without forming G.  Per
     evaluation only the loss and the M phase gradients reach the host.
+
+    use_graph (basis path): the first evaluation with or without gradient
+    captures the whole device step -- e^{i phi}, the exact focal samples,
+    the fused pass over G through the allocation-free hz_basis_loss_static,
+    the phase gradient -- into a graph on fixed buffers; later evaluations
+    copy the phases in, replay it and read loss and gradient back in one
+    transfer.  Evaluations of several objectives that replay concurrently on
+    different streams of one GPU would race on that entry's shared scratch.
     """
 
-    def __init__(self, solver, basis=None, engine_kwargs=None, device="cuda"):
+    def __init__(self, solver, basis=None, engine_kwargs=None, device="cuda", use_graph=False):
         self.solver = solver
+        self.use_graph = bool(use_graph) and basis is not None
+        self._graphs = {}
         self.cfg = solver.cfg
         self.basis = basis
         self.loss_model = make_loss(self.cfg)
@@ -751,24 +762,75 @@ class PhaseObjective:
         c = cot.permute(2, 1, 0).reshape(1, -1)
         return loss, self.engine.response_adjoint(c)[0]
 
+    def _device_step(self, ph, gradient, static=False):
+        """(loss 0-d tensor, phase gradient (M,) tensor or None) for device
+        phases ph: the whole evaluation on the device."""
+        w = torch.exp(1j*ph)
+        focal = self.response @ w if self.response is not None else None
+        if static:
+            kw = {"focal_field": focal} if focal is not None else {}
+            loss_t, g = self.loss_model.evaluate_basis(self.basis.basis_gpu, w, gradient, static=True, **kw)
+        else:
+            loss_t, g = self._loss_and_adjoint(ph, w, gradient, focal)
+        if not gradient:
+            return loss_t, None
+        gt = torch.real(torch.conj(g)*(1j*w))
+        if focal is not None:
+            lm = self.loss_model
+            gt = gt + focal_gradient(self.response, focal, ph, lm.target_weight, lm.eps)
+        return loss_t, gt
+
+    def _graph_step(self, phases, gradient):
+        """Replay of the captured device step (captured on first use)."""
+        key = bool(gradient)
+        if key not in self._graphs:
+            ph = torch.zeros(self.size, dtype=torch.float64, device=self.device)
+            out = torch.zeros(self.size + 1, dtype=torch.float64, device=self.device)
+
+            def body():
+                loss_t, gt = self._device_step(ph, key, static=True)
+                out[:1].copy_(loss_t.reshape(1))
+                if gt is not None:
+                    out[1:].copy_(gt)
+
+            # Warm-up on a side stream (first-call set-up, such as the
+            # static scratch's address, must not happen inside the capture).
+            cur = torch.cuda.current_stream(self.device)
+            side = torch.cuda.Stream(device=self.device)
+            side.wait_stream(cur)
+            with torch.cuda.stream(side):
+                body()
+            cur.wait_stream(side)
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph):
+                body()
+            # Pinned host staging: phases in and results out are asynchronous
+            # copies on the replay stream, with one synchronisation.
+            ph_host = torch.empty(self.size, dtype=torch.float64, pin_memory=True)
+            out_host = torch.empty(self.size + 1, dtype=torch.float64, pin_memory=True)
+            self._graphs[key] = (graph, ph, out, ph_host, out_host)
+        graph, ph, out, ph_host, out_host = self._graphs[key]
+        ph_host.numpy()[:] = phases
+        ph.copy_(ph_host, non_blocking=True)
+        graph.replay()
+        out_host.copy_(out, non_blocking=True)
+        torch.cuda.current_stream(self.device).synchronize()
+        res = out_host.numpy()
+        return float(res[0]), (res[1:].copy() if gradient else None)
+
     def evaluate(self, phases, gradient=False, stage="optimization"):
         phases = wrap_phases(phases)
         if phases.shape != (self.size,):
             raise ValueError(f"Expected {self.size} phases")
         self.field_evaluations += 1
-        ph = torch.as_tensor(phases, device=self.device)
-        w = torch.exp(1j*ph)
-        focal = self.response @ w if self.response is not None else None
-        loss_t, g = self._loss_and_adjoint(ph, w, gradient, focal)
-        grad = None
         if gradient:
             self.vjp_evaluations += 1
-            gt = torch.real(torch.conj(g)*(1j*w))
-            if focal is not None:
-                lm = self.loss_model
-                gt = gt + focal_gradient(self.response, focal, ph, lm.target_weight, lm.eps)
-            grad = gt.cpu().numpy()
-        loss = float(loss_t)
+        if self.use_graph:
+            loss, grad = self._graph_step(phases, gradient)
+        else:
+            loss_t, gt = self._device_step(torch.as_tensor(phases, device=self.device), gradient)
+            grad = gt.cpu().numpy() if gt is not None else None
+            loss = float(loss_t)
         if not np.isfinite(loss):
             raise FloatingPointError("Non-finite field or loss")
         if grad is not None and not np.isfinite(grad).all():
