@@ -339,6 +339,23 @@ hz_status hz_basis_loss(const void *G, int64_t ldg, int64_t nrows, int32_t m, co
 hz_status hz_basis_loss_static(const void *G, int64_t ldg, int64_t nrows, int32_t m, const void *w,
                                const hz_basis_loss_desc *desc, double *loss, void *g, void *u, void *stream);
 
+/* The device work of one phase-optimiser evaluation around the fused pass
+ * over G (PhaseProblem.evaluate, src/algorithm/baselines/common.py:275-343),
+ * one workgroup each, allocating nothing.  hz_phase_prologue: w = e^{i phi}
+ * (m complex128) from m phases (FP64), and the exact focal samples
+ * focal = R w of a row-major ntargets x m complex128 target response R.
+ * hz_phase_epilogue: out[0] = loss[0] - target_weight mean_t |focal_t| (the
+ * focal term; loss[0] is the grid loss of the fused pass) and, with
+ * with_gradient, out[1 + j] = Re(conj(g_j + (R^H c)_j) i w_j) with
+ * c_t = -target_weight/ntargets focal_t/(|focal_t| + eps) (common.py:189-195)
+ * and g = G^H c of the grid loss.  m <= 512, ntargets <= 256 (0: no focal
+ * term).  Device pointers; stream-ordered. */
+hz_status hz_phase_prologue(const double *phases, int32_t m, const void *response, int32_t ntargets, void *w,
+                            void *focal, void *stream);
+hz_status hz_phase_epilogue(const void *w, int32_t m, const void *g, const void *response, int32_t ntargets,
+                            const void *focal, double target_weight, double eps, const double *loss,
+                            int32_t with_gradient, double *out, void *stream);
+
 /* Material gradient, replacing ConservativeHelmholtz.material_vjp
  * (src/algorithm/physics/conservative_helmholtz.py:151-175): for
  * dL = -Re(lambda^H dA u) writes dL/drho and dL/dc per node (FP64, N each)

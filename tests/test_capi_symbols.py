@@ -51,6 +51,20 @@ def test_bad_arguments_are_rejected_without_touching_the_device():
     status = _capi.lib().hz_gmres_fused(0, None, None, None, None, None, None, None, None)
     assert status == _capi.HZ_BAD_ARGUMENT
     assert b"nscenes" in _capi.lib().hz_last_error()
+    # The optimiser-step kernels check pointers, sizes and the target count.
+    lib = _capi.lib()
+    assert lib.hz_phase_prologue(None, 4, None, 0, None, None, None) == _capi.HZ_BAD_ARGUMENT
+    fake = ctypes.c_void_p(16)
+    assert lib.hz_phase_prologue(fake, 513, None, 0, fake, None, None) == _capi.HZ_BAD_ARGUMENT
+    assert b"512" in lib.hz_last_error()
+    assert lib.hz_phase_prologue(fake, 4, None, 2, fake, None, None) == _capi.HZ_BAD_ARGUMENT
+    assert b"targets" in lib.hz_last_error()
+    assert lib.hz_phase_epilogue(fake, 4, None, None, 0, None, 1.0, 1e-12, fake, 1, fake, None) == \
+        _capi.HZ_BAD_ARGUMENT
+    assert b"gradient" in lib.hz_last_error()
+    assert lib.hz_phase_epilogue(fake, 4, fake, None, 0, None, 1.0, 0.0, fake, 1, fake, None) == \
+        _capi.HZ_BAD_ARGUMENT
+    assert b"eps" in lib.hz_last_error()
 
 
 def test_operator_exposes_every_entry_point():

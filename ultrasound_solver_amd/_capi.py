@@ -19,7 +19,7 @@ EXPORTS = ("hz_create", "hz_destroy", "hz_apply", "hz_gmres", "hz_set_sources", 
            "hz_autotune", "hz_set_tile_shapes", "hz_gmres_batch", "hz_gmres_fused", "hz_launch_counts",
            "hz_release_workspace",
            "hz_material_vjp", "hz_basis_field", "hz_basis_adjoint", "hz_basis_field_match", "hz_basis_loss",
-           "hz_basis_loss_static", "hz_mg_transfer", "hz_mg_level",
+           "hz_basis_loss_static", "hz_phase_prologue", "hz_phase_epilogue", "hz_mg_transfer", "hz_mg_level",
            "hz_z_transfer", "hz_surface_create", "hz_surface_destroy", "hz_surface_tables",
            "hz_surface_set_sources", "hz_surface_transform", "hz_surface_solve_flux", "hz_surface_planes",
            "hz_surface_sample", "hz_surface_target_response", "hz_deposit_sources",
@@ -126,6 +126,9 @@ def lib():
         handle.hz_basis_loss.argtypes = [c_void_p, c_int64, c_int64, c_int32, c_void_p, POINTER(BasisLossDesc),
                                          c_void_p, c_void_p, c_void_p, c_void_p]
         handle.hz_basis_loss_static.argtypes = handle.hz_basis_loss.argtypes
+        handle.hz_phase_prologue.argtypes = [c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p, c_void_p]
+        handle.hz_phase_epilogue.argtypes = [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_void_p, c_double,
+                                             c_double, c_void_p, c_int32, c_void_p, c_void_p]
         handle.hz_mg_transfer.argtypes = [c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_int32, c_void_p]
         handle.hz_mg_level.argtypes = [c_void_p, c_int32, c_void_p, c_void_p, POINTER(c_void_p), POINTER(c_int32),
                                        c_void_p]

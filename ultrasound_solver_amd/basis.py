@@ -81,6 +81,48 @@ LOSS_KINDS = {"field_match": _capi.LOSS_FIELD_MATCH, "energy": _capi.LOSS_ENERGY
               "softmax": _capi.LOSS_SOFTMAX, "ratio": _capi.LOSS_RATIO}
 
 
+PHASE_MAX_TARGETS = 256
+
+
+def phase_prologue(ph, response=None):
+    """(w = e^{i ph}, focal = response @ w or None) on the device in one
+    launch (hz_phase_prologue): ph (M,) float64, response (P, M) complex128."""
+    ph = ph.to(torch.float64).contiguous()
+    m = ph.numel()
+    if not 1 <= m <= FUSED_MAX_EMITTERS:
+        raise ValueError(f"need 1 <= M <= {FUSED_MAX_EMITTERS} emitters")
+    w = torch.empty(m, dtype=torch.complex128, device=ph.device)
+    p, r_ptr, focal = 0, 0, None
+    if response is not None:
+        response = response.contiguous()
+        p = response.shape[0]
+        if response.shape != (p, m) or response.dtype != torch.complex128 or not 1 <= p <= PHASE_MAX_TARGETS:
+            raise ValueError(f"response must be (P, M) complex128 with 1 <= P <= {PHASE_MAX_TARGETS}")
+        focal = torch.empty(p, dtype=torch.complex128, device=ph.device)
+        r_ptr = response.data_ptr()
+    _capi.check(_capi.lib().hz_phase_prologue(ctypes.c_void_p(ph.data_ptr()), m, ctypes.c_void_p(r_ptr), p,
+                                              ctypes.c_void_p(w.data_ptr()),
+                                              ctypes.c_void_p(focal.data_ptr() if focal is not None else 0),
+                                              _stream()), "hz_phase_prologue")
+    return w, focal
+
+
+def phase_epilogue(w, g, loss, response=None, focal=None, target_weight=0.0, eps=EPS, gradient=True):
+    """(M + 1,) float64 device tensor [loss + focal term, phase gradient]
+    (hz_phase_epilogue): the focal term -target_weight mean |focal| and the
+    gradient Re(conj(g + R^H c) i w) with the focal cotangent c; g = G^H c
+    of the grid loss (None without gradient), loss its 0-d value."""
+    m = w.numel()
+    out = torch.empty(m + 1, dtype=torch.float64, device=w.device)
+    p = 0 if focal is None else focal.numel()
+    _capi.check(_capi.lib().hz_phase_epilogue(
+        ctypes.c_void_p(w.data_ptr()), m, ctypes.c_void_p(g.data_ptr() if g is not None else 0),
+        ctypes.c_void_p(response.data_ptr() if p else 0), p, ctypes.c_void_p(focal.data_ptr() if p else 0),
+        float(target_weight), float(eps), ctypes.c_void_p(loss.data_ptr()), int(bool(gradient)),
+        ctypes.c_void_p(out.data_ptr()), _stream()), "hz_phase_epilogue")
+    return out
+
+
 def fused_loss(G, w, kind, target=None, weight=None, mask=None, eps=EPS, scale=0.0, temperature=1.0,
                count_a=0, count_b=0, return_field=False, static=False):
     """One pass over G for a loss of the field u = G w (hz_basis_loss):

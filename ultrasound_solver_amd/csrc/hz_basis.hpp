@@ -363,4 +363,54 @@ __global__ __launch_bounds__(256) void k_basis_loss_combine(const cx<double> *__
     if (loss && blockIdx.x == 0 && threadIdx.x == 0) *loss = lval;
 }
 
+// ------------------------------------------------ phase step of an optimiser
+// The parts of one phase-optimiser evaluation around the fused pass over G
+// (PhaseProblem.evaluate, src/algorithm/baselines/common.py:275-343), each
+// one workgroup: the prologue w = e^{i phi} and the exact focal samples
+// f = R w of a (p x m, row-major) target response R; the epilogue adds the
+// focal term -tw mean_t |f_t| to the grid loss and forms the phase gradient
+// Re(conj(g + R^H c) i w) with c_t = -tw/p f_t/(|f_t| + eps)
+// (common.py:189-195), g = G^H c of the grid loss.
+constexpr int PHASE_MAX_TARGETS = 256;
+
+__global__ __launch_bounds__(256) void k_phase_prologue(const double *__restrict__ phi, int m,
+        const cx<double> *__restrict__ R, int p, cx<double> *__restrict__ w, cx<double> *__restrict__ f) {
+    __shared__ cx<double> sw[BASIS_FUSED_MAX_M];
+    __shared__ double red[4];
+    for (int j = threadIdx.x; j < m; j += 256) {
+        double s, c;
+        sincos(phi[j], &s, &c);
+        sw[j] = mk<double>(c, s);
+        w[j] = sw[j];
+    }
+    __syncthreads();
+    for (int t = 0; t < p; ++t) {
+        cx<double> acc = mk<double>(0, 0);
+        for (int j = threadIdx.x; j < m; j += 256) acc = cfma(R[(int64_t)t * m + j], sw[j], acc);
+        const double re = block_sum_256(acc.re, red);
+        const double im = block_sum_256(acc.im, red);
+        if (threadIdx.x == 0) f[t] = mk<double>(re, im);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_phase_epilogue(const cx<double> *__restrict__ w, int m,
+        const cx<double> *__restrict__ g, const cx<double> *__restrict__ R, int p, const cx<double> *__restrict__ f,
+        double tw, double eps, const double *__restrict__ loss_in, int with_gradient, double *__restrict__ out) {
+    __shared__ cx<double> scot[PHASE_MAX_TARGETS];
+    const double scale = p > 0 ? -tw / p : 0.0;
+    for (int t = threadIdx.x; t < p; t += 256) scot[t] = (scale / (hypot(f[t].re, f[t].im) + eps)) * f[t];
+    if (threadIdx.x == 0) {
+        double sum = 0.0;
+        for (int t = 0; t < p; ++t) sum += hypot(f[t].re, f[t].im);
+        out[0] = p > 0 ? loss_in[0] + -tw * (sum / p) : loss_in[0];
+    }
+    __syncthreads();
+    if (!with_gradient) return;
+    for (int j = threadIdx.x; j < m; j += 256) {
+        cx<double> a = g[j];
+        for (int t = 0; t < p; ++t) a = cfma(conj(R[(int64_t)t * m + j]), scot[t], a);
+        out[1 + j] = a.im * w[j].re - a.re * w[j].im;
+    }
+}
+
 }  // namespace hz

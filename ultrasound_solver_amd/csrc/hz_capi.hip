@@ -2766,6 +2766,39 @@ hz_status hz_basis_loss(const void *G, int64_t ldg, int64_t nrows, int32_t m, co
     });
 }
 
+hz_status hz_phase_prologue(const double *phases, int32_t m, const void *response, int32_t ntargets, void *w,
+                            void *focal, void *stream) {
+    return guarded([&] {
+        if (!phases || !w) throw BadArgument("phases and w must be device pointers");
+        if (m < 1 || m > BASIS_FUSED_MAX_M) throw BadArgument("need 1 <= m <= 512 emitters");
+        if (ntargets < 0 || ntargets > PHASE_MAX_TARGETS) throw BadArgument("need 0 <= ntargets <= 256");
+        if (ntargets > 0 && (!response || !focal)) throw BadArgument("targets need the response and focal buffers");
+        hipLaunchKernelGGL(k_phase_prologue, dim3(1), dim3(256), 0, static_cast<hipStream_t>(stream), phases, (int)m,
+                           static_cast<const cx<double> *>(response), (int)ntargets, static_cast<cx<double> *>(w),
+                           static_cast<cx<double> *>(focal));
+        HZ_LAUNCHED();
+    });
+}
+
+hz_status hz_phase_epilogue(const void *w, int32_t m, const void *g, const void *response, int32_t ntargets,
+                            const void *focal, double target_weight, double eps, const double *loss,
+                            int32_t with_gradient, double *out, void *stream) {
+    return guarded([&] {
+        if (!w || !loss || !out) throw BadArgument("w, loss and out must be device pointers");
+        if (m < 1 || m > BASIS_FUSED_MAX_M) throw BadArgument("need 1 <= m <= 512 emitters");
+        if (ntargets < 0 || ntargets > PHASE_MAX_TARGETS) throw BadArgument("need 0 <= ntargets <= 256");
+        if (ntargets > 0 && (!response || !focal)) throw BadArgument("targets need the response and focal buffers");
+        if (with_gradient && !g) throw BadArgument("the gradient needs g = G^H c");
+        if (!(eps > 0) || !std::isfinite(target_weight)) throw BadArgument("need eps > 0 and a finite target weight");
+        hipLaunchKernelGGL(k_phase_epilogue, dim3(1), dim3(256), 0, static_cast<hipStream_t>(stream),
+                           static_cast<const cx<double> *>(w), (int)m, static_cast<const cx<double> *>(g),
+                           static_cast<const cx<double> *>(response), (int)ntargets,
+                           static_cast<const cx<double> *>(focal), target_weight, eps, loss, (int)(with_gradient != 0),
+                           out);
+        HZ_LAUNCHED();
+    });
+}
+
 }  // extern "C"
 
 // Scratch of hz_basis_loss_static: the per-workgroup and folded partials of

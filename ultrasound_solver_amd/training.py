@@ -450,7 +450,7 @@ class AmplitudeFieldLoss:
             return loss, None
         return loss, weight*weight*res*field/(amp + self.eps)
 
-    def evaluate_basis(self, G, w, with_gradient=True, focal_field=None, static=False):
+    def evaluate_basis(self, G, w, with_gradient=True, focal_field=None, static=False, focal=True):
         """(loss, G^H c) in one pass over the device basis G (hz_basis_loss)."""
         from . import basis as dev_basis
         tgt = _device_rows(self._dev, "target_rows", G, lambda: self.target.ravel(order="F"))
@@ -488,13 +488,16 @@ class FocalPressureLoss:
                 cot = cot + self.background_weight/field.numel()*field
         return loss, cot
 
-    def evaluate_basis(self, G, w, with_gradient=True, focal_field=None, static=False):
+    def evaluate_basis(self, G, w, with_gradient=True, focal_field=None, static=False, focal=True):
         """(loss, G^H c) on the device basis G: the background energy in one
         fused pass (hz_basis_loss), the focal term on the target rows."""
         from . import basis as dev_basis
         shape = tuple(self.cfg.domain.grid_size)
-        rows = _device_rows(self._dev, "rows", G, lambda: _fortran_rows(self.target_indices, shape))
-        loss, g = _basis_focal(G, rows, w, self.target_weight, self.eps, focal_field)
+        if focal:
+            rows = _device_rows(self._dev, "rows", G, lambda: _fortran_rows(self.target_indices, shape))
+            loss, g = _basis_focal(G, rows, w, self.target_weight, self.eps, focal_field)
+        else:
+            loss, g = torch.zeros((), dtype=torch.float64, device=G.device), None
         if self.background_weight > 0:
             lb, gb = dev_basis.fused_loss(G, w, "energy", scale=self.background_weight/G.shape[0], static=static)
             loss = loss + lb
@@ -553,7 +556,7 @@ class FocalContrastLoss:
         cot[mask] += self.sidelobe_weight*prob*field[mask]/(bg + self.eps)
         return loss, cot
 
-    def evaluate_basis(self, G, w, with_gradient=True, focal_field=None, static=False):
+    def evaluate_basis(self, G, w, with_gradient=True, focal_field=None, static=False, focal=True):
         """(loss, G^H c) on the device basis G: the smooth-max sidelobe term in
         one fused pass (hz_basis_loss, running-maximum softmax), the focal term
         on the target rows."""
@@ -562,7 +565,7 @@ class FocalContrastLoss:
         rows = _device_rows(self._dev, "rows", G, lambda: _fortran_rows(self.target_indices, shape))
         mask = _device_rows(self._dev, "mask_rows", G,
                             lambda: self.background_mask.ravel(order="F").astype(np.uint8))
-        loss, g = _basis_focal(G, rows, w, self.target_weight, self.eps, focal_field)
+        loss, g = _basis_focal(G, rows, w, self.target_weight, self.eps, focal_field) if focal else (0.0, None)
         ls, gs = dev_basis.fused_loss(G, w, "softmax", mask=mask, eps=self.eps, scale=self.sidelobe_weight,
                                       temperature=self.temperature,
                                       count_a=int(np.count_nonzero(self.background_mask)), static=static)
@@ -640,7 +643,7 @@ class FocalContrastRatioLoss:
         cot[rest] = -2.0*a*field[rest]/(nb*b*b)
         return a/b, cot
 
-    def evaluate_basis(self, G, w, with_gradient=True, focal_field=None, static=False):
+    def evaluate_basis(self, G, w, with_gradient=True, focal_field=None, static=False, focal=True):
         """(loss, G^H c) in one fused pass over the device basis G (hz_basis_loss)."""
         from . import basis as dev_basis
         cls = _device_rows(self._dev, "class_rows", G, lambda: (self.ball.ravel(order="F").astype(np.uint8)
@@ -764,7 +767,20 @@ class PhaseObjective:
 
     def _device_step(self, ph, gradient, static=False):
         """(loss 0-d tensor, phase gradient (M,) tensor or None) for device
-        phases ph: the whole evaluation on the device."""
+        phases ph: the whole evaluation on the device.  With a basis and the
+        focal term on exact samples (or none), three launches around the fused
+        pass: hz_phase_prologue (e^{i phi}, focal samples), the grid loss and
+        G^H c, hz_phase_epilogue (focal term, phase gradient)."""
+        focal_kind = isinstance(self.loss_model, (FocalPressureLoss, FocalContrastLoss))
+        if self.basis is not None and (self.response is not None or not focal_kind):
+            from . import basis as dev_basis
+            lm = self.loss_model
+            w, focal = dev_basis.phase_prologue(ph, self.response)
+            loss_t, g = lm.evaluate_basis(self.basis.basis_gpu, w, gradient, static=static, focal=False)
+            out = dev_basis.phase_epilogue(w, g, loss_t, self.response, focal,
+                                           getattr(lm, "target_weight", 0.0), getattr(lm, "eps", 1.0e-12),
+                                           gradient)
+            return out[0], (out[1:] if gradient else None)
         w = torch.exp(1j*ph)
         focal = self.response @ w if self.response is not None else None
         if static:
