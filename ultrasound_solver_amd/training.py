@@ -713,8 +713,10 @@ class PhaseObjective:
     the fused pass over G through the allocation-free hz_basis_loss_static,
     the phase gradient -- into a graph on fixed buffers; later evaluations
     copy the phases in, replay it and read loss and gradient back in one
-    transfer.  Evaluations of several objectives that replay concurrently on
-    different streams of one GPU would race on that entry's shared scratch.
+    transfer.  The graph holds the addresses of the device basis and target
+    response, which must stay in place.  Evaluations of several objectives
+    that replay concurrently on different streams of one GPU would race on
+    that entry's shared scratch.
     """
 
     def __init__(self, solver, basis=None, engine_kwargs=None, device="cuda", use_graph=False):
@@ -771,9 +773,10 @@ class PhaseObjective:
         focal term on exact samples (or none), three launches around the fused
         pass: hz_phase_prologue (e^{i phi}, focal samples), the grid loss and
         G^H c, hz_phase_epilogue (focal term, phase gradient)."""
+        from . import basis as dev_basis
         focal_kind = isinstance(self.loss_model, (FocalPressureLoss, FocalContrastLoss))
-        if self.basis is not None and (self.response is not None or not focal_kind):
-            from . import basis as dev_basis
+        exact = self.response is not None and self.response.shape[0] <= dev_basis.PHASE_MAX_TARGETS
+        if self.basis is not None and (exact or (self.response is None and not focal_kind)):
             lm = self.loss_model
             w, focal = dev_basis.phase_prologue(ph, self.response)
             loss_t, g = lm.evaluate_basis(self.basis.basis_gpu, w, gradient, static=static, focal=False)
