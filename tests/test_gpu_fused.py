@@ -158,3 +158,45 @@ def test_fused_batch_of_all_32_scenes():
             assert st[r]["scaled_residual"] <= 1e-6 and rel[r] <= 1e-6
             assert rel[r] == pytest.approx(st[r]["relative_residual"], rel=1e-6, abs=1e-14)
         op.release_workspace()
+
+
+def test_fused_solves_on_two_threads_with_graphs():
+    """Two lockstep solves at once, each from its own host thread on its own
+    stream with its cycle bodies captured as graphs (one thread may capture
+    while the other uploads its fused argument tables): each equals the
+    same fused solve run alone, bit for bit."""
+    import threading
+    from ultrasound_solver_amd.operator import gmres_fused
+    groups = [((14, 19), (2, 1), 30), ((16, 21), (1, 2), 40)]
+    opts = dict(rtol=1e-8, precision="c64", chunk=2, maxiter=20000, restart=10, preconditioner="multigrid",
+                use_graph=True, relative_rtol=1e-8)
+    sets = []
+    for sizes, ks, seed in groups:
+        ops = [make_op(n, seed + i, True) for i, n in enumerate(sizes)]
+        bs = [rhs(n, k, seed + i) for i, (n, k) in enumerate(zip(sizes, ks))]
+        sets.append((ops, bs))
+    alone = [gmres_fused(ops, bs, **opts) for ops, bs in sets]
+    for ops, _ in sets:
+        for op in ops:
+            op.release_workspace()
+    torch.cuda.synchronize()
+    out, errors = [None, None], []
+
+    def run(q):
+        try:
+            stream = torch.cuda.Stream()
+            with torch.cuda.stream(stream):
+                out[q] = gmres_fused(*sets[q], **opts)
+            stream.synchronize()
+        except Exception as e:  # reported below, in the test's thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=run, args=(q,)) for q in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    for q in range(2):
+        same_results(out[q], alone[q])
+        assert all(v["converged"] for _, st in out[q] for v in st)

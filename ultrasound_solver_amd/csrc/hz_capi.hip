@@ -137,10 +137,13 @@ struct FusePlan {
     DevBuf dev;
     size_t next = 0;
     bool recording = true;
-    void upload() {
+    // On the solve's stream (no legacy-stream copy, which would order itself
+    // after every blocking stream, including one another thread captures).
+    void upload(hipStream_t s) {
         if (!host.empty()) {
             dev.ensure(host.size());
-            HZ_HIP(hipMemcpy(dev.p, host.data(), host.size(), hipMemcpyHostToDevice));
+            HZ_HIP(hipMemcpyAsync(dev.p, host.data(), host.size(), hipMemcpyHostToDevice, s));
+            HZ_HIP(hipStreamSynchronize(s));
         }
         recording = false;
         next = 0;
@@ -826,7 +829,7 @@ void run_sequence(LaunchCache &lc, std::vector<uintptr_t> sig, bool fused, bool 
             Fuser rec(s, e->plan.get());
             body(rec);
             rec.flush();
-            e->plan->upload();
+            e->plan->upload(s);
         }
         if (graph) {
             if (!lc.capture) HZ_HIP(hipStreamCreateWithFlags(&lc.capture, hipStreamNonBlocking));
