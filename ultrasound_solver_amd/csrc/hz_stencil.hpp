@@ -53,7 +53,7 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblocks) {
 // tiles are resident together).  At 128^3 x 32 columns a z-boundary between
 // blocks costs about twice the fetched bytes of a y-boundary of the same
 // area (z-halo planes are read by both sides), so the split minimises
-// (py - 1) nz + 2 (pz - 1) ny: 4 x 2 there, 1.13x -> 1.05x the algorithmic
+// (py - 1) nz + 2 (pz - 1) ny: 4 x 2 there, 1.13x -> 1.06x the algorithmic
 // bytes.  Tile counts that do not divide keep xcd_tile's contiguous ranges.
 __host__ inline void xcd_blocks(TileMap &t, int ny, int nz) {
     t.py = t.pz = 0;
