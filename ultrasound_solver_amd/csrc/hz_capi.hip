@@ -247,12 +247,15 @@ bool paired(int64_t n) {
     return std::is_same<T, float>::value && n % 2 == 0;
 }
 
-// Vector width of the Gram-Schmidt passes over k columns of n (see
-// STREAM_MIN_BYTES): 3 streamed pairs, 2 cached pairs, 1 single values.
+// Vector width of the Gram-Schmidt passes over k columns of n reading
+// `vecs` basis vectors (see STREAM_MIN_BYTES): 3 streamed pairs, 2 cached
+// pairs, 4 single values with the basis streamed (complex128 only), 1
+// single values.
 template <typename T>
-int vec_width(int64_t n, int k) {
-    if (!paired<T>(n)) return 1;
-    return (int64_t)k * n * (int64_t)sizeof(cx<T>) >= STREAM_MIN_BYTES ? 3 : 2;
+int vec_width(int64_t n, int k, int vecs = 0) {
+    const int64_t bytes = (int64_t)k * n * (int64_t)sizeof(cx<T>);
+    if (!paired<T>(n)) return std::is_same<T, double>::value && vecs * bytes >= STREAM_PASS_BYTES ? 4 : 1;
+    return bytes >= STREAM_MIN_BYTES ? 3 : 2;
 }
 
 // ------------------------------------------------------------- fused launches
@@ -387,14 +390,19 @@ class Fuser {
 template <typename T>
 void launch_dots(Fuser &f, int sc, dim3 grid, const cx<T> *V, int k, int64_t n, int jp1, const cx<T> *w,
                  const int *live, double2 *part, int maxv, int64_t chunk) {
+    const int vw = vec_width<T>(n, k, jp1);
     if constexpr (std::is_same<T, float>::value) {
-        const int vw = vec_width<T>(n, k);
         if (vw == 3) {
             f.add<KDots<T, 3>>(sc, grid, {V, k, n, jp1, w, live, part, maxv, chunk});
             return;
         }
         if (vw == 2) {
             f.add<KDots<T, 2>>(sc, grid, {V, k, n, jp1, w, live, part, maxv, chunk});
+            return;
+        }
+    } else {
+        if (vw == 4) {
+            f.add<KDots<T, 4>>(sc, grid, {V, k, n, jp1, w, live, part, maxv, chunk});
             return;
         }
     }
@@ -405,8 +413,11 @@ template <typename T>
 void launch_update(Fuser &f, int sc, dim3 grid, const cx<T> *V, int k, int64_t n, int cnt_all, const int *cnt_rhs,
                    cx<T> *target, int64_t ldt, const cx<double> *coef, int maxv, const int *live, double *norm_partial,
                    int64_t chunk) {
+    // The cycle-end update (counts per column) reads at most maxv - 1
+    // vectors; paired accesses need an even target leading dimension too.
+    int vw = vec_width<T>(n, k, cnt_rhs ? maxv - 1 : cnt_all);
+    if ((vw == 2 || vw == 3) && ldt % 2 != 0) vw = 1;
     if constexpr (std::is_same<T, float>::value) {
-        const int vw = ldt % 2 == 0 ? vec_width<T>(n, k) : 1;
         if (vw == 3) {
             f.add<KUpdate<T, 3>>(sc, grid, {V, k, n, cnt_all, cnt_rhs, target, ldt, coef, maxv, live, norm_partial,
                                             chunk});
@@ -414,6 +425,12 @@ void launch_update(Fuser &f, int sc, dim3 grid, const cx<T> *V, int k, int64_t n
         }
         if (vw == 2) {
             f.add<KUpdate<T, 2>>(sc, grid, {V, k, n, cnt_all, cnt_rhs, target, ldt, coef, maxv, live, norm_partial,
+                                            chunk});
+            return;
+        }
+    } else {
+        if (vw == 4) {
+            f.add<KUpdate<T, 4>>(sc, grid, {V, k, n, cnt_all, cnt_rhs, target, ldt, coef, maxv, live, norm_partial,
                                             chunk});
             return;
         }
@@ -1228,7 +1245,7 @@ void gmres_cycles(std::vector<Member> &ms, int mode_ram, int restart, int maxite
                                   r.st.live, r.st.scales + (int64_t)j * r.k);
                 }
                 f.flush();
-                const int groups = (j + 1 + DOT_GROUP - 1) / DOT_GROUP;
+                const int groups = (j + 1 + dot_group<T>() - 1) / dot_group<T>();
                 for (int i = 0; i < S; ++i) {
                     Run &r = rs[i];
                     if (r.on)

@@ -16,24 +16,51 @@
 
 namespace hz {
 
-constexpr int DOT_GROUP = 16;  // basis vectors per dot-product pass
+constexpr int DOT_GROUP = 16;  // basis vectors per dot-product pass (complex64)
+// Complex128 dot passes take 4 vectors per workgroup: FP64 accumulators of
+// 16 vectors (64 VGPRs) leave room for few loads in flight, and the extra
+// reads of w, once per group, come from the caches (GMRES(30) at 128^3, one
+// column: 269.5, 258.3, 255.6 and 264.7 us per Arnoldi step with groups of
+// 16, 8, 4 and 2).  The sums, and so the results, do not depend on the group.
+constexpr int DOT_GROUP_F64 = 4;
+template <typename T>
+constexpr int dot_group() {
+    return sizeof(T) == 8 ? DOT_GROUP_F64 : DOT_GROUP;
+}
 
 // Vector width VW of the Gram-Schmidt passes: 1 one complex per access, 2
 // two (16-byte accesses, complex64 with even N), 3 the same with
-// non-temporal accesses.  The host picks 3 where a basis vector (k columns
+// non-temporal accesses, 4 one complex per access, the basis read with
+// non-temporal loads.  The host picks 3 where a basis vector (k columns
 // of N) is at least STREAM_MIN_BYTES: each element is read or written once
 // per launch and the vectors are far larger than the caches (128^3 x 64
 // columns: the two-step update pass 853 -> 802 ms and the dot pass 598 ->
 // 569 ms per solve of 64 columns).  Smaller vectors keep cached accesses, so
 // that what one kernel writes can still be read from the last-level cache by
 // the next (the 32-scene batch of 64^3-128^3 scenes at 16 columns ran 5 %
-// slower streamed).
+// slower streamed).  The FP64 plain form (one complex128 column, 33.5 MB at
+// 128^3) picks 4 where the pass reads at least STREAM_PASS_BYTES of basis:
+// its dot and update passes read up to 31 vectors, and w, which the stencil
+// has just written and the update pass reads again, stays cached (GMRES(30)
+// at 128^3: 294.6 -> 272.5 us per Arnoldi step; streaming every pass,
+// short ones included, measured the same there).
 constexpr int64_t STREAM_MIN_BYTES = (int64_t)512 << 20;
+constexpr int64_t STREAM_PASS_BYTES = (int64_t)256 << 20;  // the last-level cache
 template <int VW>
 __device__ __forceinline__ float4 ld_vec(const float4 *p) {
     if constexpr (VW == 3) {
         const hz_f4 t = __builtin_nontemporal_load(reinterpret_cast<const hz_f4 *>(p));
         return make_float4(t.x, t.y, t.z, t.w);
+    } else {
+        return *p;
+    }
+}
+template <int VW, typename T>
+__device__ __forceinline__ cx<T> ld_one(const cx<T> *p) {
+    if constexpr (VW == 4) {
+        typedef T t2 __attribute__((ext_vector_type(2)));
+        const t2 t = __builtin_nontemporal_load(reinterpret_cast<const t2 *>(p));
+        return mk<T>(t.x, t.y);
     } else {
         return *p;
     }
@@ -47,9 +74,9 @@ __device__ __forceinline__ void st_vec(float4 *p, const float4 &v) {
 }
 
 // partial[((r*maxv + i)*nblk + blk)] = sum over the workgroup's chunk of
-// conj(V_i[r]) * w[r], for i in the group [g*16, min(g*16+16, jp1)).
-// VW >= 2 (complex64 only; n and chunk even): each lane loads two complex
-// values per 16-byte access.
+// conj(V_i[r]) * w[r], for i in the group [g*G, min(g*G+G, jp1)), G = dot_group<T>().
+// VW 2 and 3 (complex64 only; n and chunk even): each lane loads two
+// complex values per 16-byte access.
 template <typename T, int VW>
 __device__ __forceinline__ void dots_body(const cx<T> *__restrict__ V, int k, int64_t n, int jp1,
         const cx<T> *__restrict__ w, const int *__restrict__ live, double2 *__restrict__ partial, int maxv,
@@ -57,22 +84,23 @@ __device__ __forceinline__ void dots_body(const cx<T> *__restrict__ V, int k, in
     __shared__ double red[4];
     const int r = bk.y;
     if (!live[r]) return;
-    const int g0 = bk.z * DOT_GROUP;
-    const int cnt = min(DOT_GROUP, jp1 - g0);
+    constexpr int GQ = dot_group<T>();
+    const int g0 = bk.z * GQ;
+    const int cnt = min(GQ, jp1 - g0);
     if (cnt <= 0) return;
     const int64_t p0 = bk.x * chunk;
     const int64_t p1 = min(n, p0 + chunk);
     const cx<T> *wr = w + (int64_t)r * n;
-    T are[DOT_GROUP], aim[DOT_GROUP];
+    T are[GQ], aim[GQ];
 #pragma unroll
-    for (int q = 0; q < DOT_GROUP; ++q) are[q] = aim[q] = 0;
-    if constexpr (VW >= 2) {
+    for (int q = 0; q < GQ; ++q) are[q] = aim[q] = 0;
+    if constexpr (VW == 2 || VW == 3) {
         static_assert(sizeof(T) == 4, "paired loads are for complex64");
         const float4 *w4 = reinterpret_cast<const float4 *>(wr);
         for (int64_t p = (p0 >> 1) + threadIdx.x; p < (p1 >> 1); p += 256) {
             const float4 wv = ld_vec<VW>(w4 + p);
 #pragma unroll
-            for (int q = 0; q < DOT_GROUP; ++q) {
+            for (int q = 0; q < GQ; ++q) {
                 if (q < cnt) {
                     const float4 v =
                             ld_vec<VW>(reinterpret_cast<const float4 *>(V + ((int64_t)(g0 + q) * k + r) * n) + p);
@@ -85,9 +113,9 @@ __device__ __forceinline__ void dots_body(const cx<T> *__restrict__ V, int k, in
         for (int64_t p = p0 + threadIdx.x; p < p1; p += 256) {
             const cx<T> wv = wr[p];
 #pragma unroll
-            for (int q = 0; q < DOT_GROUP; ++q) {
+            for (int q = 0; q < GQ; ++q) {
                 if (q < cnt) {
-                    const cx<T> v = V[((int64_t)(g0 + q) * k + r) * n + p];
+                    const cx<T> v = ld_one<VW>(V + ((int64_t)(g0 + q) * k + r) * n + p);
                     are[q] += v.re * wv.re + v.im * wv.im;
                     aim[q] += v.re * wv.im - v.im * wv.re;
                 }
@@ -95,7 +123,7 @@ __device__ __forceinline__ void dots_body(const cx<T> *__restrict__ V, int k, in
         }
     }
 #pragma unroll
-    for (int q = 0; q < DOT_GROUP; ++q) {
+    for (int q = 0; q < GQ; ++q) {
         if (q < cnt) {
             // FP32 (or FP64) wavefront sums, combined across wavefronts in FP64.
             const double sre = block_combine_256((double)wave_sum(are[q]), red);
@@ -191,7 +219,7 @@ struct KReduceDots {
 
 // target[r] -= sum_{i<cnt} coef[r][i] * V_i[r]; optional FP64 partial of
 // |target|^2 per workgroup (norm_partial[r*nblk + blk]).  Grid (nblk, k).
-// VW >= 2 as in k_dots (target leading dimension even as well).
+// VW as in k_dots (paired: target leading dimension even as well).
 template <typename T, int VW>
 __device__ __forceinline__ void update_body(const cx<T> *__restrict__ V, int k, int64_t n, int cnt_all,
         const int *__restrict__ cnt_rhs, cx<T> *__restrict__ target, int64_t ldt, const cx<double> *__restrict__ coef,
@@ -207,7 +235,7 @@ __device__ __forceinline__ void update_body(const cx<T> *__restrict__ V, int k, 
     const int64_t p1 = min(n, p0 + chunk);
     cx<T> *tr = target + (int64_t)r * ldt;
     T nrm = 0;
-    if constexpr (VW >= 2) {
+    if constexpr (VW == 2 || VW == 3) {
         static_assert(sizeof(T) == 4, "paired loads are for complex64");
         float4 *t4 = reinterpret_cast<float4 *>(tr);
         for (int64_t p = (p0 >> 1) + threadIdx.x; p < (p1 >> 1); p += 256) {
@@ -257,16 +285,16 @@ __device__ __forceinline__ void update_body(const cx<T> *__restrict__ V, int k, 
             cx<T> acc = tr[p];
             int q = 0;
             for (; q + 4 <= cnt; q += 4) {
-                const cx<T> v0 = V[((int64_t)(q + 0) * k + r) * n + p];
-                const cx<T> v1 = V[((int64_t)(q + 1) * k + r) * n + p];
-                const cx<T> v2 = V[((int64_t)(q + 2) * k + r) * n + p];
-                const cx<T> v3 = V[((int64_t)(q + 3) * k + r) * n + p];
+                const cx<T> v0 = ld_one<VW>(V + ((int64_t)(q + 0) * k + r) * n + p);
+                const cx<T> v1 = ld_one<VW>(V + ((int64_t)(q + 1) * k + r) * n + p);
+                const cx<T> v2 = ld_one<VW>(V + ((int64_t)(q + 2) * k + r) * n + p);
+                const cx<T> v3 = ld_one<VW>(V + ((int64_t)(q + 3) * k + r) * n + p);
                 acc = cfnma(cf[q], v0, acc);
                 acc = cfnma(cf[q + 1], v1, acc);
                 acc = cfnma(cf[q + 2], v2, acc);
                 acc = cfnma(cf[q + 3], v3, acc);
             }
-            for (; q < cnt; ++q) acc = cfnma(cf[q], V[((int64_t)q * k + r) * n + p], acc);
+            for (; q < cnt; ++q) acc = cfnma(cf[q], ld_one<VW>(V + ((int64_t)q * k + r) * n + p), acc);
             tr[p] = acc;
             nrm += norm2(acc);
         }
@@ -429,19 +457,23 @@ struct KColNorms {
 // is beta (real, >= 0): apply the previous rotations, make the new one,
 // rotate g, record the residual estimate |g_{j+1}| and the step count, and
 // retire the RHS when the estimate meets st.tol[r] or beta vanished.
-// Returns whether the RHS is still live.
-__device__ bool givens_column(GmresState &st, int r, int j, double beta, int restart) {
+// Returns whether the RHS is still live.  The column's entries 0..j are read
+// from hj (stride hs) and the rotations 0..j-1 from gcr and gsr: H and the
+// rotations in global memory, or a copy in LDS.  The entry that one rotation
+// hands to the next stays in a register, so the chain of rotations waits on
+// no memory round trip.
+__device__ bool givens_column(GmresState &st, int r, int j, double beta, int restart, const cx<double> *hj,
+                              int64_t hs, const double *gcr, const cx<double> *gsr) {
     cx<double> *col = st.H + (int64_t)r * (restart + 1) * restart;  // H(i, j) at col[i*restart + j]
     col[(j + 1) * restart + j] = mk<double>(beta, 0);
+    cx<double> a = hj[0];
     for (int q = 0; q < j; ++q) {
-        const double c = st.gc[r * restart + q];
-        const cx<double> sn = st.gs[r * restart + q];
-        const cx<double> a = col[q * restart + j];
-        const cx<double> b = col[(q + 1) * restart + j];
+        const double c = gcr[q];
+        const cx<double> sn = gsr[q];
+        const cx<double> b = hj[(q + 1) * hs];
         col[q * restart + j] = c * a + sn * b;
-        col[(q + 1) * restart + j] = c * b - conj(sn) * a;
+        a = c * b - conj(sn) * a;
     }
-    const cx<double> a = col[j * restart + j];
     const double aa = cabs(a);
     double c;
     cx<double> sn;
@@ -474,20 +506,39 @@ __device__ bool givens_column(GmresState &st, int r, int j, double beta, int res
     return true;
 }
 
+__device__ bool givens_column(GmresState &st, int r, int j, double beta, int restart) {
+    const cx<double> *col = st.H + (int64_t)r * (restart + 1) * restart;
+    return givens_column(st, r, j, beta, restart, col + j, restart, st.gc + r * restart, st.gs + r * restart);
+}
+
 // Finish Arnoldi step j: beta = ||w||, H(j+1, j) = beta, Givens update,
 // residual estimate |g_{j+1}|, early exit when it meets st.tol[r].
 __device__ __forceinline__ void arnoldi_finish_body(GmresState st, const double *__restrict__ norm_partial, int nblk,
         int j, int maxv, int restart, const Blk &bk) {
     __shared__ double red[4];
+    __shared__ cx<double> s_h[64], s_s[64];  // restart <= 63: j <= 62
+    __shared__ double s_c[64];
     const int r = bk.x;
     if (!st.live[r]) return;
+    // Column j of H and the rotations, loaded by many lanes at once for the
+    // one lane that runs the Givens chain (GMRES(30) at 128^3: 3 us less per
+    // Arnoldi step than one lane loading them one rotation at a time).
+    const cx<double> *col = st.H + (int64_t)r * (restart + 1) * restart;
+    for (int q = threadIdx.x; q <= j; q += 256) {
+        s_h[q] = col[q * restart + j];
+        if (q < j) {
+            s_c[q] = st.gc[r * restart + q];
+            s_s[q] = st.gs[r * restart + q];
+        }
+    }
     double s = 0;
     for (int b = threadIdx.x; b < nblk; b += 256) s += norm_partial[(int64_t)r * nblk + b];
     s = block_sum_256(s, red);
+    __syncthreads();  // s_h, s_c, s_s
     if (threadIdx.x != 0) return;
     const double beta = sqrt(s);
     st.scales[(int64_t)(j + 1) * bk.gx + r] = beta > 0 ? 1.0 / beta : 0.0;
-    givens_column(st, r, j, beta, restart);
+    givens_column(st, r, j, beta, restart, s_h, 1, s_c, s_s);
 }
 
 __global__ __launch_bounds__(256) void k_arnoldi_finish(GmresState st, const double *__restrict__ norm_partial,
